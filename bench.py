@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Benchmark: spin updates/s of the Monte Carlo sweep engine on one MI355X.
+
+Workload (default): Heisenberg antiferromagnet on a 32^3 simple-cubic lattice
+(32768 FP32 unit vectors), J = 1 meV, Marsaglia moves, overrelaxation every
+MCS, 19 temperatures 60 -> 6 K x REPEAT 10 = 190 independent chains, all
+resident on the GPU and advanced together; MCS 5000 with 3000 equilibration,
+observables every 10 MCS.  A "step" is one MCS of every chain (OVRR pass +
+Metropolis sweep + the scheduled measurement), exactly the per-MCS schedule of
+the reference's driver.  The timed region is the whole production run of
+`--steps` MCS; `--warmup` untimed MCS run first on a separate start state.
+
+value = Metropolis spin updates / wall second of the timed MCS loop
+        (OVRR and measurement time included, not counted as updates).
+roofline = the Metropolis colour-phase kernel: 24 B of own-spin traffic per
+        update (FP32 SoA read + write) / its average launch time from HIP
+        events on the engine stream, against 8 TB/s.
+cpu_baseline = the CPU oracle's FP64 sweep (+OVRR) of one 32^3 chain on one
+        host core, for a bounded sample of about 10 s.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "spin updates/s; PT round-trips/s (pyrochlore, learned vs plain ladder)"
+B_UPD = {0: 2.0, 1: 24.0}   # algorithmic bytes per update: Ising int8, XYZ FP32 SoA
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(model, budget_s=10.0):
+    """Oracle sweep of one chain on one core (FP64), with OVRR as scheduled."""
+    import numpy as np
+    from oracle import oracle as o
+    from ether_amd.driver import attach_neighbours
+    if model.nn_count is None:
+        attach_neighbours(model)
+    orc = o.Oracle.from_model(model)
+    n = model.n_sites
+    rng = np.random.default_rng(1)
+    v = rng.normal(size=(n, 3))
+    S = np.ascontiguousarray(v / np.linalg.norm(v, axis=1)[:, None])
+    # two-colour order of the simple-cubic lattice (what the GPU runs)
+    L = model.sc
+    idx = np.arange(n)
+    x, y, z = idx % L[0], (idx // L[0]) % L[1], idx // (L[0] * L[1])
+    order = np.concatenate([idx[(x + y + z) % 2 == 0], idx[(x + y + z) % 2 == 1]]).astype(np.int32)
+    rid = np.arange(n, dtype=np.int32)
+    beta = model.betas([20.0])[0]
+    t0 = time.perf_counter()
+    sweeps = 0
+    while time.perf_counter() - t0 < budget_s:
+        orc.overrelax_pass(S, None, order, rid, 1.0, 1992, 0, sweeps, 0)
+        orc.sweep(S, None, order, rid, beta, 1992, 0, sweeps)
+        sweeps += 1
+    dt = time.perf_counter() - t0
+    return {"value": sweeps * n / dt, "unit": "spin updates/s", "cores": 1, "kind": "port",
+            "sample": f"{sweeps} MCS (OVRR pass + Metropolis sweep) of one {L[0]}x{L[1]}x{L[2]} "
+                      f"Heisenberg chain at T=20 K, FP64 C oracle, single thread, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="sc_heis_afm_32")
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the replicas "
+              "of a run are independent and would shard across GPUs, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from ether_amd import engine as E
+    from ether_amd import workloads
+    from ether_amd.driver import attach_neighbours, make_engine
+    from ether_amd.inputs import fortran_nint
+
+    m = workloads.model(args.workload)
+    attach_neighbours(m)
+    t = m.tags
+    temps = t.temperatures()
+    R = len(temps) * t.repeat
+    eng = make_engine(m, R)
+    eng.set_betas(np.tile(m.betas(temps), t.repeat))
+    coa = fortran_nint(t.coa)
+    ovrr_on = t.ovrr and t.xyz and not t.zeeman and not t.sia
+    ovrr_iv = t.ovrr_mcs if ovrr_on else 0
+    ovrr_p = t.ovrr_para if ovrr_on else 0.0
+
+    # warmup on its own start state
+    eng.init_spins(E.INIT_RANDOM)
+    if args.warmup > 0:
+        eng.run(1, args.warmup, t.tmcs_eq, coa, ovrr_iv, ovrr_p)
+    eng.synchronize()
+
+    # timed production run
+    eng.set_spins(-1, m.initial_spins, m.initial_phi)
+    eng.init_spins(E.INIT_FRESH)
+    eng.reset_moments()
+    eng.synchronize()
+    eng.set_timing(True)
+    t0 = time.perf_counter()
+    eng.run(1, args.steps, t.tmcs_eq, coa, ovrr_iv, ovrr_p)
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+    sweep_ms, sweep_n = eng.timing(E.TIME_SWEEP)
+    ovrr_ms, ovrr_n = eng.timing(E.TIME_OVRR)
+    meas_ms, meas_n = eng.timing(E.TIME_MEASURE)
+    eng.set_timing(False)
+    mom = eng.moments()
+    updates = float(R) * m.n_sites * args.steps
+    value = updates / wall
+
+    nc, col, _ = eng.layout()
+    per_launch_updates = R * m.n_sites / nc
+    avg_ms = sweep_ms / max(sweep_n, 1)
+    achieved = B_UPD[m.model_id] * per_launch_updates / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "spin updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (random-start spins, reference in.ether tags of the workload)",
+        "config": {
+            "workload": f"{args.workload}: Heisenberg AFM simple cubic {m.sc[0]}x{m.sc[1]}x{m.sc[2]} "
+                        f"({m.n_sites} spins), J=1 meV, {len(temps)} T x {t.repeat} repeats = {R} "
+                        f"chains, Marsaglia moves, OVRR every MCS, MCS {args.steps}/{t.tmcs_eq}, coa {coa}",
+            "replicas": R, "sites": m.n_sites, "colours": nc, "mcs": args.steps,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "k_xyz_sweep (one colour phase)",
+            "bytes_per_update": B_UPD[m.model_id], "updates_per_launch": per_launch_updates,
+            "avg_launch_ms": avg_ms, "launches": sweep_n,
+        },
+        "kernel_ms": {"sweep": sweep_ms, "ovrr": ovrr_ms, "measure": meas_ms,
+                      "sweep_launches": sweep_n, "ovrr_launches": ovrr_n, "measure_launches": meas_n},
+        "acceptance_mean": float(mom[:, E.MOM_ACC].mean() / args.steps),
+    }
+    eng.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(m, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

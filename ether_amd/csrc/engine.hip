@@ -1,0 +1,1201 @@
+// engine.hip - context, device layout and the C-ABI of the sweep engine
+// (declared in include/ether_gpu.h).
+//
+// The reference keeps its lattice and spins in module `init`
+// (src/init.f90:20-98) and calls Metropolis / overrelaxation / get_tot_energy
+// / parallel_tempering once per MCS per MPI rank (src/ether.f90:159-203).
+// Here one context holds every replica of every temperature in HBM and each
+// of those calls becomes one or a few kernel launches over all replicas.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/ether_gpu.h"
+#include "engine_common.h"
+#include "philox.h"
+
+namespace ether {
+void set_global_error(const std::string& s);
+const std::string& global_error();
+}  // namespace ether
+
+using namespace ether;
+
+struct ether_ctx {
+    int model = 0, N = 0, R = 0;
+    uint64_t seed = 0;
+    std::string err;
+    // host-side model (module init)
+    int nsp = 0, nshell = 0, cap = 0;
+    std::vector<int> species;
+    std::vector<double> stagger;
+    std::vector<int> nn_count, nn_idx, colour_in;
+    std::vector<double> jexc, spin;
+    int zee = 0;
+    double h[3] = {0, 0, 0}, g = 2.0, mub = 0.0;
+    int sia = 0;
+    std::vector<double> siav;
+    int angle = 0;
+    double dphi = 0.0;
+    int compat = 0;
+    bool lattice_set = false, exchange_set = false, committed = false, betas_set = false;
+    // layout
+    int ncol = 0;
+    std::vector<int> cbeg;
+    std::vector<int> slot_of_site, site_of_slot;
+    int D = 0, ell_stride = 0;
+    int n_classes = 0;
+    std::vector<float> jclass;
+    std::vector<double> ising_j;
+    std::vector<int> ising_Z, ising_stride;
+    int ising_P = 1, ising_offset = 0, ising_table = 0;
+    int xcd_map = 0;
+    int meas_tiles = 0;
+    // device
+    uint32_t* d_ell = nullptr;
+    uint8_t* d_sp = nullptr;
+    float* d_w = nullptr;
+    float* d_jclass = nullptr;
+    float* d_zvec = nullptr;
+    float* d_dsia = nullptr;
+    int* d_stride = nullptr;
+    uint32_t* d_thr = nullptr;
+    double* d_ising_j = nullptr;
+    double* d_ising_zc = nullptr;
+    float* d_sx = nullptr;
+    float* d_sy = nullptr;
+    float* d_sz = nullptr;
+    float* d_phi = nullptr;
+    int8_t* d_s = nullptr;
+    int* d_slot_of = nullptr;
+    int* d_rep_at = nullptr;
+    float* d_beta_f = nullptr;
+    double* d_beta = nullptr;
+    unsigned long long* d_acc = nullptr;
+    double* d_mom = nullptr;
+    double* d_partial = nullptr;
+    double* d_Ept = nullptr;
+    double* d_parts = nullptr;
+    double* d_M = nullptr;
+    long long* d_bonds = nullptr;
+    int n_groups = 0;
+    int* d_gstart = nullptr;
+    unsigned long long* d_pt_acc = nullptr;
+    unsigned long long* d_pt_att = nullptr;
+    unsigned long long* d_trips = nullptr;
+    int* d_trip_state = nullptr;
+    int* d_nacc = nullptr;
+    float* d_scratch_h = nullptr;
+    float* d_scratch_e = nullptr;
+    std::vector<double> beta_h;
+    uint32_t step = 0, init_count = 0, pt_count = 0;
+    hipStream_t stream = nullptr;
+    // kernel timing: event pairs per kind, reused from a pool
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+    size_t ev_used = 0;
+    std::vector<int> ev_kind;
+    std::vector<void*> allocs;
+    // MLP / S(q) state lives in the extension files
+    void* mlp = nullptr;
+    void* sq = nullptr;
+};
+
+// --------------------------------------------------------------------------
+// helpers
+// --------------------------------------------------------------------------
+static int fail(ether_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    else set_global_error(msg);
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t _e = (expr);                                                             \
+        if (_e != hipSuccess)                                                               \
+            return fail(ctx, ETHER_ERR_HIP,                                                 \
+                        std::string(#expr) + " failed: " + hipGetErrorString(_e));          \
+    } while (0)
+
+template <class T>
+static int dev_alloc(ether_ctx* c, T** p, size_t n) {
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess)
+        return fail(c, ETHER_ERR_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    c->allocs.push_back(*p);
+    return ETHER_OK;
+}
+
+template <class T>
+static int upload(ether_ctx* c, T* d, const T* h, size_t n) {
+    HIPCHK(c, hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return ETHER_OK;
+}
+
+#define TRY(expr)                      \
+    do {                               \
+        int _rc = (expr);              \
+        if (_rc != ETHER_OK) return _rc; \
+    } while (0)
+
+static int need_committed(ether_ctx* c) {
+    if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
+    if (!c->committed) return fail(c, ETHER_ERR_STATE, "context not committed (call ether_commit)");
+    return ETHER_OK;
+}
+static int need_betas(ether_ctx* c) {
+    TRY(need_committed(c));
+    if (!c->betas_set) return fail(c, ETHER_ERR_STATE, "temperatures not set (call ether_set_betas)");
+    return ETHER_OK;
+}
+
+// --------------------------------------------------------------------------
+// life cycle
+// --------------------------------------------------------------------------
+extern "C" int ether_ctx_create(int32_t model, int32_t n_sites, int32_t n_replicas, uint64_t seed,
+                                ether_ctx** out) {
+    if (!out) return fail(nullptr, ETHER_ERR_ARG, "ether_ctx_create: null output pointer");
+    *out = nullptr;
+    if (model != ETHER_MODEL_ISING && model != ETHER_MODEL_XYZ)
+        return fail(nullptr, ETHER_ERR_ARG, "ether_ctx_create: model must be ising (0) or xyz (1)");
+    if (n_sites <= 0 || n_sites > (1 << 24) || n_replicas <= 0)
+        return fail(nullptr, ETHER_ERR_ARG, "ether_ctx_create: bad n_sites or n_replicas");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(nullptr, ETHER_ERR_NODEV, "ether_ctx_create: no HIP device visible");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess)
+        return fail(nullptr, ETHER_ERR_NODEV, "ether_ctx_create: hipGetDeviceProperties failed");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(nullptr, ETHER_ERR_NODEV,
+                    std::string("ether_ctx_create: device is ") + prop.gcnArchName +
+                        ", this library is built for gfx950 only");
+    ether_ctx* c = new ether_ctx();
+    c->model = model;
+    c->N = n_sites;
+    c->R = n_replicas;
+    c->seed = seed;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(nullptr, ETHER_ERR_HIP, "ether_ctx_create: hipStreamCreate failed");
+    }
+    *out = c;
+    return ETHER_OK;
+}
+
+extern void ether_ext_destroy(ether_ctx* c);  // mlp / sq state
+
+extern "C" void ether_ctx_destroy(ether_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    ether_ext_destroy(c);
+    for (auto& e : c->ev_pool) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int ether_last_error(const ether_ctx* c, char* buf, int32_t len) {
+    if (!buf || len <= 0) return ETHER_ERR_ARG;
+    const std::string& s = c ? c->err : global_error();
+    std::snprintf(buf, (size_t)len, "%s", s.c_str());
+    return ETHER_OK;
+}
+
+// --------------------------------------------------------------------------
+// model set-up
+// --------------------------------------------------------------------------
+extern "C" int ether_set_lattice(ether_ctx* c, int32_t n_species, const int32_t* species,
+                                 const double* stagger, int32_t n_shells, int32_t capacity,
+                                 const int32_t* nn_count, const int32_t* nn_idx,
+                                 const int32_t* colour) {
+    if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
+    if (c->committed) return fail(c, ETHER_ERR_STATE, "ether_set_lattice after commit");
+    if (n_species <= 0 || n_species > kMaxSpecies)
+        return fail(c, ETHER_ERR_LIMIT, "ether_set_lattice: n_species must be 1..16");
+    if (n_shells <= 0 || capacity <= 0 || !species || !stagger || !nn_count || !nn_idx)
+        return fail(c, ETHER_ERR_ARG, "ether_set_lattice: bad arguments");
+    const int N = c->N;
+    c->nsp = n_species;
+    c->nshell = n_shells;
+    c->cap = capacity;
+    c->species.assign(species, species + N);
+    c->stagger.assign(stagger, stagger + N);
+    c->nn_count.assign(nn_count, nn_count + (size_t)n_shells * N);
+    c->nn_idx.assign(nn_idx, nn_idx + (size_t)n_shells * N * capacity);
+    for (int i = 0; i < N; ++i)
+        if (c->species[i] < 1 || c->species[i] > n_species)
+            return fail(c, ETHER_ERR_ARG, "ether_set_lattice: species id out of range");
+    for (size_t q = 0; q < c->nn_count.size(); ++q) {
+        if (c->nn_count[q] < 0 || c->nn_count[q] > capacity)
+            return fail(c, ETHER_ERR_ARG, "ether_set_lattice: neighbour count out of range");
+        const int32_t* nb = nn_idx + q * capacity;
+        for (int k = 0; k < c->nn_count[q]; ++k)
+            if (nb[k] < 0 || nb[k] >= N)
+                return fail(c, ETHER_ERR_ARG, "ether_set_lattice: neighbour index out of range");
+    }
+    if (colour) c->colour_in.assign(colour, colour + N);
+    else c->colour_in.clear();
+    c->lattice_set = true;
+    return ETHER_OK;
+}
+
+extern "C" int ether_set_exchange(ether_ctx* c, const double* j_exc, const double* spin) {
+    if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
+    if (!c->lattice_set) return fail(c, ETHER_ERR_STATE, "ether_set_exchange before ether_set_lattice");
+    if (c->committed) return fail(c, ETHER_ERR_STATE, "ether_set_exchange after commit");
+    if (!j_exc || !spin) return fail(c, ETHER_ERR_ARG, "ether_set_exchange: null pointer");
+    c->jexc.assign(j_exc, j_exc + (size_t)c->nshell * c->nsp * c->nsp * 3);
+    c->spin.assign(spin, spin + c->nsp);
+    c->exchange_set = true;
+    return ETHER_OK;
+}
+
+extern "C" int ether_set_zeeman(ether_ctx* c, int32_t on, const double* h3, double g_factor,
+                                double mu_b) {
+    if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
+    if (c->committed) return fail(c, ETHER_ERR_STATE, "ether_set_zeeman after commit");
+    c->zee = on ? 1 : 0;
+    if (on) {
+        if (!h3) return fail(c, ETHER_ERR_ARG, "ether_set_zeeman: null field");
+        for (int x = 0; x < 3; ++x) c->h[x] = h3[x];
+        c->g = g_factor;
+        c->mub = mu_b;
+    }
+    return ETHER_OK;
+}
+
+extern "C" int ether_set_sia(ether_ctx* c, int32_t on, const double* sia) {
+    if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
+    if (c->committed) return fail(c, ETHER_ERR_STATE, "ether_set_sia after commit");
+    if (!c->lattice_set) return fail(c, ETHER_ERR_STATE, "ether_set_sia before ether_set_lattice");
+    // SIA enters only for XYZ (src/Hamiltonian.f90:50, src/get_sia_values.f90:32)
+    c->sia = (on && c->model == ETHER_MODEL_XYZ) ? 1 : 0;
+    if (c->sia) {
+        if (!sia) return fail(c, ETHER_ERR_ARG, "ether_set_sia: null vector");
+        c->siav.assign(sia, sia + 3 * c->nsp);
+    }
+    return ETHER_OK;
+}
+
+extern "C" int ether_set_proposal(ether_ctx* c, int32_t angle, double dphi_rad) {
+    if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
+    if (c->committed) return fail(c, ETHER_ERR_STATE, "ether_set_proposal after commit");
+    c->angle = (angle && c->model == ETHER_MODEL_XYZ) ? 1 : 0;
+    c->dphi = dphi_rad;
+    return ETHER_OK;
+}
+
+extern "C" int ether_set_compat(ether_ctx* c, int32_t flags) {
+    if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
+    c->compat = flags;
+    return ETHER_OK;
+}
+
+// --------------------------------------------------------------------------
+// commit: classes, colouring, slots, ELL, device buffers
+// --------------------------------------------------------------------------
+namespace {
+
+struct Edge {
+    int j;
+    int cls;
+};
+
+// Greedy colouring of the interaction graph visiting vertices in `order`.
+int greedy_colour(int N, const std::vector<int>& adj_ptr, const std::vector<int>& adj,
+                  const std::vector<int>& order, std::vector<int>& col) {
+    col.assign(N, -1);
+    std::vector<int> mark(N + 1, -1);
+    int ncol = 0;
+    for (int v : order) {
+        for (int q = adj_ptr[v]; q < adj_ptr[v + 1]; ++q) {
+            int c = col[adj[q]];
+            if (c >= 0) mark[c] = v;
+        }
+        int c = 0;
+        while (mark[c] == v) ++c;
+        col[v] = c;
+        ncol = std::max(ncol, c + 1);
+    }
+    return ncol;
+}
+
+}  // namespace
+
+extern "C" int ether_commit(ether_ctx* c) {
+    if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
+    if (c->committed) return fail(c, ETHER_ERR_STATE, "context already committed");
+    if (!c->lattice_set || !c->exchange_set)
+        return fail(c, ETHER_ERR_STATE, "ether_commit needs ether_set_lattice and ether_set_exchange");
+    const int N = c->N, nsp = c->nsp, ns = c->nshell, cap = c->cap;
+    const bool ising = c->model == ETHER_MODEL_ISING;
+
+    // ---- coupling classes and per-site edges (zero couplings dropped) ----
+    std::map<std::tuple<float, float, float>, int> fcls;
+    std::map<double, int> icls;
+    std::vector<std::vector<Edge>> edges(N);
+    c->jclass.assign(3, 0.f);
+    c->ising_j.assign(1, 0.0);
+    for (int i = 0; i < N; ++i) {
+        const int a = c->species[i] - 1;
+        for (int s = 0; s < ns; ++s) {
+            const int cnt = c->nn_count[(size_t)s * N + i];
+            const int32_t* nb = c->nn_idx.data() + ((size_t)s * N + i) * cap;
+            for (int k = 0; k < cnt; ++k) {
+                const int j = nb[k];
+                const int b = c->species[j] - 1;
+                const double* J = c->jexc.data() + (((size_t)s * nsp + a) * nsp + b) * 3;
+                int cls;
+                if (ising) {
+                    if (J[2] == 0.0) continue;
+                    auto it = icls.find(J[2]);
+                    if (it == icls.end()) {
+                        cls = (int)c->ising_j.size();
+                        icls[J[2]] = cls;
+                        c->ising_j.push_back(J[2]);
+                        c->jclass.push_back(0.f);
+                        c->jclass.push_back(0.f);
+                        c->jclass.push_back((float)J[2]);
+                    } else {
+                        cls = it->second;
+                    }
+                } else {
+                    if (J[0] == 0.0 && J[1] == 0.0 && J[2] == 0.0) continue;
+                    auto key = std::make_tuple((float)J[0], (float)J[1], (float)J[2]);
+                    auto it = fcls.find(key);
+                    if (it == fcls.end()) {
+                        cls = (int)(c->jclass.size() / 3);
+                        fcls[key] = cls;
+                        c->jclass.push_back((float)J[0]);
+                        c->jclass.push_back((float)J[1]);
+                        c->jclass.push_back((float)J[2]);
+                    } else {
+                        cls = it->second;
+                    }
+                }
+                edges[i].push_back({j, cls});
+            }
+        }
+    }
+    c->n_classes = (int)(c->jclass.size() / 3) - 1;
+    if (c->n_classes > kMaxClasses)
+        return fail(c, ETHER_ERR_LIMIT, "more than 255 distinct exchange couplings");
+    if (ising && c->n_classes > kMaxIsingClasses)
+        return fail(c, ETHER_ERR_LIMIT, "Ising path supports at most 8 distinct J_zz values");
+    if (ising && nsp > kMaxIsingClasses)
+        return fail(c, ETHER_ERR_LIMIT, "Ising path supports at most 8 species");
+
+    // ---- colouring ----
+    std::vector<int> adj_ptr(N + 1, 0), adj;
+    {
+        std::vector<std::vector<int>> und(N);
+        for (int i = 0; i < N; ++i)
+            for (const Edge& e : edges[i])
+                if (e.j != i) {
+                    und[i].push_back(e.j);
+                    und[e.j].push_back(i);
+                }
+        for (int i = 0; i < N; ++i) {
+            std::sort(und[i].begin(), und[i].end());
+            und[i].erase(std::unique(und[i].begin(), und[i].end()), und[i].end());
+            adj_ptr[i + 1] = adj_ptr[i] + (int)und[i].size();
+        }
+        adj.reserve(adj_ptr[N]);
+        for (int i = 0; i < N; ++i) adj.insert(adj.end(), und[i].begin(), und[i].end());
+    }
+    std::vector<int> col;
+    if (!c->colour_in.empty()) {
+        col = c->colour_in;
+        int mx = 0;
+        for (int i = 0; i < N; ++i) {
+            if (col[i] < 0) return fail(c, ETHER_ERR_ARG, "negative colour");
+            mx = std::max(mx, col[i]);
+        }
+        for (int i = 0; i < N; ++i)
+            for (int q = adj_ptr[i]; q < adj_ptr[i + 1]; ++q)
+                if (col[adj[q]] == col[i])
+                    return fail(c, ETHER_ERR_ARG, "colouring joins two interacting sites");
+        c->ncol = mx + 1;
+    } else {
+        // natural order, then basis-major orders for every plausible basis size
+        std::vector<int> order(N), best;
+        for (int i = 0; i < N; ++i) order[i] = i;
+        int bestn = greedy_colour(N, adj_ptr, adj, order, best);
+        for (int k = 2; k <= 64 && bestn > 2; ++k) {
+            if (N % k) continue;
+            int q = 0;
+            for (int l = 0; l < k; ++l)
+                for (int cell = 0; cell < N / k; ++cell) order[q++] = cell * k + l;
+            std::vector<int> cc;
+            int n = greedy_colour(N, adj_ptr, adj, order, cc);
+            if (n < bestn) {
+                bestn = n;
+                best.swap(cc);
+            }
+        }
+        col.swap(best);
+        c->ncol = bestn;
+    }
+    // ---- slots: colour-major, site order within a colour ----
+    c->cbeg.assign(c->ncol + 1, 0);
+    for (int i = 0; i < N; ++i) c->cbeg[col[i] + 1]++;
+    for (int q = 0; q < c->ncol; ++q) c->cbeg[q + 1] += c->cbeg[q];
+    c->slot_of_site.assign(N, 0);
+    c->site_of_slot.assign(N, 0);
+    {
+        std::vector<int> fill(c->cbeg.begin(), c->cbeg.end() - 1);
+        for (int i = 0; i < N; ++i) {
+            int s = fill[col[i]]++;
+            c->slot_of_site[i] = s;
+            c->site_of_slot[s] = i;
+        }
+    }
+    // ---- ELL ----
+    int D = 0;
+    for (int i = 0; i < N; ++i) D = std::max(D, (int)edges[i].size());
+    c->D = D;
+    c->ell_stride = (N + 63) / 64 * 64;
+    std::vector<uint32_t> ell((size_t)std::max(D, 1) * c->ell_stride, 0u);
+    for (int i = 0; i < N; ++i) {
+        const int s = c->slot_of_site[i];
+        for (size_t k = 0; k < edges[i].size(); ++k)
+            ell[k * c->ell_stride + s] =
+                (uint32_t)c->slot_of_site[edges[i][k].j] | ((uint32_t)edges[i][k].cls << 24);
+    }
+    // ---- Ising index tables ----
+    if (ising) {
+        const int K = c->n_classes;
+        c->ising_Z.assign(K + 1, 0);
+        for (int i = 0; i < N; ++i) {
+            std::vector<int> cnt(K + 1, 0);
+            for (const Edge& e : edges[i]) cnt[e.cls]++;
+            for (int k = 1; k <= K; ++k) c->ising_Z[k] = std::max(c->ising_Z[k], cnt[k]);
+        }
+        c->ising_stride.assign(K + 1, 0);
+        long long P = 1;
+        c->ising_offset = 0;
+        for (int k = 1; k <= K; ++k) {
+            c->ising_stride[k] = (int)P;
+            c->ising_offset += c->ising_Z[k] * (int)P;
+            P *= (2 * c->ising_Z[k] + 1);
+            if (P * 2 * nsp > 65536)
+                return fail(c, ETHER_ERR_LIMIT, "Ising threshold table exceeds 65536 entries");
+        }
+        c->ising_P = (int)P;
+        c->ising_table = (int)(P * 2 * nsp);
+    }
+    // ---- per-slot species / magnetisation weights ----
+    std::vector<uint8_t> sp(N);
+    std::vector<float> w(N);
+    for (int i = 0; i < N; ++i) {
+        const int s = c->slot_of_site[i];
+        sp[s] = (uint8_t)(c->species[i] - 1);
+        w[s] = (float)(c->stagger[i] * c->spin[c->species[i] - 1]);
+    }
+    std::vector<float> zvec(3 * nsp, 0.f), dsia(3 * nsp, 0.f);
+    std::vector<double> zc(nsp, 0.0);
+    for (int a = 0; a < nsp; ++a) {
+        // -g*muB*s*H  (src/Hamiltonian.f90:120)
+        double f = c->zee ? -c->g * c->mub * c->spin[a] : 0.0;
+        for (int x = 0; x < 3; ++x) zvec[3 * a + x] = (float)(f * c->h[x]);
+        zc[a] = f * c->h[2];
+        if (c->sia)
+            for (int x = 0; x < 3; ++x) dsia[3 * a + x] = (float)c->siav[3 * a + x];
+    }
+    // ---- device buffers ----
+    const size_t RN = (size_t)c->R * N;
+    const int R = c->R;
+    c->meas_tiles = (N + kMeasTile - 1) / kMeasTile;
+    c->xcd_map = (R >= 32 || R % 8 == 0) ? 1 : 0;
+    TRY(dev_alloc(c, &c->d_ell, ell.size()));
+    TRY(dev_alloc(c, &c->d_sp, (size_t)N));
+    TRY(dev_alloc(c, &c->d_w, (size_t)N));
+    TRY(dev_alloc(c, &c->d_jclass, c->jclass.size()));
+    TRY(dev_alloc(c, &c->d_zvec, zvec.size()));
+    TRY(dev_alloc(c, &c->d_dsia, dsia.size()));
+    TRY(dev_alloc(c, &c->d_ising_j, c->ising_j.size()));
+    TRY(dev_alloc(c, &c->d_ising_zc, zc.size()));
+    if (ising) {
+        TRY(dev_alloc(c, &c->d_s, RN));
+        TRY(dev_alloc(c, &c->d_stride, c->ising_stride.size()));
+        TRY(dev_alloc(c, &c->d_thr, (size_t)R * c->ising_table));
+        TRY(dev_alloc(c, &c->d_bonds, (size_t)R * (c->n_classes + 1)));
+        TRY(upload(c, c->d_stride, c->ising_stride.data(), c->ising_stride.size()));
+        HIPCHK(c, hipMemsetAsync(c->d_bonds, 0, sizeof(long long) * R * (c->n_classes + 1), c->stream));
+        // ordered start, as fresh_spins would leave the all -1 basis state flipped
+        HIPCHK(c, hipMemsetAsync(c->d_s, 1, RN, c->stream));
+    } else {
+        TRY(dev_alloc(c, &c->d_sx, RN));
+        TRY(dev_alloc(c, &c->d_sy, RN));
+        TRY(dev_alloc(c, &c->d_sz, RN));
+        if (c->angle) TRY(dev_alloc(c, &c->d_phi, RN));
+        HIPCHK(c, hipMemsetAsync(c->d_sx, 0, RN * 4, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_sy, 0, RN * 4, c->stream));
+        if (c->angle) HIPCHK(c, hipMemsetAsync(c->d_phi, 0, RN * 4, c->stream));
+        std::vector<float> ones(RN, 1.0f);
+        TRY(upload(c, c->d_sz, ones.data(), RN));
+    }
+    TRY(dev_alloc(c, &c->d_slot_of, (size_t)R));
+    TRY(dev_alloc(c, &c->d_rep_at, (size_t)R));
+    TRY(dev_alloc(c, &c->d_beta_f, (size_t)R));
+    TRY(dev_alloc(c, &c->d_beta, (size_t)R));
+    TRY(dev_alloc(c, &c->d_acc, (size_t)R));
+    TRY(dev_alloc(c, &c->d_mom, (size_t)R * ETHER_MOM_COUNT));
+    TRY(dev_alloc(c, &c->d_partial, (size_t)R * c->meas_tiles * kRecord));
+    TRY(dev_alloc(c, &c->d_Ept, (size_t)R));
+    TRY(dev_alloc(c, &c->d_parts, (size_t)R * 4));
+    TRY(dev_alloc(c, &c->d_M, (size_t)R * 3));
+    TRY(dev_alloc(c, &c->d_pt_acc, (size_t)R));
+    TRY(dev_alloc(c, &c->d_pt_att, (size_t)R));
+    TRY(dev_alloc(c, &c->d_trips, (size_t)R));
+    TRY(dev_alloc(c, &c->d_trip_state, (size_t)R));
+    TRY(dev_alloc(c, &c->d_nacc, 1));
+    TRY(dev_alloc(c, &c->d_gstart, (size_t)R + 1));
+    TRY(dev_alloc(c, &c->d_scratch_h, (size_t)N * 3));
+    TRY(dev_alloc(c, &c->d_scratch_e, (size_t)N));
+    TRY(upload(c, c->d_ell, ell.data(), ell.size()));
+    TRY(upload(c, c->d_sp, sp.data(), sp.size()));
+    TRY(upload(c, c->d_w, w.data(), w.size()));
+    TRY(upload(c, c->d_jclass, c->jclass.data(), c->jclass.size()));
+    TRY(upload(c, c->d_zvec, zvec.data(), zvec.size()));
+    TRY(upload(c, c->d_dsia, dsia.data(), dsia.size()));
+    TRY(upload(c, c->d_ising_j, c->ising_j.data(), c->ising_j.size()));
+    TRY(upload(c, c->d_ising_zc, zc.data(), zc.size()));
+    std::vector<int> ident(R);
+    for (int r = 0; r < R; ++r) ident[r] = r;
+    TRY(upload(c, c->d_slot_of, ident.data(), (size_t)R));
+    TRY(upload(c, c->d_rep_at, ident.data(), (size_t)R));
+    HIPCHK(c, hipMemsetAsync(c->d_acc, 0, sizeof(unsigned long long) * R, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_mom, 0, sizeof(double) * R * ETHER_MOM_COUNT, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_pt_acc, 0, sizeof(unsigned long long) * R, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_pt_att, 0, sizeof(unsigned long long) * R, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_trips, 0, sizeof(unsigned long long) * R, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_trip_state, 0, sizeof(int) * R, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->committed = true;
+    return ETHER_OK;
+}
+
+extern "C" int ether_get_layout(ether_ctx* c, int32_t* n_colours, int32_t* colour, int32_t* rng_id) {
+    TRY(need_committed(c));
+    if (n_colours) *n_colours = c->ncol;
+    for (int i = 0; i < c->N; ++i) {
+        int s = c->slot_of_site[i];
+        if (rng_id) rng_id[i] = s;
+        if (colour) {
+            int q = 0;
+            while (s >= c->cbeg[q + 1]) ++q;
+            colour[i] = q;
+        }
+    }
+    return ETHER_OK;
+}
+
+// --------------------------------------------------------------------------
+// temperatures
+// --------------------------------------------------------------------------
+static int upload_ising_thresholds(ether_ctx* c) {
+    const int K = c->n_classes, nsp = c->nsp, P = c->ising_P;
+    std::vector<uint32_t> thr((size_t)c->R * c->ising_table);
+    std::vector<int> n(K + 1, 0);
+    for (int t = 0; t < c->R; ++t) {
+        const double beta = c->beta_h[t];
+        for (int a = 0; a < nsp; ++a)
+            for (int up = 0; up < 2; ++up) {
+                const int sigma = up ? 1 : -1;
+                for (int idx = 0; idx < P; ++idx) {
+                    // decode n_k from the mixed-radix index
+                    int rem = idx;
+                    double hfield = 0.0;
+                    for (int k = 1; k <= K; ++k) {
+                        int digit = rem % (2 * c->ising_Z[k] + 1);
+                        rem /= (2 * c->ising_Z[k] + 1);
+                        n[k] = digit - c->ising_Z[k];
+                        hfield += c->ising_j[k] * n[k];
+                    }
+                    // dE = (S'-S)_z (h + zeeman), S'_z - S_z = -2 sigma
+                    double dE = -2.0 * sigma * hfield;
+                    if (c->zee) dE += -c->g * c->mub * c->spin[a] * (-2.0 * sigma) * c->h[2];
+                    double p = std::exp(-beta * dE);
+                    uint32_t th;
+                    if (!(p < 1.0)) th = 0xffffffffu;
+                    else {
+                        double f = std::floor(p * 4294967296.0);
+                        if (f > 4294967294.0) f = 4294967294.0;
+                        th = (uint32_t)f;
+                    }
+                    thr[(size_t)t * c->ising_table + (size_t)(a * 2 + up) * P + idx] = th;
+                }
+            }
+    }
+    TRY(upload(c, c->d_thr, thr.data(), thr.size()));
+    return ETHER_OK;
+}
+
+extern "C" int ether_set_betas(ether_ctx* c, const double* beta) {
+    TRY(need_committed(c));
+    if (!beta) return fail(c, ETHER_ERR_ARG, "ether_set_betas: null pointer");
+    c->beta_h.assign(beta, beta + c->R);
+    std::vector<float> bf(c->R);
+    for (int r = 0; r < c->R; ++r) {
+        if (!(beta[r] > 0.0) || !std::isfinite(beta[r]))
+            return fail(c, ETHER_ERR_ARG, "ether_set_betas: beta must be positive and finite");
+        bf[r] = (float)beta[r];
+    }
+    TRY(upload(c, c->d_beta, beta, (size_t)c->R));
+    TRY(upload(c, c->d_beta_f, bf.data(), (size_t)c->R));
+    std::vector<int> ident(c->R);
+    for (int r = 0; r < c->R; ++r) ident[r] = r;
+    TRY(upload(c, c->d_slot_of, ident.data(), (size_t)c->R));
+    TRY(upload(c, c->d_rep_at, ident.data(), (size_t)c->R));
+    if (c->model == ETHER_MODEL_ISING) TRY(upload_ising_thresholds(c));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->betas_set = true;
+    return ETHER_OK;
+}
+
+extern "C" int ether_set_ladders(ether_ctx* c, int32_t n_groups, const int32_t* group_start) {
+    TRY(need_committed(c));
+    if (n_groups < 0) return fail(c, ETHER_ERR_ARG, "ether_set_ladders: negative group count");
+    if (n_groups > 0) {
+        if (!group_start) return fail(c, ETHER_ERR_ARG, "ether_set_ladders: null group_start");
+        if (group_start[0] != 0 && group_start[0] < 0)
+            return fail(c, ETHER_ERR_ARG, "ether_set_ladders: bad group_start");
+        for (int g = 0; g < n_groups; ++g)
+            if (group_start[g + 1] <= group_start[g] || group_start[g + 1] > c->R)
+                return fail(c, ETHER_ERR_ARG, "ether_set_ladders: groups must be increasing and <= R");
+        TRY(upload(c, c->d_gstart, group_start, (size_t)n_groups + 1));
+    }
+    c->n_groups = n_groups;
+    HIPCHK(c, hipMemsetAsync(c->d_pt_acc, 0, sizeof(unsigned long long) * c->R, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_pt_att, 0, sizeof(unsigned long long) * c->R, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_trips, 0, sizeof(unsigned long long) * c->R, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_trip_state, 0, sizeof(int) * c->R, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ETHER_OK;
+}
+
+// --------------------------------------------------------------------------
+// argument blocks
+// --------------------------------------------------------------------------
+static XyzArgs xyz_args(ether_ctx* c, int colour) {
+    XyzArgs a;
+    a.sx = c->d_sx;
+    a.sy = c->d_sy;
+    a.sz = c->d_sz;
+    a.phi = c->d_phi;
+    a.ell = c->d_ell;
+    a.ell_stride = c->ell_stride;
+    a.D = c->D;
+    a.sp = c->d_sp;
+    a.N = c->N;
+    a.R = c->R;
+    a.c_begin = c->cbeg[colour];
+    a.c_end = c->cbeg[colour + 1];
+    a.tiles = (a.c_end - a.c_begin + kBlock - 1) / kBlock;
+    a.xcd_map = c->xcd_map;
+    a.slot_of = c->d_slot_of;
+    a.beta_slot = c->d_beta_f;
+    a.seed = c->seed;
+    a.step = c->step;
+    a.pass = 0;
+    a.frac = 1.0f;
+    a.acc = c->d_acc;
+    a.n_classes = c->n_classes;
+    a.jclass = c->d_jclass;
+    a.nsp = c->nsp;
+    a.zvec = c->d_zvec;
+    a.dsia = c->d_dsia;
+    a.dphi = (float)c->dphi;
+    return a;
+}
+
+static IsingArgs ising_args(ether_ctx* c, int colour) {
+    IsingArgs a;
+    a.s = c->d_s;
+    a.ell = c->d_ell;
+    a.ell_stride = c->ell_stride;
+    a.D = c->D;
+    a.sp = c->d_sp;
+    a.N = c->N;
+    a.R = c->R;
+    a.c_begin = c->cbeg[colour];
+    a.c_end = c->cbeg[colour + 1];
+    const int g0 = a.c_begin >> 2, g1 = (a.c_end + 3) >> 2;
+    a.tiles = (g1 - g0 + kBlock - 1) / kBlock;
+    a.xcd_map = c->xcd_map;
+    a.slot_of = c->d_slot_of;
+    a.seed = c->seed;
+    a.step = c->step;
+    a.acc = c->d_acc;
+    a.n_classes = c->n_classes;
+    a.stride = c->d_stride;
+    a.offset = c->ising_offset;
+    a.table_per_slot = c->ising_table;
+    a.P = c->ising_P;
+    a.thr = c->d_thr;
+    return a;
+}
+
+static MeasArgs meas_args(ether_ctx* c) {
+    MeasArgs a;
+    a.sx = c->d_sx;
+    a.sy = c->d_sy;
+    a.sz = c->d_sz;
+    a.s = c->d_s;
+    a.ell = c->d_ell;
+    a.ell_stride = c->ell_stride;
+    a.D = c->D;
+    a.sp = c->d_sp;
+    a.w = c->d_w;
+    a.N = c->N;
+    a.R = c->R;
+    a.tiles = c->meas_tiles;
+    a.n_classes = c->n_classes;
+    a.jclass = c->d_jclass;
+    a.nsp = c->nsp;
+    a.zvec = c->d_zvec;
+    a.dsia = c->d_dsia;
+    a.partial = c->d_partial;
+    return a;
+}
+
+// --------------------------------------------------------------------------
+// kernel timing (HIP events on the context stream)
+// --------------------------------------------------------------------------
+static int ev_begin(ether_ctx* c, int kind) {
+    if (!c->timing) return -1;
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess) return -1;
+        if (hipEventCreate(&b) != hipSuccess) {
+            (void)hipEventDestroy(a);
+            return -1;
+        }
+        c->ev_pool.push_back({a, b});
+        c->ev_kind.push_back(kind);
+    }
+    const int k = (int)c->ev_used++;
+    c->ev_kind[k] = kind;
+    (void)hipEventRecord(c->ev_pool[k].first, c->stream);
+    return k;
+}
+
+static void ev_end(ether_ctx* c, int k) {
+    if (k >= 0) (void)hipEventRecord(c->ev_pool[k].second, c->stream);
+}
+
+extern "C" int ether_set_timing(ether_ctx* c, int32_t on) {
+    if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->timing = on != 0;
+    c->ev_used = 0;
+    return ETHER_OK;
+}
+
+extern "C" int ether_get_timing(ether_ctx* c, int32_t kind, double* total_ms, int64_t* launches) {
+    if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    int64_t n = 0;
+    for (size_t k = 0; k < c->ev_used; ++k) {
+        if (c->ev_kind[k] != kind) continue;
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev_pool[k].first, c->ev_pool[k].second));
+        tot += ms;
+        ++n;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = n;
+    return ETHER_OK;
+}
+
+extern "C" int ether_synchronize(ether_ctx* c) {
+    if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ETHER_OK;
+}
+
+// --------------------------------------------------------------------------
+// hot path
+// --------------------------------------------------------------------------
+static int enqueue_sweep(ether_ctx* c) {
+    for (int q = 0; q < c->ncol; ++q) {
+        if (c->cbeg[q + 1] == c->cbeg[q]) continue;
+        const int ev = ev_begin(c, ETHER_TIME_SWEEP);
+        if (c->model == ETHER_MODEL_ISING) launch_ising_sweep(ising_args(c, q), c->stream);
+        else launch_xyz_sweep(xyz_args(c, q), c->angle, c->zee, c->sia, c->stream);
+        ev_end(c, ev);
+    }
+    HIPCHK(c, hipGetLastError());
+    c->step++;
+    return ETHER_OK;
+}
+
+static int enqueue_ovrr(ether_ctx* c, double passes) {
+    int full = (int)std::floor(passes);
+    double frac = passes - full;
+    int pass = 0;
+    for (int p = 0; p < full + (frac > 0.0 ? 1 : 0); ++p, ++pass) {
+        for (int q = 0; q < c->ncol; ++q) {
+            if (c->cbeg[q + 1] == c->cbeg[q]) continue;
+            XyzArgs a = xyz_args(c, q);
+            a.pass = (uint32_t)pass;
+            a.frac = (p < full) ? 1.0f : (float)frac;
+            const int ev = ev_begin(c, ETHER_TIME_OVRR);
+            launch_xyz_ovrr(a, c->angle, c->zee, c->stream);
+            ev_end(c, ev);
+        }
+    }
+    HIPCHK(c, hipGetLastError());
+    return ETHER_OK;
+}
+
+static int enqueue_measure(ether_ctx* c, int sample) {
+    const int ev = ev_begin(c, ETHER_TIME_MEASURE);
+    launch_measure(meas_args(c), c->model, c->zee, c->sia, c->stream);
+    FinArgs f;
+    f.partial = c->d_partial;
+    f.tiles = c->meas_tiles;
+    f.R = c->R;
+    f.N = c->N;
+    f.model = c->model;
+    f.n_classes = c->n_classes;
+    f.ising_j = c->d_ising_j;
+    f.nsp = c->nsp;
+    f.ising_zc = c->d_ising_zc;
+    f.sample = sample;
+    f.pt_etot = (c->compat & ETHER_COMPAT_PT_ETOT) ? 1 : 0;
+    f.E_pt = c->d_Ept;
+    f.parts = c->d_parts;
+    f.M = c->d_M;
+    f.bonds = c->d_bonds;
+    f.moments = c->d_mom;
+    f.slot_of = c->d_slot_of;
+    launch_finalize(f, c->stream);
+    ev_end(c, ev);
+    HIPCHK(c, hipGetLastError());
+    return ETHER_OK;
+}
+
+static int enqueue_pt(ether_ctx* c, int parity) {
+    if (c->n_groups <= 0) return ETHER_OK;
+    TRY(enqueue_measure(c, 0));
+    PtArgs p;
+    p.n_groups = c->n_groups;
+    p.gstart = c->d_gstart;
+    p.parity = parity & 1;
+    p.rep_at = c->d_rep_at;
+    p.slot_of = c->d_slot_of;
+    p.beta_slot = c->d_beta;
+    p.E_pt = c->d_Ept;
+    p.seed = c->seed;
+    p.swap_index = c->pt_count++;
+    p.acc = c->d_pt_acc;
+    p.att = c->d_pt_att;
+    p.trips = c->d_trips;
+    p.trip_state = c->d_trip_state;
+    p.n_accepted = c->d_nacc;
+    const int ev = ev_begin(c, ETHER_TIME_PT);
+    launch_pt_swap(p, c->stream);
+    ev_end(c, ev);
+    HIPCHK(c, hipGetLastError());
+    return ETHER_OK;
+}
+
+extern "C" int ether_init_spins(ether_ctx* c, int32_t mode) {
+    TRY(need_committed(c));
+    if (mode != ETHER_INIT_FRESH && mode != ETHER_INIT_RANDOM)
+        return fail(c, ETHER_ERR_ARG, "ether_init_spins: mode must be FRESH or RANDOM "
+                                      "(set a basis state with ether_set_spins)");
+    InitArgs a;
+    a.sx = c->d_sx;
+    a.sy = c->d_sy;
+    a.sz = c->d_sz;
+    a.phi = c->d_phi;
+    a.s = c->d_s;
+    a.N = c->N;
+    a.R = c->R;
+    a.model = c->model;
+    a.angle = c->angle;
+    a.mode = mode;
+    a.dphi = (float)c->dphi;
+    a.seed = c->seed;
+    a.step = c->init_count++;
+    launch_init(a, c->stream);
+    HIPCHK(c, hipGetLastError());
+    return ETHER_OK;
+}
+
+extern "C" int ether_sweep(ether_ctx* c, int32_t n_mcs, double* accept) {
+    TRY(need_betas(c));
+    if (n_mcs < 0) return fail(c, ETHER_ERR_ARG, "ether_sweep: negative n_mcs");
+    std::vector<unsigned long long> before;
+    for (int k = 0; k < n_mcs; ++k) {
+        if (accept && k == n_mcs - 1) {
+            before.resize(c->R);
+            HIPCHK(c, hipMemcpyAsync(before.data(), c->d_acc, sizeof(unsigned long long) * c->R,
+                                     hipMemcpyDeviceToHost, c->stream));
+        }
+        TRY(enqueue_sweep(c));
+    }
+    if (accept && n_mcs > 0) {
+        std::vector<unsigned long long> after(c->R);
+        HIPCHK(c, hipMemcpyAsync(after.data(), c->d_acc, sizeof(unsigned long long) * c->R,
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int r = 0; r < c->R; ++r) accept[r] = (double)(after[r] - before[r]) / c->N;
+    }
+    return ETHER_OK;
+}
+
+extern "C" int ether_overrelax(ether_ctx* c, double passes) {
+    TRY(need_committed(c));
+    if (c->model != ETHER_MODEL_XYZ) return fail(c, ETHER_ERR_STATE, "overrelaxation needs XYZ spins");
+    if (c->sia)
+        return fail(c, ETHER_ERR_STATE,
+                    "overrelaxation does not conserve the single-ion anisotropy energy");
+    if (!(passes >= 0.0)) return fail(c, ETHER_ERR_ARG, "ether_overrelax: negative passes");
+    return enqueue_ovrr(c, passes);
+}
+
+extern "C" int ether_measure(ether_ctx* c, int32_t sample, double* E, double* M) {
+    TRY(need_committed(c));
+    TRY(enqueue_measure(c, sample ? 1 : 0));
+    if (E || M) {
+        std::vector<double> parts(4 * (size_t)c->R);
+        HIPCHK(c, hipMemcpyAsync(parts.data(), c->d_parts, sizeof(double) * 4 * c->R,
+                                 hipMemcpyDeviceToHost, c->stream));
+        if (M)
+            HIPCHK(c, hipMemcpyAsync(M, c->d_M, sizeof(double) * 3 * c->R, hipMemcpyDeviceToHost,
+                                     c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (E)
+            for (int r = 0; r < c->R; ++r) E[r] = parts[4 * r] + parts[4 * r + 1] + parts[4 * r + 2];
+    }
+    return ETHER_OK;
+}
+
+extern "C" int ether_get_energy_parts(ether_ctx* c, double* parts, int64_t* bonds, int32_t* n_classes) {
+    TRY(need_committed(c));
+    if (parts)
+        HIPCHK(c, hipMemcpyAsync(parts, c->d_parts, sizeof(double) * 4 * c->R, hipMemcpyDeviceToHost,
+                                 c->stream));
+    if (n_classes) *n_classes = c->n_classes;
+    if (bonds && c->model == ETHER_MODEL_ISING) {
+        std::vector<long long> b((size_t)c->R * (c->n_classes + 1));
+        HIPCHK(c, hipMemcpyAsync(b.data(), c->d_bonds, sizeof(long long) * b.size(),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int r = 0; r < c->R; ++r)
+            for (int k = 1; k <= c->n_classes; ++k)
+                bonds[(size_t)r * c->n_classes + (k - 1)] = b[(size_t)r * (c->n_classes + 1) + k];
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ETHER_OK;
+}
+
+extern "C" int ether_read_moments(ether_ctx* c, double* moments) {
+    TRY(need_committed(c));
+    if (!moments) return fail(c, ETHER_ERR_ARG, "ether_read_moments: null pointer");
+    std::vector<unsigned long long> acc(c->R);
+    HIPCHK(c, hipMemcpyAsync(moments, c->d_mom, sizeof(double) * ETHER_MOM_COUNT * c->R,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(acc.data(), c->d_acc, sizeof(unsigned long long) * c->R,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int r = 0; r < c->R; ++r) moments[ETHER_MOM_COUNT * r + ETHER_MOM_ACC] = (double)acc[r] / c->N;
+    return ETHER_OK;
+}
+
+extern "C" int ether_reset_moments(ether_ctx* c) {
+    TRY(need_committed(c));
+    HIPCHK(c, hipMemsetAsync(c->d_mom, 0, sizeof(double) * ETHER_MOM_COUNT * c->R, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_acc, 0, sizeof(unsigned long long) * c->R, c->stream));
+    return ETHER_OK;
+}
+
+extern "C" int ether_pt_swap(ether_ctx* c, int32_t parity, int32_t* n_accepted) {
+    TRY(need_betas(c));
+    if (c->n_groups <= 0) return fail(c, ETHER_ERR_STATE, "ether_pt_swap: no ladders set");
+    HIPCHK(c, hipMemsetAsync(c->d_nacc, 0, sizeof(int), c->stream));
+    TRY(enqueue_pt(c, parity));
+    if (n_accepted) {
+        HIPCHK(c, hipMemcpyAsync(n_accepted, c->d_nacc, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return ETHER_OK;
+}
+
+extern "C" int ether_pt_state(ether_ctx* c, int32_t* slot_of, int64_t* acc, int64_t* att,
+                              int64_t* trips) {
+    TRY(need_committed(c));
+    if (slot_of)
+        HIPCHK(c, hipMemcpyAsync(slot_of, c->d_slot_of, sizeof(int) * c->R, hipMemcpyDeviceToHost,
+                                 c->stream));
+    if (acc)
+        HIPCHK(c, hipMemcpyAsync(acc, c->d_pt_acc, sizeof(int64_t) * c->R, hipMemcpyDeviceToHost,
+                                 c->stream));
+    if (att)
+        HIPCHK(c, hipMemcpyAsync(att, c->d_pt_att, sizeof(int64_t) * c->R, hipMemcpyDeviceToHost,
+                                 c->stream));
+    if (trips)
+        HIPCHK(c, hipMemcpyAsync(trips, c->d_trips, sizeof(int64_t) * c->R, hipMemcpyDeviceToHost,
+                                 c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ETHER_OK;
+}
+
+extern "C" int ether_run(ether_ctx* c, int32_t first_step, int32_t n_steps, int32_t mcs_eq,
+                         int32_t coa, int32_t ovrr_interval, double ovrr_passes,
+                         int32_t pt_interval, int32_t* swap_count) {
+    TRY(need_betas(c));
+    if (coa <= 0) return fail(c, ETHER_ERR_ARG, "ether_run: coa must be positive");
+    if (n_steps < 0) return fail(c, ETHER_ERR_ARG, "ether_run: negative n_steps");
+    const bool do_ovrr = ovrr_interval > 0 && c->model == ETHER_MODEL_XYZ && !c->sia &&
+                         ovrr_passes > 0.0;
+    const bool do_pt = pt_interval > 0 && c->n_groups > 0;
+    int sc = swap_count ? *swap_count : 0;
+    for (int32_t k = 0; k < n_steps; ++k) {
+        const int32_t stepi = first_step + k;
+        // src/ether.f90:163-169
+        if (do_ovrr && stepi % ovrr_interval == 0 && stepi % coa != 0) TRY(enqueue_ovrr(c, ovrr_passes));
+        // src/ether.f90:172
+        TRY(enqueue_sweep(c));
+        // src/ether.f90:189-198
+        if (do_pt && stepi % pt_interval == 0 && stepi % coa != 0) {
+            ++sc;
+            TRY(enqueue_pt(c, sc % 2));
+        }
+        // src/ether.f90:201 -> src/observable.f90:30-31
+        if (stepi > mcs_eq && stepi % coa == 0) TRY(enqueue_measure(c, 1));
+    }
+    if (swap_count) *swap_count = sc;
+    return ETHER_OK;
+}
+
+// --------------------------------------------------------------------------
+// spins in and out
+// --------------------------------------------------------------------------
+extern "C" int ether_get_spins(ether_ctx* c, int32_t replica, double* S, double* phi) {
+    TRY(need_committed(c));
+    if (replica < 0 || replica >= c->R || !S) return fail(c, ETHER_ERR_ARG, "ether_get_spins: bad args");
+    const int N = c->N;
+    const size_t off = (size_t)replica * N;
+    if (c->model == ETHER_MODEL_ISING) {
+        std::vector<int8_t> s(N);
+        HIPCHK(c, hipMemcpyAsync(s.data(), c->d_s + off, N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int i = 0; i < N; ++i) {
+            S[3 * i] = 0.0;
+            S[3 * i + 1] = 0.0;
+            S[3 * i + 2] = (double)s[c->slot_of_site[i]];
+            if (phi) phi[i] = 0.0;
+        }
+        return ETHER_OK;
+    }
+    std::vector<float> x(N), y(N), z(N), p;
+    HIPCHK(c, hipMemcpyAsync(x.data(), c->d_sx + off, 4 * N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(y.data(), c->d_sy + off, 4 * N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(z.data(), c->d_sz + off, 4 * N, hipMemcpyDeviceToHost, c->stream));
+    if (phi && c->angle) {
+        p.resize(N);
+        HIPCHK(c, hipMemcpyAsync(p.data(), c->d_phi + off, 4 * N, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < N; ++i) {
+        int s = c->slot_of_site[i];
+        S[3 * i] = x[s];
+        S[3 * i + 1] = y[s];
+        S[3 * i + 2] = z[s];
+        if (phi) phi[i] = c->angle ? (double)p[s] : 0.0;
+    }
+    return ETHER_OK;
+}
+
+extern "C" int ether_set_spins(ether_ctx* c, int32_t replica, const double* S, const double* phi) {
+    TRY(need_committed(c));
+    if (replica < -1 || replica >= c->R || !S)
+        return fail(c, ETHER_ERR_ARG, "ether_set_spins: bad args (replica -1 sets all)");
+    const int N = c->N;
+    const int r0 = replica < 0 ? 0 : replica, r1 = replica < 0 ? c->R : replica + 1;
+    if (c->model == ETHER_MODEL_ISING) {
+        std::vector<int8_t> s(N);
+        for (int i = 0; i < N; ++i) s[c->slot_of_site[i]] = S[3 * i + 2] > 0.0 ? 1 : -1;
+        for (int r = r0; r < r1; ++r)
+            HIPCHK(c, hipMemcpyAsync(c->d_s + (size_t)r * N, s.data(), N, hipMemcpyHostToDevice,
+                                     c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return ETHER_OK;
+    }
+    std::vector<float> x(N), y(N), z(N), p(N, 0.f);
+    for (int i = 0; i < N; ++i) {
+        int s = c->slot_of_site[i];
+        x[s] = (float)S[3 * i];
+        y[s] = (float)S[3 * i + 1];
+        z[s] = (float)S[3 * i + 2];
+        p[s] = phi ? (float)phi[i] : (float)std::atan2(S[3 * i + 1], S[3 * i]);
+    }
+    for (int r = r0; r < r1; ++r) {
+        const size_t off = (size_t)r * N;
+        HIPCHK(c, hipMemcpyAsync(c->d_sx + off, x.data(), 4 * N, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_sy + off, y.data(), 4 * N, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_sz + off, z.data(), 4 * N, hipMemcpyHostToDevice, c->stream));
+        if (c->angle)
+            HIPCHK(c, hipMemcpyAsync(c->d_phi + off, p.data(), 4 * N, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ETHER_OK;
+}
+
+extern "C" int ether_local_fields(ether_ctx* c, int32_t replica, double* h, double* e) {
+    TRY(need_committed(c));
+    if (replica < 0 || replica >= c->R) return fail(c, ETHER_ERR_ARG, "ether_local_fields: bad replica");
+    const int N = c->N;
+    launch_local_fields(meas_args(c), replica, c->d_scratch_h, c->d_scratch_e, c->zee, c->sia,
+                        c->model, c->stream);
+    HIPCHK(c, hipGetLastError());
+    std::vector<float> hh(3 * (size_t)N), ee(N);
+    HIPCHK(c, hipMemcpyAsync(hh.data(), c->d_scratch_h, 12 * (size_t)N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ee.data(), c->d_scratch_e, 4 * (size_t)N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < N; ++i) {
+        int s = c->slot_of_site[i];
+        if (h)
+            for (int x = 0; x < 3; ++x) h[3 * i + x] = hh[3 * s + x];
+        if (e) e[i] = ee[s];
+    }
+    return ETHER_OK;
+}
+
+// access for extension translation units (mlp.hip, sq.hip)
+namespace ether {
+hipStream_t ctx_stream(ether_ctx* c) { return c->stream; }
+void*& ctx_mlp(ether_ctx* c) { return c->mlp; }
+void*& ctx_sq(ether_ctx* c) { return c->sq; }
+int ctx_fail(ether_ctx* c, int code, const std::string& msg) { return fail(c, code, msg); }
+int ctx_model(ether_ctx* c) { return c->model; }
+int ctx_N(ether_ctx* c) { return c->N; }
+int ctx_R(ether_ctx* c) { return c->R; }
+bool ctx_committed(ether_ctx* c) { return c->committed; }
+const float* ctx_spin_plane(ether_ctx* c, int comp) {
+    return comp == 0 ? c->d_sx : comp == 1 ? c->d_sy : c->d_sz;
+}
+const int8_t* ctx_ising(ether_ctx* c) { return c->d_s; }
+const int* ctx_slot_of_site(ether_ctx* c) { return c->slot_of_site.data(); }
+}  // namespace ether

@@ -1,0 +1,157 @@
+"""Batched ETHER run on the GPU engine: the schedule of src/ether.f90:79-236
+with every (temperature, repeat) chain resident on the device at once.
+
+The reference farms temperatures out to MPI ranks and runs REPEAT repeats one
+after another per temperature (src/ether.f90:93-214).  Here replica
+r = k*nscan + t is repeat k of temperature t; all of them advance together in
+one ether_run call per chunk of MCS.  With PT on, each repeat's nscan slots
+form one ladder (the reference's ladder is the P ranks of a round,
+src/ether.f90:110-133; with one rank per temperature it is the whole list).
+"""
+from __future__ import annotations
+
+import math
+import os
+import time
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import engine as E
+from .inputs import Model, fortran_nint
+
+
+def attach_neighbours(model: Model) -> Model:
+    """getting_nbd (src/getting_nbd.f90:46-82) through the library's builder."""
+    t = model.tags
+    cnt, idx = E.build_neighbours(model.structure.abc, model.basis_xyz, model.sc,
+                                  [b == "o" for b in t.bc], model.shells, t.nbdfc, t.nbd_capacity)
+    model.nn_count, model.nn_idx = cnt, idx
+    return model
+
+
+def make_engine(model: Model, n_replicas: int, *, compat: int = 0, seed: Optional[int] = None,
+                colour=None) -> E.Engine:
+    if model.nn_count is None:
+        attach_neighbours(model)
+    t = model.tags
+    eng = E.Engine(model.model_id, model.n_sites, n_replicas, t.seed if seed is None else seed)
+    eng.setup(model.n_species, model.species, model.stagger, model.nn_count, model.nn_idx,
+              model.j_exc.reshape(-1), model.spin,
+              zeeman=model.h if t.zeeman else None, g_factor=model.g_factor, mu_b=model.mu_b,
+              sia=model.sia_vec.reshape(-1) if model.sia_vec is not None else None,
+              angle=t.angle, dphi=t.dphi, compat=compat, colour=colour)
+    return eng
+
+
+def repeat_observables(mom: np.ndarray, beta: float, n_sites: int) -> np.ndarray:
+    """get_eng/get_mag 'avg_*' (src/get_eng.f90:43-56, src/get_mag.f90:44-55).
+    Returns (e_avg, U_e, cv, m_avg, U_m, chi)."""
+    n = mom[E.MOM_N]
+    e, e2, e4 = mom[E.MOM_E] / n, mom[E.MOM_E2] / n, mom[E.MOM_E4] / n
+    m, m2, m4 = mom[E.MOM_M] / n, mom[E.MOM_M2] / n, mom[E.MOM_M4] / n
+    return np.array([
+        e, 1.0 - (1.0 / 3.0) * (e4 / (e2 * e2)), beta * beta * (e2 - e * e) * n_sites,
+        m, 1.0 - (1.0 / 3.0) * (m4 / (m2 * m2)), beta * (m2 - m * m) * n_sites,
+    ])
+
+
+def over_repeats(per: np.ndarray):
+    """src/observables_per_repeat.f90:29-69: mean and sqrt(sum (x-mean)^2/(m(m-1)))."""
+    m = per.shape[0]
+    mean = per.mean(axis=0)
+    sem = np.sqrt(((per - mean) ** 2).sum(axis=0) / (m * (m - 1.0))) if m > 1 else np.zeros_like(mean)
+    return mean, sem
+
+
+@dataclass
+class Results:
+    temps: np.ndarray        # (nscan,)
+    mean: np.ndarray         # (nscan, 6): e, U_e, cv, m, U_m, chi
+    sem: np.ndarray          # (nscan, 6)
+    accept_pct: np.ndarray   # (nscan,)
+    per_repeat: np.ndarray   # (repeat, nscan, 6)
+    wall_s: float
+    updates: float
+    pt_acc: Optional[np.ndarray] = None
+    pt_att: Optional[np.ndarray] = None
+    trips: Optional[np.ndarray] = None
+
+    def write(self, outdir: str) -> None:
+        """energy.dat / magnetization.dat in the reference's layout
+        (src/write_output_files.f90:82-98, format (f11.5,6(1x,es12.5),1x,f7.2))."""
+        os.makedirs(outdir, exist_ok=True)
+
+        def row(vals, acc=None):
+            s = f"{vals[0]:11.5f}" + "".join(f" {v:12.5E}" for v in vals[1:])
+            return s + (f" {acc:7.2f}" if acc is not None else "")
+
+        with open(os.path.join(outdir, "magnetization.dat"), "w") as f:
+            f.write("    # 1 Temp.        2 |M|          3 χ       4 Δ|M|          5 Δχ       "
+                    "6 U(|M|)    7 ΔU(|M|)\n")
+            for k, T in enumerate(self.temps):
+                m, s = self.mean[k], self.sem[k]
+                f.write(row([T, m[3], m[5], s[3], s[5], m[4], s[4]]) + "\n")
+        with open(os.path.join(outdir, "energy.dat"), "w") as f:
+            f.write("    # 1 Temp.          2 E      3 Cv/kB         4 ΔE   5 Δ(Cv/kB)         "
+                    "6 U(E)       7 ΔU(E) 8 Accpt. (%)\n")
+            for k, T in enumerate(self.temps):
+                m, s = self.mean[k], self.sem[k]
+                f.write(row([T, m[0], m[2], s[0], s[2], m[1], s[1]], self.accept_pct[k]) + "\n")
+
+
+def run(model: Model, *, compat: int = 0, chunk: int = 0, temps=None, repeat: Optional[int] = None,
+        eng: Optional[E.Engine] = None, seed: Optional[int] = None) -> Results:
+    """Full run: every temperature x repeat as one batch of device replicas."""
+    t = model.tags
+    temps = t.temperatures() if temps is None else np.asarray(temps, np.float64)
+    nscan = len(temps)
+    rep = t.repeat if repeat is None else repeat
+    R = nscan * rep
+    own = eng is None
+    if own:
+        eng = make_engine(model, R, compat=compat, seed=seed)
+    beta_T = model.betas(temps)
+    eng.set_betas(np.tile(beta_T, rep))
+    if t.pt:
+        eng.set_ladders([k * nscan for k in range(rep + 1)])
+    # start state: generate_supercell state, then fresh_spins (src/ether.f90:155)
+    eng.set_spins(-1, model.initial_spins, model.initial_phi)
+    eng.init_spins(E.INIT_FRESH)
+    eng.reset_moments()
+    coa = fortran_nint(t.coa)
+    if abs(coa - t.coa) > 1e-12 or coa <= 0:
+        raise ValueError("coa must be a positive integer")
+    # OVRR gate of src/ether.f90:163-167
+    ovrr_on = t.ovrr and t.xyz and not t.zeeman and not t.sia and t.ovrr_mcs > 0
+    chunk = chunk or t.tmcs
+    t0 = time.perf_counter()
+    swap = 0
+    step = 1
+    while step <= t.tmcs:
+        n = min(chunk, t.tmcs - step + 1)
+        swap = eng.run(step, n, t.tmcs_eq, coa, t.ovrr_mcs if ovrr_on else 0,
+                       t.ovrr_para if ovrr_on else 0.0, t.exchange_interval if t.pt else 0, swap)
+        step += n
+    mom = eng.moments()   # synchronises
+    wall = time.perf_counter() - t0
+    per = np.zeros((rep, nscan, 6))
+    acc = np.zeros(nscan)
+    for k in range(rep):
+        for j in range(nscan):
+            s = k * nscan + j
+            per[k, j] = repeat_observables(mom[s], beta_T[j], model.n_sites)
+            acc[j] += mom[s, E.MOM_ACC]
+    mean = np.zeros((nscan, 6))
+    sem = np.zeros((nscan, 6))
+    for j in range(nscan):
+        mean[j], sem[j] = over_repeats(per[:, j, :])
+    res = Results(temps=temps, mean=mean, sem=sem, accept_pct=acc / (rep * t.tmcs) * 100.0,
+                  per_repeat=per, wall_s=wall, updates=float(R) * model.n_sites * t.tmcs)
+    if t.pt:
+        _, pacc, patt, trips = eng.pt_state()
+        res.pt_acc, res.pt_att, res.trips = pacc, patt, trips
+    if own:
+        eng.close()
+    return res

@@ -1,0 +1,275 @@
+/*
+ * ether_gpu.h - C-ABI of the MI355X (gfx950) Monte Carlo sweep engine.
+ *
+ * The reference (mukkelian/Ether) has no FFI: its hot path is a set of
+ * implicit-interface Fortran externals that share state through module
+ * `init` (src/init.f90:20-98).  Each entry point below replaces one of those
+ * externals or one of their call sites in the driver, and is bound from
+ * Fortran through `iso_c_binding` (ether_amd/fortran/ether_gpu_mod.f90) and
+ * from Python through ctypes (ether_amd/engine.py).
+ *
+ * Conventions
+ *   - Every call returns an int status (ETHER_OK == 0).  On failure the
+ *     message is kept in the context and read with ether_last_error(); the
+ *     Fortran driver maps any nonzero status to its terminate(remark), which
+ *     keeps the reference's print-and-stop convention (src/terminate.f90:20-35).
+ *   - The context owns every device buffer.  Host arrays belong to the caller
+ *     and are copied in at set/commit time or out at get time.
+ *   - Site indices are the reference's supercell order: cell k (z) outer,
+ *     j (y), i (x), basis l innermost (src/generate_supercell.f90:115-132),
+ *     0-based here.  Species ids are 1-based as in ion(4,:).
+ *   - A "replica" is one Markov chain (one temperature x one repeat of the
+ *     reference, or one member of a tempering ladder).  A "slot" is a
+ *     temperature of the ladder; replica r sits at slot slot_of[r].  Tempering
+ *     swaps permute slots, never spins.
+ *   - One host thread per context; all kernels run in order on the context's
+ *     stream.  Calls that return host data synchronise that stream.
+ *   - No MPI, no CUDA shims.  gfx950 only.
+ */
+#ifndef ETHER_GPU_H
+#define ETHER_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ether_ctx ether_ctx;
+
+/* status codes */
+#define ETHER_OK          0
+#define ETHER_ERR_ARG     1   /* bad argument (size, pointer, range)          */
+#define ETHER_ERR_STATE   2   /* call not valid in the current state          */
+#define ETHER_ERR_HIP     3   /* HIP runtime error                            */
+#define ETHER_ERR_LIMIT   4   /* exceeds a documented capacity                */
+#define ETHER_ERR_NODEV   5   /* no gfx950 device visible                     */
+
+/* spin models: in.ether `model = ising | xyz` (src/read_input.f90:62-66) */
+#define ETHER_MODEL_ISING 0
+#define ETHER_MODEL_XYZ   1
+
+/* ether_init_spins modes (the generate_supercell start state of
+ * src/generate_supercell.f90:34-63,115-132 is written with ether_set_spins) */
+#define ETHER_INIT_FRESH  1  /* fresh_spins: one unconditional proposal per site (src/fresh_spins.f90:31-38) */
+#define ETHER_INIT_RANDOM 2  /* independent random state: Ising +-1, XYZ uniform on S^2                       */
+
+/* compat flags (ether_set_compat) */
+#define ETHER_COMPAT_PT_ETOT  1  /* tempering swap uses the double-counted E_tot (src/ether.f90:177,194)      */
+
+/* per-replica moment record read by ether_read_moments (8 doubles) */
+#define ETHER_MOM_N     0   /* number of samples                                  */
+#define ETHER_MOM_E     1   /* sum of e    (e = E_tot/(2N), src/get_eng.f90:36)   */
+#define ETHER_MOM_E2    2   /* sum of e^2                                         */
+#define ETHER_MOM_E4    3   /* sum of e^4                                         */
+#define ETHER_MOM_M     4   /* sum of m    (m = |M|/N, src/get_mag.f90:36-37)     */
+#define ETHER_MOM_M2    5   /* sum of m^2                                         */
+#define ETHER_MOM_M4    6   /* sum of m^4                                         */
+#define ETHER_MOM_ACC   7   /* sum over sweeps of accepted/N (src/ether.f90:172-174) */
+#define ETHER_MOM_COUNT 8
+
+/* ---------------------------------------------------------------------------
+ * Host-side lattice set-up (no GPU).  Replaces getting_nbd / search_nbd
+ * (src/getting_nbd.f90:46-82, src/search_nbd.f90:34-58): same capture rule
+ * |d_shell - |r_i - (r_j + image)|| <= tol over images {0,+1,-1} per closed
+ * axis, j != i, each j stored once per shell, ascending j.  Uses the lattice's
+ * translation symmetry instead of the O(N^2 * 27) scan.
+ *
+ *   abc[3*3]     lattice vectors as rows (Angstrom), abc[3*a+c]
+ *   n_basis      sites per cell; basis_xyz[3*n_basis] Cartesian positions
+ *   sc[3]        supercell repeats; open_bc[3] nonzero for 'o'
+ *   shell_d[n_shells] bond lengths; tol = NBDFC
+ *   capacity     nbd_capacity (src/defaults.f90:56); overflow -> ETHER_ERR_LIMIT
+ *   nn_count[n_shells*N]            out: neighbours of site i in shell s at [s*N+i]
+ *   nn_idx[n_shells*N*capacity]     out: [(s*N+i)*capacity + k], 0-based site ids
+ * N = sc[0]*sc[1]*sc[2]*n_basis.  Returns ETHER_OK or an error code; a message
+ * for the last failure of these context-free calls is read with
+ * ether_last_error(NULL, ...).
+ * ------------------------------------------------------------------------- */
+int ether_build_neighbours(const double* abc, int32_t n_basis, const double* basis_xyz,
+                           const int32_t* sc, const int32_t* open_bc,
+                           int32_t n_shells, const double* shell_d, double tol,
+                           int32_t capacity, int32_t* nn_count, int32_t* nn_idx);
+
+/* ---------------------------------------------------------------------------
+ * Context life cycle
+ * ------------------------------------------------------------------------- */
+
+/* Replaces the module-`init` allocation done by generate_supercell/getting_nbd.
+ * n_sites = total_ions (src/generate_supercell.f90:137), n_replicas chains,
+ * seed = in.ether `seed` (src/read_input.f90:118-120). */
+int ether_ctx_create(int32_t model, int32_t n_sites, int32_t n_replicas, uint64_t seed,
+                     ether_ctx** out);
+void ether_ctx_destroy(ether_ctx* ctx);
+
+/* Copies the last error message (NUL-terminated, truncated to len). ctx may be NULL. */
+int ether_last_error(const ether_ctx* ctx, char* buf, int32_t len);
+
+/* Lattice: species[N] (1-based, ion(4,:)), stagger[N] (stgg_ion,
+ * src/get_staggered_info.f90:78-82 and src/generate_supercell.f90:127),
+ * neighbour lists as produced by ether_build_neighbours (the nn array of
+ * src/getting_nbd.f90:34).  colour[N] may be NULL: the library then colours
+ * the interaction graph greedily (bipartite lattices get 2 colours). */
+int ether_set_lattice(ether_ctx* ctx, int32_t n_species, const int32_t* species,
+                      const double* stagger, int32_t n_shells, int32_t capacity,
+                      const int32_t* nn_count, const int32_t* nn_idx, const int32_t* colour);
+
+/* Exchange: j_exc[((s*n_species + a)*n_species + b)*3 + c] in energy units
+ * already scaled as the reference's j_exc after j_values + parameters
+ * (s_a*s_b, /1000 or /J_para: src/j_values.f90:83-84, src/parameters.f90:26-41).
+ * spin[n_species] = s (src/read_input.f90:74-75), used for M. */
+int ether_set_exchange(ether_ctx* ctx, const double* j_exc, const double* spin);
+
+/* Zeeman term -g*mu_B*s*(S.H) (src/Hamiltonian.f90:108-124). */
+int ether_set_zeeman(ether_ctx* ctx, int32_t on, const double* h3, double g_factor, double mu_b);
+
+/* Single-ion anisotropy D.(S*S) (src/Hamiltonian.f90:127-143), sia[n_species*3]
+ * scaled as sia_vec (src/get_sia_values.f90:58, src/parameters.f90).  XYZ only. */
+int ether_set_sia(ether_ctx* ctx, int32_t on, const double* sia);
+
+/* Proposal: angle=0 Marsaglia (src/George_Marsaglia.f90:29-39); angle=1 ANGLE
+ * moves with window dphi in radians (src/update_spin_details.f90:39-46). */
+int ether_set_proposal(ether_ctx* ctx, int32_t angle, double dphi_rad);
+
+int ether_set_compat(ether_ctx* ctx, int32_t flags);
+
+/* Builds the colour-sorted device layout and uploads it.  Must follow the
+ * set_* calls and precede any sweep. */
+int ether_commit(ether_ctx* ctx);
+
+/* Number of colour classes and, optionally, colour[N] and the device slot of
+ * every site (rng_id[N]); the counter-based RNG is keyed by the slot. */
+int ether_get_layout(ether_ctx* ctx, int32_t* n_colours, int32_t* colour, int32_t* rng_id);
+
+/* ---------------------------------------------------------------------------
+ * Temperatures and ladders
+ * ------------------------------------------------------------------------- */
+
+/* beta per slot (beta = 1/(kB*T), src/ether.f90:139-140).  Resets slot_of to
+ * the identity. */
+int ether_set_betas(ether_ctx* ctx, const double* beta);
+
+/* Tempering groups: slots [group_start[g], group_start[g+1]) form one ladder,
+ * ordered as the reference orders its ranks (hottest first, src/ether.f90:79-84).
+ * n_groups = 0 disables tempering. group_start has n_groups+1 entries. */
+int ether_set_ladders(ether_ctx* ctx, int32_t n_groups, const int32_t* group_start);
+
+/* ---------------------------------------------------------------------------
+ * Hot path
+ * ------------------------------------------------------------------------- */
+
+/* fresh_spins (src/ether.f90:155) and the generate_supercell start state. */
+int ether_init_spins(ether_ctx* ctx, int32_t mode);
+
+/* n_mcs Metropolis sweeps of every replica (src/Metropolis.f90:20-70), one
+ * colour class after another.  Per-replica acceptance ratios are added to the
+ * ETHER_MOM_ACC moment.  accept[R] (nullable) receives the ratio of the last
+ * sweep. */
+int ether_sweep(ether_ctx* ctx, int32_t n_mcs, double* accept);
+
+/* Overrelaxation (src/overrelaxation.f90:20-63): reflection of each spin about
+ * its local exchange (+ Zeeman) field, colour by colour.  `passes` is the
+ * expected number of reflections per site (ovrr_para): floor(passes) full
+ * passes plus a Bernoulli(frac) pass.  XYZ only; refused with SIA on. */
+int ether_overrelax(ether_ctx* ctx, double passes);
+
+/* get_tot_energy + get_tot_magnetisation (src/get_tot_energy.f90:20-49,
+ * src/get_tot_magnetisation.f90:20-42) for every replica.  With sample != 0 the
+ * observable() moments are accumulated (src/observable.f90:30-43).
+ * E[R] (nullable) = E_tot as the reference defines it (exchange double
+ * counted, Zeeman/SIA once), M[3R] (nullable) = staggered magnetisation. */
+int ether_measure(ether_ctx* ctx, int32_t sample, double* E, double* M);
+
+/* Split energies per replica: parts[4R] = {E_exchange(double counted),
+ * E_zeeman, E_sia, E_physical}.  Ising: also the exact integer bond sums
+ * bonds[R*n_classes] (nullable) and n_classes. Uses the last ether_measure. */
+int ether_get_energy_parts(ether_ctx* ctx, double* parts, int64_t* bonds, int32_t* n_classes);
+
+int ether_read_moments(ether_ctx* ctx, double* moments /* [R*ETHER_MOM_COUNT] */);
+int ether_reset_moments(ether_ctx* ctx);   /* zeroes('avg') (src/zeroes.f90:51-58) */
+
+/* parallel_tempering (src/parallel_tempering.f90:20-110) for every ladder:
+ * energies from a fresh measure pass, pairs (k,k+1) with k%2 == parity,
+ * accept with min(1, exp((beta_k - beta_k+1)(E_k - E_k+1))).  Swaps slots.
+ * n_accepted (nullable) = accepted swaps in this call. */
+int ether_pt_swap(ether_ctx* ctx, int32_t parity, int32_t* n_accepted);
+
+/* slot_of[R] (nullable), per-slot accepted/attempted swap counts with the
+ * slot above (acc[R], att[R], nullable) and completed cold->hot->cold round
+ * trips per replica (trips[R], nullable). */
+int ether_pt_state(ether_ctx* ctx, int32_t* slot_of, int64_t* acc, int64_t* att, int64_t* trips);
+
+/* One whole MCS schedule of src/ether.f90:159-203 for steps
+ * first_step..first_step+n_steps-1 (1-based like stepi):
+ *   OVRR   if ovrr_interval>0 and stepi%ovrr_interval==0 and stepi%coa!=0
+ *   sweep
+ *   PT     if pt_interval>0 and stepi%pt_interval==0 and stepi%coa!=0
+ *   sample if stepi>mcs_eq and stepi%coa==0
+ * swap_count (in/out) is the reference's per-repeat PT counter. */
+int ether_run(ether_ctx* ctx, int32_t first_step, int32_t n_steps, int32_t mcs_eq,
+              int32_t coa, int32_t ovrr_interval, double ovrr_passes,
+              int32_t pt_interval, int32_t* swap_count);
+
+/* Kernel timing with HIP events recorded on the context's stream around every
+ * launch of a kind: 0 Metropolis colour phase, 1 overrelaxation colour phase,
+ * 2 measurement (partials + finalize), 3 tempering swap.  Adds no
+ * synchronisation; ether_get_timing synchronises and reports the summed
+ * event time and the launch count since timing was switched on. */
+#define ETHER_TIME_SWEEP   0
+#define ETHER_TIME_OVRR    1
+#define ETHER_TIME_MEASURE 2
+#define ETHER_TIME_PT      3
+int ether_set_timing(ether_ctx* ctx, int32_t on);
+int ether_get_timing(ether_ctx* ctx, int32_t kind, double* total_ms, int64_t* launches);
+/* Waits for all work queued on the context's stream. */
+int ether_synchronize(ether_ctx* ctx);
+
+/* Spins of one replica in site order: S[3N] (x,y,z per site), phi[N] nullable.
+ * Ising spins are returned as (0,0,+-1). */
+int ether_get_spins(ether_ctx* ctx, int32_t replica, double* S, double* phi);
+int ether_set_spins(ether_ctx* ctx, int32_t replica, const double* S, const double* phi);
+
+/* Local exchange field h_i = sum_j J_ij o S_j (src/get_ovrr_vec.f90:41-85) and
+ * site energies e_i (src/Hamiltonian.f90:20-54, site mode) of one replica,
+ * computed by the device kernels; h[3N], e[N] nullable. */
+int ether_local_fields(ether_ctx* ctx, int32_t replica, double* h, double* e);
+
+/* ---------------------------------------------------------------------------
+ * Structure factor (utility/Structure_Factor/structure_factor.f90:124-183)
+ *   S_a(q) = sum_r S_a(r) exp(i q.r) on the commensurate grid
+ *   q = 2*pi*(m1*b1/L1 + m2*b2/L2 + m3*b3/L3), m_k in [0, L_k), a_i.b_j = delta_ij,
+ * evaluated as per-axis DFT contractions on the matrix cores (split-BF16
+ * operands, FP32 accumulation).  out[R*Q] = sum_a |S_a(q)|^2 / N^2 per
+ * replica, Q = N = L1*L2*L3 (one site per cell), q index m1 fastest.
+ * ------------------------------------------------------------------------- */
+int ether_structure_factor(ether_ctx* ctx, const int32_t* sc, float* out);
+
+/* ---------------------------------------------------------------------------
+ * Learned tempering ladder.  No reference counterpart: the nearest is the
+ * commented Hukushima stub at src/startup.f90:115-122.
+ *   A 1-64-64-2 tanh MLP maps x = (T - t_mid)/t_half to (<E>(T), log C_v(T))
+ *   normalised; weights live on the device; training is full-batch Adam with
+ *   hand-written forward/backward kernels.
+ * ------------------------------------------------------------------------- */
+#define ETHER_MLP_NPARAM (64 + 64 + 64*64 + 64 + 2*64 + 2)
+int ether_mlp_init(ether_ctx* ctx, uint64_t seed);
+int ether_mlp_set_params(ether_ctx* ctx, const float* params);
+int ether_mlp_get_params(ether_ctx* ctx, float* params);
+/* loss/grad on a batch: x[n], y[2n], w[2n] (per-output weights). grad[NPARAM]. */
+int ether_mlp_loss_grad(ether_ctx* ctx, int32_t n, const float* x, const float* y,
+                        const float* w, float* loss, float* grad);
+int ether_mlp_forward(ether_ctx* ctx, int32_t n, const float* x, float* y);
+int ether_mlp_train(ether_ctx* ctx, int32_t n, const float* x, const float* y, const float* w,
+                    int32_t iters, float lr, float* final_loss);
+/* Ladder placement from the trained surrogate: with x = (T - x_mid)/x_half and
+ * log C_v(T) = const + logcv_scale * y1(x), places n_temps temperatures on
+ * [t_lo, t_hi] (hottest first, like src/ether.f90:79-84) so that
+ * int sqrt(C_v)/T dT is equal between neighbours, which makes the Gaussian
+ * estimate erfc(dT sqrt(C_v)/(2T)) of the swap acceptance uniform. */
+int ether_ladder_place(ether_ctx* ctx, double x_mid, double x_half, double logcv_scale,
+                       double t_lo, double t_hi, int32_t n_temps, double* T_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ETHER_GPU_H */

@@ -1,0 +1,494 @@
+/*
+ * ether_oracle.c - CPU FP64 oracle (TEST INFRASTRUCTURE ONLY; see ether_oracle.h).
+ *
+ * Plain C restatement of the reference's hot-path routines.  Every function
+ * names the reference file:line it follows.  Nothing here is linked into the
+ * product library libether_gpu.so.
+ */
+#include "ether_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10 (J. K. Salmon, M. A. Moraes, R. O. Dror, D. E. Shaw,       */
+/* "Parallel random numbers: as easy as 1, 2, 3", SC'11, section 4.2).      */
+/* ------------------------------------------------------------------------ */
+#define PHILOX_M0 0xD2511F53u
+#define PHILOX_M1 0xCD9E8D57u
+#define PHILOX_W0 0x9E3779B9u
+#define PHILOX_W1 0xBB67AE85u
+
+void o_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+    uint32_t c[4] = {ctr_in[0], ctr_in[1], ctr_in[2], ctr_in[3]};
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += PHILOX_W0; k1 += PHILOX_W1; }
+        uint64_t p0 = (uint64_t)PHILOX_M0 * c[0];
+        uint64_t p1 = (uint64_t)PHILOX_M1 * c[2];
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c[1] ^ k0;
+        uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+    }
+    out[0] = c[0]; out[1] = c[1]; out[2] = c[2]; out[3] = c[3];
+}
+
+void o_rng(uint64_t seed, uint32_t id, uint32_t replica, uint32_t step, uint32_t purpose,
+           uint32_t attempt, uint32_t out[4]) {
+    uint32_t ctr[4] = {id, replica, step, (purpose << 16) | (attempt & 0xffffu)};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    o_philox4x32_10(ctr, key, out);
+}
+
+/* uniform in [-1,1) from the top 24 bits; get_random_num(-1,1) (src/get_random_num.f90:29-30) */
+static double sym24(uint32_t x24) { return (double)((int32_t)x24 - (1 << 23)) * 0x1p-23; }
+/* acceptance uniform in (0,1] */
+static double eta24(uint32_t w) { return (double)((w >> 8) + 1u) * 0x1p-24; }
+
+/* ------------------------------------------------------------------------ */
+/* Neighbour search: src/getting_nbd.f90:46-82 with src/search_nbd.f90:34-58 */
+/* ------------------------------------------------------------------------ */
+static int search_nbd(const double* p0, const double* p1, double d, double tol,
+                      const double scabc[3][3], const int32_t* open_bc) {
+    static const int shift[3] = {0, 1, -1};
+    int nx = open_bc[0] ? 1 : 3, ny = open_bc[1] ? 1 : 3, nz = open_bc[2] ? 1 : 3;
+    for (int a = 0; a < nx; ++a)
+        for (int b = 0; b < ny; ++b)
+            for (int c = 0; c < nz; ++c) {
+                double r2 = 0.0;
+                for (int x = 0; x < 3; ++x) {
+                    double img = scabc[0][x] * shift[a] + scabc[1][x] * shift[b] +
+                                 scabc[2][x] * shift[c] + p1[x];
+                    double dr = p0[x] - img;
+                    r2 += dr * dr;
+                }
+                if (fabs(d - sqrt(r2)) <= tol) return 1;
+            }
+    return 0;
+}
+
+int o_neighbours_brute(const double* abc, int32_t n_basis, const double* basis_xyz,
+                       const int32_t* sc, const int32_t* open_bc, int32_t n_shells,
+                       const double* shell_d, double tol, int32_t capacity,
+                       int32_t* nn_count, int32_t* nn_idx) {
+    int32_t n = sc[0] * sc[1] * sc[2] * n_basis;
+    double* pos = (double*)malloc(sizeof(double) * 3 * (size_t)n);
+    if (!pos) return -1;
+    /* site order and positions: src/generate_supercell.f90:115-132 */
+    int32_t s = 0;
+    for (int k = 0; k < sc[2]; ++k)
+        for (int j = 0; j < sc[1]; ++j)
+            for (int i = 0; i < sc[0]; ++i)
+                for (int l = 0; l < n_basis; ++l, ++s)
+                    for (int x = 0; x < 3; ++x)
+                        pos[3 * s + x] = abc[0 * 3 + x] * i + abc[1 * 3 + x] * j +
+                                         abc[2 * 3 + x] * k + basis_xyz[3 * l + x];
+    double scabc[3][3];
+    for (int a = 0; a < 3; ++a)
+        for (int x = 0; x < 3; ++x) scabc[a][x] = sc[a] * abc[a * 3 + x];
+    int rc = 0;
+    for (int32_t i = 0; i < n && rc == 0; ++i)
+        for (int32_t sh = 0; sh < n_shells && rc == 0; ++sh) {
+            int32_t cnt = 0;
+            for (int32_t j = 0; j < n; ++j) {
+                if (j == i) continue;
+                if (search_nbd(pos + 3 * i, pos + 3 * j, shell_d[sh], tol, scabc, open_bc)) {
+                    if (cnt >= capacity) { rc = -2; break; }
+                    nn_idx[((size_t)sh * n + i) * capacity + cnt] = j;
+                    ++cnt;
+                }
+            }
+            nn_count[(size_t)sh * n + i] = cnt;
+        }
+    free(pos);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Hamiltonian: src/Hamiltonian.f90:20-54 (dispatch), JSiSj :57-105,        */
+/* gmbSH :108-124, siaS2 :127-143.                                           */
+/* ------------------------------------------------------------------------ */
+static const double* jvec(const o_model* m, int sh, int a, int b) {
+    return m->j_exc + (((size_t)sh * m->n_species + (a - 1)) * m->n_species + (b - 1)) * 3;
+}
+
+/* JSiSj: sum over shells and neighbours of (Si o Sj) . J */
+static double jsisj(const o_model* m, const double* S, int32_t i, const double* si) {
+    double e = 0.0;
+    int a = m->species[i];
+    for (int sh = 0; sh < m->n_shells; ++sh) {
+        int cnt = m->nn_count[(size_t)sh * m->n_sites + i];
+        const int32_t* nb = m->nn_idx + ((size_t)sh * m->n_sites + i) * m->capacity;
+        for (int k = 0; k < cnt; ++k) {
+            int j = nb[k];
+            const double* J = jvec(m, sh, a, m->species[j]);
+            const double* sj = S + 3 * (size_t)j;
+            e += si[0] * sj[0] * J[0] + si[1] * sj[1] * J[1] + si[2] * sj[2] * J[2];
+        }
+    }
+    return e;
+}
+
+static double gmbsh(const o_model* m, int32_t i, const double* si) {
+    double s = m->spin[m->species[i] - 1];
+    return -m->g_factor * m->mu_b * s * (si[0] * m->h[0] + si[1] * m->h[1] + si[2] * m->h[2]);
+}
+
+static double sias2(const o_model* m, int32_t i, const double* si2) {
+    const double* d = m->sia_vec + 3 * (size_t)(m->species[i] - 1);
+    return d[0] * si2[0] + d[1] * si2[1] + d[2] * si2[2];
+}
+
+double o_site_energy(const o_model* m, const double* S, int32_t i) {
+    const double* si = S + 3 * (size_t)i;
+    double si2[3] = {si[0] * si[0], si[1] * si[1], si[2] * si[2]};
+    double e = jsisj(m, S, i, si);
+    if (m->zeeman) e += gmbsh(m, i, si);
+    if (m->sia && m->model == 1) e += sias2(m, i, si2);
+    return e;
+}
+
+double o_delta_energy(const o_model* m, const double* S, int32_t i, const double* s_new) {
+    const double* so = S + 3 * (size_t)i;
+    double d[3] = {s_new[0] - so[0], s_new[1] - so[1], s_new[2] - so[2]};
+    double d2[3] = {s_new[0] * s_new[0] - so[0] * so[0], s_new[1] * s_new[1] - so[1] * so[1],
+                    s_new[2] * s_new[2] - so[2] * so[2]};
+    double e = jsisj(m, S, i, d);
+    if (m->zeeman) e += gmbsh(m, i, d);
+    if (m->sia && m->model == 1) e += sias2(m, i, d2);
+    return e;
+}
+
+/* JSiSj_ovrr: A = sum_j J o Sj (src/get_ovrr_vec.f90:41-85) */
+void o_exchange_field(const o_model* m, const double* S, int32_t i, double* h3) {
+    h3[0] = h3[1] = h3[2] = 0.0;
+    int a = m->species[i];
+    for (int sh = 0; sh < m->n_shells; ++sh) {
+        int cnt = m->nn_count[(size_t)sh * m->n_sites + i];
+        const int32_t* nb = m->nn_idx + ((size_t)sh * m->n_sites + i) * m->capacity;
+        for (int k = 0; k < cnt; ++k) {
+            int j = nb[k];
+            const double* J = jvec(m, sh, a, m->species[j]);
+            const double* sj = S + 3 * (size_t)j;
+            h3[0] += J[0] * sj[0];
+            h3[1] += J[1] * sj[1];
+            h3[2] += J[2] * sj[2];
+        }
+    }
+}
+
+/* get_tot_energy (src/get_tot_energy.f90:35-45), split into its three terms */
+void o_total_energy(const o_model* m, const double* S, double* parts3) {
+    double ex = 0.0, ze = 0.0, si = 0.0;
+    for (int32_t i = 0; i < m->n_sites; ++i) {
+        const double* s = S + 3 * (size_t)i;
+        double s2[3] = {s[0] * s[0], s[1] * s[1], s[2] * s[2]};
+        ex += jsisj(m, S, i, s);
+        if (m->zeeman) ze += gmbsh(m, i, s);
+        if (m->sia && m->model == 1) si += sias2(m, i, s2);
+    }
+    parts3[0] = ex; parts3[1] = ze; parts3[2] = si;
+}
+
+void o_ising_bond_sums(const o_model* m, const double* S, const int32_t* class_of_edge,
+                       int32_t n_classes, int64_t* bonds) {
+    for (int k = 0; k < n_classes; ++k) bonds[k] = 0;
+    for (int32_t i = 0; i < m->n_sites; ++i) {
+        int si = S[3 * (size_t)i + 2] > 0 ? 1 : -1;
+        for (int sh = 0; sh < m->n_shells; ++sh) {
+            int cnt = m->nn_count[(size_t)sh * m->n_sites + i];
+            size_t base = ((size_t)sh * m->n_sites + i) * m->capacity;
+            for (int k = 0; k < cnt; ++k) {
+                int c = class_of_edge[base + k];
+                if (c < 0) continue;
+                int sj = S[3 * (size_t)m->nn_idx[base + k] + 2] > 0 ? 1 : -1;
+                bonds[c] += si * sj;
+            }
+        }
+    }
+}
+
+/* get_tot_magnetisation (src/get_tot_magnetisation.f90:35-38) */
+void o_magnetisation(const o_model* m, const double* S, double* M3) {
+    M3[0] = M3[1] = M3[2] = 0.0;
+    for (int32_t i = 0; i < m->n_sites; ++i) {
+        double w = m->stagger[i] * m->spin[m->species[i] - 1];
+        for (int x = 0; x < 3; ++x) M3[x] += w * S[3 * (size_t)i + x];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Proposals: src/update_spin_details.f90:28-51, George_Marsaglia :29-39     */
+/* ------------------------------------------------------------------------ */
+static void normalise(double* v) {
+    /* ref2(1:3) = ref2(1:3)/sqrt(dot_product(...)) (src/update_spin_details.f90:51) */
+    double n = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    v[0] /= n; v[1] /= n; v[2] /= n;
+}
+
+static double wrap_pi(double p) {
+    const double two_pi = 6.283185307179586476925286766559;
+    return p - two_pi * floor((p + 3.14159265358979323846264338327950) / two_pi);
+}
+
+void o_propose(const o_model* m, uint64_t seed, uint32_t id, uint32_t replica, uint32_t step,
+               uint32_t purpose, const double* s_old, double phi_old, double* s_new,
+               double* phi_new, double* eta) {
+    uint32_t w[4];
+    o_rng(seed, id, replica, step, purpose, 0, w);
+    *eta = eta24(w[2]);
+    if (m->model == 0) {           /* Ising: ref2(3) = -ref1(3) */
+        s_new[0] = s_old[0]; s_new[1] = s_old[1]; s_new[2] = -s_old[2];
+        *phi_new = phi_old;
+        return;
+    }
+    if (!m->angle) {
+        /* Marsaglia 1972: v1,v2 ~ U(-1,1), retry while s >= 1.  Two candidate
+         * pairs per Philox block; the acceptance test is exact in integers. */
+        for (uint32_t att = 0;; ++att) {
+            if (att > 0) o_rng(seed, id, replica, step, purpose, att, w);
+            uint32_t cx[2] = {w[0] >> 8, ((w[3] >> 16) << 8) | (w[0] & 0xffu)};
+            uint32_t cy[2] = {w[1] >> 8, ((w[3] & 0xffffu) << 8) | (w[1] & 0xffu)};
+            for (int c = 0; c < 2; ++c) {
+                int64_t a1 = (int64_t)cx[c] - (1 << 23), a2 = (int64_t)cy[c] - (1 << 23);
+                int64_t q = a1 * a1 + a2 * a2;
+                if (q < ((int64_t)1 << 46)) {
+                    double v1 = a1 * 0x1p-23, v2 = a2 * 0x1p-23, s = q * 0x1p-46;
+                    double r = sqrt(1.0 - s);
+                    s_new[0] = 2.0 * v1 * r;
+                    s_new[1] = 2.0 * v2 * r;
+                    s_new[2] = 1.0 - 2.0 * s;
+                    normalise(s_new);
+                    *phi_new = 0.0;   /* ref2(5) = 0 */
+                    return;
+                }
+            }
+        }
+    }
+    /* ANGLE: S_z ~ U(-1,1); phi' = phi + dphi*U(-1,1) (src/update_spin_details.f90:39-46) */
+    double sz = sym24(w[0] >> 8);
+    double phi = wrap_pi(phi_old + m->dphi * sym24(w[1] >> 8));
+    double r = sqrt(1.0 - sz * sz);
+    s_new[0] = r * cos(phi);
+    s_new[1] = r * sin(phi);
+    s_new[2] = sz;
+    normalise(s_new);
+    *phi_new = phi;
+}
+
+uint32_t o_ising_threshold(double beta, double dE) {
+    double p = exp(-beta * dE);
+    if (!(p < 1.0)) return 0xffffffffu;
+    double t = floor(p * 4294967296.0);
+    if (t > 4294967294.0) t = 4294967294.0;
+    return (uint32_t)t;
+}
+
+/* Metropolis (src/Metropolis.f90:35-62): propose, dE, accept if -beta*dE > log(eta) */
+int64_t o_sweep(const o_model* m, double* S, double* phi, const int32_t* order, int32_t n,
+                const int32_t* rng_id, double beta, uint64_t seed, uint32_t replica,
+                uint32_t step) {
+    int64_t acc = 0;
+    for (int32_t t = 0; t < n; ++t) {
+        int32_t i = order[t];
+        double* so = S + 3 * (size_t)i;
+        double sn[3], phin, eta;
+        if (m->model == 0) {
+            uint32_t w[4];
+            uint32_t id = (uint32_t)rng_id[i];
+            o_rng(seed, id >> 2, replica, step, O_RNG_METROPOLIS, 0, w);
+            sn[0] = so[0]; sn[1] = so[1]; sn[2] = -so[2];
+            double dE = o_delta_energy(m, S, i, sn);
+            uint32_t thr = o_ising_threshold(beta, dE);
+            uint32_t u = w[id & 3u];
+            if (u < thr || thr == 0xffffffffu) {
+                so[2] = sn[2];
+                ++acc;
+            }
+            continue;
+        }
+        double ph = phi ? phi[i] : 0.0;
+        o_propose(m, seed, (uint32_t)rng_id[i], replica, step, O_RNG_METROPOLIS, so, ph, sn,
+                  &phin, &eta);
+        double dE = o_delta_energy(m, S, i, sn);
+        if (-beta * dE > log(eta)) {
+            so[0] = sn[0]; so[1] = sn[1]; so[2] = sn[2];
+            if (phi) phi[i] = phin;
+            ++acc;
+        }
+    }
+    return acc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Overrelaxation: src/overrelaxation.f90:30-49                              */
+/* ------------------------------------------------------------------------ */
+void o_reflect(const o_model* m, double* S, int32_t i) {
+    double A[3];
+    o_exchange_field(m, S, i, A);
+    if (m->zeeman) {
+        double s = m->spin[m->species[i] - 1];
+        for (int x = 0; x < 3; ++x) A[x] += -m->g_factor * m->mu_b * s * m->h[x];
+    }
+    double aa = A[0] * A[0] + A[1] * A[1] + A[2] * A[2];
+    if (!(aa > 0.0)) return;
+    double* si = S + 3 * (size_t)i;
+    double f = 2.0 * (A[0] * si[0] + A[1] * si[1] + A[2] * si[2]) / aa;
+    double r[3] = {f * A[0] - si[0], f * A[1] - si[1], f * A[2] - si[2]};
+    normalise(r);
+    si[0] = r[0]; si[1] = r[1]; si[2] = r[2];
+}
+
+void o_overrelax_pass(const o_model* m, double* S, double* phi, const int32_t* order,
+                      int32_t n, const int32_t* rng_id, double frac, uint64_t seed,
+                      uint32_t replica, uint32_t step, uint32_t pass) {
+    for (int32_t t = 0; t < n; ++t) {
+        int32_t i = order[t];
+        if (frac < 1.0) {
+            uint32_t w[4];
+            o_rng(seed, (uint32_t)rng_id[i], replica, step, O_RNG_OVRR, pass, w);
+            if (!((double)(w[0] >> 8) * 0x1p-24 < frac)) continue;
+        }
+        o_reflect(m, S, i);
+        if (phi) phi[i] = atan2(S[3 * (size_t)i + 1], S[3 * (size_t)i]);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Estimators                                                                */
+/* ------------------------------------------------------------------------ */
+void o_repeat_observables(const double* mom, double beta, int32_t n_sites, double* out6) {
+    double n = mom[0];
+    /* get_eng 'avg_eng' (src/get_eng.f90:43-56) */
+    double e = mom[1] / n, e2 = mom[2] / n, e4 = mom[3] / n;
+    out6[0] = e;
+    out6[1] = 1.0 - (1.0 / 3.0) * (e4 / (e2 * e2));
+    out6[2] = beta * beta * (e2 - e * e) * n_sites;
+    /* get_mag 'avg_mag' (src/get_mag.f90:44-55) */
+    double mm = mom[4] / n, m2 = mom[5] / n, m4 = mom[6] / n;
+    out6[3] = mm;
+    out6[4] = 1.0 - (1.0 / 3.0) * (m4 / (m2 * m2));
+    out6[5] = beta * (m2 - mm * mm) * n_sites;
+}
+
+void o_over_repeats(const double* per, int32_t n_repeat, double* mean6, double* sem6) {
+    /* src/observables_per_repeat.f90:29-69 */
+    double r1 = 1.0 / (n_repeat * (n_repeat - 1.0));
+    for (int c = 0; c < 6; ++c) {
+        double s = 0.0;
+        for (int r = 0; r < n_repeat; ++r) s += per[6 * r + c];
+        double mean = s / n_repeat, e = 0.0;
+        for (int r = 0; r < n_repeat; ++r) {
+            double d = per[6 * r + c] - mean;
+            e += d * d;
+        }
+        mean6[c] = mean;
+        sem6[c] = sqrt(r1 * e);
+    }
+}
+
+/* src/parallel_tempering.f90:77-89 */
+int o_pt_accept(double beta, double beta_p, double E, double E_p, double u) {
+    double delta = -(beta_p - beta) * (E - E_p);
+    if (delta > 0.0) return 1;
+    return u < exp(delta);
+}
+
+/* ------------------------------------------------------------------------ */
+/* MLP 1-64-64-2 (no reference counterpart)                                  */
+/* ------------------------------------------------------------------------ */
+#define H 64
+static void mlp_layers(const double* p, const double** W1, const double** b1,
+                       const double** W2, const double** b2, const double** W3,
+                       const double** b3) {
+    *W1 = p; *b1 = p + H; *W2 = p + 2 * H; *b2 = *W2 + H * H; *W3 = *b2 + H; *b3 = *W3 + 2 * H;
+}
+
+void o_mlp_forward(const double* p, int32_t n, const double* x, double* y) {
+    const double *W1, *b1, *W2, *b2, *W3, *b3;
+    mlp_layers(p, &W1, &b1, &W2, &b2, &W3, &b3);
+    for (int32_t s = 0; s < n; ++s) {
+        double h1[H], h2[H];
+        for (int i = 0; i < H; ++i) h1[i] = tanh(W1[i] * x[s] + b1[i]);
+        for (int o = 0; o < H; ++o) {
+            double a = b2[o];
+            for (int i = 0; i < H; ++i) a += W2[o * H + i] * h1[i];
+            h2[o] = tanh(a);
+        }
+        for (int o = 0; o < 2; ++o) {
+            double a = b3[o];
+            for (int i = 0; i < H; ++i) a += W3[o * H + i] * h2[i];
+            y[2 * s + o] = a;
+        }
+    }
+}
+
+double o_mlp_loss_grad(const double* p, int32_t n, const double* x, const double* y,
+                       const double* w, double* grad) {
+    const double *W1, *b1, *W2, *b2, *W3, *b3;
+    mlp_layers(p, &W1, &b1, &W2, &b2, &W3, &b3);
+    double *gW1 = grad, *gb1 = grad + H, *gW2 = grad + 2 * H, *gb2 = gW2 + H * H,
+           *gW3 = gb2 + H, *gb3 = gW3 + 2 * H;
+    memset(grad, 0, sizeof(double) * O_MLP_NPARAM);
+    double loss = 0.0;
+    for (int32_t s = 0; s < n; ++s) {
+        double h1[H], h2[H], out[2], d2[H], d1[H];
+        for (int i = 0; i < H; ++i) h1[i] = tanh(W1[i] * x[s] + b1[i]);
+        for (int o = 0; o < H; ++o) {
+            double a = b2[o];
+            for (int i = 0; i < H; ++i) a += W2[o * H + i] * h1[i];
+            h2[o] = tanh(a);
+        }
+        double dout[2];
+        for (int o = 0; o < 2; ++o) {
+            double a = b3[o];
+            for (int i = 0; i < H; ++i) a += W3[o * H + i] * h2[i];
+            out[o] = a;
+            double r = out[o] - y[2 * s + o];
+            loss += w[2 * s + o] * r * r;
+            dout[o] = 2.0 * w[2 * s + o] * r / n;
+        }
+        for (int o = 0; o < 2; ++o) {
+            gb3[o] += dout[o];
+            for (int i = 0; i < H; ++i) gW3[o * H + i] += dout[o] * h2[i];
+        }
+        for (int i = 0; i < H; ++i) {
+            double g = dout[0] * W3[i] + dout[1] * W3[H + i];
+            d2[i] = g * (1.0 - h2[i] * h2[i]);
+        }
+        for (int o = 0; o < H; ++o) {
+            gb2[o] += d2[o];
+            for (int i = 0; i < H; ++i) gW2[o * H + i] += d2[o] * h1[i];
+        }
+        for (int i = 0; i < H; ++i) {
+            double g = 0.0;
+            for (int o = 0; o < H; ++o) g += d2[o] * W2[o * H + i];
+            d1[i] = g * (1.0 - h1[i] * h1[i]);
+            gb1[i] += d1[i];
+            gW1[i] += d1[i] * x[s];
+        }
+    }
+    return loss / n;
+}
+#undef H
+
+/* ------------------------------------------------------------------------ */
+/* S(q): utility/Structure_Factor/structure_factor.f90:150-177              */
+/* ------------------------------------------------------------------------ */
+double o_structure_factor(int32_t n, const double* pos, const double* S, const double* q3) {
+    double re[3] = {0, 0, 0}, im[3] = {0, 0, 0};
+    for (int32_t i = 0; i < n; ++i) {
+        double ph = q3[0] * pos[3 * i] + q3[1] * pos[3 * i + 1] + q3[2] * pos[3 * i + 2];
+        double c = cos(ph), s = sin(ph);
+        for (int a = 0; a < 3; ++a) {
+            re[a] += S[3 * i + a] * c;
+            im[a] += S[3 * i + a] * s;
+        }
+    }
+    double t = 0.0;
+    for (int a = 0; a < 3; ++a) t += re[a] * re[a] + im[a] * im[a];
+    return t / ((double)n * (double)n);
+}

@@ -1,0 +1,230 @@
+"""GPU parity against the CPU oracle, through the C-ABI.
+
+Ising: bit-exact.  The device and the oracle draw the same Philox words and
+compare them with the same integer thresholds, so after K sweeps every spin of
+every replica must be identical, and the integer bond sums behind E_tot must
+be equal.
+
+Heisenberg (FP32 on the device, FP64 in the oracle): local fields and site
+energies within 1e-6 of the oracle relative to the field scale sum_j |J_j|;
+energies within 1e-6 relative to sum_i |e_i|; after one sweep every site whose
+Metropolis margin |-beta dE - ln eta| exceeds 1e-3 makes the same decision and
+lands within 2e-6 of the oracle's spin.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from tests.helpers import colour_order, pair, random_xyz, say
+from oracle import oracle as o
+
+pytestmark = pytest.mark.gpu
+
+
+def _ising_case(name, R, K, **over):
+    say(f"ising parity {name} {over} R={R} K={K}")
+    m, eng, orc = pair(name, R, **over)
+    rng = np.random.default_rng(7)
+    temps = np.linspace(2.0, 6.0, R) if m.tags.para else np.linspace(10.0, 100.0, R)
+    betas = m.betas(temps)
+    eng.set_betas(betas)
+    states = []
+    for r in range(R):
+        S = np.zeros((m.n_sites, 3))
+        S[:, 2] = np.where(rng.random(m.n_sites) < 0.5, 1.0, -1.0)
+        eng.set_spins(r, S)
+        states.append(S)
+    eng.sweep(K)
+    nc, col, order, rid = colour_order(eng)
+    for r in range(R):
+        S = states[r]
+        acc = 0
+        for step in range(K):
+            acc += orc.sweep(S, None, order, rid, betas[r], m.tags.seed, r, step)
+        Sg, _ = eng.get_spins(r)
+        assert np.array_equal(Sg[:, 2], S[:, 2]), f"replica {r}: {np.sum(Sg[:, 2] != S[:, 2])} spins differ"
+    # energies: exact integer bond sums per coupling class
+    eng.measure()
+    parts, bonds = eng.energy_parts()
+    for r in range(R):
+        S = states[r]
+        ex, ze, si = orc.total_energy(S)
+        assert parts[r, 0] == pytest.approx(ex, rel=1e-12, abs=1e-12)
+        assert parts[r, 1] == pytest.approx(ze, rel=1e-12, abs=1e-12)
+        if bonds.shape[1] == 1:
+            # one class: B = E_exch / J exactly
+            jzz = m.j_exc.reshape(-1, 3)[np.flatnonzero(m.j_exc.reshape(-1, 3)[:, 2])[0], 2]
+            assert bonds[r, 0] == round(ex / jzz)
+    eng.close()
+
+
+def test_ising_sc_bit_exact():
+    _ising_case("sc_ising_16", 8, 6, sc="8 8 8", para=".T. 1")
+
+
+def test_ising_sc_mev_units_bit_exact():
+    _ising_case("sc_ising_16", 4, 4, sc="6 6 6")
+
+
+def test_ising_pyrochlore_two_shells_zeeman_bit_exact():
+    _ising_case("pyro_pt_16", 4, 4, sc="2 2 2", model="ising", para=".T. 1", pt=".F. 11",
+                zeeman=".T. 0 0 0.3")
+
+
+def _xyz_state(m, eng, R, seed):
+    rng = np.random.default_rng(seed)
+    states = []
+    for r in range(R):
+        S = random_xyz(rng, m.n_sites).astype(np.float32).astype(np.float64)
+        phi = np.arctan2(S[:, 1], S[:, 0]).astype(np.float32).astype(np.float64)
+        eng.set_spins(r, S, phi)
+        states.append((S, phi))
+    return states
+
+
+@pytest.mark.parametrize("name,over", [
+    ("sc_heis_afm_32", dict(sc="8 8 8")),
+    ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11")),
+    ("tri_afm_96", dict(sc="12 12 1")),
+])
+def test_xyz_local_fields_and_energy(name, over):
+    say(f"xyz fields {name} {over}")
+    m, eng, orc = pair(name, 2, **over)
+    eng.set_betas(m.betas([5.0, 10.0]))
+    states = _xyz_state(m, eng, 2, 3)
+    jabs = np.abs(m.j_exc).max() * m.nn_count.sum(axis=0).max()
+    for r in range(2):
+        S, _ = states[r]
+        h, e = eng.local_fields(r)
+        ho = np.array([orc.exchange_field(S, i) for i in range(m.n_sites)])
+        eo = np.array([orc.site_energy(S, i) for i in range(m.n_sites)])
+        assert np.max(np.abs(h - ho)) <= 1e-6 * jabs
+        escale = jabs + np.abs(m.h).max() * m.mu_b * m.g_factor + (
+            np.abs(m.sia_vec).max() if m.sia_vec is not None else 0.0)
+        assert np.max(np.abs(e - eo)) <= 1e-6 * escale
+    eng.measure()
+    parts, _ = eng.energy_parts()
+    for r in range(2):
+        S, _ = states[r]
+        ex, ze, si = orc.total_energy(S)
+        scale = sum(abs(orc.site_energy(S, i)) for i in range(m.n_sites))
+        assert abs(parts[r, 0] - ex) <= 1e-6 * scale
+        assert abs(parts[r, 1] - ze) <= 1e-6 * scale + 1e-300
+        assert abs(parts[r, 2] - si) <= 1e-6 * scale + 1e-300
+        assert parts[r, 3] == pytest.approx(0.5 * parts[r, 0] + parts[r, 1] + parts[r, 2])
+    eng.close()
+
+
+def _oracle_sweep_with_margins(m, orc, S, phi, order, rid, beta, seed, r, step):
+    """The oracle's sweep, site by site, recording -beta dE - ln eta."""
+    marg = np.zeros(m.n_sites)
+    for i in order:
+        sn, ph, eta = orc.propose(seed, int(rid[i]), r, step, S[i], phi[i])
+        dE = orc.delta_energy(S, i, sn)
+        marg[i] = -beta * dE - math.log(eta)
+        if marg[i] > 0:
+            S[i] = sn
+            phi[i] = ph
+    return marg
+
+
+@pytest.mark.parametrize("name,over", [
+    ("sc_heis_afm_32", dict(sc="8 8 8")),
+    ("tri_afm_96", dict(sc="12 12 1")),                       # ANGLE + Zeeman + SIA
+    ("tri_afm_96", dict(sc="12 12 1", angle=".F. 0")),        # Marsaglia + Zeeman + SIA
+    ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11")),
+])
+def test_xyz_one_sweep_matches_oracle(name, over):
+    say(f"xyz sweep {name} {over}")
+    R = 2
+    m, eng, orc = pair(name, R, **over)
+    temps = [2.0, 8.0] if name == "pyro_pt_16" else [5.0, 20.0]
+    betas = m.betas(temps)
+    eng.set_betas(betas)
+    states = _xyz_state(m, eng, R, 5)
+    eng.sweep(1)
+    nc, col, order, rid = colour_order(eng)
+    nbrs = [set() for _ in range(m.n_sites)]
+    for s in range(m.nn_count.shape[0]):
+        for i in range(m.n_sites):
+            nbrs[i].update(m.nn_idx[s, i, : m.nn_count[s, i]].tolist())
+    n_checked = 0
+    for r in range(R):
+        S0, phi0 = states[r]
+        S, phi = S0.copy(), phi0.copy()
+        marg = _oracle_sweep_with_margins(m, orc, S, phi, order, rid, betas[r], m.tags.seed, r, 0)
+        Sg, phig = eng.get_spins(r)
+        tainted = set()
+        for i in order:
+            if i in tainted:
+                continue
+            dev = np.abs(Sg[i] - S[i]).max()
+            if dev > 2e-6:
+                assert abs(marg[i]) < 1e-3, f"replica {r} site {i}: margin {marg[i]}, dev {dev}"
+                tainted |= nbrs[i]
+            else:
+                n_checked += 1
+    assert n_checked > 0.99 * R * m.n_sites
+    eng.close()
+
+
+@pytest.mark.parametrize("frac", [1.0, 0.5])
+def test_overrelaxation_matches_oracle(frac):
+    say(f"ovrr frac={frac}")
+    m, eng, orc = pair("sc_heis_afm_32", 2, sc="6 6 6")
+    eng.set_betas(m.betas([5.0, 10.0]))
+    states = _xyz_state(m, eng, 2, 9)
+    eng.measure()
+    e_before = eng.energy_parts()[0][:, 3].copy()
+    eng.overrelax(frac)
+    eng.measure()
+    e_after = eng.energy_parts()[0][:, 3]
+    nc, col, order, rid = colour_order(eng)
+    for r in range(2):
+        S, phi = states[r]
+        S = S.copy()
+        phi = phi.copy()
+        orc.overrelax_pass(S, phi, order, rid, frac, m.tags.seed, r, 0, 0)
+        Sg, _ = eng.get_spins(r)
+        assert np.max(np.abs(Sg - S)) < 2e-5
+        assert np.allclose(np.linalg.norm(Sg, axis=1), 1.0, atol=1e-6)
+    # reflections conserve the energy
+    assert np.allclose(e_after, e_before, rtol=1e-5, atol=1e-9)
+    eng.close()
+
+
+def test_pt_swaps_match_oracle():
+    say("pt swaps")
+    R = 8
+    m, eng, orc = pair("sc_heis_afm_32", R, sc="4 4 4", para=".T. 1")
+    betas = np.linspace(0.1, 0.45, R)
+    eng.set_betas(betas)
+    eng.set_ladders([0, R])
+    states = _xyz_state(m, eng, R, 13)
+    E = []
+    for S, _ in states:
+        ex, ze, si = orc.total_energy(S)
+        E.append(0.5 * ex + ze + si)
+    E = np.array(E)
+    rep_at = list(range(R))
+    total = 0
+    for call in range(24):
+        parity = call % 2
+        n_acc = eng.pt_swap(parity)
+        acc = 0
+        for k in range(parity, R - 1, 2):
+            ra, rb = rep_at[k], rep_at[k + 1]
+            u = (int(o.rng(m.tags.seed, k, 0, call, o.RNG_PT)[0]) >> 8) / 2.0 ** 24
+            if o.pt_accept(betas[k], betas[k + 1], E[ra], E[rb], u):
+                rep_at[k], rep_at[k + 1] = rb, ra
+                acc += 1
+        assert n_acc == acc, f"call {call}"
+        total += acc
+        slot_of, pacc, patt, trips = eng.pt_state()
+        expect = np.empty(R, np.int32)
+        for k, r in enumerate(rep_at):
+            expect[r] = k
+        assert np.array_equal(slot_of, expect), f"call {call}"
+    assert 0 < total < 24 * 4
+    eng.close()

@@ -1,0 +1,74 @@
+"""Neighbour construction.
+
+The library's translation-symmetric builder (ether_build_neighbours, host code)
+must return exactly what the oracle's restatement of the reference's O(N^2*27)
+search (src/getting_nbd.f90:46-82, src/search_nbd.f90:34-58) returns, on the
+reference's own example inputs and on open/closed and small-L edge cases.
+Known counts pin both: 6 for SC, 6/12/8 for the three SC shells, 6 for the
+pyrochlore first shell and 12 for its second.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from ether_amd import build_neighbours
+from ether_amd.inputs import read_directory
+from ether_amd import workloads
+from oracle import oracle as o
+
+EX = os.path.join(os.path.dirname(__file__), "golden", "examples")
+
+
+def _compare(abc, basis, sc, obc, shells, tol=1e-5, cap=20):
+    c1, i1 = build_neighbours(abc, basis, sc, obc, shells, tol, cap)
+    c2, i2 = o.neighbours_brute(abc, basis, sc, obc, shells, tol, cap)
+    assert np.array_equal(c1, c2)
+    for s in range(len(shells)):
+        for i in range(c1.shape[1]):
+            n = c1[s, i]
+            assert np.array_equal(i1[s, i, :n], i2[s, i, :n]), (s, i)
+    return c1
+
+
+@pytest.mark.parametrize("name,expect", [
+    ("SimpleCubic_FM", [6]),
+    ("SimpleCubic_AFM", [6]),
+    ("SimpleCubic_333", [6, 12, 8]),
+    ("Pyrochlore", [6]),
+])
+def test_examples_match_brute_force(name, expect):
+    m = read_directory(os.path.join(EX, name))
+    sc = tuple(min(x, 4) for x in m.sc)
+    cnt = _compare(m.structure.abc, m.basis_xyz, sc, [b == "o" for b in m.tags.bc], m.shells,
+                   m.tags.nbdfc, m.tags.nbd_capacity)
+    for s, e in enumerate(expect):
+        assert np.all(cnt[s] == e), (name, s, np.unique(cnt[s]))
+
+
+def test_triangular_example():
+    m = read_directory(os.path.join(EX, "Triangular"))
+    cnt = _compare(m.structure.abc, m.basis_xyz, (4, 4, 3), [False] * 3, m.shells, m.tags.nbdfc, 20)
+    # in-plane A-A / B-B at a = 4.19767 (6), stacking A-B at c/2 = 6.0842 (2)
+    assert np.all(cnt[0] == 6)
+    assert np.all(cnt[1] == 2)
+
+
+def test_pyrochlore_two_shells():
+    m = workloads.model("pyro_pt_16", sc="3 3 3")
+    cnt = _compare(m.structure.abc, m.basis_xyz, m.sc, [False] * 3, m.shells, m.tags.nbdfc, 20)
+    assert np.all(cnt[0] == 6) and np.all(cnt[1] == 12)
+
+
+@pytest.mark.parametrize("sc,obc", [((1, 1, 1), (0, 0, 0)), ((2, 2, 2), (0, 0, 0)),
+                                    ((3, 2, 5), (1, 0, 0)), ((4, 4, 4), (1, 1, 1)),
+                                    ((2, 3, 1), (0, 1, 0))])
+def test_small_and_open_edge_cases(sc, obc):
+    abc = np.eye(3)
+    _compare(abc, [[0, 0, 0]], sc, obc, [1.0, 2 ** 0.5, 3 ** 0.5])
+
+
+def test_capacity_overflow_raises():
+    from ether_amd import EtherError
+    with pytest.raises(EtherError):
+        build_neighbours(np.eye(3), [[0, 0, 0]], (4, 4, 4), (0, 0, 0), [1.0], 1e-5, 4)

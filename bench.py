@@ -63,15 +63,39 @@ def cpu_baseline(model, budget_s=10.0):
                       f"Heisenberg chain at T=20 K, FP64 C oracle, single thread, {dt:.1f} s"}
 
 
+def copy_probe(n_bytes, reps):
+    """A plain device-to-device copy of known size (the HIP runtime's copy
+    kernel), used to calibrate the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n_bytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n_bytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n_bytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n_bytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": n_bytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="run only a device copy of this many bytes (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="sc_heis_afm_32")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-pt", action="store_true", help="skip the tempering ladder comparison")
+    ap.add_argument("--pt-mcs", type=int, default=20000)
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe, 4)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the replicas "
               "of a run are independent and would shard across GPUs, see DESIGN.md", file=sys.stderr)
@@ -154,9 +178,40 @@ def main():
         "acceptance_mean": float(mom[:, E.MOM_ACC].mean() / args.steps),
     }
     eng.close()
+    if not args.no_pt:
+        out["pt"] = pt_comparison(args.pt_mcs)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, args.cpu_budget)
     print(json.dumps(out), flush=True)
+
+
+def pt_comparison(mcs):
+    """Pyrochlore tempering: linear ladder versus the learned ladder.
+
+    pyro_pt_16: 16^3 cubic cells x 16 sites (65536 spins), J1 = 1, J2 = 0.05 meV,
+    64 temperatures 1 -> 0.05 K, swaps every 11 MCS, REPEAT 4 (256 chains); the
+    MLP is trained on <e> and C_v measured in the first 2000 MCS on the linear
+    ladder.  Reports completed cold->hot->cold round trips per second of the
+    production loop and the spin updates per second of the same loop."""
+    from ether_amd import ladder, workloads
+    m = workloads.model("pyro_pt_16")
+    t0 = time.perf_counter()
+    lin, lrn, loss = ladder.compare_ladders(m, train_mcs=2000, mcs=mcs)
+    total = time.perf_counter() - t0
+
+    def rep(run):
+        return {"round_trips": run.round_trips, "wall_s": run.wall_s,
+                "round_trips_per_s": run.round_trips_per_s,
+                "spin_updates_per_s": run.updates_per_s,
+                "swap_acc_min": float(run.acc_rate.min()), "swap_acc_max": float(run.acc_rate.max()),
+                "temps": [round(float(x), 6) for x in run.temps]}
+
+    return {"workload": f"pyro_pt_16: pyrochlore 16^3x16 ({m.n_sites} spins), J1=1 J2=0.05 meV, "
+                        f"64 T 1->0.05 K x 4 repeats = 256 chains, swaps every 11 MCS, MCS {mcs}",
+            "linear": rep(lin), "learned": rep(lrn), "mlp_final_loss": loss,
+            "gain_round_trips_per_s": (lrn.round_trips_per_s / lin.round_trips_per_s
+                                       if lin.round_trips_per_s > 0 else None),
+            "total_s": total}
 
 
 if __name__ == "__main__":

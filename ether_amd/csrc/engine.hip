@@ -50,14 +50,15 @@ struct ether_ctx {
     int ncol = 0;
     std::vector<int> cbeg;
     std::vector<int> slot_of_site, site_of_slot;
-    int D = 0, ell_stride = 0;
+    int D = 0;
+    int Rp = 0;           // replicas padded to a multiple of 64 (replica-major planes)
     int n_classes = 0;
     std::vector<float> jclass;
     std::vector<double> ising_j;
     std::vector<int> ising_Z, ising_stride;
     int ising_P = 1, ising_offset = 0, ising_table = 0;
     int xcd_map = 0;
-    int meas_tiles = 0;
+    int meas_tiles = 0, meas_sites = 0, meas_rec = 0;
     // device
     uint32_t* d_ell = nullptr;
     uint8_t* d_sp = nullptr;
@@ -469,12 +470,12 @@ extern "C" int ether_commit(ether_ctx* c) {
     int D = 0;
     for (int i = 0; i < N; ++i) D = std::max(D, (int)edges[i].size());
     c->D = D;
-    c->ell_stride = (N + 63) / 64 * 64;
-    std::vector<uint32_t> ell((size_t)std::max(D, 1) * c->ell_stride, 0u);
+    // slot-major rows: a wave's whole neighbour list is one scalar-loadable run
+    std::vector<uint32_t> ell((size_t)std::max(D, 1) * N, 0u);
     for (int i = 0; i < N; ++i) {
         const int s = c->slot_of_site[i];
         for (size_t k = 0; k < edges[i].size(); ++k)
-            ell[k * c->ell_stride + s] =
+            ell[(size_t)s * D + k] =
                 (uint32_t)c->slot_of_site[edges[i][k].j] | ((uint32_t)edges[i][k].cls << 24);
     }
     // ---- Ising index tables ----
@@ -518,10 +519,15 @@ extern "C" int ether_commit(ether_ctx* c) {
             for (int x = 0; x < 3; ++x) dsia[3 * a + x] = (float)c->siav[3 * a + x];
     }
     // ---- device buffers ----
-    const size_t RN = (size_t)c->R * N;
     const int R = c->R;
-    c->meas_tiles = (N + kMeasTile - 1) / kMeasTile;
-    c->xcd_map = (R >= 32 || R % 8 == 0) ? 1 : 0;
+    c->Rp = (R + kWave - 1) / kWave * kWave;
+    const size_t RN = (size_t)c->Rp * N;
+    const int G = c->Rp / kWave;
+    // measurement: about 4096 wave units per pass, at least 8 sites each
+    c->meas_sites = std::max(8, (int)(((long long)N * G + 4095) / 4096));
+    c->meas_tiles = (N + c->meas_sites - 1) / c->meas_sites;
+    c->meas_rec = ising ? (c->n_classes + nsp + 1) : 6;
+    c->xcd_map = 1;
     TRY(dev_alloc(c, &c->d_ell, ell.size()));
     TRY(dev_alloc(c, &c->d_sp, (size_t)N));
     TRY(dev_alloc(c, &c->d_w, (size_t)N));
@@ -537,7 +543,7 @@ extern "C" int ether_commit(ether_ctx* c) {
         TRY(dev_alloc(c, &c->d_bonds, (size_t)R * (c->n_classes + 1)));
         TRY(upload(c, c->d_stride, c->ising_stride.data(), c->ising_stride.size()));
         HIPCHK(c, hipMemsetAsync(c->d_bonds, 0, sizeof(long long) * R * (c->n_classes + 1), c->stream));
-        // ordered start, as fresh_spins would leave the all -1 basis state flipped
+        // ordered start (all +1), padding replicas included
         HIPCHK(c, hipMemsetAsync(c->d_s, 1, RN, c->stream));
     } else {
         TRY(dev_alloc(c, &c->d_sx, RN));
@@ -547,7 +553,7 @@ extern "C" int ether_commit(ether_ctx* c) {
         HIPCHK(c, hipMemsetAsync(c->d_sx, 0, RN * 4, c->stream));
         HIPCHK(c, hipMemsetAsync(c->d_sy, 0, RN * 4, c->stream));
         if (c->angle) HIPCHK(c, hipMemsetAsync(c->d_phi, 0, RN * 4, c->stream));
-        std::vector<float> ones(RN, 1.0f);
+        std::vector<float> ones(RN, 1.0f);   // every spin (and padding lane) along +z
         TRY(upload(c, c->d_sz, ones.data(), RN));
     }
     TRY(dev_alloc(c, &c->d_slot_of, (size_t)R));
@@ -556,7 +562,7 @@ extern "C" int ether_commit(ether_ctx* c) {
     TRY(dev_alloc(c, &c->d_beta, (size_t)R));
     TRY(dev_alloc(c, &c->d_acc, (size_t)R));
     TRY(dev_alloc(c, &c->d_mom, (size_t)R * ETHER_MOM_COUNT));
-    TRY(dev_alloc(c, &c->d_partial, (size_t)R * c->meas_tiles * kRecord));
+    TRY(dev_alloc(c, &c->d_partial, (size_t)c->Rp * c->meas_tiles * c->meas_rec));
     TRY(dev_alloc(c, &c->d_Ept, (size_t)R));
     TRY(dev_alloc(c, &c->d_parts, (size_t)R * 4));
     TRY(dev_alloc(c, &c->d_M, (size_t)R * 3));
@@ -566,7 +572,7 @@ extern "C" int ether_commit(ether_ctx* c) {
     TRY(dev_alloc(c, &c->d_trip_state, (size_t)R));
     TRY(dev_alloc(c, &c->d_nacc, 1));
     TRY(dev_alloc(c, &c->d_gstart, (size_t)R + 1));
-    TRY(dev_alloc(c, &c->d_scratch_h, (size_t)N * 3));
+    TRY(dev_alloc(c, &c->d_scratch_h, (size_t)N * 4));
     TRY(dev_alloc(c, &c->d_scratch_e, (size_t)N));
     TRY(upload(c, c->d_ell, ell.data(), ell.size()));
     TRY(upload(c, c->d_sp, sp.data(), sp.size()));
@@ -693,6 +699,15 @@ extern "C" int ether_set_ladders(ether_ctx* c, int32_t n_groups, const int32_t* 
 // --------------------------------------------------------------------------
 // argument blocks
 // --------------------------------------------------------------------------
+// workgroups of a colour phase: (replica groups) x (site chunks of
+// kWavesPerBlock*kSitesPerWave sites), padded to a multiple of 8 for the XCD map
+static int phase_blocks(int n_sites, int groups, int xcd_map) {
+    const int chunk = kWavesPerBlock * kSitesPerWave;
+    long long nb = (long long)groups * ((n_sites + chunk - 1) / chunk);
+    if (xcd_map) nb = (nb + 7) / 8 * 8;
+    return (int)nb;
+}
+
 static XyzArgs xyz_args(ether_ctx* c, int colour) {
     XyzArgs a;
     a.sx = c->d_sx;
@@ -700,15 +715,16 @@ static XyzArgs xyz_args(ether_ctx* c, int colour) {
     a.sz = c->d_sz;
     a.phi = c->d_phi;
     a.ell = c->d_ell;
-    a.ell_stride = c->ell_stride;
     a.D = c->D;
     a.sp = c->d_sp;
     a.N = c->N;
     a.R = c->R;
+    a.Rp = c->Rp;
+    a.G = c->Rp / kWave;
     a.c_begin = c->cbeg[colour];
     a.c_end = c->cbeg[colour + 1];
-    a.tiles = (a.c_end - a.c_begin + kBlock - 1) / kBlock;
     a.xcd_map = c->xcd_map;
+    a.nblocks = phase_blocks(a.c_end - a.c_begin, a.G, a.xcd_map);
     a.slot_of = c->d_slot_of;
     a.beta_slot = c->d_beta_f;
     a.seed = c->seed;
@@ -729,16 +745,17 @@ static IsingArgs ising_args(ether_ctx* c, int colour) {
     IsingArgs a;
     a.s = c->d_s;
     a.ell = c->d_ell;
-    a.ell_stride = c->ell_stride;
     a.D = c->D;
     a.sp = c->d_sp;
     a.N = c->N;
     a.R = c->R;
+    a.Rp = c->Rp;
+    a.G4 = (c->Rp + 4 * kWave - 1) / (4 * kWave);
     a.c_begin = c->cbeg[colour];
     a.c_end = c->cbeg[colour + 1];
-    const int g0 = a.c_begin >> 2, g1 = (a.c_end + 3) >> 2;
-    a.tiles = (g1 - g0 + kBlock - 1) / kBlock;
     a.xcd_map = c->xcd_map;
+    a.nblocks = phase_blocks(a.c_end - a.c_begin, a.G4, a.xcd_map);
+    a.table_in_lds = ((long long)c->R * c->ising_table <= kIsingLdsTable) ? 1 : 0;
     a.slot_of = c->d_slot_of;
     a.seed = c->seed;
     a.step = c->step;
@@ -759,13 +776,16 @@ static MeasArgs meas_args(ether_ctx* c) {
     a.sz = c->d_sz;
     a.s = c->d_s;
     a.ell = c->d_ell;
-    a.ell_stride = c->ell_stride;
     a.D = c->D;
     a.sp = c->d_sp;
     a.w = c->d_w;
     a.N = c->N;
     a.R = c->R;
+    a.Rp = c->Rp;
+    a.G = c->Rp / kWave;
+    a.tile_sites = c->meas_sites;
     a.tiles = c->meas_tiles;
+    a.rec = c->meas_rec;
     a.n_classes = c->n_classes;
     a.jclass = c->d_jclass;
     a.nsp = c->nsp;
@@ -872,6 +892,8 @@ static int enqueue_measure(ether_ctx* c, int sample) {
     FinArgs f;
     f.partial = c->d_partial;
     f.tiles = c->meas_tiles;
+    f.rec = c->meas_rec;
+    f.Rp = c->Rp;
     f.R = c->R;
     f.N = c->N;
     f.model = c->model;
@@ -931,6 +953,7 @@ extern "C" int ether_init_spins(ether_ctx* c, int32_t mode) {
     a.s = c->d_s;
     a.N = c->N;
     a.R = c->R;
+    a.Rp = c->Rp;
     a.model = c->model;
     a.angle = c->angle;
     a.mode = mode;
@@ -1096,34 +1119,32 @@ extern "C" int ether_get_spins(ether_ctx* c, int32_t replica, double* S, double*
     TRY(need_committed(c));
     if (replica < 0 || replica >= c->R || !S) return fail(c, ETHER_ERR_ARG, "ether_get_spins: bad args");
     const int N = c->N;
-    const size_t off = (size_t)replica * N;
+    float* buf = c->d_scratch_h;   // 4N floats: x, y, z, phi planes of one replica
     if (c->model == ETHER_MODEL_ISING) {
-        std::vector<int8_t> s(N);
-        HIPCHK(c, hipMemcpyAsync(s.data(), c->d_s + off, N, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        for (int i = 0; i < N; ++i) {
-            S[3 * i] = 0.0;
-            S[3 * i + 1] = 0.0;
-            S[3 * i + 2] = (double)s[c->slot_of_site[i]];
-            if (phi) phi[i] = 0.0;
-        }
-        return ETHER_OK;
+        launch_extract(nullptr, c->d_s, N, c->Rp, replica, buf, c->stream);
+    } else {
+        launch_extract(c->d_sx, nullptr, N, c->Rp, replica, buf, c->stream);
+        launch_extract(c->d_sy, nullptr, N, c->Rp, replica, buf + N, c->stream);
+        launch_extract(c->d_sz, nullptr, N, c->Rp, replica, buf + 2 * (size_t)N, c->stream);
+        if (c->angle) launch_extract(c->d_phi, nullptr, N, c->Rp, replica, buf + 3 * (size_t)N, c->stream);
     }
-    std::vector<float> x(N), y(N), z(N), p;
-    HIPCHK(c, hipMemcpyAsync(x.data(), c->d_sx + off, 4 * N, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(y.data(), c->d_sy + off, 4 * N, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(z.data(), c->d_sz + off, 4 * N, hipMemcpyDeviceToHost, c->stream));
-    if (phi && c->angle) {
-        p.resize(N);
-        HIPCHK(c, hipMemcpyAsync(p.data(), c->d_phi + off, 4 * N, hipMemcpyDeviceToHost, c->stream));
-    }
+    HIPCHK(c, hipGetLastError());
+    std::vector<float> h(4 * (size_t)N, 0.f);
+    HIPCHK(c, hipMemcpyAsync(h.data(), buf, sizeof(float) * 4 * N, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int i = 0; i < N; ++i) {
-        int s = c->slot_of_site[i];
-        S[3 * i] = x[s];
-        S[3 * i + 1] = y[s];
-        S[3 * i + 2] = z[s];
-        if (phi) phi[i] = c->angle ? (double)p[s] : 0.0;
+        const int s = c->slot_of_site[i];
+        if (c->model == ETHER_MODEL_ISING) {
+            S[3 * i] = 0.0;
+            S[3 * i + 1] = 0.0;
+            S[3 * i + 2] = (double)h[s];
+            if (phi) phi[i] = 0.0;
+        } else {
+            S[3 * i] = h[s];
+            S[3 * i + 1] = h[N + s];
+            S[3 * i + 2] = h[2 * (size_t)N + s];
+            if (phi) phi[i] = c->angle ? (double)h[3 * (size_t)N + s] : 0.0;
+        }
     }
     return ETHER_OK;
 }
@@ -1134,31 +1155,25 @@ extern "C" int ether_set_spins(ether_ctx* c, int32_t replica, const double* S, c
         return fail(c, ETHER_ERR_ARG, "ether_set_spins: bad args (replica -1 sets all)");
     const int N = c->N;
     const int r0 = replica < 0 ? 0 : replica, r1 = replica < 0 ? c->R : replica + 1;
-    if (c->model == ETHER_MODEL_ISING) {
-        std::vector<int8_t> s(N);
-        for (int i = 0; i < N; ++i) s[c->slot_of_site[i]] = S[3 * i + 2] > 0.0 ? 1 : -1;
-        for (int r = r0; r < r1; ++r)
-            HIPCHK(c, hipMemcpyAsync(c->d_s + (size_t)r * N, s.data(), N, hipMemcpyHostToDevice,
-                                     c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        return ETHER_OK;
-    }
-    std::vector<float> x(N), y(N), z(N), p(N, 0.f);
+    std::vector<float> h(4 * (size_t)N, 0.f);
     for (int i = 0; i < N; ++i) {
-        int s = c->slot_of_site[i];
-        x[s] = (float)S[3 * i];
-        y[s] = (float)S[3 * i + 1];
-        z[s] = (float)S[3 * i + 2];
-        p[s] = phi ? (float)phi[i] : (float)std::atan2(S[3 * i + 1], S[3 * i]);
+        const int s = c->slot_of_site[i];
+        h[s] = (float)S[3 * i];
+        h[N + s] = (float)S[3 * i + 1];
+        h[2 * (size_t)N + s] = (float)S[3 * i + 2];
+        h[3 * (size_t)N + s] = phi ? (float)phi[i] : (float)std::atan2(S[3 * i + 1], S[3 * i]);
     }
-    for (int r = r0; r < r1; ++r) {
-        const size_t off = (size_t)r * N;
-        HIPCHK(c, hipMemcpyAsync(c->d_sx + off, x.data(), 4 * N, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->d_sy + off, y.data(), 4 * N, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->d_sz + off, z.data(), 4 * N, hipMemcpyHostToDevice, c->stream));
-        if (c->angle)
-            HIPCHK(c, hipMemcpyAsync(c->d_phi + off, p.data(), 4 * N, hipMemcpyHostToDevice, c->stream));
+    float* buf = c->d_scratch_h;
+    HIPCHK(c, hipMemcpyAsync(buf, h.data(), sizeof(float) * 4 * N, hipMemcpyHostToDevice, c->stream));
+    if (c->model == ETHER_MODEL_ISING) {
+        launch_insert(nullptr, c->d_s, N, c->Rp, r0, r1, buf + 2 * (size_t)N, c->stream);
+    } else {
+        launch_insert(c->d_sx, nullptr, N, c->Rp, r0, r1, buf, c->stream);
+        launch_insert(c->d_sy, nullptr, N, c->Rp, r0, r1, buf + N, c->stream);
+        launch_insert(c->d_sz, nullptr, N, c->Rp, r0, r1, buf + 2 * (size_t)N, c->stream);
+        if (c->angle) launch_insert(c->d_phi, nullptr, N, c->Rp, r0, r1, buf + 3 * (size_t)N, c->stream);
     }
+    HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return ETHER_OK;
 }
@@ -1192,6 +1207,7 @@ int ctx_fail(ether_ctx* c, int code, const std::string& msg) { return fail(c, co
 int ctx_model(ether_ctx* c) { return c->model; }
 int ctx_N(ether_ctx* c) { return c->N; }
 int ctx_R(ether_ctx* c) { return c->R; }
+int ctx_Rp(ether_ctx* c) { return c->Rp; }
 bool ctx_committed(ether_ctx* c) { return c->committed; }
 const float* ctx_spin_plane(ether_ctx* c, int comp) {
     return comp == 0 ? c->d_sx : comp == 1 ? c->d_sy : c->d_sz;

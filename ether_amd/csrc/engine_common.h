@@ -1,47 +1,52 @@
 // engine_common.h - device layout and kernel argument blocks of the sweep engine.
 //
 // Device layout (per context, all replicas resident in HBM):
-//   spins   XYZ: three FP32 planes sx, sy, sz of R*N floats (structure of
-//           arrays, replica-major, colour-sorted slots); ANGLE adds an FP32
-//           phi plane.  Ising: one int8 plane of R*N values +-1.
-//   ell     neighbour table in ELL form, ell[k*ell_stride + slot] for
-//           k < D: low 24 bits = neighbour slot, high 8 bits = coupling
-//           class (class 0 = padding, zero coupling).  Shared by replicas.
-//   classes XYZ: FP32 (Jxx,Jyy,Jzz) per class.  Ising: one J per class and
-//           a mixed-radix stride so that sum_j stride[class_j]*sigma_j
-//           indexes an exact acceptance-threshold table.
+//   spins   replica-major: element (slot, r) at slot*Rp + r, Rp = R rounded up
+//           to 64.  XYZ: three FP32 planes sx, sy, sz (structure of arrays);
+//           ANGLE adds an FP32 phi plane.  Ising: one int8 plane of +-1.
+//           A wavefront holds one site of 64 consecutive replicas, so every
+//           spin load and store of a wave is one contiguous 256-byte (FP32)
+//           or 64..256-byte (int8) segment.
+//   ell     neighbour table ell[slot*D + k], k < D: low 24 bits = neighbour
+//           slot, high 8 bits = coupling class (class 0 = padding, zero
+//           coupling).  Shared by all replicas; wave-uniform, so it is read
+//           with scalar loads.
+//   classes XYZ: FP32 (Jxx,Jyy,Jzz) per class.  Ising: one J per class and a
+//           mixed-radix stride so that sum_k stride_k * n_k (n_k = sum of the
+//           class-k neighbour spins) indexes an exact threshold table.
 //   slots   sites are renumbered so that every colour class is a contiguous
-//           slot range; a colour phase touches one range and reads only the
-//           other ranges, which is what makes the in-place update race free.
+//           slot range; a colour phase writes one range and reads only the
+//           others, which is what makes the in-place update race free.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace ether {
 
-constexpr int kBlock = 256;          // threads per workgroup for sweep/OVRR kernels
-constexpr int kMeasPerThread = 4;    // sites per thread in the measurement pass
-constexpr int kMeasTile = kBlock * kMeasPerThread;
+constexpr int kBlock = 256;          // threads per workgroup
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kSitesPerWave = 4;     // consecutive sites of a colour class per wave
 constexpr int kMaxClasses = 255;     // coupling classes (8-bit field of an ELL entry)
 constexpr int kMaxSpecies = 16;
 constexpr int kMaxIsingClasses = 8;
 constexpr int kIsingLdsTable = 4096; // thresholds staged in LDS up to this many entries
-constexpr int kRecord = 32;          // doubles per measurement partial record
 
 struct XyzArgs {
     float* sx;
     float* sy;
     float* sz;
     float* phi;
-    const uint32_t* ell;
-    int ell_stride;
+    const uint32_t* ell;   // [N][D]
     int D;
     const uint8_t* sp;
     int N;
     int R;
+    int Rp;
+    int G;                 // replica groups of 64 = Rp/64
     int c_begin;
     int c_end;
-    int tiles;
+    int nblocks;
     int xcd_map;
     const int* slot_of;
     const float* beta_slot;
@@ -59,27 +64,29 @@ struct XyzArgs {
 };
 
 struct IsingArgs {
-    int8_t* s;
-    const uint32_t* ell;
-    int ell_stride;
+    int8_t* s;             // [N][Rp]
+    const uint32_t* ell;   // [N][D]
     int D;
     const uint8_t* sp;
     int N;
     int R;
+    int Rp;
+    int G4;                // groups of 256 replicas (4 per lane)
     int c_begin;
     int c_end;
-    int tiles;
+    int nblocks;
     int xcd_map;
     const int* slot_of;
     uint64_t seed;
     uint32_t step;
     unsigned long long* acc;
     int n_classes;
-    const int* stride;       // [n_classes+1], stride[0] = 0
-    int offset;              // sum_k Z_k * stride_k
-    int table_per_slot;      // nsp * 2 * P
+    const int* stride;     // [n_classes+1], stride[0] = 0
+    int offset;            // sum_k Z_k * stride_k
+    int table_per_slot;    // nsp * 2 * P
     int P;
-    const uint32_t* thr;     // [R slots][nsp][2][P]
+    int table_in_lds;      // the whole [R][table] fits kIsingLdsTable
+    const uint32_t* thr;   // [R slots][nsp][2][P]
 };
 
 struct MeasArgs {
@@ -88,25 +95,30 @@ struct MeasArgs {
     const float* sz;
     const int8_t* s;
     const uint32_t* ell;
-    int ell_stride;
     int D;
     const uint8_t* sp;
     const float* w;
     int N;
     int R;
+    int Rp;
+    int G;
+    int tile_sites;        // sites per wave unit
     int tiles;
+    int rec;               // doubles per partial record
     int n_classes;
     const float* jclass;
     int nsp;
     const float* zvec;
     const float* dsia;
-    double* partial;         // [R][tiles][kRecord]
+    double* partial;       // [tiles][Rp][rec]
 };
 
 struct FinArgs {
     const double* partial;
     int tiles;
+    int rec;
     int R;
+    int Rp;
     int N;
     int model;
     int n_classes;
@@ -148,6 +160,7 @@ struct InitArgs {
     int8_t* s;
     int N;
     int R;
+    int Rp;
     int model;
     int angle;
     int mode;  // 1 fresh, 2 random
@@ -156,26 +169,22 @@ struct InitArgs {
     uint32_t step;
 };
 
-// blockIdx -> (replica, tile).  With xcd_map, consecutive workgroup ids
-// (which the dispatcher deals round-robin to the 8 XCDs) are arranged so that
-// every tile of replica r lands on XCD r%8 and shares that XCD's L2.
-__device__ __forceinline__ bool map_block(int R, int tiles, int xcd_map, int& r, int& tile) {
+// Workgroup -> wave unit.  With xcd_map the launch has a multiple of 8
+// workgroups; the dispatcher deals consecutive workgroup ids round-robin to
+// the 8 XCDs, so workgroup b runs on XCD b%8.  Remapping b -> (b%8)*per + b/8
+// gives every XCD one contiguous range of sites, whose spins and neighbour
+// halo then stay in that XCD's L2.
+__device__ __forceinline__ int block_index(int nblocks, int xcd_map) {
     int b = blockIdx.x;
-    if (xcd_map) {
-        int xcd = b & 7;
-        int j = b >> 3;
-        r = (j / tiles) * 8 + xcd;
-        tile = j % tiles;
-    } else {
-        r = b / tiles;
-        tile = b % tiles;
-    }
-    return r < R;
+    if (!xcd_map) return b;
+    int per = nblocks >> 3;
+    return (b & 7) * per + (b >> 3);
 }
 
-inline int blocks_for(int R, int tiles, int xcd_map) {
-    if (xcd_map) return 8 * ((R + 7) / 8) * tiles;
-    return R * tiles;
+inline int launch_blocks(long long units, int xcd_map) {
+    long long nb = (units + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (xcd_map) nb = (nb + 7) / 8 * 8;
+    return (int)nb;
 }
 
 // launchers (kernels.hip)
@@ -188,5 +197,9 @@ void launch_pt_swap(const PtArgs& a, hipStream_t st);
 void launch_init(const InitArgs& a, hipStream_t st);
 void launch_local_fields(const MeasArgs& a, int r, float* h, float* e, int zee, int sia,
                          int model, hipStream_t st);
+void launch_extract(const float* plane, const int8_t* s, int N, int Rp, int r, float* out,
+                    hipStream_t st);
+void launch_insert(float* plane, int8_t* s, int N, int Rp, int r0, int r1, const float* in,
+                   hipStream_t st);
 
 }  // namespace ether

@@ -1,30 +1,36 @@
 // kernels.hip - gfx950 kernels of the Monte Carlo sweep engine.
 //
-//   xyz_sweep     one colour phase of graph-coloured single-spin Metropolis for
-//                 XYZ spins (src/Metropolis.f90:35-62 with the proposals of
-//                 src/update_spin_details.f90:28-51 and the energy change of
-//                 src/Hamiltonian.f90:20-143)
-//   ising_sweep   the same for Ising spins, 4 sites per thread, with exact
-//                 integer acceptance thresholds
-//   xyz_ovrr      one colour phase of overrelaxation (src/overrelaxation.f90:30-49)
-//   measure       per-replica energy and magnetisation partial sums
-//                 (src/get_tot_energy.f90:35-45, src/get_tot_magnetisation.f90:35-38)
-//   finalize      fixed-order reduction of the partials plus the observable()
-//                 moment update (src/observable.f90:30-43, src/get_eng.f90:36-39,
-//                 src/get_mag.f90:36-40)
-//   pt_swap       replica exchange as a slot permutation (src/parallel_tempering.f90:50-108)
-//   init          fresh_spins / random start (src/fresh_spins.f90:31-38)
-//   local_fields  exchange field and site energy of one replica (for parity checks)
+//   k_xyz_sweep     one colour phase of graph-coloured single-spin Metropolis for
+//                   XYZ spins (src/Metropolis.f90:35-62 with the proposals of
+//                   src/update_spin_details.f90:28-51 and the energy change of
+//                   src/Hamiltonian.f90:20-143)
+//   k_ising_sweep   the same for Ising spins, four replicas per lane (packed
+//                   int8), exact integer acceptance thresholds
+//   k_xyz_ovrr      one colour phase of overrelaxation (src/overrelaxation.f90:30-49)
+//   k_*_measure     per-replica energy and magnetisation partial sums
+//                   (src/get_tot_energy.f90:35-45, src/get_tot_magnetisation.f90:35-38)
+//   k_finalize      fixed-order reduction of the partials plus the observable()
+//                   moment update (src/observable.f90:30-43, src/get_eng.f90:36-39,
+//                   src/get_mag.f90:36-40)
+//   k_pt_swap       replica exchange as a slot permutation (src/parallel_tempering.f90:50-108)
+//   k_init          fresh_spins / random start (src/fresh_spins.f90:31-38)
 //
-// All of these are HBM/L2-bound gathers: none is GEMM-shaped.  Each thread owns
-// one site (four for Ising) of one replica; a colour phase never writes a
-// site that another lane of the same phase reads.
+// Work mapping: a wavefront owns one site (slot) of 64 consecutive replicas
+// (256 replicas for Ising, four per lane) and walks a few consecutive sites of
+// its colour class.  The site, its neighbour list and its coupling classes are
+// wave-uniform (scalar loads); every spin access of the wave is one contiguous
+// segment of the replica-major planes.  None of this is GEMM-shaped; the
+// kernels are bound by HBM/L2 traffic and by the VALU work of the generator.
 #include <hip/hip_runtime.h>
 
 #include "engine_common.h"
 #include "philox.h"
 
 namespace ether {
+
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+__device__ __forceinline__ uint64_t lanes_below(int lane) { return (1ull << lane) - 1ull; }
 
 // ---------------------------------------------------------------------------
 // random-number helpers (24-bit uniforms, identical to the oracle's)
@@ -39,39 +45,87 @@ __device__ __forceinline__ float wrap_pif(float p) {
     const float two_pi = 6.28318530717958647692f;
     return p - two_pi * floorf((p + 3.14159265358979323846f) / two_pi);
 }
-
 __device__ __forceinline__ void normalise3(float& x, float& y, float& z) {
     float inv = 1.0f / sqrtf(x * x + y * y + z * z);
     x *= inv;
     y *= inv;
     z *= inv;
 }
+__device__ __forceinline__ bool in_disc(uint32_t cx, uint32_t cy) {
+    int32_t a1 = (int32_t)cx - (1 << 23), a2 = (int32_t)cy - (1 << 23);
+    return (int64_t)a1 * a1 + (int64_t)a2 * a2 < (int64_t(1) << 46);
+}
+// Marsaglia's point from an accepted pair (src/George_Marsaglia.f90:36-38), renormalised
+__device__ __forceinline__ void marsaglia_point(uint32_t cx, uint32_t cy, float& x, float& y, float& z) {
+    int32_t a1 = (int32_t)cx - (1 << 23), a2 = (int32_t)cy - (1 << 23);
+    int64_t q = (int64_t)a1 * a1 + (int64_t)a2 * a2;
+    float v1 = (float)a1 * 0x1p-23f, v2 = (float)a2 * 0x1p-23f;
+    float s = (float)q * 0x1p-46f;
+    float rt = sqrtf(1.0f - s);
+    x = 2.0f * v1 * rt;
+    y = 2.0f * v2 * rt;
+    z = 1.0f - 2.0f * s;
+    normalise3(x, y, z);
+}
+// two candidate pairs per Philox block
+__device__ __forceinline__ void candidates(u32x4 w, uint32_t& x0, uint32_t& y0, uint32_t& x1,
+                                           uint32_t& y1) {
+    x0 = w.x >> 8;
+    y0 = w.y >> 8;
+    x1 = ((w.w >> 16) << 8) | (w.x & 0xffu);
+    y1 = ((w.w & 0xffffu) << 8) | (w.y & 0xffu);
+}
 
-// George_Marsaglia (src/George_Marsaglia.f90:29-39): two candidate pairs per
-// Philox block, exact integer test a1^2 + a2^2 < 2^46 (i.e. v1^2+v2^2 < 1).
-__device__ __forceinline__ void marsaglia(u32x4 w, uint64_t seed, uint32_t id, uint32_t r,
-                                          uint32_t step, uint32_t purpose, float& x, float& y,
-                                          float& z) {
-    for (uint32_t att = 0;; ++att) {
-        if (att) w = rng_draw(seed, id, r, step, purpose, att);
-        uint32_t cx[2] = {w.x >> 8, ((w.w >> 16) << 8) | (w.x & 0xffu)};
-        uint32_t cy[2] = {w.y >> 8, ((w.w & 0xffffu) << 8) | (w.y & 0xffu)};
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            int32_t a1 = (int32_t)cx[c] - (1 << 23), a2 = (int32_t)cy[c] - (1 << 23);
-            int64_t q = (int64_t)a1 * a1 + (int64_t)a2 * a2;
-            if (q < (int64_t(1) << 46)) {
-                float v1 = (float)a1 * 0x1p-23f, v2 = (float)a2 * 0x1p-23f;
-                float s = (float)q * 0x1p-46f;
-                float rt = sqrtf(1.0f - s);
-                x = 2.0f * v1 * rt;
-                y = 2.0f * v2 * rt;
-                z = 1.0f - 2.0f * s;
-                normalise3(x, y, z);
-                return;
-            }
+// Per-lane Marsaglia with retries (attempt >= first): used by the start-state
+// kernel and as the fallback of the pooled proposal.
+__device__ void marsaglia_retry(uint64_t seed, uint32_t id, uint32_t r, uint32_t step,
+                                uint32_t purpose, uint32_t first, float& x, float& y, float& z) {
+    for (uint32_t att = first;; ++att) {
+        u32x4 w = rng_draw(seed, id, r, step, purpose, att);
+        uint32_t x0, y0, x1, y1;
+        candidates(w, x0, y0, x1, y1);
+        if (in_disc(x0, y0)) { marsaglia_point(x0, y0, x, y, z); return; }
+        if (in_disc(x1, y1)) { marsaglia_point(x1, y1, x, y, z); return; }
+    }
+}
+
+// Pooled Marsaglia proposal of one wavefront (64 replicas of one site).
+// Every lane draws two candidate pairs from its own Philox block.  A lane
+// whose first pair falls outside the disc takes, in lane order, the second
+// pair of the lanes whose second pair is inside ("donors", in lane order).
+// The assignment depends only on the inside/outside bits, so each lane still
+// gets an independent uniform point of the disc; only if donors run out does
+// a lane draw more blocks itself (attempts 1, 2, ...).  Saves the second
+// Philox block that almost every wave would otherwise pay for one lane.
+__device__ __forceinline__ void marsaglia_pooled(u32x4 w, bool active, uint64_t seed, uint32_t id,
+                                                 uint32_t r, uint32_t step, uint64_t* pool,
+                                                 float& x, float& y, float& z) {
+    uint32_t x0, y0, x1, y1;
+    candidates(w, x0, y0, x1, y1);
+    const bool ok0 = in_disc(x0, y0), ok1 = in_disc(x1, y1);
+    const uint64_t needy = __ballot(active && !ok0);
+    const uint64_t donors = __ballot(active && ok1);
+    const int lane = lane_id();
+    const uint64_t below = lanes_below(lane);
+    if (active && ok1) pool[__popcll(donors & below)] = ((uint64_t)x1 << 32) | y1;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    uint32_t ux = x0, uy = y0;
+    bool have = ok0;
+    if (active && !ok0) {
+        const int j = __popcll(needy & below);
+        if (j < __popcll(donors)) {
+            const uint64_t v = pool[j];
+            ux = (uint32_t)(v >> 32);
+            uy = (uint32_t)v;
+            have = true;
         }
     }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    if (!active) return;
+    if (have) marsaglia_point(ux, uy, x, y, z);
+    else marsaglia_retry(seed, id, r, step, RNG_METROPOLIS, 1, x, y, z);
 }
 
 // ANGLE move (src/update_spin_details.f90:39-46)
@@ -88,6 +142,23 @@ __device__ __forceinline__ void angle_move(u32x4 w, float dphi, float phi_old, f
     normalise3(x, y, z);
 }
 
+// per-lane counters of the four waves of a block, summed into the slot counters
+__device__ __forceinline__ void book_acceptance(unsigned* sAcc, int nrep_lane, const unsigned* mine,
+                                                int r_first, int R, const int* slot_of,
+                                                unsigned long long* acc) {
+    const int w = wave_id(), lane = lane_id();
+    for (int q = 0; q < nrep_lane; ++q) sAcc[(w * kWave + lane) * nrep_lane + q] = mine[q];
+    __syncthreads();
+    if (w == 0) {
+        for (int q = 0; q < nrep_lane; ++q) {
+            unsigned t = 0;
+            for (int v = 0; v < kWavesPerBlock; ++v) t += sAcc[(v * kWave + lane) * nrep_lane + q];
+            const int r = r_first + lane * nrep_lane + q;
+            if (t && r < R) atomicAdd(acc + slot_of[r], (unsigned long long)t);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // XYZ Metropolis, one colour phase
 // ---------------------------------------------------------------------------
@@ -96,6 +167,8 @@ __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
     __shared__ float sJ[3 * (kMaxClasses + 1)];
     __shared__ float sZ[3 * kMaxSpecies];
     __shared__ float sD[3 * kMaxSpecies];
+    __shared__ uint64_t sPool[kWavesPerBlock][kWave];
+    __shared__ unsigned sAcc[kBlock];
     for (int t = threadIdx.x; t < 3 * (a.n_classes + 1); t += kBlock) sJ[t] = a.jclass[t];
     if (ZEE)
         for (int t = threadIdx.x; t < 3 * a.nsp; t += kBlock) sZ[t] = a.zvec[t];
@@ -103,40 +176,43 @@ __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
         for (int t = threadIdx.x; t < 3 * a.nsp; t += kBlock) sD[t] = a.dsia[t];
     __syncthreads();
 
-    int r, tile;
-    if (!map_block(a.R, a.tiles, a.xcd_map, r, tile)) return;
-    const int slot = a.c_begin + tile * kBlock + threadIdx.x;
-    const bool active = slot < a.c_end;
-    int accepted = 0;
-    if (active) {
-        const size_t base = (size_t)r * a.N;
-        const float* __restrict__ px = a.sx + base;
-        const float* __restrict__ py = a.sy + base;
-        const float* __restrict__ pz = a.sz + base;
-        float x = px[slot], y = py[slot], z = pz[slot];
+    const int bb = block_index(a.nblocks, a.xcd_map);
+    const int g = bb % a.G, chunk = bb / a.G;
+    const int w = wave_id(), lane = lane_id();
+    const int r = g * kWave + lane;
+    const bool rv = r < a.R;
+    const float beta = rv ? a.beta_slot[a.slot_of[r]] : 0.f;
+    uint64_t* pool = sPool[w];
+    unsigned accepted = 0;
+    const int s0 = a.c_begin + (chunk * kWavesPerBlock + w) * kSitesPerWave;
+    for (int q = 0; q < kSitesPerWave; ++q) {
+        const int slot = s0 + q;                       // wave-uniform
+        if (slot >= a.c_end) break;
+        const size_t row = (size_t)slot * a.Rp + r;
+        const float x = a.sx[row], y = a.sy[row], z = a.sz[row];
         float hx = 0.f, hy = 0.f, hz = 0.f;
+        const uint32_t* nb = a.ell + (size_t)slot * a.D;
         for (int k = 0; k < a.D; ++k) {
-            uint32_t e = a.ell[(size_t)k * a.ell_stride + slot];
-            uint32_t j = e & 0xffffffu;
-            uint32_t c = e >> 24;
-            hx = fmaf(sJ[3 * c + 0], px[j], hx);
-            hy = fmaf(sJ[3 * c + 1], py[j], hy);
-            hz = fmaf(sJ[3 * c + 2], pz[j], hz);
+            const uint32_t e = nb[k];                  // scalar load
+            const size_t jr = (size_t)(e & 0xffffffu) * a.Rp + r;
+            const uint32_t c = e >> 24;
+            hx = fmaf(sJ[3 * c + 0], a.sx[jr], hx);
+            hy = fmaf(sJ[3 * c + 1], a.sy[jr], hy);
+            hz = fmaf(sJ[3 * c + 2], a.sz[jr], hz);
         }
-        int sp = (ZEE || SIA) ? a.sp[slot] : 0;
+        const int sp = (ZEE || SIA) ? a.sp[slot] : 0;
         if (ZEE) {
             hx += sZ[3 * sp + 0];
             hy += sZ[3 * sp + 1];
             hz += sZ[3 * sp + 2];
         }
-        const float beta = a.beta_slot[a.slot_of[r]];
-        u32x4 w = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, a.step, RNG_METROPOLIS, 0);
-        const float eta = eta24f(w.z);
-        float nx, ny, nz, nphi = 0.f;
+        const u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, a.step, RNG_METROPOLIS, 0);
+        const float eta = eta24f(wr.z);
+        float nx = 0.f, ny = 0.f, nz = 1.f, nphi = 0.f;
         if (ANGLE) {
-            angle_move(w, a.dphi, a.phi[base + slot], nx, ny, nz, nphi);
+            angle_move(wr, a.dphi, a.phi[row], nx, ny, nz, nphi);
         } else {
-            marsaglia(w, a.seed, (uint32_t)slot, (uint32_t)r, a.step, RNG_METROPOLIS, nx, ny, nz);
+            marsaglia_pooled(wr, rv, a.seed, (uint32_t)slot, (uint32_t)r, a.step, pool, nx, ny, nz);
         }
         float dE = (nx - x) * hx + (ny - y) * hy + (nz - z) * hz;
         if (SIA) {
@@ -144,17 +220,16 @@ __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
                   sD[3 * sp + 2] * (nz * nz - z * z);
         }
         // accept if -beta*dE > log(eta)   (src/Metropolis.f90:52)
-        if (-beta * dE > __logf(eta)) {
-            a.sx[base + slot] = nx;
-            a.sy[base + slot] = ny;
-            a.sz[base + slot] = nz;
-            if (ANGLE) a.phi[base + slot] = nphi;
-            accepted = 1;
+        if (rv && -beta * dE > __logf(eta)) {
+            a.sx[row] = nx;
+            a.sy[row] = ny;
+            a.sz[row] = nz;
+            if (ANGLE) a.phi[row] = nphi;
+            ++accepted;
         }
     }
     // acceptance is booked to the temperature slot, as the reference books it per rank
-    int cnt = __syncthreads_count(accepted);
-    if (threadIdx.x == 0 && cnt) atomicAdd(a.acc + a.slot_of[r], (unsigned long long)cnt);
+    book_acceptance(sAcc, 1, &accepted, g * kWave, a.R, a.slot_of, a.acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -168,133 +243,141 @@ __global__ __launch_bounds__(kBlock) void k_xyz_ovrr(XyzArgs a) {
     if (ZEE)
         for (int t = threadIdx.x; t < 3 * a.nsp; t += kBlock) sZ[t] = a.zvec[t];
     __syncthreads();
-    int r, tile;
-    if (!map_block(a.R, a.tiles, a.xcd_map, r, tile)) return;
-    const int slot = a.c_begin + tile * kBlock + threadIdx.x;
-    if (slot >= a.c_end) return;
-    if (a.frac < 1.0f) {
-        u32x4 w = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, a.step, RNG_OVRR, a.pass);
-        if (!((float)(w.x >> 8) * 0x1p-24f < a.frac)) return;
+    const int bb = block_index(a.nblocks, a.xcd_map);
+    const int g = bb % a.G, chunk = bb / a.G;
+    const int w = wave_id(), lane = lane_id();
+    const int r = g * kWave + lane;
+    if (r >= a.R) return;
+    const int s0 = a.c_begin + (chunk * kWavesPerBlock + w) * kSitesPerWave;
+    for (int q = 0; q < kSitesPerWave; ++q) {
+        const int slot = s0 + q;
+        if (slot >= a.c_end) break;
+        if (a.frac < 1.0f) {
+            u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, a.step, RNG_OVRR, a.pass);
+            if (!((float)(wr.x >> 8) * 0x1p-24f < a.frac)) continue;
+        }
+        const size_t row = (size_t)slot * a.Rp + r;
+        float hx = 0.f, hy = 0.f, hz = 0.f;
+        const uint32_t* nb = a.ell + (size_t)slot * a.D;
+        for (int k = 0; k < a.D; ++k) {
+            const uint32_t e = nb[k];
+            const size_t jr = (size_t)(e & 0xffffffu) * a.Rp + r;
+            const uint32_t c = e >> 24;
+            hx = fmaf(sJ[3 * c + 0], a.sx[jr], hx);
+            hy = fmaf(sJ[3 * c + 1], a.sy[jr], hy);
+            hz = fmaf(sJ[3 * c + 2], a.sz[jr], hz);
+        }
+        if (ZEE) {
+            const int sp = a.sp[slot];
+            hx += sZ[3 * sp + 0];
+            hy += sZ[3 * sp + 1];
+            hz += sZ[3 * sp + 2];
+        }
+        const float aa = hx * hx + hy * hy + hz * hz;
+        if (!(aa > 0.f)) continue;
+        const float x = a.sx[row], y = a.sy[row], z = a.sz[row];
+        const float f = 2.0f * (hx * x + hy * y + hz * z) / aa;
+        float nx = f * hx - x, ny = f * hy - y, nz = f * hz - z;
+        normalise3(nx, ny, nz);
+        a.sx[row] = nx;
+        a.sy[row] = ny;
+        a.sz[row] = nz;
+        if (ANGLE) a.phi[row] = atan2f(ny, nx);
     }
-    const size_t base = (size_t)r * a.N;
-    const float* __restrict__ px = a.sx + base;
-    const float* __restrict__ py = a.sy + base;
-    const float* __restrict__ pz = a.sz + base;
-    float hx = 0.f, hy = 0.f, hz = 0.f;
-    for (int k = 0; k < a.D; ++k) {
-        uint32_t e = a.ell[(size_t)k * a.ell_stride + slot];
-        uint32_t j = e & 0xffffffu;
-        uint32_t c = e >> 24;
-        hx = fmaf(sJ[3 * c + 0], px[j], hx);
-        hy = fmaf(sJ[3 * c + 1], py[j], hy);
-        hz = fmaf(sJ[3 * c + 2], pz[j], hz);
-    }
-    if (ZEE) {
-        int sp = a.sp[slot];
-        hx += sZ[3 * sp + 0];
-        hy += sZ[3 * sp + 1];
-        hz += sZ[3 * sp + 2];
-    }
-    float aa = hx * hx + hy * hy + hz * hz;
-    if (!(aa > 0.f)) return;
-    float x = px[slot], y = py[slot], z = pz[slot];
-    float f = 2.0f * (hx * x + hy * y + hz * z) / aa;
-    float nx = f * hx - x, ny = f * hy - y, nz = f * hz - z;
-    normalise3(nx, ny, nz);
-    a.sx[base + slot] = nx;
-    a.sy[base + slot] = ny;
-    a.sz[base + slot] = nz;
-    if (ANGLE) a.phi[base + slot] = atan2f(ny, nx);
 }
 
 // ---------------------------------------------------------------------------
-// Ising Metropolis, one colour phase.  Thread t owns the aligned slot group
-// [4g, 4g+4) and takes word (slot & 3) of the Philox block for id = g.
-// dE depends only on (species, sigma_i, sum_j stride[class_j] * sigma_j), so
-// the acceptance test is an exact integer compare with a host-built threshold.
+// Ising Metropolis, one colour phase.  Lane l of group g owns replicas
+// r0..r0+3 (r0 = 4(64g + l)) of the wave's site as one 32-bit word of int8
+// spins and takes word r&3 of the Philox block with id (slot, r>>2).
+// Down-spin counts of each coupling class are summed per byte (SWAR); dE then
+// depends only on (species, sigma_i, sum_k stride_k n_k), and the acceptance
+// test is an exact integer compare with a host-built threshold.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void class_add(uint32_t (&d)[kMaxIsingClasses + 1], int c, uint32_t v) {
+#pragma unroll
+    for (int k = 1; k <= kMaxIsingClasses; ++k)
+        if (k == c) d[k] += v;
+}
+
 __global__ __launch_bounds__(kBlock) void k_ising_sweep(IsingArgs a) {
     __shared__ int sStride[kMaxIsingClasses + 1];
     __shared__ uint32_t sThr[kIsingLdsTable];
-    int r, tile;
-    const bool live = map_block(a.R, a.tiles, a.xcd_map, r, tile);
-    const bool lds_table = a.table_per_slot <= kIsingLdsTable;
-    const uint32_t* thr = nullptr;
-    if (live) thr = a.thr + (size_t)a.slot_of[r] * a.table_per_slot;
+    __shared__ unsigned sAcc[kBlock * 4];
     for (int t = threadIdx.x; t <= a.n_classes; t += kBlock) sStride[t] = a.stride[t];
-    if (live && lds_table)
-        for (int t = threadIdx.x; t < a.table_per_slot; t += kBlock) sThr[t] = thr[t];
+    if (a.table_in_lds)
+        for (int t = threadIdx.x; t < a.R * a.table_per_slot; t += kBlock) sThr[t] = a.thr[t];
     __syncthreads();
-    if (!live) return;
-
-    const int g = (a.c_begin >> 2) + tile * kBlock + threadIdx.x;
-    const int s0 = 4 * g;
-    int accepted = 0;
-    if (s0 < a.c_end) {
-        const size_t base = (size_t)r * a.N;
-        int8_t* __restrict__ sp8 = a.s + base;
-        u32x4 w = rng_draw(a.seed, (uint32_t)g, (uint32_t)r, a.step, RNG_METROPOLIS, 0);
-        const uint32_t words[4] = {w.x, w.y, w.z, w.w};
+    const int bb = block_index(a.nblocks, a.xcd_map);
+    const int g4 = bb % a.G4, chunk = bb / a.G4;
+    const int w = wave_id(), lane = lane_id();
+    const int rid = g4 * kWave + lane;          // Philox id of this lane's 4 replicas
+    const int r0 = 4 * rid;
+    const bool lv = r0 < a.Rp;
+    int tbase[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int slot = s0 + q;
-            if (slot < a.c_begin || slot >= a.c_end) continue;
+    for (int q = 0; q < 4; ++q) {
+        const int r = r0 + q;
+        tbase[q] = (r < a.R) ? a.slot_of[r] * a.table_per_slot : -1;
+    }
+    unsigned accepted[4] = {0, 0, 0, 0};
+    const int s0 = a.c_begin + (chunk * kWavesPerBlock + w) * kSitesPerWave;
+    for (int q = 0; q < kSitesPerWave; ++q) {
+        const int slot = s0 + q;
+        if (slot >= a.c_end) break;
+        if (!lv) continue;
+        uint32_t d[kMaxIsingClasses + 1];
+        int cnt[kMaxIsingClasses + 1];
+#pragma unroll
+        for (int k = 0; k <= kMaxIsingClasses; ++k) {
+            d[k] = 0;
+            cnt[k] = 0;
+        }
+        const uint32_t* nb = a.ell + (size_t)slot * a.D;
+        for (int k = 0; k < a.D; ++k) {
+            const uint32_t e = nb[k];
+            const int c = (int)(e >> 24);
+            if (c == 0) continue;
+            const uint32_t v = *reinterpret_cast<const uint32_t*>(a.s + (size_t)(e & 0xffffffu) * a.Rp + r0);
+            class_add(d, c, (v >> 1) & 0x01010101u);   // 1 per byte where sigma = -1
+#pragma unroll
+            for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
+                if (kk == c) cnt[kk] += 1;
+        }
+        uint32_t* own = reinterpret_cast<uint32_t*>(a.s + (size_t)slot * a.Rp + r0);
+        const uint32_t v = *own;
+        const u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)rid, a.step, RNG_METROPOLIS, 0);
+        const uint32_t words[4] = {wr.x, wr.y, wr.z, wr.w};
+        const int spc = a.sp[slot];
+        uint32_t flip = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            if (tbase[b] < 0) continue;
             int idx = a.offset;
-            for (int k = 0; k < a.D; ++k) {
-                uint32_t e = a.ell[(size_t)k * a.ell_stride + slot];
-                idx += sStride[e >> 24] * (int)sp8[e & 0xffffffu];
+#pragma unroll
+            for (int k = 1; k <= kMaxIsingClasses; ++k) {
+                if (k > a.n_classes) break;
+                const int down = (int)((d[k] >> (8 * b)) & 0xffu);
+                idx += sStride[k] * (cnt[k] - 2 * down);
             }
-            const int sigma = sp8[slot];
-            const int spc = a.sp[slot];
-            const int t = (spc * 2 + (sigma > 0)) * a.P + idx;
-            const uint32_t th = lds_table ? sThr[t] : thr[t];
-            if (words[q] < th || th == 0xffffffffu) {
-                sp8[slot] = (int8_t)(-sigma);
-                ++accepted;
+            const int sigma = (int)(int8_t)((v >> (8 * b)) & 0xffu);
+            const int t = tbase[b] + (spc * 2 + (sigma > 0)) * a.P + idx;
+            const uint32_t th = a.table_in_lds ? sThr[t] : a.thr[t];
+            if (words[b] < th || th == 0xffffffffu) {
+                flip |= 0xFEu << (8 * b);            // +1 <-> -1 as int8: x ^ 0xFE
+                ++accepted[b];
             }
         }
+        if (flip) *own = v ^ flip;
     }
-    // block sum of accepted moves
-    __shared__ int sAcc;
-    if (threadIdx.x == 0) sAcc = 0;
-    __syncthreads();
-    if (accepted) atomicAdd(&sAcc, accepted);
-    __syncthreads();
-    if (threadIdx.x == 0 && sAcc) atomicAdd(a.acc + a.slot_of[r], (unsigned long long)sAcc);
+    book_acceptance(sAcc, 4, accepted, g4 * 4 * kWave, a.R, a.slot_of, a.acc);
 }
 
 // ---------------------------------------------------------------------------
-// measurement partials
+// measurement partials: wave unit = (tile of sites, replica group of 64)
+// record per (tile, replica): XYZ {E_exch, E_z, E_sia, Mx, My, Mz};
+// Ising {bonds_1..K, zsum_1..nsp, Mz} (integers held exactly in FP64)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-__device__ __forceinline__ long long wave_sum_ll(long long v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// Block-wide fixed-order sum of n values per thread into out[0..n).
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double* out) {
-    __shared__ double red[kBlock / 64][NV];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        double s = wave_sum(v[k]);
-        if (lane == 0) red[wv][k] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < NV) {
-        double s = 0.0;
-        for (int q = 0; q < kBlock / 64; ++q) s += red[q][threadIdx.x];
-        out[threadIdx.x] = s;
-    }
-}
-
 template <bool ZEE, bool SIA>
 __global__ __launch_bounds__(kBlock) void k_xyz_measure(MeasArgs a) {
     __shared__ float sJ[3 * (kMaxClasses + 1)];
@@ -306,102 +389,86 @@ __global__ __launch_bounds__(kBlock) void k_xyz_measure(MeasArgs a) {
     if (SIA)
         for (int t = threadIdx.x; t < 3 * a.nsp; t += kBlock) sD[t] = a.dsia[t];
     __syncthreads();
-    const int r = blockIdx.x / a.tiles, tile = blockIdx.x % a.tiles;
-    const size_t base = (size_t)r * a.N;
-    const float* __restrict__ px = a.sx + base;
-    const float* __restrict__ py = a.sy + base;
-    const float* __restrict__ pz = a.sz + base;
+    const int unit = blockIdx.x * kWavesPerBlock + wave_id();
+    if (unit >= a.tiles * a.G) return;
+    const int g = unit % a.G, tile = unit / a.G;
+    const int r = g * kWave + lane_id();
     double v[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int q = 0; q < kMeasPerThread; ++q) {
-        const int slot = tile * kMeasTile + q * kBlock + threadIdx.x;
-        if (slot >= a.N) continue;
+    const int s_begin = tile * a.tile_sites;
+    const int s_end = min(a.N, s_begin + a.tile_sites);
+    for (int slot = s_begin; slot < s_end; ++slot) {
+        const size_t row = (size_t)slot * a.Rp + r;
         float hx = 0.f, hy = 0.f, hz = 0.f;
+        const uint32_t* nb = a.ell + (size_t)slot * a.D;
         for (int k = 0; k < a.D; ++k) {
-            uint32_t e = a.ell[(size_t)k * a.ell_stride + slot];
-            uint32_t j = e & 0xffffffu;
-            uint32_t c = e >> 24;
-            hx = fmaf(sJ[3 * c + 0], px[j], hx);
-            hy = fmaf(sJ[3 * c + 1], py[j], hy);
-            hz = fmaf(sJ[3 * c + 2], pz[j], hz);
+            const uint32_t e = nb[k];
+            const size_t jr = (size_t)(e & 0xffffffu) * a.Rp + r;
+            const uint32_t c = e >> 24;
+            hx = fmaf(sJ[3 * c + 0], a.sx[jr], hx);
+            hy = fmaf(sJ[3 * c + 1], a.sy[jr], hy);
+            hz = fmaf(sJ[3 * c + 2], a.sz[jr], hz);
         }
-        float x = px[slot], y = py[slot], z = pz[slot];
+        const float x = a.sx[row], y = a.sy[row], z = a.sz[row];
         v[0] += (double)(x * hx + y * hy + z * hz);
-        int sp = a.sp[slot];
+        const int sp = a.sp[slot];
         if (ZEE) v[1] += (double)(x * sZ[3 * sp] + y * sZ[3 * sp + 1] + z * sZ[3 * sp + 2]);
-        if (SIA)
-            v[2] += (double)(sD[3 * sp] * x * x + sD[3 * sp + 1] * y * y + sD[3 * sp + 2] * z * z);
-        float wgt = a.w[slot];
+        if (SIA) v[2] += (double)(sD[3 * sp] * x * x + sD[3 * sp + 1] * y * y + sD[3 * sp + 2] * z * z);
+        const float wgt = a.w[slot];
         v[3] += (double)(wgt * x);
         v[4] += (double)(wgt * y);
         v[5] += (double)(wgt * z);
     }
-    block_sum<6>(v, a.partial + ((size_t)r * a.tiles + tile) * kRecord);
+    double* out = a.partial + ((size_t)tile * a.Rp + r) * a.rec;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) out[k] = v[k];
 }
 
 __global__ __launch_bounds__(kBlock) void k_ising_measure(MeasArgs a) {
-    __shared__ long long red[kBlock / 64][2 * kMaxIsingClasses + 2];
-    const int r = blockIdx.x / a.tiles, tile = blockIdx.x % a.tiles;
-    const size_t base = (size_t)r * a.N;
-    const int8_t* __restrict__ s = a.s + base;
-    const int K = a.n_classes;
-    long long bonds[kMaxIsingClasses + 1];
-    long long zs[kMaxIsingClasses];
+    const int unit = blockIdx.x * kWavesPerBlock + wave_id();
+    if (unit >= a.tiles * a.G) return;
+    const int g = unit % a.G, tile = unit / a.G;
+    const int r = g * kWave + lane_id();
+    double bonds[kMaxIsingClasses + 1];
+    double zs[kMaxIsingClasses];
 #pragma unroll
-    for (int k = 0; k <= kMaxIsingClasses; ++k) bonds[k] = 0;
+    for (int k = 0; k <= kMaxIsingClasses; ++k) bonds[k] = 0.0;
 #pragma unroll
-    for (int k = 0; k < kMaxIsingClasses; ++k) zs[k] = 0;
+    for (int k = 0; k < kMaxIsingClasses; ++k) zs[k] = 0.0;
     double mz = 0.0;
-#pragma unroll
-    for (int q = 0; q < kMeasPerThread; ++q) {
-        const int slot = tile * kMeasTile + q * kBlock + threadIdx.x;
-        if (slot >= a.N) continue;
-        const int si = s[slot];
+    const int s_begin = tile * a.tile_sites;
+    const int s_end = min(a.N, s_begin + a.tile_sites);
+    for (int slot = s_begin; slot < s_end; ++slot) {
+        const int si = a.s[(size_t)slot * a.Rp + r];
+        const uint32_t* nb = a.ell + (size_t)slot * a.D;
         for (int k = 0; k < a.D; ++k) {
-            uint32_t e = a.ell[(size_t)k * a.ell_stride + slot];
-            int c = (int)(e >> 24);
-            int sj = s[e & 0xffffffu];
+            const uint32_t e = nb[k];
+            const int c = (int)(e >> 24);
+            const int sj = a.s[(size_t)(e & 0xffffffu) * a.Rp + r];
 #pragma unroll
             for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
-                if (kk == c) bonds[kk] += si * sj;
+                if (kk == c) bonds[kk] += (double)(si * sj);
         }
         const int sp = a.sp[slot];
 #pragma unroll
         for (int kk = 0; kk < kMaxIsingClasses; ++kk)
-            if (kk == sp) zs[kk] += si;
+            if (kk == sp) zs[kk] += (double)si;
         mz += (double)(a.w[slot] * (float)si);
     }
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int k = 1; k <= K; ++k) {
-        long long t = wave_sum_ll(bonds[k]);
-        if (lane == 0) red[wv][k] = t;
-    }
-    for (int k = 0; k < a.nsp; ++k) {
-        long long t = wave_sum_ll(zs[k]);
-        if (lane == 0) red[wv][kMaxIsingClasses + 1 + k] = t;
-    }
-    double m = wave_sum(mz);
-    __shared__ double redm[kBlock / 64];
-    if (lane == 0) redm[wv] = m;
-    __syncthreads();
-    double* out = a.partial + ((size_t)r * a.tiles + tile) * kRecord;
-    long long* outl = reinterpret_cast<long long*>(out);
-    if (threadIdx.x >= 1 && threadIdx.x <= K) {
-        long long t = 0;
-        for (int q = 0; q < kBlock / 64; ++q) t += red[q][threadIdx.x];
-        outl[threadIdx.x] = t;
-    }
-    if (threadIdx.x >= 16 && threadIdx.x < 16 + a.nsp) {
-        int k = threadIdx.x - 16;
-        long long t = 0;
-        for (int q = 0; q < kBlock / 64; ++q) t += red[q][kMaxIsingClasses + 1 + k];
-        outl[16 + k] = t;
-    }
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int q = 0; q < kBlock / 64; ++q) t += redm[q];
-        out[31] = t;
-    }
+    double* out = a.partial + ((size_t)tile * a.Rp + r) * a.rec;
+    int o = 0;
+#pragma unroll
+    for (int k = 1; k <= kMaxIsingClasses; ++k)
+        if (k <= a.n_classes) out[o++] = bonds[k];
+#pragma unroll
+    for (int k = 0; k < kMaxIsingClasses; ++k)
+        if (k < a.nsp) out[o++] = zs[k];
+    out[o] = mz;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
 // One wave per replica: fixed-order reduction of the tile partials, energy
@@ -409,39 +476,29 @@ __global__ __launch_bounds__(kBlock) void k_ising_measure(MeasArgs a) {
 __global__ __launch_bounds__(64) void k_finalize(FinArgs a) {
     const int r = blockIdx.x;
     const int lane = threadIdx.x;
-    const double* p = a.partial + (size_t)r * a.tiles * kRecord;
-    double ex, ez, es, mx, my, mz;
+    double ex = 0.0, ez = 0.0, es = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
     if (a.model == 1) {
         double v[6] = {0, 0, 0, 0, 0, 0};
-        for (int t = lane; t < a.tiles; t += 64)
-            for (int k = 0; k < 6; ++k) v[k] += p[(size_t)t * kRecord + k];
+        for (int t = lane; t < a.tiles; t += 64) {
+            const double* p = a.partial + ((size_t)t * a.Rp + r) * a.rec;
+            for (int k = 0; k < 6; ++k) v[k] += p[k];
+        }
         for (int k = 0; k < 6; ++k) v[k] = wave_sum(v[k]);
         ex = v[0]; ez = v[1]; es = v[2]; mx = v[3]; my = v[4]; mz = v[5];
     } else {
-        const long long* pl = reinterpret_cast<const long long*>(p);
-        long long b[kMaxIsingClasses + 1], z[kMaxIsingClasses];
-        for (int k = 0; k <= kMaxIsingClasses; ++k) b[k] = 0;
-        for (int k = 0; k < kMaxIsingClasses; ++k) z[k] = 0;
-        double m = 0.0;
-        for (int t = lane; t < a.tiles; t += 64) {
-            for (int k = 1; k <= a.n_classes; ++k) b[k] += pl[(size_t)t * kRecord + k];
-            for (int k = 0; k < a.nsp; ++k) z[k] += pl[(size_t)t * kRecord + 16 + k];
-            m += p[(size_t)t * kRecord + 31];
+        for (int k = 0; k < a.rec; ++k) {
+            double s = 0.0;
+            for (int t = lane; t < a.tiles; t += 64) s += a.partial[((size_t)t * a.Rp + r) * a.rec + k];
+            s = wave_sum(s);
+            if (k < a.n_classes) {
+                ex += a.ising_j[k + 1] * s;
+                if (lane == 0) a.bonds[(size_t)r * (a.n_classes + 1) + k + 1] = (long long)s;
+            } else if (k < a.n_classes + a.nsp) {
+                ez += a.ising_zc[k - a.n_classes] * s;
+            } else {
+                mz = s;
+            }
         }
-        ex = 0.0;
-        for (int k = 1; k <= a.n_classes; ++k) {
-            b[k] = wave_sum_ll(b[k]);
-            ex += a.ising_j[k] * (double)b[k];
-            if (lane == 0) a.bonds[(size_t)r * (a.n_classes + 1) + k] = b[k];
-        }
-        ez = 0.0;
-        for (int k = 0; k < a.nsp; ++k) {
-            z[k] = wave_sum_ll(z[k]);
-            ez += a.ising_zc[k] * (double)z[k];
-        }
-        es = 0.0;
-        mx = 0.0; my = 0.0;
-        mz = wave_sum(m);
     }
     if (lane != 0) return;
     const double etot = ex + ez + es;
@@ -514,29 +571,29 @@ __global__ void k_pt_swap(PtArgs a) {
 // ---------------------------------------------------------------------------
 // initial states
 // ---------------------------------------------------------------------------
-
-
 __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
     const size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= (size_t)a.N * a.R) return;
-    const int r = (int)(t / a.N), slot = (int)(t % a.N);
-    u32x4 w = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, a.step, RNG_INIT, 0);
+    if (t >= (size_t)a.N * a.Rp) return;
+    const int slot = (int)(t / a.Rp), r = (int)(t % a.Rp);
+    if (r >= a.R) return;
     if (a.model == 0) {
-        if (a.mode == 1) a.s[t] = (int8_t)(-a.s[t]);
-        else a.s[t] = (w.x >> 31) ? (int8_t)1 : (int8_t)-1;
+        if (a.mode == 1) {
+            a.s[t] = (int8_t)(-a.s[t]);
+        } else {
+            u32x4 w = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, a.step, RNG_INIT, 0);
+            a.s[t] = (w.x >> 31) ? (int8_t)1 : (int8_t)-1;
+        }
         return;
     }
     float x, y, z;
     if (!a.angle) {
-        marsaglia(w, a.seed, (uint32_t)slot, (uint32_t)r, a.step, RNG_INIT, x, y, z);
+        marsaglia_retry(a.seed, (uint32_t)slot, (uint32_t)r, a.step, RNG_INIT, 0, x, y, z);
         if (a.phi) a.phi[t] = 0.f;
     } else {
+        u32x4 w = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, a.step, RNG_INIT, 0);
         float ph;
-        if (a.mode == 1) {
-            angle_move(w, a.dphi, a.phi[t], x, y, z, ph);
-        } else {
-            angle_move(w, 3.14159265358979323846f, 0.0f, x, y, z, ph);
-        }
+        if (a.mode == 1) angle_move(w, a.dphi, a.phi[t], x, y, z, ph);
+        else angle_move(w, 3.14159265358979323846f, 0.0f, x, y, z, ph);
         a.phi[t] = ph;
     }
     a.sx[t] = x;
@@ -545,39 +602,41 @@ __global__ __launch_bounds__(kBlock) void k_init(InitArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// local fields and site energies of one replica
+// local fields and site energies of one replica; spin extract / insert
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_local_fields(MeasArgs a, int r, float* h, float* e,
                                                          int zee, int sia, int model) {
     const int slot = blockIdx.x * kBlock + threadIdx.x;
     if (slot >= a.N) return;
-    const size_t base = (size_t)r * a.N;
+    const size_t row = (size_t)slot * a.Rp + r;
     float hx = 0.f, hy = 0.f, hz = 0.f, x, y, z;
+    const uint32_t* nb = a.ell + (size_t)slot * a.D;
     if (model == 1) {
         for (int k = 0; k < a.D; ++k) {
-            uint32_t en = a.ell[(size_t)k * a.ell_stride + slot];
-            uint32_t j = en & 0xffffffu, c = en >> 24;
-            hx = fmaf(a.jclass[3 * c + 0], a.sx[base + j], hx);
-            hy = fmaf(a.jclass[3 * c + 1], a.sy[base + j], hy);
-            hz = fmaf(a.jclass[3 * c + 2], a.sz[base + j], hz);
+            const uint32_t en = nb[k];
+            const size_t jr = (size_t)(en & 0xffffffu) * a.Rp + r;
+            const uint32_t c = en >> 24;
+            hx = fmaf(a.jclass[3 * c + 0], a.sx[jr], hx);
+            hy = fmaf(a.jclass[3 * c + 1], a.sy[jr], hy);
+            hz = fmaf(a.jclass[3 * c + 2], a.sz[jr], hz);
         }
-        x = a.sx[base + slot];
-        y = a.sy[base + slot];
-        z = a.sz[base + slot];
+        x = a.sx[row];
+        y = a.sy[row];
+        z = a.sz[row];
     } else {
         for (int k = 0; k < a.D; ++k) {
-            uint32_t en = a.ell[(size_t)k * a.ell_stride + slot];
-            uint32_t j = en & 0xffffffu, c = en >> 24;
-            hz = fmaf(a.jclass[3 * c + 2], (float)a.s[base + j], hz);
+            const uint32_t en = nb[k];
+            const uint32_t c = en >> 24;
+            hz = fmaf(a.jclass[3 * c + 2], (float)a.s[(size_t)(en & 0xffffffu) * a.Rp + r], hz);
         }
         x = 0.f;
         y = 0.f;
-        z = (float)a.s[base + slot];
+        z = (float)a.s[row];
     }
     h[3 * slot + 0] = hx;
     h[3 * slot + 1] = hy;
     h[3 * slot + 2] = hz;
-    int sp = a.sp[slot];
+    const int sp = a.sp[slot];
     float en = x * hx + y * hy + z * hz;
     if (zee) en += x * a.zvec[3 * sp] + y * a.zvec[3 * sp + 1] + z * a.zvec[3 * sp + 2];
     if (sia && model == 1)
@@ -585,11 +644,28 @@ __global__ __launch_bounds__(kBlock) void k_local_fields(MeasArgs a, int r, floa
     e[slot] = en;
 }
 
+__global__ void k_extract(const float* plane, const int8_t* s, int N, int Rp, int r, float* out) {
+    const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= N) return;
+    const size_t i = (size_t)slot * Rp + r;
+    out[slot] = plane ? plane[i] : (float)s[i];
+}
+
+__global__ void k_insert(float* plane, int8_t* s, int N, int Rp, int r0, int r1, const float* in) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nr = r1 - r0;
+    if (t >= (size_t)N * nr) return;
+    const int slot = (int)(t / nr), r = r0 + (int)(t % nr);
+    const size_t i = (size_t)slot * Rp + r;
+    if (plane) plane[i] = in[slot];
+    else s[i] = in[slot] > 0.f ? (int8_t)1 : (int8_t)-1;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 void launch_xyz_sweep(const XyzArgs& a, bool angle, bool zee, bool sia, hipStream_t st) {
-    dim3 grid(blocks_for(a.R, a.tiles, a.xcd_map)), block(kBlock);
+    dim3 grid(a.nblocks), block(kBlock);
 #define L(A, Z, S) hipLaunchKernelGGL((k_xyz_sweep<A, Z, S>), grid, block, 0, st, a)
     if (angle) {
         if (zee) { if (sia) L(true, true, true); else L(true, true, false); }
@@ -602,7 +678,7 @@ void launch_xyz_sweep(const XyzArgs& a, bool angle, bool zee, bool sia, hipStrea
 }
 
 void launch_xyz_ovrr(const XyzArgs& a, bool angle, bool zee, hipStream_t st) {
-    dim3 grid(blocks_for(a.R, a.tiles, a.xcd_map)), block(kBlock);
+    dim3 grid(a.nblocks), block(kBlock);
     if (angle) {
         if (zee) hipLaunchKernelGGL((k_xyz_ovrr<true, true>), grid, block, 0, st, a);
         else hipLaunchKernelGGL((k_xyz_ovrr<true, false>), grid, block, 0, st, a);
@@ -613,12 +689,12 @@ void launch_xyz_ovrr(const XyzArgs& a, bool angle, bool zee, hipStream_t st) {
 }
 
 void launch_ising_sweep(const IsingArgs& a, hipStream_t st) {
-    dim3 grid(blocks_for(a.R, a.tiles, a.xcd_map)), block(kBlock);
-    hipLaunchKernelGGL(k_ising_sweep, grid, block, 0, st, a);
+    hipLaunchKernelGGL(k_ising_sweep, dim3(a.nblocks), dim3(kBlock), 0, st, a);
 }
 
 void launch_measure(const MeasArgs& a, int model, bool zee, bool sia, hipStream_t st) {
-    dim3 grid(a.R * a.tiles), block(kBlock);
+    const long long units = (long long)a.tiles * a.G;
+    dim3 grid((unsigned)((units + kWavesPerBlock - 1) / kWavesPerBlock)), block(kBlock);
     if (model == 0) {
         hipLaunchKernelGGL(k_ising_measure, grid, block, 0, st, a);
         return;
@@ -642,7 +718,7 @@ void launch_pt_swap(const PtArgs& a, hipStream_t st) {
 }
 
 void launch_init(const InitArgs& a, hipStream_t st) {
-    size_t n = (size_t)a.N * a.R;
+    size_t n = (size_t)a.N * a.Rp;
     hipLaunchKernelGGL(k_init, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, a);
 }
 
@@ -650,6 +726,18 @@ void launch_local_fields(const MeasArgs& a, int r, float* h, float* e, int zee, 
                          int model, hipStream_t st) {
     hipLaunchKernelGGL(k_local_fields, dim3((a.N + kBlock - 1) / kBlock), dim3(kBlock), 0, st, a,
                        r, h, e, zee, sia, model);
+}
+
+void launch_extract(const float* plane, const int8_t* s, int N, int Rp, int r, float* out,
+                    hipStream_t st) {
+    hipLaunchKernelGGL(k_extract, dim3((N + 255) / 256), dim3(256), 0, st, plane, s, N, Rp, r, out);
+}
+
+void launch_insert(float* plane, int8_t* s, int N, int Rp, int r0, int r1, const float* in,
+                   hipStream_t st) {
+    size_t n = (size_t)N * (r1 - r0);
+    hipLaunchKernelGGL(k_insert, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, plane, s, N,
+                       Rp, r0, r1, in);
 }
 
 }  // namespace ether

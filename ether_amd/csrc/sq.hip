@@ -27,6 +27,7 @@ int ctx_fail(ether_ctx* c, int code, const std::string& msg);
 int ctx_model(ether_ctx* c);
 int ctx_N(ether_ctx* c);
 int ctx_R(ether_ctx* c);
+int ctx_Rp(ether_ctx* c);
 bool ctx_committed(ether_ctx* c);
 const float* ctx_spin_plane(ether_ctx* c, int comp);
 const int8_t* ctx_ising(ether_ctx* c);
@@ -138,13 +139,14 @@ void gemm(const GemmArgs& g, hipStream_t st) {
     hipLaunchKernelGGL(k_gemm_split_bf16, grid, dim3(256), 0, st, g);
 }
 
-// gather spins of every replica into site-ordered planes Z[r*3 + a][site]
+// gather spins of every replica (replica-major device planes, row stride Rp)
+// into site-ordered planes Z[r*3 + a][site]
 __global__ void k_gather(const float* sx, const float* sy, const float* sz, const int8_t* s,
-                         const int* slot_of_site, int N, int R, float* Z) {
+                         const int* slot_of_site, int N, int R, int Rp, float* Z) {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)N * R) return;
     int r = (int)(t / N), i = (int)(t % N);
-    long long src = (long long)r * N + slot_of_site[i];
+    long long src = (long long)slot_of_site[i] * Rp + r;
     if (s) {
         Z[((long long)r * 3 + 0) * N + i] = 0.f;
         Z[((long long)r * 3 + 1) * N + i] = 0.f;
@@ -281,7 +283,8 @@ extern "C" int ether_structure_factor(ether_ctx* c, const int32_t* sc, float* ou
     const bool ising = ctx_model(c) == ETHER_MODEL_ISING;
     hipLaunchKernelGGL(k_gather, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
                        ctx_spin_plane(c, 0), ctx_spin_plane(c, 1), ctx_spin_plane(c, 2),
-                       ising ? ctx_ising(c) : (const int8_t*)nullptr, s->d_map, N, R, s->d_z[0]);
+                       ising ? ctx_ising(c) : (const int8_t*)nullptr, s->d_map, N, R, ctx_Rp(c),
+                       s->d_z[0]);
     HC(c, hipGetLastError());
     // ping-pong: (z0 re, z1 im) -> (z2, z3) -> (z0, z1) ...
     int cur = 0;

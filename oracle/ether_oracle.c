@@ -287,7 +287,124 @@ uint32_t o_ising_threshold(double beta, double dE) {
     return (uint32_t)t;
 }
 
-/* Metropolis (src/Metropolis.f90:35-62): propose, dE, accept if -beta*dE > log(eta) */
+/* Marsaglia point from an accepted 24-bit pair (src/George_Marsaglia.f90:36-38) */
+static void marsaglia_from(uint32_t cx, uint32_t cy, double* s_new) {
+    int64_t a1 = (int64_t)cx - (1 << 23), a2 = (int64_t)cy - (1 << 23);
+    double v1 = a1 * 0x1p-23, v2 = a2 * 0x1p-23, s = (a1 * a1 + a2 * a2) * 0x1p-46;
+    double r = sqrt(1.0 - s);
+    s_new[0] = 2.0 * v1 * r;
+    s_new[1] = 2.0 * v2 * r;
+    s_new[2] = 1.0 - 2.0 * s;
+    normalise(s_new);
+}
+
+static int in_disc(uint32_t cx, uint32_t cy) {
+    int64_t a1 = (int64_t)cx - (1 << 23), a2 = (int64_t)cy - (1 << 23);
+    return a1 * a1 + a2 * a2 < ((int64_t)1 << 46);
+}
+
+void o_propose_site(const o_model* m, uint64_t seed, uint32_t id, int32_t R, uint32_t step,
+                    const double* s_old, const double* phi_old, double* s_new, double* phi_new,
+                    double* eta) {
+    if (m->model == 0 || m->angle) {
+        for (int32_t r = 0; r < R; ++r)
+            o_propose(m, seed, id, (uint32_t)r, step, O_RNG_METROPOLIS, s_old + 3 * (size_t)r,
+                      phi_old ? phi_old[r] : 0.0, s_new + 3 * (size_t)r, phi_new + r, eta + r);
+        return;
+    }
+    /* Marsaglia candidates pooled over each group of 64 consecutive replicas:
+     * a replica whose first pair is outside the disc takes, in replica order,
+     * the second pair of the replicas whose second pair is inside; if those run
+     * out it draws attempts 1, 2, ... itself (same rule as the device wave). */
+    for (int32_t g0 = 0; g0 < R; g0 += 64) {
+        int32_t g1 = g0 + 64 < R ? g0 + 64 : R;
+        uint32_t dx[64], dy[64];
+        int nd = 0;
+        for (int32_t r = g0; r < g1; ++r) {
+            uint32_t w[4];
+            o_rng(seed, id, (uint32_t)r, step, O_RNG_METROPOLIS, 0, w);
+            uint32_t x1 = ((w[3] >> 16) << 8) | (w[0] & 0xffu);
+            uint32_t y1 = ((w[3] & 0xffffu) << 8) | (w[1] & 0xffu);
+            if (in_disc(x1, y1)) { dx[nd] = x1; dy[nd] = y1; ++nd; }
+        }
+        int nj = 0;
+        for (int32_t r = g0; r < g1; ++r) {
+            uint32_t w[4];
+            o_rng(seed, id, (uint32_t)r, step, O_RNG_METROPOLIS, 0, w);
+            eta[r] = eta24(w[2]);
+            phi_new[r] = 0.0;
+            double* sn = s_new + 3 * (size_t)r;
+            if (in_disc(w[0] >> 8, w[1] >> 8)) {
+                marsaglia_from(w[0] >> 8, w[1] >> 8, sn);
+                continue;
+            }
+            if (nj < nd) {
+                marsaglia_from(dx[nj], dy[nj], sn);
+                ++nj;
+                continue;
+            }
+            ++nj;
+            for (uint32_t att = 1;; ++att) {
+                o_rng(seed, id, (uint32_t)r, step, O_RNG_METROPOLIS, att, w);
+                uint32_t x0 = w[0] >> 8, y0 = w[1] >> 8;
+                uint32_t x1 = ((w[3] >> 16) << 8) | (w[0] & 0xffu);
+                uint32_t y1 = ((w[3] & 0xffffu) << 8) | (w[1] & 0xffu);
+                if (in_disc(x0, y0)) { marsaglia_from(x0, y0, sn); break; }
+                if (in_disc(x1, y1)) { marsaglia_from(x1, y1, sn); break; }
+            }
+        }
+    }
+}
+
+int64_t o_sweep_batch(const o_model* m, double* S, double* phi, int32_t R, const int32_t* order,
+                      int32_t n, const int32_t* rng_id, const double* beta, uint64_t seed,
+                      uint32_t step, int64_t* acc) {
+    const size_t N = (size_t)m->n_sites;
+    double* so = (double*)malloc(sizeof(double) * 3 * (size_t)R);
+    double* sn = (double*)malloc(sizeof(double) * 3 * (size_t)R);
+    double* po = (double*)malloc(sizeof(double) * (size_t)R);
+    double* pn = (double*)malloc(sizeof(double) * (size_t)R);
+    double* eta = (double*)malloc(sizeof(double) * (size_t)R);
+    int64_t total = 0;
+    for (int32_t t = 0; t < n; ++t) {
+        const int32_t i = order[t];
+        for (int32_t r = 0; r < R; ++r) {
+            for (int x = 0; x < 3; ++x) so[3 * r + x] = S[(r * N + i) * 3 + x];
+            po[r] = phi ? phi[r * N + i] : 0.0;
+        }
+        if (m->model != 0) o_propose_site(m, seed, (uint32_t)rng_id[i], R, step, so, po, sn, pn, eta);
+        for (int32_t r = 0; r < R; ++r) {
+            double* Sr = S + r * N * 3;
+            double* s = Sr + 3 * (size_t)i;
+            int ok;
+            if (m->model == 0) {
+                /* Ising: word r&3 of the block with id (slot, r>>2) */
+                uint32_t w[4];
+                o_rng(seed, (uint32_t)rng_id[i], (uint32_t)r >> 2, step, O_RNG_METROPOLIS, 0, w);
+                double snew[3] = {s[0], s[1], -s[2]};
+                uint32_t thr = o_ising_threshold(beta[r], o_delta_energy(m, Sr, i, snew));
+                ok = w[r & 3] < thr || thr == 0xffffffffu;
+                if (ok) s[2] = -s[2];
+            } else {
+                double dE = o_delta_energy(m, Sr, i, sn + 3 * (size_t)r);
+                ok = -beta[r] * dE > log(eta[r]);
+                if (ok) {
+                    for (int x = 0; x < 3; ++x) s[x] = sn[3 * r + x];
+                    if (phi) phi[r * N + i] = pn[r];
+                }
+            }
+            if (ok) {
+                ++total;
+                if (acc) acc[r] += 1;
+            }
+        }
+    }
+    free(so); free(sn); free(po); free(pn); free(eta);
+    return total;
+}
+
+/* Metropolis (src/Metropolis.f90:35-62): propose, dE, accept if -beta*dE > log(eta).
+ * One replica on its own (a group of one: no candidate pooling). */
 int64_t o_sweep(const o_model* m, double* S, double* phi, const int32_t* order, int32_t n,
                 const int32_t* rng_id, double beta, uint64_t seed, uint32_t replica,
                 uint32_t step) {
@@ -298,12 +415,11 @@ int64_t o_sweep(const o_model* m, double* S, double* phi, const int32_t* order, 
         double sn[3], phin, eta;
         if (m->model == 0) {
             uint32_t w[4];
-            uint32_t id = (uint32_t)rng_id[i];
-            o_rng(seed, id >> 2, replica, step, O_RNG_METROPOLIS, 0, w);
+            o_rng(seed, (uint32_t)rng_id[i], replica >> 2, step, O_RNG_METROPOLIS, 0, w);
             sn[0] = so[0]; sn[1] = so[1]; sn[2] = -so[2];
             double dE = o_delta_energy(m, S, i, sn);
             uint32_t thr = o_ising_threshold(beta, dE);
-            uint32_t u = w[id & 3u];
+            uint32_t u = w[replica & 3u];
             if (u < thr || thr == 0xffffffffu) {
                 so[2] = sn[2];
                 ++acc;

@@ -95,8 +95,26 @@ void o_propose(const o_model* m, uint64_t seed, uint32_t id, uint32_t replica, u
                uint32_t purpose, const double* s_old, double phi_old, double* s_new,
                double* phi_new, double* eta);
 
-/* Ising acceptance threshold: accept iff w < thr or thr == 0xffffffff */
+/* Proposals of one site for replicas 0..R-1, as the device draws them: XYZ
+ * Marsaglia candidates are pooled over each group of 64 consecutive replicas
+ * (a replica whose first pair misses the disc takes, in replica order, the
+ * second pair of the replicas whose second pair hits it; if those run out it
+ * draws attempts 1, 2, ... itself).  s_old/s_new [R*3], phi_old (nullable),
+ * phi_new, eta [R]. */
+void o_propose_site(const o_model* m, uint64_t seed, uint32_t id, int32_t R, uint32_t step,
+                    const double* s_old, const double* phi_old, double* s_new, double* phi_new,
+                    double* eta);
+
+/* Ising acceptance threshold: accept iff w < thr or thr == 0xffffffff.
+ * The Ising word of replica r at a site is word r&3 of the Philox block with
+ * ctr {rng_id, r>>2, step, purpose}. */
 uint32_t o_ising_threshold(double beta, double dE);
+
+/* One sweep of R replicas (S[R][N][3], phi[R][N] nullable, beta[R]) in the
+ * device's order and with its proposal rule; acc[R] (nullable) += accepted. */
+int64_t o_sweep_batch(const o_model* m, double* S, double* phi, int32_t R, const int32_t* order,
+                      int32_t n, const int32_t* rng_id, const double* beta, uint64_t seed,
+                      uint32_t step, int64_t* acc);
 
 /* One Metropolis sweep (src/Metropolis.f90:35-62) visiting sites in `order`
  * (n entries); rng_id[i] is the RNG id of site i.  S [3N], phi [N] (nullable

@@ -59,6 +59,11 @@ def lib():
         L.o_magnetisation.argtypes = [C.POINTER(OModel), _vp, _vp]
         L.o_propose.argtypes = [C.POINTER(OModel), C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32,
                                 C.c_uint32, _vp, C.c_double, _vp, _vp, _vp]
+        L.o_propose_site.argtypes = [C.POINTER(OModel), C.c_uint64, C.c_uint32, C.c_int32,
+                                     C.c_uint32, _vp, _vp, _vp, _vp, _vp]
+        L.o_sweep_batch.restype = C.c_int64
+        L.o_sweep_batch.argtypes = [C.POINTER(OModel), _vp, _vp, C.c_int32, _vp, C.c_int32, _vp,
+                                    _vp, C.c_uint64, C.c_uint32, _vp]
         L.o_ising_threshold.restype = C.c_uint32
         L.o_ising_threshold.argtypes = [C.c_double, C.c_double]
         L.o_sweep.restype = C.c_int64
@@ -202,6 +207,30 @@ class Oracle:
         lib().o_propose(C.byref(self.m), C.c_uint64(seed), id_, replica, step, purpose, _p(so),
                         float(phi_old), _p(sn), C.byref(ph), C.byref(eta))
         return sn, ph.value, eta.value
+
+    def propose_site(self, seed, id_, step, s_old, phi_old=None):
+        """Proposals of one site for all R replicas (device rule incl. pooling)."""
+        so = _a(s_old, np.float64).reshape(-1, 3)
+        R = so.shape[0]
+        po = None if phi_old is None else _a(phi_old, np.float64)
+        sn = np.zeros((R, 3))
+        pn = np.zeros(R)
+        eta = np.zeros(R)
+        lib().o_propose_site(C.byref(self.m), C.c_uint64(seed), id_, R, step, _p(so), _p(po),
+                             _p(sn), _p(pn), _p(eta))
+        return sn, pn, eta
+
+    def sweep_batch(self, S, phi, order, rng_id, betas, seed, step):
+        """In place on S (R,N,3) float64 and phi (R,N) (or None); returns accepted per replica."""
+        assert S.dtype == np.float64 and S.flags.c_contiguous
+        R = S.shape[0]
+        o = _a(order, np.int32)
+        r = _a(rng_id, np.int32)
+        b = _a(betas, np.float64)
+        acc = np.zeros(R, np.int64)
+        lib().o_sweep_batch(C.byref(self.m), _p(S), _p(phi), R, _p(o), o.size, _p(r), _p(b),
+                            C.c_uint64(seed), step, _p(acc))
+        return acc
 
     def sweep(self, S, phi, order, rng_id, beta, seed, replica, step):
         """In place on S (N,3) float64 and phi (N,) float64 (or None)."""

@@ -116,31 +116,36 @@ def test_xyz_local_fields_and_energy(name, over):
     eng.close()
 
 
-def _oracle_sweep_with_margins(m, orc, S, phi, order, rid, beta, seed, r, step):
-    """The oracle's sweep, site by site, recording -beta dE - ln eta."""
-    marg = np.zeros(m.n_sites)
+def _oracle_sweep_with_margins(m, orc, states, order, rid, betas, seed, step):
+    """The oracle's sweep of all replicas, site by site with the device's proposal
+    rule (pooled Marsaglia candidates), recording -beta dE - ln eta."""
+    R = len(states)
+    S = np.stack([s for s, _ in states]).copy()
+    phi = np.stack([p for _, p in states]).copy()
+    marg = np.zeros((R, m.n_sites))
     for i in order:
-        sn, ph, eta = orc.propose(seed, int(rid[i]), r, step, S[i], phi[i])
-        dE = orc.delta_energy(S, i, sn)
-        marg[i] = -beta * dE - math.log(eta)
-        if marg[i] > 0:
-            S[i] = sn
-            phi[i] = ph
-    return marg
+        sn, pn, eta = orc.propose_site(seed, int(rid[i]), step, S[:, i, :], phi[:, i])
+        for r in range(R):
+            dE = orc.delta_energy(S[r], i, sn[r])
+            marg[r, i] = -betas[r] * dE - math.log(eta[r])
+            if marg[r, i] > 0:
+                S[r, i] = sn[r]
+                phi[r, i] = pn[r]
+    return S, phi, marg
 
 
-@pytest.mark.parametrize("name,over", [
-    ("sc_heis_afm_32", dict(sc="8 8 8")),
-    ("tri_afm_96", dict(sc="12 12 1")),                       # ANGLE + Zeeman + SIA
-    ("tri_afm_96", dict(sc="12 12 1", angle=".F. 0")),        # Marsaglia + Zeeman + SIA
-    ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11")),
+@pytest.mark.parametrize("name,over,R", [
+    ("sc_heis_afm_32", dict(sc="8 8 8"), 2),
+    ("sc_heis_afm_32", dict(sc="4 4 4"), 70),                 # two pooling groups, one partial
+    ("tri_afm_96", dict(sc="12 12 1"), 2),                    # ANGLE + Zeeman + SIA
+    ("tri_afm_96", dict(sc="12 12 1", angle=".F. 0"), 3),     # Marsaglia + Zeeman + SIA
+    ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11"), 2),
 ])
-def test_xyz_one_sweep_matches_oracle(name, over):
-    say(f"xyz sweep {name} {over}")
-    R = 2
+def test_xyz_one_sweep_matches_oracle(name, over, R):
+    say(f"xyz sweep {name} {over} R={R}")
     m, eng, orc = pair(name, R, **over)
-    temps = [2.0, 8.0] if name == "pyro_pt_16" else [5.0, 20.0]
-    betas = m.betas(temps)
+    lo, hi = (2.0, 8.0) if name == "pyro_pt_16" else (5.0, 20.0)
+    betas = m.betas(np.linspace(lo, hi, R))
     eng.set_betas(betas)
     states = _xyz_state(m, eng, R, 5)
     eng.sweep(1)
@@ -149,19 +154,17 @@ def test_xyz_one_sweep_matches_oracle(name, over):
     for s in range(m.nn_count.shape[0]):
         for i in range(m.n_sites):
             nbrs[i].update(m.nn_idx[s, i, : m.nn_count[s, i]].tolist())
+    S, phi, marg = _oracle_sweep_with_margins(m, orc, states, order, rid, betas, m.tags.seed, 0)
     n_checked = 0
     for r in range(R):
-        S0, phi0 = states[r]
-        S, phi = S0.copy(), phi0.copy()
-        marg = _oracle_sweep_with_margins(m, orc, S, phi, order, rid, betas[r], m.tags.seed, r, 0)
         Sg, phig = eng.get_spins(r)
         tainted = set()
         for i in order:
             if i in tainted:
                 continue
-            dev = np.abs(Sg[i] - S[i]).max()
+            dev = np.abs(Sg[i] - S[r, i]).max()
             if dev > 2e-6:
-                assert abs(marg[i]) < 1e-3, f"replica {r} site {i}: margin {marg[i]}, dev {dev}"
+                assert abs(marg[r, i]) < 1e-3, f"replica {r} site {i}: margin {marg[r, i]}, dev {dev}"
                 tainted |= nbrs[i]
             else:
                 n_checked += 1

@@ -72,6 +72,56 @@ def test_marsaglia_formula_first_candidate():
         assert eta == ((int(w[2]) >> 8) + 1) / 2.0 ** 24
 
 
+def test_pooled_marsaglia_rule():
+    """The device pools Marsaglia candidates over 64 replicas of a site: every
+    replica still gets a point uniform on S^2, replicas whose first pair is in
+    the disc keep it, and a group of one replica is the plain rule."""
+    orc = _one_site_oracle()
+    # group of one == plain per-replica proposal
+    for i in range(50):
+        sn, _, eta = orc.propose_site(11, i, 0, [[0, 0, 1.0]])
+        s1, _, eta1 = orc.propose(11, i, 0, 0, [0, 0, 1.0])
+        assert np.allclose(sn[0], s1, atol=0) and eta[0] == eta1
+    # first pair inside -> identical to the plain rule for that replica
+    R = 200
+    pts = []
+    for i in range(300):
+        sn, _, eta = orc.propose_site(5, i, 0, np.tile([0, 0, 1.0], (R, 1)))
+        for r in range(R):
+            w = o.rng(5, i, r, 0, o.RNG_METROPOLIS, 0)
+            a1, a2 = int(w[0] >> 8) - (1 << 23), int(w[1] >> 8) - (1 << 23)
+            if a1 * a1 + a2 * a2 < (1 << 46):
+                s1, _, _ = orc.propose(5, i, r, 0, [0, 0, 1.0])
+                assert np.allclose(sn[r], s1, atol=0)
+        pts.append(sn)
+    S = np.concatenate(pts)
+    n = len(S)
+    assert np.allclose(np.linalg.norm(S, axis=1), 1.0, atol=1e-14)
+    assert np.all(np.abs(S.mean(axis=0)) < 5 / math.sqrt(3 * n))
+    assert abs((S[:, 2] ** 2).mean() - 1 / 3) < 5 * math.sqrt(4 / 45 / n)
+    h, _ = np.histogram(S[:, 2], bins=10, range=(-1, 1))
+    assert ((h - n / 10) ** 2 / (n / 10)).sum() < 30.0
+
+
+def test_batch_sweep_matches_single_replica_sweeps_for_ising():
+    """Ising draws do not pool: the batch sweep equals per-replica sweeps."""
+    from ether_amd import build_neighbours
+    cnt, idx = build_neighbours(np.eye(3), [[0, 0, 0]], (4, 4, 4), (0, 0, 0), [1.0])
+    orc = o.Oracle(0, 64, 1, np.ones(64, np.int32), np.ones(64), cnt, idx, [0, 0, -1.0], [1.0])
+    rng = np.random.default_rng(0)
+    R = 9
+    S = np.zeros((R, 64, 3))
+    S[:, :, 2] = np.where(rng.random((R, 64)) < 0.5, 1.0, -1.0)
+    S2 = S.copy()
+    order = np.arange(64, dtype=np.int32)
+    betas = np.linspace(0.2, 0.6, R)
+    orc.sweep_batch(S, None, order, order, betas, 3, 7)
+    for r in range(R):
+        Sr = np.ascontiguousarray(S2[r])
+        orc.sweep(Sr, None, order, order, betas[r], 3, r, 7)
+        assert np.array_equal(Sr, S[r])
+
+
 def test_angle_move_distribution():
     dphi = math.radians(5.0)
     orc = _one_site_oracle(angle=True, dphi=dphi)

@@ -28,6 +28,9 @@ const std::string& global_error();
 
 using namespace ether;
 
+// longest schedule period (MCS) captured as one graph
+constexpr long long kMaxGraphPeriod = 1024;
+
 struct ether_ctx {
     int model = 0, N = 0, R = 0;
     uint64_t seed = 0;
@@ -98,6 +101,20 @@ struct ether_ctx {
     std::vector<double> beta_h;
     uint32_t step = 0, init_count = 0, pt_count = 0;
     hipStream_t stream = nullptr;
+    // device counters d_ctr = {sweep base, swap base, swap count} and their host
+    // mirrors; kernels use counter + baked offset so a captured graph replays
+    uint32_t* d_ctr = nullptr;
+    uint32_t dev_step = 0, dev_pt = 0, dev_sc = 0;
+    bool capturing = false;
+    uint32_t cap_step0 = 0, cap_pt0 = 0;
+    int use_graphs = 1;
+    struct GraphEntry {
+        int phase, P, coa, ovrr_iv, pt_iv, n_pt;
+        double ovrr_p;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
     // kernel timing: event pairs per kind, reused from a pool
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
@@ -201,6 +218,10 @@ extern "C" void ether_ctx_destroy(ether_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     ether_ext_destroy(c);
+    for (auto& g : c->graphs) {
+        (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphDestroy(g.graph);
+    }
     for (auto& e : c->ev_pool) {
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
@@ -301,8 +322,11 @@ extern "C" int ether_set_proposal(ether_ctx* c, int32_t angle, double dphi_rad) 
     return ETHER_OK;
 }
 
+static void drop_graphs(ether_ctx* c);
+
 extern "C" int ether_set_compat(ether_ctx* c, int32_t flags) {
     if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
+    if (c->compat != flags) drop_graphs(c);
     c->compat = flags;
     return ETHER_OK;
 }
@@ -469,6 +493,11 @@ extern "C" int ether_commit(ether_ctx* c) {
     // ---- ELL ----
     int D = 0;
     for (int i = 0; i < N; ++i) D = std::max(D, (int)edges[i].size());
+    // rows padded (class 0, zero coupling) to an unrolled width bucket
+    {
+        const int b = ell_bucket(std::max(D, 1));
+        D = b ? b : D;
+    }
     c->D = D;
     // slot-major rows: a wave's whole neighbour list is one scalar-loadable run
     std::vector<uint32_t> ell((size_t)std::max(D, 1) * N, 0u);
@@ -574,6 +603,9 @@ extern "C" int ether_commit(ether_ctx* c) {
     TRY(dev_alloc(c, &c->d_gstart, (size_t)R + 1));
     TRY(dev_alloc(c, &c->d_scratch_h, (size_t)N * 4));
     TRY(dev_alloc(c, &c->d_scratch_e, (size_t)N));
+    TRY(dev_alloc(c, &c->d_ctr, 4));
+    launch_set_ctr(c->d_ctr, 0, 0, 0, c->stream);
+    c->dev_step = c->dev_pt = c->dev_sc = 0;
     TRY(upload(c, c->d_ell, ell.data(), ell.size()));
     TRY(upload(c, c->d_sp, sp.data(), sp.size()));
     TRY(upload(c, c->d_w, w.data(), w.size()));
@@ -687,6 +719,7 @@ extern "C" int ether_set_ladders(ether_ctx* c, int32_t n_groups, const int32_t* 
                 return fail(c, ETHER_ERR_ARG, "ether_set_ladders: groups must be increasing and <= R");
         TRY(upload(c, c->d_gstart, group_start, (size_t)n_groups + 1));
     }
+    drop_graphs(c);   // the ladder count is baked into captured swap nodes
     c->n_groups = n_groups;
     HIPCHK(c, hipMemsetAsync(c->d_pt_acc, 0, sizeof(unsigned long long) * c->R, c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_pt_att, 0, sizeof(unsigned long long) * c->R, c->stream));
@@ -728,7 +761,8 @@ static XyzArgs xyz_args(ether_ctx* c, int colour) {
     a.slot_of = c->d_slot_of;
     a.beta_slot = c->d_beta_f;
     a.seed = c->seed;
-    a.step = c->step;
+    a.ctr = c->d_ctr;
+    a.step_off = c->step - (c->capturing ? c->cap_step0 : c->dev_step);
     a.pass = 0;
     a.frac = 1.0f;
     a.acc = c->d_acc;
@@ -738,6 +772,8 @@ static XyzArgs xyz_args(ether_ctx* c, int colour) {
     a.zvec = c->d_zvec;
     a.dsia = c->d_dsia;
     a.dphi = (float)c->dphi;
+    a.zee = c->zee;
+    a.sia = c->sia;
     return a;
 }
 
@@ -758,7 +794,8 @@ static IsingArgs ising_args(ether_ctx* c, int colour) {
     a.table_in_lds = ((long long)c->R * c->ising_table <= kIsingLdsTable) ? 1 : 0;
     a.slot_of = c->d_slot_of;
     a.seed = c->seed;
-    a.step = c->step;
+    a.ctr = c->d_ctr;
+    a.step_off = c->step - (c->capturing ? c->cap_step0 : c->dev_step);
     a.acc = c->d_acc;
     a.n_classes = c->n_classes;
     a.stride = c->d_stride;
@@ -921,13 +958,16 @@ static int enqueue_pt(ether_ctx* c, int parity) {
     PtArgs p;
     p.n_groups = c->n_groups;
     p.gstart = c->d_gstart;
-    p.parity = parity & 1;
+    // inside a captured graph the parity comes from the device swap counter
+    p.parity = c->capturing ? -1 : (parity & 1);
     p.rep_at = c->d_rep_at;
     p.slot_of = c->d_slot_of;
     p.beta_slot = c->d_beta;
     p.E_pt = c->d_Ept;
     p.seed = c->seed;
-    p.swap_index = c->pt_count++;
+    p.ctr = c->d_ctr;
+    p.pt_off = c->pt_count - (c->capturing ? c->cap_pt0 : c->dev_pt);
+    c->pt_count++;
     p.acc = c->d_pt_acc;
     p.att = c->d_pt_att;
     p.trips = c->d_trips;
@@ -1084,6 +1124,99 @@ extern "C" int ether_pt_state(ether_ctx* c, int32_t* slot_of, int64_t* acc, int6
     return ETHER_OK;
 }
 
+// One MCS of the schedule of src/ether.f90:159-203.
+static int enqueue_step(ether_ctx* c, int32_t stepi, int32_t mcs_eq, int32_t coa, int ovrr_iv,
+                        double ovrr_passes, int pt_iv, int* sc) {
+    // src/ether.f90:163-169
+    if (ovrr_iv && stepi % ovrr_iv == 0 && stepi % coa != 0) TRY(enqueue_ovrr(c, ovrr_passes));
+    // src/ether.f90:172
+    TRY(enqueue_sweep(c));
+    // src/ether.f90:189-198
+    if (pt_iv && stepi % pt_iv == 0 && stepi % coa != 0) {
+        ++*sc;
+        TRY(enqueue_pt(c, *sc % 2));
+    }
+    // src/ether.f90:201 -> src/observable.f90:30-31
+    if (stepi > mcs_eq && stepi % coa == 0) TRY(enqueue_measure(c, 1));
+    return ETHER_OK;
+}
+
+static void drop_graphs(ether_ctx* c) {
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& g : c->graphs) {
+        (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphDestroy(g.graph);
+    }
+    c->graphs.clear();
+}
+
+// Capture (once) one period of P MCS, steps congruent to 1..P mod P, either
+// all in equilibration (phase 0: no samples) or all in production (phase 1).
+// Kernels address their random streams as device counter + offset, and the
+// graph ends with a counter advance, so the same graph replays any period.
+static int graph_for(ether_ctx* c, int phase, int P, int coa, int ovrr_iv, double ovrr_passes,
+                     int pt_iv, int* index) {
+    for (size_t q = 0; q < c->graphs.size(); ++q) {
+        const auto& g = c->graphs[q];
+        if (g.phase == phase && g.P == P && g.coa == coa && g.ovrr_iv == ovrr_iv &&
+            g.ovrr_p == ovrr_passes && g.pt_iv == pt_iv) {
+            *index = (int)q;
+            return ETHER_OK;
+        }
+    }
+    const uint32_t step0 = c->step, pt0 = c->pt_count;
+    c->capturing = true;
+    c->cap_step0 = step0;
+    c->cap_pt0 = pt0;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = ETHER_OK, n_pt = 0, sc_local = 0;
+    for (int k = 1; k <= P && rc == ETHER_OK; ++k) {
+        // position k in the period stands for every stepi = k (mod P)
+        const int32_t eq = phase ? 0 : P + 1;
+        const int before = sc_local;
+        rc = enqueue_step(c, k, eq, coa, ovrr_iv, ovrr_passes, pt_iv, &sc_local);
+        n_pt += sc_local - before;
+    }
+    if (rc == ETHER_OK) launch_advance_ctr(c->d_ctr, (uint32_t)P, (uint32_t)n_pt, c->stream);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    c->capturing = false;
+    c->step = step0;
+    c->pt_count = pt0;
+    if (rc != ETHER_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(c, ETHER_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return fail(c, ETHER_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    c->graphs.push_back({phase, P, coa, ovrr_iv, pt_iv, n_pt, ovrr_passes, graph, exec});
+    *index = (int)c->graphs.size() - 1;
+    return ETHER_OK;
+}
+
+static int replay_graph(ether_ctx* c, int gi, int* sc) {
+    const auto& g = c->graphs[gi];
+    if (c->dev_step != c->step || c->dev_pt != c->pt_count || c->dev_sc != (uint32_t)*sc) {
+        launch_set_ctr(c->d_ctr, c->step, c->pt_count, (uint32_t)*sc, c->stream);
+        c->dev_step = c->step;
+        c->dev_pt = c->pt_count;
+        c->dev_sc = (uint32_t)*sc;
+    }
+    HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
+    c->step += (uint32_t)g.P;
+    c->dev_step += (uint32_t)g.P;
+    c->pt_count += (uint32_t)g.n_pt;
+    c->dev_pt += (uint32_t)g.n_pt;
+    c->dev_sc += (uint32_t)g.n_pt;
+    *sc += g.n_pt;
+    return ETHER_OK;
+}
+
 extern "C" int ether_run(ether_ctx* c, int32_t first_step, int32_t n_steps, int32_t mcs_eq,
                          int32_t coa, int32_t ovrr_interval, double ovrr_passes,
                          int32_t pt_interval, int32_t* swap_count) {
@@ -1093,22 +1226,39 @@ extern "C" int ether_run(ether_ctx* c, int32_t first_step, int32_t n_steps, int3
     const bool do_ovrr = ovrr_interval > 0 && c->model == ETHER_MODEL_XYZ && !c->sia &&
                          ovrr_passes > 0.0;
     const bool do_pt = pt_interval > 0 && c->n_groups > 0;
+    const int ovrr_iv = do_ovrr ? ovrr_interval : 0;
+    const int pt_iv = do_pt ? pt_interval : 0;
     int sc = swap_count ? *swap_count : 0;
-    for (int32_t k = 0; k < n_steps; ++k) {
-        const int32_t stepi = first_step + k;
-        // src/ether.f90:163-169
-        if (do_ovrr && stepi % ovrr_interval == 0 && stepi % coa != 0) TRY(enqueue_ovrr(c, ovrr_passes));
-        // src/ether.f90:172
-        TRY(enqueue_sweep(c));
-        // src/ether.f90:189-198
-        if (do_pt && stepi % pt_interval == 0 && stepi % coa != 0) {
-            ++sc;
-            TRY(enqueue_pt(c, sc % 2));
+    // the schedule repeats with period P = lcm(coa, ovrr, pt); whole periods that
+    // lie entirely before or after mcs_eq replay a captured graph
+    auto gcd = [](long long x, long long y) { while (y) { long long t = x % y; x = y; y = t; } return x; };
+    long long P = coa;
+    if (ovrr_iv) P = P / gcd(P, ovrr_iv) * ovrr_iv;
+    if (pt_iv) P = P / gcd(P, pt_iv) * pt_iv;
+    const bool graphs_ok = c->use_graphs && !c->timing && P <= kMaxGraphPeriod;
+    const int32_t last = first_step + n_steps - 1;
+    int32_t stepi = first_step;
+    while (stepi <= last) {
+        if (graphs_ok && (stepi - 1) % P == 0 && stepi + P - 1 <= last) {
+            const int phase = stepi > mcs_eq ? 1 : (stepi + P - 1 <= mcs_eq ? 0 : -1);
+            if (phase >= 0) {
+                int gi = -1;
+                TRY(graph_for(c, phase, (int)P, coa, ovrr_iv, ovrr_passes, pt_iv, &gi));
+                TRY(replay_graph(c, gi, &sc));
+                stepi += (int32_t)P;
+                continue;
+            }
         }
-        // src/ether.f90:201 -> src/observable.f90:30-31
-        if (stepi > mcs_eq && stepi % coa == 0) TRY(enqueue_measure(c, 1));
+        TRY(enqueue_step(c, stepi, mcs_eq, coa, ovrr_iv, ovrr_passes, pt_iv, &sc));
+        ++stepi;
     }
     if (swap_count) *swap_count = sc;
+    return ETHER_OK;
+}
+
+extern "C" int ether_set_graphs(ether_ctx* c, int32_t on) {
+    if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
+    c->use_graphs = on ? 1 : 0;
     return ETHER_OK;
 }
 

@@ -51,7 +51,8 @@ struct XyzArgs {
     const int* slot_of;
     const float* beta_slot;
     uint64_t seed;
-    uint32_t step;
+    const uint32_t* ctr;   // device counters: [0] sweep base, [1] swap base, [2] swap count
+    uint32_t step_off;     // sweep counter = ctr[0] + step_off
     uint32_t pass;
     float frac;
     unsigned long long* acc;
@@ -61,6 +62,8 @@ struct XyzArgs {
     const float* zvec;
     const float* dsia;
     float dphi;
+    int zee;               // Zeeman term on (wave-uniform runtime flag)
+    int sia;               // single-ion anisotropy on
 };
 
 struct IsingArgs {
@@ -78,7 +81,8 @@ struct IsingArgs {
     int xcd_map;
     const int* slot_of;
     uint64_t seed;
-    uint32_t step;
+    const uint32_t* ctr;
+    uint32_t step_off;
     unsigned long long* acc;
     int n_classes;
     const int* stride;     // [n_classes+1], stride[0] = 0
@@ -138,13 +142,14 @@ struct FinArgs {
 struct PtArgs {
     int n_groups;
     const int* gstart;
-    int parity;
+    int parity;            // 0/1, or -1: (ctr[2] + pt_off + 1) & 1 (the reference's swap_count parity)
     int* rep_at;
     int* slot_of;
     const double* beta_slot;
     const double* E_pt;
     uint64_t seed;
-    uint32_t swap_index;
+    const uint32_t* ctr;
+    uint32_t pt_off;       // swap index = ctr[1] + pt_off
     unsigned long long* acc;
     unsigned long long* att;
     unsigned long long* trips;
@@ -187,6 +192,10 @@ inline int launch_blocks(long long units, int xcd_map) {
     return (int)nb;
 }
 
+// ELL row width: D rounded up to an unrolled bucket (4,6,8,12,18,24,32), or D
+// itself above 32 (runtime loop).  Rows are padded with class-0 entries.
+int ell_bucket(int D);
+
 // launchers (kernels.hip)
 void launch_xyz_sweep(const XyzArgs& a, bool angle, bool zee, bool sia, hipStream_t st);
 void launch_xyz_ovrr(const XyzArgs& a, bool angle, bool zee, hipStream_t st);
@@ -199,6 +208,11 @@ void launch_local_fields(const MeasArgs& a, int r, float* h, float* e, int zee, 
                          int model, hipStream_t st);
 void launch_extract(const float* plane, const int8_t* s, int N, int Rp, int r, float* out,
                     hipStream_t st);
+// device counters: set to absolute values / advance by deltas (single-thread kernels,
+// used instead of memset nodes so that captured graphs replay correctly)
+void launch_set_ctr(uint32_t* ctr, uint32_t step, uint32_t swaps, uint32_t swap_count,
+                    hipStream_t st);
+void launch_advance_ctr(uint32_t* ctr, uint32_t dstep, uint32_t dswaps, hipStream_t st);
 void launch_insert(float* plane, int8_t* s, int N, int Rp, int r0, int r1, const float* in,
                    hipStream_t st);
 

@@ -159,16 +159,62 @@ __device__ __forceinline__ void book_acceptance(unsigned* sAcc, int nrep_lane, c
     }
 }
 
+// Exchange field h = sum_k J_{c_k} o S_{j_k} of replica lane r at the site
+// whose ELL row is nb (src/Hamiltonian.f90:72-103 / src/get_ovrr_vec.f90:64-82).
+// With DM > 0 the row has exactly DM entries (padded with class 0, J = 0) and
+// all neighbour loads are issued before the first use, so their latencies
+// overlap each other and the generator work that follows.  Accumulation order
+// is k = 0..D-1 in both forms.
+template <int DM>
+__device__ __forceinline__ void local_field(const float* __restrict__ sx,
+                                            const float* __restrict__ sy,
+                                            const float* __restrict__ sz, const uint32_t* nb,
+                                            int D, int Rp, int r, const float* sJ, float& hx,
+                                            float& hy, float& hz) {
+    hx = 0.f;
+    hy = 0.f;
+    hz = 0.f;
+    if constexpr (DM > 0) {
+        float vx[DM], vy[DM], vz[DM];
+        uint32_t cl[DM];
+#pragma unroll
+        for (int k = 0; k < DM; ++k) {
+            const uint32_t e = nb[k];
+            const size_t jr = (size_t)(e & 0xffffffu) * Rp + r;
+            cl[k] = e >> 24;
+            vx[k] = sx[jr];
+            vy[k] = sy[jr];
+            vz[k] = sz[jr];
+        }
+#pragma unroll
+        for (int k = 0; k < DM; ++k) {
+            hx = fmaf(sJ[3 * cl[k] + 0], vx[k], hx);
+            hy = fmaf(sJ[3 * cl[k] + 1], vy[k], hy);
+            hz = fmaf(sJ[3 * cl[k] + 2], vz[k], hz);
+        }
+    } else {
+        for (int k = 0; k < D; ++k) {
+            const uint32_t e = nb[k];
+            const size_t jr = (size_t)(e & 0xffffffu) * Rp + r;
+            const uint32_t c = e >> 24;
+            hx = fmaf(sJ[3 * c + 0], sx[jr], hx);
+            hy = fmaf(sJ[3 * c + 1], sy[jr], hy);
+            hz = fmaf(sJ[3 * c + 2], sz[jr], hz);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // XYZ Metropolis, one colour phase
 // ---------------------------------------------------------------------------
-template <bool ANGLE, bool ZEE, bool SIA>
+template <bool ANGLE, int DM>
 __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
     __shared__ float sJ[3 * (kMaxClasses + 1)];
     __shared__ float sZ[3 * kMaxSpecies];
     __shared__ float sD[3 * kMaxSpecies];
     __shared__ uint64_t sPool[kWavesPerBlock][kWave];
     __shared__ unsigned sAcc[kBlock];
+    const bool ZEE = a.zee != 0, SIA = a.sia != 0;   // wave-uniform
     for (int t = threadIdx.x; t < 3 * (a.n_classes + 1); t += kBlock) sJ[t] = a.jclass[t];
     if (ZEE)
         for (int t = threadIdx.x; t < 3 * a.nsp; t += kBlock) sZ[t] = a.zvec[t];
@@ -182,6 +228,7 @@ __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
     const int r = g * kWave + lane;
     const bool rv = r < a.R;
     const float beta = rv ? a.beta_slot[a.slot_of[r]] : 0.f;
+    const uint32_t step = a.ctr[0] + a.step_off;
     uint64_t* pool = sPool[w];
     unsigned accepted = 0;
     const int s0 = a.c_begin + (chunk * kWavesPerBlock + w) * kSitesPerWave;
@@ -190,29 +237,21 @@ __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
         if (slot >= a.c_end) break;
         const size_t row = (size_t)slot * a.Rp + r;
         const float x = a.sx[row], y = a.sy[row], z = a.sz[row];
-        float hx = 0.f, hy = 0.f, hz = 0.f;
-        const uint32_t* nb = a.ell + (size_t)slot * a.D;
-        for (int k = 0; k < a.D; ++k) {
-            const uint32_t e = nb[k];                  // scalar load
-            const size_t jr = (size_t)(e & 0xffffffu) * a.Rp + r;
-            const uint32_t c = e >> 24;
-            hx = fmaf(sJ[3 * c + 0], a.sx[jr], hx);
-            hy = fmaf(sJ[3 * c + 1], a.sy[jr], hy);
-            hz = fmaf(sJ[3 * c + 2], a.sz[jr], hz);
-        }
+        float hx, hy, hz;
+        local_field<DM>(a.sx, a.sy, a.sz, a.ell + (size_t)slot * a.D, a.D, a.Rp, r, sJ, hx, hy, hz);
         const int sp = (ZEE || SIA) ? a.sp[slot] : 0;
         if (ZEE) {
             hx += sZ[3 * sp + 0];
             hy += sZ[3 * sp + 1];
             hz += sZ[3 * sp + 2];
         }
-        const u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, a.step, RNG_METROPOLIS, 0);
+        const u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, step, RNG_METROPOLIS, 0);
         const float eta = eta24f(wr.z);
         float nx = 0.f, ny = 0.f, nz = 1.f, nphi = 0.f;
         if (ANGLE) {
             angle_move(wr, a.dphi, a.phi[row], nx, ny, nz, nphi);
         } else {
-            marsaglia_pooled(wr, rv, a.seed, (uint32_t)slot, (uint32_t)r, a.step, pool, nx, ny, nz);
+            marsaglia_pooled(wr, rv, a.seed, (uint32_t)slot, (uint32_t)r, step, pool, nx, ny, nz);
         }
         float dE = (nx - x) * hx + (ny - y) * hy + (nz - z) * hz;
         if (SIA) {
@@ -235,10 +274,11 @@ __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
 // ---------------------------------------------------------------------------
 // Overrelaxation, one colour phase: S' = 2(A.S)/(A.A) A - S, renormalised.
 // ---------------------------------------------------------------------------
-template <bool ANGLE, bool ZEE>
+template <bool ANGLE, int DM>
 __global__ __launch_bounds__(kBlock) void k_xyz_ovrr(XyzArgs a) {
     __shared__ float sJ[3 * (kMaxClasses + 1)];
     __shared__ float sZ[3 * kMaxSpecies];
+    const bool ZEE = a.zee != 0;
     for (int t = threadIdx.x; t < 3 * (a.n_classes + 1); t += kBlock) sJ[t] = a.jclass[t];
     if (ZEE)
         for (int t = threadIdx.x; t < 3 * a.nsp; t += kBlock) sZ[t] = a.zvec[t];
@@ -248,25 +288,18 @@ __global__ __launch_bounds__(kBlock) void k_xyz_ovrr(XyzArgs a) {
     const int w = wave_id(), lane = lane_id();
     const int r = g * kWave + lane;
     if (r >= a.R) return;
+    const uint32_t step = a.ctr[0] + a.step_off;
     const int s0 = a.c_begin + (chunk * kWavesPerBlock + w) * kSitesPerWave;
     for (int q = 0; q < kSitesPerWave; ++q) {
         const int slot = s0 + q;
         if (slot >= a.c_end) break;
         if (a.frac < 1.0f) {
-            u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, a.step, RNG_OVRR, a.pass);
+            u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, step, RNG_OVRR, a.pass);
             if (!((float)(wr.x >> 8) * 0x1p-24f < a.frac)) continue;
         }
         const size_t row = (size_t)slot * a.Rp + r;
-        float hx = 0.f, hy = 0.f, hz = 0.f;
-        const uint32_t* nb = a.ell + (size_t)slot * a.D;
-        for (int k = 0; k < a.D; ++k) {
-            const uint32_t e = nb[k];
-            const size_t jr = (size_t)(e & 0xffffffu) * a.Rp + r;
-            const uint32_t c = e >> 24;
-            hx = fmaf(sJ[3 * c + 0], a.sx[jr], hx);
-            hy = fmaf(sJ[3 * c + 1], a.sy[jr], hy);
-            hz = fmaf(sJ[3 * c + 2], a.sz[jr], hz);
-        }
+        float hx, hy, hz;
+        local_field<DM>(a.sx, a.sy, a.sz, a.ell + (size_t)slot * a.D, a.D, a.Rp, r, sJ, hx, hy, hz);
         if (ZEE) {
             const int sp = a.sp[slot];
             hx += sZ[3 * sp + 0];
@@ -300,6 +333,7 @@ __device__ __forceinline__ void class_add(uint32_t (&d)[kMaxIsingClasses + 1], i
         if (k == c) d[k] += v;
 }
 
+template <int DM>
 __global__ __launch_bounds__(kBlock) void k_ising_sweep(IsingArgs a) {
     __shared__ int sStride[kMaxIsingClasses + 1];
     __shared__ uint32_t sThr[kIsingLdsTable];
@@ -321,6 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_ising_sweep(IsingArgs a) {
         tbase[q] = (r < a.R) ? a.slot_of[r] * a.table_per_slot : -1;
     }
     unsigned accepted[4] = {0, 0, 0, 0};
+    const uint32_t step = a.ctr[0] + a.step_off;
     const int s0 = a.c_begin + (chunk * kWavesPerBlock + w) * kSitesPerWave;
     for (int q = 0; q < kSitesPerWave; ++q) {
         const int slot = s0 + q;
@@ -334,19 +369,35 @@ __global__ __launch_bounds__(kBlock) void k_ising_sweep(IsingArgs a) {
             cnt[k] = 0;
         }
         const uint32_t* nb = a.ell + (size_t)slot * a.D;
-        for (int k = 0; k < a.D; ++k) {
-            const uint32_t e = nb[k];
-            const int c = (int)(e >> 24);
-            if (c == 0) continue;
-            const uint32_t v = *reinterpret_cast<const uint32_t*>(a.s + (size_t)(e & 0xffffffu) * a.Rp + r0);
-            class_add(d, c, (v >> 1) & 0x01010101u);   // 1 per byte where sigma = -1
+        if constexpr (DM > 0) {
+            // issue every neighbour load first (the row is padded to DM with class 0)
+            uint32_t nv[DM];
 #pragma unroll
-            for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
-                if (kk == c) cnt[kk] += 1;
+            for (int k = 0; k < DM; ++k)
+                nv[k] = *reinterpret_cast<const uint32_t*>(a.s + (size_t)(nb[k] & 0xffffffu) * a.Rp + r0);
+#pragma unroll
+            for (int k = 0; k < DM; ++k) {
+                const int c = (int)(nb[k] >> 24);
+                class_add(d, c, (nv[k] >> 1) & 0x01010101u);   // 1 per byte where sigma = -1
+#pragma unroll
+                for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
+                    if (kk == c) cnt[kk] += 1;
+            }
+        } else {
+            for (int k = 0; k < a.D; ++k) {
+                const uint32_t e = nb[k];
+                const int c = (int)(e >> 24);
+                if (c == 0) continue;
+                const uint32_t v = *reinterpret_cast<const uint32_t*>(a.s + (size_t)(e & 0xffffffu) * a.Rp + r0);
+                class_add(d, c, (v >> 1) & 0x01010101u);
+#pragma unroll
+                for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
+                    if (kk == c) cnt[kk] += 1;
+            }
         }
         uint32_t* own = reinterpret_cast<uint32_t*>(a.s + (size_t)slot * a.Rp + r0);
         const uint32_t v = *own;
-        const u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)rid, a.step, RNG_METROPOLIS, 0);
+        const u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)rid, step, RNG_METROPOLIS, 0);
         const uint32_t words[4] = {wr.x, wr.y, wr.z, wr.w};
         const int spc = a.sp[slot];
         uint32_t flip = 0;
@@ -534,15 +585,17 @@ __global__ void k_pt_swap(PtArgs a) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= a.n_groups) return;
     const int s0 = a.gstart[g], s1 = a.gstart[g + 1];
+    const uint32_t swap_index = a.ctr[1] + a.pt_off;
+    const int parity = a.parity >= 0 ? a.parity : (int)((a.ctr[2] + a.pt_off + 1u) & 1u);
     int nacc = 0;
-    for (int k = s0 + a.parity; k + 1 < s1; k += 2) {
+    for (int k = s0 + parity; k + 1 < s1; k += 2) {
         const int ra = a.rep_at[k], rb = a.rep_at[k + 1];
         const double b = a.beta_slot[k], bp = a.beta_slot[k + 1];
         const double E = a.E_pt[ra], Ep = a.E_pt[rb];
         const double delta = -(bp - b) * (E - Ep);
         bool accept = delta > 0.0;
         if (!accept) {
-            u32x4 w = rng_draw(a.seed, (uint32_t)k, (uint32_t)g, a.swap_index, RNG_PT, 0);
+            u32x4 w = rng_draw(a.seed, (uint32_t)k, (uint32_t)g, swap_index, RNG_PT, 0);
             const double u = (double)(w.x >> 8) * 0x1p-24;
             accept = u < exp(delta);
         }
@@ -664,32 +717,50 @@ __global__ void k_insert(float* plane, int8_t* s, int N, int Rp, int r0, int r1,
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-void launch_xyz_sweep(const XyzArgs& a, bool angle, bool zee, bool sia, hipStream_t st) {
-    dim3 grid(a.nblocks), block(kBlock);
-#define L(A, Z, S) hipLaunchKernelGGL((k_xyz_sweep<A, Z, S>), grid, block, 0, st, a)
-    if (angle) {
-        if (zee) { if (sia) L(true, true, true); else L(true, true, false); }
-        else { if (sia) L(true, false, true); else L(true, false, false); }
-    } else {
-        if (zee) { if (sia) L(false, true, true); else L(false, true, false); }
-        else { if (sia) L(false, false, true); else L(false, false, false); }
-    }
-#undef L
+// the ELL width buckets that have unrolled instantiations (0 = runtime loop)
+int ell_bucket(int D) {
+    static const int b[] = {4, 6, 8, 12, 18, 24, 32};
+    for (int x : b)
+        if (D <= x) return x;
+    return 0;
 }
 
-void launch_xyz_ovrr(const XyzArgs& a, bool angle, bool zee, hipStream_t st) {
-    dim3 grid(a.nblocks), block(kBlock);
-    if (angle) {
-        if (zee) hipLaunchKernelGGL((k_xyz_ovrr<true, true>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((k_xyz_ovrr<true, false>), grid, block, 0, st, a);
-    } else {
-        if (zee) hipLaunchKernelGGL((k_xyz_ovrr<false, true>), grid, block, 0, st, a);
-        else hipLaunchKernelGGL((k_xyz_ovrr<false, false>), grid, block, 0, st, a);
+#define ETHER_DISPATCH_DM(D, CALL)        \
+    switch (D) {                          \
+        case 4: CALL(4); break;           \
+        case 6: CALL(6); break;           \
+        case 8: CALL(8); break;           \
+        case 12: CALL(12); break;         \
+        case 18: CALL(18); break;         \
+        case 24: CALL(24); break;         \
+        case 32: CALL(32); break;         \
+        default: CALL(0); break;          \
     }
+
+void launch_xyz_sweep(const XyzArgs& a, bool angle, bool, bool, hipStream_t st) {
+    dim3 grid(a.nblocks), block(kBlock);
+#define LA(DM) hipLaunchKernelGGL((k_xyz_sweep<true, DM>), grid, block, 0, st, a)
+#define LM(DM) hipLaunchKernelGGL((k_xyz_sweep<false, DM>), grid, block, 0, st, a)
+    if (angle) { ETHER_DISPATCH_DM(a.D, LA) }
+    else { ETHER_DISPATCH_DM(a.D, LM) }
+#undef LA
+#undef LM
+}
+
+void launch_xyz_ovrr(const XyzArgs& a, bool angle, bool, hipStream_t st) {
+    dim3 grid(a.nblocks), block(kBlock);
+#define LA(DM) hipLaunchKernelGGL((k_xyz_ovrr<true, DM>), grid, block, 0, st, a)
+#define LM(DM) hipLaunchKernelGGL((k_xyz_ovrr<false, DM>), grid, block, 0, st, a)
+    if (angle) { ETHER_DISPATCH_DM(a.D, LA) }
+    else { ETHER_DISPATCH_DM(a.D, LM) }
+#undef LA
+#undef LM
 }
 
 void launch_ising_sweep(const IsingArgs& a, hipStream_t st) {
-    hipLaunchKernelGGL(k_ising_sweep, dim3(a.nblocks), dim3(kBlock), 0, st, a);
+#define LI(DM) hipLaunchKernelGGL((k_ising_sweep<DM>), dim3(a.nblocks), dim3(kBlock), 0, st, a)
+    ETHER_DISPATCH_DM(a.D, LI)
+#undef LI
 }
 
 void launch_measure(const MeasArgs& a, int model, bool zee, bool sia, hipStream_t st) {
@@ -726,6 +797,27 @@ void launch_local_fields(const MeasArgs& a, int r, float* h, float* e, int zee, 
                          int model, hipStream_t st) {
     hipLaunchKernelGGL(k_local_fields, dim3((a.N + kBlock - 1) / kBlock), dim3(kBlock), 0, st, a,
                        r, h, e, zee, sia, model);
+}
+
+__global__ void k_set_ctr(uint32_t* ctr, uint32_t step, uint32_t swaps, uint32_t swap_count) {
+    ctr[0] = step;
+    ctr[1] = swaps;
+    ctr[2] = swap_count;
+}
+
+__global__ void k_advance_ctr(uint32_t* ctr, uint32_t dstep, uint32_t dswaps) {
+    ctr[0] += dstep;
+    ctr[1] += dswaps;
+    ctr[2] += dswaps;
+}
+
+void launch_set_ctr(uint32_t* ctr, uint32_t step, uint32_t swaps, uint32_t swap_count,
+                    hipStream_t st) {
+    hipLaunchKernelGGL(k_set_ctr, dim3(1), dim3(1), 0, st, ctr, step, swaps, swap_count);
+}
+
+void launch_advance_ctr(uint32_t* ctr, uint32_t dstep, uint32_t dswaps, hipStream_t st) {
+    hipLaunchKernelGGL(k_advance_ctr, dim3(1), dim3(1), 0, st, ctr, dstep, dswaps);
 }
 
 void launch_extract(const float* plane, const int8_t* s, int N, int Rp, int r, float* out,

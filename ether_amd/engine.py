@@ -67,6 +67,7 @@ EXPORTS = [
     ("ether_pt_state", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     ("ether_run", C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                             C.c_int32, _vp]),
+    ("ether_set_graphs", C.c_int, [_vp, C.c_int32]),
     ("ether_set_timing", C.c_int, [_vp, C.c_int32]),
     ("ether_get_timing", C.c_int, [_vp, C.c_int32, _vp, _vp]),
     ("ether_synchronize", C.c_int, [_vp]),
@@ -285,6 +286,9 @@ class Engine:
                                    int(ovrr_interval), float(ovrr_passes), int(pt_interval),
                                    C.byref(sc)))
         return sc.value
+
+    def set_graphs(self, on: bool) -> None:
+        self._c(self.lib.ether_set_graphs(self.h, 1 if on else 0))
 
     def set_timing(self, on: bool) -> None:
         self._c(self.lib.ether_set_timing(self.h, 1 if on else 0))

@@ -210,6 +210,13 @@ int ether_run(ether_ctx* ctx, int32_t first_step, int32_t n_steps, int32_t mcs_e
               int32_t coa, int32_t ovrr_interval, double ovrr_passes,
               int32_t pt_interval, int32_t* swap_count);
 
+/* ether_run replays a captured HIP graph for every whole period
+ * P = lcm(coa, ovrr_interval, pt_interval) of the schedule that lies entirely
+ * in equilibration or entirely in production (P <= 1024); other steps are
+ * launched one by one.  Both paths give bit-identical results.  on = 0 turns
+ * graphs off (default on). */
+int ether_set_graphs(ether_ctx* ctx, int32_t on);
+
 /* Kernel timing with HIP events recorded on the context's stream around every
  * launch of a kind: 0 Metropolis colour phase, 1 overrelaxation colour phase,
  * 2 measurement (partials + finalize), 3 tempering swap.  Adds no

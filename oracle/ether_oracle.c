@@ -474,6 +474,46 @@ void o_overrelax_pass(const o_model* m, double* S, double* phi, const int32_t* o
 }
 
 /* ------------------------------------------------------------------------ */
+/* One chain through the MCS loop of src/ether.f90:159-203 (no tempering):   */
+/* OVRR when stepi%ovrr_iv==0 and stepi%coa!=0 (:163-169), sweep (:172),     */
+/* sample when stepi>eq and stepi%coa==0 (src/observable.f90:30-43).         */
+/* mom[8] += {1, e, e^2, e^4, m, m^2, m^4, accepted/N per sweep}.            */
+/* ------------------------------------------------------------------------ */
+void o_run_chain(const o_model* m, double* S, double* phi, const int32_t* order, int32_t n,
+                 const int32_t* rng_id, double beta, uint64_t seed, uint32_t replica,
+                 int32_t mcs, int32_t eq, int32_t coa, int32_t ovrr_iv, double ovrr_p,
+                 double* mom) {
+    const double N = (double)m->n_sites;
+    for (int32_t stepi = 1; stepi <= mcs; ++stepi) {
+        const uint32_t sweep = (uint32_t)(stepi - 1);
+        if (ovrr_iv > 0 && stepi % ovrr_iv == 0 && stepi % coa != 0) {
+            int full = (int)floor(ovrr_p);
+            double frac = ovrr_p - full;
+            int passes = full + (frac > 0.0 ? 1 : 0);
+            for (int p = 0; p < passes; ++p)
+                o_overrelax_pass(m, S, phi, order, n, rng_id, p < full ? 1.0 : frac, seed, replica,
+                                 sweep, (uint32_t)p);
+        }
+        int64_t acc = o_sweep(m, S, phi, order, n, rng_id, beta, seed, replica, sweep);
+        mom[7] += (double)acc / N;
+        if (stepi > eq && stepi % coa == 0) {
+            double parts[3], M[3];
+            o_total_energy(m, S, parts);
+            o_magnetisation(m, S, M);
+            double e = (parts[0] + parts[1] + parts[2]) / (2.0 * N);
+            double mm = sqrt(M[0] * M[0] + M[1] * M[1] + M[2] * M[2]) / N;
+            mom[0] += 1.0;
+            mom[1] += e;
+            mom[2] += e * e;
+            mom[3] += e * e * e * e;
+            mom[4] += mm;
+            mom[5] += mm * mm;
+            mom[6] += mm * mm * mm * mm;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Estimators                                                                */
 /* ------------------------------------------------------------------------ */
 void o_repeat_observables(const double* mom, double beta, int32_t n_sites, double* out6) {

@@ -131,6 +131,13 @@ void o_overrelax_pass(const o_model* m, double* S, double* phi, const int32_t* o
                       int32_t n, const int32_t* rng_id, double frac, uint64_t seed,
                       uint32_t replica, uint32_t step, uint32_t pass);
 
+/* One chain through the MCS loop of src/ether.f90:159-203 without tempering:
+ * OVRR gate, sweep, samples; mom[8] accumulates the device's moment record. */
+void o_run_chain(const o_model* m, double* S, double* phi, const int32_t* order, int32_t n,
+                 const int32_t* rng_id, double beta, uint64_t seed, uint32_t replica,
+                 int32_t mcs, int32_t eq, int32_t coa, int32_t ovrr_iv, double ovrr_p,
+                 double* mom);
+
 /* Estimators.  Per repeat (src/get_eng.f90:43-56, src/get_mag.f90:44-55):
  * mom = {n, sum e, e2, e4, m, m2, m4}; out = {e_avg, U_e, cv, m_avg, U_m, chi} */
 void o_repeat_observables(const double* mom, double beta, int32_t n_sites, double* out6);

@@ -72,6 +72,9 @@ def lib():
         L.o_reflect.argtypes = [C.POINTER(OModel), _vp, C.c_int32]
         L.o_overrelax_pass.argtypes = [C.POINTER(OModel), _vp, _vp, _vp, C.c_int32, _vp, C.c_double,
                                        C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.o_run_chain.argtypes = [C.POINTER(OModel), _vp, _vp, _vp, C.c_int32, _vp, C.c_double,
+                                  C.c_uint64, C.c_uint32, C.c_int32, C.c_int32, C.c_int32,
+                                  C.c_int32, C.c_double, _vp]
         L.o_repeat_observables.argtypes = [_vp, C.c_double, C.c_int32, _vp]
         L.o_over_repeats.argtypes = [_vp, C.c_int32, _vp, _vp]
         L.o_pt_accept.restype = C.c_int
@@ -239,6 +242,16 @@ class Oracle:
         r = _a(rng_id, np.int32)
         return lib().o_sweep(C.byref(self.m), _p(S), _p(phi), _p(o), o.size, _p(r), float(beta),
                              C.c_uint64(seed), replica, step)
+
+    def run_chain(self, S, phi, order, rng_id, beta, seed, replica, mcs, eq, coa, ovrr_iv=0,
+                  ovrr_p=0.0):
+        """One chain through the reference's MCS loop; returns the 8-moment record."""
+        o = _a(order, np.int32)
+        r = _a(rng_id, np.int32)
+        mom = np.zeros(8)
+        lib().o_run_chain(C.byref(self.m), _p(S), _p(phi), _p(o), o.size, _p(r), float(beta),
+                          C.c_uint64(seed), replica, mcs, eq, coa, ovrr_iv, float(ovrr_p), _p(mom))
+        return mom
 
     def reflect(self, S, i):
         lib().o_reflect(C.byref(self.m), _p(S), int(i))

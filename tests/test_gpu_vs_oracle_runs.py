@@ -1,0 +1,75 @@
+"""Distributional parity of whole runs: GPU against the CPU oracle.
+
+The GPU run goes through the batched driver: colour-ordered sweeps, every
+temperature x repeat chain on the device, and the schedule of src/ether.f90.
+The oracle runs each chain on its own, in the reference's typewriter order
+(src/Metropolis.f90:39) and with an independent seed.  Both start from the
+driver's start state.  Per temperature, the reference's estimators
+(e, U_e, C_v, m, U_m, chi with their SEM over repeats,
+src/observables_per_repeat.f90) must agree to z = |dx| / sqrt(sem_a^2 + sem_b^2):
+
+  - at most 15% of the comparisons may have z > 2 (about 5% is expected by
+    chance);
+  - none may have z > 4.5.
+
+The reference itself cannot be built here, so this is the
+2-sigma parity check of the sampled distribution, run against the oracle's
+restatement of the reference's chain.
+"""
+import numpy as np
+import pytest
+
+from ether_amd import driver, workloads
+from ether_amd.inputs import fortran_nint
+from oracle import oracle as o
+from tests.helpers import say
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_run(m, seed):
+    t = m.tags
+    temps = t.temperatures()
+    betas = m.betas(temps)
+    orc = o.Oracle.from_model(m)
+    n = m.n_sites
+    order = np.arange(n, dtype=np.int32)
+    coa = fortran_nint(t.coa)
+    ovrr_on = t.ovrr and t.xyz and not t.zeeman and not t.sia
+    rng = np.random.default_rng(seed)
+    per = np.zeros((t.repeat, len(temps), 6))
+    for k in range(t.repeat):
+        for j, b in enumerate(betas):
+            if t.ising:
+                S = np.ascontiguousarray(-m.initial_spins)       # fresh_spins flips the basis state
+                phi = None
+            else:
+                v = rng.normal(size=(n, 3))
+                S = np.ascontiguousarray(v / np.linalg.norm(v, axis=1)[:, None])
+                phi = np.ascontiguousarray(np.arctan2(S[:, 1], S[:, 0])) if t.angle else None
+            mom = orc.run_chain(S, phi, order, order, b, seed, k * len(temps) + j, t.tmcs,
+                                t.tmcs_eq, coa, t.ovrr_mcs if ovrr_on else 0,
+                                t.ovrr_para if ovrr_on else 0.0)
+            per[k, j] = driver.repeat_observables(mom, b, n)
+    mean = per.mean(axis=0)
+    sem = per.std(axis=0, ddof=1) / np.sqrt(t.repeat)
+    return mean, sem
+
+
+@pytest.mark.parametrize("name,over", [
+    ("sc_ising_16", dict(sc="6 6 6", mcs="1500 500", repeat="8", temp="100 10 10")),
+    ("sc_heis_afm_32", dict(sc="6 6 6", mcs="1000 400", repeat="8", temp="60 6 6")),
+    ("tri_afm_96", dict(sc="6 6 1", mcs="1000 400", repeat="8", temp="20 2 2")),
+])
+def test_observables_agree_within_two_sigma(name, over):
+    say(f"runs {name}")
+    m = workloads.model(name, **over)
+    res = driver.run(m)
+    mean_o, sem_o = _oracle_run(m, 777)
+    z = np.abs(res.mean - mean_o) / np.sqrt(res.sem ** 2 + sem_o ** 2 + 1e-300)
+    # U is undefined where the order parameter vanishes identically; skip NaN
+    z = z[np.isfinite(z)]
+    frac = float(np.mean(z > 2.0))
+    say(f"{name}: {z.size} comparisons, {frac:.3f} beyond 2 sigma, max z {z.max():.2f}")
+    assert frac <= 0.15
+    assert z.max() < 4.5

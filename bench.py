@@ -196,19 +196,22 @@ def pt_comparison(mcs):
     from ether_amd import ladder, workloads
     m = workloads.model("pyro_pt_16")
     t0 = time.perf_counter()
-    lin, lrn, loss = ladder.compare_ladders(m, train_mcs=2000, mcs=mcs)
+    lin, lrn = ladder.compare_ladders(m, train_mcs=2000, mcs=mcs)
     total = time.perf_counter() - t0
 
     def rep(run):
-        return {"round_trips": run.round_trips, "wall_s": run.wall_s,
+        return {"round_trips": run.round_trips, "wall_s": run.wall_s, "mcs": run.mcs,
                 "round_trips_per_s": run.round_trips_per_s,
                 "spin_updates_per_s": run.updates_per_s,
                 "swap_acc_min": float(run.acc_rate.min()), "swap_acc_max": float(run.acc_rate.max()),
+                "swap_acc_cv": float(run.acc_rate.std() / max(run.acc_rate.mean(), 1e-30)),
+                "swap_acc": [round(float(x), 4) for x in run.acc_rate],
                 "temps": [round(float(x), 6) for x in run.temps]}
 
     return {"workload": f"pyro_pt_16: pyrochlore 16^3x16 ({m.n_sites} spins), J1=1 J2=0.05 meV, "
-                        f"64 T 1->0.05 K x 4 repeats = 256 chains, swaps every 11 MCS, MCS {mcs}",
-            "linear": rep(lin), "learned": rep(lrn), "mlp_final_loss": loss,
+                        f"64 T 1->0.05 K x 4 repeats = 256 chains, swaps every 11 MCS, MCS {mcs}; "
+                        f"first 2000 MCS on the linear ladder, then the ladder under test",
+            "linear": rep(lin), "learned": rep(lrn), "mlp_final_loss": lrn.mlp_loss,
             "gain_round_trips_per_s": (lrn.round_trips_per_s / lin.round_trips_per_s
                                        if lin.round_trips_per_s > 0 else None),
             "total_s": total}

@@ -685,23 +685,35 @@ static int upload_ising_thresholds(ether_ctx* c) {
     return ETHER_OK;
 }
 
-extern "C" int ether_set_betas(ether_ctx* c, const double* beta) {
-    TRY(need_committed(c));
+static int upload_betas(ether_ctx* c, const double* beta) {
     if (!beta) return fail(c, ETHER_ERR_ARG, "ether_set_betas: null pointer");
-    c->beta_h.assign(beta, beta + c->R);
     std::vector<float> bf(c->R);
     for (int r = 0; r < c->R; ++r) {
         if (!(beta[r] > 0.0) || !std::isfinite(beta[r]))
             return fail(c, ETHER_ERR_ARG, "ether_set_betas: beta must be positive and finite");
         bf[r] = (float)beta[r];
     }
+    c->beta_h.assign(beta, beta + c->R);
     TRY(upload(c, c->d_beta, beta, (size_t)c->R));
     TRY(upload(c, c->d_beta_f, bf.data(), (size_t)c->R));
+    if (c->model == ETHER_MODEL_ISING) TRY(upload_ising_thresholds(c));
+    return ETHER_OK;
+}
+
+extern "C" int ether_retune_betas(ether_ctx* c, const double* beta) {
+    TRY(need_betas(c));
+    TRY(upload_betas(c, beta));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ETHER_OK;
+}
+
+extern "C" int ether_set_betas(ether_ctx* c, const double* beta) {
+    TRY(need_committed(c));
+    TRY(upload_betas(c, beta));
     std::vector<int> ident(c->R);
     for (int r = 0; r < c->R; ++r) ident[r] = r;
     TRY(upload(c, c->d_slot_of, ident.data(), (size_t)c->R));
     TRY(upload(c, c->d_rep_at, ident.data(), (size_t)c->R));
-    if (c->model == ETHER_MODEL_ISING) TRY(upload_ising_thresholds(c));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->betas_set = true;
     return ETHER_OK;

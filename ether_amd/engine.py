@@ -55,6 +55,7 @@ EXPORTS = [
     ("ether_commit", C.c_int, [_vp]),
     ("ether_get_layout", C.c_int, [_vp, _vp, _vp, _vp]),
     ("ether_set_betas", C.c_int, [_vp, _vp]),
+    ("ether_retune_betas", C.c_int, [_vp, _vp]),
     ("ether_set_ladders", C.c_int, [_vp, C.c_int32, _vp]),
     ("ether_init_spins", C.c_int, [_vp, C.c_int32]),
     ("ether_sweep", C.c_int, [_vp, C.c_int32, _vp]),
@@ -221,6 +222,11 @@ class Engine:
         b = _arr(beta, np.float64)
         assert b.size == self.R
         self._c(self.lib.ether_set_betas(self.h, _ptr(b)))
+
+    def retune_betas(self, beta: Sequence[float]) -> None:
+        b = _arr(beta, np.float64)
+        assert b.size == self.R
+        self._c(self.lib.ether_retune_betas(self.h, _ptr(b)))
 
     def set_ladders(self, group_start: Sequence[int]) -> None:
         g = _arr(group_start, np.int32)

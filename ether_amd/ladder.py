@@ -1,17 +1,31 @@
 """Learned parallel-tempering ladder.
 
-The reference's ladder is linear in T (src/ether.f90:79-84).  Here a small MLP
-surrogate of <E>(T) and log C_v(T), trained by the library's own HIP kernels
-on measurements taken with the linear ladder, places the temperatures so that
-the thermodynamic length  int sqrt(C_v)/T dT  is equal between neighbours.
-For Gaussian energy distributions the swap acceptance between neighbours is
-erfc(dT sqrt(C_v) / (2T)), so equal length means uniform acceptance, which
-minimises the round-trip time of the deterministic even/odd swap schedule
-the reference uses (swap_count parity, src/parallel_tempering.f90:50-62).
+The reference's ladder is linear in T (src/ether.f90:79-84); an adaptive
+ladder is only a commented stub (src/startup.f90:115-122).  Here a small MLP
+surrogate of <e>(T) and log C(T), trained by the library's own HIP kernels on
+measurements from the first MCS of the run, places the temperatures so that
+the thermodynamic length  int sqrt(C)/T dT  is equal between neighbours.  For
+Gaussian energy distributions the swap acceptance of a neighbour pair is
+erfc(dT sqrt(C)/(2T)), so equal length means uniform acceptance.  Uniform
+acceptance minimises the round-trip time of the deterministic even/odd swap
+schedule that the reference uses (swap_count parity,
+src/parallel_tempering.f90:50-62).
 
-The ladder only chooses temperatures: every swap still passes the exact
-acceptance test, so the sampled distribution at each temperature is the same
-as with the linear ladder.
+Which C.  The fluctuation estimator beta^2 (<e^2> - <e>^2) N over a short
+window is biased by relaxation drift wherever the chains are still far from
+equilibrium (at T/J << 1 uniform Marsaglia proposals are almost never
+accepted).  The C that actually sets the swap acceptance is the one that
+reproduces the measured acceptance a_k of each neighbour pair through the same
+Gaussian formula:  sqrt(C(T_mid)) = 2 erfcinv(a_k) T_mid / dT_k.  The
+surrogate is trained on that acceptance-consistent C (and on <e>), measured on
+the linear ladder during the first train_mcs MCS of the run.
+
+Protocol (adaptive_run): the first train_mcs MCS of the equilibration run on
+the linear ladder, then the temperatures of the slots are retuned in place
+(every replica keeps its state and slot) and the remaining MCS run on the new
+ladder.  Every swap passes the exact acceptance test throughout, so the
+sampled distribution at each temperature is unchanged; only the production
+window after the switch is timed and its round trips counted.
 """
 from __future__ import annotations
 
@@ -20,6 +34,7 @@ from dataclasses import dataclass
 from typing import Optional
 
 import numpy as np
+from scipy.special import erfcinv
 
 from . import engine as E
 from .driver import make_engine
@@ -28,12 +43,16 @@ from .inputs import Model, fortran_nint
 
 @dataclass
 class LadderRun:
-    temps: np.ndarray          # (n,) hottest first
-    wall_s: float              # wall time of the production MCS loop
-    mcs: int
+    temps: np.ndarray          # (n,) production ladder, hottest first
+    wall_s: float              # wall time of the production window
+    mcs: int                   # MCS in the production window
     round_trips: int           # completed cold->hot->cold trips, all replicas
     acc_rate: np.ndarray       # (n-1,) swap acceptance between slot k and k+1
     updates: float
+    train_e: Optional[np.ndarray] = None
+    train_c: Optional[np.ndarray] = None
+    train_acc: Optional[np.ndarray] = None
+    mlp_loss: Optional[float] = None
 
     @property
     def round_trips_per_s(self) -> float:
@@ -44,80 +63,97 @@ class LadderRun:
         return self.updates / self.wall_s if self.wall_s > 0 else 0.0
 
 
-def _ladder_betas(model: Model, temps, repeat):
-    return np.tile(model.betas(temps), repeat)
+def _pair_acceptance(eng: E.Engine, n: int, repeat: int):
+    _, acc, att, trips = eng.pt_state()
+    a = np.zeros(n - 1)
+    tot = np.zeros(n - 1)
+    for k in range(n - 1):
+        num = sum(int(acc[g * n + k]) for g in range(repeat))
+        den = sum(int(att[g * n + k]) for g in range(repeat))
+        a[k] = num / den if den else 0.0
+        tot[k] = den
+    return a, tot, int(trips.sum())
 
 
-def measure_on_ladder(eng: E.Engine, model: Model, temps, repeat: int, mcs: int, coa: int,
-                      pt_interval: int, eq_frac: float = 0.5):
-    """Run tempering on `temps` for `mcs` MCS and return per-temperature
-    <e> and C_v (the reference's estimators, averaged over repeats)."""
-    n = len(temps)
-    eng.set_betas(_ladder_betas(model, temps, repeat))
-    eng.set_ladders([k * n for k in range(repeat + 1)])
-    eng.init_spins(E.INIT_RANDOM)
-    eng.reset_moments()
-    eq = int(mcs * eq_frac)
-    eng.run(1, mcs, eq, coa, 0, 0.0, pt_interval)
-    mom = eng.moments()
-    beta = model.betas(temps)
-    e = np.zeros(n)
-    cv = np.zeros(n)
-    for k in range(repeat):
-        for j in range(n):
-            m = mom[k * n + j]
-            cnt = m[E.MOM_N]
-            ea, e2 = m[E.MOM_E] / cnt, m[E.MOM_E2] / cnt
-            e[j] += ea / repeat
-            cv[j] += beta[j] ** 2 * (e2 - ea * ea) * model.n_sites / repeat
-    return e, cv
+def acceptance_heat_capacity(temps, acc, attempts):
+    """Heat capacity that reproduces the measured pair acceptance through
+    a = erfc(dT sqrt(C) / (2 T)), at the pair midpoints, interpolated to temps."""
+    temps = np.asarray(temps, np.float64)
+    a = np.clip(np.asarray(acc, np.float64), 0.5 / np.maximum(attempts, 1.0), 0.999)
+    x = 2.0 * erfcinv(a)
+    tm = 0.5 * (temps[1:] + temps[:-1])
+    dt = np.abs(np.diff(temps))
+    c = (x * tm / dt) ** 2
+    order = np.argsort(tm)
+    return np.interp(temps, tm[order], c[order])
 
 
-def learn_ladder(eng: E.Engine, temps, e, cv, n_out: Optional[int] = None, iters: int = 4000,
+def learn_ladder(eng: E.Engine, temps, e, c, n_out: Optional[int] = None, iters: int = 4000,
                  lr: float = 3e-3, seed: int = 1992):
-    """Train the device MLP on (T, e, log C_v) and place a new ladder."""
+    """Train the device MLP on (T, e, log C) and place a new ladder."""
     temps = np.asarray(temps, np.float64)
     n_out = n_out or len(temps)
     t_lo, t_hi = float(temps.min()), float(temps.max())
     mid, half = 0.5 * (t_lo + t_hi), 0.5 * (t_hi - t_lo)
     x = ((temps - mid) / half).astype(np.float32)
-    lcv = np.log(np.maximum(cv, 1e-12))
-    y = np.stack([(e - e.mean()) / (e.std() + 1e-30), (lcv - lcv.mean()) / (lcv.std() + 1e-30)],
+    lc = np.log(np.maximum(c, 1e-12))
+    y = np.stack([(e - e.mean()) / (e.std() + 1e-30), (lc - lc.mean()) / (lc.std() + 1e-30)],
                  axis=1).astype(np.float32)
     w = np.ones_like(y)
     eng.mlp_init(seed)
     loss = eng.mlp_train(x, y, w, iters, lr)
-    T = eng.ladder_place(mid, half, float(lcv.std() + 1e-30), t_lo, t_hi, n_out)
+    T = eng.ladder_place(mid, half, float(lc.std() + 1e-30), t_lo, t_hi, n_out)
     return T, loss
 
 
-def run_ladder(eng: E.Engine, model: Model, temps, repeat: int, mcs: int, coa: int,
-               pt_interval: int) -> LadderRun:
-    """Production tempering run on a given ladder; counts round trips."""
+def adaptive_run(eng: E.Engine, model: Model, temps, repeat: int, mcs: int, coa: int,
+                 pt_interval: int, *, learn: bool, train_mcs: int = 2000) -> LadderRun:
+    """Tempering run of mcs MCS: train_mcs on `temps`, then (learn=True) the
+    learned ladder or (learn=False) the same ladder for the remaining MCS."""
+    temps = np.asarray(temps, np.float64)
     n = len(temps)
-    eng.set_betas(_ladder_betas(model, temps, repeat))
-    eng.set_ladders([k * n for k in range(repeat + 1)])
+    groups = [k * n for k in range(repeat + 1)]
+    eq = model.tags.tmcs_eq
+    eng.set_betas(np.tile(model.betas(temps), repeat))
+    eng.set_ladders(groups)
     eng.init_spins(E.INIT_RANDOM)
+    eng.reset_moments()
+    half = train_mcs // 2
+    sc = eng.run(1, half, eq, coa, 0, 0.0, pt_interval, 0)
+    # measurement window: swap statistics and energy samples of the second half
+    eng.set_ladders(groups)
+    eng.reset_moments()
+    sc = eng.run(half + 1, train_mcs - half, half, coa, 0, 0.0, pt_interval, sc)
+    mom = eng.moments()
+    a, att, _ = _pair_acceptance(eng, n, repeat)
+    e = np.zeros(n)
+    for g in range(repeat):
+        for k in range(n):
+            m = mom[g * n + k]
+            e[k] += m[E.MOM_E] / max(m[E.MOM_N], 1.0) / repeat
+    c = acceptance_heat_capacity(temps, a, att)
+    run_temps, loss = temps, None
+    if learn:
+        run_temps, loss = learn_ladder(eng, temps, e, c, n_out=n)
+        eng.retune_betas(np.tile(model.betas(run_temps), repeat))
+    # production window
+    eng.set_ladders(groups)
     eng.reset_moments()
     eng.synchronize()
     t0 = time.perf_counter()
-    eng.run(1, mcs, mcs // 4, coa, 0, 0.0, pt_interval)
+    rest = mcs - train_mcs
+    sc = eng.run(train_mcs + 1, rest, eq, coa, 0, 0.0, pt_interval, sc)
     eng.synchronize()
     wall = time.perf_counter() - t0
-    slot_of, acc, att, trips = eng.pt_state()
-    rate = np.zeros(n - 1)
-    for k in range(n - 1):
-        a = sum(acc[g * n + k] for g in range(repeat))
-        b = sum(att[g * n + k] for g in range(repeat))
-        rate[k] = a / b if b else 0.0
-    return LadderRun(np.asarray(temps), wall, mcs, int(trips.sum()), rate,
-                     float(eng.R) * model.n_sites * mcs)
+    acc, _, trips = _pair_acceptance(eng, n, repeat)
+    return LadderRun(run_temps, wall, rest, trips, acc, float(eng.R) * model.n_sites * rest,
+                     train_e=e, train_c=c, train_acc=a, mlp_loss=loss)
 
 
 def compare_ladders(model: Model, *, train_mcs: int = 2000, mcs: Optional[int] = None,
-                    repeat: Optional[int] = None, iters: int = 4000):
-    """Linear ladder versus the learned ladder on the same model (the tempering
-    workload of the benchmark).  Returns (linear LadderRun, learned LadderRun, loss)."""
+                    repeat: Optional[int] = None):
+    """The linear ladder against the learned ladder on the same model and seed.
+    Returns (linear LadderRun, learned LadderRun)."""
     t = model.tags
     temps = t.temperatures()
     rep = t.repeat if repeat is None else repeat
@@ -125,10 +161,10 @@ def compare_ladders(model: Model, *, train_mcs: int = 2000, mcs: Optional[int] =
     coa = fortran_nint(t.coa)
     eng = make_engine(model, len(temps) * rep)
     try:
-        lin = run_ladder(eng, model, temps, rep, mcs, coa, t.exchange_interval)
-        e, cv = measure_on_ladder(eng, model, temps, rep, train_mcs, coa, t.exchange_interval)
-        T_new, loss = learn_ladder(eng, temps, e, cv)
-        learned = run_ladder(eng, model, T_new, rep, mcs, coa, t.exchange_interval)
+        lin = adaptive_run(eng, model, temps, rep, mcs, coa, t.exchange_interval, learn=False,
+                           train_mcs=train_mcs)
+        lrn = adaptive_run(eng, model, temps, rep, mcs, coa, t.exchange_interval, learn=True,
+                           train_mcs=train_mcs)
     finally:
         eng.close()
-    return lin, learned, loss
+    return lin, lrn

@@ -149,6 +149,10 @@ int ether_get_layout(ether_ctx* ctx, int32_t* n_colours, int32_t* colour, int32_
  * the identity. */
 int ether_set_betas(ether_ctx* ctx, const double* beta);
 
+/* Replaces the temperature of every slot but keeps which replica sits in
+ * which slot (slot_of), e.g. to move an adaptive ladder during equilibration. */
+int ether_retune_betas(ether_ctx* ctx, const double* beta);
+
 /* Tempering groups: slots [group_start[g], group_start[g+1]) form one ladder,
  * ordered as the reference orders its ranks (hottest first, src/ether.f90:79-84).
  * n_groups = 0 disables tempering. group_start has n_groups+1 entries. */

@@ -63,6 +63,19 @@ def cpu_baseline(model, budget_s=10.0):
                       f"Heisenberg chain at T=20 K, FP64 C oracle, single thread, {dt:.1f} s"}
 
 
+def profiled_traffic(workload):
+    """HBM bytes per launch of the sweep kernel from the committed PMC summary
+    (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, factors from a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", f"k_xyz_sweep_{workload}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch"):
+                return float(line.split("=")[1])
+    return None
+
+
 def copy_probe(n_bytes, reps):
     """A plain device-to-device copy of known size (the HIP runtime's copy
     kernel), used to calibrate the FETCH_SIZE / WRITE_SIZE counters."""
@@ -92,6 +105,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pt", action="store_true", help="skip the tempering ladder comparison")
     ap.add_argument("--pt-mcs", type=int, default=20000)
+    ap.add_argument("--split-ovrr", action="store_true",
+                    help="the reference's order: a whole OVRR pass, then the sweep")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe, 4)
@@ -112,7 +127,7 @@ def main():
     t = m.tags
     temps = t.temperatures()
     R = len(temps) * t.repeat
-    eng = make_engine(m, R)
+    eng = make_engine(m, R, compat=E.COMPAT_SPLIT_OVRR if args.split_ovrr else 0)
     eng.set_betas(np.tile(m.betas(temps), t.repeat))
     coa = fortran_nint(t.coa)
     ovrr_on = t.ovrr and t.xyz and not t.zeeman and not t.sia
@@ -163,12 +178,14 @@ def main():
         "config": {
             "workload": f"{args.workload}: Heisenberg AFM simple cubic {m.sc[0]}x{m.sc[1]}x{m.sc[2]} "
                         f"({m.n_sites} spins), J=1 meV, {len(temps)} T x {t.repeat} repeats = {R} "
-                        f"chains, Marsaglia moves, OVRR every MCS, MCS {args.steps}/{t.tmcs_eq}, coa {coa}",
+                        f"chains, Marsaglia moves, OVRR every MCS "
+                        f"({'split pass' if args.split_ovrr else 'fused into the sweep phases'}), "
+                        f"MCS {args.steps}/{t.tmcs_eq}, coa {coa}",
             "replicas": R, "sites": m.n_sites, "colours": nc, "mcs": args.steps,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args.workload),
             "kernel": "k_xyz_sweep (one colour phase)",
             "bytes_per_update": B_UPD[m.model_id], "updates_per_launch": per_launch_updates,
             "avg_launch_ms": avg_ms, "launches": sweep_n,
@@ -180,21 +197,27 @@ def main():
     eng.close()
     if not args.no_pt:
         out["pt"] = pt_comparison(args.pt_mcs)
+        # At 65536 spins the equilibrium acceptance of a 64-temperature ladder on
+        # 1 -> 0.05 K is erfc(sqrt(C) dlnT / 2) with C ~ N: ~1e-17 even for the
+        # best ladder, so no label completes a round trip there.  The same
+        # schedule on 4^3 cells (1024 spins) is where round trips can be counted.
+        out["pt_1024"] = pt_comparison(args.pt_mcs, cells=4)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, args.cpu_budget)
     print(json.dumps(out), flush=True)
 
 
-def pt_comparison(mcs):
+def pt_comparison(mcs, cells=16):
     """Pyrochlore tempering: linear ladder versus the learned ladder.
 
-    pyro_pt_16: 16^3 cubic cells x 16 sites (65536 spins), J1 = 1, J2 = 0.05 meV,
-    64 temperatures 1 -> 0.05 K, swaps every 11 MCS, REPEAT 4 (256 chains); the
-    MLP is trained on <e> and C_v measured in the first 2000 MCS on the linear
-    ladder.  Reports completed cold->hot->cold round trips per second of the
-    production loop and the spin updates per second of the same loop."""
+    pyro_pt_16: cells^3 cubic cells x 16 sites (65536 spins at 16^3), J1 = 1,
+    J2 = 0.05 meV, 64 temperatures 1 -> 0.05 K, swaps every 11 MCS, REPEAT 4
+    (256 chains); the MLP is trained on <e> and the acceptance-consistent C
+    measured in the first 2000 MCS (four windows, retuned after each).
+    Reports completed cold->hot->cold round trips per second of the production
+    loop and the spin updates per second of the same loop."""
     from ether_amd import ladder, workloads
-    m = workloads.model("pyro_pt_16")
+    m = workloads.model("pyro_pt_16", sc=f"{cells} {cells} {cells}")
     t0 = time.perf_counter()
     lin, lrn = ladder.compare_ladders(m, train_mcs=2000, mcs=mcs)
     total = time.perf_counter() - t0
@@ -208,9 +231,9 @@ def pt_comparison(mcs):
                 "swap_acc": [round(float(x), 4) for x in run.acc_rate],
                 "temps": [round(float(x), 6) for x in run.temps]}
 
-    return {"workload": f"pyro_pt_16: pyrochlore 16^3x16 ({m.n_sites} spins), J1=1 J2=0.05 meV, "
+    return {"workload": f"pyrochlore {cells}^3x16 ({m.n_sites} spins), J1=1 J2=0.05 meV, "
                         f"64 T 1->0.05 K x 4 repeats = 256 chains, swaps every 11 MCS, MCS {mcs}; "
-                        f"first 2000 MCS on the linear ladder, then the ladder under test",
+                        f"2000 training MCS, then the ladder under test",
             "linear": rep(lin), "learned": rep(lrn), "mlp_final_loss": lrn.mlp_loss,
             "gain_round_trips_per_s": (lrn.round_trips_per_s / lin.round_trips_per_s
                                        if lin.round_trips_per_s > 0 else None),

@@ -777,6 +777,7 @@ static XyzArgs xyz_args(ether_ctx* c, int colour) {
     a.step_off = c->step - (c->capturing ? c->cap_step0 : c->dev_step);
     a.pass = 0;
     a.frac = 1.0f;
+    a.fuse = 0;
     a.acc = c->d_acc;
     a.n_classes = c->n_classes;
     a.jclass = c->d_jclass;
@@ -903,12 +904,22 @@ extern "C" int ether_synchronize(ether_ctx* c) {
 // --------------------------------------------------------------------------
 // hot path
 // --------------------------------------------------------------------------
-static int enqueue_sweep(ether_ctx* c) {
+// One sweep: a Metropolis phase per colour.  fuse_frac > 0 (XYZ only) makes
+// every phase first reflect each of its sites with probability fuse_frac.
+static int enqueue_sweep(ether_ctx* c, double fuse_frac = 0.0) {
     for (int q = 0; q < c->ncol; ++q) {
         if (c->cbeg[q + 1] == c->cbeg[q]) continue;
         const int ev = ev_begin(c, ETHER_TIME_SWEEP);
-        if (c->model == ETHER_MODEL_ISING) launch_ising_sweep(ising_args(c, q), c->stream);
-        else launch_xyz_sweep(xyz_args(c, q), c->angle, c->zee, c->sia, c->stream);
+        if (c->model == ETHER_MODEL_ISING) {
+            launch_ising_sweep(ising_args(c, q), c->stream);
+        } else {
+            XyzArgs a = xyz_args(c, q);
+            if (fuse_frac > 0.0) {
+                a.fuse = 1;
+                a.frac = (float)fuse_frac;
+            }
+            launch_xyz_sweep(a, c->angle, c->zee, c->sia, c->stream);
+        }
         ev_end(c, ev);
     }
     HIPCHK(c, hipGetLastError());
@@ -1139,10 +1150,13 @@ extern "C" int ether_pt_state(ether_ctx* c, int32_t* slot_of, int64_t* acc, int6
 // One MCS of the schedule of src/ether.f90:159-203.
 static int enqueue_step(ether_ctx* c, int32_t stepi, int32_t mcs_eq, int32_t coa, int ovrr_iv,
                         double ovrr_passes, int pt_iv, int* sc) {
-    // src/ether.f90:163-169
-    if (ovrr_iv && stepi % ovrr_iv == 0 && stepi % coa != 0) TRY(enqueue_ovrr(c, ovrr_passes));
-    // src/ether.f90:172
-    TRY(enqueue_sweep(c));
+    // src/ether.f90:163-169, then the sweep of :172.  A pass of at most one
+    // reflection per site is fused into the sweep's colour phases unless the
+    // split order is asked for.
+    const bool ovrr = ovrr_iv && stepi % ovrr_iv == 0 && stepi % coa != 0;
+    const bool fuse = ovrr && ovrr_passes <= 1.0 && !(c->compat & ETHER_COMPAT_SPLIT_OVRR);
+    if (ovrr && !fuse) TRY(enqueue_ovrr(c, ovrr_passes));
+    TRY(enqueue_sweep(c, fuse ? ovrr_passes : 0.0));
     // src/ether.f90:189-198
     if (pt_iv && stepi % pt_iv == 0 && stepi % coa != 0) {
         ++*sc;

@@ -55,6 +55,7 @@ struct XyzArgs {
     uint32_t step_off;     // sweep counter = ctr[0] + step_off
     uint32_t pass;
     float frac;
+    int fuse;              // sweep: reflect (with probability frac) before the proposal
     unsigned long long* acc;
     int n_classes;
     const float* jclass;

@@ -204,8 +204,27 @@ __device__ __forceinline__ void local_field(const float* __restrict__ sx,
     }
 }
 
+// Reflection of S about the local field A: S' = 2(A.S)/(A.A) A - S, then
+// renormalised (src/overrelaxation.f90:44-49).  Returns false (S unchanged)
+// where A vanishes.
+__device__ __forceinline__ bool reflect_about(float hx, float hy, float hz, float& x, float& y,
+                                              float& z) {
+    const float aa = hx * hx + hy * hy + hz * hz;
+    if (!(aa > 0.f)) return false;
+    const float f = 2.0f * (hx * x + hy * y + hz * z) / aa;
+    x = f * hx - x;
+    y = f * hy - y;
+    z = f * hz - z;
+    normalise3(x, y, z);
+    return true;
+}
+
 // ---------------------------------------------------------------------------
-// XYZ Metropolis, one colour phase
+// XYZ Metropolis, one colour phase.  With a.fuse set the phase first reflects
+// each site about its local field (selected with probability a.frac, the OVRR
+// draw of the split pass) and proposes from the reflected spin: one pass over
+// the spins instead of an OVRR phase and a sweep phase.  Within a phase the
+// neighbours are fixed, so both moves use the same field.
 // ---------------------------------------------------------------------------
 template <bool ANGLE, int DM>
 __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
@@ -236,7 +255,8 @@ __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
         const int slot = s0 + q;                       // wave-uniform
         if (slot >= a.c_end) break;
         const size_t row = (size_t)slot * a.Rp + r;
-        const float x = a.sx[row], y = a.sy[row], z = a.sz[row];
+        float x = a.sx[row], y = a.sy[row], z = a.sz[row];
+        float cphi = ANGLE ? a.phi[row] : 0.f;
         float hx, hy, hz;
         local_field<DM>(a.sx, a.sy, a.sz, a.ell + (size_t)slot * a.D, a.D, a.Rp, r, sJ, hx, hy, hz);
         const int sp = (ZEE || SIA) ? a.sp[slot] : 0;
@@ -245,11 +265,23 @@ __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
             hy += sZ[3 * sp + 1];
             hz += sZ[3 * sp + 2];
         }
+        bool reflected = false;
+        if (a.fuse) {                                  // wave-uniform
+            bool sel = true;
+            if (a.frac < 1.0f) {
+                const u32x4 wo = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, step, RNG_OVRR, a.pass);
+                sel = (float)(wo.x >> 8) * 0x1p-24f < a.frac;
+            }
+            if (sel) {
+                reflected = reflect_about(hx, hy, hz, x, y, z);
+                if (ANGLE && reflected) cphi = atan2f(y, x);
+            }
+        }
         const u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, step, RNG_METROPOLIS, 0);
         const float eta = eta24f(wr.z);
         float nx = 0.f, ny = 0.f, nz = 1.f, nphi = 0.f;
         if (ANGLE) {
-            angle_move(wr, a.dphi, a.phi[row], nx, ny, nz, nphi);
+            angle_move(wr, a.dphi, cphi, nx, ny, nz, nphi);
         } else {
             marsaglia_pooled(wr, rv, a.seed, (uint32_t)slot, (uint32_t)r, step, pool, nx, ny, nz);
         }
@@ -265,6 +297,11 @@ __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
             a.sz[row] = nz;
             if (ANGLE) a.phi[row] = nphi;
             ++accepted;
+        } else if (rv && reflected) {
+            a.sx[row] = x;
+            a.sy[row] = y;
+            a.sz[row] = z;
+            if (ANGLE) a.phi[row] = cphi;
         }
     }
     // acceptance is booked to the temperature slot, as the reference books it per rank
@@ -306,12 +343,8 @@ __global__ __launch_bounds__(kBlock) void k_xyz_ovrr(XyzArgs a) {
             hy += sZ[3 * sp + 1];
             hz += sZ[3 * sp + 2];
         }
-        const float aa = hx * hx + hy * hy + hz * hz;
-        if (!(aa > 0.f)) continue;
-        const float x = a.sx[row], y = a.sy[row], z = a.sz[row];
-        const float f = 2.0f * (hx * x + hy * y + hz * z) / aa;
-        float nx = f * hx - x, ny = f * hy - y, nz = f * hz - z;
-        normalise3(nx, ny, nz);
+        float nx = a.sx[row], ny = a.sy[row], nz = a.sz[row];
+        if (!reflect_about(hx, hy, hz, nx, ny, nz)) continue;
         a.sx[row] = nx;
         a.sy[row] = ny;
         a.sz[row] = nz;

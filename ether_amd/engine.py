@@ -25,6 +25,7 @@ INIT_FRESH = 1
 INIT_RANDOM = 2
 TIME_SWEEP, TIME_OVRR, TIME_MEASURE, TIME_PT = 0, 1, 2, 3
 COMPAT_PT_ETOT = 1
+COMPAT_SPLIT_OVRR = 2
 
 MOM_N, MOM_E, MOM_E2, MOM_E4, MOM_M, MOM_M2, MOM_M4, MOM_ACC = range(8)
 MOM_COUNT = 8

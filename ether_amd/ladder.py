@@ -20,12 +20,15 @@ Gaussian formula:  sqrt(C(T_mid)) = 2 erfcinv(a_k) T_mid / dT_k.  The
 surrogate is trained on that acceptance-consistent C (and on <e>), measured on
 the linear ladder during the first train_mcs MCS of the run.
 
-Protocol (adaptive_run): the first train_mcs MCS of the equilibration run on
-the linear ladder, then the temperatures of the slots are retuned in place
-(every replica keeps its state and slot) and the remaining MCS run on the new
-ladder.  Every swap passes the exact acceptance test throughout, so the
-sampled distribution at each temperature is unchanged; only the production
-window after the switch is timed and its round trips counted.
+Protocol (adaptive_run): the first train_mcs MCS of the equilibration run are
+split into windows.  The first runs on the linear ladder; after each window
+the MLP is retrained on the samples of all windows so far and the temperatures
+of the slots are retuned in place (every replica keeps its state and slot), so
+a pair whose acceptance was too small to measure on one ladder is measured on
+the next.  The remaining MCS run on the last ladder.  Every swap passes the
+exact acceptance test throughout, so the sampled distribution at each
+temperature is unchanged; only the production window after the last retune
+is timed and its round trips counted.
 """
 from __future__ import annotations
 
@@ -89,41 +92,34 @@ def acceptance_heat_capacity(temps, acc, attempts):
 
 
 def learn_ladder(eng: E.Engine, temps, e, c, n_out: Optional[int] = None, iters: int = 4000,
-                 lr: float = 3e-3, seed: int = 1992):
-    """Train the device MLP on (T, e, log C) and place a new ladder."""
+                 lr: float = 3e-3, seed: int = 1992, weight=None, t_range=None):
+    """Train the device MLP on samples (T, e, log C) and place a new ladder of
+    n_out temperatures on t_range (default: the span of temps)."""
     temps = np.asarray(temps, np.float64)
     n_out = n_out or len(temps)
-    t_lo, t_hi = float(temps.min()), float(temps.max())
+    t_lo, t_hi = t_range if t_range is not None else (float(temps.min()), float(temps.max()))
     mid, half = 0.5 * (t_lo + t_hi), 0.5 * (t_hi - t_lo)
     x = ((temps - mid) / half).astype(np.float32)
     lc = np.log(np.maximum(c, 1e-12))
     y = np.stack([(e - e.mean()) / (e.std() + 1e-30), (lc - lc.mean()) / (lc.std() + 1e-30)],
                  axis=1).astype(np.float32)
-    w = np.ones_like(y)
+    w = np.ones_like(y) if weight is None else np.repeat(
+        np.asarray(weight, np.float32)[:, None], 2, axis=1)
     eng.mlp_init(seed)
     loss = eng.mlp_train(x, y, w, iters, lr)
     T = eng.ladder_place(mid, half, float(lc.std() + 1e-30), t_lo, t_hi, n_out)
     return T, loss
 
 
-def adaptive_run(eng: E.Engine, model: Model, temps, repeat: int, mcs: int, coa: int,
-                 pt_interval: int, *, learn: bool, train_mcs: int = 2000) -> LadderRun:
-    """Tempering run of mcs MCS: train_mcs on `temps`, then (learn=True) the
-    learned ladder or (learn=False) the same ladder for the remaining MCS."""
-    temps = np.asarray(temps, np.float64)
+def _window(eng: E.Engine, model: Model, temps, repeat: int, first: int, n_mcs: int, coa: int,
+            pt_interval: int, sc: int):
+    """Run n_mcs MCS from step `first` with fresh swap counters and energy
+    moments; returns (pair acceptance, attempts, <e> per slot, swap counter)."""
     n = len(temps)
-    groups = [k * n for k in range(repeat + 1)]
-    eq = model.tags.tmcs_eq
-    eng.set_betas(np.tile(model.betas(temps), repeat))
-    eng.set_ladders(groups)
-    eng.init_spins(E.INIT_RANDOM)
+    eng.set_ladders([k * n for k in range(repeat + 1)])
     eng.reset_moments()
-    half = train_mcs // 2
-    sc = eng.run(1, half, eq, coa, 0, 0.0, pt_interval, 0)
-    # measurement window: swap statistics and energy samples of the second half
-    eng.set_ladders(groups)
-    eng.reset_moments()
-    sc = eng.run(half + 1, train_mcs - half, half, coa, 0, 0.0, pt_interval, sc)
+    # every step of the window samples (eq = first - 1) so that <e> is measured
+    sc = eng.run(first, n_mcs, first - 1, coa, 0, 0.0, pt_interval, sc)
     mom = eng.moments()
     a, att, _ = _pair_acceptance(eng, n, repeat)
     e = np.zeros(n)
@@ -131,22 +127,56 @@ def adaptive_run(eng: E.Engine, model: Model, temps, repeat: int, mcs: int, coa:
         for k in range(n):
             m = mom[g * n + k]
             e[k] += m[E.MOM_E] / max(m[E.MOM_N], 1.0) / repeat
-    c = acceptance_heat_capacity(temps, a, att)
-    run_temps, loss = temps, None
-    if learn:
-        run_temps, loss = learn_ladder(eng, temps, e, c, n_out=n)
-        eng.retune_betas(np.tile(model.betas(run_temps), repeat))
+    return a, att, e, sc
+
+
+def adaptive_run(eng: E.Engine, model: Model, temps, repeat: int, mcs: int, coa: int,
+                 pt_interval: int, *, learn: bool, train_mcs: int = 2000,
+                 windows: int = 4) -> LadderRun:
+    """Tempering run of mcs MCS.  The first train_mcs MCS are `windows` equal
+    windows; each one measures the pair acceptance and <e> of the ladder it ran
+    on.  learn=True: after every window the MLP is retrained on the samples of
+    all windows so far (window k weighted 2^k, later ladders are closer to the
+    final one) and the slots are retuned to the ladder it places, so each
+    window refines the acceptance-consistent C where the previous ladder was
+    too coarse to measure it.  learn=False: every window runs on `temps`.
+    The remaining mcs - train_mcs MCS are the timed production window."""
+    temps = np.asarray(temps, np.float64)
+    n = len(temps)
+    t_range = (float(temps.min()), float(temps.max()))
+    eq = model.tags.tmcs_eq
+    eng.set_betas(np.tile(model.betas(temps), repeat))
+    eng.init_spins(E.INIT_RANDOM)
+    win = max(train_mcs // windows, 1)
+    cur, sc, first = temps, 0, 1
+    xs, es, cs, ws = [], [], [], []
+    a = c = e = None
+    loss = None
+    for k in range(windows):
+        length = win if k < windows - 1 else train_mcs - win * (windows - 1)
+        a, att, e, sc = _window(eng, model, cur, repeat, first, length, coa, pt_interval, sc)
+        first += length
+        c = acceptance_heat_capacity(cur, a, att)
+        xs.append(cur)
+        es.append(e)
+        cs.append(c)
+        ws.append(np.full(n, 2.0 ** k))
+        if learn:
+            cur, loss = learn_ladder(eng, np.concatenate(xs), np.concatenate(es),
+                                     np.concatenate(cs), n_out=n, weight=np.concatenate(ws),
+                                     t_range=t_range)
+            eng.retune_betas(np.tile(model.betas(cur), repeat))
     # production window
-    eng.set_ladders(groups)
+    eng.set_ladders([k * n for k in range(repeat + 1)])
     eng.reset_moments()
     eng.synchronize()
     t0 = time.perf_counter()
     rest = mcs - train_mcs
-    sc = eng.run(train_mcs + 1, rest, eq, coa, 0, 0.0, pt_interval, sc)
+    sc = eng.run(first, rest, eq, coa, 0, 0.0, pt_interval, sc)
     eng.synchronize()
     wall = time.perf_counter() - t0
     acc, _, trips = _pair_acceptance(eng, n, repeat)
-    return LadderRun(run_temps, wall, rest, trips, acc, float(eng.R) * model.n_sites * rest,
+    return LadderRun(cur, wall, rest, trips, acc, float(eng.R) * model.n_sites * rest,
                      train_e=e, train_c=c, train_acc=a, mlp_loss=loss)
 
 

@@ -56,6 +56,10 @@ typedef struct ether_ctx ether_ctx;
 
 /* compat flags (ether_set_compat) */
 #define ETHER_COMPAT_PT_ETOT  1  /* tempering swap uses the double-counted E_tot (src/ether.f90:177,194)      */
+#define ETHER_COMPAT_SPLIT_OVRR 2 /* ether_run: a whole OVRR pass before the sweep, as src/ether.f90:163-172;
+                                    by default a pass of at most one reflection per site is fused into the
+                                    sweep, each colour phase reflecting a site about its local field and then
+                                    proposing from the reflected spin (the same stationary distribution)     */
 
 /* per-replica moment record read by ether_read_moments (8 doubles) */
 #define ETHER_MOM_N     0   /* number of samples                                  */

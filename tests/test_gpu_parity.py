@@ -197,6 +197,113 @@ def test_overrelaxation_matches_oracle(frac):
     eng.close()
 
 
+def _oracle_fused_with_margins(m, orc, states, order, rid, betas, seed, step, frac):
+    """The fused schedule on the oracle: site by site in colour order, each
+    replica's spin is reflected about its local field (selected with
+    probability frac by the OVRR draw) and then gets the pooled Metropolis
+    proposal.  Sites of one colour share no bonds, so this order is the
+    device's colour phase.  Records -beta dE - ln eta, |A| and the in-plane
+    length of the reflected spin per site."""
+    R = len(states)
+    S = np.stack([s for s, _ in states]).copy()
+    phi = np.stack([p for _, p in states]).copy()
+    marg = np.zeros((R, m.n_sites))
+    amag = np.zeros((R, m.n_sites))
+    rho = np.ones((R, m.n_sites))
+    for i in order:
+        for r in range(R):
+            amag[r, i] = np.linalg.norm(orc.exchange_field(S[r], i))
+            if frac < 1.0:
+                w = o.rng(seed, int(rid[i]), r, step, o.RNG_OVRR, 0)
+                if not ((int(w[0]) >> 8) * 2.0 ** -24 < frac):
+                    continue
+            orc.reflect(S[r], i)
+            phi[r, i] = math.atan2(S[r, i, 1], S[r, i, 0])
+            rho[r, i] = math.hypot(S[r, i, 0], S[r, i, 1])
+        sn, pn, eta = orc.propose_site(seed, int(rid[i]), step, S[:, i, :], phi[:, i])
+        for r in range(R):
+            dE = orc.delta_energy(S[r], i, sn[r])
+            marg[r, i] = -betas[r] * dE - math.log(eta[r])
+            if marg[r, i] > 0:
+                S[r, i] = sn[r]
+                phi[r, i] = pn[r]
+    return S, phi, marg, amag, rho
+
+
+@pytest.mark.parametrize("name,over,R,frac", [
+    ("sc_heis_afm_32", dict(sc="8 8 8"), 2, 1.0),
+    ("sc_heis_afm_32", dict(sc="4 4 4"), 70, 0.5),                      # OVRR draw, two groups
+    ("tri_afm_96", dict(sc="12 12 1", sia=".F."), 2, 1.0),              # ANGLE + Zeeman
+    ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11"), 2, 1.0),
+])
+def test_fused_overrelaxation_sweep_matches_oracle(name, over, R, frac):
+    """One MCS of ether_run with OVRR due: the reflection is fused into each
+    colour phase of the sweep.  Every site whose Metropolis margin exceeds 1e-3
+    must land within the FP32 error of the reflection of the oracle's spin.
+    That error grows as 1/|A|; with ANGLE moves the proposal starts from
+    phi = atan2(S_y, S_x) of the reflected spin, whose error grows as 1/rho
+    with rho the reflected spin's in-plane length."""
+    say(f"fused ovrr+sweep {name} {over} R={R} frac={frac}")
+    m, eng, orc = pair(name, R, **over)
+    lo, hi = (2.0, 8.0) if name == "pyro_pt_16" else (5.0, 20.0)
+    betas = m.betas(np.linspace(lo, hi, R))
+    eng.set_betas(betas)
+    states = _xyz_state(m, eng, R, 11)
+    eng.run(1, 1, 10 ** 6, 10 ** 6, 1, frac)
+    nc, col, order, rid = colour_order(eng)
+    nbrs = [set() for _ in range(m.n_sites)]
+    for s in range(m.nn_count.shape[0]):
+        for i in range(m.n_sites):
+            nbrs[i].update(m.nn_idx[s, i, : m.nn_count[s, i]].tolist())
+    S, phi, marg, amag, rho = _oracle_fused_with_margins(m, orc, states, order, rid, betas,
+                                                         m.tags.seed, 0, frac)
+    jabs = np.abs(m.j_exc).max() * m.nn_count.sum(axis=0).max()
+    n_checked = 0
+    for r in range(R):
+        Sg, _ = eng.get_spins(r)
+        tainted = set()
+        for i in order:
+            if i in tainted:
+                continue
+            tol = 2e-6 * (1.0 + jabs / max(amag[r, i], 1e-3))
+            if m.tags.angle:
+                tol *= 1.0 + 1.0 / max(rho[r, i], 1e-3)
+            dev = np.abs(Sg[i] - S[r, i]).max()
+            if dev > tol:
+                assert abs(marg[r, i]) < 1e-3, f"replica {r} site {i}: margin {marg[r, i]}, dev {dev}"
+                tainted |= nbrs[i]
+            else:
+                n_checked += 1
+    assert n_checked > 0.99 * R * m.n_sites
+    eng.close()
+
+
+def test_split_ovrr_compat_equals_separate_calls():
+    """ETHER_COMPAT_SPLIT_OVRR: ether_run performs the reference's order, a whole
+    OVRR pass and then the sweep, bit for bit what ether_overrelax followed by
+    ether_sweep gives."""
+    from ether_amd import engine as E
+    from ether_amd.driver import make_engine
+    say("split ovrr compat")
+    m, eng, _ = pair("sc_heis_afm_32", 3, sc="6 6 6")
+    ref = make_engine(m, 3, compat=E.COMPAT_SPLIT_OVRR)
+    betas = m.betas([5.0, 10.0, 20.0])
+    for e in (eng, ref):
+        e.set_betas(betas)
+    st = _xyz_state(m, eng, 3, 21)
+    for r, (S, phi) in enumerate(st):
+        ref.set_spins(r, S, phi)
+    eng.overrelax(1.0)
+    eng.sweep(1)
+    ref.run(1, 1, 10 ** 6, 10 ** 6, 1, 1.0)
+    for r in range(3):
+        a, _ = eng.get_spins(r)
+        b, _ = ref.get_spins(r)
+        assert np.array_equal(a, b)
+    eng.close()
+    ref.close()
+
+
 def test_pt_swaps_match_oracle():
     say("pt swaps")
     R = 8

@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pt", action="store_true", help="skip the tempering ladder comparison")
     ap.add_argument("--pt-mcs", type=int, default=20000)
+    ap.add_argument("--pt-trips-mcs", type=int, default=200000,
+                    help="MCS of the round-trip comparison on 128 spins")
     ap.add_argument("--split-ovrr", action="store_true",
                     help="the reference's order: a whole OVRR pass, then the sweep")
     args = ap.parse_args()
@@ -129,6 +131,7 @@ def main():
     R = len(temps) * t.repeat
     eng = make_engine(m, R, compat=E.COMPAT_SPLIT_OVRR if args.split_ovrr else 0)
     eng.set_betas(np.tile(m.betas(temps), t.repeat))
+    spw, xcd, tune_ms = eng.autotune(reps=10)
     coa = fortran_nint(t.coa)
     ovrr_on = t.ovrr and t.xyz and not t.zeeman and not t.sia
     ovrr_iv = t.ovrr_mcs if ovrr_on else 0
@@ -190,6 +193,10 @@ def main():
             "bytes_per_update": B_UPD[m.model_id], "updates_per_launch": per_launch_updates,
             "avg_launch_ms": avg_ms, "launches": sweep_n,
         },
+        "launch": {"sites_per_wave": spw, "xcd_map": xcd,
+                   "autotune_ms_per_sweep": {f"spw{k}_xcd{x}": round(float(tune_ms[x][i]), 5)
+                                             for x in (0, 1)
+                                             for i, k in enumerate(E.Engine.AUTOTUNE_SPW)}},
         "kernel_ms": {"sweep": sweep_ms, "ovrr": ovrr_ms, "measure": meas_ms,
                       "sweep_launches": sweep_n, "ovrr_launches": ovrr_n, "measure_launches": meas_n},
         "acceptance_mean": float(mom[:, E.MOM_ACC].mean() / args.steps),
@@ -197,27 +204,36 @@ def main():
     eng.close()
     if not args.no_pt:
         out["pt"] = pt_comparison(args.pt_mcs)
-        # At 65536 spins the equilibrium acceptance of a 64-temperature ladder on
-        # 1 -> 0.05 K is erfc(sqrt(C) dlnT / 2) with C ~ N: ~1e-17 even for the
-        # best ladder, so no label completes a round trip there.  The same
-        # schedule on 4^3 cells (1024 spins) is where round trips can be counted.
-        out["pt_1024"] = pt_comparison(args.pt_mcs, cells=4)
+        # On 1 -> 0.05 K (T/J 0.086 -> 0.004) uniform Marsaglia proposals are
+        # accepted at about kT/4J or less, the chains' energies hardly move
+        # between swaps and no label completes a round trip on any ladder; and
+        # at 65536 spins the equilibrium acceptance of a 64-temperature ladder,
+        # erfc(sqrt(C) dlnT / 2) with C ~ N, is ~1e-17 even for the best
+        # ladder.  Round trips are counted on 2^3 cells (128 spins) on
+        # 20 -> 1 K with the same 64 temperatures and schedule, over enough
+        # swap rounds for labels to cross a 64-slot ladder.
+        out["pt_trips"] = pt_comparison(args.pt_trips_mcs, cells=2, temp="20 1 0.301587301587302")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, args.cpu_budget)
     print(json.dumps(out), flush=True)
 
 
-def pt_comparison(mcs, cells=16):
+def pt_comparison(mcs, cells=16, temp=None):
     """Pyrochlore tempering: linear ladder versus the learned ladder.
 
     pyro_pt_16: cells^3 cubic cells x 16 sites (65536 spins at 16^3), J1 = 1,
-    J2 = 0.05 meV, 64 temperatures 1 -> 0.05 K, swaps every 11 MCS, REPEAT 4
-    (256 chains); the MLP is trained on <e> and the acceptance-consistent C
-    measured in the first 2000 MCS (four windows, retuned after each).
+    J2 = 0.05 meV, 64 temperatures (1 -> 0.05 K unless `temp` gives another
+    TEMP tag), swaps every 11 MCS, REPEAT 4 (256 chains); the MLP is trained
+    on <e> and the acceptance-consistent C measured in the first 2000 MCS
+    (four windows, retuned after each).
     Reports completed cold->hot->cold round trips per second of the production
     loop and the spin updates per second of the same loop."""
     from ether_amd import ladder, workloads
-    m = workloads.model("pyro_pt_16", sc=f"{cells} {cells} {cells}")
+    over = {"sc": f"{cells} {cells} {cells}"}
+    if temp:
+        over["temp"] = temp
+    m = workloads.model("pyro_pt_16", **over)
+    temps = m.tags.temperatures()
     t0 = time.perf_counter()
     lin, lrn = ladder.compare_ladders(m, train_mcs=2000, mcs=mcs)
     total = time.perf_counter() - t0
@@ -228,11 +244,16 @@ def pt_comparison(mcs, cells=16):
                 "spin_updates_per_s": run.updates_per_s,
                 "swap_acc_min": float(run.acc_rate.min()), "swap_acc_max": float(run.acc_rate.max()),
                 "swap_acc_cv": float(run.acc_rate.std() / max(run.acc_rate.mean(), 1e-30)),
+                "ladder": run.source,
+                "round_trip_cost": run.cost,
+                "metropolis_acc_min": float(run.metro_acc.min()),
+                "metropolis_acc_max": float(run.metro_acc.max()),
                 "swap_acc": [round(float(x), 4) for x in run.acc_rate],
                 "temps": [round(float(x), 6) for x in run.temps]}
 
     return {"workload": f"pyrochlore {cells}^3x16 ({m.n_sites} spins), J1=1 J2=0.05 meV, "
-                        f"64 T 1->0.05 K x 4 repeats = 256 chains, swaps every 11 MCS, MCS {mcs}; "
+                        f"{len(temps)} T {temps[0]:g}->{temps[-1]:g} K x {m.tags.repeat} repeats, "
+                        f"swaps every {m.tags.exchange_interval} MCS, MCS {mcs}; "
                         f"2000 training MCS, then the ladder under test",
             "linear": rep(lin), "learned": rep(lrn), "mlp_final_loss": lrn.mlp_loss,
             "gain_round_trips_per_s": (lrn.round_trips_per_s / lin.round_trips_per_s

@@ -60,7 +60,8 @@ struct ether_ctx {
     std::vector<double> ising_j;
     std::vector<int> ising_Z, ising_stride;
     int ising_P = 1, ising_offset = 0, ising_table = 0;
-    int xcd_map = 0;
+    int xcd_map = 1;      // launch parameters of the sweep phases (ether_set_launch)
+    int spw = kDefaultSitesPerWave;
     int meas_tiles = 0, meas_sites = 0, meas_rec = 0;
     // device
     uint32_t* d_ell = nullptr;
@@ -551,12 +552,15 @@ extern "C" int ether_commit(ether_ctx* c) {
     const int R = c->R;
     c->Rp = (R + kWave - 1) / kWave * kWave;
     const size_t RN = (size_t)c->Rp * N;
+    // the sweep kernels address a spin plane with 32-bit buffer offsets
+    if (RN * sizeof(float) > (size_t)INT32_MAX)
+        return fail(c, ETHER_ERR_LIMIT, "ether_commit: sites x padded replicas exceeds 2^29 "
+                                      "(one FP32 spin plane must stay under 2 GiB)");
     const int G = c->Rp / kWave;
     // measurement: about 4096 wave units per pass, at least 8 sites each
     c->meas_sites = std::max(8, (int)(((long long)N * G + 4095) / 4096));
     c->meas_tiles = (N + c->meas_sites - 1) / c->meas_sites;
     c->meas_rec = ising ? (c->n_classes + nsp + 1) : 6;
-    c->xcd_map = 1;
     TRY(dev_alloc(c, &c->d_ell, ell.size()));
     TRY(dev_alloc(c, &c->d_sp, (size_t)N));
     TRY(dev_alloc(c, &c->d_w, (size_t)N));
@@ -745,9 +749,9 @@ extern "C" int ether_set_ladders(ether_ctx* c, int32_t n_groups, const int32_t* 
 // argument blocks
 // --------------------------------------------------------------------------
 // workgroups of a colour phase: (replica groups) x (site chunks of
-// kWavesPerBlock*kSitesPerWave sites), padded to a multiple of 8 for the XCD map
-static int phase_blocks(int n_sites, int groups, int xcd_map) {
-    const int chunk = kWavesPerBlock * kSitesPerWave;
+// kWavesPerBlock*spw sites), padded to a multiple of 8 for the XCD map
+static int phase_blocks(int n_sites, int groups, int xcd_map, int spw) {
+    const int chunk = kWavesPerBlock * spw;
     long long nb = (long long)groups * ((n_sites + chunk - 1) / chunk);
     if (xcd_map) nb = (nb + 7) / 8 * 8;
     return (int)nb;
@@ -769,7 +773,8 @@ static XyzArgs xyz_args(ether_ctx* c, int colour) {
     a.c_begin = c->cbeg[colour];
     a.c_end = c->cbeg[colour + 1];
     a.xcd_map = c->xcd_map;
-    a.nblocks = phase_blocks(a.c_end - a.c_begin, a.G, a.xcd_map);
+    a.spw = c->spw;
+    a.nblocks = phase_blocks(a.c_end - a.c_begin, a.G, a.xcd_map, a.spw);
     a.slot_of = c->d_slot_of;
     a.beta_slot = c->d_beta_f;
     a.seed = c->seed;
@@ -778,6 +783,7 @@ static XyzArgs xyz_args(ether_ctx* c, int colour) {
     a.pass = 0;
     a.frac = 1.0f;
     a.fuse = 0;
+    a.plane_bytes = (uint32_t)((size_t)c->N * c->Rp * sizeof(float));
     a.acc = c->d_acc;
     a.n_classes = c->n_classes;
     a.jclass = c->d_jclass;
@@ -803,7 +809,8 @@ static IsingArgs ising_args(ether_ctx* c, int colour) {
     a.c_begin = c->cbeg[colour];
     a.c_end = c->cbeg[colour + 1];
     a.xcd_map = c->xcd_map;
-    a.nblocks = phase_blocks(a.c_end - a.c_begin, a.G4, a.xcd_map);
+    a.spw = c->spw;
+    a.nblocks = phase_blocks(a.c_end - a.c_begin, a.G4, a.xcd_map, a.spw);
     a.table_in_lds = ((long long)c->R * c->ising_table <= kIsingLdsTable) ? 1 : 0;
     a.slot_of = c->d_slot_of;
     a.seed = c->seed;
@@ -1047,6 +1054,106 @@ extern "C" int ether_sweep(ether_ctx* c, int32_t n_mcs, double* accept) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         for (int r = 0; r < c->R; ++r) accept[r] = (double)(after[r] - before[r]) / c->N;
     }
+    return ETHER_OK;
+}
+
+// --------------------------------------------------------------------------
+// launch shape of the sweep phases and its autotuner
+// --------------------------------------------------------------------------
+extern "C" int ether_set_launch(ether_ctx* c, int32_t sites_per_wave, int32_t xcd_map) {
+    if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
+    if (sites_per_wave < 1 || sites_per_wave > kMaxSitesPerWave || (xcd_map != 0 && xcd_map != 1))
+        return fail(c, ETHER_ERR_ARG, "ether_set_launch: sites_per_wave must be 1..16, xcd_map 0 or 1");
+    if (c->spw != sites_per_wave || c->xcd_map != xcd_map) drop_graphs(c);   // baked into graphs
+    c->spw = sites_per_wave;
+    c->xcd_map = xcd_map;
+    return ETHER_OK;
+}
+
+// Times `reps` sweeps for every (sites per wave, XCD map) candidate on the
+// context's own lattice, replicas and couplings, keeps the fastest, and puts
+// the spins, acceptance counters and sweep counter back as they were.
+extern "C" int ether_autotune(ether_ctx* c, int32_t reps, int32_t* sites_per_wave, int32_t* xcd_map,
+                              double* table_ms) {
+    TRY(need_betas(c));
+    if (reps <= 0) return fail(c, ETHER_ERR_ARG, "ether_autotune: reps must be positive");
+    static const int kSpw[] = {1, 2, 4, 8, 16};
+    const int n_spw = (int)(sizeof(kSpw) / sizeof(kSpw[0]));
+    const size_t RN = (size_t)c->Rp * c->N;
+    std::vector<std::pair<void*, size_t>> planes;
+    if (c->model == ETHER_MODEL_ISING) {
+        planes.push_back({c->d_s, RN});
+    } else {
+        planes.push_back({c->d_sx, RN * 4});
+        planes.push_back({c->d_sy, RN * 4});
+        planes.push_back({c->d_sz, RN * 4});
+        if (c->angle) planes.push_back({c->d_phi, RN * 4});
+    }
+    planes.push_back({c->d_acc, sizeof(unsigned long long) * c->R});
+    size_t total = 0;
+    for (auto& p : planes) total += (p.second + 255) / 256 * 256;
+    char* save = nullptr;
+    HIPCHK(c, hipMalloc(&save, total));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = ETHER_OK;
+    const uint64_t step0 = c->step;
+    const bool timing0 = c->timing;
+    const int spw0 = c->spw, xcd0 = c->xcd_map;
+    double best = 1e300;
+    int best_spw = spw0, best_xcd = xcd0;
+    c->timing = false;
+    {
+        size_t off = 0;
+        for (auto& p : planes) {
+            if (hipMemcpyAsync(save + off, p.first, p.second, hipMemcpyDeviceToDevice, c->stream) !=
+                hipSuccess)
+                rc = fail(c, ETHER_ERR_HIP, "ether_autotune: save copy failed");
+            off += (p.second + 255) / 256 * 256;
+        }
+    }
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = fail(c, ETHER_ERR_HIP, "ether_autotune: event creation failed");
+    for (int i = 0; !rc && i < 2 * n_spw; ++i) {
+        c->spw = kSpw[i % n_spw];
+        c->xcd_map = i / n_spw;
+        if ((rc = enqueue_sweep(c))) break;   // warm-up of this shape
+        (void)hipEventRecord(e0, c->stream);
+        for (int k = 0; k < reps && !rc; ++k) rc = enqueue_sweep(c);
+        (void)hipEventRecord(e1, c->stream);
+        if (rc) break;
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            rc = fail(c, ETHER_ERR_HIP, "ether_autotune: sweep failed");
+            break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const double per = (double)ms / reps;
+        if (table_ms) table_ms[i] = per;
+        if (per < best) {
+            best = per;
+            best_spw = c->spw;
+            best_xcd = c->xcd_map;
+        }
+    }
+    {
+        size_t off = 0;
+        for (auto& p : planes) {
+            (void)hipMemcpyAsync(p.first, save + off, p.second, hipMemcpyDeviceToDevice, c->stream);
+            off += (p.second + 255) / 256 * 256;
+        }
+    }
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(save);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    c->step = step0;
+    c->timing = timing0;
+    c->spw = spw0;
+    c->xcd_map = xcd0;
+    if (rc) return rc;
+    TRY(ether_set_launch(c, best_spw, best_xcd));
+    if (sites_per_wave) *sites_per_wave = best_spw;
+    if (xcd_map) *xcd_map = best_xcd;
     return ETHER_OK;
 }
 

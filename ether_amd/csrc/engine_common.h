@@ -9,8 +9,9 @@
 //           or 64..256-byte (int8) segment.
 //   ell     neighbour table ell[slot*D + k], k < D: low 24 bits = neighbour
 //           slot, high 8 bits = coupling class (class 0 = padding, zero
-//           coupling).  Shared by all replicas; wave-uniform, so it is read
-//           with scalar loads.
+//           coupling).  Shared by all replicas; wave-uniform, so a wave keeps
+//           its row in scalar registers and addresses neighbour rows on the
+//           scalar unit (buffer loads with a scalar row offset).
 //   classes XYZ: FP32 (Jxx,Jyy,Jzz) per class.  Ising: one J per class and a
 //           mixed-radix stride so that sum_k stride_k * n_k (n_k = sum of the
 //           class-k neighbour spins) indexes an exact threshold table.
@@ -26,7 +27,8 @@ namespace ether {
 constexpr int kBlock = 256;          // threads per workgroup
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr int kSitesPerWave = 4;     // consecutive sites of a colour class per wave
+constexpr int kDefaultSitesPerWave = 4;  // consecutive sites of a colour class per wave
+constexpr int kMaxSitesPerWave = 16;     // (a launch parameter: ether_set_launch / ether_autotune)
 constexpr int kMaxClasses = 255;     // coupling classes (8-bit field of an ELL entry)
 constexpr int kMaxSpecies = 16;
 constexpr int kMaxIsingClasses = 8;
@@ -48,6 +50,7 @@ struct XyzArgs {
     int c_end;
     int nblocks;
     int xcd_map;
+    int spw;               // sites per wave
     const int* slot_of;
     const float* beta_slot;
     uint64_t seed;
@@ -56,6 +59,7 @@ struct XyzArgs {
     uint32_t pass;
     float frac;
     int fuse;              // sweep: reflect (with probability frac) before the proposal
+    uint32_t plane_bytes;  // bytes of one spin plane, N * Rp * 4 (< 2 GiB)
     unsigned long long* acc;
     int n_classes;
     const float* jclass;
@@ -80,6 +84,7 @@ struct IsingArgs {
     int c_end;
     int nblocks;
     int xcd_map;
+    int spw;               // sites per wave
     const int* slot_of;
     uint64_t seed;
     const uint32_t* ctr;

@@ -45,23 +45,37 @@ __device__ __forceinline__ float wrap_pif(float p) {
     const float two_pi = 6.28318530717958647692f;
     return p - two_pi * floorf((p + 3.14159265358979323846f) / two_pi);
 }
+// Unit-length renormalisation with the hardware reciprocal square root
+// (1 ulp): every vector it is applied to has length 1 to within a few ulp
+// already, and the spins are FP32 to begin with.
 __device__ __forceinline__ void normalise3(float& x, float& y, float& z) {
-    float inv = 1.0f / sqrtf(x * x + y * y + z * z);
+    const float inv = __builtin_amdgcn_rsqf(x * x + y * y + z * z);
     x *= inv;
     y *= inv;
     z *= inv;
 }
+// Marsaglia's disc test a1^2 + a2^2 < 2^46 on 24-bit coordinates, exact.
+// The FP32 sum is within 2^24 of the integer sum (|a| <= 2^23 converts
+// exactly; two roundings of at most 2^-24 relative), so outside a band of
+// 2^26 around the boundary its decision is the integer one; a wave with a
+// lane inside the band (about one pair in 10^5) takes the integer test.
 __device__ __forceinline__ bool in_disc(uint32_t cx, uint32_t cy) {
-    int32_t a1 = (int32_t)cx - (1 << 23), a2 = (int32_t)cy - (1 << 23);
-    return (int64_t)a1 * a1 + (int64_t)a2 * a2 < (int64_t(1) << 46);
+    const int32_t a1 = (int32_t)cx - (1 << 23), a2 = (int32_t)cy - (1 << 23);
+    const float f1 = (float)a1, f2 = (float)a2;
+    const float q = fmaf(f1, f1, f2 * f2);
+    bool inside = q < 0x1p46f;
+    const bool near = fabsf(q - 0x1p46f) <= 0x1p26f;
+    if (__any(near)) {
+        if (near) inside = (int64_t)a1 * a1 + (int64_t)a2 * a2 < (int64_t(1) << 46);
+    }
+    return inside;
 }
 // Marsaglia's point from an accepted pair (src/George_Marsaglia.f90:36-38), renormalised
 __device__ __forceinline__ void marsaglia_point(uint32_t cx, uint32_t cy, float& x, float& y, float& z) {
-    int32_t a1 = (int32_t)cx - (1 << 23), a2 = (int32_t)cy - (1 << 23);
-    int64_t q = (int64_t)a1 * a1 + (int64_t)a2 * a2;
-    float v1 = (float)a1 * 0x1p-23f, v2 = (float)a2 * 0x1p-23f;
-    float s = (float)q * 0x1p-46f;
-    float rt = sqrtf(1.0f - s);
+    const int32_t a1 = (int32_t)cx - (1 << 23), a2 = (int32_t)cy - (1 << 23);
+    const float v1 = (float)a1 * 0x1p-23f, v2 = (float)a2 * 0x1p-23f;   // exact
+    const float s = fmaf(v1, v1, v2 * v2);
+    const float rt = __builtin_amdgcn_sqrtf(1.0f - s);
     x = 2.0f * v1 * rt;
     y = 2.0f * v2 * rt;
     z = 1.0f - 2.0f * s;
@@ -159,18 +173,44 @@ __device__ __forceinline__ void book_acceptance(unsigned* sAcc, int nrep_lane, c
     }
 }
 
-// Exchange field h = sum_k J_{c_k} o S_{j_k} of replica lane r at the site
+// Spin planes are replica-major: the value of (slot, replica r) sits at
+// plane[slot * Rp + r].  A wave's site is uniform, so the row's byte offset
+// is a scalar (soffset) and the lane adds its own byte offset r*4 (voffset):
+// buffer loads and stores through a per-plane resource need no per-lane
+// address arithmetic, and an offset past the plane reads 0 instead of
+// faulting.  The host keeps every plane under 2 GiB.
+typedef __amdgpu_buffer_rsrc_t Plane;
+__device__ __forceinline__ Plane plane_of(const float* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float plane_ld(Plane p, uint32_t roff, uint32_t soff) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(p, (int)roff, (int)soff, 0));
+}
+__device__ __forceinline__ void plane_st(Plane p, uint32_t roff, uint32_t soff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), p, (int)roff, (int)soff, 0);
+}
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+// Coupling of class c as (Jxx, Jyy, Jzz, 0), staged in LDS (one 16-byte read)
+__device__ __forceinline__ void stage_couplings(float4* sJ4, const float* jclass, int n_classes) {
+    for (int t = threadIdx.x; t <= n_classes; t += kBlock)
+        sJ4[t] = make_float4(jclass[3 * t + 0], jclass[3 * t + 1], jclass[3 * t + 2], 0.f);
+}
+
+// Exchange field h = sum_k J_{c_k} o S_{j_k} of the lane's replica at the site
 // whose ELL row is nb (src/Hamiltonian.f90:72-103 / src/get_ovrr_vec.f90:64-82).
-// With DM > 0 the row has exactly DM entries (padded with class 0, J = 0) and
-// all neighbour loads are issued before the first use, so their latencies
-// overlap each other and the generator work that follows.  Accumulation order
-// is k = 0..D-1 in both forms.
+// The row entries are wave-uniform (read to scalar registers), so neighbour
+// row addresses and coupling classes cost no vector instructions.  With
+// DM > 0 the row has exactly DM entries (padded with class 0, J = 0) and all
+// neighbour loads are issued before the first use, so their latencies overlap
+// each other and the generator work that follows.  Accumulation order is
+// k = 0..D-1 in both forms.
 template <int DM>
-__device__ __forceinline__ void local_field(const float* __restrict__ sx,
-                                            const float* __restrict__ sy,
-                                            const float* __restrict__ sz, const uint32_t* nb,
-                                            int D, int Rp, int r, const float* sJ, float& hx,
-                                            float& hy, float& hz) {
+__device__ __forceinline__ void local_field(Plane px, Plane py, Plane pz, const uint32_t* nb, int D,
+                                            uint32_t row_bytes, uint32_t roff, const float4* sJ4,
+                                            float& hx, float& hy, float& hz) {
     hx = 0.f;
     hy = 0.f;
     hz = 0.f;
@@ -179,27 +219,28 @@ __device__ __forceinline__ void local_field(const float* __restrict__ sx,
         uint32_t cl[DM];
 #pragma unroll
         for (int k = 0; k < DM; ++k) {
-            const uint32_t e = nb[k];
-            const size_t jr = (size_t)(e & 0xffffffu) * Rp + r;
+            const uint32_t e = uniform_u32(nb[k]);
+            const uint32_t so = (e & 0xffffffu) * row_bytes;
             cl[k] = e >> 24;
-            vx[k] = sx[jr];
-            vy[k] = sy[jr];
-            vz[k] = sz[jr];
+            vx[k] = plane_ld(px, roff, so);
+            vy[k] = plane_ld(py, roff, so);
+            vz[k] = plane_ld(pz, roff, so);
         }
 #pragma unroll
         for (int k = 0; k < DM; ++k) {
-            hx = fmaf(sJ[3 * cl[k] + 0], vx[k], hx);
-            hy = fmaf(sJ[3 * cl[k] + 1], vy[k], hy);
-            hz = fmaf(sJ[3 * cl[k] + 2], vz[k], hz);
+            const float4 j = sJ4[cl[k]];
+            hx = fmaf(j.x, vx[k], hx);
+            hy = fmaf(j.y, vy[k], hy);
+            hz = fmaf(j.z, vz[k], hz);
         }
     } else {
         for (int k = 0; k < D; ++k) {
-            const uint32_t e = nb[k];
-            const size_t jr = (size_t)(e & 0xffffffu) * Rp + r;
-            const uint32_t c = e >> 24;
-            hx = fmaf(sJ[3 * c + 0], sx[jr], hx);
-            hy = fmaf(sJ[3 * c + 1], sy[jr], hy);
-            hz = fmaf(sJ[3 * c + 2], sz[jr], hz);
+            const uint32_t e = uniform_u32(nb[k]);
+            const uint32_t so = (e & 0xffffffu) * row_bytes;
+            const float4 j = sJ4[e >> 24];
+            hx = fmaf(j.x, plane_ld(px, roff, so), hx);
+            hy = fmaf(j.y, plane_ld(py, roff, so), hy);
+            hz = fmaf(j.z, plane_ld(pz, roff, so), hz);
         }
     }
 }
@@ -211,7 +252,7 @@ __device__ __forceinline__ bool reflect_about(float hx, float hy, float hz, floa
                                               float& z) {
     const float aa = hx * hx + hy * hy + hz * hz;
     if (!(aa > 0.f)) return false;
-    const float f = 2.0f * (hx * x + hy * y + hz * z) / aa;
+    const float f = 2.0f * (hx * x + hy * y + hz * z) * __builtin_amdgcn_rcpf(aa);
     x = f * hx - x;
     y = f * hy - y;
     z = f * hz - z;
@@ -228,13 +269,13 @@ __device__ __forceinline__ bool reflect_about(float hx, float hy, float hz, floa
 // ---------------------------------------------------------------------------
 template <bool ANGLE, int DM>
 __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
-    __shared__ float sJ[3 * (kMaxClasses + 1)];
+    __shared__ float4 sJ4[kMaxClasses + 1];
     __shared__ float sZ[3 * kMaxSpecies];
     __shared__ float sD[3 * kMaxSpecies];
     __shared__ uint64_t sPool[kWavesPerBlock][kWave];
     __shared__ unsigned sAcc[kBlock];
     const bool ZEE = a.zee != 0, SIA = a.sia != 0;   // wave-uniform
-    for (int t = threadIdx.x; t < 3 * (a.n_classes + 1); t += kBlock) sJ[t] = a.jclass[t];
+    stage_couplings(sJ4, a.jclass, a.n_classes);
     if (ZEE)
         for (int t = threadIdx.x; t < 3 * a.nsp; t += kBlock) sZ[t] = a.zvec[t];
     if (SIA)
@@ -245,20 +286,24 @@ __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
     const int g = bb % a.G, chunk = bb / a.G;
     const int w = wave_id(), lane = lane_id();
     const int r = g * kWave + lane;
+    const uint32_t roff = 4u * (uint32_t)r, rowb = 4u * (uint32_t)a.Rp;
+    const Plane px = plane_of(a.sx, a.plane_bytes), py = plane_of(a.sy, a.plane_bytes),
+                pz = plane_of(a.sz, a.plane_bytes);
+    const Plane pp = plane_of(ANGLE ? a.phi : a.sx, a.plane_bytes);
     const bool rv = r < a.R;
     const float beta = rv ? a.beta_slot[a.slot_of[r]] : 0.f;
     const uint32_t step = a.ctr[0] + a.step_off;
     uint64_t* pool = sPool[w];
     unsigned accepted = 0;
-    const int s0 = a.c_begin + (chunk * kWavesPerBlock + w) * kSitesPerWave;
-    for (int q = 0; q < kSitesPerWave; ++q) {
+    const int s0 = a.c_begin + (chunk * kWavesPerBlock + w) * a.spw;
+    for (int q = 0; q < a.spw; ++q) {
         const int slot = s0 + q;                       // wave-uniform
         if (slot >= a.c_end) break;
-        const size_t row = (size_t)slot * a.Rp + r;
-        float x = a.sx[row], y = a.sy[row], z = a.sz[row];
-        float cphi = ANGLE ? a.phi[row] : 0.f;
+        const uint32_t so = (uint32_t)slot * rowb;
+        float x = plane_ld(px, roff, so), y = plane_ld(py, roff, so), z = plane_ld(pz, roff, so);
+        float cphi = ANGLE ? plane_ld(pp, roff, so) : 0.f;
         float hx, hy, hz;
-        local_field<DM>(a.sx, a.sy, a.sz, a.ell + (size_t)slot * a.D, a.D, a.Rp, r, sJ, hx, hy, hz);
+        local_field<DM>(px, py, pz, a.ell + (size_t)slot * a.D, a.D, rowb, roff, sJ4, hx, hy, hz);
         const int sp = (ZEE || SIA) ? a.sp[slot] : 0;
         if (ZEE) {
             hx += sZ[3 * sp + 0];
@@ -292,16 +337,16 @@ __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
         }
         // accept if -beta*dE > log(eta)   (src/Metropolis.f90:52)
         if (rv && -beta * dE > __logf(eta)) {
-            a.sx[row] = nx;
-            a.sy[row] = ny;
-            a.sz[row] = nz;
-            if (ANGLE) a.phi[row] = nphi;
+            plane_st(px, roff, so, nx);
+            plane_st(py, roff, so, ny);
+            plane_st(pz, roff, so, nz);
+            if (ANGLE) plane_st(pp, roff, so, nphi);
             ++accepted;
         } else if (rv && reflected) {
-            a.sx[row] = x;
-            a.sy[row] = y;
-            a.sz[row] = z;
-            if (ANGLE) a.phi[row] = cphi;
+            plane_st(px, roff, so, x);
+            plane_st(py, roff, so, y);
+            plane_st(pz, roff, so, z);
+            if (ANGLE) plane_st(pp, roff, so, cphi);
         }
     }
     // acceptance is booked to the temperature slot, as the reference books it per rank
@@ -313,10 +358,10 @@ __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
 // ---------------------------------------------------------------------------
 template <bool ANGLE, int DM>
 __global__ __launch_bounds__(kBlock) void k_xyz_ovrr(XyzArgs a) {
-    __shared__ float sJ[3 * (kMaxClasses + 1)];
+    __shared__ float4 sJ4[kMaxClasses + 1];
     __shared__ float sZ[3 * kMaxSpecies];
     const bool ZEE = a.zee != 0;
-    for (int t = threadIdx.x; t < 3 * (a.n_classes + 1); t += kBlock) sJ[t] = a.jclass[t];
+    stage_couplings(sJ4, a.jclass, a.n_classes);
     if (ZEE)
         for (int t = threadIdx.x; t < 3 * a.nsp; t += kBlock) sZ[t] = a.zvec[t];
     __syncthreads();
@@ -325,30 +370,34 @@ __global__ __launch_bounds__(kBlock) void k_xyz_ovrr(XyzArgs a) {
     const int w = wave_id(), lane = lane_id();
     const int r = g * kWave + lane;
     if (r >= a.R) return;
+    const uint32_t roff = 4u * (uint32_t)r, rowb = 4u * (uint32_t)a.Rp;
+    const Plane px = plane_of(a.sx, a.plane_bytes), py = plane_of(a.sy, a.plane_bytes),
+                pz = plane_of(a.sz, a.plane_bytes);
+    const Plane pp = plane_of(ANGLE ? a.phi : a.sx, a.plane_bytes);
     const uint32_t step = a.ctr[0] + a.step_off;
-    const int s0 = a.c_begin + (chunk * kWavesPerBlock + w) * kSitesPerWave;
-    for (int q = 0; q < kSitesPerWave; ++q) {
+    const int s0 = a.c_begin + (chunk * kWavesPerBlock + w) * a.spw;
+    for (int q = 0; q < a.spw; ++q) {
         const int slot = s0 + q;
         if (slot >= a.c_end) break;
         if (a.frac < 1.0f) {
             u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, step, RNG_OVRR, a.pass);
             if (!((float)(wr.x >> 8) * 0x1p-24f < a.frac)) continue;
         }
-        const size_t row = (size_t)slot * a.Rp + r;
+        const uint32_t so = (uint32_t)slot * rowb;
         float hx, hy, hz;
-        local_field<DM>(a.sx, a.sy, a.sz, a.ell + (size_t)slot * a.D, a.D, a.Rp, r, sJ, hx, hy, hz);
+        local_field<DM>(px, py, pz, a.ell + (size_t)slot * a.D, a.D, rowb, roff, sJ4, hx, hy, hz);
         if (ZEE) {
             const int sp = a.sp[slot];
             hx += sZ[3 * sp + 0];
             hy += sZ[3 * sp + 1];
             hz += sZ[3 * sp + 2];
         }
-        float nx = a.sx[row], ny = a.sy[row], nz = a.sz[row];
+        float nx = plane_ld(px, roff, so), ny = plane_ld(py, roff, so), nz = plane_ld(pz, roff, so);
         if (!reflect_about(hx, hy, hz, nx, ny, nz)) continue;
-        a.sx[row] = nx;
-        a.sy[row] = ny;
-        a.sz[row] = nz;
-        if (ANGLE) a.phi[row] = atan2f(ny, nx);
+        plane_st(px, roff, so, nx);
+        plane_st(py, roff, so, ny);
+        plane_st(pz, roff, so, nz);
+        if (ANGLE) plane_st(pp, roff, so, atan2f(ny, nx));
     }
 }
 
@@ -389,8 +438,8 @@ __global__ __launch_bounds__(kBlock) void k_ising_sweep(IsingArgs a) {
     }
     unsigned accepted[4] = {0, 0, 0, 0};
     const uint32_t step = a.ctr[0] + a.step_off;
-    const int s0 = a.c_begin + (chunk * kWavesPerBlock + w) * kSitesPerWave;
-    for (int q = 0; q < kSitesPerWave; ++q) {
+    const int s0 = a.c_begin + (chunk * kWavesPerBlock + w) * a.spw;
+    for (int q = 0; q < a.spw; ++q) {
         const int slot = s0 + q;
         if (slot >= a.c_end) break;
         if (!lv) continue;

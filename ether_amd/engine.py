@@ -70,6 +70,8 @@ EXPORTS = [
     ("ether_run", C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                             C.c_int32, _vp]),
     ("ether_set_graphs", C.c_int, [_vp, C.c_int32]),
+    ("ether_set_launch", C.c_int, [_vp, C.c_int32, C.c_int32]),
+    ("ether_autotune", C.c_int, [_vp, C.c_int32, _vp, _vp, _vp]),
     ("ether_set_timing", C.c_int, [_vp, C.c_int32]),
     ("ether_get_timing", C.c_int, [_vp, C.c_int32, _vp, _vp]),
     ("ether_synchronize", C.c_int, [_vp]),
@@ -296,6 +298,21 @@ class Engine:
 
     def set_graphs(self, on: bool) -> None:
         self._c(self.lib.ether_set_graphs(self.h, 1 if on else 0))
+
+    def set_launch(self, sites_per_wave: int, xcd_map: int) -> None:
+        self._c(self.lib.ether_set_launch(self.h, int(sites_per_wave), int(xcd_map)))
+
+    AUTOTUNE_SPW = (1, 2, 4, 8, 16)
+
+    def autotune(self, reps: int = 20):
+        """Pick the fastest sweep launch shape; returns (sites_per_wave,
+        xcd_map, table) with table[xcd_map][k] the ms per sweep of
+        sites_per_wave = AUTOTUNE_SPW[k]."""
+        spw = C.c_int32()
+        xcd = C.c_int32()
+        table = np.zeros(2 * len(self.AUTOTUNE_SPW), np.float64)
+        self._c(self.lib.ether_autotune(self.h, int(reps), C.byref(spw), C.byref(xcd), _ptr(table)))
+        return spw.value, xcd.value, table.reshape(2, -1)
 
     def set_timing(self, on: bool) -> None:
         self._c(self.lib.ether_set_timing(self.h, 1 if on else 0))

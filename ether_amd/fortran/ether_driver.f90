@@ -270,7 +270,7 @@ program ether_gpu_driver
    character(len=400) :: line
    logical :: ok, ising
    type(c_ptr) :: ctx
-   integer(c_int) :: swap_c
+   integer(c_int) :: swap_c, spw, xcdm
    integer(c_int), allocatable :: obc(:)
    integer(8) :: t0, t1, rate
 
@@ -454,6 +454,15 @@ program ether_gpu_driver
          gstart(rep + 1) = rep*nscan
       end do
       call chk(ether_set_ladders(ctx, nrepeat, gstart))
+   end if
+   ! optional launch-shape autotuning: ETHER_AUTOTUNE = sweeps timed per candidate
+   call get_environment_variable('ETHER_AUTOTUNE', line, status=ios)
+   if (ios == 0) then
+      read (line, *, iostat=ios) q
+      if (ios == 0 .and. q > 0) then
+         call chk(ether_autotune(ctx, int(q, c_int), spw, xcdm, c_null_ptr))
+         write (6, '(a,i0,a,i0)') ' ==> autotuned sweep launch: sites per wave ', spw, ', XCD map ', xcdm
+      end if
    end if
    ! generate_supercell state, then fresh_spins (src/ether.f90:155)
    call chk(ether_set_spins(ctx, -1, S0, phi0))

@@ -146,6 +146,15 @@ module ether_gpu
          integer(c_int), intent(inout) :: swap_count
       end function
 
+      integer(c_int) function ether_autotune(ctx, reps, sites_per_wave, xcd_map, table_ms) &
+            bind(C, name="ether_autotune")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: reps
+         integer(c_int), intent(out) :: sites_per_wave, xcd_map
+         type(c_ptr), value :: table_ms
+      end function
+
       integer(c_int) function ether_pt_state(ctx, slot_of, acc, att, trips) bind(C, name="ether_pt_state")
          import :: c_int, c_ptr, c_int64_t
          type(c_ptr), value :: ctx

@@ -16,19 +16,19 @@ window is biased by relaxation drift wherever the chains are still far from
 equilibrium (at T/J << 1 uniform Marsaglia proposals are almost never
 accepted).  The C that actually sets the swap acceptance is the one that
 reproduces the measured acceptance a_k of each neighbour pair through the same
-Gaussian formula:  sqrt(C(T_mid)) = 2 erfcinv(a_k) T_mid / dT_k.  The
-surrogate is trained on that acceptance-consistent C (and on <e>), measured on
-the linear ladder during the first train_mcs MCS of the run.
+Gaussian formula:  sqrt(C(T_mid)) / T_mid = 2 erfcinv(a_k) / dT_k.  These
+densities are kept on a grid over the temperature range (LengthMap); the
+surrogate is trained on log C from that map and on <e>.
 
 Protocol (adaptive_run): the first train_mcs MCS of the equilibration run are
 split into windows.  The first runs on the linear ladder; after each window
-the MLP is retrained on the samples of all windows so far and the temperatures
-of the slots are retuned in place (every replica keeps its state and slot), so
-a pair whose acceptance was too small to measure on one ladder is measured on
-the next.  The remaining MCS run on the last ladder.  Every swap passes the
-exact acceptance test throughout, so the sampled distribution at each
-temperature is unchanged; only the production window after the last retune
-is timed and its round trips counted.
+the map takes the intervals the window measured, the MLP is retrained and the
+temperatures of the slots are retuned in place (every replica keeps its state
+and slot), so a pair whose acceptance was too small to measure on one ladder
+(a lower bound only) is measured on the next.  The remaining MCS run on the
+last ladder.  Every swap passes the exact acceptance test throughout, so the
+sampled distribution at each temperature is unchanged; only the production
+window after the last retune is timed and its round trips counted.
 """
 from __future__ import annotations
 
@@ -56,6 +56,14 @@ class LadderRun:
     train_c: Optional[np.ndarray] = None
     train_acc: Optional[np.ndarray] = None
     mlp_loss: Optional[float] = None
+    metro_acc: Optional[np.ndarray] = None   # (n,) Metropolis acceptance per slot, production
+    source: str = "plain"                     # which ladder ran the production window
+    train_cost: Optional[float] = None        # its round-trip cost measured in training
+
+    @property
+    def cost(self) -> float:
+        """Round-trip cost Lambda of the production window."""
+        return round_trip_cost(self.acc_rate, np.full(len(self.acc_rate), 1e9))
 
     @property
     def round_trips_per_s(self) -> float:
@@ -78,21 +86,48 @@ def _pair_acceptance(eng: E.Engine, n: int, repeat: int):
     return a, tot, int(trips.sum())
 
 
-def acceptance_heat_capacity(temps, acc, attempts):
-    """Heat capacity that reproduces the measured pair acceptance through
-    a = erfc(dT sqrt(C) / (2 T)), at the pair midpoints, interpolated to temps."""
+def length_density(temps, acc, attempts):
+    """Thermodynamic-length density of each interval of a ladder, from its
+    swap acceptance.  For Gaussian energy distributions a = erfc(d/2) with
+    d = int sqrt(C)/T dT over the interval, so lambda = 2 erfcinv(a) / |dT|
+    (sqrt(C)/T at the midpoint for smooth C).  A pair that accepted no swap
+    only bounds d from below; it is flagged bound=True."""
     temps = np.asarray(temps, np.float64)
-    a = np.clip(np.asarray(acc, np.float64), 0.5 / np.maximum(attempts, 1.0), 0.999)
-    x = 2.0 * erfcinv(a)
-    tm = 0.5 * (temps[1:] + temps[:-1])
-    dt = np.abs(np.diff(temps))
-    c = (x * tm / dt) ** 2
-    order = np.argsort(tm)
-    return np.interp(temps, tm[order], c[order])
+    att = np.maximum(np.asarray(attempts, np.float64), 1.0)
+    acc = np.asarray(acc, np.float64)
+    bound = acc * att < 0.5
+    a = np.clip(acc, 0.5 / att, 0.999)
+    lam = 2.0 * erfcinv(a) / np.abs(np.diff(temps))
+    return lam, bound
+
+
+class LengthMap:
+    """sqrt(C)/T on a fixed grid over the temperature range, refined window by
+    window.  A measured interval replaces what earlier windows said on its
+    span; an interval with no accepted swap only raises the value there to
+    its lower bound."""
+
+    def __init__(self, t_lo: float, t_hi: float, n: int = 64):
+        self.T = np.linspace(t_lo, t_hi, n)
+        self.lam = np.full(n, np.nan)
+
+    def add(self, temps, lam, bound) -> None:
+        temps = np.asarray(temps, np.float64)
+        lo = np.minimum(temps[:-1], temps[1:])
+        hi = np.maximum(temps[:-1], temps[1:])
+        for k in range(len(lam)):
+            sel = (self.T >= lo[k]) & (self.T <= hi[k])
+            if bound[k]:
+                self.lam[sel] = np.fmax(self.lam[sel], lam[k])
+            else:
+                self.lam[sel] = lam[k]
+
+    def heat_capacity(self) -> np.ndarray:
+        return (self.lam * self.T) ** 2
 
 
 def learn_ladder(eng: E.Engine, temps, e, c, n_out: Optional[int] = None, iters: int = 4000,
-                 lr: float = 3e-3, seed: int = 1992, weight=None, t_range=None):
+                 lr: float = 3e-3, seed: int = 1992, t_range=None):
     """Train the device MLP on samples (T, e, log C) and place a new ladder of
     n_out temperatures on t_range (default: the span of temps)."""
     temps = np.asarray(temps, np.float64)
@@ -103,8 +138,7 @@ def learn_ladder(eng: E.Engine, temps, e, c, n_out: Optional[int] = None, iters:
     lc = np.log(np.maximum(c, 1e-12))
     y = np.stack([(e - e.mean()) / (e.std() + 1e-30), (lc - lc.mean()) / (lc.std() + 1e-30)],
                  axis=1).astype(np.float32)
-    w = np.ones_like(y) if weight is None else np.repeat(
-        np.asarray(weight, np.float32)[:, None], 2, axis=1)
+    w = np.ones_like(y)
     eng.mlp_init(seed)
     loss = eng.mlp_train(x, y, w, iters, lr)
     T = eng.ladder_place(mid, half, float(lc.std() + 1e-30), t_lo, t_hi, n_out)
@@ -130,17 +164,34 @@ def _window(eng: E.Engine, model: Model, temps, repeat: int, first: int, n_mcs: 
     return a, att, e, sc
 
 
+def round_trip_cost(acc, attempts) -> float:
+    """Lambda = sum_k r_k / (1 - r_k) over the neighbour pairs of a ladder,
+    r_k the measured swap rejection rate (a pair with no accepted swap counts
+    as half an acceptance).  Under the even/odd swap schedule a label needs
+    about 2 + 2 Lambda swap rounds per round trip (Syed, Bouchard-Cote,
+    Deligiannidis and Doucet, J. R. Stat. Soc. B 84 (2022), non-reversible
+    parallel tempering), so the smaller Lambda the more round trips."""
+    att = np.maximum(np.asarray(attempts, np.float64), 1.0)
+    a = np.clip(np.asarray(acc, np.float64), 0.5 / att, 1.0)
+    return float(np.sum((1.0 - a) / a))
+
+
 def adaptive_run(eng: E.Engine, model: Model, temps, repeat: int, mcs: int, coa: int,
                  pt_interval: int, *, learn: bool, train_mcs: int = 2000,
                  windows: int = 4) -> LadderRun:
     """Tempering run of mcs MCS.  The first train_mcs MCS are `windows` equal
     windows; each one measures the pair acceptance and <e> of the ladder it ran
-    on.  learn=True: after every window the MLP is retrained on the samples of
-    all windows so far (window k weighted 2^k, later ladders are closer to the
-    final one) and the slots are retuned to the ladder it places, so each
-    window refines the acceptance-consistent C where the previous ladder was
-    too coarse to measure it.  learn=False: every window runs on `temps`.
-    The remaining mcs - train_mcs MCS are the timed production window."""
+    on.  The first window runs on `temps` (the plain ladder).  learn=True:
+    after every window but the last the length map takes the window's
+    measured intervals, the MLP is retrained on it (log C = 2 log(lambda T) on
+    the map's grid, <e> of the window interpolated) and the slots are retuned
+    to the ladder it places, so each window measures the intervals that the
+    previous ladder was too coarse to measure.  The production window then
+    runs on the measured ladder with the smallest round-trip cost Lambda; the
+    plain ladder is the fallback when no learned ladder beats it (far from
+    equilibrium the acceptance is not the Gaussian one the surrogate
+    assumes).  learn=False: every window runs on `temps`.  The remaining
+    mcs - train_mcs MCS are the timed production window."""
     temps = np.asarray(temps, np.float64)
     n = len(temps)
     t_range = (float(temps.min()), float(temps.max()))
@@ -149,23 +200,28 @@ def adaptive_run(eng: E.Engine, model: Model, temps, repeat: int, mcs: int, coa:
     eng.init_spins(E.INIT_RANDOM)
     win = max(train_mcs // windows, 1)
     cur, sc, first = temps, 0, 1
-    xs, es, cs, ws = [], [], [], []
-    a = c = e = None
+    lmap = LengthMap(*t_range)
+    best, best_cost, source = temps, np.inf, "plain"
+    a = e = c = None
     loss = None
     for k in range(windows):
         length = win if k < windows - 1 else train_mcs - win * (windows - 1)
         a, att, e, sc = _window(eng, model, cur, repeat, first, length, coa, pt_interval, sc)
         first += length
-        c = acceptance_heat_capacity(cur, a, att)
-        xs.append(cur)
-        es.append(e)
-        cs.append(c)
-        ws.append(np.full(n, 2.0 ** k))
-        if learn:
-            cur, loss = learn_ladder(eng, np.concatenate(xs), np.concatenate(es),
-                                     np.concatenate(cs), n_out=n, weight=np.concatenate(ws),
-                                     t_range=t_range)
+        cost = round_trip_cost(a, att)
+        if cost < best_cost:
+            best, best_cost, source = cur, cost, ("plain" if k == 0 else f"learned (window {k})")
+        lam, bound = length_density(cur, a, att)
+        lmap.add(cur, lam, bound)
+        c = lmap.heat_capacity()
+        if learn and k < windows - 1:
+            o = np.argsort(cur)
+            e_grid = np.interp(lmap.T, cur[o], e[o])
+            cur, loss = learn_ladder(eng, lmap.T, e_grid, c, n_out=n, t_range=t_range)
             eng.retune_betas(np.tile(model.betas(cur), repeat))
+    if learn and best is not cur:
+        cur = best
+        eng.retune_betas(np.tile(model.betas(cur), repeat))
     # production window
     eng.set_ladders([k * n for k in range(repeat + 1)])
     eng.reset_moments()
@@ -176,8 +232,12 @@ def adaptive_run(eng: E.Engine, model: Model, temps, repeat: int, mcs: int, coa:
     eng.synchronize()
     wall = time.perf_counter() - t0
     acc, _, trips = _pair_acceptance(eng, n, repeat)
+    mom = eng.moments()
+    metro = np.array([np.mean([mom[g * n + k, E.MOM_ACC] for g in range(repeat)]) / rest
+                      for k in range(n)])
     return LadderRun(cur, wall, rest, trips, acc, float(eng.R) * model.n_sites * rest,
-                     train_e=e, train_c=c, train_acc=a, mlp_loss=loss)
+                     train_e=e, train_c=c, train_acc=a, mlp_loss=loss, metro_acc=metro,
+                     source=source if learn else "plain", train_cost=best_cost)
 
 
 def compare_ladders(model: Model, *, train_mcs: int = 2000, mcs: Optional[int] = None,

@@ -225,6 +225,22 @@ int ether_run(ether_ctx* ctx, int32_t first_step, int32_t n_steps, int32_t mcs_e
  * graphs off (default on). */
 int ether_set_graphs(ether_ctx* ctx, int32_t on);
 
+/* Launch shape of the sweep colour phases (no counterpart in the reference,
+ * whose sweep is one OpenMP loop over sites, src/Metropolis.f90:35-62):
+ * sites_per_wave (1..16) consecutive sites of a colour class per wavefront,
+ * xcd_map 1 = workgroups remapped so that each XCD (own L2) sweeps one
+ * contiguous site range.  Defaults 4 and 1.  Results do not depend on the
+ * shape: every update draws from its own (site, replica, sweep) counter. */
+int ether_set_launch(ether_ctx* ctx, int32_t sites_per_wave, int32_t xcd_map);
+/* Autotuner: times `reps` sweeps for each of sites_per_wave {1,2,4,8,16} x
+ * xcd_map {0,1} on the context's own lattice and replicas, keeps the fastest
+ * (returned through the two nullable pointers), and restores the spins, the
+ * acceptance counters and the sweep counter.  table_ms[10] (nullable)
+ * receives the time per sweep of candidate i = xcd_map*5 + index of
+ * sites_per_wave.  Allocates a temporary copy of the spin planes. */
+int ether_autotune(ether_ctx* ctx, int32_t reps, int32_t* sites_per_wave, int32_t* xcd_map,
+                   double* table_ms);
+
 /* Kernel timing with HIP events recorded on the context's stream around every
  * launch of a kind: 0 Metropolis colour phase, 1 overrelaxation colour phase,
  * 2 measurement (partials + finalize), 3 tempering swap.  Adds no

@@ -1,0 +1,83 @@
+"""Launch shape and autotuner: results must not depend on them.
+
+Every update draws from its own (site, replica, sweep) Philox counter and
+reads only spins of other colour classes, so the sites-per-wave and XCD
+mapping of a colour phase change which wavefront does an update, never its
+arithmetic: spins after K sweeps are bit-identical for every shape.  The
+autotuner must leave the spins, the acceptance counters and the sweep counter
+as it found them, so that the run that follows is the run without it.
+"""
+import numpy as np
+import pytest
+
+from ether_amd import engine as E
+from tests.helpers import pair, random_xyz, say
+
+pytestmark = pytest.mark.gpu
+
+
+def _start(m, eng, R, seed):
+    rng = np.random.default_rng(seed)
+    for r in range(R):
+        if m.tags.ising:
+            S = np.zeros((m.n_sites, 3))
+            S[:, 2] = np.where(rng.random(m.n_sites) < 0.5, 1.0, -1.0)
+            eng.set_spins(r, S)
+        else:
+            S = random_xyz(rng, m.n_sites)
+            eng.set_spins(r, S, np.arctan2(S[:, 1], S[:, 0]))
+
+
+def _spins(eng, R):
+    return np.stack([eng.get_spins(r)[0] for r in range(R)])
+
+
+@pytest.mark.parametrize("name,over,R", [
+    ("sc_heis_afm_32", dict(sc="8 8 8"), 70),
+    ("sc_ising_16", dict(sc="8 8 8"), 300),
+    ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11"), 24),
+])
+def test_sweeps_do_not_depend_on_launch_shape(name, over, R):
+    say(f"launch shapes {name} R={R}")
+    ref = None
+    for spw, xcd in [(4, 1), (1, 0), (2, 1), (8, 0), (16, 1)]:
+        m, eng, _ = pair(name, R, **over)
+        eng.set_betas(m.betas(np.linspace(5.0, 50.0, R)))
+        _start(m, eng, R, 3)
+        eng.set_launch(spw, xcd)
+        acc = eng.sweep(3, want_accept=True)
+        got = (_spins(eng, R), acc)
+        eng.close()
+        if ref is None:
+            ref = got
+        else:
+            assert np.array_equal(got[0], ref[0]), f"spins differ with sites_per_wave={spw} xcd_map={xcd}"
+            assert np.array_equal(got[1], ref[1])
+
+
+@pytest.mark.parametrize("name,over,R", [
+    ("sc_heis_afm_32", dict(sc="8 8 8"), 70),
+    ("sc_ising_16", dict(sc="8 8 8"), 300),
+])
+def test_autotune_restores_state_and_changes_no_result(name, over, R):
+    say(f"autotune {name} R={R}")
+    runs = []
+    for tune in (False, True):
+        m, eng, _ = pair(name, R, **over)
+        eng.set_betas(m.betas(np.linspace(5.0, 50.0, R)))
+        _start(m, eng, R, 4)
+        eng.sweep(2)
+        before = _spins(eng, R)
+        mom_before = eng.moments()
+        if tune:
+            spw, xcd, table = eng.autotune(reps=3)
+            say(f"  chose sites_per_wave={spw} xcd_map={xcd}; ms per sweep {np.round(table, 4)}")
+            assert spw in E.Engine.AUTOTUNE_SPW and xcd in (0, 1)
+            assert np.all(table > 0)
+            assert table[xcd][E.Engine.AUTOTUNE_SPW.index(spw)] == table.min()
+            assert np.array_equal(_spins(eng, R), before)
+            assert np.array_equal(eng.moments(), mom_before)
+        eng.sweep(3)
+        runs.append(_spins(eng, R))
+        eng.close()
+    assert np.array_equal(runs[0], runs[1])

@@ -5,16 +5,22 @@ Workload (default): Heisenberg antiferromagnet on a 32^3 simple-cubic lattice
 (32768 FP32 unit vectors), J = 1 meV, Marsaglia moves, overrelaxation every
 MCS, 19 temperatures 60 -> 6 K x REPEAT 10 = 190 independent chains, all
 resident on the GPU and advanced together; MCS 5000 with 3000 equilibration,
-observables every 10 MCS.  A "step" is one MCS of every chain (OVRR pass +
-Metropolis sweep + the scheduled measurement), exactly the per-MCS schedule of
-the reference's driver.  The timed region is the whole production run of
-`--steps` MCS; `--warmup` untimed MCS run first on a separate start state.
+observables every 10 MCS.  A "step" is one MCS of every chain (OVRR +
+Metropolis sweep + the scheduled measurement), the per-MCS schedule of the
+reference's driver, with the reflection fused into each colour phase of the
+sweep.  The launch shape of the sweep is picked by the autotuner first.  The
+timed region is the whole production run of `--steps` MCS, whole schedule
+periods replayed as HIP graphs; `--warmup` untimed MCS run first on a separate
+start state.
 
 value = Metropolis spin updates / wall second of the timed MCS loop
         (OVRR and measurement time included, not counted as updates).
 roofline = the Metropolis colour-phase kernel: 24 B of own-spin traffic per
         update (FP32 SoA read + write) / its average launch time from HIP
-        events on the engine stream, against 8 TB/s.
+        events on the engine stream (over `--event-steps` more MCS of the same
+        run, launched one by one), against 8 TB/s; traffic = HBM bytes per
+        launch from the committed PMC summary under profiles/.
+pt, pt_trips = the tempering ladder comparisons (linear vs learned).
 cpu_baseline = the CPU oracle's FP64 sweep (+OVRR) of one 32^3 chain on one
         host core, for a bounded sample of about 10 s.
 """
@@ -101,6 +107,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="sc_heis_afm_32")
+    ap.add_argument("--event-steps", type=int, default=500,
+                    help="MCS after the timed region whose kernels are timed with HIP events")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-pt", action="store_true", help="skip the tempering ladder comparison")
@@ -115,7 +123,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the replicas "
-              "of a run are independent and would shard across GPUs, see DESIGN.md", file=sys.stderr)
+              "of a run are independent and would shard across GPUs by whole ladders", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -143,23 +151,32 @@ def main():
         eng.run(1, args.warmup, t.tmcs_eq, coa, ovrr_iv, ovrr_p)
     eng.synchronize()
 
-    # timed production run
+    # timed production run: steps 1..K of the schedule, whole periods replayed
+    # as captured HIP graphs
     eng.set_spins(-1, m.initial_spins, m.initial_phi)
     eng.init_spins(E.INIT_FRESH)
     eng.reset_moments()
     eng.synchronize()
-    eng.set_timing(True)
     t0 = time.perf_counter()
     eng.run(1, args.steps, t.tmcs_eq, coa, ovrr_iv, ovrr_p)
     eng.synchronize()
     wall = time.perf_counter() - t0
+    mom = eng.moments()
+    updates = float(R) * m.n_sites * args.steps
+    value = updates / wall
+
+    # event-timed continuation of the same run (steps K+1..K+E): every kernel
+    # is launched on its own between HIP events on the engine stream, which a
+    # graph replay does not allow; gives the per-launch kernel durations
+    eng.set_timing(True)
+    t1 = time.perf_counter()
+    eng.run(args.steps + 1, args.event_steps, t.tmcs_eq, coa, ovrr_iv, ovrr_p)
+    eng.synchronize()
+    wall_ev = time.perf_counter() - t1
     sweep_ms, sweep_n = eng.timing(E.TIME_SWEEP)
     ovrr_ms, ovrr_n = eng.timing(E.TIME_OVRR)
     meas_ms, meas_n = eng.timing(E.TIME_MEASURE)
     eng.set_timing(False)
-    mom = eng.moments()
-    updates = float(R) * m.n_sites * args.steps
-    value = updates / wall
 
     nc, col, _ = eng.layout()
     per_launch_updates = R * m.n_sites / nc
@@ -192,6 +209,9 @@ def main():
             "kernel": "k_xyz_sweep (one colour phase)",
             "bytes_per_update": B_UPD[m.model_id], "updates_per_launch": per_launch_updates,
             "avg_launch_ms": avg_ms, "launches": sweep_n,
+            "timing": f"HIP events around each launch on the engine stream over MCS "
+                      f"{args.steps + 1}..{args.steps + args.event_steps} of the same run "
+                      f"(launched one by one; {wall_ev / args.event_steps * 1e3:.4f} ms per MCS there)",
         },
         "launch": {"sites_per_wave": spw, "xcd_map": xcd,
                    "autotune_ms_per_sweep": {f"spw{k}_xcd{x}": round(float(tune_ms[x][i]), 5)
@@ -200,6 +220,7 @@ def main():
         "kernel_ms": {"sweep": sweep_ms, "ovrr": ovrr_ms, "measure": meas_ms,
                       "sweep_launches": sweep_n, "ovrr_launches": ovrr_n, "measure_launches": meas_n},
         "acceptance_mean": float(mom[:, E.MOM_ACC].mean() / args.steps),
+        "graphs": "whole periods of P = lcm(coa, OVRR, PT intervals) MCS replayed as captured HIP graphs",
     }
     eng.close()
     if not args.no_pt:

@@ -12,9 +12,12 @@ src/observables_per_repeat.f90) must agree to z = |dx| / sqrt(sem_a^2 + sem_b^2)
     chance);
   - none may have z > 4.5.
 
-The reference itself cannot be built here, so this is the
-2-sigma parity check of the sampled distribution, run against the oracle's
-restatement of the reference's chain.
+The reference itself cannot be built in this image (its `use mpi` needs an
+MPI module in a format the image's Fortran compiler reads, and there is
+none), so this is the 2-sigma parity check of the sampled distribution, run
+against the oracle's restatement of the reference's chain.  With OVRR on, the
+GPU runs the reflection fused into each colour phase while the oracle runs
+the reference's whole-pass order: the check covers that reordering too.
 """
 import numpy as np
 import pytest

@@ -26,6 +26,7 @@ cpu_baseline = the CPU oracle's FP64 sweep (+OVRR) of one 32^3 chain on one
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -67,6 +68,20 @@ def cpu_baseline(model, budget_s=10.0):
     return {"value": sweeps * n / dt, "unit": "spin updates/s", "cores": 1, "kind": "port",
             "sample": f"{sweeps} MCS (OVRR pass + Metropolis sweep) of one {L[0]}x{L[1]}x{L[2]} "
                       f"Heisenberg chain at T=20 K, FP64 C oracle, single thread, {dt:.1f} s"}
+
+
+def describe(name, m, n_temps, R, args, coa, ovrr_iv):
+    t = m.tags
+    kind = "Ising" if t.ising else "Heisenberg"
+    moves = "" if t.ising else (f", ANGLE moves delta {math.degrees(t.dphi):g} deg" if t.angle else ", Marsaglia moves")
+    extra = (", Zeeman" if t.zeeman else "") + (", SIA" if t.sia else "")
+    ovrr = ""
+    if ovrr_iv:
+        ovrr = (f", OVRR every {ovrr_iv} MCS "
+                f"({'split pass' if args.split_ovrr else 'fused into the sweep phases'})")
+    return (f"{name}: {kind} {m.sc[0]}x{m.sc[1]}x{m.sc[2]} cells ({m.n_sites} spins){extra}, "
+            f"{n_temps} T x {t.repeat} repeats = {R} chains{moves}{ovrr}, "
+            f"MCS {args.steps}/{t.tmcs_eq}, coa {coa}")
 
 
 def profiled_traffic(workload):
@@ -115,6 +130,8 @@ def main():
     ap.add_argument("--pt-mcs", type=int, default=20000)
     ap.add_argument("--pt-trips-mcs", type=int, default=200000,
                     help="MCS of the round-trip comparison on 128 spins")
+    ap.add_argument("--no-sq", action="store_true", help="skip the structure-factor timing")
+    ap.add_argument("--sq-calls", type=int, default=5)
     ap.add_argument("--split-ovrr", action="store_true",
                     help="the reference's order: a whole OVRR pass, then the sweep")
     args = ap.parse_args()
@@ -181,7 +198,8 @@ def main():
     nc, col, _ = eng.layout()
     per_launch_updates = R * m.n_sites / nc
     avg_ms = sweep_ms / max(sweep_n, 1)
-    achieved = B_UPD[m.model_id] * per_launch_updates / (avg_ms * 1e-3) / 1e9
+    b_upd = B_UPD[m.model_id] + (8.0 if (m.model_id == 1 and t.angle) else 0.0)
+    achieved = b_upd * per_launch_updates / (avg_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC,
         "value": value,
@@ -196,18 +214,14 @@ def main():
         "dtype": "f32",
         "data": "synthetic (random-start spins, reference in.ether tags of the workload)",
         "config": {
-            "workload": f"{args.workload}: Heisenberg AFM simple cubic {m.sc[0]}x{m.sc[1]}x{m.sc[2]} "
-                        f"({m.n_sites} spins), J=1 meV, {len(temps)} T x {t.repeat} repeats = {R} "
-                        f"chains, Marsaglia moves, OVRR every MCS "
-                        f"({'split pass' if args.split_ovrr else 'fused into the sweep phases'}), "
-                        f"MCS {args.steps}/{t.tmcs_eq}, coa {coa}",
+            "workload": describe(args.workload, m, len(temps), R, args, coa, ovrr_iv),
             "replicas": R, "sites": m.n_sites, "colours": nc, "mcs": args.steps,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args.workload),
-            "kernel": "k_xyz_sweep (one colour phase)",
-            "bytes_per_update": B_UPD[m.model_id], "updates_per_launch": per_launch_updates,
+            "kernel": ("k_ising_sweep" if m.model_id == 0 else "k_xyz_sweep") + " (one colour phase)",
+            "bytes_per_update": b_upd, "updates_per_launch": per_launch_updates,
             "avg_launch_ms": avg_ms, "launches": sweep_n,
             "timing": f"HIP events around each launch on the engine stream over MCS "
                       f"{args.steps + 1}..{args.steps + args.event_steps} of the same run "
@@ -234,9 +248,62 @@ def main():
         # 20 -> 1 K with the same 64 temperatures and schedule, over enough
         # swap rounds for labels to cross a 64-slot ladder.
         out["pt_trips"] = pt_comparison(args.pt_trips_mcs, cells=2, temp="20 1 0.301587301587302")
+    if not args.no_sq:
+        out["sq"] = sq_bench(args.sq_calls)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, args.cpu_budget)
     print(json.dumps(out), flush=True)
+
+
+def sq_bench(calls):
+    """S(q) of every replica of the triangular workload: 96 x 96 cells (9216
+    spins, 9216 commensurate q points), 32 T x 10 repeats = 320 replicas,
+    after 200 MCS from a random start.  Per-axis DFT contractions on the
+    matrix cores (split-BF16, FP32 accumulation).  The device time comes from
+    HIP events around the device part of each call; the error is measured
+    against numpy's FP64 FFT for 4 replicas."""
+    import numpy as np
+    from ether_amd import engine as E
+    from ether_amd import workloads
+    from ether_amd.driver import make_engine
+    m = workloads.model("tri_afm_96")
+    t = m.tags
+    temps = t.temperatures()
+    R = len(temps) * t.repeat
+    eng = make_engine(m, R)
+    eng.set_betas(np.tile(m.betas(temps), t.repeat))
+    eng.init_spins(E.INIT_RANDOM)
+    eng.run(1, 200, 10 ** 6, 10)
+    L = (m.sc[0], m.sc[1], m.sc[2])
+    sq = eng.structure_factor(L)                 # first call allocates and builds the tables
+    eng.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        sq = eng.structure_factor(L)
+    wall = (time.perf_counter() - t0) / calls
+    dev_ms, n = eng.timing(E.TIME_SQ)
+    eng.set_timing(False)
+    worst = 0.0
+    for r in range(4):
+        S, _ = eng.get_spins(r)
+        grid = S.reshape(L[1], L[0], 3)
+        ref = sum(np.abs(np.fft.fft2(grid[:, :, a])) ** 2 for a in range(3)).reshape(-1) / m.n_sites ** 2
+        worst = max(worst, float(np.abs(sq[r] - ref).max() / ref.max()))
+    eng.close()
+    N = m.n_sites
+    nr3 = 3.0 * N * R
+    axes = [Lk for Lk in L if Lk > 1]
+    gemms = [2] + [4] * (len(axes) - 1)          # the first axis has a zero imaginary part
+    flops = sum(g * 2.0 * nr3 * Lk for g, Lk in zip(gemms, axes))
+    ms = dev_ms / max(n, 1)
+    return {"workload": f"tri_afm_96: {L[0]}x{L[1]} cells ({N} spins, {N} q points), {R} replicas",
+            "device_ms_per_call": ms, "wall_ms_per_call": wall * 1e3, "calls": n,
+            "dft_flops": flops, "mfma_flops": 3.0 * flops,
+            "mfma_tflops_per_s": 3.0 * flops / (ms * 1e-3) / 1e12,
+            "direct_sum_flops": 12.0 * N * N * R,
+            "direct_sum_equivalent_tflops_per_s": 12.0 * N * N * R / (ms * 1e-3) / 1e12,
+            "max_error_vs_fp64_fft": worst,
+            "error_note": "max |S_gpu - S_fp64| / max S_fp64 over 4 replicas"}
 
 
 def pt_comparison(mcs, cells=16, temp=None):

@@ -1488,6 +1488,8 @@ void*& ctx_mlp(ether_ctx* c) { return c->mlp; }
 void*& ctx_sq(ether_ctx* c) { return c->sq; }
 int ctx_fail(ether_ctx* c, int code, const std::string& msg) { return fail(c, code, msg); }
 int ctx_model(ether_ctx* c) { return c->model; }
+int ctx_ev_begin(ether_ctx* c, int kind) { return ev_begin(c, kind); }
+void ctx_ev_end(ether_ctx* c, int k) { ev_end(c, k); }
 int ctx_N(ether_ctx* c) { return c->N; }
 int ctx_R(ether_ctx* c) { return c->R; }
 int ctx_Rp(ether_ctx* c) { return c->Rp; }

@@ -14,6 +14,7 @@
 // (v_mfma_f32_16x16x16_bf16), which keeps about 16 mantissa bits.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -25,6 +26,8 @@ hipStream_t ctx_stream(ether_ctx* c);
 void*& ctx_sq(ether_ctx* c);
 int ctx_fail(ether_ctx* c, int code, const std::string& msg);
 int ctx_model(ether_ctx* c);
+int ctx_ev_begin(ether_ctx* c, int kind);
+void ctx_ev_end(ether_ctx* c, int k);
 int ctx_N(ether_ctx* c);
 int ctx_R(ether_ctx* c);
 int ctx_Rp(ether_ctx* c);
@@ -134,9 +137,19 @@ __global__ __launch_bounds__(256) void k_gemm_split_bf16(GemmArgs g) {
             }
 }
 
+// The batch goes on grid.z, which holds at most 65535 blocks: larger batches
+// are launched in chunks with the operand pointers advanced.
 void gemm(const GemmArgs& g, hipStream_t st) {
-    dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, g.batch);
-    hipLaunchKernelGGL(k_gemm_split_bf16, grid, dim3(256), 0, st, g);
+    constexpr int kMaxZ = 65535;
+    for (int b0 = 0; b0 < g.batch; b0 += kMaxZ) {
+        GemmArgs h = g;
+        h.batch = std::min(kMaxZ, g.batch - b0);
+        h.A = g.A + (long long)b0 * g.a_bs;
+        h.B = g.B + (long long)b0 * g.b_bs;
+        h.C = g.C + (long long)b0 * g.c_bs;
+        dim3 grid((h.N + BN - 1) / BN, (h.M + BM - 1) / BM, h.batch);
+        hipLaunchKernelGGL(k_gemm_split_bf16, grid, dim3(256), 0, st, h);
+    }
 }
 
 // gather spins of every replica (replica-major device planes, row stride Rp)
@@ -281,6 +294,7 @@ extern "C" int ether_structure_factor(ether_ctx* c, const int32_t* sc, float* ou
     }
     const long long tot = (long long)N * R;
     const bool ising = ctx_model(c) == ETHER_MODEL_ISING;
+    const int ev = ctx_ev_begin(c, ETHER_TIME_SQ);
     hipLaunchKernelGGL(k_gather, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
                        ctx_spin_plane(c, 0), ctx_spin_plane(c, 1), ctx_spin_plane(c, 2),
                        ising ? ctx_ising(c) : (const int8_t*)nullptr, s->d_map, N, R, ctx_Rp(c),
@@ -300,6 +314,7 @@ extern "C" int ether_structure_factor(ether_ctx* c, const int32_t* sc, float* ou
     if (zi_zero) HC(c, hipMemsetAsync(s->d_z[cur + 1], 0, sizeof(float) * N * R * 3, st));
     hipLaunchKernelGGL(k_sq_norm, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, s->d_z[cur],
                        s->d_z[cur + 1], N, R, (float)(1.0 / ((double)N * N)), s->d_out);
+    ctx_ev_end(c, ev);
     HC(c, hipGetLastError());
     HC(c, hipMemcpyAsync(out, s->d_out, sizeof(float) * N * R, hipMemcpyDeviceToHost, st));
     HC(c, hipStreamSynchronize(st));

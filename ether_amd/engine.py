@@ -23,7 +23,7 @@ MODEL_ISING = 0
 MODEL_XYZ = 1
 INIT_FRESH = 1
 INIT_RANDOM = 2
-TIME_SWEEP, TIME_OVRR, TIME_MEASURE, TIME_PT = 0, 1, 2, 3
+TIME_SWEEP, TIME_OVRR, TIME_MEASURE, TIME_PT, TIME_SQ = 0, 1, 2, 3, 4
 COMPAT_PT_ETOT = 1
 COMPAT_SPLIT_OVRR = 2
 

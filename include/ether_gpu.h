@@ -243,13 +243,15 @@ int ether_autotune(ether_ctx* ctx, int32_t reps, int32_t* sites_per_wave, int32_
 
 /* Kernel timing with HIP events recorded on the context's stream around every
  * launch of a kind: 0 Metropolis colour phase, 1 overrelaxation colour phase,
- * 2 measurement (partials + finalize), 3 tempering swap.  Adds no
+ * 2 measurement (partials + finalize), 3 tempering swap, 4 structure factor
+ * (device part of one ether_structure_factor call).  Adds no
  * synchronisation; ether_get_timing synchronises and reports the summed
  * event time and the launch count since timing was switched on. */
 #define ETHER_TIME_SWEEP   0
 #define ETHER_TIME_OVRR    1
 #define ETHER_TIME_MEASURE 2
 #define ETHER_TIME_PT      3
+#define ETHER_TIME_SQ      4
 int ether_set_timing(ether_ctx* ctx, int32_t on);
 int ether_get_timing(ether_ctx* ctx, int32_t kind, double* total_ms, int64_t* launches);
 /* Waits for all work queued on the context's stream. */

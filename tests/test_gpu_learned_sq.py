@@ -112,3 +112,54 @@ def test_structure_factor_matches_direct_sum(L):
         assert np.all(err <= 1e-4 * max(ref.max(), 1.0 / N) + 1e-6), (r, err.max(), ref.max())
     say(f"S(q) worst abs error vs FP64: {worst:.3e}")
     eng.close()
+
+
+def test_structure_factor_triangular_96_against_fp64_fft():
+    """S(q) at the triangular benchmark size (96 x 96 cells, one site each,
+    9216 q points on the commensurate grid) for 64 replicas, against numpy's
+    FP64 FFT: on a one-site-per-cell supercell q.r = 2 pi (m1 n1/L1 + m2 n2/L2),
+    so S_a(q) is the 2-D DFT of spin component a over the cell indices, and
+    |S_a(q)|^2 does not depend on the sign convention for real spins.  States:
+    random, a 120-degree order with a Bragg peak at K, and a ferromagnet with
+    S(0) = 1.  Split-BF16 products (hi*hi + hi*lo + lo*hi) accumulated in FP32
+    drop lo*lo, about 2^-16 of each product; on a uniform state those errors add
+    up coherently, so the bound is 1e-4 of the largest S(q) of the replica.  The
+    worst error seen is printed."""
+    say("sq triangular 96x96 vs FP64 FFT")
+    R = 64
+    m, eng, _ = pair("tri_afm_96", R)
+    assert m.n_sites == 96 * 96
+    eng.set_betas(m.betas([5.0] * R))
+    rng = np.random.default_rng(11)
+    L1, L2 = 96, 96
+    n1 = np.arange(m.n_sites) % L1
+    n2 = np.arange(m.n_sites) // L1
+    states = []
+    for r in range(R):
+        if r % 3 == 0:
+            S = random_xyz(rng, m.n_sites)
+        elif r % 3 == 1:
+            ang = 2 * np.pi / 3 * ((n1 + 2 * n2) % 3)          # three-sublattice 120-degree state
+            S = np.stack([np.cos(ang), np.sin(ang), 0.05 * rng.normal(size=m.n_sites)], axis=1)
+            S /= np.linalg.norm(S, axis=1)[:, None]
+        else:
+            S = np.tile([0.0, 0.6, 0.8], (m.n_sites, 1))
+        S = S.astype(np.float32).astype(np.float64)
+        eng.set_spins(r, S, np.arctan2(S[:, 1], S[:, 0]))
+        states.append(S)
+    sq = eng.structure_factor((L1, L2, 1))
+    worst_rel = 0.0
+    for r in range(R):
+        grid = states[r].reshape(L2, L1, 3)                     # [n2][n1][a]
+        ref = np.zeros((L2, L1))
+        for a in range(3):
+            ref += np.abs(np.fft.fft2(grid[:, :, a])) ** 2
+        ref = ref.reshape(-1) / m.n_sites ** 2                  # q index m1 + L1*m2
+        err = np.abs(sq[r].astype(np.float64) - ref).max()
+        rel = err / ref.max()
+        worst_rel = max(worst_rel, rel)
+        assert rel <= 1e-4, (r, err, ref.max())
+        if r % 3 == 2:
+            assert abs(sq[r][0] - 1.0) < 1e-4                    # ferromagnet: S(0) = 1
+    say(f"S(q) 96x96: worst error vs FP64 FFT {worst_rel:.2e} of the peak")
+    eng.close()

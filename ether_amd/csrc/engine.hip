@@ -60,6 +60,8 @@ struct ether_ctx {
     std::vector<double> ising_j;
     std::vector<int> ising_Z, ising_stride;
     int ising_P = 1, ising_offset = 0, ising_table = 0;
+    int rec_ncol = 0;     // colour phases to emit while a batch records (its largest count)
+    uint64_t graph_gen = 0;  // bumped whenever captured graphs are dropped
     int xcd_map = 1;      // launch parameters of the sweep phases (ether_set_launch)
     int spw = kDefaultSitesPerWave;
     int meas_tiles = 0, meas_sites = 0, meas_rec = 0;
@@ -913,14 +915,29 @@ extern "C" int ether_synchronize(ether_ctx* c) {
 // --------------------------------------------------------------------------
 // One sweep: a Metropolis phase per colour.  fuse_frac > 0 (XYZ only) makes
 // every phase first reflect each of its sites with probability fuse_frac.
+// While a batch records, every context emits one launch per colour up to the
+// batch's largest colour count (empty ones included), so that the recorded
+// sequences of all lattices line up launch by launch.
+static int phase_count(const ether_ctx* c) {
+    return g_recorder ? std::max(c->ncol, c->rec_ncol) : c->ncol;
+}
+static bool phase_empty(const ether_ctx* c, int q) {
+    return q >= c->ncol || c->cbeg[q + 1] == c->cbeg[q];
+}
+
 static int enqueue_sweep(ether_ctx* c, double fuse_frac = 0.0) {
-    for (int q = 0; q < c->ncol; ++q) {
-        if (c->cbeg[q + 1] == c->cbeg[q]) continue;
+    for (int q = 0; q < phase_count(c); ++q) {
+        const bool empty = phase_empty(c, q);
+        if (empty && !g_recorder) continue;
+        const int qq = std::min(q, c->ncol - 1);
         const int ev = ev_begin(c, ETHER_TIME_SWEEP);
         if (c->model == ETHER_MODEL_ISING) {
-            launch_ising_sweep(ising_args(c, q), c->stream);
+            IsingArgs a = ising_args(c, qq);
+            if (empty) a.c_begin = a.c_end = 0;
+            launch_ising_sweep(a, c->stream);
         } else {
-            XyzArgs a = xyz_args(c, q);
+            XyzArgs a = xyz_args(c, qq);
+            if (empty) a.c_begin = a.c_end = 0;
             if (fuse_frac > 0.0) {
                 a.fuse = 1;
                 a.frac = (float)fuse_frac;
@@ -939,9 +956,11 @@ static int enqueue_ovrr(ether_ctx* c, double passes) {
     double frac = passes - full;
     int pass = 0;
     for (int p = 0; p < full + (frac > 0.0 ? 1 : 0); ++p, ++pass) {
-        for (int q = 0; q < c->ncol; ++q) {
-            if (c->cbeg[q + 1] == c->cbeg[q]) continue;
-            XyzArgs a = xyz_args(c, q);
+        for (int q = 0; q < phase_count(c); ++q) {
+            const bool empty = phase_empty(c, q);
+            if (empty && !g_recorder) continue;
+            XyzArgs a = xyz_args(c, std::min(q, c->ncol - 1));
+            if (empty) a.c_begin = a.c_end = 0;
             a.pass = (uint32_t)pass;
             a.frac = (p < full) ? 1.0f : (float)frac;
             const int ev = ev_begin(c, ETHER_TIME_OVRR);
@@ -1275,6 +1294,7 @@ static int enqueue_step(ether_ctx* c, int32_t stepi, int32_t mcs_eq, int32_t coa
 }
 
 static void drop_graphs(ether_ctx* c) {
+    c->graph_gen++;
     (void)hipStreamSynchronize(c->stream);
     for (auto& g : c->graphs) {
         (void)hipGraphExecDestroy(g.exec);
@@ -1392,6 +1412,386 @@ extern "C" int ether_run(ether_ctx* c, int32_t first_step, int32_t n_steps, int3
 extern "C" int ether_set_graphs(ether_ctx* c, int32_t on) {
     if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
     c->use_graphs = on ? 1 : 0;
+    return ETHER_OK;
+}
+
+// --------------------------------------------------------------------------
+// batches: many lattices (one committed context each) advanced by the same
+// schedule with one launch per kernel kind and template instance
+// --------------------------------------------------------------------------
+struct ether_batch {
+    std::vector<ether_ctx*> ctx;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    int max_ncol = 0;
+    uint32_t** d_ctrs = nullptr;   // the contexts' device counters
+    uint32_t* d_vals = nullptr;    // staging for counter resets
+    char* d_arena = nullptr;       // argument blocks and item tables of launches issued one by one
+    size_t arena_cap = 0;
+    int use_graphs = 1;
+    uint64_t gen = 0;              // sum of the contexts' graph generations when graphs were captured
+    struct Graph {
+        int phase, P, coa, ovrr_iv, pt_iv, n_pt;
+        double ovrr_p;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+        char* d_arena;
+    };
+    std::vector<Graph> graphs;
+};
+
+namespace {
+
+struct BatchedLaunch {
+    int kind, variant;
+    size_t args_off, items_off;
+    int n_items;
+};
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// Lays out the recorded launches of every context as batched launches: level
+// v (the v-th recorded launch of each context) becomes one launch per (kind,
+// variant) present at that level, issued in level order, so each lattice's own
+// launch order is kept.  Workgroup j of a batched launch runs
+// list[(j % 8) * per + j / 8] of the lattice-major list of (argument block,
+// local workgroup) pairs: each XCD gets a contiguous run of one lattice.
+int plan_batch(const std::vector<LaunchRecorder>& recs, std::vector<char>& host,
+               std::vector<BatchedLaunch>& out, std::string& err) {
+    const size_t nlev = recs[0].recs.size();
+    for (const auto& r : recs)
+        if (r.recs.size() != nlev) {
+            err = "ether_batch_run: the lattices of a batch do not run the same schedule";
+            return ETHER_ERR_STATE;
+        }
+    for (size_t v = 0; v < nlev; ++v) {
+        const int kind = recs[0].recs[v].kind;
+        std::vector<int> variants;
+        for (const auto& r : recs) {
+            const LaunchRecord& q = r.recs[v];
+            if (q.kind != kind) {
+                err = "ether_batch_run: the lattices of a batch do not run the same schedule";
+                return ETHER_ERR_STATE;
+            }
+            if (q.units > 0 && std::find(variants.begin(), variants.end(), q.variant) == variants.end())
+                variants.push_back(q.variant);
+        }
+        for (int var : variants) {
+            const size_t aoff = align256(host.size());
+            host.resize(aoff);
+            std::vector<int2> list;
+            int na = 0;
+            for (const auto& r : recs) {
+                const LaunchRecord& q = r.recs[v];
+                if (q.units <= 0 || q.variant != var) continue;
+                const size_t o = host.size();
+                host.resize(o + (size_t)q.bytes);
+                std::memcpy(host.data() + o, &q.u, (size_t)q.bytes);
+                for (int blk = 0; blk < q.units; ++blk) list.push_back(make_int2(na, blk));
+                ++na;
+            }
+            const int n = (int)list.size();
+            const int nb = (n + 7) / 8 * 8, per = nb / 8;
+            const size_t ioff = align256(host.size());
+            host.resize(ioff + sizeof(int2) * (size_t)nb);
+            int2* it = reinterpret_cast<int2*>(host.data() + ioff);
+            for (int j = 0; j < nb; ++j) {
+                const int src = (j % 8) * per + j / 8;
+                it[j] = src < n ? list[src] : make_int2(-1, 0);
+            }
+            out.push_back({kind, var, aoff, ioff, nb});
+        }
+    }
+    return ETHER_OK;
+}
+
+void issue_batch(const std::vector<BatchedLaunch>& ls, const char* d_base, hipStream_t st) {
+    for (const auto& L : ls)
+        launch_batch(L.kind, L.variant, d_base + L.args_off,
+                     reinterpret_cast<const int2*>(d_base + L.items_off), L.n_items, st);
+}
+
+int batch_fail(const std::string& msg, int code) {
+    set_global_error(msg);
+    return code;
+}
+
+}  // namespace
+
+static void batch_drop_graphs(ether_batch* b) {
+    (void)hipStreamSynchronize(b->stream);
+    for (auto& g : b->graphs) {
+        (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphDestroy(g.graph);
+        (void)hipFree(g.d_arena);
+    }
+    b->graphs.clear();
+}
+
+extern "C" int ether_batch_create(ether_ctx* const* ctxs, int32_t n, ether_batch** out) {
+    if (!out) return batch_fail("ether_batch_create: null output", ETHER_ERR_ARG);
+    *out = nullptr;
+    if (!ctxs || n <= 0) return batch_fail("ether_batch_create: no contexts", ETHER_ERR_ARG);
+    for (int l = 0; l < n; ++l) {
+        ether_ctx* c = ctxs[l];
+        if (!c || !c->committed) return batch_fail("ether_batch_create: context not committed", ETHER_ERR_STATE);
+        if (c->model != ctxs[0]->model || c->sia != ctxs[0]->sia)
+            return batch_fail("ether_batch_create: contexts must share the spin model and the SIA switch",
+                              ETHER_ERR_ARG);
+        for (int k = 0; k < l; ++k)
+            if (ctxs[k] == c) return batch_fail("ether_batch_create: a context is listed twice", ETHER_ERR_ARG);
+    }
+    ether_batch* b = new ether_batch();
+    b->ctx.assign(ctxs, ctxs + n);
+    for (ether_ctx* c : b->ctx) b->max_ncol = std::max(b->max_ncol, c->ncol);
+    std::vector<uint32_t*> ptrs(n);
+    for (int l = 0; l < n; ++l) ptrs[l] = b->ctx[l]->d_ctr;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMalloc(&b->d_ctrs, sizeof(uint32_t*) * n);
+    if (e == hipSuccess) e = hipMalloc(&b->d_vals, sizeof(uint32_t) * 3 * n);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_ctrs, ptrs.data(), sizeof(uint32_t*) * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        ether_batch_destroy(b);
+        return batch_fail(std::string("ether_batch_create: ") + hipGetErrorString(e), ETHER_ERR_HIP);
+    }
+    *out = b;
+    return ETHER_OK;
+}
+
+extern "C" void ether_batch_destroy(ether_batch* b) {
+    if (!b) return;
+    if (b->stream) batch_drop_graphs(b);
+    (void)hipFree(b->d_arena);
+    (void)hipFree(b->d_ctrs);
+    (void)hipFree(b->d_vals);
+    if (b->done) (void)hipEventDestroy(b->done);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+extern "C" int ether_batch_set_graphs(ether_batch* b, int32_t on) {
+    if (!b) return batch_fail("null batch", ETHER_ERR_ARG);
+    b->use_graphs = on ? 1 : 0;
+    return ETHER_OK;
+}
+
+// Records the launches of `steps` (stepi, eq) on every context, in the mode
+// the caller set up (eager, or capturing with period-relative counters).
+static int batch_record(ether_batch* b, int32_t first, int32_t count, int32_t eq, int32_t coa,
+                        int ovrr_iv, double ovrr_p, int pt_iv, int sc0,
+                        std::vector<LaunchRecorder>& recs, int* sc_out) {
+    const int n = (int)b->ctx.size();
+    recs.assign(n, LaunchRecorder());
+    for (int l = 0; l < n; ++l) {
+        ether_ctx* c = b->ctx[l];
+        int sc = sc0;
+        g_recorder = &recs[l];
+        int rc = ETHER_OK;
+        for (int32_t s = first; s < first + count && rc == ETHER_OK; ++s)
+            rc = enqueue_step(c, s, eq, coa, ovrr_iv, ovrr_p, pt_iv, &sc);
+        g_recorder = nullptr;
+        if (rc) return batch_fail(c->err, rc);
+        if (l == 0) *sc_out = sc;
+        else if (sc != *sc_out)
+            return batch_fail("ether_batch_run: the lattices of a batch do not run the same schedule",
+                              ETHER_ERR_STATE);
+    }
+    return ETHER_OK;
+}
+
+static int batch_eager_step(ether_batch* b, int32_t stepi, int32_t mcs_eq, int32_t coa, int ovrr_iv,
+                            double ovrr_p, int pt_iv, int* sc) {
+    std::vector<LaunchRecorder> recs;
+    int sc_out = *sc;
+    TRY(batch_record(b, stepi, 1, mcs_eq, coa, ovrr_iv, ovrr_p, pt_iv, *sc, recs, &sc_out));
+    std::vector<char> host;
+    std::vector<BatchedLaunch> ls;
+    std::string err;
+    int rc = plan_batch(recs, host, ls, err);
+    if (rc) return batch_fail(err, rc);
+    if (host.size() > b->arena_cap) {
+        (void)hipStreamSynchronize(b->stream);
+        (void)hipFree(b->d_arena);
+        b->d_arena = nullptr;
+        b->arena_cap = 0;
+        const size_t cap = align256(host.size() * 2);
+        if (hipMalloc(&b->d_arena, cap) != hipSuccess)
+            return batch_fail("ether_batch_run: argument arena allocation failed", ETHER_ERR_HIP);
+        b->arena_cap = cap;
+    }
+    // stream-ordered after the launches that read the previous step's arguments
+    if (hipMemcpyAsync(b->d_arena, host.data(), host.size(), hipMemcpyHostToDevice, b->stream) != hipSuccess)
+        return batch_fail("ether_batch_run: argument upload failed", ETHER_ERR_HIP);
+    issue_batch(ls, b->d_arena, b->stream);
+    if (hipGetLastError() != hipSuccess) return batch_fail("ether_batch_run: launch failed", ETHER_ERR_HIP);
+    *sc = sc_out;
+    return ETHER_OK;
+}
+
+// Capture (once) one period of P MCS of every lattice as one graph, as
+// graph_for does for a single context; the graph ends with an advance of
+// every context's device counters.
+static int batch_graph_for(ether_batch* b, int phase, int P, int coa, int ovrr_iv, double ovrr_p,
+                           int pt_iv, int* index) {
+    for (size_t q = 0; q < b->graphs.size(); ++q) {
+        const auto& g = b->graphs[q];
+        if (g.phase == phase && g.P == P && g.coa == coa && g.ovrr_iv == ovrr_iv &&
+            g.ovrr_p == ovrr_p && g.pt_iv == pt_iv) {
+            *index = (int)q;
+            return ETHER_OK;
+        }
+    }
+    const int n = (int)b->ctx.size();
+    std::vector<uint32_t> step0(n), pt0(n);
+    for (int l = 0; l < n; ++l) {
+        ether_ctx* c = b->ctx[l];
+        step0[l] = c->step;
+        pt0[l] = c->pt_count;
+        c->capturing = true;
+        c->cap_step0 = c->step;
+        c->cap_pt0 = c->pt_count;
+    }
+    std::vector<LaunchRecorder> recs;
+    int n_pt = 0;
+    int rc = batch_record(b, 1, P, phase ? 0 : P + 1, coa, ovrr_iv, ovrr_p, pt_iv, 0, recs, &n_pt);
+    for (int l = 0; l < n; ++l) {
+        ether_ctx* c = b->ctx[l];
+        c->capturing = false;
+        c->step = step0[l];
+        c->pt_count = pt0[l];
+    }
+    if (rc) return rc;
+    std::vector<char> host;
+    std::vector<BatchedLaunch> ls;
+    std::string err;
+    if ((rc = plan_batch(recs, host, ls, err))) return batch_fail(err, rc);
+    char* arena = nullptr;
+    if (hipMalloc(&arena, std::max<size_t>(host.size(), 256)) != hipSuccess)
+        return batch_fail("ether_batch_run: graph argument arena allocation failed", ETHER_ERR_HIP);
+    if (hipMemcpy(arena, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(arena);
+        return batch_fail("ether_batch_run: graph argument upload failed", ETHER_ERR_HIP);
+    }
+    hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        issue_batch(ls, arena, b->stream);
+        launch_ctr_batch(b->d_ctrs, n, nullptr, (uint32_t)P, (uint32_t)n_pt, b->stream);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(b->stream, &graph);
+    if (e == hipSuccess) e = e2;
+    hipGraphExec_t exec = nullptr;
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipFree(arena);
+        return batch_fail(std::string("ether_batch_run: graph capture: ") + hipGetErrorString(e),
+                          ETHER_ERR_HIP);
+    }
+    b->graphs.push_back({phase, P, coa, ovrr_iv, pt_iv, n_pt, ovrr_p, graph, exec, arena});
+    *index = (int)b->graphs.size() - 1;
+    return ETHER_OK;
+}
+
+static int batch_replay(ether_batch* b, int gi, int* sc) {
+    const auto& g = b->graphs[gi];
+    const int n = (int)b->ctx.size();
+    bool reset = false;
+    std::vector<uint32_t> vals(3 * (size_t)n);
+    for (int l = 0; l < n; ++l) {
+        ether_ctx* c = b->ctx[l];
+        vals[3 * l + 0] = c->step;
+        vals[3 * l + 1] = c->pt_count;
+        vals[3 * l + 2] = (uint32_t)*sc;
+        if (c->dev_step != c->step || c->dev_pt != c->pt_count || c->dev_sc != (uint32_t)*sc) reset = true;
+    }
+    if (reset) {
+        if (hipMemcpyAsync(b->d_vals, vals.data(), sizeof(uint32_t) * vals.size(), hipMemcpyHostToDevice,
+                           b->stream) != hipSuccess)
+            return batch_fail("ether_batch_run: counter upload failed", ETHER_ERR_HIP);
+        launch_ctr_batch(b->d_ctrs, n, b->d_vals, 0, 0, b->stream);
+        for (ether_ctx* c : b->ctx) {
+            c->dev_step = c->step;
+            c->dev_pt = c->pt_count;
+            c->dev_sc = (uint32_t)*sc;
+        }
+    }
+    if (hipGraphLaunch(g.exec, b->stream) != hipSuccess)
+        return batch_fail("ether_batch_run: graph launch failed", ETHER_ERR_HIP);
+    for (ether_ctx* c : b->ctx) {
+        c->step += (uint32_t)g.P;
+        c->dev_step += (uint32_t)g.P;
+        c->pt_count += (uint32_t)g.n_pt;
+        c->dev_pt += (uint32_t)g.n_pt;
+        c->dev_sc += (uint32_t)g.n_pt;
+    }
+    *sc += g.n_pt;
+    return ETHER_OK;
+}
+
+extern "C" int ether_batch_run(ether_batch* b, int32_t first_step, int32_t n_steps, int32_t mcs_eq,
+                               int32_t coa, int32_t ovrr_interval, double ovrr_passes,
+                               int32_t pt_interval, int32_t* swap_count) {
+    if (!b) return batch_fail("null batch", ETHER_ERR_ARG);
+    if (coa <= 0) return batch_fail("ether_batch_run: coa must be positive", ETHER_ERR_ARG);
+    if (n_steps < 0) return batch_fail("ether_batch_run: negative n_steps", ETHER_ERR_ARG);
+    ether_ctx* c0 = b->ctx[0];
+    const bool do_ovrr = ovrr_interval > 0 && c0->model == ETHER_MODEL_XYZ && !c0->sia && ovrr_passes > 0.0;
+    const bool do_pt = pt_interval > 0 && c0->n_groups > 0;
+    uint64_t gen = 0;
+    for (ether_ctx* c : b->ctx) {
+        int rc = need_betas(c);
+        if (rc) return batch_fail(c->err, rc);
+        if (c->timing) return batch_fail("ether_batch_run: kernel timing must be off in batched contexts",
+                                         ETHER_ERR_STATE);
+        if ((c->n_groups > 0) != (c0->n_groups > 0))
+            return batch_fail("ether_batch_run: either every lattice of a batch has ladders or none",
+                              ETHER_ERR_STATE);
+        gen += c->graph_gen;
+    }
+    if (gen != b->gen) {   // a context changed what captured graphs bake in
+        batch_drop_graphs(b);
+        b->gen = gen;
+    }
+    for (ether_ctx* c : b->ctx) {
+        (void)hipStreamSynchronize(c->stream);
+        c->rec_ncol = b->max_ncol;
+    }
+    const int ovrr_iv = do_ovrr ? ovrr_interval : 0;
+    const int pt_iv = do_pt ? pt_interval : 0;
+    int sc = swap_count ? *swap_count : 0;
+    auto gcd = [](long long x, long long y) { while (y) { long long t = x % y; x = y; y = t; } return x; };
+    long long P = coa;
+    if (ovrr_iv) P = P / gcd(P, ovrr_iv) * ovrr_iv;
+    if (pt_iv) P = P / gcd(P, pt_iv) * pt_iv;
+    const bool graphs_ok = b->use_graphs && P <= kMaxGraphPeriod;
+    const int32_t last = first_step + n_steps - 1;
+    int32_t stepi = first_step;
+    int rc = ETHER_OK;
+    while (stepi <= last && rc == ETHER_OK) {
+        if (graphs_ok && (stepi - 1) % P == 0 && stepi + P - 1 <= last) {
+            const int phase = stepi > mcs_eq ? 1 : (stepi + P - 1 <= mcs_eq ? 0 : -1);
+            if (phase >= 0) {
+                int gi = -1;
+                rc = batch_graph_for(b, phase, (int)P, coa, ovrr_iv, ovrr_passes, pt_iv, &gi);
+                if (rc == ETHER_OK) rc = batch_replay(b, gi, &sc);
+                stepi += (int32_t)P;
+                continue;
+            }
+        }
+        rc = batch_eager_step(b, stepi, mcs_eq, coa, ovrr_iv, ovrr_passes, pt_iv, &sc);
+        ++stepi;
+    }
+    // later work on each context's own stream waits for the batch
+    (void)hipEventRecord(b->done, b->stream);
+    for (ether_ctx* c : b->ctx) {
+        c->rec_ncol = 0;
+        (void)hipStreamWaitEvent(c->stream, b->done, 0);
+    }
+    if (rc) return rc;
+    if (swap_count) *swap_count = sc;
     return ETHER_OK;
 }
 

@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace ether {
 
 constexpr int kBlock = 256;          // threads per workgroup
@@ -201,6 +203,46 @@ inline int launch_blocks(long long units, int xcd_map) {
 // ELL row width: D rounded up to an unrolled bucket (4,6,8,12,18,24,32), or D
 // itself above 32 (runtime loop).  Rows are padded with class-0 entries.
 int ell_bucket(int D);
+
+// ---------------------------------------------------------------------------
+// Launch recording for batched execution.  While g_recorder is set (by the
+// batch runner, on the calling host thread), the launchers below append the
+// launch they would issue, with its argument block, instead of issuing it.
+// The batch runner then issues one launch per (kind, variant) covering the
+// recorded launches of every lattice (k_*_batch kernels).
+// ---------------------------------------------------------------------------
+enum LaunchKind : int {
+    LK_XYZ_SWEEP = 0,
+    LK_XYZ_OVRR = 1,
+    LK_ISING_SWEEP = 2,
+    LK_MEASURE = 3,
+    LK_FINALIZE = 4,
+    LK_PT = 5,
+};
+struct LaunchRecord {
+    int kind;
+    int variant;   // template instance: sweep (angle << 16) | D; measure (model << 2) | (zee << 1) | sia
+    int units;     // workgroups of the launch on its own, without XCD padding (0: nothing to do)
+    int bytes;     // size of the argument block
+    union {
+        XyzArgs x;
+        IsingArgs i;
+        MeasArgs m;
+        FinArgs f;
+        PtArgs p;
+    } u;
+};
+struct LaunchRecorder {
+    std::vector<LaunchRecord> recs;
+};
+extern thread_local LaunchRecorder* g_recorder;
+// one batched launch: args[n_args] of the record kind, items[n_items] =
+// (argument index or -1, local workgroup)
+void launch_batch(int kind, int variant, const void* d_args, const int2* d_items, int n_items,
+                  hipStream_t st);
+// counters of several contexts: set (vals[3*i..]) or advance by (dstep, dswaps)
+void launch_ctr_batch(uint32_t* const* d_ctrs, int n, const uint32_t* d_vals, uint32_t dstep,
+                      uint32_t dswaps, hipStream_t st);
 
 // launchers (kernels.hip)
 void launch_xyz_sweep(const XyzArgs& a, bool angle, bool zee, bool sia, hipStream_t st);

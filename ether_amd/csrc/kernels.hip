@@ -268,7 +268,7 @@ __device__ __forceinline__ bool reflect_about(float hx, float hy, float hz, floa
 // neighbours are fixed, so both moves use the same field.
 // ---------------------------------------------------------------------------
 template <bool ANGLE, int DM>
-__global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
+__device__ __forceinline__ void xyz_sweep_block(const XyzArgs& a, int bb) {
     __shared__ float4 sJ4[kMaxClasses + 1];
     __shared__ float sZ[3 * kMaxSpecies];
     __shared__ float sD[3 * kMaxSpecies];
@@ -282,7 +282,6 @@ __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
         for (int t = threadIdx.x; t < 3 * a.nsp; t += kBlock) sD[t] = a.dsia[t];
     __syncthreads();
 
-    const int bb = block_index(a.nblocks, a.xcd_map);
     const int g = bb % a.G, chunk = bb / a.G;
     const int w = wave_id(), lane = lane_id();
     const int r = g * kWave + lane;
@@ -353,11 +352,27 @@ __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
     book_acceptance(sAcc, 1, &accepted, g * kWave, a.R, a.slot_of, a.acc);
 }
 
+template <bool ANGLE, int DM>
+__global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
+    xyz_sweep_block<ANGLE, DM>(a, block_index(a.nblocks, a.xcd_map));
+}
+
+// Batched form: workgroup b runs local workgroup items[b].y of the launch
+// described by args[items[b].x] (one lattice's colour phase); x < 0 pads.
+template <bool ANGLE, int DM>
+__global__ __launch_bounds__(kBlock) void k_xyz_sweep_batch(const XyzArgs* __restrict__ args,
+                                                           const int2* __restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    if (it.x < 0) return;
+    const XyzArgs a = args[it.x];
+    xyz_sweep_block<ANGLE, DM>(a, it.y);
+}
+
 // ---------------------------------------------------------------------------
 // Overrelaxation, one colour phase: S' = 2(A.S)/(A.A) A - S, renormalised.
 // ---------------------------------------------------------------------------
 template <bool ANGLE, int DM>
-__global__ __launch_bounds__(kBlock) void k_xyz_ovrr(XyzArgs a) {
+__device__ __forceinline__ void xyz_ovrr_block(const XyzArgs& a, int bb) {
     __shared__ float4 sJ4[kMaxClasses + 1];
     __shared__ float sZ[3 * kMaxSpecies];
     const bool ZEE = a.zee != 0;
@@ -365,7 +380,6 @@ __global__ __launch_bounds__(kBlock) void k_xyz_ovrr(XyzArgs a) {
     if (ZEE)
         for (int t = threadIdx.x; t < 3 * a.nsp; t += kBlock) sZ[t] = a.zvec[t];
     __syncthreads();
-    const int bb = block_index(a.nblocks, a.xcd_map);
     const int g = bb % a.G, chunk = bb / a.G;
     const int w = wave_id(), lane = lane_id();
     const int r = g * kWave + lane;
@@ -401,6 +415,20 @@ __global__ __launch_bounds__(kBlock) void k_xyz_ovrr(XyzArgs a) {
     }
 }
 
+template <bool ANGLE, int DM>
+__global__ __launch_bounds__(kBlock) void k_xyz_ovrr(XyzArgs a) {
+    xyz_ovrr_block<ANGLE, DM>(a, block_index(a.nblocks, a.xcd_map));
+}
+
+template <bool ANGLE, int DM>
+__global__ __launch_bounds__(kBlock) void k_xyz_ovrr_batch(const XyzArgs* __restrict__ args,
+                                                          const int2* __restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    if (it.x < 0) return;
+    const XyzArgs a = args[it.x];
+    xyz_ovrr_block<ANGLE, DM>(a, it.y);
+}
+
 // ---------------------------------------------------------------------------
 // Ising Metropolis, one colour phase.  Lane l of group g owns replicas
 // r0..r0+3 (r0 = 4(64g + l)) of the wave's site as one 32-bit word of int8
@@ -416,7 +444,7 @@ __device__ __forceinline__ void class_add(uint32_t (&d)[kMaxIsingClasses + 1], i
 }
 
 template <int DM>
-__global__ __launch_bounds__(kBlock) void k_ising_sweep(IsingArgs a) {
+__device__ __forceinline__ void ising_sweep_block(const IsingArgs& a, int bb) {
     __shared__ int sStride[kMaxIsingClasses + 1];
     __shared__ uint32_t sThr[kIsingLdsTable];
     __shared__ unsigned sAcc[kBlock * 4];
@@ -424,7 +452,6 @@ __global__ __launch_bounds__(kBlock) void k_ising_sweep(IsingArgs a) {
     if (a.table_in_lds)
         for (int t = threadIdx.x; t < a.R * a.table_per_slot; t += kBlock) sThr[t] = a.thr[t];
     __syncthreads();
-    const int bb = block_index(a.nblocks, a.xcd_map);
     const int g4 = bb % a.G4, chunk = bb / a.G4;
     const int w = wave_id(), lane = lane_id();
     const int rid = g4 * kWave + lane;          // Philox id of this lane's 4 replicas
@@ -506,13 +533,27 @@ __global__ __launch_bounds__(kBlock) void k_ising_sweep(IsingArgs a) {
     book_acceptance(sAcc, 4, accepted, g4 * 4 * kWave, a.R, a.slot_of, a.acc);
 }
 
+template <int DM>
+__global__ __launch_bounds__(kBlock) void k_ising_sweep(IsingArgs a) {
+    ising_sweep_block<DM>(a, block_index(a.nblocks, a.xcd_map));
+}
+
+template <int DM>
+__global__ __launch_bounds__(kBlock) void k_ising_sweep_batch(const IsingArgs* __restrict__ args,
+                                                             const int2* __restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    if (it.x < 0) return;
+    const IsingArgs a = args[it.x];
+    ising_sweep_block<DM>(a, it.y);
+}
+
 // ---------------------------------------------------------------------------
 // measurement partials: wave unit = (tile of sites, replica group of 64)
 // record per (tile, replica): XYZ {E_exch, E_z, E_sia, Mx, My, Mz};
 // Ising {bonds_1..K, zsum_1..nsp, Mz} (integers held exactly in FP64)
 // ---------------------------------------------------------------------------
 template <bool ZEE, bool SIA>
-__global__ __launch_bounds__(kBlock) void k_xyz_measure(MeasArgs a) {
+__device__ __forceinline__ void xyz_measure_block(const MeasArgs& a, int blk) {
     __shared__ float sJ[3 * (kMaxClasses + 1)];
     __shared__ float sZ[3 * kMaxSpecies];
     __shared__ float sD[3 * kMaxSpecies];
@@ -522,7 +563,7 @@ __global__ __launch_bounds__(kBlock) void k_xyz_measure(MeasArgs a) {
     if (SIA)
         for (int t = threadIdx.x; t < 3 * a.nsp; t += kBlock) sD[t] = a.dsia[t];
     __syncthreads();
-    const int unit = blockIdx.x * kWavesPerBlock + wave_id();
+    const int unit = blk * kWavesPerBlock + wave_id();
     if (unit >= a.tiles * a.G) return;
     const int g = unit % a.G, tile = unit / a.G;
     const int r = g * kWave + lane_id();
@@ -556,8 +597,22 @@ __global__ __launch_bounds__(kBlock) void k_xyz_measure(MeasArgs a) {
     for (int k = 0; k < 6; ++k) out[k] = v[k];
 }
 
-__global__ __launch_bounds__(kBlock) void k_ising_measure(MeasArgs a) {
-    const int unit = blockIdx.x * kWavesPerBlock + wave_id();
+template <bool ZEE, bool SIA>
+__global__ __launch_bounds__(kBlock) void k_xyz_measure(MeasArgs a) {
+    xyz_measure_block<ZEE, SIA>(a, blockIdx.x);
+}
+
+template <bool ZEE, bool SIA>
+__global__ __launch_bounds__(kBlock) void k_xyz_measure_batch(const MeasArgs* __restrict__ args,
+                                                             const int2* __restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    if (it.x < 0) return;
+    const MeasArgs a = args[it.x];
+    xyz_measure_block<ZEE, SIA>(a, it.y);
+}
+
+__device__ __forceinline__ void ising_measure_block(const MeasArgs& a, int blk) {
+    const int unit = blk * kWavesPerBlock + wave_id();
     if (unit >= a.tiles * a.G) return;
     const int g = unit % a.G, tile = unit / a.G;
     const int r = g * kWave + lane_id();
@@ -598,6 +653,18 @@ __global__ __launch_bounds__(kBlock) void k_ising_measure(MeasArgs a) {
     out[o] = mz;
 }
 
+__global__ __launch_bounds__(kBlock) void k_ising_measure(MeasArgs a) {
+    ising_measure_block(a, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ising_measure_batch(const MeasArgs* __restrict__ args,
+                                                               const int2* __restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    if (it.x < 0) return;
+    const MeasArgs a = args[it.x];
+    ising_measure_block(a, it.y);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -606,8 +673,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // One wave per replica: fixed-order reduction of the tile partials, energy
 // assembly and the observable() moments.
-__global__ __launch_bounds__(64) void k_finalize(FinArgs a) {
-    const int r = blockIdx.x;
+__device__ __forceinline__ void finalize_block(const FinArgs& a, int blk) {
+    const int r = blk;
     const int lane = threadIdx.x;
     double ex = 0.0, ez = 0.0, es = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
     if (a.model == 1) {
@@ -660,11 +727,23 @@ __global__ __launch_bounds__(64) void k_finalize(FinArgs a) {
     }
 }
 
+__global__ __launch_bounds__(64) void k_finalize(FinArgs a) {
+    finalize_block(a, blockIdx.x);
+}
+
+__global__ __launch_bounds__(64) void k_finalize_batch(const FinArgs* __restrict__ args,
+                                                      const int2* __restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    if (it.x < 0) return;
+    const FinArgs a = args[it.x];
+    finalize_block(a, it.y);
+}
+
 // ---------------------------------------------------------------------------
 // replica exchange: one thread per ladder, slots permuted, spins untouched
 // ---------------------------------------------------------------------------
-__global__ void k_pt_swap(PtArgs a) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void pt_swap_block(const PtArgs& a, int blk) {
+    const int g = blk * blockDim.x + threadIdx.x;
     if (g >= a.n_groups) return;
     const int s0 = a.gstart[g], s1 = a.gstart[g + 1];
     const uint32_t swap_index = a.ctr[1] + a.pt_off;
@@ -701,6 +780,17 @@ __global__ void k_pt_swap(PtArgs a) {
         if (a.trip_state[rh] == 1) a.trip_state[rh] = 2;
     }
     if (nacc) atomicAdd(a.n_accepted, nacc);
+}
+
+__global__ void k_pt_swap(PtArgs a) {
+    pt_swap_block(a, blockIdx.x);
+}
+
+__global__ void k_pt_swap_batch(const PtArgs* __restrict__ args, const int2* __restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    if (it.x < 0) return;
+    const PtArgs a = args[it.x];
+    pt_swap_block(a, it.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -819,7 +909,29 @@ int ell_bucket(int D) {
         default: CALL(0); break;          \
     }
 
+thread_local LaunchRecorder* g_recorder = nullptr;
+
+static int sweep_units(int groups, int n_sites, int spw) {
+    const int chunk = kWavesPerBlock * spw;
+    return n_sites > 0 ? groups * ((n_sites + chunk - 1) / chunk) : 0;
+}
+
+template <class A>
+static void record(int kind, int variant, int units, const A& a) {
+    LaunchRecord r;
+    r.kind = kind;
+    r.variant = variant;
+    r.units = units;
+    r.bytes = (int)sizeof(A);
+    *reinterpret_cast<A*>(&r.u) = a;
+    g_recorder->recs.push_back(r);
+}
+
 void launch_xyz_sweep(const XyzArgs& a, bool angle, bool, bool, hipStream_t st) {
+    if (g_recorder) {
+        record(LK_XYZ_SWEEP, (angle ? 65536 : 0) | a.D, sweep_units(a.G, a.c_end - a.c_begin, a.spw), a);
+        return;
+    }
     dim3 grid(a.nblocks), block(kBlock);
 #define LA(DM) hipLaunchKernelGGL((k_xyz_sweep<true, DM>), grid, block, 0, st, a)
 #define LM(DM) hipLaunchKernelGGL((k_xyz_sweep<false, DM>), grid, block, 0, st, a)
@@ -830,6 +942,10 @@ void launch_xyz_sweep(const XyzArgs& a, bool angle, bool, bool, hipStream_t st) 
 }
 
 void launch_xyz_ovrr(const XyzArgs& a, bool angle, bool, hipStream_t st) {
+    if (g_recorder) {
+        record(LK_XYZ_OVRR, (angle ? 65536 : 0) | a.D, sweep_units(a.G, a.c_end - a.c_begin, a.spw), a);
+        return;
+    }
     dim3 grid(a.nblocks), block(kBlock);
 #define LA(DM) hipLaunchKernelGGL((k_xyz_ovrr<true, DM>), grid, block, 0, st, a)
 #define LM(DM) hipLaunchKernelGGL((k_xyz_ovrr<false, DM>), grid, block, 0, st, a)
@@ -840,6 +956,10 @@ void launch_xyz_ovrr(const XyzArgs& a, bool angle, bool, hipStream_t st) {
 }
 
 void launch_ising_sweep(const IsingArgs& a, hipStream_t st) {
+    if (g_recorder) {
+        record(LK_ISING_SWEEP, a.D, sweep_units(a.G4, a.c_end - a.c_begin, a.spw), a);
+        return;
+    }
 #define LI(DM) hipLaunchKernelGGL((k_ising_sweep<DM>), dim3(a.nblocks), dim3(kBlock), 0, st, a)
     ETHER_DISPATCH_DM(a.D, LI)
 #undef LI
@@ -847,6 +967,11 @@ void launch_ising_sweep(const IsingArgs& a, hipStream_t st) {
 
 void launch_measure(const MeasArgs& a, int model, bool zee, bool sia, hipStream_t st) {
     const long long units = (long long)a.tiles * a.G;
+    if (g_recorder) {
+        record(LK_MEASURE, (model << 2) | ((zee ? 1 : 0) << 1) | (sia ? 1 : 0),
+               (int)((units + kWavesPerBlock - 1) / kWavesPerBlock), a);
+        return;
+    }
     dim3 grid((unsigned)((units + kWavesPerBlock - 1) / kWavesPerBlock)), block(kBlock);
     if (model == 0) {
         hipLaunchKernelGGL(k_ising_measure, grid, block, 0, st, a);
@@ -862,12 +987,77 @@ void launch_measure(const MeasArgs& a, int model, bool zee, bool sia, hipStream_
 }
 
 void launch_finalize(const FinArgs& a, hipStream_t st) {
+    if (g_recorder) {
+        record(LK_FINALIZE, 0, a.R, a);
+        return;
+    }
     hipLaunchKernelGGL(k_finalize, dim3(a.R), dim3(64), 0, st, a);
 }
 
 void launch_pt_swap(const PtArgs& a, hipStream_t st) {
     int blocks = (a.n_groups + 63) / 64;
+    if (g_recorder) {
+        record(LK_PT, 0, blocks, a);
+        return;
+    }
     hipLaunchKernelGGL(k_pt_swap, dim3(blocks), dim3(64), 0, st, a);
+}
+
+void launch_batch(int kind, int variant, const void* d_args, const int2* d_items, int n_items,
+                  hipStream_t st) {
+    if (n_items <= 0) return;
+    const dim3 grid(n_items), block(kBlock);
+    switch (kind) {
+    case LK_XYZ_SWEEP: {
+        const XyzArgs* a = static_cast<const XyzArgs*>(d_args);
+#define LA(DM) hipLaunchKernelGGL((k_xyz_sweep_batch<true, DM>), grid, block, 0, st, a, d_items)
+#define LM(DM) hipLaunchKernelGGL((k_xyz_sweep_batch<false, DM>), grid, block, 0, st, a, d_items)
+        if (variant & 65536) { ETHER_DISPATCH_DM(variant & 65535, LA) }
+        else { ETHER_DISPATCH_DM(variant & 65535, LM) }
+#undef LA
+#undef LM
+        break;
+    }
+    case LK_XYZ_OVRR: {
+        const XyzArgs* a = static_cast<const XyzArgs*>(d_args);
+#define LA(DM) hipLaunchKernelGGL((k_xyz_ovrr_batch<true, DM>), grid, block, 0, st, a, d_items)
+#define LM(DM) hipLaunchKernelGGL((k_xyz_ovrr_batch<false, DM>), grid, block, 0, st, a, d_items)
+        if (variant & 65536) { ETHER_DISPATCH_DM(variant & 65535, LA) }
+        else { ETHER_DISPATCH_DM(variant & 65535, LM) }
+#undef LA
+#undef LM
+        break;
+    }
+    case LK_ISING_SWEEP: {
+        const IsingArgs* a = static_cast<const IsingArgs*>(d_args);
+#define LI(DM) hipLaunchKernelGGL((k_ising_sweep_batch<DM>), grid, block, 0, st, a, d_items)
+        ETHER_DISPATCH_DM(variant, LI)
+#undef LI
+        break;
+    }
+    case LK_MEASURE: {
+        const MeasArgs* a = static_cast<const MeasArgs*>(d_args);
+        switch (variant) {
+        case 0: case 1: case 2: case 3:
+            hipLaunchKernelGGL(k_ising_measure_batch, grid, block, 0, st, a, d_items); break;
+        case 4: hipLaunchKernelGGL((k_xyz_measure_batch<false, false>), grid, block, 0, st, a, d_items); break;
+        case 5: hipLaunchKernelGGL((k_xyz_measure_batch<false, true>), grid, block, 0, st, a, d_items); break;
+        case 6: hipLaunchKernelGGL((k_xyz_measure_batch<true, false>), grid, block, 0, st, a, d_items); break;
+        default: hipLaunchKernelGGL((k_xyz_measure_batch<true, true>), grid, block, 0, st, a, d_items); break;
+        }
+        break;
+    }
+    case LK_FINALIZE:
+        hipLaunchKernelGGL(k_finalize_batch, grid, dim3(64), 0, st,
+                           static_cast<const FinArgs*>(d_args), d_items);
+        break;
+    case LK_PT:
+        hipLaunchKernelGGL(k_pt_swap_batch, grid, dim3(64), 0, st, static_cast<const PtArgs*>(d_args),
+                           d_items);
+        break;
+    default:
+        break;
+    }
 }
 
 void launch_init(const InitArgs& a, hipStream_t st) {
@@ -891,6 +1081,29 @@ __global__ void k_advance_ctr(uint32_t* ctr, uint32_t dstep, uint32_t dswaps) {
     ctr[0] += dstep;
     ctr[1] += dswaps;
     ctr[2] += dswaps;
+}
+
+__global__ void k_ctr_batch(uint32_t* const* ctrs, int n, const uint32_t* vals, uint32_t dstep,
+                            uint32_t dswaps) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t* c = ctrs[i];
+    if (vals) {
+        c[0] = vals[3 * i + 0];
+        c[1] = vals[3 * i + 1];
+        c[2] = vals[3 * i + 2];
+    } else {
+        c[0] += dstep;
+        c[1] += dswaps;
+        c[2] += dswaps;
+    }
+}
+
+void launch_ctr_batch(uint32_t* const* d_ctrs, int n, const uint32_t* d_vals, uint32_t dstep,
+                      uint32_t dswaps, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_ctr_batch, dim3((n + 63) / 64), dim3(64), 0, st, d_ctrs, n, d_vals, dstep,
+                       dswaps);
 }
 
 void launch_set_ctr(uint32_t* ctr, uint32_t step, uint32_t swaps, uint32_t swap_count,

@@ -71,6 +71,11 @@ EXPORTS = [
                             C.c_int32, _vp]),
     ("ether_set_graphs", C.c_int, [_vp, C.c_int32]),
     ("ether_set_launch", C.c_int, [_vp, C.c_int32, C.c_int32]),
+    ("ether_batch_create", C.c_int, [_vp, C.c_int32, C.POINTER(_vp)]),
+    ("ether_batch_run", C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                  C.c_double, C.c_int32, _vp]),
+    ("ether_batch_set_graphs", C.c_int, [_vp, C.c_int32]),
+    ("ether_batch_destroy", None, [_vp]),
     ("ether_autotune", C.c_int, [_vp, C.c_int32, _vp, _vp, _vp]),
     ("ether_set_timing", C.c_int, [_vp, C.c_int32]),
     ("ether_get_timing", C.c_int, [_vp, C.c_int32, _vp, _vp]),
@@ -393,3 +398,39 @@ class Engine:
         self._c(self.lib.ether_ladder_place(self.h, float(x_mid), float(x_half), float(logcv_scale),
                                             float(t_lo), float(t_hi), int(n), _ptr(T)))
         return T
+
+
+class Batch:
+    """Several engines advanced together by one launch per kernel kind
+    (ether_batch_*): each engine is one lattice with its own replicas,
+    temperatures and ladders; results are those of Engine.run on each."""
+
+    def __init__(self, engines):
+        self.lib = load_library()
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e.h.value for e in self.engines])
+        h = C.c_void_p()
+        _check(self.lib.ether_batch_create(C.cast(arr, C.c_void_p), len(self.engines), C.byref(h)))
+        self.h = h
+
+    def run(self, first_step: int, n_steps: int, mcs_eq: int, coa: int, ovrr_interval: int = 0,
+            ovrr_passes: float = 0.0, pt_interval: int = 0, swap_count: int = 0) -> int:
+        sc = C.c_int32(swap_count)
+        _check(self.lib.ether_batch_run(self.h, int(first_step), int(n_steps), int(mcs_eq), int(coa),
+                                        int(ovrr_interval), float(ovrr_passes), int(pt_interval),
+                                        C.byref(sc)))
+        return sc.value
+
+    def set_graphs(self, on: bool) -> None:
+        _check(self.lib.ether_batch_set_graphs(self.h, 1 if on else 0))
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.ether_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

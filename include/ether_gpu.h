@@ -225,6 +225,27 @@ int ether_run(ether_ctx* ctx, int32_t first_step, int32_t n_steps, int32_t mcs_e
  * graphs off (default on). */
 int ether_set_graphs(ether_ctx* ctx, int32_t on);
 
+/* ---------------------------------------------------------------------------
+ * Batches of lattices (no counterpart in the reference, which runs one
+ * lattice per process: src/ether.f90:110-214 over MPI ranks).  Each lattice
+ * is an ordinary committed context with its own sites, couplings, replicas,
+ * temperatures, ladders and seed; a batch advances all of them through the
+ * schedule of ether_run with one launch per kernel kind and template
+ * instance (instead of one per lattice), whole periods replayed as one
+ * captured graph.  Every lattice's result is bit-identical to ether_run on
+ * its context alone.  Requirements: the same spin model and SIA switch,
+ * either all or none with ladders, kernel timing off.  The contexts stay
+ * usable (moments, spins, tempering state); destroy the batch before any of
+ * its contexts.  Errors are read with ether_last_error(NULL, ...).
+ * ------------------------------------------------------------------------- */
+typedef struct ether_batch ether_batch;
+int ether_batch_create(ether_ctx* const* ctxs, int32_t n, ether_batch** out);
+int ether_batch_run(ether_batch* batch, int32_t first_step, int32_t n_steps, int32_t mcs_eq,
+                    int32_t coa, int32_t ovrr_interval, double ovrr_passes, int32_t pt_interval,
+                    int32_t* swap_count);
+int ether_batch_set_graphs(ether_batch* batch, int32_t on);
+void ether_batch_destroy(ether_batch* batch);
+
 /* Launch shape of the sweep colour phases (no counterpart in the reference,
  * whose sweep is one OpenMP loop over sites, src/Metropolis.f90:35-62):
  * sites_per_wave (1..16) consecutive sites of a colour class per wavefront,

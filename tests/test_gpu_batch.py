@@ -1,0 +1,87 @@
+"""Batches of uneven lattices: one launch per kernel kind for all of them.
+
+Each lattice is an ordinary engine (its own sites, couplings, replicas,
+temperatures, ladders and seed).  A Batch advances all of them through the
+schedule of ether_run with batched launches and graph replays; every
+lattice's spins, moments and tempering state must be bit-identical to the
+same engine run on its own.
+"""
+import numpy as np
+import pytest
+
+from ether_amd import engine as E
+from ether_amd import workloads
+from ether_amd.driver import attach_neighbours, make_engine
+from tests.helpers import random_xyz, say
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(name, R, seed, over, temps):
+    m = workloads.model(name, **over)
+    attach_neighbours(m)
+    eng = make_engine(m, R, seed=seed)
+    eng.set_betas(m.betas(temps))
+    eng.set_ladders([0, R])
+    rng = np.random.default_rng(seed)
+    for r in range(R):
+        if m.tags.ising:
+            S = np.zeros((m.n_sites, 3))
+            S[:, 2] = np.where(rng.random(m.n_sites) < 0.5, 1.0, -1.0)
+            eng.set_spins(r, S)
+        else:
+            S = random_xyz(rng, m.n_sites)
+            eng.set_spins(r, S, np.arctan2(S[:, 1], S[:, 0]))
+    return m, eng
+
+
+def _state(eng):
+    spins = np.stack([eng.get_spins(r)[0] for r in range(eng.R)])
+    return spins, eng.moments(), [np.asarray(x) for x in eng.pt_state()]
+
+
+XYZ = [
+    ("sc_heis_afm_32", 20, 11, dict(sc="6 6 6"), np.linspace(60, 6, 20)),
+    ("tri_afm_96", 13, 12, dict(sc="12 12 1", sia=".F."), np.linspace(20, 2, 13)),       # ANGLE, 3 colours
+    ("pyro_pt_16", 16, 13, dict(sc="2 2 2", pt=".T. 11"), np.geomspace(20, 1, 16)),        # 4 colours, D 18
+    ("sc_heis_afm_32", 70, 14, dict(sc="4 4 4"), np.linspace(40, 4, 70)),                  # two replica groups
+]
+ISING = [
+    ("sc_ising_16", 30, 21, dict(sc="6 6 6"), np.linspace(100, 10, 30)),
+    ("pyro_pt_16", 20, 22, dict(sc="2 2 2", model="ising", para=".T. 1", pt=".T. 11",
+                                zeeman=".T. 0 0 0.3"), np.linspace(6, 2, 20)),
+]
+
+
+@pytest.mark.parametrize("cases,graphs", [(XYZ, True), (XYZ, False), (ISING, True)])
+def test_batch_equals_lattices_run_alone(cases, graphs):
+    say(f"batch of {len(cases)} lattices, graphs={graphs}")
+    alone, batched = [], []
+    for name, R, seed, over, temps in cases:
+        m, a = _engine(name, R, seed, over, temps)
+        _, b = _engine(name, R, seed, over, temps)
+        alone.append((m, a))
+        batched.append(b)
+    batch = E.Batch(batched)
+    batch.set_graphs(graphs)
+    # two calls: 37 MCS (a partial period at the end) and 63 MCS; OVRR every MCS
+    # (fused), samples every 5 MCS after 20, swaps every 5 MCS
+    sc = 0
+    sc = batch.run(1, 37, 20, 5, 1, 1.0, 5, sc)
+    sc = batch.run(38, 63, 20, 5, 1, 1.0, 5, sc)
+    for m, a in alone:
+        sca = a.run(1, 37, 20, 5, 1, 1.0, 5, 0)
+        sca = a.run(38, 63, 20, 5, 1, 1.0, 5, sca)
+        assert sca == sc
+    for (m, a), b in zip(alone, batched):
+        sa, ma, pa = _state(a)
+        sb, mb, pb = _state(b)
+        assert np.array_equal(sa, sb), f"{m.tags.model} N={m.n_sites}: spins differ"
+        assert np.array_equal(ma, mb), f"{m.tags.model} N={m.n_sites}: moments differ"
+        for x, y in zip(pa, pb):
+            assert np.array_equal(x, y), f"{m.tags.model} N={m.n_sites}: tempering state differs"
+        assert ma[:, E.MOM_N].sum() > 0
+    batch.close()
+    for (m, a), b in zip(alone, batched):
+        a.close()
+        b.close()

@@ -131,6 +131,8 @@ def main():
     ap.add_argument("--pt-trips-mcs", type=int, default=200000,
                     help="MCS of the round-trip comparison on 128 spins")
     ap.add_argument("--no-sq", action="store_true", help="skip the structure-factor timing")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven workload")
+    ap.add_argument("--batch-mcs", type=int, default=100)
     ap.add_argument("--sq-calls", type=int, default=5)
     ap.add_argument("--split-ovrr", action="store_true",
                     help="the reference's order: a whole OVRR pass, then the sweep")
@@ -250,6 +252,8 @@ def main():
         out["pt_trips"] = pt_comparison(args.pt_trips_mcs, cells=2, temp="20 1 0.301587301587302")
     if not args.no_sq:
         out["sq"] = sq_bench(args.sq_calls)
+    if not args.no_batch:
+        out["batch"] = batch_bench(args.batch_mcs)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, args.cpu_budget)
     print(json.dumps(out), flush=True)
@@ -304,6 +308,78 @@ def sq_bench(calls):
             "direct_sum_equivalent_tflops_per_s": 12.0 * N * N * R / (ms * 1e-3) / 1e12,
             "max_error_vs_fp64_fft": worst,
             "error_note": "max |S_gpu - S_fp64| / max S_fp64 over 4 replicas"}
+
+
+def batch_bench(mcs, n_lattices=256):
+    """The batched uneven workload: 256 lattices (simple-cubic Ising and
+    Heisenberg, triangular, pyrochlore; 5.35 M spins, 8-32 replicas each,
+    114 M spin updates per MCS), each with its own tempering ladder (swaps
+    every 5 MCS) and a sample every 10 MCS.  Batched: two batches (Ising,
+    Heisenberg) issue one launch per kernel kind and template for all their
+    lattices, whole periods replayed as graphs.  Plain: the same engines run
+    one after another with ether_run (one launch per lattice per kernel, each
+    with its own graphs).  Both time `mcs` MCS after a 10-MCS warmup."""
+    import numpy as np
+    from ether_amd import engine as E
+    from ether_amd import workloads
+    from ether_amd.driver import attach_neighbours, make_engine
+    t_setup = time.perf_counter()
+    specs = workloads.c5_lattices(1992, n_lattices)
+    engines, updates = {"ising": [], "xyz": []}, 0.0
+    for sp in specs:
+        m = workloads.c5_model(sp)
+        attach_neighbours(m)
+        eng = make_engine(m, sp["R"], seed=sp["seed"])
+        eng.set_betas(m.betas(workloads.c5_temperatures(sp)))
+        eng.set_ladders([0, sp["R"]])
+        eng.init_spins(E.INIT_RANDOM)
+        engines["ising" if m.tags.ising else "xyz"].append(eng)
+        updates += float(m.n_sites) * sp["R"]
+    setup_s = time.perf_counter() - t_setup
+    allg = engines["ising"] + engines["xyz"]
+    batches = [E.Batch(v) for v in engines.values() if v]
+    sched = dict(mcs_eq=0, coa=10, pt=5)
+
+    def run_batched(first, n):
+        for b in batches:
+            b.run(first, n, sched["mcs_eq"], sched["coa"], 0, 0.0, sched["pt"])
+
+    def run_plain(first, n):
+        for e in allg:
+            e.run(first, n, sched["mcs_eq"], sched["coa"], 0, 0.0, sched["pt"])
+
+    def sync():
+        for e in allg:
+            e.synchronize()
+
+    step = 1
+    run_batched(step, 10)
+    step += 10
+    sync()
+    t0 = time.perf_counter()
+    run_batched(step, mcs)
+    sync()
+    t_batched = time.perf_counter() - t0
+    step += mcs
+    run_plain(step, 10)
+    step += 10
+    sync()
+    t0 = time.perf_counter()
+    run_plain(step, mcs)
+    sync()
+    t_plain = time.perf_counter() - t0
+    for b in batches:
+        b.close()
+    for e in allg:
+        e.close()
+    return {"workload": f"{len(specs)} lattices (simple-cubic Ising/Heisenberg 8..48, triangular "
+                        f"24..128, pyrochlore 4..12 cells), {int(updates)} spin updates per MCS, own "
+                        f"ladders of 8-32 replicas, swaps every 5 MCS, sample every 10 MCS; {mcs} MCS",
+            "batched_spin_updates_per_s": updates * mcs / t_batched,
+            "plain_spin_updates_per_s": updates * mcs / t_plain,
+            "speedup": t_plain / t_batched,
+            "batched_ms_per_mcs": t_batched / mcs * 1e3, "plain_ms_per_mcs": t_plain / mcs * 1e3,
+            "lattices": {k: len(v) for k, v in engines.items()}, "setup_s": setup_s}
 
 
 def pt_comparison(mcs, cells=16, temp=None):

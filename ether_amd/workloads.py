@@ -11,12 +11,19 @@ same reader as a user's run directory.
   pyro_pt_16        pyrochlore 16^3 cubic cells x 16 sites, J1 = 1, J2 = 0.05 meV,
                     tempering with 64 replicas, swaps every 11 MCS, T 1 -> 0.05 K,
                     MCS 20000/5000, REPEAT 4
+  c5 (c5_lattices)  256 uneven lattices for one batch: simple-cubic Ising and
+                    Heisenberg (edge 8..48), triangular (24..128 squared),
+                    pyrochlore (4..12 cubed cells), J = +-U(0.5, 1.5) meV, each
+                    with its own ladder of 8..32 replicas from T0 to 2 T0,
+                    T0 = U(0.2, 2) |J| / kB
 Every one of them can be shrunk (sc, mcs, repeat, temperatures) for tests.
 """
 from __future__ import annotations
 
 import math
-from typing import Dict, Optional
+from typing import Dict, List, Optional
+
+import numpy as np
 
 from .inputs import Model, build_model, parse_in_ether, parse_j_exchange, parse_sia, parse_structure
 
@@ -130,3 +137,67 @@ def write_dir(name: str, path: str, **over) -> None:
     for k, v in f.items():
         with open(os.path.join(path, names[k]), "w") as fh:
             fh.write(v)
+
+
+# ---------------------------------------------------------------------------
+# The batched uneven workload
+# ---------------------------------------------------------------------------
+KB_MEV_PER_K = 0.08617333262
+C5_KINDS = ("sc_ising", "sc_heis", "tri", "pyro")
+
+
+def c5_lattices(seed: int = 1992, n: int = 256) -> List[dict]:
+    """The lattices of the batched workload, drawn with numpy's PCG64 seeded
+    with `seed`, in this order per lattice: kind (uniform over C5_KINDS), size
+    (simple-cubic edge U{8..48}, triangular edge U{24..128}, pyrochlore cells
+    per edge U{4..12}), |J| ~ U(0.5, 1.5) meV with a uniform sign (negative =
+    ferromagnetic in ETHER's E = sum J S.S), t ~ U(0.2, 2), replicas
+    R ~ U{8..32}.  The ladder runs geometrically from T0 = t |J| / kB to 2 T0."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        kind = C5_KINDS[int(rng.integers(4))]
+        if kind in ("sc_ising", "sc_heis"):
+            size = int(rng.integers(8, 49))
+        elif kind == "tri":
+            size = int(rng.integers(24, 129))
+        else:
+            size = int(rng.integers(4, 13))
+        J = float(rng.uniform(0.5, 1.5)) * (1.0 if rng.random() < 0.5 else -1.0)
+        t = float(rng.uniform(0.2, 2.0))
+        R = int(rng.integers(8, 33))
+        out.append(dict(kind=kind, size=size, J=J, t=t, R=R, seed=seed * 1000 + k))
+    return out
+
+
+def c5_files(spec: dict) -> Dict[str, str]:
+    """Input files of one lattice of the batched workload."""
+    kind, L, J = spec["kind"], spec["size"], spec["J"]
+    tags = dict(model="ising" if kind == "sc_ising" else "xyz", mcs="1000 100", temp="1 1 1",
+                spin="1", species="A", bc="c c c", repeat="1", coa="10", seed=str(spec["seed"]))
+    if kind != "sc_ising":
+        tags["angle"] = ".F. 0"
+    if kind in ("sc_ising", "sc_heis"):
+        tags["sc"] = f"{L} {L} {L}"
+        files_ = dict(structure=SC_STRUCTURE, j_exchange=f"1\n1.0 1\nA A {J} {J} {J}\n")
+    elif kind == "tri":
+        tags["sc"] = f"{L} {L} 1"
+        files_ = dict(structure=TRI_STRUCTURE, j_exchange=f"1\n1.0 1\nA A {J} {J} {J}\n")
+    else:
+        tags["sc"] = f"{L} {L} {L}"
+        files_ = dict(structure=PYRO_STRUCTURE, j_exchange=f"1\n{PYRO_D1:.6f} 1\nA A {J} {J} {J}\n")
+    files_["in.ether"] = _tags(tags)
+    return files_
+
+
+def c5_model(spec: dict) -> Model:
+    f = c5_files(spec)
+    st = parse_structure(f["structure"])
+    t = parse_in_ether(f["in.ether"], st.n_species)
+    return build_model(t, st, parse_j_exchange(f["j_exchange"]), None, None)
+
+
+def c5_temperatures(spec: dict) -> np.ndarray:
+    """The lattice's ladder in K, hottest first."""
+    t0 = spec["t"] * abs(spec["J"]) / KB_MEV_PER_K
+    return np.geomspace(2.0 * t0, t0, spec["R"])

@@ -400,6 +400,7 @@ def pt_comparison(mcs, cells=16, temp=None):
     temps = m.tags.temperatures()
     t0 = time.perf_counter()
     lin, lrn = ladder.compare_ladders(m, train_mcs=2000, mcs=mcs)
+    plain = ladder.plain_run(m, train_mcs=2000, mcs=mcs)
     total = time.perf_counter() - t0
 
     def rep(run):
@@ -419,7 +420,11 @@ def pt_comparison(mcs, cells=16, temp=None):
                         f"{len(temps)} T {temps[0]:g}->{temps[-1]:g} K x {m.tags.repeat} repeats, "
                         f"swaps every {m.tags.exchange_interval} MCS, MCS {mcs}; "
                         f"2000 training MCS, then the ladder under test",
-            "linear": rep(lin), "learned": rep(lrn), "mlp_final_loss": lrn.mlp_loss,
+            "linear": rep(lin), "learned": rep(lrn),
+            "plain_per_ladder": rep(plain),
+            "plain_note": "one context per repeat (64 replicas each), linear ladder, launched "
+                          "one lattice at a time; linear/learned run all repeats in one context",
+            "mlp_final_loss": lrn.mlp_loss,
             "gain_round_trips_per_s": (lrn.round_trips_per_s / lin.round_trips_per_s
                                        if lin.round_trips_per_s > 0 else None),
             "total_s": total}

@@ -64,6 +64,8 @@ struct ether_ctx {
     uint64_t graph_gen = 0;  // bumped whenever captured graphs are dropped
     int xcd_map = 1;      // launch parameters of the sweep phases (ether_set_launch)
     int spw = kDefaultSitesPerWave;
+    int lps_shift = 6;    // lanes per site of the sweep (1 << lps_shift); set at commit
+    int lanes_req = 0;    // ether_set_lanes (0: chosen from R)
     int meas_tiles = 0, meas_sites = 0, meas_rec = 0;
     // device
     uint32_t* d_ell = nullptr;
@@ -365,6 +367,43 @@ int greedy_colour(int N, const std::vector<int>& adj_ptr, const std::vector<int>
 
 }  // namespace
 
+// Lanes per site of the sweep: the smallest of {64, 32, 16 (, 8 for Ising)}
+// lanes that holds every replica of a site (an Ising lane holds 4 replicas),
+// 64 when the replicas need more than one wave.
+static int auto_lanes_shift(int model, int R) {
+    const int need = model == ETHER_MODEL_ISING ? (R + 3) / 4 : R;
+    const int lo = model == ETHER_MODEL_ISING ? 3 : 4;
+    for (int sft = lo; sft < 6; ++sft)
+        if (need <= (1 << sft)) return sft;
+    return 6;
+}
+static int lanes_shift_of(int lanes) {
+    int sft = 0;
+    while ((1 << sft) < lanes) ++sft;
+    return sft;
+}
+
+extern "C" int ether_set_lanes(ether_ctx* c, int32_t lanes_per_site) {
+    if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
+    const int lo = c->model == ETHER_MODEL_ISING ? 8 : 16;
+    const bool pow2 = lanes_per_site > 0 && (lanes_per_site & (lanes_per_site - 1)) == 0;
+    if (lanes_per_site != 0 && (!pow2 || lanes_per_site < lo || lanes_per_site > kWave))
+        return fail(c, ETHER_ERR_ARG, "ether_set_lanes: lanes per site must be 0 (automatic) or a power "
+                                      "of two from 16 (8 for Ising) to 64");
+    if (lanes_per_site != 0 && lanes_per_site < kWave) {
+        const int per = c->model == ETHER_MODEL_ISING ? 4 : 1;
+        if (lanes_per_site * per < c->R)
+            return fail(c, ETHER_ERR_ARG, "ether_set_lanes: fewer lanes per site than replicas");
+    }
+    c->lanes_req = lanes_per_site;
+    if (c->committed) {
+        const int sft = lanes_per_site ? lanes_shift_of(lanes_per_site) : auto_lanes_shift(c->model, c->R);
+        if (sft != c->lps_shift) drop_graphs(c);
+        c->lps_shift = sft;
+    }
+    return ETHER_OK;
+}
+
 extern "C" int ether_commit(ether_ctx* c) {
     if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
     if (c->committed) return fail(c, ETHER_ERR_STATE, "context already committed");
@@ -559,6 +598,8 @@ extern "C" int ether_commit(ether_ctx* c) {
         return fail(c, ETHER_ERR_LIMIT, "ether_commit: sites x padded replicas exceeds 2^29 "
                                       "(one FP32 spin plane must stay under 2 GiB)");
     const int G = c->Rp / kWave;
+    c->lps_shift = auto_lanes_shift(c->model, R);
+    if (c->lanes_req) c->lps_shift = lanes_shift_of(c->lanes_req);
     // measurement: about 4096 wave units per pass, at least 8 sites each
     c->meas_sites = std::max(8, (int)(((long long)N * G + 4095) / 4096));
     c->meas_tiles = (N + c->meas_sites - 1) / c->meas_sites;
@@ -776,7 +817,8 @@ static XyzArgs xyz_args(ether_ctx* c, int colour) {
     a.c_end = c->cbeg[colour + 1];
     a.xcd_map = c->xcd_map;
     a.spw = c->spw;
-    a.nblocks = phase_blocks(a.c_end - a.c_begin, a.G, a.xcd_map, a.spw);
+    a.lps_shift = c->lps_shift;
+    a.nblocks = phase_blocks(a.c_end - a.c_begin, a.G, a.xcd_map, a.spw << (6 - a.lps_shift));
     a.slot_of = c->d_slot_of;
     a.beta_slot = c->d_beta_f;
     a.seed = c->seed;
@@ -812,7 +854,8 @@ static IsingArgs ising_args(ether_ctx* c, int colour) {
     a.c_end = c->cbeg[colour + 1];
     a.xcd_map = c->xcd_map;
     a.spw = c->spw;
-    a.nblocks = phase_blocks(a.c_end - a.c_begin, a.G4, a.xcd_map, a.spw);
+    a.lps_shift = c->lps_shift;
+    a.nblocks = phase_blocks(a.c_end - a.c_begin, a.G4, a.xcd_map, a.spw << (6 - a.lps_shift));
     a.table_in_lds = ((long long)c->R * c->ising_table <= kIsingLdsTable) ? 1 : 0;
     a.slot_of = c->d_slot_of;
     a.seed = c->seed;
@@ -961,6 +1004,8 @@ static int enqueue_ovrr(ether_ctx* c, double passes) {
             if (empty && !g_recorder) continue;
             XyzArgs a = xyz_args(c, std::min(q, c->ncol - 1));
             if (empty) a.c_begin = a.c_end = 0;
+            a.lps_shift = 6;   // the overrelaxation kernel updates one site per wave
+            a.nblocks = phase_blocks(a.c_end - a.c_begin, a.G, a.xcd_map, a.spw);
             a.pass = (uint32_t)pass;
             a.frac = (p < full) ? 1.0f : (float)frac;
             const int ev = ev_begin(c, ETHER_TIME_OVRR);

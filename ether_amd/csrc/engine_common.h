@@ -52,7 +52,8 @@ struct XyzArgs {
     int c_end;
     int nblocks;
     int xcd_map;
-    int spw;               // sites per wave
+    int spw;               // steps per wave (sites per wave = spw << (6 - lps_shift))
+    int lps_shift;         // lanes per site = 1 << lps_shift (6: one site per wave)
     const int* slot_of;
     const float* beta_slot;
     uint64_t seed;
@@ -86,7 +87,8 @@ struct IsingArgs {
     int c_end;
     int nblocks;
     int xcd_map;
-    int spw;               // sites per wave
+    int spw;               // steps per wave (sites per wave = spw << (6 - lps_shift))
+    int lps_shift;         // lanes per site = 1 << lps_shift, 4 replicas per lane (6: one site)
     const int* slot_of;
     uint64_t seed;
     const uint32_t* ctr;
@@ -221,7 +223,8 @@ enum LaunchKind : int {
 };
 struct LaunchRecord {
     int kind;
-    int variant;   // template instance: sweep (angle << 16) | D; measure (model << 2) | (zee << 1) | sia
+    int variant;   // template instance: sweep (lanes << 17) | (angle << 16) | D;
+                   // measure (model << 2) | (zee << 1) | sia
     int units;     // workgroups of the launch on its own, without XCD padding (0: nothing to do)
     int bytes;     // size of the argument block
     union {

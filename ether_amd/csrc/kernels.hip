@@ -157,8 +157,10 @@ __device__ __forceinline__ void angle_move(u32x4 w, float dphi, float phi_old, f
 }
 
 // per-lane counters of the four waves of a block, summed into the slot counters
+// Lane l of every wave of the block holds the counts of replicas
+// r_lane + q (q < nrep_lane); r_lane is the same in every wave of the block.
 __device__ __forceinline__ void book_acceptance(unsigned* sAcc, int nrep_lane, const unsigned* mine,
-                                                int r_first, int R, const int* slot_of,
+                                                int r_lane, int R, const int* slot_of,
                                                 unsigned long long* acc) {
     const int w = wave_id(), lane = lane_id();
     for (int q = 0; q < nrep_lane; ++q) sAcc[(w * kWave + lane) * nrep_lane + q] = mine[q];
@@ -167,7 +169,7 @@ __device__ __forceinline__ void book_acceptance(unsigned* sAcc, int nrep_lane, c
         for (int q = 0; q < nrep_lane; ++q) {
             unsigned t = 0;
             for (int v = 0; v < kWavesPerBlock; ++v) t += sAcc[(v * kWave + lane) * nrep_lane + q];
-            const int r = r_first + lane * nrep_lane + q;
+            const int r = r_lane + q;
             if (t && r < R) atomicAdd(acc + slot_of[r], (unsigned long long)t);
         }
     }
@@ -349,7 +351,7 @@ __device__ __forceinline__ void xyz_sweep_block(const XyzArgs& a, int bb) {
         }
     }
     // acceptance is booked to the temperature slot, as the reference books it per rank
-    book_acceptance(sAcc, 1, &accepted, g * kWave, a.R, a.slot_of, a.acc);
+    book_acceptance(sAcc, 1, &accepted, r, a.R, a.slot_of, a.acc);
 }
 
 template <bool ANGLE, int DM>
@@ -530,7 +532,7 @@ __device__ __forceinline__ void ising_sweep_block(const IsingArgs& a, int bb) {
         }
         if (flip) *own = v ^ flip;
     }
-    book_acceptance(sAcc, 4, accepted, g4 * 4 * kWave, a.R, a.slot_of, a.acc);
+    book_acceptance(sAcc, 4, accepted, r0, a.R, a.slot_of, a.acc);
 }
 
 template <int DM>
@@ -545,6 +547,303 @@ __global__ __launch_bounds__(kBlock) void k_ising_sweep_batch(const IsingArgs* _
     if (it.x < 0) return;
     const IsingArgs a = args[it.x];
     ising_sweep_block<DM>(a, it.y);
+}
+
+// ---------------------------------------------------------------------------
+// Several sites per wavefront, for contexts with few replicas: lanes per site
+// LPS = 1 << a.lps_shift (XYZ: 16 or 32 replicas per site; Ising: 8, 16 or 32
+// lanes of 4 replicas), so a wave updates 64 / LPS sites per step instead of
+// one site with mostly idle lanes.  Each lane's replica, Philox counter,
+// neighbour order and arithmetic are those of the one-site-per-wave kernels,
+// and the pooled Marsaglia proposal pools within the lanes of one site (the
+// same replicas as a 64-lane pool when R <= LPS), so results are bit-identical.
+// The site differs between lane groups: neighbour rows are addressed per lane.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float plane_ldv(Plane p, uint32_t voff) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(p, (int)voff, 0, 0));
+}
+__device__ __forceinline__ void plane_stv(Plane p, uint32_t voff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), p, (int)voff, 0, 0);
+}
+
+template <int DM>
+__device__ __forceinline__ void local_field_lanes(Plane px, Plane py, Plane pz, const uint32_t* nb,
+                                                  int D, uint32_t row_bytes, uint32_t roff,
+                                                  const float4* sJ4, float& hx, float& hy, float& hz) {
+    hx = 0.f;
+    hy = 0.f;
+    hz = 0.f;
+    if constexpr (DM > 0) {
+        float vx[DM], vy[DM], vz[DM];
+        uint32_t cl[DM];
+#pragma unroll
+        for (int k = 0; k < DM; ++k) {
+            const uint32_t e = nb[k];
+            const uint32_t vo = (e & 0xffffffu) * row_bytes + roff;
+            cl[k] = e >> 24;
+            vx[k] = plane_ldv(px, vo);
+            vy[k] = plane_ldv(py, vo);
+            vz[k] = plane_ldv(pz, vo);
+        }
+#pragma unroll
+        for (int k = 0; k < DM; ++k) {
+            const float4 j = sJ4[cl[k]];
+            hx = fmaf(j.x, vx[k], hx);
+            hy = fmaf(j.y, vy[k], hy);
+            hz = fmaf(j.z, vz[k], hz);
+        }
+    } else {
+        for (int k = 0; k < D; ++k) {
+            const uint32_t e = nb[k];
+            const uint32_t vo = (e & 0xffffffu) * row_bytes + roff;
+            const float4 j = sJ4[e >> 24];
+            hx = fmaf(j.x, plane_ldv(px, vo), hx);
+            hy = fmaf(j.y, plane_ldv(py, vo), hy);
+            hz = fmaf(j.z, plane_ldv(pz, vo), hz);
+        }
+    }
+}
+
+// marsaglia_pooled restricted to the lanes of `gmask` (one site's lanes); the
+// pool slots of the group start at gbase
+__device__ __forceinline__ void marsaglia_pooled_group(u32x4 w, bool active, uint64_t gmask, int gbase,
+                                                       uint64_t seed, uint32_t id, uint32_t r,
+                                                       uint32_t step, uint64_t* pool, float& x,
+                                                       float& y, float& z) {
+    uint32_t x0, y0, x1, y1;
+    candidates(w, x0, y0, x1, y1);
+    const bool ok0 = in_disc(x0, y0), ok1 = in_disc(x1, y1);
+    const uint64_t needy = __ballot(active && !ok0) & gmask;
+    const uint64_t donors = __ballot(active && ok1) & gmask;
+    const int lane = lane_id();
+    const uint64_t below = lanes_below(lane) & gmask;
+    if (active && ok1) pool[gbase + __popcll(donors & below)] = ((uint64_t)x1 << 32) | y1;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    uint32_t ux = x0, uy = y0;
+    bool have = ok0;
+    if (active && !ok0) {
+        const int j = __popcll(needy & below);
+        if (j < __popcll(donors)) {
+            const uint64_t v = pool[gbase + j];
+            ux = (uint32_t)(v >> 32);
+            uy = (uint32_t)v;
+            have = true;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    if (!active) return;
+    if (have) marsaglia_point(ux, uy, x, y, z);
+    else marsaglia_retry(seed, id, r, step, RNG_METROPOLIS, 1, x, y, z);
+}
+
+template <bool ANGLE, int DM>
+__device__ __forceinline__ void xyz_sweep_lanes_block(const XyzArgs& a, int bb) {
+    __shared__ float4 sJ4[kMaxClasses + 1];
+    __shared__ float sZ[3 * kMaxSpecies];
+    __shared__ float sD[3 * kMaxSpecies];
+    __shared__ uint64_t sPool[kWavesPerBlock][kWave];
+    __shared__ unsigned sAcc[kBlock];
+    const bool ZEE = a.zee != 0, SIA = a.sia != 0;
+    stage_couplings(sJ4, a.jclass, a.n_classes);
+    if (ZEE)
+        for (int t = threadIdx.x; t < 3 * a.nsp; t += kBlock) sZ[t] = a.zvec[t];
+    if (SIA)
+        for (int t = threadIdx.x; t < 3 * a.nsp; t += kBlock) sD[t] = a.dsia[t];
+    __syncthreads();
+
+    const int w = wave_id(), lane = lane_id();
+    const int ls = a.lps_shift, S = kWave >> ls;               // lanes per site, sites per step
+    const int sub = lane >> ls, r = lane & ((1 << ls) - 1);
+    const uint64_t gmask = ls >= 6 ? ~0ull : (((1ull << (1 << ls)) - 1ull) << (sub << ls));
+    const int gbase = sub << ls;
+    const uint32_t roff = 4u * (uint32_t)r, rowb = 4u * (uint32_t)a.Rp;
+    const Plane px = plane_of(a.sx, a.plane_bytes), py = plane_of(a.sy, a.plane_bytes),
+                pz = plane_of(a.sz, a.plane_bytes);
+    const Plane pp = plane_of(ANGLE ? a.phi : a.sx, a.plane_bytes);
+    const bool rok = r < a.R;
+    const float beta = rok ? a.beta_slot[a.slot_of[r]] : 0.f;
+    const uint32_t step = a.ctr[0] + a.step_off;
+    uint64_t* pool = sPool[w];
+    unsigned accepted = 0;
+    const int s0 = a.c_begin + (bb * kWavesPerBlock + w) * a.spw * S;
+    for (int q = 0; q < a.spw; ++q) {
+        const int base = s0 + q * S;                           // wave-uniform
+        if (base >= a.c_end) break;
+        const bool sv = base + sub < a.c_end;
+        const int slot = sv ? base + sub : base;               // an idle lane group reads a valid row
+        const bool rv = rok && sv;
+        const uint32_t vo = (uint32_t)slot * rowb + roff;
+        float x = plane_ldv(px, vo), y = plane_ldv(py, vo), z = plane_ldv(pz, vo);
+        float cphi = ANGLE ? plane_ldv(pp, vo) : 0.f;
+        float hx, hy, hz;
+        local_field_lanes<DM>(px, py, pz, a.ell + (size_t)slot * a.D, a.D, rowb, roff, sJ4, hx, hy, hz);
+        const int sp = (ZEE || SIA) ? a.sp[slot] : 0;
+        if (ZEE) {
+            hx += sZ[3 * sp + 0];
+            hy += sZ[3 * sp + 1];
+            hz += sZ[3 * sp + 2];
+        }
+        bool reflected = false;
+        if (a.fuse) {
+            bool sel = true;
+            if (a.frac < 1.0f) {
+                const u32x4 wo = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, step, RNG_OVRR, a.pass);
+                sel = (float)(wo.x >> 8) * 0x1p-24f < a.frac;
+            }
+            if (sel) {
+                reflected = reflect_about(hx, hy, hz, x, y, z);
+                if (ANGLE && reflected) cphi = atan2f(y, x);
+            }
+        }
+        const u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, step, RNG_METROPOLIS, 0);
+        const float eta = eta24f(wr.z);
+        float nx = 0.f, ny = 0.f, nz = 1.f, nphi = 0.f;
+        if (ANGLE) {
+            angle_move(wr, a.dphi, cphi, nx, ny, nz, nphi);
+        } else {
+            marsaglia_pooled_group(wr, rv, gmask, gbase, a.seed, (uint32_t)slot, (uint32_t)r, step, pool,
+                                   nx, ny, nz);
+        }
+        float dE = (nx - x) * hx + (ny - y) * hy + (nz - z) * hz;
+        if (SIA) {
+            dE += sD[3 * sp + 0] * (nx * nx - x * x) + sD[3 * sp + 1] * (ny * ny - y * y) +
+                  sD[3 * sp + 2] * (nz * nz - z * z);
+        }
+        if (rv && -beta * dE > __logf(eta)) {
+            plane_stv(px, vo, nx);
+            plane_stv(py, vo, ny);
+            plane_stv(pz, vo, nz);
+            if (ANGLE) plane_stv(pp, vo, nphi);
+            ++accepted;
+        } else if (rv && reflected) {
+            plane_stv(px, vo, x);
+            plane_stv(py, vo, y);
+            plane_stv(pz, vo, z);
+            if (ANGLE) plane_stv(pp, vo, cphi);
+        }
+    }
+    book_acceptance(sAcc, 1, &accepted, r, a.R, a.slot_of, a.acc);
+}
+
+template <bool ANGLE, int DM>
+__global__ __launch_bounds__(kBlock) void k_xyz_sweep_lanes(XyzArgs a) {
+    xyz_sweep_lanes_block<ANGLE, DM>(a, block_index(a.nblocks, a.xcd_map));
+}
+
+template <bool ANGLE, int DM>
+__global__ __launch_bounds__(kBlock) void k_xyz_sweep_lanes_batch(const XyzArgs* __restrict__ args,
+                                                                 const int2* __restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    if (it.x < 0) return;
+    const XyzArgs a = args[it.x];
+    xyz_sweep_lanes_block<ANGLE, DM>(a, it.y);
+}
+
+template <int DM>
+__device__ __forceinline__ void ising_sweep_lanes_block(const IsingArgs& a, int bb) {
+    __shared__ int sStride[kMaxIsingClasses + 1];
+    __shared__ uint32_t sThr[kIsingLdsTable];
+    __shared__ unsigned sAcc[kBlock * 4];
+    for (int t = threadIdx.x; t <= a.n_classes; t += kBlock) sStride[t] = a.stride[t];
+    if (a.table_in_lds)
+        for (int t = threadIdx.x; t < a.R * a.table_per_slot; t += kBlock) sThr[t] = a.thr[t];
+    __syncthreads();
+    const int w = wave_id(), lane = lane_id();
+    const int ls = a.lps_shift, S = kWave >> ls;
+    const int sub = lane >> ls;
+    const int rid = lane & ((1 << ls) - 1);                     // Philox id of the lane's 4 replicas
+    const int r0 = 4 * rid;
+    const bool lv = r0 < a.Rp;
+    int tbase[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = r0 + q;
+        tbase[q] = (r < a.R) ? a.slot_of[r] * a.table_per_slot : -1;
+    }
+    unsigned accepted[4] = {0, 0, 0, 0};
+    const uint32_t step = a.ctr[0] + a.step_off;
+    const int s0 = a.c_begin + (bb * kWavesPerBlock + w) * a.spw * S;
+    for (int q = 0; q < a.spw; ++q) {
+        const int base = s0 + q * S;
+        if (base >= a.c_end) break;
+        const int slot = base + sub;
+        if (!lv || slot >= a.c_end) continue;                   // no wave-level operation below
+        uint32_t d[kMaxIsingClasses + 1];
+        int cnt[kMaxIsingClasses + 1];
+#pragma unroll
+        for (int k = 0; k <= kMaxIsingClasses; ++k) {
+            d[k] = 0;
+            cnt[k] = 0;
+        }
+        const uint32_t* nb = a.ell + (size_t)slot * a.D;
+        if constexpr (DM > 0) {
+            uint32_t nv[DM];
+#pragma unroll
+            for (int k = 0; k < DM; ++k)
+                nv[k] = *reinterpret_cast<const uint32_t*>(a.s + (size_t)(nb[k] & 0xffffffu) * a.Rp + r0);
+#pragma unroll
+            for (int k = 0; k < DM; ++k) {
+                const int c = (int)(nb[k] >> 24);
+                class_add(d, c, (nv[k] >> 1) & 0x01010101u);
+#pragma unroll
+                for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
+                    if (kk == c) cnt[kk] += 1;
+            }
+        } else {
+            for (int k = 0; k < a.D; ++k) {
+                const uint32_t e = nb[k];
+                const int c = (int)(e >> 24);
+                if (c == 0) continue;
+                const uint32_t v = *reinterpret_cast<const uint32_t*>(a.s + (size_t)(e & 0xffffffu) * a.Rp + r0);
+                class_add(d, c, (v >> 1) & 0x01010101u);
+#pragma unroll
+                for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
+                    if (kk == c) cnt[kk] += 1;
+            }
+        }
+        uint32_t* own = reinterpret_cast<uint32_t*>(a.s + (size_t)slot * a.Rp + r0);
+        const uint32_t v = *own;
+        const u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)rid, step, RNG_METROPOLIS, 0);
+        const uint32_t words[4] = {wr.x, wr.y, wr.z, wr.w};
+        const int spc = a.sp[slot];
+        uint32_t flip = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            if (tbase[b] < 0) continue;
+            int idx = a.offset;
+#pragma unroll
+            for (int k = 1; k <= kMaxIsingClasses; ++k) {
+                if (k > a.n_classes) break;
+                const int down = (int)((d[k] >> (8 * b)) & 0xffu);
+                idx += sStride[k] * (cnt[k] - 2 * down);
+            }
+            const int sigma = (int)(int8_t)((v >> (8 * b)) & 0xffu);
+            const int t = tbase[b] + (spc * 2 + (sigma > 0)) * a.P + idx;
+            const uint32_t th = a.table_in_lds ? sThr[t] : a.thr[t];
+            if (words[b] < th || th == 0xffffffffu) {
+                flip |= 0xFEu << (8 * b);
+                ++accepted[b];
+            }
+        }
+        if (flip) *own = v ^ flip;
+    }
+    book_acceptance(sAcc, 4, accepted, r0, a.R, a.slot_of, a.acc);
+}
+
+template <int DM>
+__global__ __launch_bounds__(kBlock) void k_ising_sweep_lanes(IsingArgs a) {
+    ising_sweep_lanes_block<DM>(a, block_index(a.nblocks, a.xcd_map));
+}
+
+template <int DM>
+__global__ __launch_bounds__(kBlock) void k_ising_sweep_lanes_batch(const IsingArgs* __restrict__ args,
+                                                                   const int2* __restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    if (it.x < 0) return;
+    const IsingArgs a = args[it.x];
+    ising_sweep_lanes_block<DM>(a, it.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -927,12 +1226,26 @@ static void record(int kind, int variant, int units, const A& a) {
     g_recorder->recs.push_back(r);
 }
 
+// sites a wave updates per step of its loop
+static int sites_per_step(int lps_shift) { return kWave >> lps_shift; }
+
 void launch_xyz_sweep(const XyzArgs& a, bool angle, bool, bool, hipStream_t st) {
+    const bool lanes = a.lps_shift < 6;
     if (g_recorder) {
-        record(LK_XYZ_SWEEP, (angle ? 65536 : 0) | a.D, sweep_units(a.G, a.c_end - a.c_begin, a.spw), a);
+        record(LK_XYZ_SWEEP, (angle ? 65536 : 0) | (lanes ? 131072 : 0) | a.D,
+               sweep_units(a.G, a.c_end - a.c_begin, a.spw * sites_per_step(a.lps_shift)), a);
         return;
     }
     dim3 grid(a.nblocks), block(kBlock);
+    if (lanes) {
+#define LA(DM) hipLaunchKernelGGL((k_xyz_sweep_lanes<true, DM>), grid, block, 0, st, a)
+#define LM(DM) hipLaunchKernelGGL((k_xyz_sweep_lanes<false, DM>), grid, block, 0, st, a)
+        if (angle) { ETHER_DISPATCH_DM(a.D, LA) }
+        else { ETHER_DISPATCH_DM(a.D, LM) }
+#undef LA
+#undef LM
+        return;
+    }
 #define LA(DM) hipLaunchKernelGGL((k_xyz_sweep<true, DM>), grid, block, 0, st, a)
 #define LM(DM) hipLaunchKernelGGL((k_xyz_sweep<false, DM>), grid, block, 0, st, a)
     if (angle) { ETHER_DISPATCH_DM(a.D, LA) }
@@ -956,8 +1269,16 @@ void launch_xyz_ovrr(const XyzArgs& a, bool angle, bool, hipStream_t st) {
 }
 
 void launch_ising_sweep(const IsingArgs& a, hipStream_t st) {
+    const bool lanes = a.lps_shift < 6;
     if (g_recorder) {
-        record(LK_ISING_SWEEP, a.D, sweep_units(a.G4, a.c_end - a.c_begin, a.spw), a);
+        record(LK_ISING_SWEEP, (lanes ? 131072 : 0) | a.D,
+               sweep_units(a.G4, a.c_end - a.c_begin, a.spw * sites_per_step(a.lps_shift)), a);
+        return;
+    }
+    if (lanes) {
+#define LI(DM) hipLaunchKernelGGL((k_ising_sweep_lanes<DM>), dim3(a.nblocks), dim3(kBlock), 0, st, a)
+        ETHER_DISPATCH_DM(a.D, LI)
+#undef LI
         return;
     }
 #define LI(DM) hipLaunchKernelGGL((k_ising_sweep<DM>), dim3(a.nblocks), dim3(kBlock), 0, st, a)
@@ -1010,6 +1331,15 @@ void launch_batch(int kind, int variant, const void* d_args, const int2* d_items
     switch (kind) {
     case LK_XYZ_SWEEP: {
         const XyzArgs* a = static_cast<const XyzArgs*>(d_args);
+        if (variant & 131072) {
+#define LA(DM) hipLaunchKernelGGL((k_xyz_sweep_lanes_batch<true, DM>), grid, block, 0, st, a, d_items)
+#define LM(DM) hipLaunchKernelGGL((k_xyz_sweep_lanes_batch<false, DM>), grid, block, 0, st, a, d_items)
+            if (variant & 65536) { ETHER_DISPATCH_DM(variant & 65535, LA) }
+            else { ETHER_DISPATCH_DM(variant & 65535, LM) }
+#undef LA
+#undef LM
+            break;
+        }
 #define LA(DM) hipLaunchKernelGGL((k_xyz_sweep_batch<true, DM>), grid, block, 0, st, a, d_items)
 #define LM(DM) hipLaunchKernelGGL((k_xyz_sweep_batch<false, DM>), grid, block, 0, st, a, d_items)
         if (variant & 65536) { ETHER_DISPATCH_DM(variant & 65535, LA) }
@@ -1030,8 +1360,14 @@ void launch_batch(int kind, int variant, const void* d_args, const int2* d_items
     }
     case LK_ISING_SWEEP: {
         const IsingArgs* a = static_cast<const IsingArgs*>(d_args);
+        if (variant & 131072) {
+#define LI(DM) hipLaunchKernelGGL((k_ising_sweep_lanes_batch<DM>), grid, block, 0, st, a, d_items)
+            ETHER_DISPATCH_DM(variant & 65535, LI)
+#undef LI
+            break;
+        }
 #define LI(DM) hipLaunchKernelGGL((k_ising_sweep_batch<DM>), grid, block, 0, st, a, d_items)
-        ETHER_DISPATCH_DM(variant, LI)
+        ETHER_DISPATCH_DM(variant & 65535, LI)
 #undef LI
         break;
     }

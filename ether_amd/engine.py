@@ -71,6 +71,7 @@ EXPORTS = [
                             C.c_int32, _vp]),
     ("ether_set_graphs", C.c_int, [_vp, C.c_int32]),
     ("ether_set_launch", C.c_int, [_vp, C.c_int32, C.c_int32]),
+    ("ether_set_lanes", C.c_int, [_vp, C.c_int32]),
     ("ether_batch_create", C.c_int, [_vp, C.c_int32, C.POINTER(_vp)]),
     ("ether_batch_run", C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                   C.c_double, C.c_int32, _vp]),
@@ -306,6 +307,9 @@ class Engine:
 
     def set_launch(self, sites_per_wave: int, xcd_map: int) -> None:
         self._c(self.lib.ether_set_launch(self.h, int(sites_per_wave), int(xcd_map)))
+
+    def set_lanes(self, lanes_per_site: int) -> None:
+        self._c(self.lib.ether_set_lanes(self.h, int(lanes_per_site)))
 
     AUTOTUNE_SPW = (1, 2, 4, 8, 16)
 

@@ -155,6 +155,27 @@ module ether_gpu
          type(c_ptr), value :: table_ms
       end function
 
+      integer(c_int) function ether_batch_create(ctxs, n, batch) bind(C, name="ether_batch_create")
+         import :: c_int, c_ptr
+         type(c_ptr), intent(in) :: ctxs(*)
+         integer(c_int), value :: n
+         type(c_ptr), intent(out) :: batch
+      end function
+
+      integer(c_int) function ether_batch_run(batch, first_step, n_steps, mcs_eq, coa, ovrr_interval, &
+            ovrr_passes, pt_interval, swap_count) bind(C, name="ether_batch_run")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: batch
+         integer(c_int), value :: first_step, n_steps, mcs_eq, coa, ovrr_interval, pt_interval
+         real(c_double), value :: ovrr_passes
+         integer(c_int), intent(inout) :: swap_count
+      end function
+
+      subroutine ether_batch_destroy(batch) bind(C, name="ether_batch_destroy")
+         import :: c_ptr
+         type(c_ptr), value :: batch
+      end subroutine
+
       integer(c_int) function ether_pt_state(ctx, slot_of, acc, att, trips) bind(C, name="ether_pt_state")
          import :: c_int, c_ptr, c_int64_t
          type(c_ptr), value :: ctx

@@ -258,3 +258,53 @@ def compare_ladders(model: Model, *, train_mcs: int = 2000, mcs: Optional[int] =
     finally:
         eng.close()
     return lin, lrn
+
+
+def plain_run(model: Model, *, train_mcs: int = 2000, mcs: Optional[int] = None,
+              repeat: Optional[int] = None) -> LadderRun:
+    """The plain GPU form: one context per ladder (repeat), each with the
+    linear ladder and launched on its own (one launch per lattice per kernel,
+    its own graphs), the contexts run one after another by the host.  Same
+    schedule and windows as adaptive_run(learn=False); the production windows
+    of all ladders are timed together."""
+    t = model.tags
+    temps = np.asarray(t.temperatures(), np.float64)
+    n = len(temps)
+    rep = t.repeat if repeat is None else repeat
+    mcs = t.tmcs if mcs is None else mcs
+    coa = fortran_nint(t.coa)
+    eq = t.tmcs_eq
+    engines = [make_engine(model, n, seed=t.seed + 7919 * k) for k in range(rep)]
+    try:
+        for eng in engines:
+            eng.set_betas(model.betas(temps))
+            eng.set_ladders([0, n])
+            eng.init_spins(E.INIT_RANDOM)
+            eng.run(1, train_mcs, eq, coa, 0, 0.0, t.exchange_interval, 0)
+        scs = []
+        for eng in engines:
+            eng.set_ladders([0, n])
+            eng.reset_moments()
+            eng.synchronize()
+        t0 = time.perf_counter()
+        rest = mcs - train_mcs
+        for eng in engines:
+            scs.append(eng.run(train_mcs + 1, rest, eq, coa, 0, 0.0, t.exchange_interval, 0))
+        for eng in engines:
+            eng.synchronize()
+        wall = time.perf_counter() - t0
+        acc = np.zeros(n - 1)
+        att = np.zeros(n - 1)
+        trips = 0
+        metro = np.zeros(n)
+        for eng in engines:
+            _, a, b, tr = eng.pt_state()
+            acc += a[: n - 1]
+            att += b[: n - 1]
+            trips += int(tr.sum())
+            metro += eng.moments()[:, E.MOM_ACC] / rest / rep
+        return LadderRun(temps, wall, rest, trips, acc / np.maximum(att, 1),
+                         float(n * rep) * model.n_sites * rest, metro_acc=metro, source="plain")
+    finally:
+        for eng in engines:
+            eng.close()

@@ -253,6 +253,13 @@ void ether_batch_destroy(ether_batch* batch);
  * contiguous site range.  Defaults 4 and 1.  Results do not depend on the
  * shape: every update draws from its own (site, replica, sweep) counter. */
 int ether_set_launch(ether_ctx* ctx, int32_t sites_per_wave, int32_t xcd_map);
+/* Lanes of a wavefront given to one site in the sweep: 64 (one site per wave),
+ * or 32 / 16 (8 for Ising, whose lanes hold 4 replicas each) so that a wave
+ * updates 2 / 4 / 8 sites at once when a context has few replicas.  0 (the
+ * default) picks the fewest lanes that hold all replicas.  Results do not
+ * depend on it.  In ether_set_launch, sites_per_wave then counts steps of
+ * 64 / lanes sites. */
+int ether_set_lanes(ether_ctx* ctx, int32_t lanes_per_site);
 /* Autotuner: times `reps` sweeps for each of sites_per_wave {1,2,4,8,16} x
  * xcd_map {0,1} on the context's own lattice and replicas, keeps the fastest
  * (returned through the two nullable pointers), and restores the spins, the

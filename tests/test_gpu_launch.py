@@ -32,16 +32,23 @@ def _spins(eng, R):
     return np.stack([eng.get_spins(r)[0] for r in range(R)])
 
 
-@pytest.mark.parametrize("name,over,R", [
-    ("sc_heis_afm_32", dict(sc="8 8 8"), 70),
-    ("sc_ising_16", dict(sc="8 8 8"), 300),
-    ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11"), 24),
+@pytest.mark.parametrize("name,over,R,lanes", [
+    ("sc_heis_afm_32", dict(sc="8 8 8"), 70, [0]),
+    ("sc_ising_16", dict(sc="8 8 8"), 300, [0]),
+    ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11"), 24, [64, 32, 0]),          # 0: 32 lanes
+    ("tri_afm_96", dict(sc="12 12 1"), 13, [64, 16, 32]),                    # ANGLE, 3 colours
+    ("sc_ising_16", dict(sc="6 6 6"), 30, [64, 8, 16, 32]),
 ])
-def test_sweeps_do_not_depend_on_launch_shape(name, over, R):
-    say(f"launch shapes {name} R={R}")
+def test_sweeps_do_not_depend_on_launch_shape(name, over, R, lanes):
+    """Sites per wave, the XCD map and lanes per site (several sites per wave
+    for few replicas) change which lanes do an update, never its result."""
+    say(f"launch shapes {name} R={R} lanes {lanes}")
     ref = None
-    for spw, xcd in [(4, 1), (1, 0), (2, 1), (8, 0), (16, 1)]:
+    shapes = [(4, 1, lanes[0]), (1, 0, lanes[-1]), (2, 1, lanes[0]), (8, 0, lanes[-1]), (16, 1, lanes[0])]
+    shapes += [(4, 1, ln) for ln in lanes]
+    for spw, xcd, ln in shapes:
         m, eng, _ = pair(name, R, **over)
+        eng.set_lanes(ln)
         eng.set_betas(m.betas(np.linspace(5.0, 50.0, R)))
         _start(m, eng, R, 3)
         eng.set_launch(spw, xcd)
@@ -51,7 +58,8 @@ def test_sweeps_do_not_depend_on_launch_shape(name, over, R):
         if ref is None:
             ref = got
         else:
-            assert np.array_equal(got[0], ref[0]), f"spins differ with sites_per_wave={spw} xcd_map={xcd}"
+            assert np.array_equal(got[0], ref[0]), \
+                f"spins differ with sites_per_wave={spw} xcd_map={xcd} lanes={ln}"
             assert np.array_equal(got[1], ref[1])
 
 

@@ -7,4 +7,4 @@ driver (driver), the learned tempering ladder (ladder) and the benchmark
 workloads (workloads).  The Fortran driver in ether_amd/fortran binds the same
 C-ABI through iso_c_binding.
 """
-from .engine import Engine, EtherError, build_neighbours, load_library  # noqa: F401
+from .engine import Engine, EtherError, build_neighbours, build_spiral_partners, load_library  # noqa: F401

@@ -67,6 +67,10 @@ struct ether_ctx {
     int lps_shift = 6;    // lanes per site of the sweep (1 << lps_shift); set at commit
     int lanes_req = 0;    // ether_set_lanes (0: chosen from R)
     int meas_tiles = 0, meas_sites = 0, meas_rec = 0;
+    // spiral-state partners (ether_set_spiral): site-space rows, -1 padded
+    int spiral_cap = 0;
+    double spiral_count = 0.0;
+    std::vector<int> spiral_rows;
     // device
     uint32_t* d_ell = nullptr;
     uint8_t* d_sp = nullptr;
@@ -94,6 +98,9 @@ struct ether_ctx {
     double* d_parts = nullptr;
     double* d_M = nullptr;
     long long* d_bonds = nullptr;
+    int* d_spiral = nullptr;
+    double* d_mom_sp = nullptr;
+    double* d_psi = nullptr;
     int n_groups = 0;
     int* d_gstart = nullptr;
     unsigned long long* d_pt_acc = nullptr;
@@ -603,7 +610,7 @@ extern "C" int ether_commit(ether_ctx* c) {
     // measurement: about 4096 wave units per pass, at least 8 sites each
     c->meas_sites = std::max(8, (int)(((long long)N * G + 4095) / 4096));
     c->meas_tiles = (N + c->meas_sites - 1) / c->meas_sites;
-    c->meas_rec = ising ? (c->n_classes + nsp + 1) : 6;
+    c->meas_rec = ising ? (c->n_classes + nsp + 1) : (c->spiral_cap ? 9 : 6);
     TRY(dev_alloc(c, &c->d_ell, ell.size()));
     TRY(dev_alloc(c, &c->d_sp, (size_t)N));
     TRY(dev_alloc(c, &c->d_w, (size_t)N));
@@ -651,6 +658,24 @@ extern "C" int ether_commit(ether_ctx* c) {
     TRY(dev_alloc(c, &c->d_scratch_h, (size_t)N * 4));
     TRY(dev_alloc(c, &c->d_scratch_e, (size_t)N));
     TRY(dev_alloc(c, &c->d_ctr, 4));
+    if (c->spiral_cap) {
+        // partner rows in slot space, in the order the sites' rows were given
+        std::vector<int> rows((size_t)N * c->spiral_cap, -1);
+        for (int i = 0; i < N; ++i) {
+            const int s = c->slot_of_site[i];
+            for (int k = 0; k < c->spiral_cap; ++k) {
+                const int j = c->spiral_rows[(size_t)i * c->spiral_cap + k];
+                if (j < 0) break;
+                rows[(size_t)s * c->spiral_cap + k] = c->slot_of_site[j];
+            }
+        }
+        TRY(dev_alloc(c, &c->d_spiral, rows.size()));
+        TRY(dev_alloc(c, &c->d_mom_sp, (size_t)R * 3));
+        TRY(dev_alloc(c, &c->d_psi, (size_t)R));
+        TRY(upload(c, c->d_spiral, rows.data(), rows.size()));
+        HIPCHK(c, hipMemsetAsync(c->d_mom_sp, 0, sizeof(double) * 3 * R, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_psi, 0, sizeof(double) * R, c->stream));
+    }
     launch_set_ctr(c->d_ctr, 0, 0, 0, c->stream);
     c->dev_step = c->dev_pt = c->dev_sc = 0;
     TRY(upload(c, c->d_ell, ell.data(), ell.size()));
@@ -893,6 +918,8 @@ static MeasArgs meas_args(ether_ctx* c) {
     a.nsp = c->nsp;
     a.zvec = c->d_zvec;
     a.dsia = c->d_dsia;
+    a.spiral = c->d_spiral;
+    a.spiral_cap = c->model == ETHER_MODEL_ISING ? 0 : c->spiral_cap;
     a.partial = c->d_partial;
     return a;
 }
@@ -1040,6 +1067,9 @@ static int enqueue_measure(ether_ctx* c, int sample) {
     f.bonds = c->d_bonds;
     f.moments = c->d_mom;
     f.slot_of = c->d_slot_of;
+    f.spiral_count = c->spiral_cap ? c->spiral_count : 0.0;
+    f.psi = c->d_psi;
+    f.mom_sp = c->d_mom_sp;
     launch_finalize(f, c->stream);
     ev_end(c, ev);
     HIPCHK(c, hipGetLastError());
@@ -1283,7 +1313,58 @@ extern "C" int ether_read_moments(ether_ctx* c, double* moments) {
 extern "C" int ether_reset_moments(ether_ctx* c) {
     TRY(need_committed(c));
     HIPCHK(c, hipMemsetAsync(c->d_mom, 0, sizeof(double) * ETHER_MOM_COUNT * c->R, c->stream));
+    if (c->d_mom_sp)
+        HIPCHK(c, hipMemsetAsync(c->d_mom_sp, 0, sizeof(double) * 3 * c->R, c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_acc, 0, sizeof(unsigned long long) * c->R, c->stream));
+    return ETHER_OK;
+}
+
+extern "C" int ether_set_spiral(ether_ctx* c, int32_t capacity, const int32_t* partners) {
+    if (!c) return fail(c, ETHER_ERR_ARG, "ether_set_spiral: null context");
+    if (c->committed) return fail(c, ETHER_ERR_STATE, "ether_set_spiral must precede ether_commit");
+    if (capacity == 0) {
+        c->spiral_cap = 0;
+        c->spiral_count = 0.0;
+        c->spiral_rows.clear();
+        return ETHER_OK;
+    }
+    if (c->model != ETHER_MODEL_XYZ)
+        return fail(c, ETHER_ERR_ARG, "the spiral-state parameter needs vector spins (ssp with ISING)");
+    if (capacity < 0 || capacity > 4096 || !partners)
+        return fail(c, ETHER_ERR_ARG, "ether_set_spiral: capacity out of range or null partners");
+    const int N = c->N;
+    long long count = 0;
+    for (int i = 0; i < N; ++i) {
+        bool ended = false;
+        for (int k = 0; k < capacity; ++k) {
+            const int j = partners[(size_t)i * capacity + k];
+            if (j < 0) {
+                ended = true;
+                continue;
+            }
+            if (ended || j >= N || j == i)
+                return fail(c, ETHER_ERR_ARG, "ether_set_spiral: partner row " + std::to_string(i) +
+                                                  " has an entry out of range, a self pair or a gap");
+            ++count;
+        }
+    }
+    if (count == 0)
+        return fail(c, ETHER_ERR_ARG, "ether_set_spiral: no partner pairs (spiral_count = 0)");
+    c->spiral_cap = capacity;
+    c->spiral_count = (double)count;
+    c->spiral_rows.assign(partners, partners + (size_t)N * capacity);
+    return ETHER_OK;
+}
+
+extern "C" int ether_read_spiral(ether_ctx* c, double* moments, double* psi) {
+    TRY(need_committed(c));
+    if (!c->spiral_cap) return fail(c, ETHER_ERR_STATE, "ether_read_spiral: no spiral partners set");
+    if (moments)
+        HIPCHK(c, hipMemcpyAsync(moments, c->d_mom_sp, sizeof(double) * 3 * c->R, hipMemcpyDeviceToHost,
+                                 c->stream));
+    if (psi)
+        HIPCHK(c, hipMemcpyAsync(psi, c->d_psi, sizeof(double) * c->R, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return ETHER_OK;
 }
 

@@ -124,6 +124,8 @@ struct MeasArgs {
     int nsp;
     const float* zvec;
     const float* dsia;
+    const int* spiral;     // [N slots][spiral_cap] partner slots, -1 ends a row (XYZ)
+    int spiral_cap;        // 0: no spiral-state observable
     double* partial;       // [tiles][Rp][rec]
 };
 
@@ -147,6 +149,9 @@ struct FinArgs {
     long long* bonds;        // [R][n_classes+1]
     double* moments;         // [R slots][8]
     const int* slot_of;      // [R]
+    double spiral_count;     // > 0: record words 6..8 hold sum S_i x S_j over the partners
+    double* psi;             // [R] spiral-state parameter of the measured state
+    double* mom_sp;          // [R slots][3] sums of psi, psi^2, psi^4 over samples
 };
 
 struct PtArgs {

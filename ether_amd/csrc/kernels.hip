@@ -867,6 +867,7 @@ __device__ __forceinline__ void xyz_measure_block(const MeasArgs& a, int blk) {
     const int g = unit % a.G, tile = unit / a.G;
     const int r = g * kWave + lane_id();
     double v[6] = {0, 0, 0, 0, 0, 0};
+    double ss[3] = {0, 0, 0};
     const int s_begin = tile * a.tile_sites;
     const int s_end = min(a.N, s_begin + a.tile_sites);
     for (int slot = s_begin; slot < s_end; ++slot) {
@@ -890,10 +891,28 @@ __device__ __forceinline__ void xyz_measure_block(const MeasArgs& a, int blk) {
         v[3] += (double)(wgt * x);
         v[4] += (double)(wgt * y);
         v[5] += (double)(wgt * z);
+        // spiral state: sum of S_i x S_j over the site's partners (src/get_spiral_state.f90:88-112)
+        if (a.spiral_cap) {
+            const int* pj = a.spiral + (size_t)slot * a.spiral_cap;
+            for (int k = 0; k < a.spiral_cap; ++k) {
+                const int j = pj[k];
+                if (j < 0) break;
+                const size_t jr = (size_t)j * a.Rp + r;
+                const double bx = a.sx[jr], by = a.sy[jr], bz = a.sz[jr];
+                ss[0] += (double)y * bz - (double)z * by;
+                ss[1] += (double)z * bx - (double)x * bz;
+                ss[2] += (double)x * by - (double)y * bx;
+            }
+        }
     }
     double* out = a.partial + ((size_t)tile * a.Rp + r) * a.rec;
 #pragma unroll
     for (int k = 0; k < 6; ++k) out[k] = v[k];
+    if (a.spiral_cap) {
+        out[6] = ss[0];
+        out[7] = ss[1];
+        out[8] = ss[2];
+    }
 }
 
 template <bool ZEE, bool SIA>
@@ -976,14 +995,18 @@ __device__ __forceinline__ void finalize_block(const FinArgs& a, int blk) {
     const int r = blk;
     const int lane = threadIdx.x;
     double ex = 0.0, ez = 0.0, es = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+    double psi = 0.0;
     if (a.model == 1) {
-        double v[6] = {0, 0, 0, 0, 0, 0};
+        const int nv = a.spiral_count > 0.0 ? 9 : 6;
+        double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int t = lane; t < a.tiles; t += 64) {
             const double* p = a.partial + ((size_t)t * a.Rp + r) * a.rec;
-            for (int k = 0; k < 6; ++k) v[k] += p[k];
+            for (int k = 0; k < nv; ++k) v[k] += p[k];
         }
-        for (int k = 0; k < 6; ++k) v[k] = wave_sum(v[k]);
+        for (int k = 0; k < nv; ++k) v[k] = wave_sum(v[k]);
         ex = v[0]; ez = v[1]; es = v[2]; mx = v[3]; my = v[4]; mz = v[5];
+        // |sum S_i x S_j| / spiral_count (src/get_spiral_state.f90:114-117)
+        if (nv == 9) psi = sqrt(v[6] * v[6] + v[7] * v[7] + v[8] * v[8]) / a.spiral_count;
     } else {
         for (int k = 0; k < a.rec; ++k) {
             double s = 0.0;
@@ -1023,7 +1046,15 @@ __device__ __forceinline__ void finalize_block(const FinArgs& a, int blk) {
         mo[4] += m;
         mo[5] += m2;
         mo[6] += m2 * m2;
+        if (a.spiral_count > 0.0) {
+            double* ms = a.mom_sp + 3 * (size_t)a.slot_of[r];
+            const double p2 = psi * psi;
+            ms[0] += psi;
+            ms[1] += p2;
+            ms[2] += p2 * p2;
+        }
     }
+    if (a.spiral_count > 0.0) a.psi[r] = psi;
 }
 
 __global__ __launch_bounds__(64) void k_finalize(FinArgs a) {

@@ -172,3 +172,99 @@ extern "C" int ether_build_neighbours(const double* abc, int32_t n_basis, const 
     }
     return ETHER_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Spiral-state partners (src/get_spiral_state.f90:47-82, case
+// 'ions_for_spiral'): site j != i is a partner of i when |r_j - r_i| <= dis
+// and the angle between (r_j - r_i)/|r_j - r_i| and dir, acos(u.dir) in
+// degrees, is <= latency.  Positions are the supercell's Cartesian positions
+// without periodic images, dir is used as given, and partners are listed in
+// ascending j.  A uniform grid of cells no smaller than `dis` replaces the
+// O(N^2) scan.
+// ---------------------------------------------------------------------------
+extern "C" int ether_build_spiral_partners(int32_t n_sites, const double* pos, const double* dir,
+                                           double dis, double latency_deg, int32_t capacity,
+                                           int32_t* partners, int64_t* count) {
+    using ether::set_global_error;
+    if (n_sites <= 0 || !pos || !dir || !partners || capacity <= 0 || !(dis > 0.0)) {
+        set_global_error("ether_build_spiral_partners: bad arguments");
+        return ETHER_ERR_ARG;
+    }
+    const int N = n_sites;
+    double lo[3], hi[3];
+    for (int x = 0; x < 3; ++x) lo[x] = hi[x] = pos[x];
+    for (int i = 0; i < N; ++i)
+        for (int x = 0; x < 3; ++x) {
+            lo[x] = std::min(lo[x], pos[3 * i + x]);
+            hi[x] = std::max(hi[x], pos[3 * i + x]);
+        }
+    // cell edge h >= dis, so every partner lies in the 27 cells around a site;
+    // h grows until the grid holds at most about 4 cells per site
+    double h = dis;
+    int nb[3];
+    for (;;) {
+        double cells = 1.0;
+        for (int x = 0; x < 3; ++x) {
+            nb[x] = (int)std::min(1048576.0, std::floor((hi[x] - lo[x]) / h) + 1.0);
+            cells *= nb[x];
+        }
+        if (cells <= 4.0 * N + 64.0) break;
+        h *= 1.25;
+    }
+    auto bin_of = [&](const double* p, int* b) {
+        for (int x = 0; x < 3; ++x) b[x] = std::min(nb[x] - 1, std::max(0, (int)std::floor((p[x] - lo[x]) / h)));
+    };
+    const size_t nbins = (size_t)nb[0] * nb[1] * nb[2];
+    std::vector<int> head(nbins + 1, 0), items(N);
+    std::vector<int> bin(N);
+    for (int i = 0; i < N; ++i) {
+        int b[3];
+        bin_of(pos + 3 * i, b);
+        bin[i] = (b[2] * nb[1] + b[1]) * nb[0] + b[0];
+        head[bin[i] + 1]++;
+    }
+    for (size_t q = 0; q < nbins; ++q) head[q + 1] += head[q];
+    {
+        std::vector<int> fill(head.begin(), head.end() - 1);
+        for (int i = 0; i < N; ++i) items[fill[bin[i]]++] = i;   // ascending i within a bin
+    }
+    const double to_deg = 180.0 / M_PI;
+    int64_t total = 0;
+    std::vector<int> cand;
+    for (int i = 0; i < N; ++i) {
+        int b[3];
+        bin_of(pos + 3 * i, b);
+        cand.clear();
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int x = b[0] + dx, y = b[1] + dy, z = b[2] + dz;
+                    if (x < 0 || y < 0 || z < 0 || x >= nb[0] || y >= nb[1] || z >= nb[2]) continue;
+                    const size_t q = ((size_t)z * nb[1] + y) * nb[0] + x;
+                    for (int t = head[q]; t < head[q + 1]; ++t) cand.push_back(items[t]);
+                }
+        std::sort(cand.begin(), cand.end());
+        int found = 0;
+        int32_t* out = partners + (size_t)i * capacity;
+        for (int k = 0; k < capacity; ++k) out[k] = -1;
+        for (int j : cand) {
+            if (j == i) continue;
+            const double r[3] = {pos[3 * j] - pos[3 * i], pos[3 * j + 1] - pos[3 * i + 1],
+                                 pos[3 * j + 2] - pos[3 * i + 2]};
+            const double d = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+            if (!(d <= dis)) continue;
+            const double u[3] = {r[0] / d, r[1] / d, r[2] / d};   // as the reference: uni_vec = r21/dis
+            const double ang = std::acos(u[0] * dir[0] + u[1] * dir[1] + u[2] * dir[2]) * to_deg;
+            if (!(ang <= latency_deg)) continue;
+            if (found == capacity) {
+                set_global_error("ether_build_spiral_partners: more partners than spiral_capacity for site " +
+                                 std::to_string(i + 1));
+                return ETHER_ERR_LIMIT;
+            }
+            out[found++] = j;
+            ++total;
+        }
+    }
+    if (count) *count = total;
+    return ETHER_OK;
+}

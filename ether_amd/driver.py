@@ -31,17 +31,26 @@ def attach_neighbours(model: Model) -> Model:
     return model
 
 
+def spiral_partners(model: Model):
+    """get_spiral_state('ions_for_spiral') (src/get_spiral_state.f90:47-82) through
+    the library's builder: (partners (N, spiral_capacity), spiral_count)."""
+    t = model.tags
+    return E.build_spiral_partners(model.positions, t.ss_direc, t.ss_dis, t.ss_lat, t.spiral_capacity)
+
+
 def make_engine(model: Model, n_replicas: int, *, compat: int = 0, seed: Optional[int] = None,
                 colour=None) -> E.Engine:
     if model.nn_count is None:
         attach_neighbours(model)
     t = model.tags
+    # the spiral state is a vector-spin observable (src/ether.f90:69, src/write_output_files.f90:92)
+    spiral = spiral_partners(model)[0] if (t.ssp and not t.ising) else None
     eng = E.Engine(model.model_id, model.n_sites, n_replicas, t.seed if seed is None else seed)
     eng.setup(model.n_species, model.species, model.stagger, model.nn_count, model.nn_idx,
               model.j_exc.reshape(-1), model.spin,
               zeeman=model.h if t.zeeman else None, g_factor=model.g_factor, mu_b=model.mu_b,
               sia=model.sia_vec.reshape(-1) if model.sia_vec is not None else None,
-              angle=t.angle, dphi=t.dphi, compat=compat, colour=colour)
+              angle=t.angle, dphi=t.dphi, compat=compat, colour=colour, spiral=spiral)
     return eng
 
 
@@ -55,6 +64,35 @@ def repeat_observables(mom: np.ndarray, beta: float, n_sites: int) -> np.ndarray
         e, 1.0 - (1.0 / 3.0) * (e4 / (e2 * e2)), beta * beta * (e2 - e * e) * n_sites,
         m, 1.0 - (1.0 / 3.0) * (m4 / (m2 * m2)), beta * (m2 - m * m) * n_sites,
     ])
+
+
+def spiral_observables(sums: np.ndarray, n: float, beta: float, count: int) -> np.ndarray:
+    """get_spiral_state 'avg_spiral_state' (src/get_spiral_state.f90:122-140).
+    Returns (psi_avg, U_psi, chi_psi)."""
+    p, p2, p4 = sums[0] / n, sums[1] / n, sums[2] / n
+    return np.array([p, 1.5 - 0.5 * (p4 / (p2 * p2)), beta * (p2 - p * p) * count])
+
+
+def fortran_heads(labels, gaps) -> str:
+    """A header record written as the reference writes its character(len=20)
+    labels with A13 edits: each label's first 13 bytes, blank-padded on the
+    right, after `gaps[k]` blanks of nX editing (src/init.f90:82,
+    src/write_output_files.f90:103-104)."""
+    out = b""
+    for lab, g in zip(labels, gaps):
+        out += b" " * g + (lab.encode() + b" " * 20)[:13]
+    return out.decode(errors="replace")
+
+
+FMT103 = (0, 0, 0, 1, 1, 2, 0)
+FMT104 = (0, 0, 0, 0, 1, 1, 0, 0)
+M_HEAD = ("# 1 Temp.", "2 |M|", "3 χ", "4 Δ|M|", "5 Δχ", "6 U(|M|)", "7 ΔU(|M|)")
+SS_HEAD = ("# 1 Temp.", "2 |ψ|", "3 ψ", "4 Δ|ψ|", "5 Δψ", "6 U(|ψ|)", "7 ΔU(|ψ|)")
+
+
+def e_head(para: bool):
+    return ("# 1 Temp.", "2 E", "3 Cv" if para else "3 Cv/kB", "4 ΔE",
+            "5 ΔCv" if para else "5 Δ(Cv/kB)", "6 U(E)", "7 ΔU(E)", "8 Accpt. (%)")
 
 
 def over_repeats(per: np.ndarray):
@@ -77,10 +115,15 @@ class Results:
     pt_acc: Optional[np.ndarray] = None
     pt_att: Optional[np.ndarray] = None
     trips: Optional[np.ndarray] = None
+    para: bool = False
+    spiral_mean: Optional[np.ndarray] = None   # (nscan, 3): psi, U_psi, chi_psi
+    spiral_sem: Optional[np.ndarray] = None
 
     def write(self, outdir: str) -> None:
-        """energy.dat / magnetization.dat in the reference's layout
-        (src/write_output_files.f90:82-98, format (f11.5,6(1x,es12.5),1x,f7.2))."""
+        """energy.dat / magnetization.dat / spiral_state_parameters.dat in the
+        reference's layout (src/write_output_files.f90:27-98, rows in format
+        (f11.5,6(1x,es12.5),1x,f7.2)).  spiral_state_parameters.dat is created
+        in every run and holds rows only with the spiral state on."""
         os.makedirs(outdir, exist_ok=True)
 
         def row(vals, acc=None):
@@ -88,17 +131,21 @@ class Results:
             return s + (f" {acc:7.2f}" if acc is not None else "")
 
         with open(os.path.join(outdir, "magnetization.dat"), "w") as f:
-            f.write("    # 1 Temp.        2 |M|          3 χ       4 Δ|M|          5 Δχ       "
-                    "6 U(|M|)    7 ΔU(|M|)\n")
+            f.write(fortran_heads(M_HEAD, FMT103) + "\n")
             for k, T in enumerate(self.temps):
                 m, s = self.mean[k], self.sem[k]
                 f.write(row([T, m[3], m[5], s[3], s[5], m[4], s[4]]) + "\n")
         with open(os.path.join(outdir, "energy.dat"), "w") as f:
-            f.write("    # 1 Temp.          2 E      3 Cv/kB         4 ΔE   5 Δ(Cv/kB)         "
-                    "6 U(E)       7 ΔU(E) 8 Accpt. (%)\n")
+            f.write(fortran_heads(e_head(self.para), FMT104) + "\n")
             for k, T in enumerate(self.temps):
                 m, s = self.mean[k], self.sem[k]
                 f.write(row([T, m[0], m[2], s[0], s[2], m[1], s[1]], self.accept_pct[k]) + "\n")
+        with open(os.path.join(outdir, "spiral_state_parameters.dat"), "w") as f:
+            if self.spiral_mean is not None:
+                f.write(fortran_heads(SS_HEAD, FMT103) + "\n")
+                for k, T in enumerate(self.temps):
+                    m, s = self.spiral_mean[k], self.spiral_sem[k]
+                    f.write(row([T, m[0], m[2], s[0], s[2], m[1], s[1]]) + "\n")
 
 
 def run(model: Model, *, compat: int = 0, chunk: int = 0, temps=None, repeat: Optional[int] = None,
@@ -148,7 +195,18 @@ def run(model: Model, *, compat: int = 0, chunk: int = 0, temps=None, repeat: Op
     for j in range(nscan):
         mean[j], sem[j] = over_repeats(per[:, j, :])
     res = Results(temps=temps, mean=mean, sem=sem, accept_pct=acc / (rep * t.tmcs) * 100.0,
-                  per_repeat=per, wall_s=wall, updates=float(R) * model.n_sites * t.tmcs)
+                  per_repeat=per, wall_s=wall, updates=float(R) * model.n_sites * t.tmcs, para=t.para)
+    if getattr(eng, "spiral_count", 0):
+        sums, _ = eng.spiral()
+        sper = np.zeros((rep, nscan, 3))
+        for k in range(rep):
+            for j in range(nscan):
+                s = k * nscan + j
+                sper[k, j] = spiral_observables(sums[s], mom[s, E.MOM_N], beta_T[j], eng.spiral_count)
+        res.spiral_mean = np.zeros((nscan, 3))
+        res.spiral_sem = np.zeros((nscan, 3))
+        for j in range(nscan):
+            res.spiral_mean[j], res.spiral_sem[j] = over_repeats(sper[:, j, :])
     if t.pt:
         _, pacc, patt, trips = eng.pt_state()
         res.pt_acc, res.pt_att, res.trips = pacc, patt, trips

@@ -44,6 +44,8 @@ _vp = C.c_void_p
 EXPORTS = [
     ("ether_build_neighbours", C.c_int,
      [_vp, C.c_int32, _vp, _vp, _vp, C.c_int32, _vp, C.c_double, C.c_int32, _vp, _vp]),
+    ("ether_build_spiral_partners", C.c_int,
+     [C.c_int32, _vp, _vp, C.c_double, C.c_double, C.c_int32, _vp, _vp]),
     ("ether_ctx_create", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(_vp)]),
     ("ether_ctx_destroy", None, [_vp]),
     ("ether_last_error", C.c_int, [_vp, C.c_char_p, C.c_int32]),
@@ -65,6 +67,8 @@ EXPORTS = [
     ("ether_get_energy_parts", C.c_int, [_vp, _vp, _vp, _vp]),
     ("ether_read_moments", C.c_int, [_vp, _vp]),
     ("ether_reset_moments", C.c_int, [_vp]),
+    ("ether_set_spiral", C.c_int, [_vp, C.c_int32, _vp]),
+    ("ether_read_spiral", C.c_int, [_vp, _vp, _vp]),
     ("ether_pt_swap", C.c_int, [_vp, C.c_int32, _vp]),
     ("ether_pt_state", C.c_int, [_vp, _vp, _vp, _vp, _vp]),
     ("ether_run", C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
@@ -157,6 +161,20 @@ def build_neighbours(abc, basis_xyz, sc, open_bc, shell_d, tol=1e-5, capacity=20
     return cnt.reshape(len(shells), n), idx.reshape(len(shells), n, capacity)
 
 
+def build_spiral_partners(pos, direc, dis, latency_deg, capacity=5):
+    """Spiral-state partners of every site (see ether_build_spiral_partners):
+    returns (partners (N, capacity) int32, -1 padded; spiral_count)."""
+    lib = load_library()
+    p = _arr(pos, np.float64).reshape(-1, 3)
+    d = _arr(direc, np.float64).reshape(3)
+    out = np.full(p.shape[0] * int(capacity), -1, np.int32)
+    cnt = C.c_int64()
+    rc = lib.ether_build_spiral_partners(p.shape[0], _ptr(p), _ptr(d), float(dis), float(latency_deg),
+                                         int(capacity), _ptr(out), C.byref(cnt))
+    _check(rc)
+    return out.reshape(p.shape[0], int(capacity)), cnt.value
+
+
 class Engine:
     """One engine context: R replicas of an N-site lattice resident on the GPU."""
 
@@ -204,8 +222,9 @@ class Engine:
 
     def setup(self, n_species, species, stagger, nn_count, nn_idx, j_exc, spin, *,
               zeeman=None, g_factor=2.0, mu_b=0.0, sia=None, angle=False, dphi=0.0,
-              compat=0, colour=None):
-        """set_lattice + set_exchange + set_zeeman + set_sia + set_proposal + commit."""
+              compat=0, colour=None, spiral=None):
+        """set_lattice + set_exchange + set_zeeman + set_sia + set_proposal
+        (+ set_spiral with spiral = partners (N, capacity)) + commit."""
         self.set_lattice(n_species, species, stagger, nn_count, nn_idx, colour)
         j = _arr(j_exc, np.float64)
         s = _arr(spin, np.float64)
@@ -218,7 +237,16 @@ class Engine:
             self._c(self.lib.ether_set_sia(self.h, 1, _ptr(d)))
         self._c(self.lib.ether_set_proposal(self.h, 1 if angle else 0, float(dphi)))
         self._c(self.lib.ether_set_compat(self.h, int(compat)))
+        if spiral is not None:
+            self.set_spiral(spiral)
         self._c(self.lib.ether_commit(self.h))
+
+    def set_spiral(self, partners) -> None:
+        """Spiral-state partners (N, capacity), -1 padded; before commit."""
+        pr = _arr(partners, np.int32)
+        cap = pr.shape[-1] if pr.size else 0
+        self._c(self.lib.ether_set_spiral(self.h, int(cap), _ptr(pr)))
+        self.spiral_count = int((pr >= 0).sum())
 
     def layout(self):
         nc = C.c_int32()
@@ -280,6 +308,13 @@ class Engine:
 
     def reset_moments(self) -> None:
         self._c(self.lib.ether_reset_moments(self.h))
+
+    def spiral(self):
+        """(sums of psi, psi^2, psi^4 per slot (R, 3), psi of the last measured state (R,))."""
+        mom = np.zeros(3 * self.R, np.float64)
+        psi = np.zeros(self.R, np.float64)
+        self._c(self.lib.ether_read_spiral(self.h, _ptr(mom), _ptr(psi)))
+        return mom.reshape(self.R, 3), psi
 
     def pt_swap(self, parity: int) -> int:
         n = C.c_int32()

@@ -9,8 +9,8 @@
 ! runs the temperature / repeat / MCS schedule of src/ether.f90:79-214 on the
 ! GPU through the C-ABI (module ether_gpu): every temperature x repeat chain
 ! is a device replica and all of them advance together.  Results go to
-! energy.dat and magnetization.dat in the reference's format
-! (src/write_output_files.f90:82-98).  Any failure prints a remark and stops
+! energy.dat, magnetization.dat and spiral_state_parameters.dat in the
+! reference's format (src/write_output_files.f90:27-98).  Any failure prints a remark and stops
 ! with a nonzero status, like the reference's terminate (src/terminate.f90).
 module ether_driver_input
    use iso_c_binding
@@ -260,6 +260,10 @@ program ether_gpu_driver
    integer :: nb, n, i, j, k, l, s, a, b, q, nsh, npairs, u, ios, nscan, R, t, rep, coa_i
    integer :: start, nchunk, step, sw, rc, ovrr_iv, cnt, cap
    integer, allocatable :: b_sp(:), b_idx(:), species(:), nn_count(:), nn_idx(:), gstart(:)
+   integer(c_int), allocatable :: sp_partners(:)
+   integer(c_int64_t) :: sp_count
+   real(dp), allocatable :: sp_sums(:, :), sp_per(:, :, :), sp_mean(:, :), sp_sem(:, :), pos(:, :)
+   character(len=20) :: hd(8)
    real(dp), allocatable :: b_xyz(:, :), stagger(:), S0(:, :), phi0(:), shells(:), jx(:, :, :, :)
    real(dp), allocatable :: siav(:, :), temps(:), betas(:), mom(:, :), per(:, :, :)
    real(dp), allocatable :: mean(:, :), sem(:, :), accp(:), stg_id(:)
@@ -302,7 +306,7 @@ program ether_gpu_driver
 
    ! ---- supercell: k (z) outer, j, i, basis l inner (src/generate_supercell.f90:115-132)
    n = sc(1)*sc(2)*sc(3)*nb
-   allocate (species(n), stagger(n), S0(3, n), phi0(n), stg_id(nspecies))
+   allocate (species(n), stagger(n), S0(3, n), phi0(n), stg_id(nspecies), pos(3, n))
    stg_id = 1.0_dp
    inquire (file='staggered', exist=ok)
    if (stg .and. ok .and. .not. cb) then
@@ -323,6 +327,7 @@ program ether_gpu_driver
             do l = 1, nb
                q = q + 1
                species(q) = b_sp(l)
+               pos(:, q) = lat(:, 1)*(i - 1) + lat(:, 2)*(j - 1) + lat(:, 3)*(k - 1) + b_xyz(:, l)
                if (cb) then
                   stagger(q) = merge(1.0_dp, -1.0_dp, mod(i + j + k + l, 2) == 0)
                else
@@ -446,6 +451,15 @@ program ether_gpu_driver
    q = 0
    if (ios == 0) read (line, *, iostat=ios) q
    call chk(ether_set_compat(ctx, int(q, c_int)))
+   ! spiral-state partners (src/ether.f90:69, src/get_spiral_state.f90:47-82)
+   sp_count = 0
+   if (ssp .and. .not. ising) then
+      allocate (sp_partners(n*spiral_capacity))
+      rc = ether_build_spiral_partners(n, pos, ss_direc, ss_dis, ss_lat, spiral_capacity, &
+                                       sp_partners, sp_count)
+      if (rc /= ETHER_OK) call terminate(ether_message(c_null_ptr))
+      call chk(ether_set_spiral(ctx, spiral_capacity, sp_partners))
+   end if
    call chk(ether_commit(ctx))
    call chk(ether_set_betas(ctx, betas))
    if (pt) then
@@ -536,24 +550,64 @@ program ether_gpu_driver
       accp(t) = accp(t)/(nrepeat*tmcs)*100.0_dp
    end do
 
+   ! spiral state per repeat and over repeats (src/get_spiral_state.f90:122-140,
+   ! src/observables_per_repeat.f90:71-94)
+   if (sp_count > 0) then
+      allocate (sp_sums(3, R), sp_per(3, nrepeat, nscan), sp_mean(3, nscan), sp_sem(3, nscan))
+      call chk(ether_read_spiral(ctx, sp_sums, c_null_ptr))
+      do rep = 0, nrepeat - 1
+         do t = 1, nscan
+            q = rep*nscan + t
+            nn = mom(1, q)
+            e = sp_sums(1, q)/nn
+            e2 = sp_sums(2, q)/nn
+            e4 = sp_sums(3, q)/nn
+            sp_per(1, rep + 1, t) = e
+            sp_per(2, rep + 1, t) = 1.5_dp - 0.5_dp*(e4/(e2**2))
+            sp_per(3, rep + 1, t) = betas(q)*(e2 - e**2)*real(sp_count, dp)
+         end do
+      end do
+      do t = 1, nscan
+         do k = 1, 3
+            sp_mean(k, t) = sum(sp_per(k, :, t))/nrepeat
+            sp_sem(k, t) = sqrt(r1*sum((sp_per(k, :, t) - sp_mean(k, t))**2))
+         end do
+      end do
+   end if
+
+   ! output headers as src/write_output_files.f90:35-66 writes them
    open (newunit=u, file='magnetization.dat', status='replace', action='write')
-   write (u, '(7a13)') '# 1 Temp.', '2 |M|', '3 chi', '4 d|M|', '5 dchi', '6 U(|M|)', '7 dU(|M|)'
+   hd(1:7) = [character(len=20) :: '# 1 Temp.', '2 |M|', '3 χ', '4 Δ|M|', '5 Δχ', '6 U(|M|)', '7 ΔU(|M|)']
+   write (u, 103) (hd(i), i = 1, 7)
    do t = 1, nscan
       write (u, 102) temps(t), mean(4, t), mean(6, t), sem(4, t), sem(6, t), mean(5, t), sem(5, t)
    end do
    close (u)
    open (newunit=u, file='energy.dat', status='replace', action='write')
+   hd = [character(len=20) :: '# 1 Temp.', '2 E', '3 Cv/kB', '4 ΔE', '5 Δ(Cv/kB)', '6 U(E)', '7 ΔU(E)', &
+         '8 Accpt. (%)']
    if (para) then
-      write (u, '(8a13)') '# 1 Temp.', '2 E', '3 Cv', '4 dE', '5 dCv', '6 U(E)', '7 dU(E)', '8 Accpt. (%)'
-   else
-      write (u, '(8a13)') '# 1 Temp.', '2 E', '3 Cv/kB', '4 dE', '5 d(Cv/kB)', '6 U(E)', '7 dU(E)', &
-         '8 Accpt. (%)'
+      hd(3) = '3 Cv'
+      hd(5) = '5 ΔCv'
    end if
+   write (u, 104) (hd(i), i = 1, 8)
    do t = 1, nscan
       write (u, 102) temps(t), mean(1, t), mean(3, t), sem(1, t), sem(3, t), mean(2, t), sem(2, t), accp(t)
    end do
    close (u)
+   open (newunit=u, file='spiral_state_parameters.dat', status='replace', action='write')
+   if (sp_count > 0) then
+      hd(1:7) = [character(len=20) :: '# 1 Temp.', '2 |ψ|', '3 ψ', '4 Δ|ψ|', '5 Δψ', '6 U(|ψ|)', '7 ΔU(|ψ|)']
+      write (u, 103) (hd(i), i = 1, 7)
+      do t = 1, nscan
+         write (u, 102) temps(t), sp_mean(1, t), sp_mean(3, t), sp_sem(1, t), sp_sem(3, t), &
+            sp_mean(2, t), sp_sem(2, t)
+      end do
+   end if
+   close (u)
 102 format(f11.5, 1x, es12.5, 1x, es12.5, 1x, es12.5, 1x, es12.5, 1x, es12.5, 1x, es12.5, 1x, f7.2)
+103 format(A13, A13, A13, 1x, A13, 1x, A13, 2x, A13, A13)
+104 format(A13, A13, A13, A13, 1x, A13, 1x, A13, A13, A13)
 
    write (s1, '(f8.3)') real(t1 - t0, dp)/real(rate, dp)
    write (s2, '(es8.2)') real(R, dp)*n*tmcs/(real(t1 - t0, dp)/real(rate, dp))

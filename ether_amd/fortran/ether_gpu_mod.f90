@@ -131,6 +131,30 @@ module ether_gpu
          type(c_ptr), value :: ctx
       end function
 
+      integer(c_int) function ether_build_spiral_partners(n_sites, pos, dir, dis, latency_deg, &
+            capacity, partners, count) bind(C, name="ether_build_spiral_partners")
+         import :: c_int, c_int64_t, c_double
+         integer(c_int), value :: n_sites, capacity
+         real(c_double), intent(in) :: pos(*), dir(*)
+         real(c_double), value :: dis, latency_deg
+         integer(c_int), intent(out) :: partners(*)
+         integer(c_int64_t), intent(out) :: count
+      end function
+
+      integer(c_int) function ether_set_spiral(ctx, capacity, partners) bind(C, name="ether_set_spiral")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: capacity
+         integer(c_int), intent(in) :: partners(*)
+      end function
+
+      integer(c_int) function ether_read_spiral(ctx, moments, psi) bind(C, name="ether_read_spiral")
+         import :: c_ptr, c_int, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: moments(*)
+         type(c_ptr), value :: psi
+      end function
+
       integer(c_int) function ether_read_moments(ctx, moments) bind(C, name="ether_read_moments")
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: ctx

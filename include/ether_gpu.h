@@ -96,6 +96,22 @@ int ether_build_neighbours(const double* abc, int32_t n_basis, const double* bas
                            int32_t capacity, int32_t* nn_count, int32_t* nn_idx);
 
 /* ---------------------------------------------------------------------------
+ * Spiral-state partners: replaces get_spiral_state('ions_for_spiral')
+ * (src/get_spiral_state.f90:47-82), which fills ifs(spiral_capacity, N) and
+ * spiral_count.  pos[3N] Cartesian site positions (ion(6:8,:)), dir[3] =
+ * ss_direc, dis = ss_dis, latency_deg = ss_lat (src/read_input.f90:128-139).
+ * Site j != i is a partner of i when |r_j - r_i| <= dis and
+ * acos((r_j - r_i).dir / |r_j - r_i|) in degrees <= latency_deg; no periodic
+ * images.  partners[N*capacity] out: 0-based partner ids in ascending order,
+ * -1 after the last one; count out (nullable) = spiral_count.  More than
+ * `capacity` partners for a site -> ETHER_ERR_LIMIT.  Context-free: errors are
+ * read with ether_last_error(NULL, ...).
+ * ------------------------------------------------------------------------- */
+int ether_build_spiral_partners(int32_t n_sites, const double* pos, const double* dir,
+                                double dis, double latency_deg, int32_t capacity,
+                                int32_t* partners, int64_t* count);
+
+/* ---------------------------------------------------------------------------
  * Context life cycle
  * ------------------------------------------------------------------------- */
 
@@ -195,6 +211,19 @@ int ether_get_energy_parts(ether_ctx* ctx, double* parts, int64_t* bonds, int32_
 
 int ether_read_moments(ether_ctx* ctx, double* moments /* [R*ETHER_MOM_COUNT] */);
 int ether_reset_moments(ether_ctx* ctx);   /* zeroes('avg') (src/zeroes.f90:51-58) */
+
+/* Spiral-state parameter (ssp, XYZ only; src/get_spiral_state.f90:84-120).
+ * ether_set_spiral, before ether_commit: partners[N*capacity] as produced by
+ * ether_build_spiral_partners (0-based, -1 after a row's last partner);
+ * spiral_count = the number of partner entries; capacity 0 switches it off.
+ * With partners set, every measure pass also forms psi = |sum_i sum_j
+ * S_i x S_j| / spiral_count per replica, and sampling passes add psi, psi^2,
+ * psi^4 to the replica's temperature slot ('spiral_state' case).
+ * ether_read_spiral: moments[3R] (nullable) = those sums per slot (the sample
+ * count is the ETHER_MOM_N moment); psi[R] (nullable) = psi of each replica's
+ * last measured state.  ether_reset_moments clears the sums too. */
+int ether_set_spiral(ether_ctx* ctx, int32_t capacity, const int32_t* partners);
+int ether_read_spiral(ether_ctx* ctx, double* moments, double* psi);
 
 /* parallel_tempering (src/parallel_tempering.f90:20-110) for every ladder:
  * energies from a fresh measure pass, pairs (k,k+1) with k%2 == parity,

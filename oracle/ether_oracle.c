@@ -546,6 +546,61 @@ void o_over_repeats(const double* per, int32_t n_repeat, double* mean6, double* 
     }
 }
 
+/* src/get_spiral_state.f90:47-82 */
+int64_t o_spiral_partners(int32_t n, const double* pos, const double* dir, double dis,
+                          double latency_deg, int32_t cap, int32_t* partners) {
+    const double to_angle = 180.0 / (4.0 * atan(1.0));   /* src/init.f90:44,55 */
+    int64_t count = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        int32_t found = 0;
+        for (int32_t k = 0; k < cap; ++k) partners[(int64_t)i * cap + k] = -1;
+        for (int32_t j = 0; j < n; ++j) {
+            if (j == i) continue;
+            double r[3], d2 = 0.0;
+            for (int a = 0; a < 3; ++a) {
+                r[a] = pos[3 * j + a] - pos[3 * i + a];
+                d2 += r[a] * r[a];
+            }
+            double d = sqrt(d2);
+            if (!(d <= dis)) continue;
+            double c = 0.0;
+            for (int a = 0; a < 3; ++a) c += (r[a] / d) * dir[a];
+            if (!(acos(c) * to_angle <= latency_deg)) continue;
+            if (found == cap) return -1;
+            partners[(int64_t)i * cap + found++] = j;
+            ++count;
+        }
+    }
+    return count;
+}
+
+/* src/get_spiral_state.f90:84-120 */
+double o_spiral_psi(int32_t n, const double* S, const int32_t* partners, int32_t cap,
+                    int64_t spiral_count) {
+    double ss[3] = {0.0, 0.0, 0.0};
+    for (int32_t i = 0; i < n; ++i) {
+        const double* a = S + 3 * i;
+        for (int32_t k = 0; k < cap; ++k) {
+            int32_t j = partners[(int64_t)i * cap + k];
+            if (j < 0) break;
+            const double* b = S + 3 * j;
+            ss[0] += a[1] * b[2] - a[2] * b[1];
+            ss[1] += a[2] * b[0] - a[0] * b[2];
+            ss[2] += a[0] * b[1] - a[1] * b[0];
+        }
+    }
+    return sqrt(ss[0] * ss[0] + ss[1] * ss[1] + ss[2] * ss[2]) / (double)spiral_count;
+}
+
+/* src/get_spiral_state.f90:122-140 */
+void o_spiral_observables(const double* sums, double n_samples, double beta, int64_t spiral_count,
+                          double* out3) {
+    double p = sums[0] / n_samples, p2 = sums[1] / n_samples, p4 = sums[2] / n_samples;
+    out3[0] = p;
+    out3[1] = 1.5 - 0.5 * (p4 / (p2 * p2));
+    out3[2] = beta * (p2 - p * p) * (double)spiral_count;
+}
+
 /* src/parallel_tempering.f90:77-89 */
 int o_pt_accept(double beta, double beta_p, double E, double E_p, double u) {
     double delta = -(beta_p - beta) * (E - E_p);

@@ -25,6 +25,9 @@
  *   overrelaxation  involution, |S|=1, energy conservation
  *   estimators      closed forms on synthetic sample series
  *   tempering       two-level analytic acceptance
+ *   spiral state    partner counts on a cubic grid (one +z partner per site
+ *                   below the top layer), psi of rotated planar spirals
+ *                   (= sin of the pitch angle), estimators on constant series
  *   mlp             central finite differences
  *   structure fac.  FM S(0)=1, Neel peak, direct FP64 sum
  */
@@ -143,6 +146,22 @@ void o_run_chain(const o_model* m, double* S, double* phi, const int32_t* order,
 void o_repeat_observables(const double* mom, double beta, int32_t n_sites, double* out6);
 /* Over repeats (src/observables_per_repeat.f90:29-69): per[repeat*6] -> mean[6], sem[6] */
 void o_over_repeats(const double* per, int32_t n_repeat, double* mean6, double* sem6);
+
+/* Spiral state (src/get_spiral_state.f90).  Partners, case 'ions_for_spiral'
+ * (:47-82): the reference's O(N^2) scan over j != i in ascending order; pos
+ * [3N] Cartesian.  partners[N*cap] gets 0-based ids, -1 after a row's last;
+ * returns spiral_count, or -1 if a site has more than cap partners. */
+int64_t o_spiral_partners(int32_t n, const double* pos, const double* dir, double dis,
+                          double latency_deg, int32_t cap, int32_t* partners);
+/* Case 'spiral_state' (:84-120): psi = |sum_i sum_j S_i x S_j| / spiral_count,
+ * S[3N] one replica's spins, in the reference's summation order. */
+double o_spiral_psi(int32_t n, const double* S, const int32_t* partners, int32_t cap,
+                    int64_t spiral_count);
+/* Case 'avg_spiral_state' (:122-140): sums = {sum psi, psi^2, psi^4} over
+ * n_samples -> out3 = {<psi>, U = 1.5 - 0.5 <psi^4>/<psi^2>^2,
+ * chi = beta (<psi^2> - <psi>^2) spiral_count}. */
+void o_spiral_observables(const double* sums, double n_samples, double beta, int64_t spiral_count,
+                          double* out3);
 
 /* PT decision (src/parallel_tempering.f90:77-89) */
 int o_pt_accept(double beta, double beta_p, double E, double E_p, double u);

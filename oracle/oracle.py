@@ -78,6 +78,11 @@ def lib():
         L.o_repeat_observables.argtypes = [_vp, C.c_double, C.c_int32, _vp]
         L.o_over_repeats.argtypes = [_vp, C.c_int32, _vp, _vp]
         L.o_pt_accept.restype = C.c_int
+        L.o_spiral_partners.restype = C.c_int64
+        L.o_spiral_partners.argtypes = [C.c_int32, _vp, _vp, C.c_double, C.c_double, C.c_int32, _vp]
+        L.o_spiral_psi.restype = C.c_double
+        L.o_spiral_psi.argtypes = [C.c_int32, _vp, _vp, C.c_int32, C.c_int64]
+        L.o_spiral_observables.argtypes = [_vp, C.c_double, C.c_double, C.c_int64, _vp]
         L.o_pt_accept.argtypes = [C.c_double] * 5
         L.o_mlp_forward.argtypes = [_vp, C.c_int32, _vp, _vp]
         L.o_mlp_loss_grad.restype = C.c_double
@@ -280,6 +285,29 @@ def over_repeats(per):
     sem = np.zeros(6)
     lib().o_over_repeats(_p(p), p.shape[0], _p(mean), _p(sem))
     return mean, sem
+
+
+def spiral_partners(pos, direc, dis, latency_deg, cap=5):
+    """(partners (N, cap) int32 -1 padded, spiral_count); count -1 on overflow."""
+    p = _a(pos, np.float64).reshape(-1, 3)
+    d = _a(direc, np.float64).reshape(3)
+    out = np.full(p.shape[0] * cap, -1, np.int32)
+    cnt = lib().o_spiral_partners(p.shape[0], _p(p), _p(d), float(dis), float(latency_deg), int(cap),
+                                  _p(out))
+    return out.reshape(p.shape[0], cap), int(cnt)
+
+
+def spiral_psi(S, partners, count):
+    s = _a(S, np.float64).reshape(-1, 3)
+    pr = _a(partners, np.int32)
+    return lib().o_spiral_psi(s.shape[0], _p(s), _p(pr), pr.shape[-1], int(count))
+
+
+def spiral_observables(sums3, n_samples, beta, count):
+    s = _a(sums3, np.float64)
+    out = np.zeros(3)
+    lib().o_spiral_observables(_p(s), float(n_samples), float(beta), int(count), _p(out))
+    return out
 
 
 def pt_accept(beta, beta_p, E, E_p, u):

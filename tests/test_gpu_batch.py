@@ -4,7 +4,8 @@ Each lattice is an ordinary engine (its own sites, couplings, replicas,
 temperatures, ladders and seed).  A Batch advances all of them through the
 schedule of ether_run with batched launches and graph replays; every
 lattice's spins, moments and tempering state must be bit-identical to the
-same engine run on its own.
+same engine run on its own (spiral-state sums included where a lattice has
+spiral partners).
 """
 import numpy as np
 import pytest
@@ -37,12 +38,16 @@ def _engine(name, R, seed, over, temps):
 
 def _state(eng):
     spins = np.stack([eng.get_spins(r)[0] for r in range(eng.R)])
-    return spins, eng.moments(), [np.asarray(x) for x in eng.pt_state()]
+    extra = [np.asarray(x) for x in eng.pt_state()]
+    if getattr(eng, "spiral_count", 0):
+        extra += list(eng.spiral())
+    return spins, eng.moments(), extra
 
 
 XYZ = [
-    ("sc_heis_afm_32", 20, 11, dict(sc="6 6 6"), np.linspace(60, 6, 20)),
-    ("tri_afm_96", 13, 12, dict(sc="12 12 1", sia=".F."), np.linspace(20, 2, 13)),       # ANGLE, 3 colours
+    ("sc_heis_afm_32", 20, 11, dict(sc="6 6 6", ssp=".T.", ss_dis="1.01"), np.linspace(60, 6, 20)),
+    ("tri_afm_96", 13, 12, dict(sc="12 12 1", sia=".F.", ssp=".T.", ss_dis="1.01", ss_direc="1 0 0"),
+     np.linspace(20, 2, 13)),                                                              # ANGLE, 3 colours
     ("pyro_pt_16", 16, 13, dict(sc="2 2 2", pt=".T. 11"), np.geomspace(20, 1, 16)),        # 4 colours, D 18
     ("sc_heis_afm_32", 70, 14, dict(sc="4 4 4"), np.linspace(40, 4, 70)),                  # two replica groups
 ]
@@ -79,7 +84,7 @@ def test_batch_equals_lattices_run_alone(cases, graphs):
         assert np.array_equal(sa, sb), f"{m.tags.model} N={m.n_sites}: spins differ"
         assert np.array_equal(ma, mb), f"{m.tags.model} N={m.n_sites}: moments differ"
         for x, y in zip(pa, pb):
-            assert np.array_equal(x, y), f"{m.tags.model} N={m.n_sites}: tempering state differs"
+            assert np.array_equal(x, y), f"{m.tags.model} N={m.n_sites}: tempering/spiral state differs"
         assert ma[:, E.MOM_N].sum() > 0
     batch.close()
     for (m, a), b in zip(alone, batched):

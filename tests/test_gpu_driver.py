@@ -4,7 +4,9 @@ The Fortran driver (ether_amd/lib/ether_gpu_driver, iso_c_binding) and the
 Python driver (ether_amd.driver, ctypes) read the same run directory and use
 the same seed.  For Ising and for Marsaglia-move Heisenberg runs the device
 state after fresh_spins does not depend on the host-side start state, so both
-drivers must write identical energy.dat / magnetization.dat numbers.
+drivers must write identical energy.dat / magnetization.dat numbers (and
+spiral_state_parameters.dat with the spiral state on), with identical header
+records.
 A simple-cubic Heisenberg ferromagnet run also checks the physics: the energy
 per spin is monotone in T and approaches the ground-state value 3J at low T.
 """
@@ -30,7 +32,8 @@ def _table(path):
 
 @pytest.mark.parametrize("name,over", [
     ("sc_ising_16", dict(sc="6 6 6", mcs="400 200", repeat="3", temp="100 10 30")),
-    ("sc_heis_afm_32", dict(sc="6 6 6", mcs="400 200", repeat="3", temp="60 6 18")),
+    ("sc_heis_afm_32", dict(sc="6 6 6", mcs="400 200", repeat="3", temp="60 6 18",
+                            ssp=".T.", ss_dis="1.01")),
     ("pyro_pt_16", dict(sc="2 2 2", mcs="440 220", repeat="2", temp="1 0.05 0.19")),
 ])
 def test_fortran_and_python_drivers_agree(tmp_path, name, over):
@@ -41,11 +44,20 @@ def test_fortran_and_python_drivers_agree(tmp_path, name, over):
     assert r.returncode == 0, r.stdout + r.stderr
     res = driver.run(read_directory(str(tmp_path)))
     res.write(str(tmp_path / "py"))
-    for f in ("energy.dat", "magnetization.dat"):
+    files = ["energy.dat", "magnetization.dat"]
+    if "ssp" in over:
+        files.append("spiral_state_parameters.dat")
+    else:
+        assert os.path.getsize(tmp_path / "spiral_state_parameters.dat") == 0
+    for f in files:
         a = _table(tmp_path / f)
         b = _table(tmp_path / "py" / f)
-        assert a.shape == b.shape
+        assert a.shape == b.shape and a.shape[0] > 0
         assert np.allclose(a, b, rtol=2e-5, atol=2e-5), f
+        # header records are the reference's A13 edits of character(len=20) labels
+        ha = open(tmp_path / f, encoding="utf-8").readline()
+        hb = open(tmp_path / "py" / f, encoding="utf-8").readline()
+        assert ha == hb, f
 
 
 def test_heisenberg_ferromagnet_energy_curve():

@@ -19,6 +19,7 @@ from typing import Optional
 import numpy as np
 
 from . import engine as E
+from . import outputs
 from .inputs import Model, fortran_nint
 
 
@@ -118,12 +119,17 @@ class Results:
     para: bool = False
     spiral_mean: Optional[np.ndarray] = None   # (nscan, 3): psi, U_psi, chi_psi
     spiral_sem: Optional[np.ndarray] = None
+    final_S: Optional[np.ndarray] = None       # (nscan, N, 3) at each T after the last repeat
+    final_phi: Optional[np.ndarray] = None     # (nscan, N)
+    model: Optional[Model] = None
 
     def write(self, outdir: str) -> None:
         """energy.dat / magnetization.dat / spiral_state_parameters.dat in the
         reference's layout (src/write_output_files.f90:27-98, rows in format
         (f11.5,6(1x,es12.5),1x,f7.2)).  spiral_state_parameters.dat is created
-        in every run and holds rows only with the spiral state on."""
+        in every run and holds rows only with the spiral state on.  With the
+        final configurations at hand, ETHER.obs, ETHER.spn, gss.dat,
+        moment_vectors.dat and the spK files follow (ether_amd.outputs)."""
         os.makedirs(outdir, exist_ok=True)
 
         def row(vals, acc=None):
@@ -146,6 +152,8 @@ class Results:
                 for k, T in enumerate(self.temps):
                     m, s = self.spiral_mean[k], self.spiral_sem[k]
                     f.write(row([T, m[0], m[2], s[0], s[2], m[1], s[1]]) + "\n")
+        if self.final_S is not None and self.model is not None:
+            outputs.write_all(outdir, self, self.model)
 
 
 def run(model: Model, *, compat: int = 0, chunk: int = 0, temps=None, repeat: Optional[int] = None,
@@ -210,6 +218,8 @@ def run(model: Model, *, compat: int = 0, chunk: int = 0, temps=None, repeat: Op
     if t.pt:
         _, pacc, patt, trips = eng.pt_state()
         res.pt_acc, res.pt_att, res.trips = pacc, patt, trips
+    res.final_S, res.final_phi = outputs.final_states(eng, nscan, rep)
+    res.model = model
     if own:
         eng.close()
     return res

@@ -264,6 +264,12 @@ program ether_gpu_driver
    integer(c_int64_t) :: sp_count
    real(dp), allocatable :: sp_sums(:, :), sp_per(:, :, :), sp_mean(:, :), sp_sem(:, :), pos(:, :)
    character(len=20) :: hd(8)
+   character(len=30) :: fname
+   integer :: reclo, recls, ug, uo, us, uv
+   integer(c_int), allocatable :: slot_of(:), rep_at(:)
+   integer(c_int64_t), allocatable :: acc64(:), att64(:), trips64(:)
+   integer, allocatable :: tions(:), inc(:)
+   real(dp), allocatable :: sfin(:, :), phifin(:), ionrec(:, :), obs(:), mmv(:, :)
    real(dp), allocatable :: b_xyz(:, :), stagger(:), S0(:, :), phi0(:), shells(:), jx(:, :, :, :)
    real(dp), allocatable :: siav(:, :), temps(:), betas(:), mom(:, :), per(:, :, :)
    real(dp), allocatable :: mean(:, :), sem(:, :), accp(:), stg_id(:)
@@ -609,10 +615,116 @@ program ether_gpu_driver
 103 format(A13, A13, A13, 1x, A13, 1x, A13, 2x, A13, A13)
 104 format(A13, A13, A13, A13, 1x, A13, 1x, A13, A13, A13)
 
+   ! ---- the configuration at each temperature after the last repeat, and the
+   !      files built from it: ETHER.obs / ETHER.spn direct-access records
+   !      (src/process_observables.f90:34-77, src/ether.f90:216-231), gss.dat
+   !      (src/write_gss.f90:34-78), moment_vectors.dat
+   !      (src/write_output_files.f90:66-80) and <T>spK.xsf
+   !      (src/write_spins_at_K.f90:33-66).  Replica r = rep*nscan + t holds slot
+   !      slot_of(r); the one in slot (nrepeat-1)*nscan + t is at temperature t.
+   allocate (slot_of(R), rep_at(R), acc64(R), att64(R), trips64(R))
+   allocate (sfin(3, n), phifin(n), ionrec(0:8, n), obs(20 + 3*nspecies), mmv(3, nspecies), tions(nspecies))
+   call chk(ether_pt_state(ctx, slot_of, acc64, att64, trips64))
+   do q = 1, R
+      rep_at(slot_of(q) + 1) = q - 1
+   end do
+   tions = 0
+   do i = 1, n
+      tions(species(i)) = tions(species(i)) + 1
+   end do
+   cnt = 0
+   allocate (inc(size(species_incl)))
+   do i = 1, size(species_incl)
+      do j = 1, nspecies
+         if (species_incl(i) == species_name(j)) then
+            cnt = cnt + 1
+            inc(cnt) = j
+         end if
+      end do
+   end do
+   inquire (iolength=reclo) obs
+   inquire (iolength=recls) ionrec
+   open (newunit=uo, file='ETHER.obs', form='unformatted', access='direct', recl=reclo, status='replace')
+   open (newunit=us, file='ETHER.spn', form='unformatted', access='direct', recl=recls, status='replace')
+   open (newunit=ug, file='gss.dat', status='replace', action='write')
+   write (ug, '(I5,1x, I5, 1x, I5, 1x, I5)') nscan, sc
+   write (ug, '(I5,1x, I5, 1x, I5, 1x, I5, 1x, I5, 1x, I5)') 1, 1, 1, sc
+   write (ug, '(I5,1x, I5, f11.5, 1x, f11.5, 1x, f11.5,1x, A6)') nb, nspecies, ht, lt, tint
+   fname = ' K'
+   if (para) fname = ' KbT/J'
+   write (ug, '(A20)') fname
+   write (ug, '(I8,1x,I8,1x,I8,1x,I8,1x,I8)') tions
+   do i = 1, 3
+      write (ug, '(f10.5,1x,f10.5,1x,f10.5)') lat(:, i)
+   end do
+   write (ug, *) '# lattice points'
+   do i = 1, n
+      write (ug, '(f11.6, 1x, f11.6, 1x, f11.6, 1X, i8)') pos(:, i), i
+   end do
+   open (newunit=uv, file='moment_vectors.dat', status='replace', action='write')
+   write (uv, *) '# Showing moment vectors (Mx, My, Mz) for species(sp):'
+   write (uv, 105) (species_name(inc(i)), i = 1, cnt)
+   write (uv, *) '# Temp, sp1(Mx,My,Mz), sp1(Mx,My,Mz),..spN(Mx,My,Mz), sp1|M|, sp2|M|,..spN|M|'
+   do t = 1, nscan
+      call chk(ether_get_spins(ctx, rep_at((nrepeat - 1)*nscan + t), sfin, phifin))
+      mmv = 0.0_dp
+      do i = 1, n
+         mmv(:, species(i)) = mmv(:, species(i)) + stagger(i)*sfin(:, i)*spin(species(i))
+      end do
+      obs = 0.0_dp
+      obs(1:14) = [temps(t), mean(4, t), mean(6, t), sem(4, t), sem(6, t), mean(5, t), sem(5, t), &
+                   mean(1, t), mean(3, t), sem(1, t), sem(3, t), mean(2, t), sem(2, t), accp(t)]
+      if (sp_count > 0) obs(15:20) = [sp_mean(1, t), sp_mean(3, t), sp_sem(1, t), sp_sem(3, t), &
+                                      sp_mean(2, t), sp_sem(2, t)]
+      do j = 1, nspecies
+         obs(20 + 3*(j - 1) + 1:20 + 3*j) = mmv(:, j)
+      end do
+      write (uo, rec=t) obs
+      do i = 1, n
+         ionrec(0, i) = real(i, dp)
+         ionrec(1:3, i) = sfin(:, i)
+         ionrec(4, i) = real(species(i), dp)
+         ionrec(5, i) = phifin(i)
+         ionrec(6:8, i) = pos(:, i)
+      end do
+      write (us, rec=t) ionrec
+      write (ug, '(f11.5)') temps(t)
+      do i = 1, n
+         write (ug, '(f11.6, 1x, f11.6, 1x, f11.6, 1X, i8)') sfin(:, i), species(i)
+      end do
+      write (uv, 101) temps(t), ((mmv(:, inc(i))/tions(inc(i))), i = 1, cnt), &
+         (sqrt(dot_product(mmv(:, inc(i)), mmv(:, inc(i))))/tions(inc(i)), i = 1, cnt)
+      write (fname, '(f9.4,"spK.xsf")') temps(t)
+      open (newunit=u, file=adjustl(fname), status='replace', action='write')
+      write (u, *) 'CRYSTAL'
+      write (u, *) 'PRIMVEC'
+      do a = 1, 3
+         write (u, *) lat(:, a)*sc(a)
+      end do
+      write (u, *) 'CONVEC'
+      do a = 1, 3
+         write (u, *) lat(:, a)*sc(a)
+      end do
+      write (u, *) 'PRIMCOORD'
+      write (u, *) n, " 1"
+      do i = 1, n
+         write (u, 106) species_name(species(i)), pos(:, i), sfin(:, i)
+      end do
+      close (u)
+   end do
+   close (uo)
+   close (us)
+   close (ug)
+   close (uv)
+101 format(f11.5, 2X, 500f9.3)
+105 format(' #	', 5A3)
+106 format(A5, 1x, f13.7, 1x, f13.7, 1x, f13.7, 1x, f13.7, 1x, f13.7, 1x, f13.7)
+
    write (s1, '(f8.3)') real(t1 - t0, dp)/real(rate, dp)
    write (s2, '(es8.2)') real(R, dp)*n*tmcs/(real(t1 - t0, dp)/real(rate, dp))
    write (6, '(a)') ' ==> MCS loop: '//trim(adjustl(s1))//' s, '//trim(adjustl(s2))//' spin updates/s'
-   write (6, '(a)') ' ==> energy.dat and magnetization.dat written'
+   write (6, '(a)') ' ==> energy.dat, magnetization.dat, spiral_state_parameters.dat, ETHER.obs, '// &
+      'ETHER.spn, gss.dat, moment_vectors.dat and spK.xsf files written'
    call ether_ctx_destroy(ctx)
 
 contains

@@ -6,7 +6,8 @@ the same seed.  For Ising and for Marsaglia-move Heisenberg runs the device
 state after fresh_spins does not depend on the host-side start state, so both
 drivers must write identical energy.dat / magnetization.dat numbers (and
 spiral_state_parameters.dat with the spiral state on), with identical header
-records.
+records, and identical ETHER.spn / ETHER.obs records, gss.dat,
+moment_vectors.dat and spK.xsf files.
 A simple-cubic Heisenberg ferromagnet run also checks the physics: the energy
 per spin is monotone in T and approaches the ground-state value 3J at low T.
 """
@@ -16,7 +17,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from ether_amd import driver, workloads
+from ether_amd import driver, outputs, workloads
 from ether_amd.inputs import read_directory
 from tests.helpers import say
 
@@ -58,6 +59,39 @@ def test_fortran_and_python_drivers_agree(tmp_path, name, over):
         ha = open(tmp_path / f, encoding="utf-8").readline()
         hb = open(tmp_path / "py" / f, encoding="utf-8").readline()
         assert ha == hb, f
+    _same_state_files(tmp_path, res)
+
+
+def _same_state_files(tmp_path, res):
+    """Files built from the final configurations: both drivers read the same
+    device state, so the records agree bit for bit and the text records byte
+    for byte (the Fortran driver formats them with the reference's own format
+    statements; list-directed xsf header lines are compared as numbers)."""
+    m = res.model
+    N, nscan = m.n_sites, len(res.temps)
+    a = np.fromfile(tmp_path / "ETHER.spn", "<f8")
+    b = np.fromfile(tmp_path / "py" / "ETHER.spn", "<f8")
+    assert a.size == nscan * N * 9 and np.array_equal(a, b)
+    nrec = outputs.TOTAL_OBSERVABLES + 3 * m.n_species
+    a = outputs.read_direct(str(tmp_path / "ETHER.obs"), nrec)
+    b = outputs.read_direct(str(tmp_path / "py" / "ETHER.obs"), nrec)
+    assert a.shape == (nscan, nrec)
+    assert np.array_equal(a[:, 20:], b[:, 20:])                 # moment vectors: same sums
+    assert np.allclose(a, b, rtol=1e-12, atol=1e-14)            # estimators: FP64 in two languages
+    for f in ("gss.dat", "moment_vectors.dat"):
+        ta = open(tmp_path / f, encoding="utf-8").read()
+        tb = open(tmp_path / "py" / f, encoding="utf-8").read()
+        assert ta == tb, f
+    for T in res.temps:
+        f = outputs.spk_name(T)
+        la = open(tmp_path / f).read().splitlines()
+        lb = open(tmp_path / "py" / f).read().splitlines()
+        assert len(la) == len(lb) == 11 + N
+        assert la[11:] == lb[11:]
+        for x, y in zip(la[:11], lb[:11]):
+            assert x.split()[0].isalpha() == y.split()[0].isalpha()
+            if not x.split()[0].isalpha():
+                assert np.allclose([float(v) for v in x.split()], [float(v) for v in y.split()])
 
 
 def test_heisenberg_ferromagnet_energy_curve():

@@ -14,13 +14,13 @@ import math
 import os
 import time
 from dataclasses import dataclass
-from typing import Optional
+from typing import List, Optional, Sequence
 
 import numpy as np
 
 from . import engine as E
 from . import outputs
-from .inputs import Model, fortran_nint
+from .inputs import EtherInputError, GpuTags, Model, fortran_nint
 
 
 def attach_neighbours(model: Model) -> Model:
@@ -53,6 +53,16 @@ def make_engine(model: Model, n_replicas: int, *, compat: int = 0, seed: Optiona
               sia=model.sia_vec.reshape(-1) if model.sia_vec is not None else None,
               angle=t.angle, dphi=t.dphi, compat=compat, colour=colour, spiral=spiral)
     return eng
+
+
+def apply_gpu_tags(eng: E.Engine, g: GpuTags) -> None:
+    """gpu.ether settings that act on a committed context: graph replay,
+    launch shape and lanes per site (none of them changes any result)."""
+    eng.set_graphs(g.graphs)
+    if g.spw:
+        eng.set_launch(g.spw, g.xcd)
+    if g.lanes:
+        eng.set_lanes(g.lanes)
 
 
 def repeat_observables(mom: np.ndarray, beta: float, n_sites: int) -> np.ndarray:
@@ -122,6 +132,8 @@ class Results:
     final_S: Optional[np.ndarray] = None       # (nscan, N, 3) at each T after the last repeat
     final_phi: Optional[np.ndarray] = None     # (nscan, N)
     model: Optional[Model] = None
+    ladder_source: str = "linear"              # linear, or which trained ladder ran production
+    counted_mcs: int = 0                       # MCS whose acceptances accept_pct averages
 
     def write(self, outdir: str) -> None:
         """energy.dat / magnetization.dat / spiral_state_parameters.dat in the
@@ -156,41 +168,53 @@ class Results:
             outputs.write_all(outdir, self, self.model)
 
 
-def run(model: Model, *, compat: int = 0, chunk: int = 0, temps=None, repeat: Optional[int] = None,
-        eng: Optional[E.Engine] = None, seed: Optional[int] = None) -> Results:
-    """Full run: every temperature x repeat as one batch of device replicas."""
+def _schedule(model: Model):
+    """(coa, OVRR interval, OVRR passes, PT interval) of src/ether.f90:159-198."""
     t = model.tags
-    temps = t.temperatures() if temps is None else np.asarray(temps, np.float64)
-    nscan = len(temps)
-    rep = t.repeat if repeat is None else repeat
-    R = nscan * rep
-    own = eng is None
-    if own:
-        eng = make_engine(model, R, compat=compat, seed=seed)
-    beta_T = model.betas(temps)
-    eng.set_betas(np.tile(beta_T, rep))
-    if t.pt:
-        eng.set_ladders([k * nscan for k in range(rep + 1)])
-    # start state: generate_supercell state, then fresh_spins (src/ether.f90:155)
-    eng.set_spins(-1, model.initial_spins, model.initial_phi)
-    eng.init_spins(E.INIT_FRESH)
-    eng.reset_moments()
     coa = fortran_nint(t.coa)
     if abs(coa - t.coa) > 1e-12 or coa <= 0:
         raise ValueError("coa must be a positive integer")
     # OVRR gate of src/ether.f90:163-167
     ovrr_on = t.ovrr and t.xyz and not t.zeeman and not t.sia and t.ovrr_mcs > 0
-    chunk = chunk or t.tmcs
-    t0 = time.perf_counter()
-    swap = 0
-    step = 1
-    while step <= t.tmcs:
-        n = min(chunk, t.tmcs - step + 1)
-        swap = eng.run(step, n, t.tmcs_eq, coa, t.ovrr_mcs if ovrr_on else 0,
-                       t.ovrr_para if ovrr_on else 0.0, t.exchange_interval if t.pt else 0, swap)
-        step += n
-    mom = eng.moments()   # synchronises
-    wall = time.perf_counter() - t0
+    return (coa, t.ovrr_mcs if ovrr_on else 0, t.ovrr_para if ovrr_on else 0.0,
+            t.exchange_interval if t.pt else 0)
+
+
+def _prepare(model: Model, temps, rep: int, eng: E.Engine) -> None:
+    """Temperatures, ladders and the start state (src/ether.f90:139-156)."""
+    eng.set_betas(np.tile(model.betas(temps), rep))
+    if model.tags.pt:
+        eng.set_ladders([k * len(temps) for k in range(rep + 1)])
+    # start state: generate_supercell state, then fresh_spins (src/ether.f90:155)
+    eng.set_spins(-1, model.initial_spins, model.initial_phi)
+    eng.init_spins(E.INIT_FRESH)
+    eng.reset_moments()
+
+
+def _train(model: Model, temps, rep: int, eng: E.Engine, sched):
+    """gpu.ether `ladder = learned`: train the ladder inside the equilibration
+    MCS and keep the one with the best measured round-trip cost.  Returns
+    (temperatures, next step, swap count, source)."""
+    t, g = model.tags, model.gpu
+    if not t.pt:
+        raise EtherInputError("ladder = learned needs parallel tempering (pt = .T.)")
+    ntr = g.train_mcs or t.tmcs_eq // 2
+    if ntr <= 0 or ntr > t.tmcs_eq:
+        raise EtherInputError("ladder = learned: the training MCS must lie within the equilibration MCS")
+    from .ladder import train_ladder   # ladder imports this module
+    coa, ovrr_iv, ovrr_p, pt_iv = sched
+    tr = train_ladder(eng, model, temps, rep, coa, pt_iv, ntr, windows=g.windows,
+                      ovrr_iv=ovrr_iv, ovrr_p=ovrr_p)
+    eng.set_ladders([k * len(temps) for k in range(rep + 1)])
+    eng.reset_moments()
+    return np.asarray(tr.ladder), tr.next_step, tr.swap_count, tr.source
+
+
+def _results(model: Model, eng: E.Engine, temps, rep: int, mom: np.ndarray, wall: float,
+             counted: int, source: str) -> Results:
+    t = model.tags
+    nscan = len(temps)
+    beta_T = model.betas(temps)
     per = np.zeros((rep, nscan, 6))
     acc = np.zeros(nscan)
     for k in range(rep):
@@ -202,8 +226,10 @@ def run(model: Model, *, compat: int = 0, chunk: int = 0, temps=None, repeat: Op
     sem = np.zeros((nscan, 6))
     for j in range(nscan):
         mean[j], sem[j] = over_repeats(per[:, j, :])
-    res = Results(temps=temps, mean=mean, sem=sem, accept_pct=acc / (rep * t.tmcs) * 100.0,
-                  per_repeat=per, wall_s=wall, updates=float(R) * model.n_sites * t.tmcs, para=t.para)
+    res = Results(temps=np.asarray(temps), mean=mean, sem=sem,
+                  accept_pct=acc / (rep * counted) * 100.0, per_repeat=per, wall_s=wall,
+                  updates=float(eng.R) * model.n_sites * counted, para=t.para,
+                  ladder_source=source, counted_mcs=counted)
     if getattr(eng, "spiral_count", 0):
         sums, _ = eng.spiral()
         sper = np.zeros((rep, nscan, 3))
@@ -220,6 +246,90 @@ def run(model: Model, *, compat: int = 0, chunk: int = 0, temps=None, repeat: Op
         res.pt_acc, res.pt_att, res.trips = pacc, patt, trips
     res.final_S, res.final_phi = outputs.final_states(eng, nscan, rep)
     res.model = model
+    return res
+
+
+def run(model: Model, *, compat: int = 0, chunk: int = 0, temps=None, repeat: Optional[int] = None,
+        eng: Optional[E.Engine] = None, seed: Optional[int] = None) -> Results:
+    """Full run: every temperature x repeat as one batch of device replicas.
+    The run directory's gpu.ether settings (model.gpu) apply on top."""
+    t, g = model.tags, model.gpu
+    temps = t.temperatures() if temps is None else np.asarray(temps, np.float64)
+    nscan = len(temps)
+    rep = t.repeat if repeat is None else repeat
+    R = nscan * rep
+    own = eng is None
+    if own:
+        eng = make_engine(model, R, compat=compat | g.compat, seed=seed)
+        apply_gpu_tags(eng, g)
+    sched = _schedule(model)
+    coa, ovrr_iv, ovrr_p, pt_iv = sched
+    _prepare(model, temps, rep, eng)
+    if g.autotune:
+        eng.autotune(g.autotune)      # restores every replica's state
+    t0 = time.perf_counter()
+    step, swap, source = 1, 0, "linear"
+    if g.ladder == "learned":
+        temps, step, swap, source = _train(model, temps, rep, eng, sched)
+    first = step
+    chunk = g.chunk or chunk or t.tmcs
+    while step <= t.tmcs:
+        n = min(chunk, t.tmcs - step + 1)
+        swap = eng.run(step, n, t.tmcs_eq, coa, ovrr_iv, ovrr_p, pt_iv, swap)
+        step += n
+    mom = eng.moments()   # synchronises
+    wall = time.perf_counter() - t0
+    res = _results(model, eng, temps, rep, mom, wall, t.tmcs - first + 1, source)
     if own:
         eng.close()
     return res
+
+
+def run_batch(models: Sequence[Model], *, seeds: Optional[Sequence[Optional[int]]] = None) -> List[Results]:
+    """A batch manifest's runs (batch.ether): each run directory is its own
+    context (lattice, couplings, temperatures, repeats, seed); runs whose
+    schedules are equal (mcs, coa, OVRR and PT tags) and that share the spin
+    model and the SIA switch advance together through one batched schedule
+    (ether_batch_run), one launch per kernel kind for the whole group.  Every
+    run's results equal those of run() on it alone."""
+    seeds = list(seeds) if seeds is not None else [None] * len(models)
+    for m in models:
+        if m.gpu.ladder != "linear" or m.gpu.autotune:
+            raise EtherInputError("a batched run takes neither `ladder = learned` nor `autotune`")
+    groups = {}
+    for k, m in enumerate(models):
+        key = (m.tags.tmcs, m.tags.tmcs_eq) + _schedule(m) + (m.model_id, m.sia_vec is not None)
+        groups.setdefault(key, []).append(k)
+    out: List[Optional[Results]] = [None] * len(models)
+    for key, idx in groups.items():
+        tmcs, eq, coa, ovrr_iv, ovrr_p, pt_iv = key[:6]
+        engs, temps, batch = [], [], None
+        try:
+            for k in idx:
+                m = models[k]
+                tk = m.tags.temperatures()
+                e = make_engine(m, len(tk) * m.tags.repeat, compat=m.gpu.compat, seed=seeds[k])
+                apply_gpu_tags(e, m.gpu)
+                _prepare(m, tk, m.tags.repeat, e)
+                engs.append(e)
+                temps.append(tk)
+            batch = E.Batch(engs)
+            batch.set_graphs(all(models[k].gpu.graphs for k in idx))
+            t0 = time.perf_counter()
+            chunk = min(models[k].gpu.chunk or tmcs for k in idx)
+            step, swap = 1, 0
+            while step <= tmcs:
+                n = min(chunk, tmcs - step + 1)
+                swap = batch.run(step, n, eq, coa, ovrr_iv, ovrr_p, pt_iv, swap)
+                step += n
+            moms = [e.moments() for e in engs]
+            wall = time.perf_counter() - t0
+            for j, k in enumerate(idx):
+                out[k] = _results(models[k], engs[j], temps[j], models[k].tags.repeat, moms[j], wall,
+                                  tmcs, "linear")
+        finally:
+            if batch is not None:
+                batch.close()          # before its contexts
+            for e in engs:
+                e.close()
+    return out

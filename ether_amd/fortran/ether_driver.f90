@@ -26,6 +26,9 @@ module ether_driver_input
    real(dp) :: nbdfc = 1.0d-5, ss_dis = 4.0_dp, ss_lat = 2.0_dp, ss_direc(3) = (/0.0_dp, 0.0_dp, 1.0_dp/)
    logical :: stg = .false., angle = .false., zeeman = .false., sia = .false., para = .false.
    logical :: ovrr = .false., pt = .false., cb = .false., ssp = .false.
+   ! gpu.ether: engine settings the reference has no tags for
+   integer :: gpu_autotune = 0, gpu_spw = 0, gpu_xcd = 1, gpu_lanes = 0, gpu_compat = 0, gpu_chunk = 0
+   logical :: gpu_graphs = .true.
    character :: bc(3) = (/'c', 'c', 'c'/)
    real(dp), allocatable :: spin(:)
    character(len=2), allocatable :: species_incl(:)
@@ -207,6 +210,58 @@ contains
       close (u)
    end subroutine
 
+   ! gpu.ether (optional): same syntax as in.ether; kept in its own file because
+   ! the reference's parser stops on unknown keys (src/read_input.f90:144-146)
+   subroutine read_gpu_input()
+      character(len=400) :: line, key, value
+      character(len=16) :: word
+      integer :: u, ios, i, j
+      logical :: ok
+      inquire (file='gpu.ether', exist=ok)
+      if (.not. ok) return
+      open (newunit=u, file='gpu.ether', status='old', action='read')
+      do
+         read (u, '(a)', iostat=ios) line
+         if (ios /= 0) exit
+         do i = 1, len(line)
+            if (line(i:i) == achar(9)) line(i:i) = ' '
+         end do
+         i = scan(line, '!#')
+         if (i > 0) line = line(:i - 1)
+         if (len_trim(line) == 0) cycle
+         line = lower(line)
+         j = index(line, '=')
+         if (j <= 0) call terminate("Missing '=' in the given line: "//trim(adjustl(line)))
+         key = adjustl(line(:j - 1))
+         value = line(j + 1:)
+         ios = 0
+         select case (trim(key))
+         case ('ladder')
+            read (value, *, iostat=ios) word
+            if (trim(word) == 'learned') call terminate('ladder = learned is run by the Python driver '// &
+                                                        '(python -m ether_amd)')
+            if (trim(word) /= 'linear') call terminate("ladder must be 'linear' or 'learned'")
+         case ('autotune')
+            read (value, *, iostat=ios) gpu_autotune
+         case ('launch')
+            read (value, *, iostat=ios) gpu_spw, gpu_xcd
+         case ('lanes')
+            read (value, *, iostat=ios) gpu_lanes
+         case ('graphs')
+            read (value, *, iostat=ios) gpu_graphs
+         case ('compat')
+            read (value, *, iostat=ios) gpu_compat
+         case ('chunk')
+            read (value, *, iostat=ios) gpu_chunk
+            if (gpu_chunk < 0) call terminate('chunk must be >= 0')
+         case default
+            call terminate('Found unknown/missing information in gpu.ether: '//trim(adjustl(line)))
+         end select
+         if (ios /= 0) call terminate('cannot read the value of tag '//trim(key))
+      end do
+      close (u)
+   end subroutine
+
    ! read_structure + frac_to_cart (src/read_structure.f90:51-106, src/frac_to_cart.f90:45-49)
    subroutine read_structure()
       character(len=400) :: line
@@ -286,6 +341,7 @@ program ether_gpu_driver
 
    call read_species_count()
    call read_input()
+   call read_gpu_input()
    call read_structure()
    ising = (model == 'ising')
    write (6, '(a)') ' ==> ETHER (MI355X engine) - inputs read'
@@ -456,7 +512,7 @@ program ether_gpu_driver
    call get_environment_variable('ETHER_COMPAT', line, status=ios)
    q = 0
    if (ios == 0) read (line, *, iostat=ios) q
-   call chk(ether_set_compat(ctx, int(q, c_int)))
+   call chk(ether_set_compat(ctx, int(ior(q, gpu_compat), c_int)))
    ! spiral-state partners (src/ether.f90:69, src/get_spiral_state.f90:47-82)
    sp_count = 0
    if (ssp .and. .not. ising) then
@@ -467,6 +523,9 @@ program ether_gpu_driver
       call chk(ether_set_spiral(ctx, spiral_capacity, sp_partners))
    end if
    call chk(ether_commit(ctx))
+   call chk(ether_set_graphs(ctx, merge(1, 0, gpu_graphs)))
+   if (gpu_spw > 0) call chk(ether_set_launch(ctx, gpu_spw, gpu_xcd))
+   if (gpu_lanes > 0) call chk(ether_set_lanes(ctx, gpu_lanes))
    call chk(ether_set_betas(ctx, betas))
    if (pt) then
       allocate (gstart(nrepeat + 1))
@@ -475,14 +534,14 @@ program ether_gpu_driver
       end do
       call chk(ether_set_ladders(ctx, nrepeat, gstart))
    end if
-   ! optional launch-shape autotuning: ETHER_AUTOTUNE = sweeps timed per candidate
+   ! optional launch-shape autotuning: sweeps timed per candidate, from gpu.ether's
+   ! `autotune` or the ETHER_AUTOTUNE environment variable
    call get_environment_variable('ETHER_AUTOTUNE', line, status=ios)
-   if (ios == 0) then
-      read (line, *, iostat=ios) q
-      if (ios == 0 .and. q > 0) then
-         call chk(ether_autotune(ctx, int(q, c_int), spw, xcdm, c_null_ptr))
-         write (6, '(a,i0,a,i0)') ' ==> autotuned sweep launch: sites per wave ', spw, ', XCD map ', xcdm
-      end if
+   q = gpu_autotune
+   if (ios == 0) read (line, *, iostat=ios) q
+   if (q > 0) then
+      call chk(ether_autotune(ctx, int(q, c_int), spw, xcdm, c_null_ptr))
+      write (6, '(a,i0,a,i0)') ' ==> autotuned sweep launch: sites per wave ', spw, ', XCD map ', xcdm
    end if
    ! generate_supercell state, then fresh_spins (src/ether.f90:155)
    call chk(ether_set_spins(ctx, -1, S0, phi0))
@@ -506,6 +565,7 @@ program ether_gpu_driver
    swap_c = 0
    step = 1
    nchunk = 1000
+   if (gpu_chunk > 0) nchunk = gpu_chunk
    do while (step <= tmcs)
       start = step
       q = min(nchunk, tmcs - step + 1)

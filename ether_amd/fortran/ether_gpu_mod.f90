@@ -131,6 +131,24 @@ module ether_gpu
          type(c_ptr), value :: ctx
       end function
 
+      integer(c_int) function ether_set_graphs(ctx, on) bind(C, name="ether_set_graphs")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: on
+      end function
+
+      integer(c_int) function ether_set_launch(ctx, sites_per_wave, xcd_map) bind(C, name="ether_set_launch")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: sites_per_wave, xcd_map
+      end function
+
+      integer(c_int) function ether_set_lanes(ctx, lanes_per_site) bind(C, name="ether_set_lanes")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: lanes_per_site
+      end function
+
       integer(c_int) function ether_build_spiral_partners(n_sites, pos, dir, dis, latency_deg, &
             capacity, partners, count) bind(C, name="ether_build_spiral_partners")
          import :: c_int, c_int64_t, c_double

@@ -321,6 +321,7 @@ class Model:
     nn_idx: Optional[np.ndarray] = None
     initial_spins: Optional[np.ndarray] = None  # (N, 3) generate_supercell start state
     initial_phi: Optional[np.ndarray] = None
+    gpu: "GpuTags" = field(default_factory=lambda: GpuTags())
 
     @property
     def model_id(self) -> int:
@@ -434,6 +435,93 @@ def build_model(tags: InEther, st: Structure, ex: Exchange, sia_lines=None, stg_
                  initial_spins=init[ll], initial_phi=phi0[ll])
 
 
+GPU_FILE = "gpu.ether"
+BATCH_FILE = "batch.ether"
+
+
+@dataclass
+class GpuTags:
+    """Engine settings of a run directory's gpu.ether: the tags the reference
+    has no counterpart for.  They live in their own file because the
+    reference's parser stops on any unknown key (src/read_input.f90:144-146),
+    so in.ether stays a valid reference input.  Same syntax as in.ether
+    (key = values; '!' or '#' comments; Fortran logicals)."""
+    ladder: str = "linear"      # linear | learned (tempering runs)
+    train_mcs: int = 0          # learned: MCS of the training windows (0: tmcs_eq // 2)
+    windows: int = 4            # learned: training windows
+    autotune: int = 0           # sweeps timed per launch shape (0: off)
+    spw: int = 0                # launch: sites per wavefront (0: engine default)
+    xcd: int = 1                # launch: XCD-aware block order
+    lanes: int = 0              # lanes per site of the sweep (0: chosen from the replica count)
+    graphs: bool = True         # replay captured HIP graphs
+    compat: int = 0             # compat flags (1: PT on E_tot, 2: OVRR as a separate pass)
+    chunk: int = 0              # MCS per ether_run call (0: the whole run)
+
+
+def parse_gpu_ether(text: str) -> GpuTags:
+    g = GpuTags()
+    for raw in process_lines(text):
+        line = raw.lower()
+        j = line.find("=")
+        if j < 0:
+            raise EtherInputError(f"Missing '=' in the given line: {raw.strip()}")
+        key = line[:j].strip()
+        v = _toks(line[j + 1:])
+        try:
+            if key == "ladder":
+                g.ladder = v[0]
+                if g.ladder not in ("linear", "learned"):
+                    raise EtherInputError(f"ladder must be 'linear' or 'learned', not '{v[0]}'")
+                if len(v) > 1:
+                    g.train_mcs = int(v[1])
+                if len(v) > 2:
+                    g.windows = int(v[2])
+                if g.train_mcs < 0 or g.windows < 1:
+                    raise EtherInputError("ladder: training MCS must be >= 0 and windows >= 1")
+            elif key == "autotune":
+                g.autotune = int(v[0])
+            elif key == "launch":
+                g.spw, g.xcd = int(v[0]), int(v[1])
+                if g.spw not in (1, 2, 4, 8, 16) or g.xcd not in (0, 1):
+                    raise EtherInputError("launch: sites per wave in {1,2,4,8,16}, XCD map 0 or 1")
+            elif key == "lanes":
+                g.lanes = int(v[0])
+                if g.lanes not in (0, 8, 16, 32, 64):
+                    raise EtherInputError("lanes: 0 (automatic), 8 (Ising only), 16, 32 or 64")
+            elif key == "graphs":
+                g.graphs = _logical(v[0])
+            elif key == "compat":
+                g.compat = int(v[0])
+                if g.compat < 0 or g.compat > 3:
+                    raise EtherInputError("compat: a sum of the flags 1 and 2")
+            elif key == "chunk":
+                g.chunk = int(v[0])
+                if g.chunk < 0:
+                    raise EtherInputError("chunk must be >= 0")
+            else:
+                raise EtherInputError(f"Found unknown/missing information in {GPU_FILE}: {raw.strip()}")
+        except (IndexError, ValueError) as e:
+            if isinstance(e, EtherInputError):
+                raise
+            raise EtherInputError(f"cannot read tag '{key}' from: {raw.strip()}") from e
+    return g
+
+
+def parse_batch_manifest(text: str, base: str = ".") -> List[str]:
+    """batch.ether: one run directory per line (relative to the manifest's
+    directory), comments as in in.ether.  The runs advance together through
+    one schedule per group of equal (mcs, coa, ovrr, pt) tags."""
+    out = []
+    for raw in process_lines(text):
+        d = raw.strip()
+        out.append(d if os.path.isabs(d) else os.path.normpath(os.path.join(base, d)))
+    if not out:
+        raise EtherInputError(f"{BATCH_FILE} lists no run directory")
+    if len(set(out)) != len(out):
+        raise EtherInputError(f"{BATCH_FILE} lists a run directory twice")
+    return out
+
+
 def read_directory(path: str) -> Model:
     """Read in.ether and the input files of a run directory, as the reference does
     from its current directory."""
@@ -454,4 +542,7 @@ def read_directory(path: str) -> Model:
         tags.sia and os.path.exists(os.path.join(path, "single_ion_anisotropy"))) else None
     stg = parse_staggered(rd("staggered")) if (
         tags.stg and os.path.exists(os.path.join(path, "staggered"))) else None
-    return build_model(tags, st, ex, sia, stg)
+    m = build_model(tags, st, ex, sia, stg)
+    if os.path.exists(os.path.join(path, GPU_FILE)):
+        m.gpu = parse_gpu_ether(rd(GPU_FILE))
+    return m

@@ -146,14 +146,14 @@ def learn_ladder(eng: E.Engine, temps, e, c, n_out: Optional[int] = None, iters:
 
 
 def _window(eng: E.Engine, model: Model, temps, repeat: int, first: int, n_mcs: int, coa: int,
-            pt_interval: int, sc: int):
+            pt_interval: int, sc: int, ovrr_iv: int = 0, ovrr_p: float = 0.0):
     """Run n_mcs MCS from step `first` with fresh swap counters and energy
     moments; returns (pair acceptance, attempts, <e> per slot, swap counter)."""
     n = len(temps)
     eng.set_ladders([k * n for k in range(repeat + 1)])
     eng.reset_moments()
-    # every step of the window samples (eq = first - 1) so that <e> is measured
-    sc = eng.run(first, n_mcs, first - 1, coa, 0, 0.0, pt_interval, sc)
+    # every sampling step of the window counts (eq = first - 1) so that <e> is measured
+    sc = eng.run(first, n_mcs, first - 1, coa, ovrr_iv, ovrr_p, pt_interval, sc)
     mom = eng.moments()
     a, att, _ = _pair_acceptance(eng, n, repeat)
     e = np.zeros(n)
@@ -176,37 +176,51 @@ def round_trip_cost(acc, attempts) -> float:
     return float(np.sum((1.0 - a) / a))
 
 
-def adaptive_run(eng: E.Engine, model: Model, temps, repeat: int, mcs: int, coa: int,
-                 pt_interval: int, *, learn: bool, train_mcs: int = 2000,
-                 windows: int = 4) -> LadderRun:
-    """Tempering run of mcs MCS.  The first train_mcs MCS are `windows` equal
-    windows; each one measures the pair acceptance and <e> of the ladder it ran
-    on.  The first window runs on `temps` (the plain ladder).  learn=True:
-    after every window but the last the length map takes the window's
-    measured intervals, the MLP is retrained on it (log C = 2 log(lambda T) on
-    the map's grid, <e> of the window interpolated) and the slots are retuned
-    to the ladder it places, so each window measures the intervals that the
-    previous ladder was too coarse to measure.  The production window then
-    runs on the measured ladder with the smallest round-trip cost Lambda; the
-    plain ladder is the fallback when no learned ladder beats it (far from
-    equilibrium the acceptance is not the Gaussian one the surrogate
-    assumes).  learn=False: every window runs on `temps`.  The remaining
-    mcs - train_mcs MCS are the timed production window."""
+@dataclass
+class Training:
+    ladder: np.ndarray         # the ladder chosen for production (hottest first)
+    next_step: int             # first MCS after the training windows
+    swap_count: int
+    source: str                # "plain" or "learned (window k)"
+    cost: float                # its measured round-trip cost Lambda
+    e: Optional[np.ndarray] = None
+    c: Optional[np.ndarray] = None
+    acc: Optional[np.ndarray] = None
+    loss: Optional[float] = None
+
+
+def train_ladder(eng: E.Engine, model: Model, temps, repeat: int, coa: int, pt_interval: int,
+                 train_mcs: int, *, windows: int = 4, learn: bool = True, first: int = 1,
+                 swap_count: int = 0, ovrr_iv: int = 0, ovrr_p: float = 0.0) -> Training:
+    """The training windows of a tempering run: train_mcs MCS from step
+    `first`, split into `windows` equal windows; each one measures the pair
+    acceptance and <e> of the ladder it ran on.  The first window runs on
+    `temps` (the plain ladder).  learn=True: after every window but the last
+    the length map takes the window's measured intervals, the MLP is retrained
+    on it (log C = 2 log(lambda T) on the map's grid, <e> of the window
+    interpolated) and the slots are retuned to the ladder it places, so each
+    window measures the intervals that the previous ladder was too coarse to
+    measure.  The ladder with the smallest measured round-trip cost Lambda is
+    then set for production; the plain ladder is the fallback when no learned
+    ladder beats it (far from equilibrium the acceptance is not the Gaussian
+    one the surrogate assumes).  learn=False: every window runs on `temps`.
+    Every replica keeps its state and slot throughout; the windows follow the
+    run's schedule (OVRR every ovrr_iv MCS with ovrr_p passes)."""
     temps = np.asarray(temps, np.float64)
     n = len(temps)
     t_range = (float(temps.min()), float(temps.max()))
-    eq = model.tags.tmcs_eq
-    eng.set_betas(np.tile(model.betas(temps), repeat))
-    eng.init_spins(E.INIT_RANDOM)
     win = max(train_mcs // windows, 1)
-    cur, sc, first = temps, 0, 1
+    cur, sc = temps, swap_count
     lmap = LengthMap(*t_range)
     best, best_cost, source = temps, np.inf, "plain"
     a = e = c = None
     loss = None
     for k in range(windows):
         length = win if k < windows - 1 else train_mcs - win * (windows - 1)
-        a, att, e, sc = _window(eng, model, cur, repeat, first, length, coa, pt_interval, sc)
+        if length <= 0:
+            break
+        a, att, e, sc = _window(eng, model, cur, repeat, first, length, coa, pt_interval, sc,
+                                ovrr_iv, ovrr_p)
         first += length
         cost = round_trip_cost(a, att)
         if cost < best_cost:
@@ -222,6 +236,24 @@ def adaptive_run(eng: E.Engine, model: Model, temps, repeat: int, mcs: int, coa:
     if learn and best is not cur:
         cur = best
         eng.retune_betas(np.tile(model.betas(cur), repeat))
+    return Training(np.asarray(cur), first, sc, source if learn else "plain", best_cost,
+                    e=e, c=c, acc=a, loss=loss)
+
+
+def adaptive_run(eng: E.Engine, model: Model, temps, repeat: int, mcs: int, coa: int,
+                 pt_interval: int, *, learn: bool, train_mcs: int = 2000,
+                 windows: int = 4) -> LadderRun:
+    """Tempering run of mcs MCS from random states: train_ladder over the
+    first train_mcs MCS, then the remaining mcs - train_mcs MCS are the timed
+    production window on the chosen ladder."""
+    temps = np.asarray(temps, np.float64)
+    n = len(temps)
+    eq = model.tags.tmcs_eq
+    eng.set_betas(np.tile(model.betas(temps), repeat))
+    eng.init_spins(E.INIT_RANDOM)
+    tr = train_ladder(eng, model, temps, repeat, coa, pt_interval, train_mcs, windows=windows,
+                      learn=learn)
+    cur, first, sc = tr.ladder, tr.next_step, tr.swap_count
     # production window
     eng.set_ladders([k * n for k in range(repeat + 1)])
     eng.reset_moments()
@@ -236,8 +268,8 @@ def adaptive_run(eng: E.Engine, model: Model, temps, repeat: int, mcs: int, coa:
     metro = np.array([np.mean([mom[g * n + k, E.MOM_ACC] for g in range(repeat)]) / rest
                       for k in range(n)])
     return LadderRun(cur, wall, rest, trips, acc, float(eng.R) * model.n_sites * rest,
-                     train_e=e, train_c=c, train_acc=a, mlp_loss=loss, metro_acc=metro,
-                     source=source if learn else "plain", train_cost=best_cost)
+                     train_e=tr.e, train_c=tr.c, train_acc=tr.acc, mlp_loss=tr.loss, metro_acc=metro,
+                     source=tr.source, train_cost=tr.cost)
 
 
 def compare_ladders(model: Model, *, train_mcs: int = 2000, mcs: Optional[int] = None,

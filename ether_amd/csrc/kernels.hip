@@ -26,6 +26,12 @@
 #include "engine_common.h"
 #include "philox.h"
 
+// Every fused multiply-add in this file is written as fmaf and the compiler
+// contracts nothing else: a kernel's rounding then does not depend on how the
+// code around an expression is arranged, so the variants of one update (launch
+// shapes, lanes per site, prefetching, batches) give bit-identical spins.
+#pragma clang fp contract(off)
+
 namespace ether {
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
@@ -43,13 +49,17 @@ __device__ __forceinline__ float eta24f(uint32_t w) {
 }
 __device__ __forceinline__ float wrap_pif(float p) {
     const float two_pi = 6.28318530717958647692f;
-    return p - two_pi * floorf((p + 3.14159265358979323846f) / two_pi);
+    return fmaf(-two_pi, floorf((p + 3.14159265358979323846f) / two_pi), p);
+}
+// a.b, accumulated x, y, z in order
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+    return fmaf(az, bz, fmaf(ay, by, ax * bx));
 }
 // Unit-length renormalisation with the hardware reciprocal square root
 // (1 ulp): every vector it is applied to has length 1 to within a few ulp
 // already, and the spins are FP32 to begin with.
 __device__ __forceinline__ void normalise3(float& x, float& y, float& z) {
-    const float inv = __builtin_amdgcn_rsqf(x * x + y * y + z * z);
+    const float inv = __builtin_amdgcn_rsqf(dot3(x, y, z, x, y, z));
     x *= inv;
     y *= inv;
     z *= inv;
@@ -78,7 +88,7 @@ __device__ __forceinline__ void marsaglia_point(uint32_t cx, uint32_t cy, float&
     const float rt = __builtin_amdgcn_sqrtf(1.0f - s);
     x = 2.0f * v1 * rt;
     y = 2.0f * v2 * rt;
-    z = 1.0f - 2.0f * s;
+    z = fmaf(-2.0f, s, 1.0f);
     normalise3(x, y, z);
 }
 // two candidate pairs per Philox block
@@ -147,7 +157,7 @@ __device__ __forceinline__ void angle_move(u32x4 w, float dphi, float phi_old, f
                                            float& z, float& phi) {
     float sz = sym24f(w.x >> 8);
     phi = wrap_pif(phi_old + dphi * sym24f(w.y >> 8));
-    float rt = sqrtf(fmaxf(0.0f, 1.0f - sz * sz));
+    float rt = sqrtf(fmaxf(0.0f, fmaf(-sz, sz, 1.0f)));
     float sn, cs;
     sincosf(phi, &sn, &cs);
     x = rt * cs;
@@ -252,14 +262,26 @@ __device__ __forceinline__ void local_field(Plane px, Plane py, Plane pz, const 
 // where A vanishes.
 __device__ __forceinline__ bool reflect_about(float hx, float hy, float hz, float& x, float& y,
                                               float& z) {
-    const float aa = hx * hx + hy * hy + hz * hz;
+    const float aa = dot3(hx, hy, hz, hx, hy, hz);
     if (!(aa > 0.f)) return false;
-    const float f = 2.0f * (hx * x + hy * y + hz * z) * __builtin_amdgcn_rcpf(aa);
-    x = f * hx - x;
-    y = f * hy - y;
-    z = f * hz - z;
+    const float f = 2.0f * dot3(hx, hy, hz, x, y, z) * __builtin_amdgcn_rcpf(aa);
+    x = fmaf(f, hx, -x);
+    y = fmaf(f, hy, -y);
+    z = fmaf(f, hz, -z);
     normalise3(x, y, z);
     return true;
+}
+
+// Energy change of S -> S' in the local field h (src/Hamiltonian.f90:72-124,
+// field already holding the Zeeman term) and the single-ion term
+// D.(S'^2 - S^2) (src/Hamiltonian.f90:127-143).
+__device__ __forceinline__ float delta_field(float nx, float ny, float nz, float x, float y, float z,
+                                             float hx, float hy, float hz) {
+    return dot3(nx - x, ny - y, nz - z, hx, hy, hz);
+}
+__device__ __forceinline__ float delta_sia(const float* D, float nx, float ny, float nz, float x,
+                                           float y, float z) {
+    return dot3(D[0], D[1], D[2], fmaf(nx, nx, -(x * x)), fmaf(ny, ny, -(y * y)), fmaf(nz, nz, -(z * z)));
 }
 
 // ---------------------------------------------------------------------------
@@ -296,15 +318,10 @@ __device__ __forceinline__ void xyz_sweep_block(const XyzArgs& a, int bb) {
     const uint32_t step = a.ctr[0] + a.step_off;
     uint64_t* pool = sPool[w];
     unsigned accepted = 0;
-    const int s0 = a.c_begin + (chunk * kWavesPerBlock + w) * a.spw;
-    for (int q = 0; q < a.spw; ++q) {
-        const int slot = s0 + q;                       // wave-uniform
-        if (slot >= a.c_end) break;
+
+    // the update of one site from its spin (x, y, z, phi) and local exchange field
+    auto update = [&](int slot, float x, float y, float z, float cphi, float hx, float hy, float hz) {
         const uint32_t so = (uint32_t)slot * rowb;
-        float x = plane_ld(px, roff, so), y = plane_ld(py, roff, so), z = plane_ld(pz, roff, so);
-        float cphi = ANGLE ? plane_ld(pp, roff, so) : 0.f;
-        float hx, hy, hz;
-        local_field<DM>(px, py, pz, a.ell + (size_t)slot * a.D, a.D, rowb, roff, sJ4, hx, hy, hz);
         const int sp = (ZEE || SIA) ? a.sp[slot] : 0;
         if (ZEE) {
             hx += sZ[3 * sp + 0];
@@ -331,11 +348,8 @@ __device__ __forceinline__ void xyz_sweep_block(const XyzArgs& a, int bb) {
         } else {
             marsaglia_pooled(wr, rv, a.seed, (uint32_t)slot, (uint32_t)r, step, pool, nx, ny, nz);
         }
-        float dE = (nx - x) * hx + (ny - y) * hy + (nz - z) * hz;
-        if (SIA) {
-            dE += sD[3 * sp + 0] * (nx * nx - x * x) + sD[3 * sp + 1] * (ny * ny - y * y) +
-                  sD[3 * sp + 2] * (nz * nz - z * z);
-        }
+        float dE = delta_field(nx, ny, nz, x, y, z, hx, hy, hz);
+        if (SIA) dE += delta_sia(sD + 3 * sp, nx, ny, nz, x, y, z);
         // accept if -beta*dE > log(eta)   (src/Metropolis.f90:52)
         if (rv && -beta * dE > __logf(eta)) {
             plane_st(px, roff, so, nx);
@@ -349,6 +363,18 @@ __device__ __forceinline__ void xyz_sweep_block(const XyzArgs& a, int bb) {
             plane_st(pz, roff, so, z);
             if (ANGLE) plane_st(pp, roff, so, cphi);
         }
+    };
+
+    const int s0 = a.c_begin + (chunk * kWavesPerBlock + w) * a.spw;
+    for (int q = 0; q < a.spw; ++q) {
+        const int slot = s0 + q;                       // wave-uniform
+        if (slot >= a.c_end) break;
+        const uint32_t so = (uint32_t)slot * rowb;
+        const float x = plane_ld(px, roff, so), y = plane_ld(py, roff, so), z = plane_ld(pz, roff, so);
+        const float cphi = ANGLE ? plane_ld(pp, roff, so) : 0.f;
+        float hx, hy, hz;
+        local_field<DM>(px, py, pz, a.ell + (size_t)slot * a.D, a.D, rowb, roff, sJ4, hx, hy, hz);
+        update(slot, x, y, z, cphi, hx, hy, hz);
     }
     // acceptance is booked to the temperature slot, as the reference books it per rank
     book_acceptance(sAcc, 1, &accepted, r, a.R, a.slot_of, a.acc);
@@ -706,11 +732,8 @@ __device__ __forceinline__ void xyz_sweep_lanes_block(const XyzArgs& a, int bb) 
             marsaglia_pooled_group(wr, rv, gmask, gbase, a.seed, (uint32_t)slot, (uint32_t)r, step, pool,
                                    nx, ny, nz);
         }
-        float dE = (nx - x) * hx + (ny - y) * hy + (nz - z) * hz;
-        if (SIA) {
-            dE += sD[3 * sp + 0] * (nx * nx - x * x) + sD[3 * sp + 1] * (ny * ny - y * y) +
-                  sD[3 * sp + 2] * (nz * nz - z * z);
-        }
+        float dE = delta_field(nx, ny, nz, x, y, z, hx, hy, hz);
+        if (SIA) dE += delta_sia(sD + 3 * sp, nx, ny, nz, x, y, z);
         if (rv && -beta * dE > __logf(eta)) {
             plane_stv(px, vo, nx);
             plane_stv(py, vo, ny);
@@ -883,10 +906,10 @@ __device__ __forceinline__ void xyz_measure_block(const MeasArgs& a, int blk) {
             hz = fmaf(sJ[3 * c + 2], a.sz[jr], hz);
         }
         const float x = a.sx[row], y = a.sy[row], z = a.sz[row];
-        v[0] += (double)(x * hx + y * hy + z * hz);
+        v[0] += (double)dot3(x, y, z, hx, hy, hz);
         const int sp = a.sp[slot];
-        if (ZEE) v[1] += (double)(x * sZ[3 * sp] + y * sZ[3 * sp + 1] + z * sZ[3 * sp + 2]);
-        if (SIA) v[2] += (double)(sD[3 * sp] * x * x + sD[3 * sp + 1] * y * y + sD[3 * sp + 2] * z * z);
+        if (ZEE) v[1] += (double)dot3(x, y, z, sZ[3 * sp], sZ[3 * sp + 1], sZ[3 * sp + 2]);
+        if (SIA) v[2] += (double)dot3(sD[3 * sp], sD[3 * sp + 1], sD[3 * sp + 2], x * x, y * y, z * z);
         const float wgt = a.w[slot];
         v[3] += (double)(wgt * x);
         v[4] += (double)(wgt * y);
@@ -1192,10 +1215,10 @@ __global__ __launch_bounds__(kBlock) void k_local_fields(MeasArgs a, int r, floa
     h[3 * slot + 1] = hy;
     h[3 * slot + 2] = hz;
     const int sp = a.sp[slot];
-    float en = x * hx + y * hy + z * hz;
-    if (zee) en += x * a.zvec[3 * sp] + y * a.zvec[3 * sp + 1] + z * a.zvec[3 * sp + 2];
+    float en = dot3(x, y, z, hx, hy, hz);
+    if (zee) en += dot3(x, y, z, a.zvec[3 * sp], a.zvec[3 * sp + 1], a.zvec[3 * sp + 2]);
     if (sia && model == 1)
-        en += a.dsia[3 * sp] * x * x + a.dsia[3 * sp + 1] * y * y + a.dsia[3 * sp + 2] * z * z;
+        en += dot3(a.dsia[3 * sp], a.dsia[3 * sp + 1], a.dsia[3 * sp + 2], x * x, y * y, z * z);
     e[slot] = en;
 }
 

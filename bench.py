@@ -262,8 +262,10 @@ def main():
 def sq_bench(calls):
     """S(q) of every replica of the triangular workload: 96 x 96 cells (9216
     spins, 9216 commensurate q points), 32 T x 10 repeats = 320 replicas,
-    after 200 MCS from a random start.  Per-axis DFT contractions on the
-    matrix cores (split-BF16, FP32 accumulation).  The device time comes from
+    after 200 MCS from a random start.  One fused complex DFT GEMM per
+    lattice axis on the matrix cores (split-BF16, FP32 accumulation), the
+    first reading the spin planes, the last forming |S(q)|^2; then a
+    transpose to replica-major output.  The device time comes from
     HIP events around the device part of each call; the error is measured
     against numpy's FP64 FFT for 4 replicas."""
     import numpy as np

@@ -131,25 +131,29 @@ __device__ __forceinline__ void marsaglia_pooled(u32x4 w, bool active, uint64_t 
     const uint64_t donors = __ballot(active && ok1);
     const int lane = lane_id();
     const uint64_t below = lanes_below(lane);
-    if (active && ok1) pool[__popcll(donors & below)] = ((uint64_t)x1 << 32) | y1;
+    // branch-free exchange: non-donors write to their own spare entry, every
+    // lane reads, and a select keeps what a needy lane is owed
+    const bool give = active && ok1;
+    pool[give ? __popcll(donors & below) : kWave + lane] = ((uint64_t)x1 << 32) | y1;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
-    uint32_t ux = x0, uy = y0;
-    bool have = ok0;
-    if (active && !ok0) {
-        const int j = __popcll(needy & below);
-        if (j < __popcll(donors)) {
-            const uint64_t v = pool[j];
-            ux = (uint32_t)(v >> 32);
-            uy = (uint32_t)v;
-            have = true;
-        }
+    const int j = __popcll(needy & below);
+    const bool take = active && !ok0 && j < __popcll(donors);
+    const uint64_t v = pool[j];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t ux = take ? (uint32_t)(v >> 32) : x0, uy = take ? (uint32_t)v : y0;
+    const bool have = ok0 || take;
+    float px, py, pz;
+    marsaglia_point(ux, uy, px, py, pz);
+    if (__any(active && !have)) {
+        if (active && !have) marsaglia_retry(seed, id, r, step, RNG_METROPOLIS, 1, px, py, pz);
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    if (!active) return;
-    if (have) marsaglia_point(ux, uy, x, y, z);
-    else marsaglia_retry(seed, id, r, step, RNG_METROPOLIS, 1, x, y, z);
+    if (active) {
+        x = px;
+        y = py;
+        z = pz;
+    }
 }
 
 // ANGLE move (src/update_spin_details.f90:39-46)
@@ -296,7 +300,7 @@ __device__ __forceinline__ void xyz_sweep_block(const XyzArgs& a, int bb) {
     __shared__ float4 sJ4[kMaxClasses + 1];
     __shared__ float sZ[3 * kMaxSpecies];
     __shared__ float sD[3 * kMaxSpecies];
-    __shared__ uint64_t sPool[kWavesPerBlock][kWave];
+    __shared__ uint64_t sPool[kWavesPerBlock][2 * kWave];
     __shared__ unsigned sAcc[kBlock];
     const bool ZEE = a.zee != 0, SIA = a.sia != 0;   // wave-uniform
     stage_couplings(sJ4, a.jclass, a.n_classes);
@@ -350,18 +354,15 @@ __device__ __forceinline__ void xyz_sweep_block(const XyzArgs& a, int bb) {
         }
         float dE = delta_field(nx, ny, nz, x, y, z, hx, hy, hz);
         if (SIA) dE += delta_sia(sD + 3 * sp, nx, ny, nz, x, y, z);
-        // accept if -beta*dE > log(eta)   (src/Metropolis.f90:52)
-        if (rv && -beta * dE > __logf(eta)) {
-            plane_st(px, roff, so, nx);
-            plane_st(py, roff, so, ny);
-            plane_st(pz, roff, so, nz);
-            if (ANGLE) plane_st(pp, roff, so, nphi);
-            ++accepted;
-        } else if (rv && reflected) {
-            plane_st(px, roff, so, x);
-            plane_st(py, roff, so, y);
-            plane_st(pz, roff, so, z);
-            if (ANGLE) plane_st(pp, roff, so, cphi);
+        // accept if -beta*dE > log(eta)   (src/Metropolis.f90:52); one store
+        // path for an accepted move and for a rejected move after a reflection
+        const bool acc = rv && -beta * dE > __logf(eta);
+        accepted += acc ? 1u : 0u;
+        if (acc || (rv && reflected)) {
+            plane_st(px, roff, so, acc ? nx : x);
+            plane_st(py, roff, so, acc ? ny : y);
+            plane_st(pz, roff, so, acc ? nz : z);
+            if (ANGLE) plane_st(pp, roff, so, acc ? nphi : cphi);
         }
     };
 

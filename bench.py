@@ -70,7 +70,7 @@ def cpu_baseline(model, budget_s=10.0):
                       f"Heisenberg chain at T=20 K, FP64 C oracle, single thread, {dt:.1f} s"}
 
 
-def describe(name, m, n_temps, R, args, coa, ovrr_iv):
+def describe(name, m, n_temps, R, steps, split_ovrr, coa, ovrr_iv):
     t = m.tags
     kind = "Ising" if t.ising else "Heisenberg"
     moves = "" if t.ising else (f", ANGLE moves delta {math.degrees(t.dphi):g} deg" if t.angle else ", Marsaglia moves")
@@ -78,16 +78,16 @@ def describe(name, m, n_temps, R, args, coa, ovrr_iv):
     ovrr = ""
     if ovrr_iv:
         ovrr = (f", OVRR every {ovrr_iv} MCS "
-                f"({'split pass' if args.split_ovrr else 'fused into the sweep phases'})")
+                f"({'split pass' if split_ovrr else 'fused into the sweep phases'})")
     return (f"{name}: {kind} {m.sc[0]}x{m.sc[1]}x{m.sc[2]} cells ({m.n_sites} spins){extra}, "
             f"{n_temps} T x {t.repeat} repeats = {R} chains{moves}{ovrr}, "
-            f"MCS {args.steps}/{t.tmcs_eq}, coa {coa}")
+            f"MCS {steps}/{t.tmcs_eq}, coa {coa}")
 
 
-def profiled_traffic(workload):
+def profiled_traffic(kernel, workload):
     """HBM bytes per launch of the sweep kernel from the committed PMC summary
     (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, factors from a plain copy), or None."""
-    path = os.path.join(ROOT, "profiles", f"k_xyz_sweep_{workload}_traffic.md")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.md")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -134,6 +134,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven workload")
     ap.add_argument("--batch-mcs", type=int, default=100)
     ap.add_argument("--sq-calls", type=int, default=5)
+    ap.add_argument("--no-ising", action="store_true", help="skip the Ising simple-cubic section")
+    ap.add_argument("--ising-steps", type=int, default=2000)
     ap.add_argument("--split-ovrr", action="store_true",
                     help="the reference's order: a whole OVRR pass, then the sweep")
     args = ap.parse_args()
@@ -145,100 +147,8 @@ def main():
               "of a run are independent and would shard across GPUs by whole ladders", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-    from ether_amd import engine as E
-    from ether_amd import workloads
-    from ether_amd.driver import attach_neighbours, make_engine
-    from ether_amd.inputs import fortran_nint
-
-    m = workloads.model(args.workload)
-    attach_neighbours(m)
-    t = m.tags
-    temps = t.temperatures()
-    R = len(temps) * t.repeat
-    eng = make_engine(m, R, compat=E.COMPAT_SPLIT_OVRR if args.split_ovrr else 0)
-    eng.set_betas(np.tile(m.betas(temps), t.repeat))
-    spw, xcd, tune_ms = eng.autotune(reps=10)
-    coa = fortran_nint(t.coa)
-    ovrr_on = t.ovrr and t.xyz and not t.zeeman and not t.sia
-    ovrr_iv = t.ovrr_mcs if ovrr_on else 0
-    ovrr_p = t.ovrr_para if ovrr_on else 0.0
-
-    # warmup on its own start state
-    eng.init_spins(E.INIT_RANDOM)
-    if args.warmup > 0:
-        eng.run(1, args.warmup, t.tmcs_eq, coa, ovrr_iv, ovrr_p)
-    eng.synchronize()
-
-    # timed production run: steps 1..K of the schedule, whole periods replayed
-    # as captured HIP graphs
-    eng.set_spins(-1, m.initial_spins, m.initial_phi)
-    eng.init_spins(E.INIT_FRESH)
-    eng.reset_moments()
-    eng.synchronize()
-    t0 = time.perf_counter()
-    eng.run(1, args.steps, t.tmcs_eq, coa, ovrr_iv, ovrr_p)
-    eng.synchronize()
-    wall = time.perf_counter() - t0
-    mom = eng.moments()
-    updates = float(R) * m.n_sites * args.steps
-    value = updates / wall
-
-    # event-timed continuation of the same run (steps K+1..K+E): every kernel
-    # is launched on its own between HIP events on the engine stream, which a
-    # graph replay does not allow; gives the per-launch kernel durations
-    eng.set_timing(True)
-    t1 = time.perf_counter()
-    eng.run(args.steps + 1, args.event_steps, t.tmcs_eq, coa, ovrr_iv, ovrr_p)
-    eng.synchronize()
-    wall_ev = time.perf_counter() - t1
-    sweep_ms, sweep_n = eng.timing(E.TIME_SWEEP)
-    ovrr_ms, ovrr_n = eng.timing(E.TIME_OVRR)
-    meas_ms, meas_n = eng.timing(E.TIME_MEASURE)
-    eng.set_timing(False)
-
-    nc, col, _ = eng.layout()
-    per_launch_updates = R * m.n_sites / nc
-    avg_ms = sweep_ms / max(sweep_n, 1)
-    b_upd = B_UPD[m.model_id] + (8.0 if (m.model_id == 1 and t.angle) else 0.0)
-    achieved = b_upd * per_launch_updates / (avg_ms * 1e-3) / 1e9
-    out = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "spin updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (random-start spins, reference in.ether tags of the workload)",
-        "config": {
-            "workload": describe(args.workload, m, len(temps), R, args, coa, ovrr_iv),
-            "replicas": R, "sites": m.n_sites, "colours": nc, "mcs": args.steps,
-        },
-        "roofline": {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args.workload),
-            "kernel": ("k_ising_sweep" if m.model_id == 0 else "k_xyz_sweep") + " (one colour phase)",
-            "bytes_per_update": b_upd, "updates_per_launch": per_launch_updates,
-            "avg_launch_ms": avg_ms, "launches": sweep_n,
-            "timing": f"HIP events around each launch on the engine stream over MCS "
-                      f"{args.steps + 1}..{args.steps + args.event_steps} of the same run "
-                      f"(launched one by one; {wall_ev / args.event_steps * 1e3:.4f} ms per MCS there)",
-        },
-        "launch": {"sites_per_wave": spw, "xcd_map": xcd,
-                   "autotune_ms_per_sweep": {f"spw{k}_xcd{x}": round(float(tune_ms[x][i]), 5)
-                                             for x in (0, 1)
-                                             for i, k in enumerate(E.Engine.AUTOTUNE_SPW)}},
-        "kernel_ms": {"sweep": sweep_ms, "ovrr": ovrr_ms, "measure": meas_ms,
-                      "sweep_launches": sweep_n, "ovrr_launches": ovrr_n, "measure_launches": meas_n},
-        "acceptance_mean": float(mom[:, E.MOM_ACC].mean() / args.steps),
-        "graphs": "whole periods of P = lcm(coa, OVRR, PT intervals) MCS replayed as captured HIP graphs",
-    }
-    eng.close()
+    out = measure(args.workload, args.steps, args.warmup, args.event_steps, args.split_ovrr, args)
+    out = {"metric": METRIC, **out}
     if not args.no_pt:
         out["pt"] = pt_comparison(args.pt_mcs)
         # On 1 -> 0.05 K (T/J 0.086 -> 0.004) uniform Marsaglia proposals are
@@ -254,9 +164,119 @@ def main():
         out["sq"] = sq_bench(args.sq_calls)
     if not args.no_batch:
         out["batch"] = batch_bench(args.batch_mcs)
+    if not args.no_ising and args.workload != "sc_ising_16":
+        ising = measure("sc_ising_16", args.ising_steps, 20, 100, False, args)
+        out["ising"] = {k: ising[k] for k in ("value", "unit", "ms_per_step", "config", "roofline",
+                                              "launch", "acceptance_mean")}
     if not args.no_cpu:
+        from ether_amd import workloads
+        from ether_amd.driver import attach_neighbours
+        m = workloads.model(args.workload)
+        attach_neighbours(m)
         out["cpu_baseline"] = cpu_baseline(m, args.cpu_budget)
     print(json.dumps(out), flush=True)
+
+
+def measure(workload, steps, warmup, event_steps, split_ovrr, args):
+    """The headline measurement on one workload: autotuned launch shape, an
+    untimed warmup on its own start state, `steps` timed MCS of the schedule
+    (whole periods replayed as graphs), then `event_steps` more MCS launched
+    one by one between HIP events for the kernel durations."""
+    import numpy as np
+    from ether_amd import engine as E
+    from ether_amd import workloads
+    from ether_amd.driver import attach_neighbours, make_engine
+    from ether_amd.inputs import fortran_nint
+
+    m = workloads.model(workload)
+    attach_neighbours(m)
+    t = m.tags
+    temps = t.temperatures()
+    R = len(temps) * t.repeat
+    eng = make_engine(m, R, compat=E.COMPAT_SPLIT_OVRR if split_ovrr else 0)
+    eng.set_betas(np.tile(m.betas(temps), t.repeat))
+    spw, xcd, tune_ms = eng.autotune(reps=10)
+    coa = fortran_nint(t.coa)
+    ovrr_on = t.ovrr and t.xyz and not t.zeeman and not t.sia
+    ovrr_iv = t.ovrr_mcs if ovrr_on else 0
+    ovrr_p = t.ovrr_para if ovrr_on else 0.0
+
+    # warmup on its own start state
+    eng.init_spins(E.INIT_RANDOM)
+    if warmup > 0:
+        eng.run(1, warmup, t.tmcs_eq, coa, ovrr_iv, ovrr_p)
+    eng.synchronize()
+
+    # timed production run: steps 1..K of the schedule, whole periods replayed
+    # as captured HIP graphs
+    eng.set_spins(-1, m.initial_spins, m.initial_phi)
+    eng.init_spins(E.INIT_FRESH)
+    eng.reset_moments()
+    eng.synchronize()
+    t0 = time.perf_counter()
+    eng.run(1, steps, t.tmcs_eq, coa, ovrr_iv, ovrr_p)
+    eng.synchronize()
+    wall = time.perf_counter() - t0
+    mom = eng.moments()
+    updates = float(R) * m.n_sites * steps
+    value = updates / wall
+
+    # event-timed continuation of the same run (steps K+1..K+E): every kernel
+    # is launched on its own between HIP events on the engine stream, which a
+    # graph replay does not allow; gives the per-launch kernel durations
+    eng.set_timing(True)
+    t1 = time.perf_counter()
+    eng.run(steps + 1, event_steps, t.tmcs_eq, coa, ovrr_iv, ovrr_p)
+    eng.synchronize()
+    wall_ev = time.perf_counter() - t1
+    sweep_ms, sweep_n = eng.timing(E.TIME_SWEEP)
+    ovrr_ms, ovrr_n = eng.timing(E.TIME_OVRR)
+    meas_ms, meas_n = eng.timing(E.TIME_MEASURE)
+    eng.set_timing(False)
+
+    nc, col, _ = eng.layout()
+    kernel = "k_ising_sweep" if m.model_id == 0 else "k_xyz_sweep"
+    per_launch_updates = R * m.n_sites / nc
+    avg_ms = sweep_ms / max(sweep_n, 1)
+    b_upd = B_UPD[m.model_id] + (8.0 if (m.model_id == 1 and t.angle) else 0.0)
+    achieved = b_upd * per_launch_updates / (avg_ms * 1e-3) / 1e9
+    out = {
+        "value": value,
+        "unit": "spin updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": wall / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (random-start spins, reference in.ether tags of the workload)",
+        "config": {
+            "workload": describe(workload, m, len(temps), R, steps, split_ovrr, coa, ovrr_iv),
+            "replicas": R, "sites": m.n_sites, "colours": nc, "mcs": steps,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(kernel, workload),
+            "kernel": kernel + " (one colour phase)",
+            "bytes_per_update": b_upd, "updates_per_launch": per_launch_updates,
+            "avg_launch_ms": avg_ms, "launches": sweep_n,
+            "timing": f"HIP events around each launch on the engine stream over MCS "
+                      f"{steps + 1}..{steps + event_steps} of the same run "
+                      f"(launched one by one; {wall_ev / event_steps * 1e3:.4f} ms per MCS there)",
+        },
+        "launch": {"sites_per_wave": spw, "xcd_map": xcd,
+                   "autotune_ms_per_sweep": {f"spw{k}_xcd{x}": round(float(tune_ms[x][i]), 5)
+                                             for x in (0, 1)
+                                             for i, k in enumerate(E.Engine.AUTOTUNE_SPW)}},
+        "kernel_ms": {"sweep": sweep_ms, "ovrr": ovrr_ms, "measure": meas_ms,
+                      "sweep_launches": sweep_n, "ovrr_launches": ovrr_n, "measure_launches": meas_n},
+        "acceptance_mean": float(mom[:, E.MOM_ACC].mean() / steps),
+        "graphs": "whole periods of P = lcm(coa, OVRR, PT intervals) MCS replayed as captured HIP graphs",
+    }
+    eng.close()
+    return out
 
 
 def sq_bench(calls):

@@ -208,6 +208,11 @@ __device__ __forceinline__ void plane_st(Plane p, uint32_t roff, uint32_t soff, 
 __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
+// A wave-uniform row of a table that no kernel writes (the ELL neighbour
+// table), read through the constant address space: the compiler then fetches
+// it with scalar loads into SGPRs instead of a vector load and readfirstlane.
+typedef const __attribute__((address_space(4))) uint32_t* ConstRow;
+__device__ __forceinline__ ConstRow const_row(const uint32_t* p) { return (ConstRow)p; }
 
 // Coupling of class c as (Jxx, Jyy, Jzz, 0), staged in LDS (one 16-byte read)
 __device__ __forceinline__ void stage_couplings(float4* sJ4, const float* jclass, int n_classes) {
@@ -230,12 +235,13 @@ __device__ __forceinline__ void local_field(Plane px, Plane py, Plane pz, const 
     hx = 0.f;
     hy = 0.f;
     hz = 0.f;
+    const ConstRow row = const_row(nb);
     if constexpr (DM > 0) {
         float vx[DM], vy[DM], vz[DM];
         uint32_t cl[DM];
 #pragma unroll
         for (int k = 0; k < DM; ++k) {
-            const uint32_t e = uniform_u32(nb[k]);
+            const uint32_t e = row[k];
             const uint32_t so = (e & 0xffffffu) * row_bytes;
             cl[k] = e >> 24;
             vx[k] = plane_ld(px, roff, so);
@@ -251,7 +257,7 @@ __device__ __forceinline__ void local_field(Plane px, Plane py, Plane pz, const 
         }
     } else {
         for (int k = 0; k < D; ++k) {
-            const uint32_t e = uniform_u32(nb[k]);
+            const uint32_t e = row[k];
             const uint32_t so = (e & 0xffffffu) * row_bytes;
             const float4 j = sJ4[e >> 24];
             hx = fmaf(j.x, plane_ld(px, roff, so), hx);
@@ -506,7 +512,7 @@ __device__ __forceinline__ void ising_sweep_block(const IsingArgs& a, int bb) {
             d[k] = 0;
             cnt[k] = 0;
         }
-        const uint32_t* nb = a.ell + (size_t)slot * a.D;
+        const ConstRow nb = const_row(a.ell + (size_t)slot * a.D);   // wave-uniform row
         if constexpr (DM > 0) {
             // issue every neighbour load first (the row is padded to DM with class 0)
             uint32_t nv[DM];
@@ -897,7 +903,7 @@ __device__ __forceinline__ void xyz_measure_block(const MeasArgs& a, int blk) {
     for (int slot = s_begin; slot < s_end; ++slot) {
         const size_t row = (size_t)slot * a.Rp + r;
         float hx = 0.f, hy = 0.f, hz = 0.f;
-        const uint32_t* nb = a.ell + (size_t)slot * a.D;
+        const ConstRow nb = const_row(a.ell + (size_t)slot * a.D);   // wave-uniform row
         for (int k = 0; k < a.D; ++k) {
             const uint32_t e = nb[k];
             const size_t jr = (size_t)(e & 0xffffffu) * a.Rp + r;
@@ -969,7 +975,7 @@ __device__ __forceinline__ void ising_measure_block(const MeasArgs& a, int blk) 
     const int s_end = min(a.N, s_begin + a.tile_sites);
     for (int slot = s_begin; slot < s_end; ++slot) {
         const int si = a.s[(size_t)slot * a.Rp + r];
-        const uint32_t* nb = a.ell + (size_t)slot * a.D;
+        const ConstRow nb = const_row(a.ell + (size_t)slot * a.D);   // wave-uniform row
         for (int k = 0; k < a.D; ++k) {
             const uint32_t e = nb[k];
             const int c = (int)(e >> 24);

@@ -17,9 +17,10 @@ value = Metropolis spin updates / wall second of the timed MCS loop
         (OVRR and measurement time included, not counted as updates).
 roofline = the Metropolis colour-phase kernel: 24 B of own-spin traffic per
         update (FP32 SoA read + write) / its average launch time from HIP
-        events on the engine stream (over `--event-steps` more MCS of the same
-        run, launched one by one), against 8 TB/s; traffic = HBM bytes per
-        launch from the committed PMC summary under profiles/.
+        events on the engine stream (one event pair around the colour phases
+        of each sweep over `--event-steps` more MCS of the same run, launched
+        one by one), against 8 TB/s; traffic = HBM bytes per launch from the
+        committed PMC summary under profiles/.
 pt, pt_trips = the tempering ladder comparisons (linear vs learned).
 cpu_baseline = the CPU oracle's FP64 sweep (+OVRR) of one 32^3 chain on one
         host core, for a bounded sample of about 10 s.
@@ -262,9 +263,10 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
             "kernel": kernel + " (one colour phase)",
             "bytes_per_update": b_upd, "updates_per_launch": per_launch_updates,
             "avg_launch_ms": avg_ms, "launches": sweep_n,
-            "timing": f"HIP events around each launch on the engine stream over MCS "
-                      f"{steps + 1}..{steps + event_steps} of the same run "
-                      f"(launched one by one; {wall_ev / event_steps * 1e3:.4f} ms per MCS there)",
+            "timing": f"HIP events on the engine stream around the back-to-back colour phases of "
+                      f"every sweep over MCS {steps + 1}..{steps + event_steps} of the same run "
+                      f"(launched one by one; {wall_ev / event_steps * 1e3:.4f} ms per MCS there), "
+                      f"divided by the phases they cover",
         },
         "launch": {"sites_per_wave": spw, "xcd_map": xcd,
                    "autotune_ms_per_sweep": {f"spw{k}_xcd{x}": round(float(tune_ms[x][i]), 5)

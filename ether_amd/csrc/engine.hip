@@ -132,6 +132,7 @@ struct ether_ctx {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
     size_t ev_used = 0;
     std::vector<int> ev_kind;
+    std::vector<int> ev_count;
     std::vector<void*> allocs;
     // MLP / S(q) state lives in the extension files
     void* mlp = nullptr;
@@ -927,7 +928,10 @@ static MeasArgs meas_args(ether_ctx* c) {
 // --------------------------------------------------------------------------
 // kernel timing (HIP events on the context stream)
 // --------------------------------------------------------------------------
-static int ev_begin(ether_ctx* c, int kind) {
+// An event pair brackets `count` back-to-back launches of one kind (the
+// colour phases of a sweep share one pair, so the events' own cost is paid
+// once per sweep rather than once per phase).
+static int ev_begin(ether_ctx* c, int kind, int count = 1) {
     if (!c->timing) return -1;
     if (c->ev_used == c->ev_pool.size()) {
         hipEvent_t a, b;
@@ -938,9 +942,11 @@ static int ev_begin(ether_ctx* c, int kind) {
         }
         c->ev_pool.push_back({a, b});
         c->ev_kind.push_back(kind);
+        c->ev_count.push_back(count);
     }
     const int k = (int)c->ev_used++;
     c->ev_kind[k] = kind;
+    c->ev_count[k] = count;
     (void)hipEventRecord(c->ev_pool[k].first, c->stream);
     return k;
 }
@@ -967,7 +973,7 @@ extern "C" int ether_get_timing(ether_ctx* c, int32_t kind, double* total_ms, in
         float ms = 0.f;
         HIPCHK(c, hipEventElapsedTime(&ms, c->ev_pool[k].first, c->ev_pool[k].second));
         tot += ms;
-        ++n;
+        n += c->ev_count[k];
     }
     if (total_ms) *total_ms = tot;
     if (launches) *launches = n;
@@ -996,11 +1002,14 @@ static bool phase_empty(const ether_ctx* c, int q) {
 }
 
 static int enqueue_sweep(ether_ctx* c, double fuse_frac = 0.0) {
+    int launches = 0;
+    for (int q = 0; q < phase_count(c); ++q)
+        if (!phase_empty(c, q) || g_recorder) ++launches;
+    const int ev = ev_begin(c, ETHER_TIME_SWEEP, launches);
     for (int q = 0; q < phase_count(c); ++q) {
         const bool empty = phase_empty(c, q);
         if (empty && !g_recorder) continue;
         const int qq = std::min(q, c->ncol - 1);
-        const int ev = ev_begin(c, ETHER_TIME_SWEEP);
         if (c->model == ETHER_MODEL_ISING) {
             IsingArgs a = ising_args(c, qq);
             if (empty) a.c_begin = a.c_end = 0;
@@ -1014,8 +1023,8 @@ static int enqueue_sweep(ether_ctx* c, double fuse_frac = 0.0) {
             }
             launch_xyz_sweep(a, c->angle, c->zee, c->sia, c->stream);
         }
-        ev_end(c, ev);
     }
+    ev_end(c, ev);
     HIPCHK(c, hipGetLastError());
     c->step++;
     return ETHER_OK;

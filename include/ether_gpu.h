@@ -298,12 +298,14 @@ int ether_set_lanes(ether_ctx* ctx, int32_t lanes_per_site);
 int ether_autotune(ether_ctx* ctx, int32_t reps, int32_t* sites_per_wave, int32_t* xcd_map,
                    double* table_ms);
 
-/* Kernel timing with HIP events recorded on the context's stream around every
- * launch of a kind: 0 Metropolis colour phase, 1 overrelaxation colour phase,
+/* Kernel timing with HIP events recorded on the context's stream around the
+ * launches of a kind: 0 Metropolis colour phases (one event pair around the
+ * back-to-back phases of a sweep), 1 overrelaxation colour phase,
  * 2 measurement (partials + finalize), 3 tempering swap, 4 structure factor
  * (device part of one ether_structure_factor call).  Adds no
  * synchronisation; ether_get_timing synchronises and reports the summed
- * event time and the launch count since timing was switched on. */
+ * event time and the number of launches it covers since timing was switched
+ * on. */
 #define ETHER_TIME_SWEEP   0
 #define ETHER_TIME_OVRR    1
 #define ETHER_TIME_MEASURE 2

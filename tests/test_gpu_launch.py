@@ -11,6 +11,8 @@ import numpy as np
 import pytest
 
 from ether_amd import engine as E
+from ether_amd import workloads
+from ether_amd.driver import attach_neighbours, make_engine
 from tests.helpers import pair, random_xyz, say
 
 pytestmark = pytest.mark.gpu
@@ -89,3 +91,20 @@ def test_autotune_restores_state_and_changes_no_result(name, over, R):
         runs.append(_spins(eng, R))
         eng.close()
     assert np.array_equal(runs[0], runs[1])
+
+
+def test_event_timing_counts_every_colour_phase():
+    """One event pair brackets the colour phases of a sweep; the reported
+    launch count is still one per phase."""
+    m = workloads.model("sc_heis_afm_32", sc="6 6 6")
+    attach_neighbours(m)
+    eng = make_engine(m, 40, seed=3)
+    eng.set_betas(m.betas(np.linspace(30, 3, 40)))
+    eng.init_spins(E.INIT_RANDOM)
+    nc = eng.layout()[0]
+    eng.set_timing(True)
+    eng.run(1, 12, 1000, 10, 1, 1.0, 0, 0)
+    ms, n = eng.timing(E.TIME_SWEEP)
+    eng.set_timing(False)
+    assert n == 12 * nc and ms > 0.0
+    eng.close()

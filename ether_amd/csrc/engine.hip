@@ -73,6 +73,8 @@ struct ether_ctx {
     std::vector<int> spiral_rows;
     // device
     uint32_t* d_ell = nullptr;
+    uint32_t* d_ell_m = nullptr;    // measurement rows (d_ell itself where a site bonds to itself)
+    int D_m = 0, bond_factor = 1;
     uint8_t* d_sp = nullptr;
     float* d_w = nullptr;
     float* d_jclass = nullptr;
@@ -557,6 +559,35 @@ extern "C" int ether_commit(ether_ctx* c) {
             ell[(size_t)s * D + k] =
                 (uint32_t)c->slot_of_site[edges[i][k].j] | ((uint32_t)edges[i][k].cls << 24);
     }
+    // ---- measurement rows: each bond from its endpoint with the smaller slot,
+    // counted twice; a self bond (a lattice one cell thick along a periodic
+    // axis) keeps the full rows, counted once ----
+    bool self_bond = false;
+    std::vector<std::vector<uint32_t>> upper(N);
+    for (int i = 0; i < N && !self_bond; ++i) {
+        const int s = c->slot_of_site[i];
+        for (const Edge& e : edges[i]) {
+            const int sj = c->slot_of_site[e.j];
+            if (sj == s) {
+                self_bond = true;
+                break;
+            }
+            if (sj > s) upper[s].push_back((uint32_t)sj | ((uint32_t)e.cls << 24));
+        }
+    }
+    std::vector<uint32_t> ell_m;
+    if (self_bond) {
+        c->D_m = D;
+        c->bond_factor = 1;
+    } else {
+        int Dm = 0;
+        for (int s = 0; s < N; ++s) Dm = std::max(Dm, (int)upper[s].size());
+        c->D_m = Dm;
+        c->bond_factor = 2;
+        ell_m.assign((size_t)std::max(Dm, 1) * N, 0u);
+        for (int s = 0; s < N; ++s)
+            for (size_t k = 0; k < upper[s].size(); ++k) ell_m[(size_t)s * Dm + k] = upper[s][k];
+    }
     // ---- Ising index tables ----
     if (ising) {
         const int K = c->n_classes;
@@ -608,11 +639,16 @@ extern "C" int ether_commit(ether_ctx* c) {
     const int G = c->Rp / kWave;
     c->lps_shift = auto_lanes_shift(c->model, R);
     if (c->lanes_req) c->lps_shift = lanes_shift_of(c->lanes_req);
-    // measurement: about 4096 wave units per pass, at least 8 sites each
-    c->meas_sites = std::max(8, (int)(((long long)N * G + 4095) / 4096));
+    // measurement: about 4096 wave units per pass of at least 8 sites each;
+    // a workgroup's four waves share one tile and one partial record
+    c->meas_sites = kWavesPerBlock * std::max(8, (int)(((long long)N * G + 4095) / 4096));
     c->meas_tiles = (N + c->meas_sites - 1) / c->meas_sites;
     c->meas_rec = ising ? (c->n_classes + nsp + 1) : (c->spiral_cap ? 9 : 6);
+    // four zero entries past the last row: a 4-wide chunk of the last row stays in the table
+    ell.resize(ell.size() + 4, 0u);
+    if (!ell_m.empty()) ell_m.resize(ell_m.size() + 4, 0u);
     TRY(dev_alloc(c, &c->d_ell, ell.size()));
+    if (!ell_m.empty()) TRY(dev_alloc(c, &c->d_ell_m, ell_m.size()));
     TRY(dev_alloc(c, &c->d_sp, (size_t)N));
     TRY(dev_alloc(c, &c->d_w, (size_t)N));
     TRY(dev_alloc(c, &c->d_jclass, c->jclass.size()));
@@ -680,6 +716,7 @@ extern "C" int ether_commit(ether_ctx* c) {
     launch_set_ctr(c->d_ctr, 0, 0, 0, c->stream);
     c->dev_step = c->dev_pt = c->dev_sc = 0;
     TRY(upload(c, c->d_ell, ell.data(), ell.size()));
+    if (!ell_m.empty()) TRY(upload(c, c->d_ell_m, ell_m.data(), ell_m.size()));
     TRY(upload(c, c->d_sp, sp.data(), sp.size()));
     TRY(upload(c, c->d_w, w.data(), w.size()));
     TRY(upload(c, c->d_jclass, c->jclass.data(), c->jclass.size()));
@@ -919,6 +956,10 @@ static MeasArgs meas_args(ether_ctx* c) {
     a.nsp = c->nsp;
     a.zvec = c->d_zvec;
     a.dsia = c->d_dsia;
+    a.ell_m = c->d_ell_m ? c->d_ell_m : c->d_ell;
+    a.D_m = c->D_m;
+    a.bond_factor = c->bond_factor;
+    a.plane_bytes = (uint32_t)((size_t)c->Rp * c->N * sizeof(float));
     a.spiral = c->d_spiral;
     a.spiral_cap = c->model == ETHER_MODEL_ISING ? 0 : c->spiral_cap;
     a.partial = c->d_partial;

@@ -124,6 +124,13 @@ struct MeasArgs {
     int nsp;
     const float* zvec;
     const float* dsia;
+    // measurement rows: every bond once, from its endpoint with the smaller
+    // slot (bond_factor 2), or the full rows where a site bonds to itself
+    // (bond_factor 1); padding entries are 0 (slot 0, class 0, zero coupling)
+    const uint32_t* ell_m;
+    int D_m;
+    int bond_factor;
+    uint32_t plane_bytes;
     const int* spiral;     // [N slots][spiral_cap] partner slots, -1 ends a row (XYZ)
     int spiral_cap;        // 0: no spiral-state observable
     double* partial;       // [tiles][Rp][rec]

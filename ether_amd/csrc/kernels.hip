@@ -213,6 +213,8 @@ __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
 // it with scalar loads into SGPRs instead of a vector load and readfirstlane.
 typedef const __attribute__((address_space(4))) uint32_t* ConstRow;
 __device__ __forceinline__ ConstRow const_row(const uint32_t* p) { return (ConstRow)p; }
+typedef const __attribute__((address_space(4))) float* ConstF32;
+typedef const __attribute__((address_space(4))) uint8_t* ConstU8;
 
 // Coupling of class c as (Jxx, Jyy, Jzz, 0), staged in LDS (one 16-byte read)
 __device__ __forceinline__ void stage_couplings(float4* sJ4, const float* jclass, int n_classes) {
@@ -876,8 +878,24 @@ __global__ __launch_bounds__(kBlock) void k_ising_sweep_lanes_batch(const IsingA
     ising_sweep_lanes_block<DM>(a, it.y);
 }
 
+// Sum over the workgroup's waves of each lane's value (waves in order), written
+// by wave 0 to out[k]; called by every thread of the workgroup.
+__device__ __forceinline__ void block_record(double* buf /* [kWavesPerBlock][kWave] */, double v,
+                                             double* out_k) {
+    const int w = wave_id(), lane = lane_id();
+    buf[w * kWave + lane] = v;
+    __syncthreads();
+    if (w == 0) {
+        double t = buf[lane];
+        for (int q = 1; q < kWavesPerBlock; ++q) t += buf[q * kWave + lane];
+        *out_k = t;
+    }
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
-// measurement partials: wave unit = (tile of sites, replica group of 64)
+// measurement partials: workgroup = (tile of sites, replica group of 64), the
+// tile split between its waves and their sums added in the workgroup
 // record per (tile, replica): XYZ {E_exch, E_z, E_sia, Mx, My, Mz};
 // Ising {bonds_1..K, zsum_1..nsp, Mz} (integers held exactly in FP64)
 // ---------------------------------------------------------------------------
@@ -891,33 +909,50 @@ __device__ __forceinline__ void xyz_measure_block(const MeasArgs& a, int blk) {
         for (int t = threadIdx.x; t < 3 * a.nsp; t += kBlock) sZ[t] = a.zvec[t];
     if (SIA)
         for (int t = threadIdx.x; t < 3 * a.nsp; t += kBlock) sD[t] = a.dsia[t];
+    __shared__ double sRed[kWavesPerBlock * kWave];
     __syncthreads();
-    const int unit = blk * kWavesPerBlock + wave_id();
-    if (unit >= a.tiles * a.G) return;
-    const int g = unit % a.G, tile = unit / a.G;
+    const int g = blk % a.G, tile = blk / a.G;             // workgroup-uniform
     const int r = g * kWave + lane_id();
+    const uint32_t roff = 4u * (uint32_t)r, rowb = 4u * (uint32_t)a.Rp;
+    const Plane px = plane_of(a.sx, a.plane_bytes), py = plane_of(a.sy, a.plane_bytes),
+                pz = plane_of(a.sz, a.plane_bytes);
+    const ConstF32 wtab = (ConstF32)a.w;
+    const ConstU8 sptab = (ConstU8)a.sp;
     double v[6] = {0, 0, 0, 0, 0, 0};
     double ss[3] = {0, 0, 0};
-    const int s_begin = tile * a.tile_sites;
-    const int s_end = min(a.N, s_begin + a.tile_sites);
+    const int per_wave = (a.tile_sites + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int s_begin = tile * a.tile_sites + wave_id() * per_wave;
+    const int s_end = min(min(a.N, (tile + 1) * a.tile_sites), s_begin + per_wave);
     for (int slot = s_begin; slot < s_end; ++slot) {
-        const size_t row = (size_t)slot * a.Rp + r;
+        const uint32_t so = (uint32_t)slot * rowb;
+        const float x = plane_ld(px, roff, so), y = plane_ld(py, roff, so), z = plane_ld(pz, roff, so);
+        // exchange over the measurement row, four neighbours' loads in flight at a time
         float hx = 0.f, hy = 0.f, hz = 0.f;
-        const ConstRow nb = const_row(a.ell + (size_t)slot * a.D);   // wave-uniform row
-        for (int k = 0; k < a.D; ++k) {
-            const uint32_t e = nb[k];
-            const size_t jr = (size_t)(e & 0xffffffu) * a.Rp + r;
-            const uint32_t c = e >> 24;
-            hx = fmaf(sJ[3 * c + 0], a.sx[jr], hx);
-            hy = fmaf(sJ[3 * c + 1], a.sy[jr], hy);
-            hz = fmaf(sJ[3 * c + 2], a.sz[jr], hz);
+        const ConstRow nb = const_row(a.ell_m + (size_t)slot * a.D_m);   // wave-uniform row
+        for (int k0 = 0; k0 < a.D_m; k0 += 4) {
+            float vx[4], vy[4], vz[4];
+            uint32_t cl[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t e = (k0 + k < a.D_m) ? nb[k0 + k] : 0u;   // 0: class 0, zero coupling
+                const uint32_t o = (e & 0xffffffu) * rowb;
+                cl[k] = e >> 24;
+                vx[k] = plane_ld(px, roff, o);
+                vy[k] = plane_ld(py, roff, o);
+                vz[k] = plane_ld(pz, roff, o);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                hx = fmaf(sJ[3 * cl[k] + 0], vx[k], hx);
+                hy = fmaf(sJ[3 * cl[k] + 1], vy[k], hy);
+                hz = fmaf(sJ[3 * cl[k] + 2], vz[k], hz);
+            }
         }
-        const float x = a.sx[row], y = a.sy[row], z = a.sz[row];
-        v[0] += (double)dot3(x, y, z, hx, hy, hz);
-        const int sp = a.sp[slot];
+        v[0] += (double)((float)a.bond_factor * dot3(x, y, z, hx, hy, hz));
+        const int sp = (ZEE || SIA) ? (int)sptab[slot] : 0;
         if (ZEE) v[1] += (double)dot3(x, y, z, sZ[3 * sp], sZ[3 * sp + 1], sZ[3 * sp + 2]);
         if (SIA) v[2] += (double)dot3(sD[3 * sp], sD[3 * sp + 1], sD[3 * sp + 2], x * x, y * y, z * z);
-        const float wgt = a.w[slot];
+        const float wgt = wtab[slot];
         v[3] += (double)(wgt * x);
         v[4] += (double)(wgt * y);
         v[5] += (double)(wgt * z);
@@ -927,8 +962,8 @@ __device__ __forceinline__ void xyz_measure_block(const MeasArgs& a, int blk) {
             for (int k = 0; k < a.spiral_cap; ++k) {
                 const int j = pj[k];
                 if (j < 0) break;
-                const size_t jr = (size_t)j * a.Rp + r;
-                const double bx = a.sx[jr], by = a.sy[jr], bz = a.sz[jr];
+                const uint32_t o = (uint32_t)j * rowb;
+                const double bx = plane_ld(px, roff, o), by = plane_ld(py, roff, o), bz = plane_ld(pz, roff, o);
                 ss[0] += (double)y * bz - (double)z * by;
                 ss[1] += (double)z * bx - (double)x * bz;
                 ss[2] += (double)x * by - (double)y * bx;
@@ -937,11 +972,10 @@ __device__ __forceinline__ void xyz_measure_block(const MeasArgs& a, int blk) {
     }
     double* out = a.partial + ((size_t)tile * a.Rp + r) * a.rec;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) out[k] = v[k];
+    for (int k = 0; k < 6; ++k) block_record(sRed, v[k], out + k);
     if (a.spiral_cap) {
-        out[6] = ss[0];
-        out[7] = ss[1];
-        out[8] = ss[2];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) block_record(sRed, ss[k], out + 6 + k);
     }
 }
 
@@ -960,10 +994,11 @@ __global__ __launch_bounds__(kBlock) void k_xyz_measure_batch(const MeasArgs* __
 }
 
 __device__ __forceinline__ void ising_measure_block(const MeasArgs& a, int blk) {
-    const int unit = blk * kWavesPerBlock + wave_id();
-    if (unit >= a.tiles * a.G) return;
-    const int g = unit % a.G, tile = unit / a.G;
+    __shared__ double sRed[kWavesPerBlock * kWave];
+    const int g = blk % a.G, tile = blk / a.G;             // workgroup-uniform
     const int r = g * kWave + lane_id();
+    const ConstF32 wtab = (ConstF32)a.w;
+    const ConstU8 sptab = (ConstU8)a.sp;
     double bonds[kMaxIsingClasses + 1];
     double zs[kMaxIsingClasses];
 #pragma unroll
@@ -971,34 +1006,44 @@ __device__ __forceinline__ void ising_measure_block(const MeasArgs& a, int blk) 
 #pragma unroll
     for (int k = 0; k < kMaxIsingClasses; ++k) zs[k] = 0.0;
     double mz = 0.0;
-    const int s_begin = tile * a.tile_sites;
-    const int s_end = min(a.N, s_begin + a.tile_sites);
+    const int per_wave = (a.tile_sites + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int s_begin = tile * a.tile_sites + wave_id() * per_wave;
+    const int s_end = min(min(a.N, (tile + 1) * a.tile_sites), s_begin + per_wave);
     for (int slot = s_begin; slot < s_end; ++slot) {
         const int si = a.s[(size_t)slot * a.Rp + r];
-        const ConstRow nb = const_row(a.ell + (size_t)slot * a.D);   // wave-uniform row
-        for (int k = 0; k < a.D; ++k) {
-            const uint32_t e = nb[k];
-            const int c = (int)(e >> 24);
-            const int sj = a.s[(size_t)(e & 0xffffffu) * a.Rp + r];
+        const ConstRow nb = const_row(a.ell_m + (size_t)slot * a.D_m);   // wave-uniform row
+        for (int k0 = 0; k0 < a.D_m; k0 += 4) {
+            int sj[4], cl[4];
 #pragma unroll
-            for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
-                if (kk == c) bonds[kk] += (double)(si * sj);
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t e = (k0 + k < a.D_m) ? nb[k0 + k] : 0u;   // 0: class 0, no bond
+                cl[k] = (int)(e >> 24);
+                sj[k] = a.s[(size_t)(e & 0xffffffu) * a.Rp + r];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
+                    if (kk == cl[k]) bonds[kk] += (double)(si * sj[k]);
         }
-        const int sp = a.sp[slot];
+        const int sp = (int)sptab[slot];
 #pragma unroll
         for (int kk = 0; kk < kMaxIsingClasses; ++kk)
             if (kk == sp) zs[kk] += (double)si;
-        mz += (double)(a.w[slot] * (float)si);
+        mz += (double)(wtab[slot] * (float)si);
     }
+    // each bond was counted once, from its endpoint with the smaller slot
+#pragma unroll
+    for (int k = 1; k <= kMaxIsingClasses; ++k) bonds[k] *= (double)a.bond_factor;
     double* out = a.partial + ((size_t)tile * a.Rp + r) * a.rec;
     int o = 0;
 #pragma unroll
     for (int k = 1; k <= kMaxIsingClasses; ++k)
-        if (k <= a.n_classes) out[o++] = bonds[k];
+        if (k <= a.n_classes) block_record(sRed, bonds[k], out + o++);
 #pragma unroll
     for (int k = 0; k < kMaxIsingClasses; ++k)
-        if (k < a.nsp) out[o++] = zs[k];
-    out[o] = mz;
+        if (k < a.nsp) block_record(sRed, zs[k], out + o++);
+    block_record(sRed, mz, out + o);
 }
 
 __global__ __launch_bounds__(kBlock) void k_ising_measure(MeasArgs a) {
@@ -1348,13 +1393,12 @@ void launch_ising_sweep(const IsingArgs& a, hipStream_t st) {
 }
 
 void launch_measure(const MeasArgs& a, int model, bool zee, bool sia, hipStream_t st) {
-    const long long units = (long long)a.tiles * a.G;
+    const long long blocks = (long long)a.tiles * a.G;    // one workgroup per (tile, replica group)
     if (g_recorder) {
-        record(LK_MEASURE, (model << 2) | ((zee ? 1 : 0) << 1) | (sia ? 1 : 0),
-               (int)((units + kWavesPerBlock - 1) / kWavesPerBlock), a);
+        record(LK_MEASURE, (model << 2) | ((zee ? 1 : 0) << 1) | (sia ? 1 : 0), (int)blocks, a);
         return;
     }
-    dim3 grid((unsigned)((units + kWavesPerBlock - 1) / kWavesPerBlock)), block(kBlock);
+    dim3 grid((unsigned)blocks), block(kBlock);
     if (model == 0) {
         hipLaunchKernelGGL(k_ising_measure, grid, block, 0, st, a);
         return;

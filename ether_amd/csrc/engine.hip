@@ -74,6 +74,7 @@ struct ether_ctx {
     // device
     uint32_t* d_ell = nullptr;
     uint32_t* d_ell_m = nullptr;    // measurement rows (d_ell itself where a site bonds to itself)
+    uint32_t* d_ell_m_n = nullptr;  // their lengths
     int D_m = 0, bond_factor = 1;
     uint8_t* d_sp = nullptr;
     float* d_w = nullptr;
@@ -559,34 +560,38 @@ extern "C" int ether_commit(ether_ctx* c) {
             ell[(size_t)s * D + k] =
                 (uint32_t)c->slot_of_site[edges[i][k].j] | ((uint32_t)edges[i][k].cls << 24);
     }
-    // ---- measurement rows: each bond from its endpoint with the smaller slot,
-    // counted twice; a self bond (a lattice one cell thick along a periodic
-    // axis) keeps the full rows, counted once ----
+    // ---- measurement rows: each bond from its endpoint with the smaller site
+    // index (in the supercell order each site then holds about half of its
+    // bonds), counted twice; a self bond (a lattice one cell thick along a
+    // periodic axis) keeps the full rows, counted once.  ell_m_n[slot] is the
+    // row's length. ----
     bool self_bond = false;
     std::vector<std::vector<uint32_t>> upper(N);
     for (int i = 0; i < N && !self_bond; ++i) {
         const int s = c->slot_of_site[i];
         for (const Edge& e : edges[i]) {
-            const int sj = c->slot_of_site[e.j];
-            if (sj == s) {
+            if (e.j == i) {
                 self_bond = true;
                 break;
             }
-            if (sj > s) upper[s].push_back((uint32_t)sj | ((uint32_t)e.cls << 24));
+            if (e.j > i) upper[s].push_back((uint32_t)c->slot_of_site[e.j] | ((uint32_t)e.cls << 24));
         }
     }
-    std::vector<uint32_t> ell_m;
+    std::vector<uint32_t> ell_m, ell_m_n(N, 0u);
     if (self_bond) {
         c->D_m = D;
         c->bond_factor = 1;
+        for (int i = 0; i < N; ++i) ell_m_n[c->slot_of_site[i]] = (uint32_t)edges[i].size();
     } else {
         int Dm = 0;
         for (int s = 0; s < N; ++s) Dm = std::max(Dm, (int)upper[s].size());
         c->D_m = Dm;
         c->bond_factor = 2;
         ell_m.assign((size_t)std::max(Dm, 1) * N, 0u);
-        for (int s = 0; s < N; ++s)
+        for (int s = 0; s < N; ++s) {
+            ell_m_n[s] = (uint32_t)upper[s].size();
             for (size_t k = 0; k < upper[s].size(); ++k) ell_m[(size_t)s * Dm + k] = upper[s][k];
+        }
     }
     // ---- Ising index tables ----
     if (ising) {
@@ -649,6 +654,7 @@ extern "C" int ether_commit(ether_ctx* c) {
     if (!ell_m.empty()) ell_m.resize(ell_m.size() + 4, 0u);
     TRY(dev_alloc(c, &c->d_ell, ell.size()));
     if (!ell_m.empty()) TRY(dev_alloc(c, &c->d_ell_m, ell_m.size()));
+    TRY(dev_alloc(c, &c->d_ell_m_n, ell_m_n.size()));
     TRY(dev_alloc(c, &c->d_sp, (size_t)N));
     TRY(dev_alloc(c, &c->d_w, (size_t)N));
     TRY(dev_alloc(c, &c->d_jclass, c->jclass.size()));
@@ -717,6 +723,7 @@ extern "C" int ether_commit(ether_ctx* c) {
     c->dev_step = c->dev_pt = c->dev_sc = 0;
     TRY(upload(c, c->d_ell, ell.data(), ell.size()));
     if (!ell_m.empty()) TRY(upload(c, c->d_ell_m, ell_m.data(), ell_m.size()));
+    TRY(upload(c, c->d_ell_m_n, ell_m_n.data(), ell_m_n.size()));
     TRY(upload(c, c->d_sp, sp.data(), sp.size()));
     TRY(upload(c, c->d_w, w.data(), w.size()));
     TRY(upload(c, c->d_jclass, c->jclass.data(), c->jclass.size()));
@@ -957,6 +964,7 @@ static MeasArgs meas_args(ether_ctx* c) {
     a.zvec = c->d_zvec;
     a.dsia = c->d_dsia;
     a.ell_m = c->d_ell_m ? c->d_ell_m : c->d_ell;
+    a.ell_m_n = c->d_ell_m_n;
     a.D_m = c->D_m;
     a.bond_factor = c->bond_factor;
     a.plane_bytes = (uint32_t)((size_t)c->Rp * c->N * sizeof(float));

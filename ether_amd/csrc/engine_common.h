@@ -125,9 +125,11 @@ struct MeasArgs {
     const float* zvec;
     const float* dsia;
     // measurement rows: every bond once, from its endpoint with the smaller
-    // slot (bond_factor 2), or the full rows where a site bonds to itself
-    // (bond_factor 1); padding entries are 0 (slot 0, class 0, zero coupling)
+    // site index (bond_factor 2), or the full rows where a site bonds to
+    // itself (bond_factor 1); padding entries are 0 (slot 0, class 0, zero
+    // coupling)
     const uint32_t* ell_m;
+    const uint32_t* ell_m_n;   // [N] entries used in each measurement row
     int D_m;
     int bond_factor;
     uint32_t plane_bytes;

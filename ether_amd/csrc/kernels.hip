@@ -923,30 +923,50 @@ __device__ __forceinline__ void xyz_measure_block(const MeasArgs& a, int blk) {
     const int per_wave = (a.tile_sites + kWavesPerBlock - 1) / kWavesPerBlock;
     const int s_begin = tile * a.tile_sites + wave_id() * per_wave;
     const int s_end = min(min(a.N, (tile + 1) * a.tile_sites), s_begin + per_wave);
-    for (int slot = s_begin; slot < s_end; ++slot) {
-        const uint32_t so = (uint32_t)slot * rowb;
-        const float x = plane_ld(px, roff, so), y = plane_ld(py, roff, so), z = plane_ld(pz, roff, so);
-        // exchange over the measurement row, four neighbours' loads in flight at a time
-        float hx = 0.f, hy = 0.f, hz = 0.f;
-        const ConstRow nb = const_row(a.ell_m + (size_t)slot * a.D_m);   // wave-uniform row
-        for (int k0 = 0; k0 < a.D_m; k0 += 4) {
-            float vx[4], vy[4], vz[4];
-            uint32_t cl[4];
+    // one site's own spin and the first four entries of its measurement row
+    struct SiteM {
+        float x, y, z;
+        float vx[4], vy[4], vz[4];
+        uint32_t cl[4];
+    };
+    const ConstRow rown = const_row(a.ell_m_n);
+    auto chunk = [&](ConstRow nb, int n, int k0, float* vx, float* vy, float* vz, uint32_t* cl) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint32_t e = (k0 + k < a.D_m) ? nb[k0 + k] : 0u;   // 0: class 0, zero coupling
-                const uint32_t o = (e & 0xffffffu) * rowb;
-                cl[k] = e >> 24;
-                vx[k] = plane_ld(px, roff, o);
-                vy[k] = plane_ld(py, roff, o);
-                vz[k] = plane_ld(pz, roff, o);
-            }
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t e = (k0 + k < n) ? nb[k0 + k] : 0u;   // 0: class 0, zero coupling
+            const uint32_t o = (e & 0xffffffu) * rowb;
+            cl[k] = e >> 24;
+            vx[k] = plane_ld(px, roff, o);
+            vy[k] = plane_ld(py, roff, o);
+            vz[k] = plane_ld(pz, roff, o);
+        }
+    };
+    auto load = [&](int slot, SiteM& d) {
+        const uint32_t so = (uint32_t)slot * rowb;
+        d.x = plane_ld(px, roff, so);
+        d.y = plane_ld(py, roff, so);
+        d.z = plane_ld(pz, roff, so);
+        chunk(const_row(a.ell_m + (size_t)slot * a.D_m), (int)rown[slot], 0, d.vx, d.vy, d.vz, d.cl);
+    };
+    auto accumulate = [&](int slot, const SiteM& d) {
+        const float x = d.x, y = d.y, z = d.z;
+        float hx = 0.f, hy = 0.f, hz = 0.f;
+        auto add = [&](const float* vx, const float* vy, const float* vz, const uint32_t* cl) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 hx = fmaf(sJ[3 * cl[k] + 0], vx[k], hx);
                 hy = fmaf(sJ[3 * cl[k] + 1], vy[k], hy);
                 hz = fmaf(sJ[3 * cl[k] + 2], vz[k], hz);
             }
+        };
+        add(d.vx, d.vy, d.vz, d.cl);
+        const ConstRow nb = const_row(a.ell_m + (size_t)slot * a.D_m);   // wave-uniform row
+        const int n = (int)rown[slot];
+        for (int k0 = 4; k0 < n; k0 += 4) {
+            float vx[4], vy[4], vz[4];
+            uint32_t cl[4];
+            chunk(nb, n, k0, vx, vy, vz, cl);
+            add(vx, vy, vz, cl);
         }
         v[0] += (double)((float)a.bond_factor * dot3(x, y, z, hx, hy, hz));
         const int sp = (ZEE || SIA) ? (int)sptab[slot] : 0;
@@ -969,6 +989,16 @@ __device__ __forceinline__ void xyz_measure_block(const MeasArgs& a, int blk) {
                 ss[2] += (double)x * by - (double)y * bx;
             }
         }
+    };
+    // two sites at a time: the second site's loads are in flight while the
+    // first is summed (sites are still added in order)
+    for (int slot = s_begin; slot < s_end; slot += 2) {
+        const bool two = slot + 1 < s_end;                  // wave-uniform
+        SiteM A, B;
+        load(slot, A);
+        load(two ? slot + 1 : slot, B);
+        accumulate(slot, A);
+        if (two) accumulate(slot + 1, B);
     }
     double* out = a.partial + ((size_t)tile * a.Rp + r) * a.rec;
 #pragma unroll
@@ -1012,11 +1042,12 @@ __device__ __forceinline__ void ising_measure_block(const MeasArgs& a, int blk) 
     for (int slot = s_begin; slot < s_end; ++slot) {
         const int si = a.s[(size_t)slot * a.Rp + r];
         const ConstRow nb = const_row(a.ell_m + (size_t)slot * a.D_m);   // wave-uniform row
-        for (int k0 = 0; k0 < a.D_m; k0 += 4) {
+        const int n = (int)const_row(a.ell_m_n)[slot];
+        for (int k0 = 0; k0 < n; k0 += 4) {
             int sj[4], cl[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const uint32_t e = (k0 + k < a.D_m) ? nb[k0 + k] : 0u;   // 0: class 0, no bond
+                const uint32_t e = (k0 + k < n) ? nb[k0 + k] : 0u;   // 0: class 0, no bond
                 cl[k] = (int)(e >> 24);
                 sj[k] = a.s[(size_t)(e & 0xffffffu) * a.Rp + r];
             }

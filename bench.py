@@ -233,6 +233,7 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
     sweep_ms, sweep_n = eng.timing(E.TIME_SWEEP)
     ovrr_ms, ovrr_n = eng.timing(E.TIME_OVRR)
     meas_ms, meas_n = eng.timing(E.TIME_MEASURE)
+    smeas_ms, smeas_n = eng.timing(E.TIME_SWEEP_MEAS)
     eng.set_timing(False)
 
     nc, col, _ = eng.layout()
@@ -273,7 +274,13 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
                                              for x in (0, 1)
                                              for i, k in enumerate(E.Engine.AUTOTUNE_SPW)}},
         "kernel_ms": {"sweep": sweep_ms, "ovrr": ovrr_ms, "measure": meas_ms,
-                      "sweep_launches": sweep_n, "ovrr_launches": ovrr_n, "measure_launches": meas_n},
+                      "sweep_measuring": smeas_ms,
+                      "sweep_launches": sweep_n, "ovrr_launches": ovrr_n, "measure_launches": meas_n,
+                      "sweep_measuring_launches": smeas_n,
+                      "note": "sweep: colour phases of the sweeps no measurement follows (the roofline "
+                              "kernel); sweep_measuring: those of the sweeps whose phases also book "
+                              "the measurement that follows (XYZ); measure: the measurement passes, "
+                              "or only their finalize after a sweep that booked them"},
         "acceptance_mean": float(mom[:, E.MOM_ACC].mean() / steps),
         "graphs": "whole periods of P = lcm(coa, OVRR, PT intervals) MCS replayed as captured HIP graphs",
     }

@@ -76,6 +76,15 @@ struct ether_ctx {
     uint32_t* d_ell_m = nullptr;    // measurement rows (d_ell itself where a site bonds to itself)
     uint32_t* d_ell_m_n = nullptr;  // their lengths
     int D_m = 0, bond_factor = 1;
+    // fused measurement (XYZ): possible when every bond has its mirror with
+    // the same coupling, no site bonds to itself and no colour has neighbours
+    // of both earlier and later colours; meas_lower[q] = 1 where colour q's
+    // neighbours are all of earlier colours; mscale = fixed-point scales
+    // (exchange, Zeeman, SIA, moment)
+    bool meas_fused_ok = false;
+    std::vector<int> meas_lower;
+    double mscale[4] = {1.0, 1.0, 1.0, 1.0};
+    unsigned long long* d_macc = nullptr;
     uint8_t* d_sp = nullptr;
     float* d_w = nullptr;
     float* d_jclass = nullptr;
@@ -173,6 +182,21 @@ template <class T>
 static int upload(ether_ctx* c, T* d, const T* h, size_t n) {
     HIPCHK(c, hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
     return ETHER_OK;
+}
+
+// The acceptance counters are kAccCopies copies of [R] (the sweeps spread
+// their atomics over them): an asynchronous copy of all of them, and the
+// total of one slot once the stream has been synchronised.
+static int copy_acc_async(ether_ctx* c, std::vector<unsigned long long>& raw) {
+    raw.resize((size_t)kAccCopies * c->R);
+    HIPCHK(c, hipMemcpyAsync(raw.data(), c->d_acc, sizeof(unsigned long long) * raw.size(),
+                             hipMemcpyDeviceToHost, c->stream));
+    return ETHER_OK;
+}
+static unsigned long long acc_total(const std::vector<unsigned long long>& raw, int R, int r) {
+    unsigned long long t = 0;
+    for (int k = 0; k < kAccCopies; ++k) t += raw[(size_t)k * R + r];
+    return t;
 }
 
 #define TRY(expr)                      \
@@ -565,6 +589,10 @@ extern "C" int ether_commit(ether_ctx* c) {
     // bonds), counted twice; a self bond (a lattice one cell thick along a
     // periodic axis) keeps the full rows, counted once.  ell_m_n[slot] is the
     // row's length. ----
+    // Booking a bond once and counting it twice is the reference's double sum
+    // only if every bond i -> j has its mirror j -> i with the same coupling
+    // (src/j_values.f90:83-84 makes j_exc symmetric; neighbour lists built by
+    // distance are); otherwise the full rows are used too.
     bool self_bond = false;
     std::vector<std::vector<uint32_t>> upper(N);
     for (int i = 0; i < N && !self_bond; ++i) {
@@ -577,8 +605,20 @@ extern "C" int ether_commit(ether_ctx* c) {
             if (e.j > i) upper[s].push_back((uint32_t)c->slot_of_site[e.j] | ((uint32_t)e.cls << 24));
         }
     }
+    bool mirrored = !self_bond;
+    if (mirrored) {
+        std::vector<uint64_t> fwd, rev;
+        for (int i = 0; i < N; ++i)
+            for (const Edge& e : edges[i]) {
+                fwd.push_back(((uint64_t)i << 32) | ((uint64_t)e.j << 8) | (uint64_t)e.cls);
+                rev.push_back(((uint64_t)e.j << 32) | ((uint64_t)i << 8) | (uint64_t)e.cls);
+            }
+        std::sort(fwd.begin(), fwd.end());
+        std::sort(rev.begin(), rev.end());
+        mirrored = fwd == rev;
+    }
     std::vector<uint32_t> ell_m, ell_m_n(N, 0u);
-    if (self_bond) {
+    if (!mirrored) {
         c->D_m = D;
         c->bond_factor = 1;
         for (int i = 0; i < N; ++i) ell_m_n[c->slot_of_site[i]] = (uint32_t)edges[i].size();
@@ -633,6 +673,48 @@ extern "C" int ether_commit(ether_ctx* c) {
         if (c->sia)
             for (int x = 0; x < 3; ++x) dsia[3 * a + x] = (float)c->siav[3 * a + x];
     }
+    // ---- fused measurement (XYZ without spiral partners): a phase books the
+    // exchange bonds of its sites when all their neighbours are of earlier
+    // colours, none when all come later; a colour with both (three or more
+    // colours) would need the field split per neighbour, and such lattices
+    // measure in a pass of their own.  Fixed-point scales: powers of two with
+    // |term| * scale <= 2^48 / N for every site term, so that the sum over
+    // all sites stays below 2^48 (kernels round each term to an integer,
+    // exactly, below 2^51) ----
+    c->meas_fused_ok = !ising && mirrored && !c->spiral_cap;
+    c->meas_lower.assign(c->ncol, 0);
+    for (int q = 0; q < c->ncol && c->meas_fused_ok; ++q) {
+        bool lo = false, hi = false;
+        for (int s = c->cbeg[q]; s < c->cbeg[q + 1]; ++s)
+            for (const Edge& e : edges[c->site_of_slot[s]])
+                (c->slot_of_site[e.j] < c->cbeg[q] ? lo : hi) = true;
+        c->meas_lower[q] = lo ? 1 : 0;
+        if (lo && hi) c->meas_fused_ok = false;
+    }
+    if (c->meas_fused_ok) {
+        double bound[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int i = 0; i < N; ++i) {
+            double t = 0.0;          // |h| <= sum of the largest |J| component per bond
+            for (const Edge& e : edges[i])
+                t += std::max(std::fabs(c->jclass[3 * e.cls]),
+                              std::max(std::fabs(c->jclass[3 * e.cls + 1]), std::fabs(c->jclass[3 * e.cls + 2])));
+            bound[0] = std::max(bound[0], 2.0 * t);
+        }
+        for (int a = 0; a < nsp; ++a) {
+            bound[1] = std::max(bound[1], (double)std::fabs(zvec[3 * a]) + std::fabs(zvec[3 * a + 1]) +
+                                              std::fabs(zvec[3 * a + 2]));
+            bound[2] = std::max(bound[2], (double)std::fabs(dsia[3 * a]) + std::fabs(dsia[3 * a + 1]) +
+                                              std::fabs(dsia[3 * a + 2]));
+        }
+        for (int s = 0; s < N; ++s) bound[3] = std::max(bound[3], (double)std::fabs(w[s]));
+        for (int k = 0; k < 4; ++k) {
+            // 2x for spins a few ulp longer than 1 and the FP32 roundings of a term
+            const double total = 2.0 * (double)N * bound[k];
+            c->mscale[k] = total > 0.0 ? std::ldexp(1.0, std::max(-1000, std::min(1000,
+                                             (int)std::floor(std::log2(std::ldexp(1.0, 49) / total)))))
+                                       : 1.0;
+        }
+    }
     // ---- device buffers ----
     const int R = c->R;
     c->Rp = (R + kWave - 1) / kWave * kWave;
@@ -686,7 +768,7 @@ extern "C" int ether_commit(ether_ctx* c) {
     TRY(dev_alloc(c, &c->d_rep_at, (size_t)R));
     TRY(dev_alloc(c, &c->d_beta_f, (size_t)R));
     TRY(dev_alloc(c, &c->d_beta, (size_t)R));
-    TRY(dev_alloc(c, &c->d_acc, (size_t)R));
+    TRY(dev_alloc(c, &c->d_acc, (size_t)kAccCopies * R));
     TRY(dev_alloc(c, &c->d_mom, (size_t)R * ETHER_MOM_COUNT));
     TRY(dev_alloc(c, &c->d_partial, (size_t)c->Rp * c->meas_tiles * c->meas_rec));
     TRY(dev_alloc(c, &c->d_Ept, (size_t)R));
@@ -701,6 +783,11 @@ extern "C" int ether_commit(ether_ctx* c) {
     TRY(dev_alloc(c, &c->d_scratch_h, (size_t)N * 4));
     TRY(dev_alloc(c, &c->d_scratch_e, (size_t)N));
     TRY(dev_alloc(c, &c->d_ctr, 4));
+    if (c->meas_fused_ok) {
+        const size_t n = (size_t)kMeasCopies * c->Rp * kMeasWords;
+        TRY(dev_alloc(c, &c->d_macc, n));
+        HIPCHK(c, hipMemsetAsync(c->d_macc, 0, sizeof(unsigned long long) * n, c->stream));
+    }
     if (c->spiral_cap) {
         // partner rows in slot space, in the order the sites' rows were given
         std::vector<int> rows((size_t)N * c->spiral_cap, -1);
@@ -735,7 +822,7 @@ extern "C" int ether_commit(ether_ctx* c) {
     for (int r = 0; r < R; ++r) ident[r] = r;
     TRY(upload(c, c->d_slot_of, ident.data(), (size_t)R));
     TRY(upload(c, c->d_rep_at, ident.data(), (size_t)R));
-    HIPCHK(c, hipMemsetAsync(c->d_acc, 0, sizeof(unsigned long long) * R, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_acc, 0, sizeof(unsigned long long) * kAccCopies * R, c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_mom, 0, sizeof(double) * R * ETHER_MOM_COUNT, c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_pt_acc, 0, sizeof(unsigned long long) * R, c->stream));
     HIPCHK(c, hipMemsetAsync(c->d_pt_att, 0, sizeof(unsigned long long) * R, c->stream));
@@ -907,6 +994,11 @@ static XyzArgs xyz_args(ether_ctx* c, int colour) {
     a.dphi = (float)c->dphi;
     a.zee = c->zee;
     a.sia = c->sia;
+    a.meas = 0;
+    a.meas_lower = 0;
+    a.w = c->d_w;
+    a.macc = c->d_macc;
+    for (int k = 0; k < 4; ++k) a.mscale[k] = c->mscale[k];
     return a;
 }
 
@@ -1050,11 +1142,13 @@ static bool phase_empty(const ether_ctx* c, int q) {
     return q >= c->ncol || c->cbeg[q + 1] == c->cbeg[q];
 }
 
-static int enqueue_sweep(ether_ctx* c, double fuse_frac = 0.0) {
+// meas: the phases also book the fused measurement that the next
+// enqueue_measure(..., fused = true) completes.
+static int enqueue_sweep(ether_ctx* c, double fuse_frac = 0.0, bool meas = false) {
     int launches = 0;
     for (int q = 0; q < phase_count(c); ++q)
         if (!phase_empty(c, q) || g_recorder) ++launches;
-    const int ev = ev_begin(c, ETHER_TIME_SWEEP, launches);
+    const int ev = ev_begin(c, meas ? ETHER_TIME_SWEEP_MEAS : ETHER_TIME_SWEEP, launches);
     for (int q = 0; q < phase_count(c); ++q) {
         const bool empty = phase_empty(c, q);
         if (empty && !g_recorder) continue;
@@ -1069,6 +1163,16 @@ static int enqueue_sweep(ether_ctx* c, double fuse_frac = 0.0) {
             if (fuse_frac > 0.0) {
                 a.fuse = 1;
                 a.frac = (float)fuse_frac;
+            }
+            if (meas) {
+                a.meas = 1;
+                a.meas_lower = empty ? 0 : c->meas_lower[qq];
+                // a workgroup books its sums with one atomic per (replica,
+                // word), so the phase's atomics scale with its workgroups:
+                // the longest walk per wave keeps them few (the spins do not
+                // depend on the launch shape)
+                a.spw = kMaxSitesPerWave;
+                a.nblocks = phase_blocks(a.c_end - a.c_begin, a.G, a.xcd_map, a.spw << (6 - a.lps_shift));
             }
             launch_xyz_sweep(a, c->angle, c->zee, c->sia, c->stream);
         }
@@ -1102,9 +1206,18 @@ static int enqueue_ovrr(ether_ctx* c, double passes) {
     return ETHER_OK;
 }
 
-static int enqueue_measure(ether_ctx* c, int sample) {
+// fused: the record was booked by the sweep just enqueued (enqueue_sweep with
+// meas); only the finalize runs.  While a batch records, an empty measurement
+// launch keeps the launch sequence aligned with lattices that measure apart.
+static int enqueue_measure(ether_ctx* c, int sample, bool fused = false) {
     const int ev = ev_begin(c, ETHER_TIME_MEASURE);
-    launch_measure(meas_args(c), c->model, c->zee, c->sia, c->stream);
+    if (!fused) {
+        launch_measure(meas_args(c), c->model, c->zee, c->sia, c->stream);
+    } else if (g_recorder) {
+        MeasArgs m = meas_args(c);
+        m.tiles = 0;
+        launch_measure(m, c->model, c->zee, c->sia, c->stream);
+    }
     FinArgs f;
     f.partial = c->d_partial;
     f.tiles = c->meas_tiles;
@@ -1128,15 +1241,17 @@ static int enqueue_measure(ether_ctx* c, int sample) {
     f.spiral_count = c->spiral_cap ? c->spiral_count : 0.0;
     f.psi = c->d_psi;
     f.mom_sp = c->d_mom_sp;
+    f.macc = fused ? c->d_macc : nullptr;
+    for (int k = 0; k < 4; ++k) f.mscale[k] = c->mscale[k];
     launch_finalize(f, c->stream);
     ev_end(c, ev);
     HIPCHK(c, hipGetLastError());
     return ETHER_OK;
 }
 
-static int enqueue_pt(ether_ctx* c, int parity) {
+static int enqueue_pt(ether_ctx* c, int parity, bool fused = false) {
     if (c->n_groups <= 0) return ETHER_OK;
-    TRY(enqueue_measure(c, 0));
+    TRY(enqueue_measure(c, 0, fused));
     PtArgs p;
     p.n_groups = c->n_groups;
     p.gstart = c->d_gstart;
@@ -1192,19 +1307,15 @@ extern "C" int ether_sweep(ether_ctx* c, int32_t n_mcs, double* accept) {
     if (n_mcs < 0) return fail(c, ETHER_ERR_ARG, "ether_sweep: negative n_mcs");
     std::vector<unsigned long long> before;
     for (int k = 0; k < n_mcs; ++k) {
-        if (accept && k == n_mcs - 1) {
-            before.resize(c->R);
-            HIPCHK(c, hipMemcpyAsync(before.data(), c->d_acc, sizeof(unsigned long long) * c->R,
-                                     hipMemcpyDeviceToHost, c->stream));
-        }
+        if (accept && k == n_mcs - 1) TRY(copy_acc_async(c, before));
         TRY(enqueue_sweep(c));
     }
     if (accept && n_mcs > 0) {
-        std::vector<unsigned long long> after(c->R);
-        HIPCHK(c, hipMemcpyAsync(after.data(), c->d_acc, sizeof(unsigned long long) * c->R,
-                                 hipMemcpyDeviceToHost, c->stream));
+        std::vector<unsigned long long> after;
+        TRY(copy_acc_async(c, after));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        for (int r = 0; r < c->R; ++r) accept[r] = (double)(after[r] - before[r]) / c->N;
+        for (int r = 0; r < c->R; ++r)
+            accept[r] = (double)(acc_total(after, c->R, r) - acc_total(before, c->R, r)) / c->N;
     }
     return ETHER_OK;
 }
@@ -1241,7 +1352,7 @@ extern "C" int ether_autotune(ether_ctx* c, int32_t reps, int32_t* sites_per_wav
         planes.push_back({c->d_sz, RN * 4});
         if (c->angle) planes.push_back({c->d_phi, RN * 4});
     }
-    planes.push_back({c->d_acc, sizeof(unsigned long long) * c->R});
+    planes.push_back({c->d_acc, sizeof(unsigned long long) * kAccCopies * c->R});
     size_t total = 0;
     for (auto& p : planes) total += (p.second + 255) / 256 * 256;
     char* save = nullptr;
@@ -1358,13 +1469,13 @@ extern "C" int ether_get_energy_parts(ether_ctx* c, double* parts, int64_t* bond
 extern "C" int ether_read_moments(ether_ctx* c, double* moments) {
     TRY(need_committed(c));
     if (!moments) return fail(c, ETHER_ERR_ARG, "ether_read_moments: null pointer");
-    std::vector<unsigned long long> acc(c->R);
+    std::vector<unsigned long long> acc;
     HIPCHK(c, hipMemcpyAsync(moments, c->d_mom, sizeof(double) * ETHER_MOM_COUNT * c->R,
                              hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(acc.data(), c->d_acc, sizeof(unsigned long long) * c->R,
-                             hipMemcpyDeviceToHost, c->stream));
+    TRY(copy_acc_async(c, acc));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (int r = 0; r < c->R; ++r) moments[ETHER_MOM_COUNT * r + ETHER_MOM_ACC] = (double)acc[r] / c->N;
+    for (int r = 0; r < c->R; ++r)
+        moments[ETHER_MOM_COUNT * r + ETHER_MOM_ACC] = (double)acc_total(acc, c->R, r) / c->N;
     return ETHER_OK;
 }
 
@@ -1373,7 +1484,7 @@ extern "C" int ether_reset_moments(ether_ctx* c) {
     HIPCHK(c, hipMemsetAsync(c->d_mom, 0, sizeof(double) * ETHER_MOM_COUNT * c->R, c->stream));
     if (c->d_mom_sp)
         HIPCHK(c, hipMemsetAsync(c->d_mom_sp, 0, sizeof(double) * 3 * c->R, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->d_acc, 0, sizeof(unsigned long long) * c->R, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_acc, 0, sizeof(unsigned long long) * kAccCopies * c->R, c->stream));
     return ETHER_OK;
 }
 
@@ -1465,15 +1576,21 @@ static int enqueue_step(ether_ctx* c, int32_t stepi, int32_t mcs_eq, int32_t coa
     // split order is asked for.
     const bool ovrr = ovrr_iv && stepi % ovrr_iv == 0 && stepi % coa != 0;
     const bool fuse = ovrr && ovrr_passes <= 1.0 && !(c->compat & ETHER_COMPAT_SPLIT_OVRR);
+    const bool pt = pt_iv && stepi % pt_iv == 0 && stepi % coa != 0;
+    const bool sample = stepi > mcs_eq && stepi % coa == 0;
+    // a measurement follows this sweep (for the swap or the sample): the
+    // sweep's phases book it as they go (XYZ), instead of a pass of its own
+    const bool fm = c->meas_fused_ok && !(c->compat & ETHER_COMPAT_MEASURE_PASS) &&
+                    ((pt && c->n_groups > 0) || sample);
     if (ovrr && !fuse) TRY(enqueue_ovrr(c, ovrr_passes));
-    TRY(enqueue_sweep(c, fuse ? ovrr_passes : 0.0));
+    TRY(enqueue_sweep(c, fuse ? ovrr_passes : 0.0, fm));
     // src/ether.f90:189-198
-    if (pt_iv && stepi % pt_iv == 0 && stepi % coa != 0) {
+    if (pt) {
         ++*sc;
-        TRY(enqueue_pt(c, *sc % 2));
+        TRY(enqueue_pt(c, *sc % 2, fm));
     }
     // src/ether.f90:201 -> src/observable.f90:30-31
-    if (stepi > mcs_eq && stepi % coa == 0) TRY(enqueue_measure(c, 1));
+    if (sample) TRY(enqueue_measure(c, 1, fm));
     return ETHER_OK;
 }
 

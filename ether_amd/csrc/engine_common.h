@@ -35,6 +35,10 @@ constexpr int kMaxClasses = 255;     // coupling classes (8-bit field of an ELL 
 constexpr int kMaxSpecies = 16;
 constexpr int kMaxIsingClasses = 8;
 constexpr int kIsingLdsTable = 4096; // thresholds staged in LDS up to this many entries
+constexpr int kMeasWords = 6;        // fused measurement record: E_exch, E_z, E_sia, Mx, My, Mz
+constexpr int kMeasCopies = 16;      // copies of the fused sums the workgroups spread their atomics over
+constexpr int kAccCopies = 16;       // copies of the acceptance counters, likewise
+static_assert(kMeasCopies <= kWave, "the finalize reads one copy per lane");
 
 struct XyzArgs {
     float* sx;
@@ -63,7 +67,7 @@ struct XyzArgs {
     float frac;
     int fuse;              // sweep: reflect (with probability frac) before the proposal
     uint32_t plane_bytes;  // bytes of one spin plane, N * Rp * 4 (< 2 GiB)
-    unsigned long long* acc;
+    unsigned long long* acc;   // [kAccCopies][R slots] accepted moves
     int n_classes;
     const float* jclass;
     int nsp;
@@ -72,6 +76,19 @@ struct XyzArgs {
     float dphi;
     int zee;               // Zeeman term on (wave-uniform runtime flag)
     int sia;               // single-ion anisotropy on
+    // Fused measurement (set on the phases of a sweep that a measurement
+    // follows).  Each updated site adds the terms of its final spin to the
+    // record {E_exch, E_z, E_sia, Mx, My, Mz} of its replica: where all its
+    // neighbours are of earlier colours (final already) the exchange energy
+    // of its bonds, counted twice; its Zeeman and SIA energies and its
+    // weighted spin.  A term is rounded per site to an integer multiple of
+    // 1/mscale and summed exactly (int64 in macc), so the totals do not
+    // depend on which wave or workgroup adds what, nor in which order.
+    int meas;              // 0: off
+    int meas_lower;        // 1: the phase's neighbours are all of earlier colours, 0: none are
+    const float* w;        // [N] magnetisation weight per slot
+    unsigned long long* macc;  // [kMeasCopies][Rp][kMeasWords] two's-complement int64 sums
+    double mscale[4];      // powers of two for exchange, Zeeman, SIA, magnetisation
 };
 
 struct IsingArgs {
@@ -93,7 +110,7 @@ struct IsingArgs {
     uint64_t seed;
     const uint32_t* ctr;
     uint32_t step_off;
-    unsigned long long* acc;
+    unsigned long long* acc;   // [kAccCopies][R slots] accepted moves
     int n_classes;
     const int* stride;     // [n_classes+1], stride[0] = 0
     int offset;            // sum_k Z_k * stride_k
@@ -161,6 +178,9 @@ struct FinArgs {
     double spiral_count;     // > 0: record words 6..8 hold sum S_i x S_j over the partners
     double* psi;             // [R] spiral-state parameter of the measured state
     double* mom_sp;          // [R slots][3] sums of psi, psi^2, psi^4 over samples
+    unsigned long long* macc;  // non-null: the record comes from the fused measurement's
+                               // [kMeasCopies][Rp][kMeasWords] sums (cleared here for the next one)
+    double mscale[4];
 };
 
 struct PtArgs {
@@ -237,7 +257,7 @@ enum LaunchKind : int {
 };
 struct LaunchRecord {
     int kind;
-    int variant;   // template instance: sweep (lanes << 17) | (angle << 16) | D;
+    int variant;   // template instance: sweep (meas << 18) | (lanes << 17) | (angle << 16) | D;
                    // measure (model << 2) | (zee << 1) | sia
     int units;     // workgroups of the launch on its own, without XCD padding (0: nothing to do)
     int bytes;     // size of the argument block

@@ -8,7 +8,9 @@
 //                   int8), exact integer acceptance thresholds
 //   k_xyz_ovrr      one colour phase of overrelaxation (src/overrelaxation.f90:30-49)
 //   k_*_measure     per-replica energy and magnetisation partial sums
-//                   (src/get_tot_energy.f90:35-45, src/get_tot_magnetisation.f90:35-38)
+//                   (src/get_tot_energy.f90:35-45, src/get_tot_magnetisation.f90:35-38);
+//                   for XYZ the same record is usually booked by the sweep that
+//                   precedes the measurement (k_xyz_sweep<..., MEAS = true>)
 //   k_finalize      fixed-order reduction of the partials plus the observable()
 //                   moment update (src/observable.f90:30-43, src/get_eng.f90:36-39,
 //                   src/get_mag.f90:36-40)
@@ -64,6 +66,17 @@ __device__ __forceinline__ void normalise3(float& x, float& y, float& z) {
     y *= inv;
     z *= inv;
 }
+// round(v * scale) as an integer-valued double.  scale is a power of two
+// chosen on the host so that |v * scale| < 2^50 for every site term and the
+// sum of the terms of all sites stays below 2^51: adding 1.5 * 2^52 puts the
+// value in the binade whose spacing is 1, so the addition rounds it to an
+// integer and the subtraction is exact.  Sums of such terms are exact in any
+// order.
+__device__ __forceinline__ double fixed_point(float v, double scale) {
+    const double magic = 6755399441055744.0;   // 1.5 * 2^52
+    return fma((double)v, scale, magic) - magic;
+}
+
 // Marsaglia's disc test a1^2 + a2^2 < 2^46 on 24-bit coordinates, exact.
 // The FP32 sum is within 2^24 of the integer sum (|a| <= 2^23 converts
 // exactly; two roundings of at most 2^-24 relative), so outside a band of
@@ -173,6 +186,9 @@ __device__ __forceinline__ void angle_move(u32x4 w, float dphi, float phi_old, f
 // per-lane counters of the four waves of a block, summed into the slot counters
 // Lane l of every wave of the block holds the counts of replicas
 // r_lane + q (q < nrep_lane); r_lane is the same in every wave of the block.
+// The sum goes to copy blockIdx % kAccCopies of the counters: device-scope
+// atomics on one address serialise, and every workgroup books every slot of
+// its replica group (the host adds the copies).
 __device__ __forceinline__ void book_acceptance(unsigned* sAcc, int nrep_lane, const unsigned* mine,
                                                 int r_lane, int R, const int* slot_of,
                                                 unsigned long long* acc) {
@@ -184,7 +200,8 @@ __device__ __forceinline__ void book_acceptance(unsigned* sAcc, int nrep_lane, c
             unsigned t = 0;
             for (int v = 0; v < kWavesPerBlock; ++v) t += sAcc[(v * kWave + lane) * nrep_lane + q];
             const int r = r_lane + q;
-            if (t && r < R) atomicAdd(acc + slot_of[r], (unsigned long long)t);
+            if (t && r < R)
+                atomicAdd(acc + (size_t)(blockIdx.x % kAccCopies) * R + slot_of[r], (unsigned long long)t);
         }
     }
 }
@@ -296,14 +313,64 @@ __device__ __forceinline__ float delta_sia(const float* D, float nx, float ny, f
     return dot3(D[0], D[1], D[2], fmaf(nx, nx, -(x * x)), fmaf(ny, ny, -(y * y)), fmaf(nz, nz, -(z * z)));
 }
 
+// Fused measurement: the record terms of one site's final spin f, added to
+// the lane's fixed-point sums m (see XyzArgs::meas).  In a phase whose
+// neighbours are all of earlier colours (final already) the exchange term is
+// f.hl, hl the site's exchange field, counted twice (every bond is booked
+// once, from its later-coloured end); in a phase whose neighbours all come
+// later it is left to them.  The others are the standalone measurement's
+// per-site terms.
+__device__ __forceinline__ void book_site(double (&m)[kMeasWords], const XyzArgs& a, bool zee,
+                                          bool sia, const float* sZ, const float* sD, int sp,
+                                          int slot, float fx, float fy, float fz, const float* hl) {
+    if (a.meas_lower) m[0] += fixed_point(2.0f * dot3(fx, fy, fz, hl[0], hl[1], hl[2]), a.mscale[0]);
+    if (zee) m[1] += fixed_point(dot3(fx, fy, fz, sZ[3 * sp], sZ[3 * sp + 1], sZ[3 * sp + 2]), a.mscale[1]);
+    if (sia)
+        m[2] += fixed_point(dot3(sD[3 * sp], sD[3 * sp + 1], sD[3 * sp + 2], fx * fx, fy * fy, fz * fz),
+                            a.mscale[2]);
+    const float wgt = ((ConstF32)a.w)[slot];
+    m[3] += fixed_point(wgt * fx, a.mscale[3]);
+    m[4] += fixed_point(wgt * fy, a.mscale[3]);
+    m[5] += fixed_point(wgt * fz, a.mscale[3]);
+}
+
+// Adds a workgroup's fixed-point sums to macc.  Lanes of one replica (one
+// per wave, or several lane groups per wave in the lanes kernels: lane l
+// holds replica r_base + (l & (per - 1))) are summed through LDS, then one
+// int64 atomic per (replica, word) into copy blockIdx % kMeasCopies of the
+// sums: device-scope atomics on one address serialise, and hundreds of
+// workgroups book every replica.  The terms are integers whose sums stay
+// below 2^51, so FP64 adds them exactly and the result is the same in any
+// order.  Called by every thread of the workgroup.
+__device__ __forceinline__ void book_measurement(double* buf /* [kMeasWords][kBlock] */,
+                                                 const double (&m)[kMeasWords], int per, int r_base,
+                                                 const XyzArgs& a) {
+#pragma unroll
+    for (int k = 0; k < kMeasWords; ++k) buf[k * kBlock + threadIdx.x] = m[k];
+    __syncthreads();
+    const int w = wave_id(), lane = lane_id();
+    for (int k = w; k < kMeasWords; k += kWavesPerBlock) {
+        // words that are zero for the whole launch take no atomics
+        if ((k == 0 && !a.meas_lower) || (k == 1 && !a.zee) || (k == 2 && !a.sia)) continue;
+        const int r = r_base + lane;
+        if (lane < per && r < a.R) {
+            double t = 0.0;
+            for (int i = lane; i < kBlock; i += per) t += buf[k * kBlock + i];
+            const size_t copy = blockIdx.x % kMeasCopies;
+            atomicAdd(a.macc + (copy * a.Rp + r) * kMeasWords + k, (unsigned long long)(long long)t);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // XYZ Metropolis, one colour phase.  With a.fuse set the phase first reflects
 // each site about its local field (selected with probability a.frac, the OVRR
 // draw of the split pass) and proposes from the reflected spin: one pass over
 // the spins instead of an OVRR phase and a sweep phase.  Within a phase the
-// neighbours are fixed, so both moves use the same field.
+// neighbours are fixed, so both moves use the same field.  With MEAS the
+// phase also books the fused measurement (book_site / book_measurement).
 // ---------------------------------------------------------------------------
-template <bool ANGLE, int DM>
+template <bool ANGLE, int DM, bool MEAS>
 __device__ __forceinline__ void xyz_sweep_block(const XyzArgs& a, int bb) {
     __shared__ float4 sJ4[kMaxClasses + 1];
     __shared__ float sZ[3 * kMaxSpecies];
@@ -330,9 +397,12 @@ __device__ __forceinline__ void xyz_sweep_block(const XyzArgs& a, int bb) {
     const uint32_t step = a.ctr[0] + a.step_off;
     uint64_t* pool = sPool[w];
     unsigned accepted = 0;
+    double m[kMeasWords] = {0, 0, 0, 0, 0, 0};
 
-    // the update of one site from its spin (x, y, z, phi) and local exchange field
-    auto update = [&](int slot, float x, float y, float z, float cphi, float hx, float hy, float hz) {
+    // the update of one site from its spin (x, y, z, phi) and local exchange
+    // field (hl: the part from earlier colours, for the fused measurement)
+    auto update = [&](int slot, float x, float y, float z, float cphi, float hx, float hy, float hz,
+                      const float* hl) {
         const uint32_t so = (uint32_t)slot * rowb;
         const int sp = (ZEE || SIA) ? a.sp[slot] : 0;
         if (ZEE) {
@@ -372,6 +442,14 @@ __device__ __forceinline__ void xyz_sweep_block(const XyzArgs& a, int bb) {
             plane_st(pz, roff, so, acc ? nz : z);
             if (ANGLE) plane_st(pp, roff, so, acc ? nphi : cphi);
         }
+        if constexpr (MEAS) {
+            if (acc) {
+                x = nx;
+                y = ny;
+                z = nz;
+            }
+            book_site(m, a, ZEE, SIA, sZ, sD, sp, slot, x, y, z, hl);
+        }
     };
 
     const int s0 = a.c_begin + (chunk * kWavesPerBlock + w) * a.spw;
@@ -381,28 +459,34 @@ __device__ __forceinline__ void xyz_sweep_block(const XyzArgs& a, int bb) {
         const uint32_t so = (uint32_t)slot * rowb;
         const float x = plane_ld(px, roff, so), y = plane_ld(py, roff, so), z = plane_ld(pz, roff, so);
         const float cphi = ANGLE ? plane_ld(pp, roff, so) : 0.f;
+        const uint32_t* nb = a.ell + (size_t)slot * a.D;
         float hx, hy, hz;
-        local_field<DM>(px, py, pz, a.ell + (size_t)slot * a.D, a.D, rowb, roff, sJ4, hx, hy, hz);
-        update(slot, x, y, z, cphi, hx, hy, hz);
+        local_field<DM>(px, py, pz, nb, a.D, rowb, roff, sJ4, hx, hy, hz);
+        const float hl[3] = {hx, hy, hz};             // exchange field, before Zeeman
+        update(slot, x, y, z, cphi, hx, hy, hz, hl);
     }
     // acceptance is booked to the temperature slot, as the reference books it per rank
     book_acceptance(sAcc, 1, &accepted, r, a.R, a.slot_of, a.acc);
+    if constexpr (MEAS) {
+        __shared__ double sMeas[kMeasWords * kBlock];
+        book_measurement(sMeas, m, kWave, g * kWave, a);
+    }
 }
 
-template <bool ANGLE, int DM>
+template <bool ANGLE, int DM, bool MEAS>
 __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
-    xyz_sweep_block<ANGLE, DM>(a, block_index(a.nblocks, a.xcd_map));
+    xyz_sweep_block<ANGLE, DM, MEAS>(a, block_index(a.nblocks, a.xcd_map));
 }
 
 // Batched form: workgroup b runs local workgroup items[b].y of the launch
 // described by args[items[b].x] (one lattice's colour phase); x < 0 pads.
-template <bool ANGLE, int DM>
+template <bool ANGLE, int DM, bool MEAS>
 __global__ __launch_bounds__(kBlock) void k_xyz_sweep_batch(const XyzArgs* __restrict__ args,
                                                            const int2* __restrict__ items) {
     const int2 it = items[blockIdx.x];
     if (it.x < 0) return;
     const XyzArgs a = args[it.x];
-    xyz_sweep_block<ANGLE, DM>(a, it.y);
+    xyz_sweep_block<ANGLE, DM, MEAS>(a, it.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -673,7 +757,7 @@ __device__ __forceinline__ void marsaglia_pooled_group(u32x4 w, bool active, uin
     else marsaglia_retry(seed, id, r, step, RNG_METROPOLIS, 1, x, y, z);
 }
 
-template <bool ANGLE, int DM>
+template <bool ANGLE, int DM, bool MEAS>
 __device__ __forceinline__ void xyz_sweep_lanes_block(const XyzArgs& a, int bb) {
     __shared__ float4 sJ4[kMaxClasses + 1];
     __shared__ float sZ[3 * kMaxSpecies];
@@ -702,6 +786,7 @@ __device__ __forceinline__ void xyz_sweep_lanes_block(const XyzArgs& a, int bb) 
     const uint32_t step = a.ctr[0] + a.step_off;
     uint64_t* pool = sPool[w];
     unsigned accepted = 0;
+    double m[kMeasWords] = {0, 0, 0, 0, 0, 0};
     const int s0 = a.c_begin + (bb * kWavesPerBlock + w) * a.spw * S;
     for (int q = 0; q < a.spw; ++q) {
         const int base = s0 + q * S;                           // wave-uniform
@@ -712,8 +797,10 @@ __device__ __forceinline__ void xyz_sweep_lanes_block(const XyzArgs& a, int bb) 
         const uint32_t vo = (uint32_t)slot * rowb + roff;
         float x = plane_ldv(px, vo), y = plane_ldv(py, vo), z = plane_ldv(pz, vo);
         float cphi = ANGLE ? plane_ldv(pp, vo) : 0.f;
+        const uint32_t* nb = a.ell + (size_t)slot * a.D;
         float hx, hy, hz;
-        local_field_lanes<DM>(px, py, pz, a.ell + (size_t)slot * a.D, a.D, rowb, roff, sJ4, hx, hy, hz);
+        local_field_lanes<DM>(px, py, pz, nb, a.D, rowb, roff, sJ4, hx, hy, hz);
+        const float hl[3] = {hx, hy, hz};             // exchange field, before Zeeman
         const int sp = (ZEE || SIA) ? a.sp[slot] : 0;
         if (ZEE) {
             hx += sZ[3 * sp + 0];
@@ -743,7 +830,8 @@ __device__ __forceinline__ void xyz_sweep_lanes_block(const XyzArgs& a, int bb) 
         }
         float dE = delta_field(nx, ny, nz, x, y, z, hx, hy, hz);
         if (SIA) dE += delta_sia(sD + 3 * sp, nx, ny, nz, x, y, z);
-        if (rv && -beta * dE > __logf(eta)) {
+        const bool acc = rv && -beta * dE > __logf(eta);
+        if (acc) {
             plane_stv(px, vo, nx);
             plane_stv(py, vo, ny);
             plane_stv(pz, vo, nz);
@@ -755,22 +843,36 @@ __device__ __forceinline__ void xyz_sweep_lanes_block(const XyzArgs& a, int bb) 
             plane_stv(pz, vo, z);
             if (ANGLE) plane_stv(pp, vo, cphi);
         }
+        if constexpr (MEAS) {
+            if (sv) {                                  // an idle lane group books nothing
+                if (acc) {
+                    x = nx;
+                    y = ny;
+                    z = nz;
+                }
+                book_site(m, a, ZEE, SIA, sZ, sD, sp, slot, x, y, z, hl);
+            }
+        }
     }
     book_acceptance(sAcc, 1, &accepted, r, a.R, a.slot_of, a.acc);
+    if constexpr (MEAS) {
+        __shared__ double sMeas[kMeasWords * kBlock];
+        book_measurement(sMeas, m, 1 << ls, 0, a);
+    }
 }
 
-template <bool ANGLE, int DM>
+template <bool ANGLE, int DM, bool MEAS>
 __global__ __launch_bounds__(kBlock) void k_xyz_sweep_lanes(XyzArgs a) {
-    xyz_sweep_lanes_block<ANGLE, DM>(a, block_index(a.nblocks, a.xcd_map));
+    xyz_sweep_lanes_block<ANGLE, DM, MEAS>(a, block_index(a.nblocks, a.xcd_map));
 }
 
-template <bool ANGLE, int DM>
+template <bool ANGLE, int DM, bool MEAS>
 __global__ __launch_bounds__(kBlock) void k_xyz_sweep_lanes_batch(const XyzArgs* __restrict__ args,
                                                                  const int2* __restrict__ items) {
     const int2 it = items[blockIdx.x];
     if (it.x < 0) return;
     const XyzArgs a = args[it.x];
-    xyz_sweep_lanes_block<ANGLE, DM>(a, it.y);
+    xyz_sweep_lanes_block<ANGLE, DM, MEAS>(a, it.y);
 }
 
 template <int DM>
@@ -1102,7 +1204,27 @@ __device__ __forceinline__ void finalize_block(const FinArgs& a, int blk) {
     const int lane = threadIdx.x;
     double ex = 0.0, ez = 0.0, es = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
     double psi = 0.0;
-    if (a.model == 1) {
+    if (a.model == 1 && a.macc) {
+        // fused measurement: exact fixed-point sums (lane c < kMeasCopies
+        // reads copy c), cleared for the next one
+        double v[kMeasWords] = {0, 0, 0, 0, 0, 0};
+        unsigned long long* q = a.macc + ((size_t)lane * a.Rp + r) * kMeasWords;
+        if (lane < kMeasCopies) {
+#pragma unroll
+            for (int k = 0; k < kMeasWords; ++k) {
+                v[k] = (double)(long long)q[k];
+                q[k] = 0ull;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kMeasWords; ++k) v[k] = wave_sum(v[k]);
+        ex = v[0] / a.mscale[0];
+        ez = v[1] / a.mscale[1];
+        es = v[2] / a.mscale[2];
+        mx = v[3] / a.mscale[3];
+        my = v[4] / a.mscale[3];
+        mz = v[5] / a.mscale[3];
+    } else if (a.model == 1) {
         const int nv = a.spiral_count > 0.0 ? 9 : 6;
         double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         for (int t = lane; t < a.tiles; t += 64) {
@@ -1366,29 +1488,35 @@ static void record(int kind, int variant, int units, const A& a) {
 // sites a wave updates per step of its loop
 static int sites_per_step(int lps_shift) { return kWave >> lps_shift; }
 
-void launch_xyz_sweep(const XyzArgs& a, bool angle, bool, bool, hipStream_t st) {
-    const bool lanes = a.lps_shift < 6;
-    if (g_recorder) {
-        record(LK_XYZ_SWEEP, (angle ? 65536 : 0) | (lanes ? 131072 : 0) | a.D,
-               sweep_units(a.G, a.c_end - a.c_begin, a.spw * sites_per_step(a.lps_shift)), a);
-        return;
-    }
+template <bool MEAS>
+static void launch_xyz_sweep_t(const XyzArgs& a, bool angle, bool lanes, hipStream_t st) {
     dim3 grid(a.nblocks), block(kBlock);
     if (lanes) {
-#define LA(DM) hipLaunchKernelGGL((k_xyz_sweep_lanes<true, DM>), grid, block, 0, st, a)
-#define LM(DM) hipLaunchKernelGGL((k_xyz_sweep_lanes<false, DM>), grid, block, 0, st, a)
+#define LA(DM) hipLaunchKernelGGL((k_xyz_sweep_lanes<true, DM, MEAS>), grid, block, 0, st, a)
+#define LM(DM) hipLaunchKernelGGL((k_xyz_sweep_lanes<false, DM, MEAS>), grid, block, 0, st, a)
         if (angle) { ETHER_DISPATCH_DM(a.D, LA) }
         else { ETHER_DISPATCH_DM(a.D, LM) }
 #undef LA
 #undef LM
         return;
     }
-#define LA(DM) hipLaunchKernelGGL((k_xyz_sweep<true, DM>), grid, block, 0, st, a)
-#define LM(DM) hipLaunchKernelGGL((k_xyz_sweep<false, DM>), grid, block, 0, st, a)
+#define LA(DM) hipLaunchKernelGGL((k_xyz_sweep<true, DM, MEAS>), grid, block, 0, st, a)
+#define LM(DM) hipLaunchKernelGGL((k_xyz_sweep<false, DM, MEAS>), grid, block, 0, st, a)
     if (angle) { ETHER_DISPATCH_DM(a.D, LA) }
     else { ETHER_DISPATCH_DM(a.D, LM) }
 #undef LA
 #undef LM
+}
+
+void launch_xyz_sweep(const XyzArgs& a, bool angle, bool, bool, hipStream_t st) {
+    const bool lanes = a.lps_shift < 6;
+    if (g_recorder) {
+        record(LK_XYZ_SWEEP, (a.meas ? 262144 : 0) | (angle ? 65536 : 0) | (lanes ? 131072 : 0) | a.D,
+               sweep_units(a.G, a.c_end - a.c_begin, a.spw * sites_per_step(a.lps_shift)), a);
+        return;
+    }
+    if (a.meas) launch_xyz_sweep_t<true>(a, angle, lanes, st);
+    else launch_xyz_sweep_t<false>(a, angle, lanes, st);
 }
 
 void launch_xyz_ovrr(const XyzArgs& a, bool angle, bool, hipStream_t st) {
@@ -1460,6 +1588,27 @@ void launch_pt_swap(const PtArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k_pt_swap, dim3(blocks), dim3(64), 0, st, a);
 }
 
+template <bool MEAS>
+static void launch_xyz_sweep_batch_t(const XyzArgs* a, const int2* d_items, int variant, dim3 grid,
+                                     hipStream_t st) {
+    const dim3 block(kBlock);
+    if (variant & 131072) {
+#define LA(DM) hipLaunchKernelGGL((k_xyz_sweep_lanes_batch<true, DM, MEAS>), grid, block, 0, st, a, d_items)
+#define LM(DM) hipLaunchKernelGGL((k_xyz_sweep_lanes_batch<false, DM, MEAS>), grid, block, 0, st, a, d_items)
+        if (variant & 65536) { ETHER_DISPATCH_DM(variant & 65535, LA) }
+        else { ETHER_DISPATCH_DM(variant & 65535, LM) }
+#undef LA
+#undef LM
+        return;
+    }
+#define LA(DM) hipLaunchKernelGGL((k_xyz_sweep_batch<true, DM, MEAS>), grid, block, 0, st, a, d_items)
+#define LM(DM) hipLaunchKernelGGL((k_xyz_sweep_batch<false, DM, MEAS>), grid, block, 0, st, a, d_items)
+    if (variant & 65536) { ETHER_DISPATCH_DM(variant & 65535, LA) }
+    else { ETHER_DISPATCH_DM(variant & 65535, LM) }
+#undef LA
+#undef LM
+}
+
 void launch_batch(int kind, int variant, const void* d_args, const int2* d_items, int n_items,
                   hipStream_t st) {
     if (n_items <= 0) return;
@@ -1467,21 +1616,8 @@ void launch_batch(int kind, int variant, const void* d_args, const int2* d_items
     switch (kind) {
     case LK_XYZ_SWEEP: {
         const XyzArgs* a = static_cast<const XyzArgs*>(d_args);
-        if (variant & 131072) {
-#define LA(DM) hipLaunchKernelGGL((k_xyz_sweep_lanes_batch<true, DM>), grid, block, 0, st, a, d_items)
-#define LM(DM) hipLaunchKernelGGL((k_xyz_sweep_lanes_batch<false, DM>), grid, block, 0, st, a, d_items)
-            if (variant & 65536) { ETHER_DISPATCH_DM(variant & 65535, LA) }
-            else { ETHER_DISPATCH_DM(variant & 65535, LM) }
-#undef LA
-#undef LM
-            break;
-        }
-#define LA(DM) hipLaunchKernelGGL((k_xyz_sweep_batch<true, DM>), grid, block, 0, st, a, d_items)
-#define LM(DM) hipLaunchKernelGGL((k_xyz_sweep_batch<false, DM>), grid, block, 0, st, a, d_items)
-        if (variant & 65536) { ETHER_DISPATCH_DM(variant & 65535, LA) }
-        else { ETHER_DISPATCH_DM(variant & 65535, LM) }
-#undef LA
-#undef LM
+        if (variant & 262144) launch_xyz_sweep_batch_t<true>(a, d_items, variant, grid, st);
+        else launch_xyz_sweep_batch_t<false>(a, d_items, variant, grid, st);
         break;
     }
     case LK_XYZ_OVRR: {

@@ -23,9 +23,10 @@ MODEL_ISING = 0
 MODEL_XYZ = 1
 INIT_FRESH = 1
 INIT_RANDOM = 2
-TIME_SWEEP, TIME_OVRR, TIME_MEASURE, TIME_PT, TIME_SQ = 0, 1, 2, 3, 4
+TIME_SWEEP, TIME_OVRR, TIME_MEASURE, TIME_PT, TIME_SQ, TIME_SWEEP_MEAS = 0, 1, 2, 3, 4, 5
 COMPAT_PT_ETOT = 1
 COMPAT_SPLIT_OVRR = 2
+COMPAT_MEASURE_PASS = 4
 
 MOM_N, MOM_E, MOM_E2, MOM_E4, MOM_M, MOM_M2, MOM_M4, MOM_ACC = range(8)
 MOM_COUNT = 8

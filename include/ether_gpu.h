@@ -60,6 +60,10 @@ typedef struct ether_ctx ether_ctx;
                                     by default a pass of at most one reflection per site is fused into the
                                     sweep, each colour phase reflecting a site about its local field and then
                                     proposing from the reflected spin (the same stationary distribution)     */
+#define ETHER_COMPAT_MEASURE_PASS 4 /* ether_run: every measurement as a pass of its own after the sweep, as
+                                    get_tot_energy after Metropolis (src/ether.f90:172-177); by default an
+                                    XYZ sweep that a measurement follows books it as its phases update the
+                                    sites (the same energies to FP32 rounding)                            */
 
 /* per-replica moment record read by ether_read_moments (8 doubles) */
 #define ETHER_MOM_N     0   /* number of samples                                  */
@@ -301,16 +305,19 @@ int ether_autotune(ether_ctx* ctx, int32_t reps, int32_t* sites_per_wave, int32_
 /* Kernel timing with HIP events recorded on the context's stream around the
  * launches of a kind: 0 Metropolis colour phases (one event pair around the
  * back-to-back phases of a sweep), 1 overrelaxation colour phase,
- * 2 measurement (partials + finalize), 3 tempering swap, 4 structure factor
- * (device part of one ether_structure_factor call).  Adds no
+ * 2 measurement (partials + finalize, or the finalize alone after a sweep
+ * that booked it), 3 tempering swap, 4 structure factor (device part of one
+ * ether_structure_factor call), 5 the colour phases of a sweep that also
+ * books the measurement that follows it (XYZ, inside ether_run).  Adds no
  * synchronisation; ether_get_timing synchronises and reports the summed
  * event time and the number of launches it covers since timing was switched
  * on. */
-#define ETHER_TIME_SWEEP   0
-#define ETHER_TIME_OVRR    1
-#define ETHER_TIME_MEASURE 2
-#define ETHER_TIME_PT      3
-#define ETHER_TIME_SQ      4
+#define ETHER_TIME_SWEEP      0
+#define ETHER_TIME_OVRR       1
+#define ETHER_TIME_MEASURE    2
+#define ETHER_TIME_PT         3
+#define ETHER_TIME_SQ         4
+#define ETHER_TIME_SWEEP_MEAS 5
 int ether_set_timing(ether_ctx* ctx, int32_t on);
 int ether_get_timing(ether_ctx* ctx, int32_t kind, double* total_ms, int64_t* launches);
 /* Waits for all work queued on the context's stream. */

@@ -107,4 +107,12 @@ def test_event_timing_counts_every_colour_phase():
     ms, n = eng.timing(E.TIME_SWEEP)
     eng.set_timing(False)
     assert n == 12 * nc and ms > 0.0
+    # with samples every 4 MCS, the sweeps before them book the measurement
+    # (their own timing kind) and only the finalize is timed as measurement
+    eng.set_timing(True)
+    eng.run(13, 12, 0, 4, 1, 1.0, 0, 0)
+    assert eng.timing(E.TIME_SWEEP)[1] == 9 * nc
+    assert eng.timing(E.TIME_SWEEP_MEAS)[1] == 3 * nc
+    assert eng.timing(E.TIME_MEASURE)[1] == 3
+    eng.set_timing(False)
     eng.close()

@@ -71,7 +71,7 @@ def cpu_baseline(model, budget_s=10.0):
                       f"Heisenberg chain at T=20 K, FP64 C oracle, single thread, {dt:.1f} s"}
 
 
-def describe(name, m, n_temps, R, steps, split_ovrr, coa, ovrr_iv):
+def describe(name, m, n_temps, R, steps, split_ovrr, coa, ovrr_iv, measure_pass=False):
     t = m.tags
     kind = "Ising" if t.ising else "Heisenberg"
     moves = "" if t.ising else (f", ANGLE moves delta {math.degrees(t.dphi):g} deg" if t.angle else ", Marsaglia moves")
@@ -82,7 +82,7 @@ def describe(name, m, n_temps, R, steps, split_ovrr, coa, ovrr_iv):
                 f"({'split pass' if split_ovrr else 'fused into the sweep phases'})")
     return (f"{name}: {kind} {m.sc[0]}x{m.sc[1]}x{m.sc[2]} cells ({m.n_sites} spins){extra}, "
             f"{n_temps} T x {t.repeat} repeats = {R} chains{moves}{ovrr}, "
-            f"MCS {steps}/{t.tmcs_eq}, coa {coa}")
+            f"MCS {steps}/{t.tmcs_eq}, coa {coa}" + (", measurement as a pass of its own" if measure_pass else ""))
 
 
 def profiled_traffic(kernel, workload):
@@ -139,6 +139,8 @@ def main():
     ap.add_argument("--ising-steps", type=int, default=2000)
     ap.add_argument("--split-ovrr", action="store_true",
                     help="the reference's order: a whole OVRR pass, then the sweep")
+    ap.add_argument("--measure-pass", action="store_true",
+                    help="every measurement as a pass of its own after the sweep")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe, 4)
@@ -194,7 +196,8 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
     t = m.tags
     temps = t.temperatures()
     R = len(temps) * t.repeat
-    eng = make_engine(m, R, compat=E.COMPAT_SPLIT_OVRR if split_ovrr else 0)
+    eng = make_engine(m, R, compat=(E.COMPAT_SPLIT_OVRR if split_ovrr else 0) |
+                      (E.COMPAT_MEASURE_PASS if args.measure_pass else 0))
     eng.set_betas(np.tile(m.betas(temps), t.repeat))
     spw, xcd, tune_ms = eng.autotune(reps=10)
     coa = fortran_nint(t.coa)
@@ -255,7 +258,8 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
         "dtype": "f32",
         "data": "synthetic (random-start spins, reference in.ether tags of the workload)",
         "config": {
-            "workload": describe(workload, m, len(temps), R, steps, split_ovrr, coa, ovrr_iv),
+            "workload": describe(workload, m, len(temps), R, steps, split_ovrr, coa, ovrr_iv,
+                                 args.measure_pass),
             "replicas": R, "sites": m.n_sites, "colours": nc, "mcs": steps,
         },
         "roofline": {

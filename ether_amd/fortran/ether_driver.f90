@@ -251,6 +251,8 @@ contains
             read (value, *, iostat=ios) gpu_graphs
          case ('compat')
             read (value, *, iostat=ios) gpu_compat
+            if (ios == 0 .and. (gpu_compat < 0 .or. gpu_compat > 7)) &
+               call terminate('compat: a sum of the flags 1, 2 and 4')
          case ('chunk')
             read (value, *, iostat=ios) gpu_chunk
             if (gpu_chunk < 0) call terminate('chunk must be >= 0')

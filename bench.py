@@ -85,16 +85,18 @@ def describe(name, m, n_temps, R, steps, split_ovrr, coa, ovrr_iv, measure_pass=
             f"MCS {steps}/{t.tmcs_eq}, coa {coa}" + (", measurement as a pass of its own" if measure_pass else ""))
 
 
-def profiled_traffic(kernel, workload):
-    """HBM bytes per launch of the sweep kernel from the committed PMC summary
-    (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, factors from a plain copy), or None."""
+def profiled_traffic(kernel, workload, spw):
+    """HBM bytes per launch of the sweep kernel at `spw` sites per wave from
+    the committed PMC summary (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, factors from
+    a plain copy; lines `traffic_bytes_per_launch[spw=N] = bytes`), or None."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.md")
     if not os.path.exists(path):
         return None
+    key = f"traffic_bytes_per_launch[spw={spw}]"
     with open(path) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_launch"):
-                return float(line.split("=")[1])
+            if line.startswith(key):
+                return float(line.split("=", 2)[2])
     return None
 
 
@@ -141,12 +143,14 @@ def main():
                     help="the reference's order: a whole OVRR pass, then the sweep")
     ap.add_argument("--measure-pass", action="store_true",
                     help="every measurement as a pass of its own after the sweep")
+    ap.add_argument("--launch", type=int, nargs=2, metavar=("SPW", "XCD"), default=None,
+                    help="sites per wave and XCD map of the sweep instead of the autotuner's choice")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe, 4)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the replicas "
+        print(f"bench.py: runs on one GPU only (asked for {args.gpus}); the replicas "
               "of a run are independent and would shard across GPUs by whole ladders", file=sys.stderr)
         sys.exit(2)
 
@@ -199,7 +203,12 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
     eng = make_engine(m, R, compat=(E.COMPAT_SPLIT_OVRR if split_ovrr else 0) |
                       (E.COMPAT_MEASURE_PASS if args.measure_pass else 0))
     eng.set_betas(np.tile(m.betas(temps), t.repeat))
-    spw, xcd, tune_ms = eng.autotune(reps=10)
+    if args.launch:
+        spw, xcd = args.launch
+        eng.set_launch(spw, xcd)
+        tune_ms = None
+    else:
+        spw, xcd, tune_ms = eng.autotune(reps=10)
     coa = fortran_nint(t.coa)
     ovrr_on = t.ovrr and t.xyz and not t.zeeman and not t.sia
     ovrr_iv = t.ovrr_mcs if ovrr_on else 0
@@ -264,7 +273,7 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(kernel, workload),
+            "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(kernel, workload, spw),
             "kernel": kernel + " (one colour phase)",
             "bytes_per_update": b_upd, "updates_per_launch": per_launch_updates,
             "avg_launch_ms": avg_ms, "launches": sweep_n,
@@ -274,9 +283,9 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
                       f"divided by the phases they cover",
         },
         "launch": {"sites_per_wave": spw, "xcd_map": xcd,
-                   "autotune_ms_per_sweep": {f"spw{k}_xcd{x}": round(float(tune_ms[x][i]), 5)
-                                             for x in (0, 1)
-                                             for i, k in enumerate(E.Engine.AUTOTUNE_SPW)}},
+                   "autotune_ms_per_sweep": None if tune_ms is None else
+                   {f"spw{k}_xcd{x}": round(float(tune_ms[x][i]), 5)
+                    for x in (0, 1) for i, k in enumerate(E.Engine.AUTOTUNE_SPW)}},
         "kernel_ms": {"sweep": sweep_ms, "ovrr": ovrr_ms, "measure": meas_ms,
                       "sweep_measuring": smeas_ms,
                       "sweep_launches": sweep_n, "ovrr_launches": ovrr_n, "measure_launches": meas_n,

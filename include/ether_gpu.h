@@ -294,8 +294,10 @@ int ether_set_launch(ether_ctx* ctx, int32_t sites_per_wave, int32_t xcd_map);
  * 64 / lanes sites. */
 int ether_set_lanes(ether_ctx* ctx, int32_t lanes_per_site);
 /* Autotuner: times `reps` sweeps for each of sites_per_wave {1,2,4,8,16} x
- * xcd_map {0,1} on the context's own lattice and replicas, keeps the fastest
- * (returned through the two nullable pointers), and restores the spins, the
+ * xcd_map {0,1} on the context's own lattice and replicas, in two rounds
+ * (each candidate keeps its faster one), keeps the fastest, or among the
+ * candidates within 1% of it the one with the most sites per wave (returned
+ * through the two nullable pointers), and restores the spins, the
  * acceptance counters and the sweep counter.  table_ms[10] (nullable)
  * receives the time per sweep of candidate i = xcd_map*5 + index of
  * sites_per_wave.  Allocates a temporary copy of the spin planes. */

@@ -84,7 +84,12 @@ def test_autotune_restores_state_and_changes_no_result(name, over, R):
             say(f"  chose sites_per_wave={spw} xcd_map={xcd}; ms per sweep {np.round(table, 4)}")
             assert spw in E.Engine.AUTOTUNE_SPW and xcd in (0, 1)
             assert np.all(table > 0)
-            assert table[xcd][E.Engine.AUTOTUNE_SPW.index(spw)] == table.min()
+            # the fastest, or the most sites per wave among those within 1% of it
+            chosen = table[xcd][E.Engine.AUTOTUNE_SPW.index(spw)]
+            assert chosen <= 1.01 * table.min()
+            near = [E.Engine.AUTOTUNE_SPW[i] for x in (0, 1) for i in range(table.shape[1])
+                    if table[x][i] <= 1.01 * table.min()]
+            assert spw == max(near)
             assert np.array_equal(_spins(eng, R), before)
             assert np.array_equal(eng.moments(), mom_before)
         eng.sweep(3)

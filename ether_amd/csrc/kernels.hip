@@ -239,6 +239,44 @@ __device__ __forceinline__ void stage_couplings(float4* sJ4, const float* jclass
         sJ4[t] = make_float4(jclass[3 * t + 0], jclass[3 * t + 1], jclass[3 * t + 2], 0.f);
 }
 
+// A short row's gathers (DM <= 8) and their sum, apart: work that does not
+// depend on the neighbours (the proposal's Philox block) goes between
+// issuing the loads and waiting for them.  Same order of accumulation as
+// local_field, so the field is bit-identical.
+template <int DM>
+struct RowVals {
+    float x[DM], y[DM], z[DM];
+    uint32_t cl[DM];
+};
+template <int DM>
+__device__ __forceinline__ void gather_row(Plane px, Plane py, Plane pz, const uint32_t* nb,
+                                           uint32_t row_bytes, uint32_t roff, RowVals<DM>& v) {
+    const ConstRow row = const_row(nb);
+#pragma unroll
+    for (int k = 0; k < DM; ++k) {
+        const uint32_t e = row[k];
+        const uint32_t so = (e & 0xffffffu) * row_bytes;
+        v.cl[k] = e >> 24;
+        v.x[k] = plane_ld(px, roff, so);
+        v.y[k] = plane_ld(py, roff, so);
+        v.z[k] = plane_ld(pz, roff, so);
+    }
+}
+template <int DM>
+__device__ __forceinline__ void sum_row(const RowVals<DM>& v, const float4* sJ4, float& hx, float& hy,
+                                        float& hz) {
+    hx = 0.f;
+    hy = 0.f;
+    hz = 0.f;
+#pragma unroll
+    for (int k = 0; k < DM; ++k) {
+        const float4 j = sJ4[v.cl[k]];
+        hx = fmaf(j.x, v.x[k], hx);
+        hy = fmaf(j.y, v.y[k], hy);
+        hz = fmaf(j.z, v.z[k], hz);
+    }
+}
+
 // Neighbours gathered per chunk of a long row: all DM loads of a row in
 // flight at once cost registers (130 VGPRs at DM = 18, 3 waves per SIMD);
 // chunks of 6 with a scheduling barrier between them keep 4 waves per SIMD
@@ -416,7 +454,7 @@ __device__ __forceinline__ void xyz_sweep_block(const XyzArgs& a, int bb) {
     // the update of one site from its spin (x, y, z, phi) and local exchange
     // field (hl: the part from earlier colours, for the fused measurement)
     auto update = [&](int slot, float x, float y, float z, float cphi, float hx, float hy, float hz,
-                      const float* hl) {
+                      const float* hl, u32x4 wr) {
         const uint32_t so = (uint32_t)slot * rowb;
         const int sp = (ZEE || SIA) ? a.sp[slot] : 0;
         if (ZEE) {
@@ -436,7 +474,6 @@ __device__ __forceinline__ void xyz_sweep_block(const XyzArgs& a, int bb) {
                 if (ANGLE && reflected) cphi = atan2f(y, x);
             }
         }
-        const u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, step, RNG_METROPOLIS, 0);
         const float eta = eta24f(wr.z);
         float nx = 0.f, ny = 0.f, nz = 1.f, nphi = 0.f;
         if (ANGLE) {
@@ -475,9 +512,22 @@ __device__ __forceinline__ void xyz_sweep_block(const XyzArgs& a, int bb) {
         const float cphi = ANGLE ? plane_ld(pp, roff, so) : 0.f;
         const uint32_t* nb = a.ell + (size_t)slot * a.D;
         float hx, hy, hz;
-        local_field<DM>(px, py, pz, nb, a.D, rowb, roff, sJ4, hx, hy, hz);
+        u32x4 wr;
+        if constexpr (DM > 0 && DM <= 8) {
+            RowVals<DM> v;
+            gather_row<DM>(px, py, pz, nb, rowb, roff, v);
+            wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, step, RNG_METROPOLIS, 0);
+            // an empty asm that takes the block: the compiler computes it
+            // here, while the gathers are in flight, instead of sinking it
+            // to its first use after the field sum
+            asm volatile("" : "+v"(wr.x), "+v"(wr.y), "+v"(wr.z), "+v"(wr.w));
+            sum_row<DM>(v, sJ4, hx, hy, hz);
+        } else {
+            local_field<DM>(px, py, pz, nb, a.D, rowb, roff, sJ4, hx, hy, hz);
+            wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, step, RNG_METROPOLIS, 0);
+        }
         const float hl[3] = {hx, hy, hz};             // exchange field, before Zeeman
-        update(slot, x, y, z, cphi, hx, hy, hz, hl);
+        update(slot, x, y, z, cphi, hx, hy, hz, hl, wr);
     }
     // acceptance is booked to the temperature slot, as the reference books it per rank
     book_acceptance(sAcc, 1, &accepted, r, a.R, a.slot_of, a.acc);
@@ -613,12 +663,21 @@ __device__ __forceinline__ void ising_sweep_block(const IsingArgs& a, int bb) {
             cnt[k] = 0;
         }
         const ConstRow nb = const_row(a.ell + (size_t)slot * a.D);   // wave-uniform row
+        uint32_t* own = reinterpret_cast<uint32_t*>(a.s + (size_t)slot * a.Rp + r0);
+        uint32_t v;
+        u32x4 wr;
         if constexpr (DM > 0) {
-            // issue every neighbour load first (the row is padded to DM with class 0)
+            // issue every neighbour load and the own load first (the row is
+            // padded to DM with class 0), then draw the Philox block while
+            // they are in flight (the empty asm keeps the draw here instead
+            // of at its first use)
             uint32_t nv[DM];
 #pragma unroll
             for (int k = 0; k < DM; ++k)
                 nv[k] = *reinterpret_cast<const uint32_t*>(a.s + (size_t)(nb[k] & 0xffffffu) * a.Rp + r0);
+            v = *own;
+            wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)rid, step, RNG_METROPOLIS, 0);
+            asm volatile("" : "+v"(wr.x), "+v"(wr.y), "+v"(wr.z), "+v"(wr.w));
 #pragma unroll
             for (int k = 0; k < DM; ++k) {
                 const int c = (int)(nb[k] >> 24);
@@ -632,16 +691,15 @@ __device__ __forceinline__ void ising_sweep_block(const IsingArgs& a, int bb) {
                 const uint32_t e = nb[k];
                 const int c = (int)(e >> 24);
                 if (c == 0) continue;
-                const uint32_t v = *reinterpret_cast<const uint32_t*>(a.s + (size_t)(e & 0xffffffu) * a.Rp + r0);
-                class_add(d, c, (v >> 1) & 0x01010101u);
+                const uint32_t nbv = *reinterpret_cast<const uint32_t*>(a.s + (size_t)(e & 0xffffffu) * a.Rp + r0);
+                class_add(d, c, (nbv >> 1) & 0x01010101u);
 #pragma unroll
                 for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
                     if (kk == c) cnt[kk] += 1;
             }
+            v = *own;
+            wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)rid, step, RNG_METROPOLIS, 0);
         }
-        uint32_t* own = reinterpret_cast<uint32_t*>(a.s + (size_t)slot * a.Rp + r0);
-        const uint32_t v = *own;
-        const u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)rid, step, RNG_METROPOLIS, 0);
         const uint32_t words[4] = {wr.x, wr.y, wr.z, wr.w};
         const int spc = a.sp[slot];
         uint32_t flip = 0;
@@ -697,6 +755,21 @@ __device__ __forceinline__ float plane_ldv(Plane p, uint32_t voff) {
 }
 __device__ __forceinline__ void plane_stv(Plane p, uint32_t voff, float v) {
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), p, (int)voff, 0, 0);
+}
+
+// gather_row with a per-lane row (the lanes kernels)
+template <int DM>
+__device__ __forceinline__ void gather_row_lanes(Plane px, Plane py, Plane pz, const uint32_t* nb,
+                                                 uint32_t row_bytes, uint32_t roff, RowVals<DM>& v) {
+#pragma unroll
+    for (int k = 0; k < DM; ++k) {
+        const uint32_t e = nb[k];
+        const uint32_t vo = (e & 0xffffffu) * row_bytes + roff;
+        v.cl[k] = e >> 24;
+        v.x[k] = plane_ldv(px, vo);
+        v.y[k] = plane_ldv(py, vo);
+        v.z[k] = plane_ldv(pz, vo);
+    }
 }
 
 template <int DM>
@@ -818,7 +891,17 @@ __device__ __forceinline__ void xyz_sweep_lanes_block(const XyzArgs& a, int bb) 
         float cphi = ANGLE ? plane_ldv(pp, vo) : 0.f;
         const uint32_t* nb = a.ell + (size_t)slot * a.D;
         float hx, hy, hz;
-        local_field_lanes<DM>(px, py, pz, nb, a.D, rowb, roff, sJ4, hx, hy, hz);
+        u32x4 wr;
+        if constexpr (DM > 0 && DM <= 8) {
+            RowVals<DM> v;
+            gather_row_lanes<DM>(px, py, pz, nb, rowb, roff, v);
+            wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, step, RNG_METROPOLIS, 0);
+            asm volatile("" : "+v"(wr.x), "+v"(wr.y), "+v"(wr.z), "+v"(wr.w));   // as in xyz_sweep_block
+            sum_row<DM>(v, sJ4, hx, hy, hz);
+        } else {
+            local_field_lanes<DM>(px, py, pz, nb, a.D, rowb, roff, sJ4, hx, hy, hz);
+            wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, step, RNG_METROPOLIS, 0);
+        }
         const float hl[3] = {hx, hy, hz};             // exchange field, before Zeeman
         const int sp = (ZEE || SIA) ? a.sp[slot] : 0;
         if (ZEE) {
@@ -838,7 +921,6 @@ __device__ __forceinline__ void xyz_sweep_lanes_block(const XyzArgs& a, int bb) 
                 if (ANGLE && reflected) cphi = atan2f(y, x);
             }
         }
-        const u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, step, RNG_METROPOLIS, 0);
         const float eta = eta24f(wr.z);
         float nx = 0.f, ny = 0.f, nz = 1.f, nphi = 0.f;
         if (ANGLE) {
@@ -931,11 +1013,17 @@ __device__ __forceinline__ void ising_sweep_lanes_block(const IsingArgs& a, int 
             cnt[k] = 0;
         }
         const uint32_t* nb = a.ell + (size_t)slot * a.D;
+        uint32_t* own = reinterpret_cast<uint32_t*>(a.s + (size_t)slot * a.Rp + r0);
+        uint32_t v;
+        u32x4 wr;
         if constexpr (DM > 0) {
             uint32_t nv[DM];
 #pragma unroll
             for (int k = 0; k < DM; ++k)
                 nv[k] = *reinterpret_cast<const uint32_t*>(a.s + (size_t)(nb[k] & 0xffffffu) * a.Rp + r0);
+            v = *own;
+            wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)rid, step, RNG_METROPOLIS, 0);
+            asm volatile("" : "+v"(wr.x), "+v"(wr.y), "+v"(wr.z), "+v"(wr.w));   // as in ising_sweep_block
 #pragma unroll
             for (int k = 0; k < DM; ++k) {
                 const int c = (int)(nb[k] >> 24);
@@ -949,16 +1037,15 @@ __device__ __forceinline__ void ising_sweep_lanes_block(const IsingArgs& a, int 
                 const uint32_t e = nb[k];
                 const int c = (int)(e >> 24);
                 if (c == 0) continue;
-                const uint32_t v = *reinterpret_cast<const uint32_t*>(a.s + (size_t)(e & 0xffffffu) * a.Rp + r0);
-                class_add(d, c, (v >> 1) & 0x01010101u);
+                const uint32_t nbv = *reinterpret_cast<const uint32_t*>(a.s + (size_t)(e & 0xffffffu) * a.Rp + r0);
+                class_add(d, c, (nbv >> 1) & 0x01010101u);
 #pragma unroll
                 for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
                     if (kk == c) cnt[kk] += 1;
             }
+            v = *own;
+            wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)rid, step, RNG_METROPOLIS, 0);
         }
-        uint32_t* own = reinterpret_cast<uint32_t*>(a.s + (size_t)slot * a.Rp + r0);
-        const uint32_t v = *own;
-        const u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)rid, step, RNG_METROPOLIS, 0);
         const uint32_t words[4] = {wr.x, wr.y, wr.z, wr.w};
         const int spc = a.sp[slot];
         uint32_t flip = 0;

@@ -8,7 +8,9 @@ resident on the GPU and advanced together; MCS 5000 with 3000 equilibration,
 observables every 10 MCS.  A "step" is one MCS of every chain (OVRR +
 Metropolis sweep + the scheduled measurement), the per-MCS schedule of the
 reference's driver, with the reflection fused into each colour phase of the
-sweep.  The launch shape of the sweep is picked by the autotuner first.  The
+sweep and the measurement booked by the sweep before it (two colours).  The
+launch shape of the sweep is picked by the autotuner first (`--launch` pins
+it).  The
 timed region is the whole production run of `--steps` MCS, whole schedule
 periods replayed as HIP graphs; `--warmup` untimed MCS run first on a separate
 start state.
@@ -19,8 +21,8 @@ roofline = the Metropolis colour-phase kernel: 24 B of own-spin traffic per
         update (FP32 SoA read + write) / its average launch time from HIP
         events on the engine stream (one event pair around the colour phases
         of each sweep over `--event-steps` more MCS of the same run, launched
-        one by one), against 8 TB/s; traffic = HBM bytes per launch from the
-        committed PMC summary under profiles/.
+        one by one), against 8 TB/s; traffic = HBM bytes per launch at the
+        shape the run keeps, from the committed PMC summary under profiles/.
 pt, pt_trips = the tempering ladder comparisons (linear vs learned).
 cpu_baseline = the CPU oracle's FP64 sweep (+OVRR) of one 32^3 chain on one
         host core, for a bounded sample of about 10 s.

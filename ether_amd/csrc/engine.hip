@@ -736,7 +736,11 @@ extern "C" int ether_commit(ether_ctx* c) {
     long long row_entries = 0;
     for (int s = 0; s < N; ++s) row_entries += ell_m_n[s];
     const long long units = row_entries > 4LL * N ? 4096 : 16384;
-    c->meas_sites = kWavesPerBlock * std::max(8, (int)(((long long)N * G + units - 1) / units));
+    // Ising: a lane holds four replicas, so a group is 256 replicas and there
+    // are fewer wave units per site; waves walk from 2 sites up
+    const int groups = ising ? (c->Rp + 4 * kWave - 1) / (4 * kWave) : G;
+    const int min_sites = ising ? 2 : 8;
+    c->meas_sites = kWavesPerBlock * std::max(min_sites, (int)(((long long)N * groups + units - 1) / units));
     c->meas_tiles = (N + c->meas_sites - 1) / c->meas_sites;
     c->meas_rec = ising ? (c->n_classes + nsp + 1) : (c->spiral_cap ? 9 : 6);
     // four zero entries past the last row: a 4-wide chunk of the last row stays in the table

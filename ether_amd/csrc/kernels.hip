@@ -1101,6 +1101,19 @@ __device__ __forceinline__ void block_record(double* buf /* [kWavesPerBlock][kWa
     __syncthreads();
 }
 
+// block_record for lanes that may hold no replica (valid false: nothing written)
+__device__ __forceinline__ void block_record_if(double* buf, double v, bool valid, double* out_k) {
+    const int w = wave_id(), lane = lane_id();
+    buf[w * kWave + lane] = v;
+    __syncthreads();
+    if (w == 0 && valid) {
+        double t = buf[lane];
+        for (int q = 1; q < kWavesPerBlock; ++q) t += buf[q * kWave + lane];
+        *out_k = t;
+    }
+    __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // measurement partials: workgroup = (tile of sites, replica group of 64), the
 // tile split between its waves and their sums added in the workgroup
@@ -1233,58 +1246,105 @@ __global__ __launch_bounds__(kBlock) void k_xyz_measure_batch(const MeasArgs* __
     xyz_measure_block<ZEE, SIA>(a, it.y);
 }
 
+// Ising measurement with four replicas per lane (one 32-bit word of int8
+// spins, as the sweep holds them): workgroup = (tile of sites, group of 256
+// replicas).  Per site and coupling class the bonds whose two spins differ
+// are counted per byte (SWAR: ((s_j ^ s_i) >> 1) & 0x01010101 is 1 per byte
+// where they differ), then added into 16-bit fields (even and odd bytes
+// apart); a class's bond sum is then (its bonds) - 2 (the differing ones).
+// The spin sum per species comes from the down spins the same way.  All
+// integers, exact in any order.
 __device__ __forceinline__ void ising_measure_block(const MeasArgs& a, int blk) {
     __shared__ double sRed[kWavesPerBlock * kWave];
-    const int g = blk % a.G, tile = blk / a.G;             // workgroup-uniform
-    const int r = g * kWave + lane_id();
+    const int G4 = (a.Rp + 4 * kWave - 1) / (4 * kWave);
+    const int g4 = blk % G4, tile = blk / G4;             // workgroup-uniform
+    const int r0 = 4 * (g4 * kWave + lane_id());
+    const bool lv = r0 < a.Rp;
     const ConstF32 wtab = (ConstF32)a.w;
     const ConstU8 sptab = (ConstU8)a.sp;
-    double bonds[kMaxIsingClasses + 1];
-    double zs[kMaxIsingClasses];
+    const ConstRow rown = const_row(a.ell_m_n);
+    uint32_t de[kMaxIsingClasses + 1], dodd[kMaxIsingClasses + 1];   // differing bonds
+    int nbond[kMaxIsingClasses + 1];                                 // bonds (wave-uniform)
+    uint32_t ze[kMaxIsingClasses], zodd[kMaxIsingClasses];           // down spins
+    int nsite[kMaxIsingClasses];                                     // sites (wave-uniform)
 #pragma unroll
-    for (int k = 0; k <= kMaxIsingClasses; ++k) bonds[k] = 0.0;
+    for (int k = 0; k <= kMaxIsingClasses; ++k) {
+        de[k] = dodd[k] = 0u;
+        nbond[k] = 0;
+    }
 #pragma unroll
-    for (int k = 0; k < kMaxIsingClasses; ++k) zs[k] = 0.0;
-    double mz = 0.0;
+    for (int k = 0; k < kMaxIsingClasses; ++k) {
+        ze[k] = zodd[k] = 0u;
+        nsite[k] = 0;
+    }
+    double mz[4] = {0.0, 0.0, 0.0, 0.0};
     const int per_wave = (a.tile_sites + kWavesPerBlock - 1) / kWavesPerBlock;
     const int s_begin = tile * a.tile_sites + wave_id() * per_wave;
     const int s_end = min(min(a.N, (tile + 1) * a.tile_sites), s_begin + per_wave);
     for (int slot = s_begin; slot < s_end; ++slot) {
-        const int si = a.s[(size_t)slot * a.Rp + r];
+        const uint32_t v = lv ? *reinterpret_cast<const uint32_t*>(a.s + (size_t)slot * a.Rp + r0) : 0u;
         const ConstRow nb = const_row(a.ell_m + (size_t)slot * a.D_m);   // wave-uniform row
-        const int n = (int)const_row(a.ell_m_n)[slot];
+        const int n = (int)rown[slot];
+        uint32_t d[kMaxIsingClasses + 1];                                // this site's, per byte
+#pragma unroll
+        for (int k = 0; k <= kMaxIsingClasses; ++k) d[k] = 0u;
         for (int k0 = 0; k0 < n; k0 += 4) {
-            int sj[4], cl[4];
+            uint32_t sj[4];
+            int cl[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 const uint32_t e = (k0 + k < n) ? nb[k0 + k] : 0u;   // 0: class 0, no bond
                 cl[k] = (int)(e >> 24);
-                sj[k] = a.s[(size_t)(e & 0xffffffu) * a.Rp + r];
+                sj[k] = lv ? *reinterpret_cast<const uint32_t*>(a.s + (size_t)(e & 0xffffffu) * a.Rp + r0) : 0u;
             }
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
+            for (int k = 0; k < 4; ++k) {
+                class_add(d, cl[k], ((sj[k] ^ v) >> 1) & 0x01010101u);
 #pragma unroll
                 for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
-                    if (kk == cl[k]) bonds[kk] += (double)(si * sj[k]);
+                    if (kk == cl[k]) nbond[kk] += 1;
+            }
+        }
+#pragma unroll
+        for (int kk = 1; kk <= kMaxIsingClasses; ++kk) {
+            de[kk] += d[kk] & 0x00ff00ffu;
+            dodd[kk] += (d[kk] >> 8) & 0x00ff00ffu;
         }
         const int sp = (int)sptab[slot];
+        const uint32_t down = (v >> 1) & 0x01010101u;
 #pragma unroll
         for (int kk = 0; kk < kMaxIsingClasses; ++kk)
-            if (kk == sp) zs[kk] += (double)si;
-        mz += (double)(wtab[slot] * (float)si);
+            if (kk == sp) {
+                ze[kk] += down & 0x00ff00ffu;
+                zodd[kk] += (down >> 8) & 0x00ff00ffu;
+                nsite[kk] += 1;
+            }
+        const float w = wtab[slot];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) mz[b] += (double)(((down >> (8 * b)) & 1u) ? -w : w);
     }
-    // each bond was counted once, from its endpoint with the smaller slot
+    // per replica b of the lane: 16-bit field b/2 of the even (b even) or odd word
+    auto field = [](uint32_t even, uint32_t odd, int b) -> int {
+        const uint32_t w = (b & 1) ? odd : even;
+        return (int)((b & 2) ? (w >> 16) : (w & 0xffffu));
+    };
+    // each bond was counted once, from its endpoint with the smaller site index
+    // (bond_factor 2), or on full rows (1)
 #pragma unroll
-    for (int k = 1; k <= kMaxIsingClasses; ++k) bonds[k] *= (double)a.bond_factor;
-    double* out = a.partial + ((size_t)tile * a.Rp + r) * a.rec;
-    int o = 0;
+    for (int b = 0; b < 4; ++b) {
+        double* out = a.partial + ((size_t)tile * a.Rp + r0 + b) * a.rec;
+        int o = 0;
 #pragma unroll
-    for (int k = 1; k <= kMaxIsingClasses; ++k)
-        if (k <= a.n_classes) block_record(sRed, bonds[k], out + o++);
+        for (int k = 1; k <= kMaxIsingClasses; ++k)
+            if (k <= a.n_classes)
+                block_record_if(sRed, (double)((nbond[k] - 2 * field(de[k], dodd[k], b)) * a.bond_factor),
+                                lv, out + o++);
 #pragma unroll
-    for (int k = 0; k < kMaxIsingClasses; ++k)
-        if (k < a.nsp) block_record(sRed, zs[k], out + o++);
-    block_record(sRed, mz, out + o);
+        for (int k = 0; k < kMaxIsingClasses; ++k)
+            if (k < a.nsp)
+                block_record_if(sRed, (double)(nsite[k] - 2 * field(ze[k], zodd[k], b)), lv, out + o++);
+        block_record_if(sRed, mz[b], lv, out + o);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_ising_measure(MeasArgs a) {
@@ -1691,7 +1751,8 @@ void launch_ising_sweep(const IsingArgs& a, hipStream_t st) {
 }
 
 void launch_measure(const MeasArgs& a, int model, bool zee, bool sia, hipStream_t st) {
-    const long long blocks = (long long)a.tiles * a.G;    // one workgroup per (tile, replica group)
+    // one workgroup per (tile, replica group): 64 replicas (XYZ), 256 (Ising, four per lane)
+    const long long blocks = (long long)a.tiles * (model == 0 ? (a.Rp + 4 * kWave - 1) / (4 * kWave) : a.G);
     if (g_recorder) {
         record(LK_MEASURE, (model << 2) | ((zee ? 1 : 0) << 1) | (sia ? 1 : 0), (int)blocks, a);
         return;

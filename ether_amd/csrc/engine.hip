@@ -736,10 +736,14 @@ extern "C" int ether_commit(ether_ctx* c) {
     long long row_entries = 0;
     for (int s = 0; s < N; ++s) row_entries += ell_m_n[s];
     const long long units = row_entries > 4LL * N ? 4096 : 16384;
-    // Ising: a lane holds four replicas, so a group is 256 replicas and there
-    // are fewer wave units per site; waves walk from 2 sites up
-    const int groups = ising ? (c->Rp + 4 * kWave - 1) / (4 * kWave) : G;
-    const int min_sites = ising ? 2 : 8;
+    // Ising with 128 or more replicas: a lane holds four replicas, so a group
+    // is 256 replicas and there are fewer wave units per site; waves walk
+    // from 2 sites up (16^3 cells, 190 replicas: 11.2 us per pass against
+    // 18.8 us with one replica per lane).  Fewer replicas keep one per lane
+    // and walks from 8 sites, as XYZ.
+    const bool wide = ising && c->Rp >= 2 * kWave;     // the launcher's choice of Ising kernel
+    const int groups = wide ? (c->Rp + 4 * kWave - 1) / (4 * kWave) : G;
+    const int min_sites = wide ? 2 : 8;
     c->meas_sites = kWavesPerBlock * std::max(min_sites, (int)(((long long)N * groups + units - 1) / units));
     c->meas_tiles = (N + c->meas_sites - 1) / c->meas_sites;
     c->meas_rec = ising ? (c->n_classes + nsp + 1) : (c->spiral_cap ? 9 : 6);

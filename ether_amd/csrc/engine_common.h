@@ -258,7 +258,7 @@ enum LaunchKind : int {
 struct LaunchRecord {
     int kind;
     int variant;   // template instance: sweep (meas << 18) | (lanes << 17) | (angle << 16) | D;
-                   // measure (model << 2) | (zee << 1) | sia
+                   // measure (wide Ising << 3) | (model << 2) | (zee << 1) | sia
     int units;     // workgroups of the launch on its own, without XCD padding (0: nothing to do)
     int bytes;     // size of the argument block
     union {

@@ -1101,15 +1101,23 @@ __device__ __forceinline__ void block_record(double* buf /* [kWavesPerBlock][kWa
     __syncthreads();
 }
 
-// block_record for lanes that may hold no replica (valid false: nothing written)
-__device__ __forceinline__ void block_record_if(double* buf, double v, bool valid, double* out_k) {
+// block_record of the four replicas of a lane at once (Ising, four per
+// lane): out_k[b * stride] gets replica b's sum; lanes that hold no replica
+// (valid false) write nothing
+__device__ __forceinline__ void block_record4(double* buf /* [kWavesPerBlock][kWave][4] */,
+                                              const double (&v)[4], bool valid, double* out_k,
+                                              int stride) {
     const int w = wave_id(), lane = lane_id();
-    buf[w * kWave + lane] = v;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) buf[(w * kWave + lane) * 4 + b] = v[b];
     __syncthreads();
     if (w == 0 && valid) {
-        double t = buf[lane];
-        for (int q = 1; q < kWavesPerBlock; ++q) t += buf[q * kWave + lane];
-        *out_k = t;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            double t = buf[lane * 4 + b];
+            for (int q = 1; q < kWavesPerBlock; ++q) t += buf[(q * kWave + lane) * 4 + b];
+            out_k[b * stride] = t;
+        }
     }
     __syncthreads();
 }
@@ -1255,7 +1263,7 @@ __global__ __launch_bounds__(kBlock) void k_xyz_measure_batch(const MeasArgs* __
 // The spin sum per species comes from the down spins the same way.  All
 // integers, exact in any order.
 __device__ __forceinline__ void ising_measure_block(const MeasArgs& a, int blk) {
-    __shared__ double sRed[kWavesPerBlock * kWave];
+    __shared__ double sRed[kWavesPerBlock * kWave * 4];
     const int G4 = (a.Rp + 4 * kWave - 1) / (4 * kWave);
     const int g4 = blk % G4, tile = blk / G4;             // workgroup-uniform
     const int r0 = 4 * (g4 * kWave + lane_id());
@@ -1330,25 +1338,90 @@ __device__ __forceinline__ void ising_measure_block(const MeasArgs& a, int blk) 
     };
     // each bond was counted once, from its endpoint with the smaller site index
     // (bond_factor 2), or on full rows (1)
+    double* out = a.partial + ((size_t)tile * a.Rp + r0) * a.rec;
+    int o = 0;
+    double t[4];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        double* out = a.partial + ((size_t)tile * a.Rp + r0 + b) * a.rec;
-        int o = 0;
+    for (int k = 1; k <= kMaxIsingClasses; ++k)
+        if (k <= a.n_classes) {
 #pragma unroll
-        for (int k = 1; k <= kMaxIsingClasses; ++k)
-            if (k <= a.n_classes)
-                block_record_if(sRed, (double)((nbond[k] - 2 * field(de[k], dodd[k], b)) * a.bond_factor),
-                                lv, out + o++);
+            for (int b = 0; b < 4; ++b)
+                t[b] = (double)((nbond[k] - 2 * field(de[k], dodd[k], b)) * a.bond_factor);
+            block_record4(sRed, t, lv, out + o++, a.rec);
+        }
 #pragma unroll
-        for (int k = 0; k < kMaxIsingClasses; ++k)
-            if (k < a.nsp)
-                block_record_if(sRed, (double)(nsite[k] - 2 * field(ze[k], zodd[k], b)), lv, out + o++);
-        block_record_if(sRed, mz[b], lv, out + o);
+    for (int k = 0; k < kMaxIsingClasses; ++k)
+        if (k < a.nsp) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) t[b] = (double)(nsite[k] - 2 * field(ze[k], zodd[k], b));
+            block_record4(sRed, t, lv, out + o++, a.rec);
+        }
+    block_record4(sRed, mz, lv, out + o, a.rec);
+}
+
+// Ising measurement with one replica per lane (int8 loads): for fewer than
+// 128 replicas, where a four-replica lane layout would leave most lanes idle
+// (a batch of small lattices ran 5% slower with it).
+__device__ __forceinline__ void ising_measure_narrow_block(const MeasArgs& a, int blk) {
+    __shared__ double sRed[kWavesPerBlock * kWave];
+    const int g = blk % a.G, tile = blk / a.G;             // workgroup-uniform
+    const int r = g * kWave + lane_id();
+    const ConstF32 wtab = (ConstF32)a.w;
+    const ConstU8 sptab = (ConstU8)a.sp;
+    double bonds[kMaxIsingClasses + 1];
+    double zs[kMaxIsingClasses];
+#pragma unroll
+    for (int k = 0; k <= kMaxIsingClasses; ++k) bonds[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < kMaxIsingClasses; ++k) zs[k] = 0.0;
+    double mz = 0.0;
+    const int per_wave = (a.tile_sites + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int s_begin = tile * a.tile_sites + wave_id() * per_wave;
+    const int s_end = min(min(a.N, (tile + 1) * a.tile_sites), s_begin + per_wave);
+    for (int slot = s_begin; slot < s_end; ++slot) {
+        const int si = a.s[(size_t)slot * a.Rp + r];
+        const ConstRow nb = const_row(a.ell_m + (size_t)slot * a.D_m);   // wave-uniform row
+        const int n = (int)const_row(a.ell_m_n)[slot];
+        for (int k0 = 0; k0 < n; k0 += 4) {
+            int sj[4], cl[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t e = (k0 + k < n) ? nb[k0 + k] : 0u;   // 0: class 0, no bond
+                cl[k] = (int)(e >> 24);
+                sj[k] = a.s[(size_t)(e & 0xffffffu) * a.Rp + r];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
+                    if (kk == cl[k]) bonds[kk] += (double)(si * sj[k]);
+        }
+        const int sp = (int)sptab[slot];
+#pragma unroll
+        for (int kk = 0; kk < kMaxIsingClasses; ++kk)
+            if (kk == sp) zs[kk] += (double)si;
+        mz += (double)(wtab[slot] * (float)si);
     }
+    // each bond was counted once, from its endpoint with the smaller slot
+#pragma unroll
+    for (int k = 1; k <= kMaxIsingClasses; ++k) bonds[k] *= (double)a.bond_factor;
+    double* out = a.partial + ((size_t)tile * a.Rp + r) * a.rec;
+    int o = 0;
+#pragma unroll
+    for (int k = 1; k <= kMaxIsingClasses; ++k)
+        if (k <= a.n_classes) block_record(sRed, bonds[k], out + o++);
+#pragma unroll
+    for (int k = 0; k < kMaxIsingClasses; ++k)
+        if (k < a.nsp) block_record(sRed, zs[k], out + o++);
+    block_record(sRed, mz, out + o);
 }
 
 __global__ __launch_bounds__(kBlock) void k_ising_measure(MeasArgs a) {
     ising_measure_block(a, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ising_measure_narrow(MeasArgs a) {
+    ising_measure_narrow_block(a, blockIdx.x);
 }
 
 __global__ __launch_bounds__(kBlock) void k_ising_measure_batch(const MeasArgs* __restrict__ args,
@@ -1357,6 +1430,14 @@ __global__ __launch_bounds__(kBlock) void k_ising_measure_batch(const MeasArgs* 
     if (it.x < 0) return;
     const MeasArgs a = args[it.x];
     ising_measure_block(a, it.y);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ising_measure_narrow_batch(const MeasArgs* __restrict__ args,
+                                                               const int2* __restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    if (it.x < 0) return;
+    const MeasArgs a = args[it.x];
+    ising_measure_narrow_block(a, it.y);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -1751,15 +1832,18 @@ void launch_ising_sweep(const IsingArgs& a, hipStream_t st) {
 }
 
 void launch_measure(const MeasArgs& a, int model, bool zee, bool sia, hipStream_t st) {
-    // one workgroup per (tile, replica group): 64 replicas (XYZ), 256 (Ising, four per lane)
-    const long long blocks = (long long)a.tiles * (model == 0 ? (a.Rp + 4 * kWave - 1) / (4 * kWave) : a.G);
+    // one workgroup per (tile, replica group): 64 replicas (XYZ, and Ising
+    // below 128 replicas), 256 (Ising, four per lane)
+    const bool wide = model == 0 && a.Rp >= 2 * kWave;
+    const long long blocks = (long long)a.tiles * (wide ? (a.Rp + 4 * kWave - 1) / (4 * kWave) : a.G);
     if (g_recorder) {
-        record(LK_MEASURE, (model << 2) | ((zee ? 1 : 0) << 1) | (sia ? 1 : 0), (int)blocks, a);
+        record(LK_MEASURE, (wide ? 8 : 0) | (model << 2) | ((zee ? 1 : 0) << 1) | (sia ? 1 : 0), (int)blocks, a);
         return;
     }
     dim3 grid((unsigned)blocks), block(kBlock);
     if (model == 0) {
-        hipLaunchKernelGGL(k_ising_measure, grid, block, 0, st, a);
+        if (wide) hipLaunchKernelGGL(k_ising_measure, grid, block, 0, st, a);
+        else hipLaunchKernelGGL(k_ising_measure_narrow, grid, block, 0, st, a);
         return;
     }
     if (zee) {
@@ -1847,6 +1931,8 @@ void launch_batch(int kind, int variant, const void* d_args, const int2* d_items
         const MeasArgs* a = static_cast<const MeasArgs*>(d_args);
         switch (variant) {
         case 0: case 1: case 2: case 3:
+            hipLaunchKernelGGL(k_ising_measure_narrow_batch, grid, block, 0, st, a, d_items); break;
+        case 8: case 9: case 10: case 11:
             hipLaunchKernelGGL(k_ising_measure_batch, grid, block, 0, st, a, d_items); break;
         case 4: hipLaunchKernelGGL((k_xyz_measure_batch<false, false>), grid, block, 0, st, a, d_items); break;
         case 5: hipLaunchKernelGGL((k_xyz_measure_batch<false, true>), grid, block, 0, st, a, d_items); break;

@@ -44,11 +44,12 @@ def _ising_case(name, R, K, **over):
             acc += orc.sweep(S, None, order, rid, betas[r], m.tags.seed, r, step)
         Sg, _ = eng.get_spins(r)
         assert np.array_equal(Sg[:, 2], S[:, 2]), f"replica {r}: {np.sum(Sg[:, 2] != S[:, 2])} spins differ"
-    # energies: exact integer bond sums per coupling class
-    eng.measure()
+    # energies: exact integer bond sums per coupling class; magnetisation too
+    _, M = eng.measure()
     parts, bonds = eng.energy_parts()
     for r in range(R):
         S = states[r]
+        assert np.allclose(M[r], orc.magnetisation(S), rtol=0, atol=1e-9)
         ex, ze, si = orc.total_energy(S)
         assert parts[r, 0] == pytest.approx(ex, rel=1e-12, abs=1e-12)
         assert parts[r, 1] == pytest.approx(ze, rel=1e-12, abs=1e-12)
@@ -69,6 +70,16 @@ def test_ising_sc_mev_units_bit_exact():
 
 def test_ising_pyrochlore_two_shells_zeeman_bit_exact():
     _ising_case("pyro_pt_16", 4, 4, sc="2 2 2", model="ising", para=".T. 1", pt=".F. 11",
+                zeeman=".T. 0 0 0.3")
+
+
+# 128 replicas or more: the measurement pass holds four replicas per lane
+def test_ising_sc_many_replicas_bit_exact():
+    _ising_case("sc_ising_16", 150, 2, sc="6 6 6")
+
+
+def test_ising_pyrochlore_many_replicas_bit_exact():
+    _ising_case("pyro_pt_16", 130, 2, sc="2 2 2", model="ising", para=".T. 1", pt=".F. 11",
                 zeeman=".T. 0 0 0.3")
 
 

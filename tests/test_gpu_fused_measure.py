@@ -178,3 +178,37 @@ def test_unmirrored_bonds_measure_apart(edit):
         scale = sum(abs(orc.site_energy(S, i)) for i in range(m.n_sites))
         assert abs(parts[r, 0] - ex) <= 1e-6 * scale
     eng.close()
+
+
+def test_swap_step_books_the_energies_it_swaps_on():
+    """A swap step's sweep books the measurement the swap reads (E per
+    replica): the energies equal the pass's on the state after the step, and
+    the swap statistics equal those of the same run with the pass."""
+    say("fused measurement on swap steps")
+    runs = []
+    for compat in (0, E.COMPAT_MEASURE_PASS):
+        m, eng, orc = pair("sc_heis_afm_32", 24, sc="6 6 6")
+        eng.set_compat(compat)
+        eng.set_betas(m.betas(np.geomspace(30.0, 3.0, 24)))
+        eng.set_ladders([0, 24])
+        _start(m, eng, 24, 9)
+        eng.set_timing(True)
+        eng.run(1, 4, 100, 1000, 1, 1.0, 2)            # swaps at steps 2 and 4, no samples
+        booked = eng.timing(E.TIME_SWEEP_MEAS)[1]
+        eng.set_timing(False)
+        parts, _ = eng.energy_parts()
+        slot, acc, att, _ = eng.pt_state()
+        E_s, _ = eng.measure()
+        parts_s, _ = eng.energy_parts()
+        for r in range(24):
+            S, _ = eng.get_spins(r)
+            scale = sum(abs(orc.site_energy(S, i)) for i in range(m.n_sites))
+            assert abs(parts[r, 0] - parts_s[r, 0]) <= 1e-6 * scale
+        runs.append((booked, slot.copy(), acc.copy(), att.copy()))
+        eng.close()
+    (b0, s0, a0, t0), (b1, s1, a1, t1) = runs
+    assert b0 == 2 * 2 and b1 == 0                      # two swap steps x two colours
+    assert np.array_equal(t0, t1) and t0.sum() > 0
+    # the booked and the passed energies differ by FP32 rounding only, which
+    # decides no swap here
+    assert np.array_equal(s0, s1) and np.array_equal(a0, a1)

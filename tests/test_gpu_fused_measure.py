@@ -212,3 +212,46 @@ def test_swap_step_books_the_energies_it_swaps_on():
     # the booked and the passed energies differ by FP32 rounding only, which
     # decides no swap here
     assert np.array_equal(s0, s1) and np.array_equal(a0, a1)
+
+
+def test_booked_measurement_at_full_size():
+    """The 32 x 32 x 32 antiferromagnet with 190 replicas (three waves of
+    replicas per site, OVRR fused): the booked energies equal the pass's for
+    every replica, and the oracle's FP64 sums on three of them.  With unit
+    spins and one coupling, |e_i| <= sum_j |J_ij|, the site energy of the
+    aligned state, so that state's sum of |site energies| bounds every
+    state's."""
+    say("fused measurement sc_heis_afm_32 32^3 R=190")
+    R = 190
+    m, eng, orc = pair("sc_heis_afm_32", R)
+    N = m.n_sites
+    assert N == 32 ** 3
+    eng.set_betas(m.betas(np.linspace(5.0, 50.0, R)))
+    _start(m, eng, R, 17)
+    eng.set_timing(True)
+    eng.run(1, 3, 0, 3, 1, 1.0)
+    nc = eng.layout()[0]
+    assert nc == 2 and eng.timing(E.TIME_SWEEP_MEAS)[1] == nc
+    eng.set_timing(False)
+    parts_f, _ = eng.energy_parts()
+    mom_f = eng.moments().copy()
+    slot = eng.pt_state()[0]
+    eng.reset_moments()
+    eng.measure(sample=True)
+    parts_s, _ = eng.energy_parts()
+    mom_s = eng.moments()
+    up = np.zeros((N, 3))
+    up[:, 2] = 1.0
+    bound = sum(abs(orc.site_energy(up, i)) for i in range(N))
+    assert bound > 0
+    assert np.all(np.abs(parts_f[:, 0] - parts_s[:, 0]) <= 1e-6 * bound)
+    assert np.all(np.abs(mom_f[slot, E.MOM_M] - mom_s[slot, E.MOM_M]) <= 1e-6)
+    assert np.all(mom_f[slot, E.MOM_N] == 1.0)
+    for r in (0, 95, R - 1):
+        S, _ = eng.get_spins(r)
+        ex, _, _ = orc.total_energy(S)
+        scale = sum(abs(orc.site_energy(S, i)) for i in range(N))
+        assert scale <= bound * (1 + 1e-12)
+        assert abs(parts_f[r, 0] - ex) <= 1e-6 * scale, (r, parts_f[r, 0], ex)
+        assert abs(mom_f[slot[r], E.MOM_M] - np.linalg.norm(orc.magnetisation(S)) / N) <= 1e-6
+    eng.close()

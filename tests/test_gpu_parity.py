@@ -349,3 +349,37 @@ def test_pt_swaps_match_oracle():
         assert np.array_equal(slot_of, expect), f"call {call}"
     assert 0 < total < 24 * 4
     eng.close()
+
+
+def test_pyrochlore_at_full_size_with_swaps():
+    """16^3 pyrochlore cells (65536 spins, four colours), four ladders of 64
+    replicas, OVRR every MCS and swaps every 11 MCS over 22 MCS: each
+    ladder's slots stay a permutation of its own range, swaps were tried in
+    every ladder, spins stay unit vectors, and the measurement pass's
+    exchange energy matches the oracle's FP64 sum on three replicas."""
+    say("pyrochlore 16^3 x 16, 4 x 64 replicas, swaps every 11 MCS")
+    R, L = 256, 64
+    m, eng, orc = pair("pyro_pt_16", R)
+    N = m.n_sites
+    assert N == 16 ** 3 * 16
+    eng.set_betas(m.betas(np.tile(np.geomspace(1.0, 0.05, L), R // L)))
+    eng.set_ladders(list(range(0, R + 1, L)))
+    rng = np.random.default_rng(23)
+    for r in range(R):
+        S = random_xyz(rng, N)
+        eng.set_spins(r, S, np.arctan2(S[:, 1], S[:, 0]))
+    eng.run(1, 22, 1000, 1000, 1, 1.0, 11)
+    slot, acc, att, _ = eng.pt_state()
+    for k in range(R // L):
+        assert np.array_equal(np.sort(slot[k * L:(k + 1) * L]), np.arange(k * L, (k + 1) * L))
+        assert att[k * L:(k + 1) * L].sum() > 0
+    assert np.all(acc <= att)
+    eng.measure()
+    parts, _ = eng.energy_parts()
+    for r in (0, 131, R - 1):
+        S, _ = eng.get_spins(r)
+        assert np.allclose(np.linalg.norm(S, axis=1), 1.0, atol=1e-5)
+        ex, _, _ = orc.total_energy(S)
+        scale = sum(abs(orc.site_energy(S, i)) for i in range(N))
+        assert abs(parts[r, 0] - ex) <= 1e-6 * scale, (r, parts[r, 0], ex)
+    eng.close()

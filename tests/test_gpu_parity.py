@@ -78,6 +78,12 @@ def test_ising_sc_many_replicas_bit_exact():
     _ising_case("sc_ising_16", 150, 2, sc="6 6 6")
 
 
+def test_ising_sc_full_size_bit_exact():
+    # 16^3 sites, 190 replicas: three waves of replicas per site, four per lane
+    # in the measurement
+    _ising_case("sc_ising_16", 190, 2)
+
+
 def test_ising_pyrochlore_many_replicas_bit_exact():
     _ising_case("pyro_pt_16", 130, 2, sc="2 2 2", model="ising", para=".T. 1", pt=".F. 11",
                 zeeman=".T. 0 0 0.3")

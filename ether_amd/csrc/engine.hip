@@ -654,6 +654,12 @@ extern "C" int ether_commit(ether_ctx* c) {
         }
         c->ising_P = (int)P;
         c->ising_table = (int)(P * 2 * nsp);
+        // the sweep counts a site's down neighbours per class in one byte per
+        // replica (SWAR), the measurement its differing bonds per row likewise
+        for (int k = 1; k <= K; ++k)
+            if (c->ising_Z[k] > 255)
+                return fail(c, ETHER_ERR_LIMIT, "Ising path supports at most 255 neighbours of one "
+                                                "coupling class per site");
     }
     // ---- per-slot species / magnetisation weights ----
     std::vector<uint8_t> sp(N);
@@ -745,6 +751,15 @@ extern "C" int ether_commit(ether_ctx* c) {
     const int groups = wide ? (c->Rp + 4 * kWave - 1) / (4 * kWave) : G;
     const int min_sites = wide ? 2 : 8;
     c->meas_sites = kWavesPerBlock * std::max(min_sites, (int)(((long long)N * groups + units - 1) / units));
+    if (wide) {
+        // a wave adds its sites' per-byte counts of differing bonds into 16-bit
+        // fields: at most 65535 / D_m sites per wave, and at most 255 entries
+        // in one measurement row
+        if (c->D_m > 255)
+            return fail(c, ETHER_ERR_LIMIT, "Ising path supports at most 255 neighbours per site");
+        const int max_per_wave = std::max(1, 65535 / std::max(c->D_m, 1));
+        c->meas_sites = std::min(c->meas_sites, kWavesPerBlock * max_per_wave);
+    }
     c->meas_tiles = (N + c->meas_sites - 1) / c->meas_sites;
     c->meas_rec = ising ? (c->n_classes + nsp + 1) : (c->spiral_cap ? 9 : 6);
     // four zero entries past the last row: a 4-wide chunk of the last row stays in the table

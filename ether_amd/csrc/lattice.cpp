@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/ether_gpu.h"
+#include "philox.h"
 
 namespace ether {
 thread_local std::string g_last_error;
@@ -266,5 +267,63 @@ extern "C" int ether_build_spiral_partners(int32_t n_sites, const double* pos, c
         }
     }
     if (count) *count = total;
+    return ETHER_OK;
+}
+
+// ---------------------------------------------------------------------------
+// start state of the basis sites (src/generate_supercell.f90:34-63)
+// ---------------------------------------------------------------------------
+extern "C" int ether_start_state(uint64_t seed, int32_t model, int32_t angle, int32_t n_basis,
+                                 const int32_t* file_index, double* S, double* phi) {
+    if (n_basis <= 0 || !file_index || !S || !phi) {
+        ether::set_global_error("ether_start_state: bad argument");
+        return ETHER_ERR_ARG;
+    }
+    constexpr uint32_t kRngStart = 5;
+    for (int32_t l = 0; l < n_basis; ++l) {
+        double* s = S + 3 * (size_t)l;
+        double p = 0.0;
+        if (model == ETHER_MODEL_ISING) {
+            // atom(i,3) = (-1)**i
+            s[0] = 0.0;
+            s[1] = 0.0;
+            s[2] = (file_index[l] % 2 == 0) ? 1.0 : -1.0;
+        } else if (!angle) {
+            // George_Marsaglia: (v1, v2) uniform in the unit disc, by rejection
+            // on two 24-bit candidate pairs per Philox block
+            bool done = false;
+            for (uint32_t att = 0; !done; ++att) {
+                const ether::u32x4 w = ether::rng_draw(seed, (uint32_t)l, 0, 0, kRngStart, att);
+                const uint32_t cx[2] = {w.x >> 8, ((w.w >> 16) << 8) | (w.x & 0xffu)};
+                const uint32_t cy[2] = {w.y >> 8, ((w.w & 0xffffu) << 8) | (w.y & 0xffu)};
+                for (int c = 0; c < 2 && !done; ++c) {
+                    const int64_t a1 = (int64_t)cx[c] - (1 << 23), a2 = (int64_t)cy[c] - (1 << 23);
+                    const int64_t q = a1 * a1 + a2 * a2;
+                    if (q >= ((int64_t)1 << 46)) continue;
+                    const double v1 = a1 * 0x1p-23, v2 = a2 * 0x1p-23, sq = q * 0x1p-46;
+                    const double r = std::sqrt(1.0 - sq);
+                    s[0] = 2.0 * v1 * r;
+                    s[1] = 2.0 * v2 * r;
+                    s[2] = 1.0 - 2.0 * sq;
+                    done = true;
+                }
+            }
+        } else {
+            const ether::u32x4 w = ether::rng_draw(seed, (uint32_t)l, 0, 0, kRngStart, 0);
+            const double sz = (double)((int32_t)(w.x >> 8) - (1 << 23)) * 0x1p-23;
+            p = 6.283185307179586476925286766559 * (double)(w.y >> 8) * 0x1p-24;
+            const double r = std::sqrt(1.0 - sz * sz);
+            s[0] = r * std::cos(p);
+            s[1] = r * std::sin(p);
+            s[2] = sz;
+        }
+        if (model != ETHER_MODEL_ISING) {
+            const double n = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+            s[0] /= n;
+            s[1] /= n;
+            s[2] /= n;
+        }
+        phi[l] = p;
+    }
     return ETHER_OK;
 }

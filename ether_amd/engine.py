@@ -47,6 +47,7 @@ EXPORTS = [
      [_vp, C.c_int32, _vp, _vp, _vp, C.c_int32, _vp, C.c_double, C.c_int32, _vp, _vp]),
     ("ether_build_spiral_partners", C.c_int,
      [C.c_int32, _vp, _vp, C.c_double, C.c_double, C.c_int32, _vp, _vp]),
+    ("ether_start_state", C.c_int, [C.c_uint64, C.c_int32, C.c_int32, C.c_int32, _vp, _vp, _vp]),
     ("ether_ctx_create", C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(_vp)]),
     ("ether_ctx_destroy", None, [_vp]),
     ("ether_last_error", C.c_int, [_vp, C.c_char_p, C.c_int32]),
@@ -142,6 +143,17 @@ def _check(rc: int, ctx=None) -> None:
         buf = C.create_string_buffer(1024)
         lib.ether_last_error(ctx, buf, 1024)
         raise EtherError(rc, buf.value.decode(errors="replace"))
+
+
+def start_state(seed, model, angle, file_index):
+    """Start state of the basis sites (see ether_start_state): (S [nb, 3], phi [nb])."""
+    lib = load_library()
+    fi = _arr(file_index, np.int32)
+    S = np.zeros((fi.size, 3))
+    phi = np.zeros(fi.size)
+    _check(lib.ether_start_state(C.c_uint64(int(seed)), int(model), int(angle), fi.size, _ptr(fi),
+                                 _ptr(S), _ptr(phi)))
+    return S, phi
 
 
 def build_neighbours(abc, basis_xyz, sc, open_bc, shell_d, tol=1e-5, capacity=20):

@@ -328,6 +328,7 @@ program ether_gpu_driver
    integer, allocatable :: tions(:), inc(:)
    real(dp), allocatable :: sfin(:, :), phifin(:), ionrec(:, :), obs(:), mmv(:, :)
    real(dp), allocatable :: b_xyz(:, :), stagger(:), S0(:, :), phi0(:), shells(:), jx(:, :, :, :)
+   real(dp), allocatable :: start_s(:, :), start_phi(:)
    real(dp), allocatable :: siav(:, :), temps(:), betas(:), mom(:, :), per(:, :, :)
    real(dp), allocatable :: mean(:, :), sem(:, :), accp(:), stg_id(:)
    real(dp) :: kb, mub, gf, jv(3), ovrr_p, c1, r1
@@ -402,24 +403,21 @@ program ether_gpu_driver
       end do
    end do
    ! start state: Ising (-1)^i by file index; XYZ random per basis site
-   ! (src/generate_supercell.f90:34-63), copied to every cell
-   call random_seed()
+   ! (src/generate_supercell.f90:34-63), a function of SEED as the reference's
+   ! random_seed(put=seed+rank) makes it (src/ether.f90:63), drawn by the
+   ! library so that this driver and the Python one start alike; copied to
+   ! every cell
+   allocate (start_s(3, nb), start_phi(nb))
+   rc = ether_start_state(int(seed_value, c_int64_t), merge(ETHER_MODEL_ISING, ETHER_MODEL_XYZ, ising), &
+                          merge(1, 0, angle .and. .not. ising), nb, b_idx, start_s, start_phi)
+   if (rc /= ETHER_OK) call terminate(ether_message(c_null_ptr))
    do l = 1, nb
-      if (ising) then
-         jv = (/0.0_dp, 0.0_dp, real((-1)**b_idx(l), dp)/)
-         c1 = 0.0_dp
-      else
-         call random_number(jv)
-         jv(3) = 2.0_dp*jv(1) - 1.0_dp
-         c1 = 2.0_dp*pi*jv(2)
-         jv(1) = sqrt(max(0.0_dp, 1.0_dp - jv(3)**2))*cos(c1)
-         jv(2) = sqrt(max(0.0_dp, 1.0_dp - jv(3)**2))*sin(c1)
-      end if
       do q = l, n, nb
-         S0(:, q) = jv
-         phi0(q) = c1
+         S0(:, q) = start_s(:, l)
+         phi0(q) = start_phi(l)
       end do
    end do
+   deallocate (start_s, start_phi)
 
    ! ---- j_exchange (src/j_values.f90:39-87): J * s_a * s_b, symmetric
    inquire (file='j_exchange', exist=ok)

@@ -25,6 +25,15 @@ module ether_gpu
          integer(c_int), intent(out) :: nn_count(*), nn_idx(*)
       end function
 
+      integer(c_int) function ether_start_state(seed, model, angle, n_basis, file_index, S, phi) &
+            bind(C, name="ether_start_state")
+         import :: c_int, c_int64_t, c_double
+         integer(c_int64_t), value :: seed
+         integer(c_int), value :: model, angle, n_basis
+         integer(c_int), intent(in) :: file_index(*)
+         real(c_double), intent(out) :: S(*), phi(*)
+      end function
+
       integer(c_int) function ether_ctx_create(model, n_sites, n_replicas, seed, ctx) &
             bind(C, name="ether_ctx_create")
          import :: c_int, c_int64_t, c_ptr

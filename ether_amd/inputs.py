@@ -331,8 +331,7 @@ class Model:
         return 1.0 / (self.kb * np.asarray(temps, np.float64))
 
 
-def build_model(tags: InEther, st: Structure, ex: Exchange, sia_lines=None, stg_lines=None,
-                rng: Optional[np.random.Generator] = None) -> Model:
+def build_model(tags: InEther, st: Structure, ex: Exchange, sia_lines=None, stg_lines=None) -> Model:
     """generate_supercell + j_values + get_sia_values + parameters."""
     nsp = st.n_species
     if tags.spin is None:
@@ -413,21 +412,11 @@ def build_model(tags: InEther, st: Structure, ex: Exchange, sia_lines=None, stg_
             sia_vec = sia_vec / 1000.0
         kb, mub, g = KB_EV, MUB_EV, tags.g_factor
     # start state (src/generate_supercell.f90:34-63): Ising (-1)^i by file index,
-    # XYZ random per basis site, copied to every cell
-    rng = rng or np.random.default_rng(tags.seed)
-    init = np.zeros((nb, 3))
-    phi0 = np.zeros(nb)
-    for l in range(nb):
-        if tags.ising:
-            init[l, 2] = (-1.0) ** int(b_idx[l])
-        elif not tags.angle:
-            v = rng.normal(size=3)
-            init[l] = v / np.linalg.norm(v)
-        else:
-            z = rng.uniform(-1, 1)
-            p = rng.uniform(0, 2 * np.pi)
-            init[l] = [math.sqrt(1 - z * z) * math.cos(p), math.sqrt(1 - z * z) * math.sin(p), z]
-            phi0[l] = p
+    # XYZ random per basis site (a pure function of SEED, drawn by the library
+    # so that every driver starts from the same state), copied to every cell
+    from .engine import start_state
+    init, phi0 = start_state(tags.seed, 0 if tags.ising else 1, 1 if (tags.angle and not tags.ising) else 0,
+                             b_idx)
     return Model(tags=tags, structure=st, n_species=nsp, basis_species=b_sp, basis_xyz=b_xyz,
                  basis_file_index=b_idx, sc=sc, n_sites=n, species=species.astype(np.int32),
                  stagger=stagger.astype(np.float64), positions=pos, shells=np.array(ex.shells),

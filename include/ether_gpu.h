@@ -116,6 +116,23 @@ int ether_build_spiral_partners(int32_t n_sites, const double* pos, const double
                                 int32_t* partners, int64_t* count);
 
 /* ---------------------------------------------------------------------------
+ * Start state of the basis sites: replaces the draw of generate_supercell
+ * (src/generate_supercell.f90:34-63), which the reference takes from the
+ * Fortran runtime's random_number after random_seed(put=seed+rank)
+ * (src/ether.f90:63), whose stream depends on the compiler.  Here it is a
+ * pure function of `seed`, so every driver that calls it gets the same start
+ * state: Ising S = (0, 0, (-1)^file_index[l]) with file_index the 1-based
+ * atom number in structure.vasp; XYZ Marsaglia (angle = 0) a uniform point
+ * of S^2 by George_Marsaglia's disc rule (src/George_Marsaglia.f90:29-39),
+ * phi = 0; ANGLE (angle = 1) S_z ~ U(-1,1), phi ~ U(0, 2 pi).  Draws use
+ * Philox4x32-10 with counter {l, 0, 0, 5 << 16 | attempt}.  S[3*n_basis]
+ * (x, y, z per basis site), phi[n_basis].  Host-only, context-free; the
+ * supercell copies basis site l to every cell (src/generate_supercell.f90:115-132).
+ * ------------------------------------------------------------------------- */
+int ether_start_state(uint64_t seed, int32_t model, int32_t angle, int32_t n_basis,
+                      const int32_t* file_index, double* S, double* phi);
+
+/* ---------------------------------------------------------------------------
  * Context life cycle
  * ------------------------------------------------------------------------- */
 

@@ -279,6 +279,36 @@ void o_propose(const o_model* m, uint64_t seed, uint32_t id, uint32_t replica, u
     *phi_new = phi;
 }
 
+/* generate_supercell start state (src/generate_supercell.f90:34-63): Ising
+ * atom(i,3) = (-1)**i by structure-file index; XYZ George_Marsaglia, or for
+ * ANGLE S_z = get_random_num(-1,1) and phi = get_random_num(0, 2 pi). */
+void o_start_state(uint64_t seed, int32_t model, int32_t angle, int32_t nb,
+                   const int32_t* file_index, double* S, double* phi) {
+    o_model m;
+    memset(&m, 0, sizeof m);
+    m.model = 1;
+    for (int32_t l = 0; l < nb; ++l) {
+        double* s = S + 3 * (size_t)l;
+        phi[l] = 0.0;
+        if (model == 0) {
+            s[0] = 0.0; s[1] = 0.0;
+            s[2] = (file_index[l] % 2 == 0) ? 1.0 : -1.0;
+        } else if (!angle) {
+            double old[3] = {0.0, 0.0, 1.0}, eta;
+            o_propose(&m, seed, (uint32_t)l, 0, 0, O_RNG_START, old, 0.0, s, &phi[l], &eta);
+        } else {
+            uint32_t w[4];
+            o_rng(seed, (uint32_t)l, 0, 0, O_RNG_START, 0, w);
+            double sz = sym24(w[0] >> 8);
+            double p = 6.283185307179586476925286766559 * (double)(w[1] >> 8) * 0x1p-24;
+            double r = sqrt(1.0 - sz * sz);
+            s[0] = r * cos(p); s[1] = r * sin(p); s[2] = sz;
+            normalise(s);
+            phi[l] = p;
+        }
+    }
+}
+
 uint32_t o_ising_threshold(double beta, double dE) {
     double p = exp(-beta * dE);
     if (!(p < 1.0)) return 0xffffffffu;

@@ -47,6 +47,7 @@ extern "C" {
 #define O_RNG_INIT       2
 #define O_RNG_OVRR       3
 #define O_RNG_PT         4
+#define O_RNG_START      5
 
 void o_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void o_rng(uint64_t seed, uint32_t id, uint32_t replica, uint32_t step, uint32_t purpose,
@@ -107,6 +108,14 @@ void o_propose(const o_model* m, uint64_t seed, uint32_t id, uint32_t replica, u
 void o_propose_site(const o_model* m, uint64_t seed, uint32_t id, int32_t R, uint32_t step,
                     const double* s_old, const double* phi_old, double* s_new, double* phi_new,
                     double* eta);
+
+/* Start state of the basis sites (src/generate_supercell.f90:34-63), drawn
+ * with the Philox contract for (seed, id = basis index l, replica 0, step 0,
+ * O_RNG_START): Ising (-1)^file_index[l]; Marsaglia: the point o_propose
+ * draws; ANGLE: S_z ~ U(-1,1), phi ~ U(0, 2 pi) from words 0 and 1.
+ * S [3*nb], phi [nb]. */
+void o_start_state(uint64_t seed, int32_t model, int32_t angle, int32_t nb,
+                   const int32_t* file_index, double* S, double* phi);
 
 /* Ising acceptance threshold: accept iff w < thr or thr == 0xffffffff.
  * The Ising word of replica r at a site is word r&3 of the Philox block with

@@ -18,7 +18,7 @@ _lib = None
 
 _vp = C.c_void_p
 NPARAM = 64 + 64 + 64 * 64 + 64 + 2 * 64 + 2
-RNG_METROPOLIS, RNG_INIT, RNG_OVRR, RNG_PT = 1, 2, 3, 4
+RNG_METROPOLIS, RNG_INIT, RNG_OVRR, RNG_PT, RNG_START = 1, 2, 3, 4, 5
 
 
 class OModel(C.Structure):
@@ -64,6 +64,7 @@ def lib():
         L.o_sweep_batch.restype = C.c_int64
         L.o_sweep_batch.argtypes = [C.POINTER(OModel), _vp, _vp, C.c_int32, _vp, C.c_int32, _vp,
                                     _vp, C.c_uint64, C.c_uint32, _vp]
+        L.o_start_state.argtypes = [C.c_uint64, C.c_int32, C.c_int32, C.c_int32, _vp, _vp, _vp]
         L.o_ising_threshold.restype = C.c_uint32
         L.o_ising_threshold.argtypes = [C.c_double, C.c_double]
         L.o_sweep.restype = C.c_int64
@@ -113,6 +114,15 @@ def rng(seed, id_, replica, step, purpose, attempt=0):
     out = np.zeros(4, np.uint32)
     lib().o_rng(C.c_uint64(seed), id_, replica, step, purpose, attempt, _p(out))
     return out
+
+
+def start_state(seed, model, angle, file_index):
+    """generate_supercell start state of the basis sites: (S [nb, 3], phi [nb])."""
+    fi = _a(file_index, np.int32)
+    S = np.zeros((fi.size, 3))
+    phi = np.zeros(fi.size)
+    lib().o_start_state(C.c_uint64(seed), int(model), int(angle), fi.size, _p(fi), _p(S), _p(phi))
+    return S, phi
 
 
 def neighbours_brute(abc, basis_xyz, sc, open_bc, shells, tol=1e-5, capacity=20):

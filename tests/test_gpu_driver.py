@@ -2,9 +2,10 @@
 
 The Fortran driver (ether_amd/lib/ether_gpu_driver, iso_c_binding) and the
 Python driver (ether_amd.driver, ctypes) read the same run directory and use
-the same seed.  For Ising and for Marsaglia-move Heisenberg runs the device
-state after fresh_spins does not depend on the host-side start state, so both
-drivers must write identical energy.dat / magnetization.dat numbers (and
+the same seed.  Both take the generate_supercell start state from the
+library's ether_start_state, a function of SEED, so for Ising, Marsaglia and
+ANGLE runs alike both drivers must write identical energy.dat /
+magnetization.dat numbers (and
 spiral_state_parameters.dat with the spiral state on), with identical header
 records, and identical ETHER.spn / ETHER.obs records, gss.dat,
 moment_vectors.dat and spK.xsf files.
@@ -36,6 +37,8 @@ def _table(path):
     ("sc_heis_afm_32", dict(sc="6 6 6", mcs="400 200", repeat="3", temp="60 6 18",
                             ssp=".T.", ss_dis="1.01")),
     ("pyro_pt_16", dict(sc="2 2 2", mcs="440 220", repeat="2", temp="1 0.05 0.19")),
+    # ANGLE moves: the start phi feeds fresh_spins' angle move
+    ("tri_afm_96", dict(sc="12 12 1", mcs="400 200", repeat="2", temp="20 0.5 6.5")),
 ])
 def test_fortran_and_python_drivers_agree(tmp_path, name, over):
     say(f"drivers {name}")
@@ -43,6 +46,14 @@ def test_fortran_and_python_drivers_agree(tmp_path, name, over):
     r = subprocess.run([DRIVER], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     say(r.stdout[-400:])
     assert r.returncode == 0, r.stdout + r.stderr
+    if name == "tri_afm_96":
+        # the same SEED twice: the same start state, the same files
+        again = tmp_path / "again"
+        workloads.write_dir(name, str(again), **over)
+        r = subprocess.run([DRIVER], cwd=again, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        for f in ("energy.dat", "magnetization.dat", "ETHER.spn"):
+            assert open(tmp_path / f, "rb").read() == open(again / f, "rb").read(), f
     res = driver.run(read_directory(str(tmp_path)))
     res.write(str(tmp_path / "py"))
     files = ["energy.dat", "magnetization.dat"]

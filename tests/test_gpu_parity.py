@@ -47,17 +47,40 @@ def _ising_case(name, R, K, **over):
     # energies: exact integer bond sums per coupling class; magnetisation too
     _, M = eng.measure()
     parts, bonds = eng.energy_parts()
+    coe, ncls = ising_classes(m)
+    assert bonds.shape[1] == ncls
     for r in range(R):
         S = states[r]
         assert np.allclose(M[r], orc.magnetisation(S), rtol=0, atol=1e-9)
         ex, ze, si = orc.total_energy(S)
         assert parts[r, 0] == pytest.approx(ex, rel=1e-12, abs=1e-12)
         assert parts[r, 1] == pytest.approx(ze, rel=1e-12, abs=1e-12)
+        # every coupling class: the oracle's integer bond sums of the same classes
+        assert np.array_equal(bonds[r], orc.ising_bond_sums(S, coe, ncls)), f"replica {r}"
         if bonds.shape[1] == 1:
             # one class: B = E_exch / J exactly
             jzz = m.j_exc.reshape(-1, 3)[np.flatnonzero(m.j_exc.reshape(-1, 3)[:, 2])[0], 2]
             assert bonds[r, 0] == round(ex / jzz)
     eng.close()
+
+
+def ising_classes(m):
+    """The engine's Ising coupling classes: distinct nonzero J_zz numbered in
+    order of first appearance over sites, shells and neighbours; returns the
+    class of every neighbour entry (-1 for zero couplings) and the count."""
+    ns, n = m.nn_count.shape
+    nsp = m.n_species
+    jz = m.j_exc.reshape(ns, nsp, nsp, 3)[..., 2]
+    seen = {}
+    coe = np.full(m.nn_idx.shape, -1, np.int32)
+    for i in range(n):
+        a = m.species[i] - 1
+        for s in range(ns):
+            for k in range(m.nn_count[s, i]):
+                J = jz[s, a, m.species[m.nn_idx[s, i, k]] - 1]
+                if J != 0.0:
+                    coe[s, i, k] = seen.setdefault(float(J), len(seen))
+    return coe, len(seen)
 
 
 def test_ising_sc_bit_exact():
@@ -189,7 +212,7 @@ def test_xyz_one_sweep_matches_oracle(name, over, R):
     eng.close()
 
 
-@pytest.mark.parametrize("frac", [1.0, 0.5])
+@pytest.mark.parametrize("frac", [1.0, 0.5, 0.3])
 def test_overrelaxation_matches_oracle(frac):
     say(f"ovrr frac={frac}")
     m, eng, orc = pair("sc_heis_afm_32", 2, sc="6 6 6")
@@ -250,6 +273,7 @@ def _oracle_fused_with_margins(m, orc, states, order, rid, betas, seed, step, fr
 @pytest.mark.parametrize("name,over,R,frac", [
     ("sc_heis_afm_32", dict(sc="8 8 8"), 2, 1.0),
     ("sc_heis_afm_32", dict(sc="4 4 4"), 70, 0.5),                      # OVRR draw, two groups
+    ("sc_heis_afm_32", dict(sc="6 6 6"), 3, 0.3),                       # selection below one half
     ("tri_afm_96", dict(sc="12 12 1", sia=".F."), 2, 1.0),              # ANGLE + Zeeman
     ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11"), 2, 1.0),
 ])
@@ -321,10 +345,22 @@ def test_split_ovrr_compat_equals_separate_calls():
     ref.close()
 
 
-def test_pt_swaps_match_oracle():
-    say("pt swaps")
+@pytest.mark.parametrize("etot", [False, True])
+def test_pt_swaps_match_oracle(etot):
+    """Swap decisions of 24 rounds against o_pt_accept (src/parallel_tempering.f90:77-89)
+    on the physical energy, or with ETHER_COMPAT_PT_ETOT on the reference's
+    E_tot (exchange double counted, Zeeman once: src/ether.f90:177,194)."""
+    from ether_amd import engine as EE
+    from ether_amd.driver import make_engine
+    say(f"pt swaps etot={etot}")
     R = 8
-    m, eng, orc = pair("sc_heis_afm_32", R, sc="4 4 4", para=".T. 1")
+    over = dict(sc="4 4 4", para=".T. 1")
+    if etot:
+        over["zeeman"] = ".T. 0 0 0.8"
+    m, eng, orc = pair("sc_heis_afm_32", R, **over)
+    if etot:
+        eng.close()
+        eng = make_engine(m, R, seed=1992, compat=EE.COMPAT_PT_ETOT)
     betas = np.linspace(0.1, 0.45, R)
     eng.set_betas(betas)
     eng.set_ladders([0, R])
@@ -332,7 +368,7 @@ def test_pt_swaps_match_oracle():
     E = []
     for S, _ in states:
         ex, ze, si = orc.total_energy(S)
-        E.append(0.5 * ex + ze + si)
+        E.append(ex + ze + si if etot else 0.5 * ex + ze + si)
     E = np.array(E)
     rep_at = list(range(R))
     total = 0
@@ -389,3 +425,30 @@ def test_pyrochlore_at_full_size_with_swaps():
         scale = sum(abs(orc.site_energy(S, i)) for i in range(N))
         assert abs(parts[r, 0] - ex) <= 1e-6 * scale, (r, parts[r, 0], ex)
     eng.close()
+
+
+def test_ising_refuses_more_than_255_bonds_of_one_class():
+    """The Ising kernels count a site's down neighbours per coupling class in
+    one byte per replica: a star of 300 sites (site 0 bonded to all others with
+    one J) is refused at commit with ETHER_ERR_LIMIT instead of overflowing;
+    a star of 200 commits."""
+    from ether_amd import engine as EE
+    say("ising class overflow refusal")
+    for n, ok in ((300, False), (200, True)):
+        cap = n - 1
+        cnt = np.zeros((1, n), np.int32)
+        idx = np.full((1, n, cap), -1, np.int32)
+        cnt[0, 0] = n - 1
+        idx[0, 0, : n - 1] = np.arange(1, n)
+        cnt[0, 1:] = 1
+        idx[0, 1:, 0] = 0
+        eng = EE.Engine(EE.MODEL_ISING, n, 4, 1)
+        try:
+            if ok:
+                eng.setup(1, np.ones(n, np.int32), np.ones(n), cnt, idx, [0, 0, 1.0], [1.0])
+            else:
+                with pytest.raises(EE.EtherError) as ei:
+                    eng.setup(1, np.ones(n, np.int32), np.ones(n), cnt, idx, [0, 0, 1.0], [1.0])
+                assert ei.value.code == EE.ETHER_ERR_LIMIT and "255" in str(ei.value)
+        finally:
+            eng.close()

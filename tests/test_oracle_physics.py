@@ -313,3 +313,113 @@ def test_structure_factor_closed_forms():
     S[:, 0] = sign
     assert o.structure_factor(pos, S, [np.pi, np.pi, np.pi]) == pytest.approx(1.0)
     assert o.structure_factor(pos, S, [0, 0, 0]) == pytest.approx(0.0, abs=1e-24)
+
+
+def test_ising_bond_sums_against_pair_count_from_positions():
+    """o_ising_bond_sums against a count written from positions: two species,
+    four shells, classes = (shell, species pair); every ordered pair (i, j),
+    j != i, at a shell distance under some image adds sigma_i sigma_j to its
+    class.  A wrong class index, a transposed species pair or sigma_i sigma_i
+    fails it."""
+    rng = np.random.default_rng(17)
+    L = 3
+    abc = np.eye(3)
+    basis = [[0, 0, 0], [0.5, 0.5, 0.5]]
+    shells = [1.0, 2 ** 0.5, 3 ** 0.5, 0.75 ** 0.5]
+    cnt, idx = build_neighbours(abc, basis, (L, L, L), (0, 0, 0), shells)
+    n = 2 * L ** 3
+    species = np.tile([1, 2], L ** 3).astype(np.int32)
+    pos = np.array([[i, j, k] for k in range(L) for j in range(L) for i in range(L)], float)
+    pos = (pos[:, None, :] + np.array(basis)[None]).reshape(-1, 3)
+    ns = len(shells)
+    cls_of = np.arange(ns * 4).reshape(ns, 2, 2)       # class of (shell, sp_i, sp_j)
+    jt = np.zeros((ns, 2, 2, 3))
+    jt[..., 2] = rng.normal(size=(ns, 2, 2))
+    orc = o.Oracle(0, n, 2, species, np.ones(n), cnt, idx, jt.reshape(-1), [1.0, 1.0])
+    S = np.zeros((n, 3))
+    S[:, 2] = np.where(rng.random(n) < 0.5, 1.0, -1.0)
+    coe = np.full(idx.shape, -1, np.int32)
+    for s in range(ns):
+        for i in range(n):
+            for k in range(cnt[s, i]):
+                coe[s, i, k] = cls_of[s, species[i] - 1, species[idx[s, i, k]] - 1]
+    got = orc.ising_bond_sums(S, coe, ns * 4)
+    want = np.zeros(ns * 4, np.int64)
+    shifts = [np.array(v) for v in itertools.product((0, 1, -1), repeat=3)]
+    for i in range(n):
+        for j in range(n):
+            if j == i:
+                continue
+            for s, d in enumerate(shells):
+                if any(abs(d - np.linalg.norm(pos[i] - (pos[j] + sh * L))) <= 1e-5 for sh in shifts):
+                    want[cls_of[s, species[i] - 1, species[j] - 1]] += int(S[i, 2] * S[j, 2])
+    assert np.array_equal(got, want)
+    # and the classes carry the whole exchange energy: sum_k J_k B_k = E_exch
+    jk = jt[..., 2].reshape(-1)
+    assert float(np.dot(jk, want)) == pytest.approx(orc.total_energy(S)[0], rel=1e-12, abs=1e-12)
+
+
+def test_overrelax_pass_selection_rate():
+    """A pass with frac = 0.3 reflects each site with probability 0.3: over
+    5120 site draws the reflected count lies within 4 sigma of the binomial
+    mean (an inverted compare gives 0.7, a dropped test 1.0).  Unselected
+    sites keep their spin; selected ones are the reflection about the field
+    of the spins present when the site is visited."""
+    rng = np.random.default_rng(19)
+    L = 8
+    orc, _, _ = sc_oracle(L, [-1.0, 0.3], shells=(1.0, 2 ** 0.5))
+    n = L ** 3
+    order = np.arange(n, dtype=np.int32)
+    rid = np.arange(n, dtype=np.int32)
+    frac, total, hits = 0.3, 0, 0
+    for step in range(10):
+        S = random_spins(rng, n)
+        phi = np.arctan2(S[:, 1], S[:, 0])
+        S1, phi1 = S.copy(), phi.copy()
+        orc.overrelax_pass(S1, phi1, order, rid, frac, 1992, 0, step, 0)
+        # replay site by site: a site either keeps its spin or is reflected
+        cur = S.copy()
+        for i in range(n):
+            if np.array_equal(S1[i], S[i]):
+                continue
+            tmp = cur.copy()
+            orc.reflect(tmp, i)
+            assert np.allclose(S1[i], tmp[i], atol=1e-13)
+            cur[i] = S1[i]
+            hits += 1
+        total += n
+    sd = math.sqrt(total * frac * (1 - frac))
+    assert abs(hits - frac * total) < 4 * sd, (hits, total)
+
+
+def test_start_state_oracle_and_library():
+    """generate_supercell's start state (src/generate_supercell.f90:34-63):
+    the library's ether_start_state equals the oracle's restatement; Ising is
+    (-1)^i by file index; Marsaglia points are uniform on S^2 (<S> = 0,
+    <S_z^2> = 1/3, flat S_z histogram); ANGLE has S_z ~ U(-1,1) and
+    phi ~ U(0, 2 pi) with S_x = sqrt(1-S_z^2) cos phi."""
+    from ether_amd.engine import start_state
+    idx = np.arange(1, 6)
+    S, phi = start_state(7, 0, 0, idx)
+    assert np.array_equal(S[:, 2], (-1.0) ** idx) and not S[:, :2].any()
+    nb = 20000
+    fi = np.arange(1, nb + 1)
+    for angle in (0, 1):
+        S, phi = start_state(1992, 1, angle, fi)
+        So, po = o.start_state(1992, 1, angle, fi)
+        assert np.allclose(S, So, atol=1e-15, rtol=0) and np.allclose(phi, po, atol=1e-15, rtol=0)
+        assert np.allclose(np.linalg.norm(S, axis=1), 1.0, atol=1e-14)
+        sd = 1.0 / math.sqrt(3 * nb)
+        assert np.all(np.abs(S.mean(axis=0)) < 4.5 * sd)
+        assert abs((S[:, 2] ** 2).mean() - 1 / 3) < 4.5 * math.sqrt(4 / 45 / nb)
+        h, _ = np.histogram(S[:, 2], bins=20, range=(-1, 1))
+        assert ((h - nb / 20) ** 2 / (nb / 20)).sum() < 45.0      # chi^2, 19 dof, p ~ 1e-3
+        if angle:
+            assert np.all((phi >= 0) & (phi < 2 * math.pi))
+            assert np.allclose(S[:, 0], np.sqrt(1 - S[:, 2] ** 2) * np.cos(phi), atol=1e-12)
+            hp, _ = np.histogram(phi, bins=20, range=(0, 2 * math.pi))
+            assert ((hp - nb / 20) ** 2 / (nb / 20)).sum() < 45.0
+        else:
+            assert not phi.any()
+    # a function of the seed
+    assert not np.allclose(start_state(1, 1, 0, fi[:8])[0], start_state(2, 1, 0, fi[:8])[0])

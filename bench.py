@@ -10,10 +10,10 @@ Metropolis sweep + the scheduled measurement), the per-MCS schedule of the
 reference's driver, with the reflection fused into each colour phase of the
 sweep and the measurement booked by the sweep before it (two colours).  The
 launch shape of the sweep is picked by the autotuner first (`--launch` pins
-it).  The
-timed region is the whole production run of `--steps` MCS, whole schedule
-periods replayed as HIP graphs; `--warmup` untimed MCS run first on a separate
-start state.
+it).  The run starts from the start state and fresh_spins; `--warmup` untimed
+production MCS follow; the timed region is `--steps` production MCS (samples
+every 10 MCS included) from the next schedule period, whole periods replayed
+as HIP graphs that are captured and uploaded before the clock starts.
 
 value = Metropolis spin updates / wall second of the timed MCS loop
         (OVRR and measurement time included, not counted as updates).
@@ -169,6 +169,8 @@ def main():
         # 20 -> 1 K with the same 64 temperatures and schedule, over enough
         # swap rounds for labels to cross a 64-slot ladder.
         out["pt_trips"] = pt_comparison(args.pt_trips_mcs, cells=2, temp="20 1 0.301587301587302")
+        # the second metric in brief, where a reader of the config sees it
+        out["config"]["pt_round_trips"] = {k: pt_brief(out[k]) for k in ("pt", "pt_trips")}
     if not args.no_sq:
         out["sq"] = sq_bench(args.sq_calls)
     if not args.no_batch:
@@ -187,10 +189,12 @@ def main():
 
 
 def measure(workload, steps, warmup, event_steps, split_ovrr, args):
-    """The headline measurement on one workload: autotuned launch shape, an
-    untimed warmup on its own start state, `steps` timed MCS of the schedule
-    (whole periods replayed as graphs), then `event_steps` more MCS launched
-    one by one between HIP events for the kernel durations."""
+    """The headline measurement on one workload: autotuned launch shape, the
+    start state and fresh_spins, `warmup` untimed production MCS, `steps`
+    timed production MCS of the schedule from the next period boundary (whole
+    periods replayed as graphs captured before the timed region), then
+    `event_steps` more MCS launched one by one between HIP events for the
+    kernel durations."""
     import numpy as np
     from ether_amd import engine as E
     from ether_amd import workloads
@@ -216,32 +220,34 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
     ovrr_iv = t.ovrr_mcs if ovrr_on else 0
     ovrr_p = t.ovrr_para if ovrr_on else 0.0
 
-    # warmup on its own start state
-    eng.init_spins(E.INIT_RANDOM)
-    if warmup > 0:
-        eng.run(1, warmup, t.tmcs_eq, coa, ovrr_iv, ovrr_p)
-    eng.synchronize()
-
-    # timed production run: steps 1..K of the schedule, whole periods replayed
-    # as captured HIP graphs
+    # the schedule's period: whole periods are replayed as captured graphs
+    eq = t.tmcs_eq
+    period = coa if not ovrr_iv else coa * ovrr_iv // math.gcd(coa, ovrr_iv)
+    # warmup: the start state, fresh_spins, then W untimed production MCS
     eng.set_spins(-1, m.initial_spins, m.initial_phi)
     eng.init_spins(E.INIT_FRESH)
+    if warmup > 0:
+        eng.run(eq + 1, warmup, eq, coa, ovrr_iv, ovrr_p)
+    # timed window: K production MCS (samples included) from the next period
+    # boundary; every graph they replay is captured and uploaded beforehand
+    first = eq + 1 + period * -(-warmup // period)
+    eng.run_prepare(first, steps, eq, coa, ovrr_iv, ovrr_p)
     eng.reset_moments()
     eng.synchronize()
     t0 = time.perf_counter()
-    eng.run(1, steps, t.tmcs_eq, coa, ovrr_iv, ovrr_p)
+    eng.run(first, steps, eq, coa, ovrr_iv, ovrr_p)
     eng.synchronize()
     wall = time.perf_counter() - t0
     mom = eng.moments()
     updates = float(R) * m.n_sites * steps
     value = updates / wall
 
-    # event-timed continuation of the same run (steps K+1..K+E): every kernel
-    # is launched on its own between HIP events on the engine stream, which a
-    # graph replay does not allow; gives the per-launch kernel durations
+    # event-timed continuation of the same run (the next E steps): every
+    # kernel is launched on its own between HIP events on the engine stream,
+    # which a graph replay does not allow; gives the per-launch kernel durations
     eng.set_timing(True)
     t1 = time.perf_counter()
-    eng.run(steps + 1, event_steps, t.tmcs_eq, coa, ovrr_iv, ovrr_p)
+    eng.run(first + steps, event_steps, eq, coa, ovrr_iv, ovrr_p)
     eng.synchronize()
     wall_ev = time.perf_counter() - t1
     sweep_ms, sweep_n = eng.timing(E.TIME_SWEEP)
@@ -272,6 +278,8 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
             "workload": describe(workload, m, len(temps), R, steps, split_ovrr, coa, ovrr_iv,
                                  args.measure_pass),
             "replicas": R, "sites": m.n_sites, "colours": nc, "mcs": steps,
+            "timed_steps": f"production MCS {first}..{first + steps - 1} (equilibration {eq}), "
+                           f"after {warmup} untimed production MCS",
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -280,7 +288,7 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
             "bytes_per_update": b_upd, "updates_per_launch": per_launch_updates,
             "avg_launch_ms": avg_ms, "launches": sweep_n,
             "timing": f"HIP events on the engine stream around the back-to-back colour phases of "
-                      f"every sweep over MCS {steps + 1}..{steps + event_steps} of the same run "
+                      f"every sweep over MCS {first + steps}..{first + steps + event_steps - 1} of the same run "
                       f"(launched one by one; {wall_ev / event_steps * 1e3:.4f} ms per MCS there), "
                       f"divided by the phases they cover",
         },
@@ -426,6 +434,19 @@ def batch_bench(mcs, n_lattices=256):
             "speedup": t_plain / t_batched,
             "batched_ms_per_mcs": t_batched / mcs * 1e3, "plain_ms_per_mcs": t_plain / mcs * 1e3,
             "lattices": {k: len(v) for k, v in engines.items()}, "setup_s": setup_s}
+
+
+def pt_brief(cmp):
+    """Round trips, round trips per second, round-trip cost and the lowest
+    swap acceptance per ladder of one pt_comparison."""
+    def one(r):
+        return {"round_trips": r["round_trips"], "per_s": round(r["round_trips_per_s"], 4),
+                "cost": round(r["round_trip_cost"], 2), "swap_acc_min": round(r["swap_acc_min"], 5),
+                "spin_updates_per_s": float(f"{r['spin_updates_per_s']:.4g}")}
+    return {"workload": cmp["workload"].split(";")[0],
+            "linear": one(cmp["linear"]), "learned": one(cmp["learned"]),
+            "plain_per_ladder": one(cmp["plain_per_ladder"]),
+            "learned_ladder_kept": cmp["learned"]["ladder"]}
 
 
 def pt_comparison(mcs, cells=16, temp=None):

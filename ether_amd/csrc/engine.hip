@@ -1755,6 +1755,48 @@ extern "C" int ether_run(ether_ctx* c, int32_t first_step, int32_t n_steps, int3
     return ETHER_OK;
 }
 
+// Captures, instantiates and uploads every graph that ether_run over the same
+// steps would replay, without running anything, so that a later ether_run
+// pays no capture or instantiation.
+extern "C" int ether_run_prepare(ether_ctx* c, int32_t first_step, int32_t n_steps, int32_t mcs_eq,
+                                 int32_t coa, int32_t ovrr_interval, double ovrr_passes,
+                                 int32_t pt_interval) {
+    TRY(need_betas(c));
+    if (coa <= 0) return fail(c, ETHER_ERR_ARG, "ether_run_prepare: coa must be positive");
+    if (n_steps < 0) return fail(c, ETHER_ERR_ARG, "ether_run_prepare: negative n_steps");
+    const bool do_ovrr = ovrr_interval > 0 && c->model == ETHER_MODEL_XYZ && !c->sia &&
+                         ovrr_passes > 0.0;
+    const bool do_pt = pt_interval > 0 && c->n_groups > 0;
+    const int ovrr_iv = do_ovrr ? ovrr_interval : 0;
+    const int pt_iv = do_pt ? pt_interval : 0;
+    auto gcd = [](long long x, long long y) { while (y) { long long t = x % y; x = y; y = t; } return x; };
+    long long P = coa;
+    if (ovrr_iv) P = P / gcd(P, ovrr_iv) * ovrr_iv;
+    if (pt_iv) P = P / gcd(P, pt_iv) * pt_iv;
+    if (!(c->use_graphs && !c->timing && P <= kMaxGraphPeriod)) return ETHER_OK;
+    const int32_t last = first_step + n_steps - 1;
+    bool want[2] = {false, false};
+    for (int32_t stepi = first_step; stepi <= last;) {
+        if ((stepi - 1) % P == 0 && stepi + P - 1 <= last) {
+            const int phase = stepi > mcs_eq ? 1 : (stepi + P - 1 <= mcs_eq ? 0 : -1);
+            if (phase >= 0) {
+                want[phase] = true;
+                stepi += (int32_t)P;
+                continue;
+            }
+        }
+        ++stepi;
+    }
+    for (int phase = 0; phase < 2; ++phase) {
+        if (!want[phase]) continue;
+        int gi = -1;
+        TRY(graph_for(c, phase, (int)P, coa, ovrr_iv, ovrr_passes, pt_iv, &gi));
+        HIPCHK(c, hipGraphUpload(c->graphs[gi].exec, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ETHER_OK;
+}
+
 extern "C" int ether_set_graphs(ether_ctx* c, int32_t on) {
     if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
     c->use_graphs = on ? 1 : 0;
@@ -2233,6 +2275,7 @@ hipStream_t ctx_stream(ether_ctx* c) { return c->stream; }
 void*& ctx_mlp(ether_ctx* c) { return c->mlp; }
 void*& ctx_sq(ether_ctx* c) { return c->sq; }
 int ctx_fail(ether_ctx* c, int code, const std::string& msg) { return fail(c, code, msg); }
+int ctx_groups(const ether_ctx* c) { return c->n_groups; }
 int ctx_model(ether_ctx* c) { return c->model; }
 int ctx_ev_begin(ether_ctx* c, int kind) { return ev_begin(c, kind); }
 void ctx_ev_end(ether_ctx* c, int k) { ev_end(c, k); }

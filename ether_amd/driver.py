@@ -199,8 +199,9 @@ def _train(model: Model, temps, rep: int, eng: E.Engine, sched):
     if not t.pt:
         raise EtherInputError("ladder = learned needs parallel tempering (pt = .T.)")
     ntr = g.train_mcs or t.tmcs_eq // 2
-    if ntr <= 0 or ntr > t.tmcs_eq:
-        raise EtherInputError("ladder = learned: the training MCS must lie within the equilibration MCS")
+    if ntr <= 0 or ntr + 3 * (ntr // 8) > t.tmcs_eq:
+        raise EtherInputError("ladder = learned: the training MCS (and three validation windows "
+                              "of 1/8 of them) must lie within the equilibration MCS")
     from .ladder import train_ladder   # ladder imports this module
     coa, ovrr_iv, ovrr_p, pt_iv = sched
     tr = train_ladder(eng, model, temps, rep, coa, pt_iv, ntr, windows=g.windows,

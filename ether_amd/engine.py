@@ -76,6 +76,8 @@ EXPORTS = [
     ("ether_run", C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                             C.c_int32, _vp]),
     ("ether_set_graphs", C.c_int, [_vp, C.c_int32]),
+    ("ether_run_prepare", C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                    C.c_double, C.c_int32]),
     ("ether_set_launch", C.c_int, [_vp, C.c_int32, C.c_int32]),
     ("ether_set_lanes", C.c_int, [_vp, C.c_int32]),
     ("ether_batch_create", C.c_int, [_vp, C.c_int32, C.POINTER(_vp)]),
@@ -99,6 +101,9 @@ EXPORTS = [
     ("ether_mlp_train", C.c_int, [_vp, C.c_int32, _vp, _vp, _vp, C.c_int32, C.c_float, _vp]),
     ("ether_ladder_place", C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double,
                                      C.c_double, C.c_int32, _vp]),
+    ("ether_ladder_train", C.c_int, [_vp, C.c_int32, _vp, C.c_double, C.c_int32, C.c_int32, C.c_int32,
+                                     C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_int32,
+                                     C.c_int32, C.c_uint64, _vp, _vp, _vp]),
 ]
 
 _lib: Optional[C.CDLL] = None
@@ -350,6 +355,12 @@ class Engine:
                                    C.byref(sc)))
         return sc.value
 
+    def run_prepare(self, first_step: int, n_steps: int, mcs_eq: int, coa: int, ovrr_interval: int = 0,
+                    ovrr_passes: float = 0.0, pt_interval: int = 0) -> None:
+        """Capture and upload the graphs run() over the same steps would replay."""
+        self._c(self.lib.ether_run_prepare(self.h, int(first_step), int(n_steps), int(mcs_eq), int(coa),
+                                           int(ovrr_interval), float(ovrr_passes), int(pt_interval)))
+
     def set_graphs(self, on: bool) -> None:
         self._c(self.lib.ether_set_graphs(self.h, 1 if on else 0))
 
@@ -450,6 +461,22 @@ class Engine:
         self._c(self.lib.ether_ladder_place(self.h, float(x_mid), float(x_half), float(logcv_scale),
                                             float(t_lo), float(t_hi), int(n), _ptr(T)))
         return T
+
+    def ladder_train(self, temps, kb, first_step, train_mcs, windows, validate_mcs, coa,
+                     ovrr_interval, ovrr_passes, pt_interval, learn, placement=0, mlp_seed=1992,
+                     swap_count=0):
+        """The learned ladder's training and validation windows (see
+        ether_ladder_train).  Returns (production ladder, swap count, report[7])."""
+        t = _arr(temps, np.float64)
+        T = np.zeros(t.size, np.float64)
+        rep = np.zeros(7, np.float64)
+        sc = C.c_int32(int(swap_count))
+        self._c(self.lib.ether_ladder_train(self.h, t.size, _ptr(t), float(kb), int(first_step),
+                                            int(train_mcs), int(windows), int(validate_mcs), int(coa),
+                                            int(ovrr_interval), float(ovrr_passes), int(pt_interval),
+                                            1 if learn else 0, int(placement), C.c_uint64(int(mlp_seed)),
+                                            C.byref(sc), _ptr(T), _ptr(rep)))
+        return T, sc.value, rep
 
 
 class Batch:

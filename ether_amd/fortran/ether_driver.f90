@@ -28,7 +28,8 @@ module ether_driver_input
    logical :: ovrr = .false., pt = .false., cb = .false., ssp = .false.
    ! gpu.ether: engine settings the reference has no tags for
    integer :: gpu_autotune = 0, gpu_spw = 0, gpu_xcd = 1, gpu_lanes = 0, gpu_compat = 0, gpu_chunk = 0
-   logical :: gpu_graphs = .true.
+   logical :: gpu_graphs = .true., gpu_learned = .false.
+   integer :: gpu_train_mcs = 0, gpu_windows = 4
    character :: bc(3) = (/'c', 'c', 'c'/)
    real(dp), allocatable :: spin(:)
    character(len=2), allocatable :: species_incl(:)
@@ -238,9 +239,21 @@ contains
          select case (trim(key))
          case ('ladder')
             read (value, *, iostat=ios) word
-            if (trim(word) == 'learned') call terminate('ladder = learned is run by the Python driver '// &
-                                                        '(python -m ether_amd)')
-            if (trim(word) /= 'linear') call terminate("ladder must be 'linear' or 'learned'")
+            if (trim(word) /= 'linear' .and. trim(word) /= 'learned') &
+               call terminate("ladder must be 'linear' or 'learned'")
+            gpu_learned = trim(word) == 'learned'
+            ! optional training MCS and window count after the word
+            if (gpu_learned) then
+               read (value, *, iostat=ios) word, gpu_train_mcs, gpu_windows
+               if (ios /= 0) then
+                  gpu_windows = 4
+                  read (value, *, iostat=ios) word, gpu_train_mcs
+                  if (ios /= 0) gpu_train_mcs = 0
+                  ios = 0
+               end if
+               if (gpu_train_mcs < 0 .or. gpu_windows < 1) &
+                  call terminate('ladder: training MCS must be >= 0 and windows >= 1')
+            end if
          case ('autotune')
             read (value, *, iostat=ios) gpu_autotune
          case ('launch')
@@ -315,7 +328,7 @@ program ether_gpu_driver
    implicit none
 
    integer :: nb, n, i, j, k, l, s, a, b, q, nsh, npairs, u, ios, nscan, R, t, rep, coa_i
-   integer :: start, nchunk, step, sw, rc, ovrr_iv, cnt, cap
+   integer :: start, nchunk, step, sw, rc, ovrr_iv, cnt, cap, first_counted
    integer, allocatable :: b_sp(:), b_idx(:), species(:), nn_count(:), nn_idx(:), gstart(:)
    integer(c_int), allocatable :: sp_partners(:)
    integer(c_int64_t) :: sp_count
@@ -329,7 +342,8 @@ program ether_gpu_driver
    real(dp), allocatable :: sfin(:, :), phifin(:), ionrec(:, :), obs(:), mmv(:, :)
    real(dp), allocatable :: b_xyz(:, :), stagger(:), S0(:, :), phi0(:), shells(:), jx(:, :, :, :)
    real(dp), allocatable :: start_s(:, :), start_phi(:)
-   real(dp), allocatable :: siav(:, :), temps(:), betas(:), mom(:, :), per(:, :, :)
+   real(dp), allocatable :: siav(:, :), temps(:), betas(:), mom(:, :), per(:, :, :), tnew(:)
+   real(dp) :: lreport(7)
    real(dp), allocatable :: mean(:, :), sem(:, :), accp(:), stg_id(:)
    real(dp) :: kb, mub, gf, jv(3), ovrr_p, c1, r1
    real(dp) :: e, e2, e4, mm, m2, m4, nn
@@ -564,6 +578,35 @@ program ether_gpu_driver
    call system_clock(t0, rate)
    swap_c = 0
    step = 1
+   ! gpu.ether `ladder = learned`: the ladder is trained inside the equilibration
+   ! MCS by the library (ether_ladder_train) and replaces the linear one for the
+   ! rest of the run, where it beat the linear ladder in validation
+   if (gpu_learned) then
+      if (.not. pt) call terminate('ladder = learned needs parallel tempering (pt = .T.)')
+      q = gpu_train_mcs
+      if (q == 0) q = tmcs_eq/2
+      if (q <= 0 .or. q + 3*(q/8) > tmcs_eq) &
+         call terminate('ladder = learned: the training MCS (and three validation windows '// &
+                        'of 1/8 of them) must lie within the equilibration MCS')
+      allocate (tnew(nscan))
+      call chk(ether_ladder_train(ctx, nscan, temps, kb, 1, q, gpu_windows, q/8, coa_i, ovrr_iv, &
+                                  ovrr_p, exchange_interval, 1, 0, 1992_c_int64_t, swap_c, tnew, &
+                                  lreport))
+      temps = tnew
+      do rep = 0, nrepeat - 1
+         do t = 1, nscan
+            betas(rep*nscan + t) = 1.0_dp/(kb*temps(t))
+         end do
+      end do
+      step = 1 + nint(lreport(7))
+      call chk(ether_reset_moments(ctx))
+      if (lreport(5) > 0.5_dp) then
+         write (6, '(a)') ' ==> learned tempering ladder kept after validation'
+      else
+         write (6, '(a)') ' ==> learned tempering ladder did not beat the linear one; linear kept'
+      end if
+   end if
+   first_counted = step
    nchunk = 1000
    if (gpu_chunk > 0) nchunk = gpu_chunk
    do while (step <= tmcs)
@@ -613,7 +656,7 @@ program ether_gpu_driver
          mean(k, t) = sum(per(k, :, t))/nrepeat
          sem(k, t) = sqrt(r1*sum((per(k, :, t) - mean(k, t))**2))
       end do
-      accp(t) = accp(t)/(nrepeat*tmcs)*100.0_dp
+      accp(t) = accp(t)/(nrepeat*(tmcs - first_counted + 1))*100.0_dp
    end do
 
    ! spiral state per repeat and over repeats (src/get_spiral_state.f90:122-140,

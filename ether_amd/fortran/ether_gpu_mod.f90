@@ -34,6 +34,20 @@ module ether_gpu
          real(c_double), intent(out) :: S(*), phi(*)
       end function
 
+      integer(c_int) function ether_ladder_train(ctx, n_temps, temps, kb, first_step, train_mcs, &
+            windows, validate_mcs, coa, ovrr_interval, ovrr_passes, pt_interval, learn, placement, &
+            mlp_seed, swap_count, T_out, report) bind(C, name="ether_ladder_train")
+         import :: c_int, c_int64_t, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: n_temps, first_step, train_mcs, windows, validate_mcs, coa
+         integer(c_int), value :: ovrr_interval, pt_interval, learn, placement
+         real(c_double), intent(in) :: temps(*)
+         real(c_double), value :: kb, ovrr_passes
+         integer(c_int64_t), value :: mlp_seed
+         integer(c_int), intent(inout) :: swap_count
+         real(c_double), intent(out) :: T_out(*), report(*)
+      end function
+
       integer(c_int) function ether_ctx_create(model, n_sites, n_replicas, seed, ctx) &
             bind(C, name="ether_ctx_create")
          import :: c_int, c_int64_t, c_ptr

@@ -17,18 +17,21 @@ equilibrium (at T/J << 1 uniform Marsaglia proposals are almost never
 accepted).  The C that actually sets the swap acceptance is the one that
 reproduces the measured acceptance a_k of each neighbour pair through the same
 Gaussian formula:  sqrt(C(T_mid)) / T_mid = 2 erfcinv(a_k) / dT_k.  These
-densities are kept on a grid over the temperature range (LengthMap); the
+densities are kept on a grid over the temperature range (the length map); the
 surrogate is trained on log C from that map and on <e>.
 
-Protocol (adaptive_run): the first train_mcs MCS of the equilibration run are
-split into windows.  The first runs on the linear ladder; after each window
-the map takes the intervals the window measured, the MLP is retrained and the
-temperatures of the slots are retuned in place (every replica keeps its state
-and slot), so a pair whose acceptance was too small to measure on one ladder
-(a lower bound only) is measured on the next.  The remaining MCS run on the
-last ladder.  Every swap passes the exact acceptance test throughout, so the
-sampled distribution at each temperature is unchanged; only the production
-window after the last retune is timed and its round trips counted.
+Protocol (ether_ladder_train in the library, so that the Fortran driver runs
+the same one): the first train_mcs MCS are split into windows.  The first runs
+on the linear ladder; after each window the map takes the intervals the window
+measured, the MLP is retrained and the temperatures of the slots are retuned in
+place (every replica keeps its state and slot), so a pair whose acceptance was
+too small to measure on one ladder (a lower bound only) is measured on the
+next.  Three validation windows follow (plain, learned, plain): the learned
+ladder is kept only where it beats the plain one on both the round-trip cost
+and the lowest pair acceptance.  Every swap passes the exact acceptance test
+throughout, so the sampled distribution at each temperature is unchanged; only
+the production window after the validation is timed and its round trips
+counted.
 """
 from __future__ import annotations
 
@@ -37,7 +40,6 @@ from dataclasses import dataclass
 from typing import Optional
 
 import numpy as np
-from scipy.special import erfcinv
 
 from . import engine as E
 from .driver import make_engine
@@ -52,13 +54,10 @@ class LadderRun:
     round_trips: int           # completed cold->hot->cold trips, all replicas
     acc_rate: np.ndarray       # (n-1,) swap acceptance between slot k and k+1
     updates: float
-    train_e: Optional[np.ndarray] = None
-    train_c: Optional[np.ndarray] = None
-    train_acc: Optional[np.ndarray] = None
     mlp_loss: Optional[float] = None
     metro_acc: Optional[np.ndarray] = None   # (n,) Metropolis acceptance per slot, production
     source: str = "plain"                     # which ladder ran the production window
-    train_cost: Optional[float] = None        # its round-trip cost measured in training
+    train_cost: Optional[float] = None        # its round-trip cost measured in validation
 
     @property
     def cost(self) -> float:
@@ -86,84 +85,6 @@ def _pair_acceptance(eng: E.Engine, n: int, repeat: int):
     return a, tot, int(trips.sum())
 
 
-def length_density(temps, acc, attempts):
-    """Thermodynamic-length density of each interval of a ladder, from its
-    swap acceptance.  For Gaussian energy distributions a = erfc(d/2) with
-    d = int sqrt(C)/T dT over the interval, so lambda = 2 erfcinv(a) / |dT|
-    (sqrt(C)/T at the midpoint for smooth C).  A pair that accepted no swap
-    only bounds d from below; it is flagged bound=True."""
-    temps = np.asarray(temps, np.float64)
-    att = np.maximum(np.asarray(attempts, np.float64), 1.0)
-    acc = np.asarray(acc, np.float64)
-    bound = acc * att < 0.5
-    a = np.clip(acc, 0.5 / att, 0.999)
-    lam = 2.0 * erfcinv(a) / np.abs(np.diff(temps))
-    return lam, bound
-
-
-class LengthMap:
-    """sqrt(C)/T on a fixed grid over the temperature range, refined window by
-    window.  A measured interval replaces what earlier windows said on its
-    span; an interval with no accepted swap only raises the value there to
-    its lower bound."""
-
-    def __init__(self, t_lo: float, t_hi: float, n: int = 64):
-        self.T = np.linspace(t_lo, t_hi, n)
-        self.lam = np.full(n, np.nan)
-
-    def add(self, temps, lam, bound) -> None:
-        temps = np.asarray(temps, np.float64)
-        lo = np.minimum(temps[:-1], temps[1:])
-        hi = np.maximum(temps[:-1], temps[1:])
-        for k in range(len(lam)):
-            sel = (self.T >= lo[k]) & (self.T <= hi[k])
-            if bound[k]:
-                self.lam[sel] = np.fmax(self.lam[sel], lam[k])
-            else:
-                self.lam[sel] = lam[k]
-
-    def heat_capacity(self) -> np.ndarray:
-        return (self.lam * self.T) ** 2
-
-
-def learn_ladder(eng: E.Engine, temps, e, c, n_out: Optional[int] = None, iters: int = 4000,
-                 lr: float = 3e-3, seed: int = 1992, t_range=None):
-    """Train the device MLP on samples (T, e, log C) and place a new ladder of
-    n_out temperatures on t_range (default: the span of temps)."""
-    temps = np.asarray(temps, np.float64)
-    n_out = n_out or len(temps)
-    t_lo, t_hi = t_range if t_range is not None else (float(temps.min()), float(temps.max()))
-    mid, half = 0.5 * (t_lo + t_hi), 0.5 * (t_hi - t_lo)
-    x = ((temps - mid) / half).astype(np.float32)
-    lc = np.log(np.maximum(c, 1e-12))
-    y = np.stack([(e - e.mean()) / (e.std() + 1e-30), (lc - lc.mean()) / (lc.std() + 1e-30)],
-                 axis=1).astype(np.float32)
-    w = np.ones_like(y)
-    eng.mlp_init(seed)
-    loss = eng.mlp_train(x, y, w, iters, lr)
-    T = eng.ladder_place(mid, half, float(lc.std() + 1e-30), t_lo, t_hi, n_out)
-    return T, loss
-
-
-def _window(eng: E.Engine, model: Model, temps, repeat: int, first: int, n_mcs: int, coa: int,
-            pt_interval: int, sc: int, ovrr_iv: int = 0, ovrr_p: float = 0.0):
-    """Run n_mcs MCS from step `first` with fresh swap counters and energy
-    moments; returns (pair acceptance, attempts, <e> per slot, swap counter)."""
-    n = len(temps)
-    eng.set_ladders([k * n for k in range(repeat + 1)])
-    eng.reset_moments()
-    # every sampling step of the window counts (eq = first - 1) so that <e> is measured
-    sc = eng.run(first, n_mcs, first - 1, coa, ovrr_iv, ovrr_p, pt_interval, sc)
-    mom = eng.moments()
-    a, att, _ = _pair_acceptance(eng, n, repeat)
-    e = np.zeros(n)
-    for g in range(repeat):
-        for k in range(n):
-            m = mom[g * n + k]
-            e[k] += m[E.MOM_E] / max(m[E.MOM_N], 1.0) / repeat
-    return a, att, e, sc
-
-
 def round_trip_cost(acc, attempts) -> float:
     """Lambda = sum_k r_k / (1 - r_k) over the neighbour pairs of a ladder,
     r_k the measured swap rejection rate (a pair with no accepted swap counts
@@ -176,90 +97,68 @@ def round_trip_cost(acc, attempts) -> float:
     return float(np.sum((1.0 - a) / a))
 
 
+PLACEMENT_MLP, PLACEMENT_MAP = 0, 1
+
+
 @dataclass
 class Training:
-    ladder: np.ndarray         # the ladder chosen for production (hottest first)
-    next_step: int             # first MCS after the training windows
+    ladder: np.ndarray         # the ladder set for production (hottest first)
+    next_step: int             # first MCS after the training and validation windows
     swap_count: int
-    source: str                # "plain" or "learned (window k)"
-    cost: float                # its measured round-trip cost Lambda
-    e: Optional[np.ndarray] = None
-    c: Optional[np.ndarray] = None
-    acc: Optional[np.ndarray] = None
+    source: str                # "plain" or "learned"
+    cost: float                # its round-trip cost Lambda in validation
+    cost_plain: float = float("nan")
+    min_acc: float = float("nan")
+    min_acc_plain: float = float("nan")
     loss: Optional[float] = None
 
 
 def train_ladder(eng: E.Engine, model: Model, temps, repeat: int, coa: int, pt_interval: int,
                  train_mcs: int, *, windows: int = 4, learn: bool = True, first: int = 1,
-                 swap_count: int = 0, ovrr_iv: int = 0, ovrr_p: float = 0.0) -> Training:
-    """The training windows of a tempering run: train_mcs MCS from step
-    `first`, split into `windows` equal windows; each one measures the pair
-    acceptance and <e> of the ladder it ran on.  The first window runs on
-    `temps` (the plain ladder).  learn=True: after every window but the last
-    the length map takes the window's measured intervals, the MLP is retrained
-    on it (log C = 2 log(lambda T) on the map's grid, <e> of the window
-    interpolated) and the slots are retuned to the ladder it places, so each
-    window measures the intervals that the previous ladder was too coarse to
-    measure.  The ladder with the smallest measured round-trip cost Lambda is
-    then set for production; the plain ladder is the fallback when no learned
-    ladder beats it (far from equilibrium the acceptance is not the Gaussian
-    one the surrogate assumes).  learn=False: every window runs on `temps`.
-    Every replica keeps its state and slot throughout; the windows follow the
-    run's schedule (OVRR every ovrr_iv MCS with ovrr_p passes)."""
+                 swap_count: int = 0, ovrr_iv: int = 0, ovrr_p: float = 0.0,
+                 validate_mcs: Optional[int] = None, placement: int = PLACEMENT_MLP) -> Training:
+    """The training windows of a tempering run (ether_train_ladder in the
+    library, which the Fortran driver calls too): train_mcs MCS from step
+    `first` in `windows` windows, the first on `temps` (the plain ladder); with
+    learn=True the ladder is re-placed after each window from the measured
+    swap acceptances (by the device MLP, or straight from the length map with
+    placement=PLACEMENT_MAP), and the slots are retuned in place.  Then three
+    validation windows of validate_mcs MCS (default train_mcs / 8): plain,
+    learned, plain; the learned ladder is kept only if its round-trip cost is
+    lower and its lowest pair acceptance higher than the plain ladder's,
+    otherwise production runs on the plain ladder.  learn=False runs the same
+    windows on `temps`.  Every replica keeps its state and slot; the windows
+    follow the run's schedule (OVRR every ovrr_iv MCS with ovrr_p passes)."""
     temps = np.asarray(temps, np.float64)
-    n = len(temps)
-    t_range = (float(temps.min()), float(temps.max()))
-    win = max(train_mcs // windows, 1)
-    cur, sc = temps, swap_count
-    lmap = LengthMap(*t_range)
-    best, best_cost, source = temps, np.inf, "plain"
-    a = e = c = None
-    loss = None
-    for k in range(windows):
-        length = win if k < windows - 1 else train_mcs - win * (windows - 1)
-        if length <= 0:
-            break
-        a, att, e, sc = _window(eng, model, cur, repeat, first, length, coa, pt_interval, sc,
-                                ovrr_iv, ovrr_p)
-        first += length
-        cost = round_trip_cost(a, att)
-        if cost < best_cost:
-            best, best_cost, source = cur, cost, ("plain" if k == 0 else f"learned (window {k})")
-        lam, bound = length_density(cur, a, att)
-        lmap.add(cur, lam, bound)
-        c = lmap.heat_capacity()
-        if learn and k < windows - 1:
-            o = np.argsort(cur)
-            e_grid = np.interp(lmap.T, cur[o], e[o])
-            cur, loss = learn_ladder(eng, lmap.T, e_grid, c, n_out=n, t_range=t_range)
-            eng.retune_betas(np.tile(model.betas(cur), repeat))
-    if learn and best is not cur:
-        cur = best
-        eng.retune_betas(np.tile(model.betas(cur), repeat))
-    return Training(np.asarray(cur), first, sc, source if learn else "plain", best_cost,
-                    e=e, c=c, acc=a, loss=loss)
+    val = train_mcs // 8 if validate_mcs is None else validate_mcs
+    T, sc, rep = eng.ladder_train(temps, model.kb, first, train_mcs, windows, val, coa, ovrr_iv,
+                                  ovrr_p, pt_interval, learn, placement, swap_count=swap_count)
+    kept = bool(rep[4])
+    return Training(T, first + int(rep[6]), sc, "learned" if kept else "plain",
+                    float(rep[1] if kept else rep[0]), cost_plain=float(rep[0]), min_acc=float(rep[3]),
+                    min_acc_plain=float(rep[2]), loss=None if np.isnan(rep[5]) else float(rep[5]))
 
 
 def adaptive_run(eng: E.Engine, model: Model, temps, repeat: int, mcs: int, coa: int,
-                 pt_interval: int, *, learn: bool, train_mcs: int = 2000,
-                 windows: int = 4) -> LadderRun:
-    """Tempering run of mcs MCS from random states: train_ladder over the
-    first train_mcs MCS, then the remaining mcs - train_mcs MCS are the timed
-    production window on the chosen ladder."""
+                 pt_interval: int, *, learn: bool, train_mcs: int = 2000, windows: int = 4,
+                 validate_mcs: Optional[int] = None, placement: int = PLACEMENT_MLP) -> LadderRun:
+    """Tempering run of mcs MCS from random states: train_ladder (training and
+    validation windows) first, then the remaining MCS are the timed
+    production window on the ladder it set."""
     temps = np.asarray(temps, np.float64)
     n = len(temps)
     eq = model.tags.tmcs_eq
     eng.set_betas(np.tile(model.betas(temps), repeat))
     eng.init_spins(E.INIT_RANDOM)
     tr = train_ladder(eng, model, temps, repeat, coa, pt_interval, train_mcs, windows=windows,
-                      learn=learn)
+                      learn=learn, validate_mcs=validate_mcs, placement=placement)
     cur, first, sc = tr.ladder, tr.next_step, tr.swap_count
     # production window
     eng.set_ladders([k * n for k in range(repeat + 1)])
     eng.reset_moments()
     eng.synchronize()
+    rest = mcs - (first - 1)
     t0 = time.perf_counter()
-    rest = mcs - train_mcs
     sc = eng.run(first, rest, eq, coa, 0, 0.0, pt_interval, sc)
     eng.synchronize()
     wall = time.perf_counter() - t0
@@ -268,12 +167,11 @@ def adaptive_run(eng: E.Engine, model: Model, temps, repeat: int, mcs: int, coa:
     metro = np.array([np.mean([mom[g * n + k, E.MOM_ACC] for g in range(repeat)]) / rest
                       for k in range(n)])
     return LadderRun(cur, wall, rest, trips, acc, float(eng.R) * model.n_sites * rest,
-                     train_e=tr.e, train_c=tr.c, train_acc=tr.acc, mlp_loss=tr.loss, metro_acc=metro,
-                     source=tr.source, train_cost=tr.cost)
+                     mlp_loss=tr.loss, metro_acc=metro, source=tr.source, train_cost=tr.cost)
 
 
 def compare_ladders(model: Model, *, train_mcs: int = 2000, mcs: Optional[int] = None,
-                    repeat: Optional[int] = None):
+                    repeat: Optional[int] = None, placement: int = PLACEMENT_MLP):
     """The linear ladder against the learned ladder on the same model and seed.
     Returns (linear LadderRun, learned LadderRun)."""
     t = model.tags
@@ -286,7 +184,7 @@ def compare_ladders(model: Model, *, train_mcs: int = 2000, mcs: Optional[int] =
         lin = adaptive_run(eng, model, temps, rep, mcs, coa, t.exchange_interval, learn=False,
                            train_mcs=train_mcs)
         lrn = adaptive_run(eng, model, temps, rep, mcs, coa, t.exchange_interval, learn=True,
-                           train_mcs=train_mcs)
+                           train_mcs=train_mcs, placement=placement)
     finally:
         eng.close()
     return lin, lrn
@@ -296,9 +194,10 @@ def plain_run(model: Model, *, train_mcs: int = 2000, mcs: Optional[int] = None,
               repeat: Optional[int] = None) -> LadderRun:
     """The plain GPU form: one context per ladder (repeat), each with the
     linear ladder and launched on its own (one launch per lattice per kernel,
-    its own graphs), the contexts run one after another by the host.  Same
-    schedule and windows as adaptive_run(learn=False); the production windows
-    of all ladders are timed together."""
+    its own graphs), the contexts run one after another by the host.  The
+    same MCS before production as adaptive_run (train_mcs plus three
+    validation windows of train_mcs / 8); the production windows of all
+    ladders are timed together."""
     t = model.tags
     temps = np.asarray(t.temperatures(), np.float64)
     n = len(temps)
@@ -306,22 +205,23 @@ def plain_run(model: Model, *, train_mcs: int = 2000, mcs: Optional[int] = None,
     mcs = t.tmcs if mcs is None else mcs
     coa = fortran_nint(t.coa)
     eq = t.tmcs_eq
+    pre = train_mcs + 3 * (train_mcs // 8)
     engines = [make_engine(model, n, seed=t.seed + 7919 * k) for k in range(rep)]
     try:
         for eng in engines:
             eng.set_betas(model.betas(temps))
             eng.set_ladders([0, n])
             eng.init_spins(E.INIT_RANDOM)
-            eng.run(1, train_mcs, eq, coa, 0, 0.0, t.exchange_interval, 0)
+            eng.run(1, pre, eq, coa, 0, 0.0, t.exchange_interval, 0)
         scs = []
         for eng in engines:
             eng.set_ladders([0, n])
             eng.reset_moments()
             eng.synchronize()
         t0 = time.perf_counter()
-        rest = mcs - train_mcs
+        rest = mcs - pre
         for eng in engines:
-            scs.append(eng.run(train_mcs + 1, rest, eq, coa, 0, 0.0, t.exchange_interval, 0))
+            scs.append(eng.run(pre + 1, rest, eq, coa, 0, 0.0, t.exchange_interval, 0))
         for eng in engines:
             eng.synchronize()
         wall = time.perf_counter() - t0

@@ -275,6 +275,14 @@ int ether_run(ether_ctx* ctx, int32_t first_step, int32_t n_steps, int32_t mcs_e
  * graphs off (default on). */
 int ether_set_graphs(ether_ctx* ctx, int32_t on);
 
+/* Captures, instantiates and uploads every graph that ether_run with the same
+ * arguments would replay, and runs nothing, so that the ether_run that
+ * follows spends no time on capture or instantiation (a timed production
+ * loop, a serving loop).  No counterpart in the reference.  A no-op with
+ * graphs off or kernel timing on. */
+int ether_run_prepare(ether_ctx* ctx, int32_t first_step, int32_t n_steps, int32_t mcs_eq,
+                      int32_t coa, int32_t ovrr_interval, double ovrr_passes, int32_t pt_interval);
+
 /* ---------------------------------------------------------------------------
  * Batches of lattices (no counterpart in the reference, which runs one
  * lattice per process: src/ether.f90:110-214 over MPI ranks).  Each lattice
@@ -386,6 +394,42 @@ int ether_mlp_train(ether_ctx* ctx, int32_t n, const float* x, const float* y, c
  * estimate erfc(dT sqrt(C_v)/(2T)) of the swap acceptance uniform. */
 int ether_ladder_place(ether_ctx* ctx, double x_mid, double x_half, double logcv_scale,
                        double t_lo, double t_hi, int32_t n_temps, double* T_out);
+
+/* Training protocol of the learned ladder, in the library so that every
+ * driver runs the same one.  No reference counterpart (the nearest is the
+ * stub at src/startup.f90:115-122); the plain ladder it starts from and falls
+ * back to is the reference's linear one (src/ether.f90:79-84).
+ * The context holds R = L x n_temps replicas; every one of its L ladders is
+ * slots [g*n_temps, (g+1)*n_temps), and the slots already hold `temps`
+ * (hottest first) through ether_set_betas.  From first_step it runs, with the
+ * ether_run schedule (coa, OVRR, swaps every pt_interval MCS):
+ *   - `windows` windows over train_mcs MCS, the first on the plain ladder;
+ *     each measures the pair swap acceptance a_k, which gives the length
+ *     density sqrt(C)/T = 2 erfcinv(a_k)/|dT_k| on a 64-point map of
+ *     [T_lo, T_hi] (an interval with no accepted swap bounds it from below),
+ *     and <e> per slot; with learn != 0 the ladder is then re-placed at equal
+ *     steps of int sqrt(C)/T dT, by the MLP (placement 0: ether_mlp_init with
+ *     mlp_seed, then ether_mlp_train on <e> and log C of the map, 4000 Adam
+ *     steps, and ether_ladder_place) or straight from the map (placement 1),
+ *     and the slots are retuned in place (ether_retune_betas);
+ *   - with validate_mcs > 0, three windows of validate_mcs MCS: plain,
+ *     learned, plain.  The learned ladder is kept only if its round-trip cost
+ *     Lambda = sum_k (1 - a_k)/a_k is below the plain ladder's and its lowest
+ *     pair acceptance above it (plain: the mean of its two windows);
+ *     otherwise the slots return to the plain ladder.
+ * learn = 0 runs the same windows on the plain ladder only.  Replicas keep
+ * their state and slot throughout; the swap counters are reset at the end.
+ * T_out[n_temps] = the ladder the slots hold for production, hottest first;
+ * swap_count in/out as in ether_run; report[7] (nullable) = {Lambda plain,
+ * Lambda learned, lowest acceptance plain, lowest acceptance learned (all
+ * from validation), learned kept (0/1), last MLP loss (NaN if none), MCS run}.
+ * The production run continues at first_step + report[6]. */
+int ether_ladder_train(ether_ctx* ctx, int32_t n_temps, const double* temps, double kb,
+                       int32_t first_step, int32_t train_mcs, int32_t windows,
+                       int32_t validate_mcs, int32_t coa, int32_t ovrr_interval,
+                       double ovrr_passes, int32_t pt_interval, int32_t learn,
+                       int32_t placement, uint64_t mlp_seed, int32_t* swap_count,
+                       double* T_out, double* report);
 
 #ifdef __cplusplus
 }

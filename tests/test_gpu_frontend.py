@@ -1,4 +1,4 @@
-"""gpu.ether settings, the learned ladder in a driver run, and batch manifests.
+"""gpu.ether settings, the learned ladder in both drivers, and batch manifests.
 
 Launch shape, lanes per site, graph replay and chunking change no result, so
 a run directory with such a gpu.ether must give bit-identical moments to the
@@ -65,6 +65,21 @@ def test_learned_ladder_through_the_driver(tmp_path):
     # the ETHER.obs temperatures are the ladder that ran production
     m = read_directory(str(tmp_path))
     assert np.allclose(_obs(str(tmp_path), m)[:, 0], T, atol=5e-6)
+    # the Fortran driver runs the same training (ether_ladder_train) and
+    # validation, so it writes the same ladder and numbers
+    fdir = tmp_path / "fortran"
+    workloads.write_dir("pyro_pt_16", str(fdir), sc="2 2 2", mcs="6000 3000", repeat="2",
+                        temp="20 1 1.2666666666666666", pt=".T. 3")
+    (fdir / "gpu.ether").write_text("ladder = learned 2000 4\n")
+    say("learned ladder via the Fortran driver")
+    r = subprocess.run([DRIVER], cwd=fdir, capture_output=True, text=True, timeout=600)
+    say(r.stdout[-300:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "learned tempering ladder" in r.stdout
+    for f in ("energy.dat", "magnetization.dat"):
+        a = np.array([[float(x) for x in ln.split()] for ln in open(tmp_path / f) if not ln.lstrip().startswith("#")])
+        b = np.array([[float(x) for x in ln.split()] for ln in open(fdir / f) if not ln.lstrip().startswith("#")])
+        assert a.shape == b.shape and np.allclose(a, b, rtol=2e-5, atol=2e-5), f
 
 
 def test_batch_manifest_equals_runs_alone(tmp_path):

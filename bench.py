@@ -223,15 +223,16 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
     # the schedule's period: whole periods are replayed as captured graphs
     eq = t.tmcs_eq
     period = coa if not ovrr_iv else coa * ovrr_iv // math.gcd(coa, ovrr_iv)
+    # the timed window is K production MCS (samples included) from the period
+    # boundary after the warmup; every graph it replays is captured and
+    # uploaded first, so that the warmup runs right before the clock starts
+    first = eq + 1 + period * -(-warmup // period)
+    eng.run_prepare(first, steps, eq, coa, ovrr_iv, ovrr_p)
     # warmup: the start state, fresh_spins, then W untimed production MCS
     eng.set_spins(-1, m.initial_spins, m.initial_phi)
     eng.init_spins(E.INIT_FRESH)
     if warmup > 0:
         eng.run(eq + 1, warmup, eq, coa, ovrr_iv, ovrr_p)
-    # timed window: K production MCS (samples included) from the next period
-    # boundary; every graph they replay is captured and uploaded beforehand
-    first = eq + 1 + period * -(-warmup // period)
-    eng.run_prepare(first, steps, eq, coa, ovrr_iv, ovrr_p)
     eng.reset_moments()
     eng.synchronize()
     t0 = time.perf_counter()

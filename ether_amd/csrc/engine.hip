@@ -33,6 +33,7 @@ constexpr long long kMaxGraphPeriod = 1024;
 
 struct ether_ctx {
     int model = 0, N = 0, R = 0;
+    int device = 0;       // the HIP device current at ether_ctx_create
     uint64_t seed = 0;
     std::string err;
     // host-side model (module init)
@@ -100,6 +101,7 @@ struct ether_ctx {
     float* d_phi = nullptr;
     int8_t* d_s = nullptr;
     int* d_slot_of = nullptr;
+    int* d_site_of_slot = nullptr;   // site of each slot (the structure factor's line form)
     int* d_rep_at = nullptr;
     float* d_beta_f = nullptr;
     double* d_beta = nullptr;
@@ -240,6 +242,7 @@ extern "C" int ether_ctx_create(int32_t model, int32_t n_sites, int32_t n_replic
                     std::string("ether_ctx_create: device is ") + prop.gcnArchName +
                         ", this library is built for gfx950 only");
     ether_ctx* c = new ether_ctx();
+    c->device = dev;
     c->model = model;
     c->N = n_sites;
     c->R = n_replicas;
@@ -796,6 +799,7 @@ extern "C" int ether_commit(ether_ctx* c) {
         TRY(upload(c, c->d_sz, ones.data(), RN));
     }
     TRY(dev_alloc(c, &c->d_slot_of, (size_t)R));
+    TRY(dev_alloc(c, &c->d_site_of_slot, (size_t)N));
     TRY(dev_alloc(c, &c->d_rep_at, (size_t)R));
     TRY(dev_alloc(c, &c->d_beta_f, (size_t)R));
     TRY(dev_alloc(c, &c->d_beta, (size_t)R));
@@ -843,6 +847,7 @@ extern "C" int ether_commit(ether_ctx* c) {
     if (!ell_m.empty()) TRY(upload(c, c->d_ell_m, ell_m.data(), ell_m.size()));
     TRY(upload(c, c->d_ell_m_n, ell_m_n.data(), ell_m_n.size()));
     TRY(upload(c, c->d_sp, sp.data(), sp.size()));
+    TRY(upload(c, c->d_site_of_slot, c->site_of_slot.data(), (size_t)N));
     TRY(upload(c, c->d_w, w.data(), w.size()));
     TRY(upload(c, c->d_jclass, c->jclass.data(), c->jclass.size()));
     TRY(upload(c, c->d_zvec, zvec.data(), zvec.size()));
@@ -2288,4 +2293,5 @@ const float* ctx_spin_plane(ether_ctx* c, int comp) {
 }
 const int8_t* ctx_ising(ether_ctx* c) { return c->d_s; }
 const int* ctx_slot_of_site(ether_ctx* c) { return c->slot_of_site.data(); }
+const int* ctx_site_of_slot_dev(ether_ctx* c) { return c->d_site_of_slot; }
 }  // namespace ether

@@ -367,8 +367,32 @@ int ether_local_fields(ether_ctx* ctx, int32_t replica, double* h, double* e);
  * evaluated as per-axis DFT contractions on the matrix cores (split-BF16
  * operands, FP32 accumulation).  out[R*Q] = sum_a |S_a(q)|^2 / N^2 per
  * replica, Q = N = L1*L2*L3 (one site per cell), q index m1 fastest.
+ * Keeps its intermediate planes and tables between calls.
  * ------------------------------------------------------------------------- */
 int ether_structure_factor(ether_ctx* ctx, const int32_t* sc, float* out);
+
+/* The same grid for a supercell of L1 x L2 x L3 cells with n_basis sites per
+ * cell (N = L1*L2*L3*n_basis, sites in the supercell order, basis site
+ * innermost): frac[3*n_basis] are the basis sites' fractional coordinates
+ * (r_l = sum_k frac[3l+k] a_k), so that q.r = 2*pi*sum_k m_k (n_k + frac_lk)/L_k.
+ * out[R*Q], Q = L1*L2*L3 grid points, m1 fastest: sum_a |S_a(q)|^2 / N^2. */
+int ether_structure_factor_basis(ether_ctx* ctx, const int32_t* sc, int32_t n_basis,
+                                 const double* frac, float* out);
+
+/* The reference's own S(q) (utility/Structure_Factor/structure_factor.f90:116-177):
+ * q_m = q_start + m*dq for m = 0..n_q-1, with n_q = |nint((q_stop - q_start)/dq)| + 1
+ * (checked), and S_a(q_m) = sum_i S_a(r_i) exp(i q_m k_vec.r_i) over every site,
+ * where k_vec = b1 + b2 + b3 with b the reciprocal vectors (2 pi included) of
+ * the lattice vectors as the utility scales them (:95-116) and pos[3N] the
+ * Cartesian site positions in site order (gss.dat's ion(4:6)).
+ * out[R*n_q*4], per replica and q: {sum_a |S_a|^2 / N^2, |S_x|/N, |S_y|/N,
+ * |S_z|/N} (:175-177; Ising spins are (0, 0, +-1)).  The phases are formed in
+ * FP64 and reduced to [-pi, pi] before their FP32 sine and cosine; the
+ * contraction is split-BF16 on the matrix cores with FP32 accumulation.
+ * Keeps its device buffers (positions, k_vec.r, output) between calls. */
+int ether_structure_factor_line(ether_ctx* ctx, const double* pos, const double* k_vec,
+                                double q_start, double q_stop, double dq, int32_t n_q, float* out);
+
 
 /* ---------------------------------------------------------------------------
  * Learned tempering ladder.  No reference counterpart: the nearest is the

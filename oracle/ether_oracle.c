@@ -719,6 +719,28 @@ double o_mlp_loss_grad(const double* p, int32_t n, const double* x, const double
 /* ------------------------------------------------------------------------ */
 /* S(q): utility/Structure_Factor/structure_factor.f90:150-177              */
 /* ------------------------------------------------------------------------ */
+/* S(q) with the per-component moduli of the reference's output line
+ * (utility/Structure_Factor/structure_factor.f90:152-177): out4 = {sum_a
+ * |S_a(q)|^2 / N^2, |S_x(q)|/N, |S_y(q)|/N, |S_z(q)|/N}. */
+void o_structure_factor_parts(int32_t n, const double* pos, const double* S, const double* q3, double* out4) {
+    double re[3] = {0, 0, 0}, im[3] = {0, 0, 0};
+    for (int32_t i = 0; i < n; ++i) {
+        double ph = q3[0] * pos[3 * i] + q3[1] * pos[3 * i + 1] + q3[2] * pos[3 * i + 2];
+        double c = cos(ph), s = sin(ph);
+        for (int a = 0; a < 3; ++a) {
+            re[a] += S[3 * i + a] * c;
+            im[a] += S[3 * i + a] * s;
+        }
+    }
+    out4[0] = 0.0;
+    for (int a = 0; a < 3; ++a) {
+        double m2 = re[a] * re[a] + im[a] * im[a];
+        out4[0] += m2;
+        out4[1 + a] = sqrt(m2) / (double)n;
+    }
+    out4[0] /= (double)n * (double)n;
+}
+
 double o_structure_factor(int32_t n, const double* pos, const double* S, const double* q3) {
     double re[3] = {0, 0, 0}, im[3] = {0, 0, 0};
     for (int32_t i = 0; i < n; ++i) {

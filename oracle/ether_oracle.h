@@ -185,6 +185,9 @@ double o_mlp_loss_grad(const double* p, int32_t n, const double* x, const double
 /* S(q) direct sum (utility/Structure_Factor/structure_factor.f90:150-177) at
  * arbitrary q (Cartesian, 1/Angstrom): returns sum_a |S_a(q)|^2 / N^2 */
 double o_structure_factor(int32_t n, const double* pos, const double* S, const double* q3);
+/* the reference's whole output line (:175-177): out4 = {sum_a |S_a|^2/N^2,
+ * |S_x|/N, |S_y|/N, |S_z|/N} */
+void o_structure_factor_parts(int32_t n, const double* pos, const double* S, const double* q3, double* out4);
 
 #ifdef __cplusplus
 }

@@ -90,6 +90,7 @@ def lib():
         L.o_mlp_loss_grad.argtypes = [_vp, C.c_int32, _vp, _vp, _vp, _vp]
         L.o_structure_factor.restype = C.c_double
         L.o_structure_factor.argtypes = [C.c_int32, _vp, _vp, _vp]
+        L.o_structure_factor_parts.argtypes = [C.c_int32, _vp, _vp, _vp, _vp]
         _lib = L
     return _lib
 
@@ -347,3 +348,13 @@ def structure_factor(pos, S, q):
     S = _a(S, np.float64)
     q = _a(q, np.float64)
     return lib().o_structure_factor(pos.shape[0], _p(pos), _p(S), _p(q))
+
+
+def structure_factor_parts(pos, S, q):
+    """{sum_a |S_a(q)|^2/N^2, |S_x|/N, |S_y|/N, |S_z|/N} by the FP64 direct sum."""
+    p = _a(pos, np.float64).reshape(-1, 3)
+    s = _a(S, np.float64).reshape(-1, 3)
+    qq = _a(q, np.float64)
+    out = np.zeros(4)
+    lib().o_structure_factor_parts(p.shape[0], _p(p), _p(s), _p(qq), _p(out))
+    return out

@@ -315,6 +315,30 @@ def test_structure_factor_closed_forms():
     assert o.structure_factor(pos, S, [0, 0, 0]) == pytest.approx(0.0, abs=1e-24)
 
 
+def test_structure_factor_parts_closed_forms():
+    """The reference's whole output line (structure_factor.f90:175-177):
+    an FM state along x gives {1, 1, 0, 0} at q = 0; a planar spiral
+    S = (cos Q.r, sin Q.r, 0) gives |S_x|/N = |S_y|/N = 1/2 and total 1/2 at
+    q = Q, and total 0 at q = 2Q; the first entry is o_structure_factor."""
+    L = 8
+    pos = np.array([[i, j, k] for k in range(4) for j in range(4) for i in range(L)], float)
+    n = len(pos)
+    S = np.tile([1.0, 0.0, 0.0], (n, 1))
+    assert np.allclose(o.structure_factor_parts(pos, S, [0, 0, 0]), [1, 1, 0, 0], atol=1e-14)
+    Q = np.array([2 * np.pi * 3 / L, 0, 0])
+    ph = pos @ Q
+    S = np.stack([np.cos(ph), np.sin(ph), np.zeros(n)], axis=1)
+    got = o.structure_factor_parts(pos, S, Q)
+    assert np.allclose(got, [0.5, 0.5, 0.5, 0.0], atol=1e-12)
+    assert np.allclose(o.structure_factor_parts(pos, S, 2 * Q), 0.0, atol=1e-12)
+    rng = np.random.default_rng(3)
+    S = random_spins(rng, n)
+    q = rng.normal(size=3)
+    got = o.structure_factor_parts(pos, S, q)
+    assert got[0] == pytest.approx(o.structure_factor(pos, S, q), rel=1e-13)
+    assert got[0] == pytest.approx(float((got[1:] ** 2).sum()), rel=1e-12)
+
+
 def test_ising_bond_sums_against_pair_count_from_positions():
     """o_ising_bond_sums against a count written from positions: two species,
     four shells, classes = (shell, species pair); every ordered pair (i, j),

@@ -24,6 +24,7 @@
 namespace ether {
 void set_global_error(const std::string& s);
 const std::string& global_error();
+int sq_scratch_setup(int device, std::string& err);   // sq.hip
 }  // namespace ether
 
 using namespace ether;
@@ -241,6 +242,13 @@ extern "C" int ether_ctx_create(int32_t model, int32_t n_sites, int32_t n_replic
         return fail(nullptr, ETHER_ERR_NODEV,
                     std::string("ether_ctx_create: device is ") + prop.gcnArchName +
                         ", this library is built for gfx950 only");
+    {
+        // the device array of the allocation-free structure factor: its
+        // address on this device, looked up once
+        std::string err;
+        const int rc = sq_scratch_setup(dev, err);
+        if (rc != ETHER_OK) return fail(nullptr, rc, "ether_ctx_create: " + err);
+    }
     ether_ctx* c = new ether_ctx();
     c->device = dev;
     c->model = model;
@@ -2294,4 +2302,5 @@ const float* ctx_spin_plane(ether_ctx* c, int comp) {
 const int8_t* ctx_ising(ether_ctx* c) { return c->d_s; }
 const int* ctx_slot_of_site(ether_ctx* c) { return c->slot_of_site.data(); }
 const int* ctx_site_of_slot_dev(ether_ctx* c) { return c->d_site_of_slot; }
+int ctx_device(ether_ctx* c) { return c->device; }
 }  // namespace ether

@@ -40,6 +40,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -60,6 +62,7 @@ const float* ctx_spin_plane(ether_ctx* c, int comp);
 const int8_t* ctx_ising(ether_ctx* c);
 const int* ctx_slot_of_site(ether_ctx* c);
 const int* ctx_site_of_slot_dev(ether_ctx* c);
+int ctx_device(ether_ctx* c);
 }  // namespace ether
 
 using namespace ether;
@@ -730,4 +733,71 @@ extern "C" int ether_structure_factor_line(ether_ctx* c, const double* pos, cons
         s->lout_n = nout;
     }
     return sq_line_run(c, pos, k_vec, q_start, dq, n_q, s->d_pos, s->d_p, s->d_lout, out);
+}
+
+// ---------------------------------------------------------------------------
+// The line form without allocation.  Its device buffers (positions, k.r,
+// output staging) are carved from one static device array of each GPU, whose
+// address is looked up once per device when a context is created there.
+// Calls on one GPU share the array: each call fills what it reads and
+// synchronises before returning, and calls that overlap on one GPU (from two
+// host threads) race.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch_ptr;   // device -> address of its copy of `scratch`
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+}  // namespace
+
+namespace ether {
+// Looks up (once) the address of `scratch` on `device`, with that device current.
+int sq_scratch_setup(int device, std::string& err) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr.count(device)) return ETHER_OK;
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) {
+        err = "hipGetDevice failed";
+        return ETHER_ERR_HIP;
+    }
+    if (cur != device && hipSetDevice(device) != hipSuccess) {
+        err = "hipSetDevice failed";
+        return ETHER_ERR_HIP;
+    }
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p) {
+        err = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+        return ETHER_ERR_HIP;
+    }
+    g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+    return ETHER_OK;
+}
+}  // namespace ether
+
+extern "C" int ether_structure_factor_line_noalloc(ether_ctx* c, const double* pos, const double* k_vec,
+                                                   double q_start, double q_stop, double dq, int32_t n_q,
+                                                   float* out) {
+    int rc = sq_line_check(c, pos, k_vec, q_start, q_stop, dq, n_q, out);
+    if (rc) return rc;
+    const size_t N = (size_t)ctx_N(c), R = (size_t)ctx_R(c);
+    const size_t off_p = align256(sizeof(double) * 3 * N);
+    const size_t off_l = off_p + align256(sizeof(double) * N);
+    const size_t need = off_l + sizeof(float) * R * (size_t)n_q * 4;
+    if (need > kScratchBytes)
+        return ctx_fail(c, ETHER_ERR_LIMIT, "ether_structure_factor_line_noalloc: needs " + std::to_string(need) +
+                                                " bytes of device scratch (32 N + 16 R n_q), more than the " +
+                                                std::to_string(kScratchBytes) + " it has");
+    unsigned char* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        auto it = g_scratch_ptr.find(ctx_device(c));
+        if (it != g_scratch_ptr.end()) base = it->second;
+    }
+    if (!base) return ctx_fail(c, ETHER_ERR_STATE, "ether_structure_factor_line_noalloc: no scratch on this device");
+    return sq_line_run(c, pos, k_vec, q_start, dq, n_q, reinterpret_cast<double*>(base),
+                       reinterpret_cast<double*>(base + off_p), reinterpret_cast<float*>(base + off_l), out);
 }

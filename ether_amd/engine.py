@@ -96,6 +96,8 @@ EXPORTS = [
     ("ether_structure_factor_basis", C.c_int, [_vp, _vp, C.c_int32, _vp, _vp]),
     ("ether_structure_factor_line", C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_double,
                                               C.c_int32, _vp]),
+    ("ether_structure_factor_line_noalloc", C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_double,
+                                                      C.c_int32, _vp]),
     ("ether_mlp_init", C.c_int, [_vp, C.c_uint64]),
     ("ether_mlp_set_params", C.c_int, [_vp, _vp]),
     ("ether_mlp_get_params", C.c_int, [_vp, _vp]),
@@ -431,15 +433,16 @@ class Engine:
         self._c(self.lib.ether_structure_factor_basis(self.h, _ptr(s), f.shape[0], _ptr(f), _ptr(out)))
         return out.reshape(self.R, Q)
 
-    def structure_factor_line(self, pos, k_vec, q_start, q_stop, dq) -> np.ndarray:
-        """The reference's q-line S(q): (R, n_q, 4) = {sum_a |S_a|^2/N^2, |S_x|/N, |S_y|/N, |S_z|/N}."""
+    def structure_factor_line(self, pos, k_vec, q_start, q_stop, dq, noalloc=False) -> np.ndarray:
+        """The reference's q-line S(q): (R, n_q, 4) = {sum_a |S_a|^2/N^2, |S_x|/N, |S_y|/N, |S_z|/N};
+        noalloc=True calls the allocation-free entry point."""
         p = _arr(pos, np.float64).reshape(-1, 3)
         k = _arr(k_vec, np.float64)
         x = (q_stop - q_start) / dq
         n_q = abs(int(np.floor(x + 0.5)) if x >= 0 else -int(np.floor(-x + 0.5))) + 1
         out = np.zeros(self.R * n_q * 4, np.float32)
-        self._c(self.lib.ether_structure_factor_line(self.h, _ptr(p), _ptr(k), float(q_start), float(q_stop),
-                                                     float(dq), int(n_q), _ptr(out)))
+        fn = self.lib.ether_structure_factor_line_noalloc if noalloc else self.lib.ether_structure_factor_line
+        self._c(fn(self.h, _ptr(p), _ptr(k), float(q_start), float(q_stop), float(dq), int(n_q), _ptr(out)))
         return out.reshape(self.R, n_q, 4)
 
     # learned ladder

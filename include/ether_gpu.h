@@ -393,6 +393,19 @@ int ether_structure_factor_basis(ether_ctx* ctx, const int32_t* sc, int32_t n_ba
 int ether_structure_factor_line(ether_ctx* ctx, const double* pos, const double* k_vec,
                                 double q_start, double q_stop, double dq, int32_t n_q, float* out);
 
+/* ether_structure_factor_line without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, kernels and results as ether_structure_factor_line; its
+ * device buffers (positions, k.r, output staging: 32*N + 16*R*n_q bytes, each
+ * part rounded up to 256 bytes) come from one static 16 MiB device array per
+ * GPU instead of the context, and larger requests are refused with
+ * ETHER_ERR_LIMIT.  Calls on one GPU share that array: each call writes what
+ * it reads before reading it and synchronises the context's stream before it
+ * returns, so calls that overlap on one GPU (from two host threads) race. */
+int ether_structure_factor_line_noalloc(ether_ctx* ctx, const double* pos, const double* k_vec,
+                                        double q_start, double q_stop, double dq, int32_t n_q,
+                                        float* out);
 
 /* ---------------------------------------------------------------------------
  * Learned tempering ladder.  No reference counterpart: the nearest is the

@@ -255,12 +255,23 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
     ovrr_ms, ovrr_n = eng.timing(E.TIME_OVRR)
     meas_ms, meas_n = eng.timing(E.TIME_MEASURE)
     smeas_ms, smeas_n = eng.timing(E.TIME_SWEEP_MEAS)
+    res_ms, res_n = eng.timing(E.TIME_RESIDENT)
     eng.set_timing(False)
 
     nc, col, _ = eng.layout()
-    kernel = "k_ising_sweep" if m.model_id == 0 else "k_xyz_sweep"
-    per_launch_updates = R * m.n_sites / nc
-    avg_ms = sweep_ms / max(sweep_n, 1)
+    if res_n > 0:
+        # Ising with the lattice resident in LDS: one launch runs the sweeps
+        # between two samples; the unit is one sweep of every replica
+        kernel = "k_ising_resident"
+        unit = "one sweep (all colour phases) inside a resident launch"
+        per_launch_updates = R * m.n_sites
+        avg_ms = res_ms / res_n
+        sweep_ms, sweep_n = res_ms, res_n
+    else:
+        kernel = "k_ising_sweep" if m.model_id == 0 else "k_xyz_sweep"
+        unit = "one colour phase"
+        per_launch_updates = R * m.n_sites / nc
+        avg_ms = sweep_ms / max(sweep_n, 1)
     b_upd = B_UPD[m.model_id] + (8.0 if (m.model_id == 1 and t.angle) else 0.0)
     achieved = b_upd * per_launch_updates / (avg_ms * 1e-3) / 1e9
     out = {
@@ -285,13 +296,16 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(kernel, workload, spw),
-            "kernel": kernel + " (one colour phase)",
+            "kernel": f"{kernel} ({unit})",
             "bytes_per_update": b_upd, "updates_per_launch": per_launch_updates,
             "avg_launch_ms": avg_ms, "launches": sweep_n,
-            "timing": f"HIP events on the engine stream around the back-to-back colour phases of "
-                      f"every sweep over MCS {first + steps}..{first + steps + event_steps - 1} of the same run "
-                      f"(launched one by one; {wall_ev / event_steps * 1e3:.4f} ms per MCS there), "
-                      f"divided by the phases they cover",
+            "timing": (f"HIP events on the engine stream around the back-to-back colour phases of "
+                       f"every sweep over MCS {first + steps}..{first + steps + event_steps - 1} of the same run "
+                       f"(launched one by one; {wall_ev / event_steps * 1e3:.4f} ms per MCS there), "
+                       f"divided by the phases they cover") if res_n == 0 else
+                      (f"HIP events on the engine stream around every resident launch over MCS "
+                       f"{first + steps}..{first + steps + event_steps - 1} of the same run "
+                       f"({wall_ev / event_steps * 1e3:.4f} ms per MCS there), divided by the sweeps they cover"),
         },
         "launch": {"sites_per_wave": spw, "xcd_map": xcd,
                    "autotune_ms_per_sweep": None if tune_ms is None else

@@ -69,6 +69,12 @@ struct ether_ctx {
     int lps_shift = 6;    // lanes per site of the sweep (1 << lps_shift); set at commit
     int lanes_req = 0;    // ether_set_lanes (0: chosen from R)
     int meas_tiles = 0, meas_sites = 0, meas_rec = 0;
+    // Ising sweeps with the lattice resident in LDS (k_ising_resident):
+    // possible when a word of every site plus the thresholds fit the budget
+    int resident = 1;       // ether_set_resident
+    bool res_ok = false;
+    int res_block = 256, res_thr_lds = 0;
+    size_t res_lds = 0;
     // spiral-state partners (ether_set_spiral): site-space rows, -1 padded
     int spiral_cap = 0;
     double spiral_count = 0.0;
@@ -773,6 +779,18 @@ extern "C" int ether_commit(ether_ctx* c) {
     }
     c->meas_tiles = (N + c->meas_sites - 1) / c->meas_sites;
     c->meas_rec = ising ? (c->n_classes + nsp + 1) : (c->spiral_cap ? 9 : 6);
+    // resident Ising sweeps: one word (4 replicas) of every site in LDS, and
+    // the four replicas' threshold tables when they fit too
+    c->res_ok = false;
+    if (ising && c->ncol <= kMaxResColours && (size_t)N * 4 <= kResidentLds) {
+        c->res_ok = true;
+        c->res_thr_lds = ((size_t)N * 4 + (size_t)16 * c->ising_table <= kResidentLds) ? 1 : 0;
+        c->res_lds = (size_t)N * 4 + (c->res_thr_lds ? (size_t)16 * c->ising_table : 0);
+        int maxcol = 0;
+        for (int q = 0; q < c->ncol; ++q) maxcol = std::max(maxcol, c->cbeg[q + 1] - c->cbeg[q]);
+        // about two sites of the largest colour per thread, 64 to 1024 threads
+        c->res_block = std::max(64, std::min(1024, (maxcol / 2 + 63) / 64 * 64));
+    }
     // four zero entries past the last row: a 4-wide chunk of the last row stays in the table
     ell.resize(ell.size() + 4, 0u);
     if (!ell_m.empty()) ell_m.resize(ell_m.size() + 4, 0u);
@@ -1186,6 +1204,9 @@ static bool phase_empty(const ether_ctx* c, int q) {
     return q >= c->ncol || c->cbeg[q + 1] == c->cbeg[q];
 }
 
+static bool resident_path(const ether_ctx* c);
+static int enqueue_resident(ether_ctx* c, int n_sweeps, bool meas);
+
 // meas: the phases also book the fused measurement that the next
 // enqueue_measure(..., fused = true) completes.
 static int enqueue_sweep(ether_ctx* c, double fuse_frac = 0.0, bool meas = false) {
@@ -1253,9 +1274,13 @@ static int enqueue_ovrr(ether_ctx* c, double passes) {
 // fused: the record was booked by the sweep just enqueued (enqueue_sweep with
 // meas); only the finalize runs.  While a batch records, an empty measurement
 // launch keeps the launch sequence aligned with lattices that measure apart.
-static int enqueue_measure(ether_ctx* c, int sample, bool fused = false) {
+// booked: what precedes the finalize.  kBookedPass: nothing (a measurement
+// pass runs here); kBookedXyz: the sweep's fixed-point sums (d_macc);
+// kBookedResident: the resident Ising kernel's record in tile 0.
+enum Booked : int { kBookedPass = 0, kBookedXyz = 1, kBookedResident = 2 };
+static int enqueue_measure(ether_ctx* c, int sample, int booked = kBookedPass) {
     const int ev = ev_begin(c, ETHER_TIME_MEASURE);
-    if (!fused) {
+    if (booked == kBookedPass) {
         launch_measure(meas_args(c), c->model, c->zee, c->sia, c->stream);
     } else if (g_recorder) {
         MeasArgs m = meas_args(c);
@@ -1264,7 +1289,7 @@ static int enqueue_measure(ether_ctx* c, int sample, bool fused = false) {
     }
     FinArgs f;
     f.partial = c->d_partial;
-    f.tiles = c->meas_tiles;
+    f.tiles = booked == kBookedResident ? 1 : c->meas_tiles;
     f.rec = c->meas_rec;
     f.Rp = c->Rp;
     f.R = c->R;
@@ -1285,7 +1310,7 @@ static int enqueue_measure(ether_ctx* c, int sample, bool fused = false) {
     f.spiral_count = c->spiral_cap ? c->spiral_count : 0.0;
     f.psi = c->d_psi;
     f.mom_sp = c->d_mom_sp;
-    f.macc = fused ? c->d_macc : nullptr;
+    f.macc = booked == kBookedXyz ? c->d_macc : nullptr;
     for (int k = 0; k < 4; ++k) f.mscale[k] = c->mscale[k];
     launch_finalize(f, c->stream);
     ev_end(c, ev);
@@ -1293,9 +1318,9 @@ static int enqueue_measure(ether_ctx* c, int sample, bool fused = false) {
     return ETHER_OK;
 }
 
-static int enqueue_pt(ether_ctx* c, int parity, bool fused = false) {
+static int enqueue_pt(ether_ctx* c, int parity, int booked = kBookedPass) {
     if (c->n_groups <= 0) return ETHER_OK;
-    TRY(enqueue_measure(c, 0, fused));
+    TRY(enqueue_measure(c, 0, booked));
     PtArgs p;
     p.n_groups = c->n_groups;
     p.gstart = c->d_gstart;
@@ -1350,9 +1375,20 @@ extern "C" int ether_sweep(ether_ctx* c, int32_t n_mcs, double* accept) {
     TRY(need_betas(c));
     if (n_mcs < 0) return fail(c, ETHER_ERR_ARG, "ether_sweep: negative n_mcs");
     std::vector<unsigned long long> before;
-    for (int k = 0; k < n_mcs; ++k) {
-        if (accept && k == n_mcs - 1) TRY(copy_acc_async(c, before));
-        TRY(enqueue_sweep(c));
+    if (resident_path(c)) {
+        // all but the last sweep in one launch, the last one on its own when
+        // its acceptance is asked for
+        const int head = accept ? n_mcs - 1 : n_mcs;
+        if (head > 0) TRY(enqueue_resident(c, head, false));
+        if (accept && n_mcs > 0) {
+            TRY(copy_acc_async(c, before));
+            TRY(enqueue_resident(c, 1, false));
+        }
+    } else {
+        for (int k = 0; k < n_mcs; ++k) {
+            if (accept && k == n_mcs - 1) TRY(copy_acc_async(c, before));
+            TRY(enqueue_sweep(c));
+        }
     }
     if (accept && n_mcs > 0) {
         std::vector<unsigned long long> after;
@@ -1645,10 +1681,81 @@ static int enqueue_step(ether_ctx* c, int32_t stepi, int32_t mcs_eq, int32_t coa
     // src/ether.f90:189-198
     if (pt) {
         ++*sc;
-        TRY(enqueue_pt(c, *sc % 2, fm));
+        TRY(enqueue_pt(c, *sc % 2, fm ? kBookedXyz : kBookedPass));
     }
     // src/ether.f90:201 -> src/observable.f90:30-31
-    if (sample) TRY(enqueue_measure(c, 1, fm));
+    if (sample) TRY(enqueue_measure(c, 1, fm ? kBookedXyz : kBookedPass));
+    return ETHER_OK;
+}
+
+// The resident Ising path: n_sweeps sweeps in one launch, the last one
+// booking the measurement of the state it leaves when meas is set.
+static bool resident_path(const ether_ctx* c) {
+    return c->resident && c->res_ok && !g_recorder;
+}
+
+static int enqueue_resident(ether_ctx* c, int n_sweeps, bool meas) {
+    IsingResArgs a;
+    a.s = c->d_s;
+    a.ell = c->d_ell;
+    a.D = c->D;
+    a.sp = c->d_sp;
+    a.w = c->d_w;
+    a.N = c->N;
+    a.R = c->R;
+    a.Rp = c->Rp;
+    a.ncol = c->ncol;
+    for (int q = 0; q <= kMaxResColours; ++q) a.cbeg[q] = c->cbeg[std::min(q, c->ncol)];
+    a.slot_of = c->d_slot_of;
+    a.seed = c->seed;
+    a.ctr = c->d_ctr;
+    a.step_off = c->step - (c->capturing ? c->cap_step0 : c->dev_step);
+    a.n_sweeps = n_sweeps;
+    a.acc = c->d_acc;
+    a.n_classes = c->n_classes;
+    for (int k = 0; k <= kMaxIsingClasses; ++k) a.stride[k] = k <= c->n_classes ? c->ising_stride[k] : 0;
+    a.offset = c->ising_offset;
+    a.table_per_slot = c->ising_table;
+    a.P = c->ising_P;
+    a.table_in_lds = c->res_thr_lds;
+    a.thr = c->d_thr;
+    a.meas = meas ? 1 : 0;
+    a.nsp = c->nsp;
+    a.rec = c->meas_rec;
+    a.partial = c->d_partial;
+    const int ev = ev_begin(c, ETHER_TIME_RESIDENT, n_sweeps);
+    launch_ising_resident(a, c->res_block, c->res_lds, c->stream);
+    ev_end(c, ev);
+    HIPCHK(c, hipGetLastError());
+    c->step += (uint32_t)n_sweeps;
+    return ETHER_OK;
+}
+
+// Steps first..first+count-1 of the schedule.  On the resident path the
+// sweeps between two measurements (a swap or a sample) are one launch, which
+// books the measurement the next swap or sample reads.
+static int enqueue_steps(ether_ctx* c, int32_t first, int32_t count, int32_t mcs_eq, int32_t coa,
+                         int ovrr_iv, double ovrr_passes, int pt_iv, int* sc) {
+    if (!resident_path(c)) {
+        for (int32_t k = 0; k < count; ++k)
+            TRY(enqueue_step(c, first + k, mcs_eq, coa, ovrr_iv, ovrr_passes, pt_iv, sc));
+        return ETHER_OK;
+    }
+    int pending = 0;
+    for (int32_t stepi = first; stepi < first + count; ++stepi) {
+        const bool pt = pt_iv && c->n_groups > 0 && stepi % pt_iv == 0 && stepi % coa != 0;
+        const bool sample = stepi > mcs_eq && stepi % coa == 0;
+        ++pending;
+        if (!pt && !sample) continue;
+        TRY(enqueue_resident(c, pending, true));
+        pending = 0;
+        if (pt) {
+            ++*sc;
+            TRY(enqueue_pt(c, *sc % 2, kBookedResident));
+        }
+        if (sample) TRY(enqueue_measure(c, 1, kBookedResident));
+    }
+    if (pending) TRY(enqueue_resident(c, pending, false));
     return ETHER_OK;
 }
 
@@ -1682,12 +1789,11 @@ static int graph_for(ether_ctx* c, int phase, int P, int coa, int ovrr_iv, doubl
     c->cap_pt0 = pt0;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = ETHER_OK, n_pt = 0, sc_local = 0;
-    for (int k = 1; k <= P && rc == ETHER_OK; ++k) {
+    {
         // position k in the period stands for every stepi = k (mod P)
         const int32_t eq = phase ? 0 : P + 1;
-        const int before = sc_local;
-        rc = enqueue_step(c, k, eq, coa, ovrr_iv, ovrr_passes, pt_iv, &sc_local);
-        n_pt += sc_local - before;
+        rc = enqueue_steps(c, 1, P, eq, coa, ovrr_iv, ovrr_passes, pt_iv, &sc_local);
+        n_pt = sc_local;
     }
     if (rc == ETHER_OK) launch_advance_ctr(c->d_ctr, (uint32_t)P, (uint32_t)n_pt, c->stream);
     hipGraph_t graph = nullptr;
@@ -1761,8 +1867,11 @@ extern "C" int ether_run(ether_ctx* c, int32_t first_step, int32_t n_steps, int3
                 continue;
             }
         }
-        TRY(enqueue_step(c, stepi, mcs_eq, coa, ovrr_iv, ovrr_passes, pt_iv, &sc));
-        ++stepi;
+        // launched one by one up to the next period boundary
+        int32_t stop = stepi + 1;
+        while (stop <= last && (stop - 1) % P != 0) ++stop;
+        TRY(enqueue_steps(c, stepi, stop - stepi, mcs_eq, coa, ovrr_iv, ovrr_passes, pt_iv, &sc));
+        stepi = stop;
     }
     if (swap_count) *swap_count = sc;
     return ETHER_OK;
@@ -1807,6 +1916,13 @@ extern "C" int ether_run_prepare(ether_ctx* c, int32_t first_step, int32_t n_ste
         HIPCHK(c, hipGraphUpload(c->graphs[gi].exec, c->stream));
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ETHER_OK;
+}
+
+extern "C" int ether_set_resident(ether_ctx* c, int32_t on) {
+    if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
+    drop_graphs(c);   // captured graphs hold the launches of the path they were captured on
+    c->resident = on ? 1 : 0;
     return ETHER_OK;
 }
 

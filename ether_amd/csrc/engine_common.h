@@ -120,6 +120,42 @@ struct IsingArgs {
     const uint32_t* thr;   // [R slots][nsp][2][P]
 };
 
+// Ising sweeps with a replica word's whole lattice resident in LDS: one
+// workgroup per group of 4 replicas (one 32-bit word of int8 spins per
+// site), n_sweeps sweeps of every colour phase with workgroup barriers
+// between the phases, and optionally the measurement of the final state.
+constexpr int kMaxResColours = 16;
+constexpr size_t kResidentLds = 60 * 1024;   // LDS per workgroup of the resident kernel (dynamic part)
+struct IsingResArgs {
+    int8_t* s;             // [N][Rp]
+    const uint32_t* ell;   // [N][D]
+    int D;
+    const uint8_t* sp;
+    const float* w;        // [N] magnetisation weight per slot
+    int N;
+    int R;
+    int Rp;
+    int ncol;
+    int cbeg[kMaxResColours + 1];
+    const int* slot_of;
+    uint64_t seed;
+    const uint32_t* ctr;
+    uint32_t step_off;     // sweep k of the launch draws with step ctr[0] + step_off + k
+    int n_sweeps;
+    unsigned long long* acc;
+    int n_classes;
+    int stride[kMaxIsingClasses + 1];
+    int offset;
+    int table_per_slot;
+    int P;
+    int table_in_lds;      // the four replicas' tables are staged in LDS
+    const uint32_t* thr;
+    int meas;              // book the final state's measurement record (tile 0 of partial)
+    int nsp;
+    int rec;
+    double* partial;
+};
+
 struct MeasArgs {
     const float* sx;
     const float* sy;
@@ -285,6 +321,7 @@ void launch_ctr_batch(uint32_t* const* d_ctrs, int n, const uint32_t* d_vals, ui
 void launch_xyz_sweep(const XyzArgs& a, bool angle, bool zee, bool sia, hipStream_t st);
 void launch_xyz_ovrr(const XyzArgs& a, bool angle, bool zee, hipStream_t st);
 void launch_ising_sweep(const IsingArgs& a, hipStream_t st);
+void launch_ising_resident(const IsingResArgs& a, int block, size_t lds_bytes, hipStream_t st);
 void launch_measure(const MeasArgs& a, int model, bool zee, bool sia, hipStream_t st);
 void launch_finalize(const FinArgs& a, hipStream_t st);
 void launch_pt_swap(const PtArgs& a, hipStream_t st);

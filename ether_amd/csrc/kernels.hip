@@ -741,6 +741,202 @@ __global__ __launch_bounds__(kBlock) void k_ising_sweep_batch(const IsingArgs* _
 }
 
 // ---------------------------------------------------------------------------
+// Ising sweeps with the lattice resident in LDS.  One workgroup holds one
+// word of four replicas (r0 = 4 * blockIdx.x .. r0 + 3) for every site: the
+// N words are loaded once, n_sweeps sweeps run over the colour classes with a
+// workgroup barrier after each phase (a phase reads only other colours, so
+// the in-place update is race-free), and the words are written back once.
+// Every update is the one k_ising_sweep makes: the same Philox block (slot,
+// word, sweep step), the same per-byte down counts per coupling class and the
+// same integer threshold, so the spins are bit-identical to launching the
+// colour phases one by one.  With meas set, the final state's record (bond
+// sums per class over full rows, spin sums per species, M_z) is written as
+// tile 0 of the measurement partials for k_finalize (integers, exact in FP64).
+// ---------------------------------------------------------------------------
+// class_add with a class that differs between lanes: every class takes a
+// select, so the counters stay in registers (an indexed update would put them
+// in scratch)
+// NC > 0: the context has exactly NC classes (one class needs no select);
+// NC = 0: up to kMaxIsingClasses, bounded by nc
+template <int NC>
+__device__ __forceinline__ void class_add_lane(uint32_t (&d)[kMaxIsingClasses + 1],
+                                               int (&cnt)[kMaxIsingClasses + 1], int nc, int c, uint32_t v) {
+    if constexpr (NC == 1) {
+        const uint32_t on = (uint32_t)(c == 1);     // padding entries are class 0
+        d[1] += v * on;
+        cnt[1] += (int)on;
+    } else {
+#pragma unroll
+        for (int k = 1; k <= (NC > 0 ? NC : kMaxIsingClasses); ++k) {
+            if (NC > 0 || k <= nc) {                // uniform: the context's class count
+                const uint32_t on = (uint32_t)(k == c);
+                d[k] += v * on;
+                cnt[k] += (int)on;
+            }
+        }
+    }
+}
+
+template <int DM, int NC>
+__global__ void __launch_bounds__(1024) k_ising_resident(IsingResArgs a) {
+    constexpr int KC = NC > 0 ? NC : kMaxIsingClasses;   // classes the loops visit
+    extern __shared__ __attribute__((aligned(16))) uint32_t sRes[];
+    uint32_t* sS = sRes;                               // [N] words
+    uint32_t* sThr = sRes + a.N;                       // [4][table_per_slot] when staged
+    __shared__ unsigned sAccB[4];
+    __shared__ int sCnt[(kMaxIsingClasses + kMaxIsingClasses) * 4];
+    __shared__ double sMz[16][4];
+    __shared__ int sCb[kMaxResColours + 1];           // colour boundaries (an indexed argument would go to scratch)
+    const int rid = blockIdx.x, r0 = 4 * rid;
+    const int t = threadIdx.x, nt = blockDim.x;
+    int tbase[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int r = r0 + b;
+        tbase[b] = (r < a.R) ? (a.table_in_lds ? b * a.table_per_slot : a.slot_of[r] * a.table_per_slot) : -1;
+    }
+    for (int q = t; q < a.N; q += nt) sS[q] = *reinterpret_cast<const uint32_t*>(a.s + (size_t)q * a.Rp + r0);
+    if (a.table_in_lds)
+        for (int b = 0; b < 4; ++b) {
+            const int r = r0 + b;
+            if (r >= a.R) continue;
+            const uint32_t* src = a.thr + (size_t)a.slot_of[r] * a.table_per_slot;
+            for (int q = t; q < a.table_per_slot; q += nt) sThr[b * a.table_per_slot + q] = src[q];
+        }
+    if (t < 4) sAccB[t] = 0u;
+    if (t <= kMaxResColours) sCb[t] = a.cbeg[t];
+    for (int q = t; q < 2 * kMaxIsingClasses * 4; q += nt) sCnt[q] = 0;
+    __syncthreads();
+    unsigned accepted[4] = {0u, 0u, 0u, 0u};
+    const uint32_t step0 = a.ctr[0] + a.step_off;
+    for (int k = 0; k < a.n_sweeps; ++k) {
+        const uint32_t step = step0 + (uint32_t)k;
+        for (int c = 0; c < a.ncol; ++c) {
+            const int c_end = sCb[c + 1];
+            for (int slot = sCb[c] + t; slot < c_end; slot += nt) {
+                const uint32_t* nb = a.ell + (size_t)slot * a.D;
+                uint32_t d[kMaxIsingClasses + 1];
+                int cnt[kMaxIsingClasses + 1];
+#pragma unroll
+                for (int kk = 0; kk <= kMaxIsingClasses; ++kk) {
+                    d[kk] = 0u;
+                    cnt[kk] = 0;
+                }
+                const int nD = DM > 0 ? DM : a.D;   // rows padded with class 0 up to DM
+#pragma unroll
+                for (int j = 0; j < nD; ++j) {
+                    const uint32_t e = nb[j];
+                    const int cl = (int)(e >> 24);
+                    const uint32_t nv = sS[e & 0xffffffu];
+                    class_add_lane<NC>(d, cnt, a.n_classes, cl, (nv >> 1) & 0x01010101u);   // 1 per byte where sigma = -1
+                }
+                const uint32_t v = sS[slot];
+                const u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)rid, step, RNG_METROPOLIS, 0);
+                const uint32_t words[4] = {wr.x, wr.y, wr.z, wr.w};
+                const int spc = a.sp[slot];
+                uint32_t flip = 0u;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    if (tbase[b] < 0) continue;
+                    int idx = a.offset;
+#pragma unroll
+                    for (int kk = 1; kk <= KC; ++kk) {
+                        if (NC == 0 && kk > a.n_classes) break;
+                        const int down = (int)((d[kk] >> (8 * b)) & 0xffu);
+                        idx += a.stride[kk] * (cnt[kk] - 2 * down);
+                    }
+                    const int sigma = (int)(int8_t)((v >> (8 * b)) & 0xffu);
+                    const int ti = tbase[b] + (spc * 2 + (sigma > 0)) * a.P + idx;
+                    const uint32_t th = a.table_in_lds ? sThr[ti] : a.thr[ti];   // uniform choice
+                    if (words[b] < th || th == 0xffffffffu) {
+                        flip |= 0xFEu << (8 * b);            // +1 <-> -1 as int8: x ^ 0xFE
+                        ++accepted[b];
+                    }
+                }
+                if (flip) sS[slot] = v ^ flip;
+            }
+            __syncthreads();
+        }
+    }
+    for (int q = t; q < a.N; q += nt) *reinterpret_cast<uint32_t*>(a.s + (size_t)q * a.Rp + r0) = sS[q];
+    // acceptances: integers, summed in LDS, one device atomic per replica
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+        if (accepted[b]) atomicAdd(&sAccB[b], accepted[b]);
+    if (a.meas) {
+        // bond sums per class over full rows (each bond from both ends, the
+        // reference's double sum), spin sums per species, M_z per replica
+        int bond[kMaxIsingClasses + 1][4], zs[kMaxIsingClasses][4];
+        double mz[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk <= kMaxIsingClasses; ++kk)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) bond[kk][b] = 0;
+#pragma unroll
+        for (int kk = 0; kk < kMaxIsingClasses; ++kk)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) zs[kk][b] = 0;
+        for (int slot = t; slot < a.N; slot += nt) {
+            const uint32_t* nb = a.ell + (size_t)slot * a.D;
+            const uint32_t v = sS[slot];
+            uint32_t d[kMaxIsingClasses + 1];
+            int cnt[kMaxIsingClasses + 1];
+#pragma unroll
+            for (int kk = 0; kk <= kMaxIsingClasses; ++kk) {
+                d[kk] = 0u;
+                cnt[kk] = 0;
+            }
+            for (int j = 0; j < a.D; ++j) {
+                const uint32_t e = nb[j];
+                const int cl = (int)(e >> 24);
+                class_add_lane<NC>(d, cnt, a.n_classes, cl, ((sS[e & 0xffffffu] ^ v) >> 1) & 0x01010101u);   // 1 per byte where they differ
+            }
+            const int spc = a.sp[slot];
+            const float w = a.w[slot];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+#pragma unroll
+                for (int kk = 1; kk <= KC; ++kk)
+                    bond[kk][b] += cnt[kk] - 2 * (int)((d[kk] >> (8 * b)) & 0xffu);
+                const int sigma = (int)(int8_t)((v >> (8 * b)) & 0xffu);
+#pragma unroll
+                for (int kk = 0; kk < kMaxIsingClasses; ++kk) zs[kk][b] += sigma * (int)(kk == spc);
+                mz[b] += (double)(sigma > 0 ? w : -w);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+#pragma unroll
+            for (int kk = 1; kk <= KC; ++kk)
+                if (kk <= a.n_classes && bond[kk][b]) atomicAdd(&sCnt[(kk - 1) * 4 + b], bond[kk][b]);
+#pragma unroll
+            for (int kk = 0; kk < kMaxIsingClasses; ++kk)
+                if (kk < a.nsp && zs[kk][b]) atomicAdd(&sCnt[(kMaxIsingClasses + kk) * 4 + b], zs[kk][b]);
+            // M_z: a fixed-order wave reduction, then the waves in order (the
+            // sum of +-w in FP64 is exact in any order anyway)
+            double m = mz[b];
+            for (int off = 32; off > 0; off >>= 1) m += __shfl_xor(m, off);
+            if ((t & 63) == 0) sMz[t >> 6][b] = m;
+        }
+    }
+    __syncthreads();
+    if (t < 4) {
+        const int b = t, r = r0 + b;
+        if (r < a.R && sAccB[b])
+            atomicAdd(a.acc + (size_t)(blockIdx.x % kAccCopies) * a.R + a.slot_of[r], (unsigned long long)sAccB[b]);
+        if (a.meas && r < a.Rp) {
+            double* out = a.partial + (size_t)r * a.rec;    // tile 0
+            int o = 0;
+            for (int kk = 0; kk < a.n_classes; ++kk) out[o++] = (double)sCnt[kk * 4 + b];
+            for (int kk = 0; kk < a.nsp; ++kk) out[o++] = (double)sCnt[(kMaxIsingClasses + kk) * 4 + b];
+            double m = 0.0;
+            for (int wv = 0; wv < (nt + 63) / 64; ++wv) m += sMz[wv][b];
+            out[o] = m;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Several sites per wavefront, for contexts with few replicas: lanes per site
 // LPS = 1 << a.lps_shift (XYZ: 16 or 32 replicas per site; Ising: 8, 16 or 32
 // lanes of 4 replicas), so a wave updates 64 / LPS sites per step instead of
@@ -1829,6 +2025,23 @@ void launch_ising_sweep(const IsingArgs& a, hipStream_t st) {
 #define LI(DM) hipLaunchKernelGGL((k_ising_sweep<DM>), dim3(a.nblocks), dim3(kBlock), 0, st, a)
     ETHER_DISPATCH_DM(a.D, LI)
 #undef LI
+}
+
+void launch_ising_resident(const IsingResArgs& a, int block, size_t lds_bytes, hipStream_t st) {
+    const dim3 grid((unsigned)(a.Rp / 4)), blk((unsigned)block);
+    if (a.n_classes == 1) {
+#define LR(DM) hipLaunchKernelGGL((k_ising_resident<DM, 1>), grid, blk, lds_bytes, st, a)
+        ETHER_DISPATCH_DM(a.D, LR)
+#undef LR
+    } else if (a.n_classes == 2) {
+#define LR(DM) hipLaunchKernelGGL((k_ising_resident<DM, 2>), grid, blk, lds_bytes, st, a)
+        ETHER_DISPATCH_DM(a.D, LR)
+#undef LR
+    } else {
+#define LR(DM) hipLaunchKernelGGL((k_ising_resident<DM, 0>), grid, blk, lds_bytes, st, a)
+        ETHER_DISPATCH_DM(a.D, LR)
+#undef LR
+    }
 }
 
 void launch_measure(const MeasArgs& a, int model, bool zee, bool sia, hipStream_t st) {

@@ -23,7 +23,7 @@ MODEL_ISING = 0
 MODEL_XYZ = 1
 INIT_FRESH = 1
 INIT_RANDOM = 2
-TIME_SWEEP, TIME_OVRR, TIME_MEASURE, TIME_PT, TIME_SQ, TIME_SWEEP_MEAS = 0, 1, 2, 3, 4, 5
+TIME_SWEEP, TIME_OVRR, TIME_MEASURE, TIME_PT, TIME_SQ, TIME_SWEEP_MEAS, TIME_RESIDENT = 0, 1, 2, 3, 4, 5, 6
 COMPAT_PT_ETOT = 1
 COMPAT_SPLIT_OVRR = 2
 COMPAT_MEASURE_PASS = 4
@@ -76,6 +76,7 @@ EXPORTS = [
     ("ether_run", C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                             C.c_int32, _vp]),
     ("ether_set_graphs", C.c_int, [_vp, C.c_int32]),
+    ("ether_set_resident", C.c_int, [_vp, C.c_int32]),
     ("ether_run_prepare", C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                     C.c_double, C.c_int32]),
     ("ether_set_launch", C.c_int, [_vp, C.c_int32, C.c_int32]),
@@ -368,6 +369,10 @@ class Engine:
 
     def set_graphs(self, on: bool) -> None:
         self._c(self.lib.ether_set_graphs(self.h, 1 if on else 0))
+
+    def set_resident(self, on: bool) -> None:
+        """Ising sweeps with the lattice resident in LDS (default on where it fits)."""
+        self._c(self.lib.ether_set_resident(self.h, 1 if on else 0))
 
     def set_launch(self, sites_per_wave: int, xcd_map: int) -> None:
         self._c(self.lib.ether_set_launch(self.h, int(sites_per_wave), int(xcd_map)))

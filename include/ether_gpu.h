@@ -275,6 +275,19 @@ int ether_run(ether_ctx* ctx, int32_t first_step, int32_t n_steps, int32_t mcs_e
  * graphs off (default on). */
 int ether_set_graphs(ether_ctx* ctx, int32_t on);
 
+/* Ising sweeps with the lattice resident in LDS (no counterpart in the
+ * reference, whose sweep is one loop over sites, src/Metropolis.f90:35-62):
+ * one workgroup per word of four replicas holds that word of every site in
+ * LDS and runs all the sweeps between two measurements (a swap or a sample)
+ * in one launch, with a workgroup barrier after each colour phase, booking
+ * the measurement the next swap or sample reads.  Used by ether_sweep and
+ * ether_run when the context is Ising, has at most 16 colours and
+ * 4*N (+ 16 * its threshold table) bytes fit 60 KiB; batches keep the phase
+ * kernels.  Results are bit-identical to launching the colour phases one by
+ * one (the same Philox counters, counts and integer thresholds).  on = 0
+ * forces the phase kernels (default on). */
+int ether_set_resident(ether_ctx* ctx, int32_t on);
+
 /* Captures, instantiates and uploads every graph that ether_run with the same
  * arguments would replay, and runs nothing, so that the ether_run that
  * follows spends no time on capture or instantiation (a timed production
@@ -335,7 +348,8 @@ int ether_autotune(ether_ctx* ctx, int32_t reps, int32_t* sites_per_wave, int32_
  * 2 measurement (partials + finalize, or the finalize alone after a sweep
  * that booked it), 3 tempering swap, 4 structure factor (device part of one
  * ether_structure_factor call), 5 the colour phases of a sweep that also
- * books the measurement that follows it (XYZ, inside ether_run).  Adds no
+ * books the measurement that follows it (XYZ, inside ether_run), 6 the
+ * resident Ising launches (ether_set_resident; counted in sweeps).  Adds no
  * synchronisation; ether_get_timing synchronises and reports the summed
  * event time and the number of launches it covers since timing was switched
  * on. */
@@ -345,6 +359,7 @@ int ether_autotune(ether_ctx* ctx, int32_t reps, int32_t* sites_per_wave, int32_
 #define ETHER_TIME_PT         3
 #define ETHER_TIME_SQ         4
 #define ETHER_TIME_SWEEP_MEAS 5
+#define ETHER_TIME_RESIDENT   6  /* resident Ising sweeps: one event pair per launch, counted in sweeps */
 int ether_set_timing(ether_ctx* ctx, int32_t on);
 int ether_get_timing(ether_ctx* ctx, int32_t kind, double* total_ms, int64_t* launches);
 /* Waits for all work queued on the context's stream. */

@@ -9,11 +9,11 @@ observables every 10 MCS.  A "step" is one MCS of every chain (OVRR +
 Metropolis sweep + the scheduled measurement), the per-MCS schedule of the
 reference's driver, with the reflection fused into each colour phase of the
 sweep and the measurement booked by the sweep before it (two colours).  The
-launch shape of the sweep is picked by the autotuner first (`--launch` pins
-it).  The run starts from the start state and fresh_spins; `--warmup` untimed
+launch shape of the sweep is picked by the autotuner (`--launch` pins it) on
+the start state after fresh_spins, which it restores; `--warmup` untimed
 production MCS follow; the timed region is `--steps` production MCS (samples
 every 10 MCS included) from the next schedule period, whole periods replayed
-as HIP graphs that are captured and uploaded before the clock starts.
+as HIP graphs that are captured and uploaded before the warmup.
 
 value = Metropolis spin updates / wall second of the timed MCS loop
         (OVRR and measurement time included, not counted as updates).
@@ -209,12 +209,18 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
     eng = make_engine(m, R, compat=(E.COMPAT_SPLIT_OVRR if split_ovrr else 0) |
                       (E.COMPAT_MEASURE_PASS if args.measure_pass else 0))
     eng.set_betas(np.tile(m.betas(temps), t.repeat))
+    # the start state and fresh_spins
+    eng.set_spins(-1, m.initial_spins, m.initial_phi)
+    eng.init_spins(E.INIT_FRESH)
+    # launch shape: the autotuner times every candidate on this context and
+    # puts the spins back; it is the last long GPU work before the timed
+    # region, so the GPU starts that region out of its idle power state
     if args.launch:
         spw, xcd = args.launch
         eng.set_launch(spw, xcd)
         tune_ms = None
     else:
-        spw, xcd, tune_ms = eng.autotune(reps=10)
+        spw, xcd, tune_ms = eng.autotune(reps=40)
     coa = fortran_nint(t.coa)
     ovrr_on = t.ovrr and t.xyz and not t.zeeman and not t.sia
     ovrr_iv = t.ovrr_mcs if ovrr_on else 0
@@ -225,12 +231,11 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
     period = coa if not ovrr_iv else coa * ovrr_iv // math.gcd(coa, ovrr_iv)
     # the timed window is K production MCS (samples included) from the period
     # boundary after the warmup; every graph it replays is captured and
-    # uploaded first, so that the warmup runs right before the clock starts
+    # uploaded before the warmup, so that the warmup runs right before the
+    # clock starts
     first = eq + 1 + period * -(-warmup // period)
     eng.run_prepare(first, steps, eq, coa, ovrr_iv, ovrr_p)
-    # warmup: the start state, fresh_spins, then W untimed production MCS
-    eng.set_spins(-1, m.initial_spins, m.initial_phi)
-    eng.init_spins(E.INIT_FRESH)
+    # warmup: W untimed production MCS
     if warmup > 0:
         eng.run(eq + 1, warmup, eq, coa, ovrr_iv, ovrr_p)
     eng.reset_moments()

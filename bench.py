@@ -84,21 +84,24 @@ def describe(name, m, n_temps, R, steps, split_ovrr, coa, ovrr_iv, measure_pass=
                 f"({'split pass' if split_ovrr else 'fused into the sweep phases'})")
     return (f"{name}: {kind} {m.sc[0]}x{m.sc[1]}x{m.sc[2]} cells ({m.n_sites} spins){extra}, "
             f"{n_temps} T x {t.repeat} repeats = {R} chains{moves}{ovrr}, "
-            f"MCS {steps}/{t.tmcs_eq}, coa {coa}" + (", measurement as a pass of its own" if measure_pass else ""))
+            f"MCS {steps}/{t.tmcs_eq}, coa {coa}" + (f", swaps every {t.exchange_interval} MCS on one ladder "
+                                                     f"per repeat" if t.pt else "") +
+            (", measurement as a pass of its own" if measure_pass else ""))
 
 
-def profiled_traffic(kernel, workload, spw):
-    """HBM bytes per launch of the sweep kernel at `spw` sites per wave from
-    the committed PMC summary (FETCH_SIZE x2.0 + WRITE_SIZE x1.0, factors from
-    a plain copy; lines `traffic_bytes_per_launch[spw=N] = bytes`), or None."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.md")
-    if not os.path.exists(path):
-        return None
-    key = f"traffic_bytes_per_launch[spw={spw}]"
-    with open(path) as f:
-        for line in f:
-            if line.startswith(key):
-                return float(line.split("=", 2)[2])
+def profiled_traffic(kernel, workload, key):
+    """HBM bytes per launch (or per unit) of the dominant kernel from the
+    committed PMC summary under profiles/ (FETCH_SIZE x2.0 + WRITE_SIZE x1.0,
+    factors from a plain copy): the line `<key> = bytes` of
+    profiles/<kernel>_<workload>_traffic.md or _stats.md, or None."""
+    for suffix in ("traffic", "stats"):
+        path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_{suffix}.md")
+        if not os.path.exists(path):
+            continue
+        with open(path) as f:
+            for line in f:
+                if line.startswith(key + " ="):
+                    return float(line.split("=")[-1])
     return None
 
 
@@ -226,22 +229,30 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
     ovrr_iv = t.ovrr_mcs if ovrr_on else 0
     ovrr_p = t.ovrr_para if ovrr_on else 0.0
 
+    # tempering (pt = .T.): one ladder per repeat, swaps every exchange_interval MCS
+    pt_iv = t.exchange_interval if t.pt else 0
+    sc = 0
+    if pt_iv:
+        eng.set_ladders([k * len(temps) for k in range(t.repeat + 1)])
     # the schedule's period: whole periods are replayed as captured graphs
     eq = t.tmcs_eq
-    period = coa if not ovrr_iv else coa * ovrr_iv // math.gcd(coa, ovrr_iv)
+    period = coa
+    for iv in (ovrr_iv, pt_iv):
+        if iv:
+            period = period * iv // math.gcd(period, iv)
     # the timed window is K production MCS (samples included) from the period
     # boundary after the warmup; every graph it replays is captured and
     # uploaded before the warmup, so that the warmup runs right before the
     # clock starts
     first = eq + 1 + period * -(-warmup // period)
-    eng.run_prepare(first, steps, eq, coa, ovrr_iv, ovrr_p)
+    eng.run_prepare(first, steps, eq, coa, ovrr_iv, ovrr_p, pt_iv)
     # warmup: W untimed production MCS
     if warmup > 0:
-        eng.run(eq + 1, warmup, eq, coa, ovrr_iv, ovrr_p)
+        sc = eng.run(eq + 1, warmup, eq, coa, ovrr_iv, ovrr_p, pt_iv, sc)
     eng.reset_moments()
     eng.synchronize()
     t0 = time.perf_counter()
-    eng.run(first, steps, eq, coa, ovrr_iv, ovrr_p)
+    sc = eng.run(first, steps, eq, coa, ovrr_iv, ovrr_p, pt_iv, sc)
     eng.synchronize()
     wall = time.perf_counter() - t0
     mom = eng.moments()
@@ -253,7 +264,7 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
     # which a graph replay does not allow; gives the per-launch kernel durations
     eng.set_timing(True)
     t1 = time.perf_counter()
-    eng.run(first + steps, event_steps, eq, coa, ovrr_iv, ovrr_p)
+    eng.run(first + steps, event_steps, eq, coa, ovrr_iv, ovrr_p, pt_iv, sc)
     eng.synchronize()
     wall_ev = time.perf_counter() - t1
     sweep_ms, sweep_n = eng.timing(E.TIME_SWEEP)
@@ -300,7 +311,9 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(kernel, workload, spw),
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": profiled_traffic(kernel, workload, "traffic_bytes_per_sweep" if res_n > 0 else
+                                        f"traffic_bytes_per_launch[spw={spw}]"),
             "kernel": f"{kernel} ({unit})",
             "bytes_per_update": b_upd, "updates_per_launch": per_launch_updates,
             "avg_launch_ms": avg_ms, "launches": sweep_n,

@@ -212,7 +212,7 @@ def _train(model: Model, temps, rep: int, eng: E.Engine, sched):
 
 
 def _results(model: Model, eng: E.Engine, temps, rep: int, mom: np.ndarray, wall: float,
-             counted: int, source: str) -> Results:
+             counted: int, source: str, spiral_sums: Optional[np.ndarray] = None) -> Results:
     t = model.tags
     nscan = len(temps)
     beta_T = model.betas(temps)
@@ -232,7 +232,7 @@ def _results(model: Model, eng: E.Engine, temps, rep: int, mom: np.ndarray, wall
                   updates=float(eng.R) * model.n_sites * counted, para=t.para,
                   ladder_source=source, counted_mcs=counted)
     if getattr(eng, "spiral_count", 0):
-        sums, _ = eng.spiral()
+        sums = spiral_sums if spiral_sums is not None else eng.spiral()[0]
         sper = np.zeros((rep, nscan, 3))
         for k in range(rep):
             for j in range(nscan):
@@ -245,7 +245,9 @@ def _results(model: Model, eng: E.Engine, temps, rep: int, mom: np.ndarray, wall
     if t.pt:
         _, pacc, patt, trips = eng.pt_state()
         res.pt_acc, res.pt_att, res.trips = pacc, patt, trips
-    res.final_S, res.final_phi = outputs.final_states(eng, nscan, rep)
+    # the configuration at each temperature after the last repeat: the last
+    # ladder of the context (all of it when the repeats ran one after another)
+    res.final_S, res.final_phi = outputs.final_states(eng, nscan, eng.R // nscan)
     res.model = model
     return res
 
@@ -258,14 +260,19 @@ def run(model: Model, *, compat: int = 0, chunk: int = 0, temps=None, repeat: Op
     temps = t.temperatures() if temps is None else np.asarray(temps, np.float64)
     nscan = len(temps)
     rep = t.repeat if repeat is None else repeat
-    R = nscan * rep
+    flags = compat | g.compat
+    chain = bool(flags & E.COMPAT_CHAIN_REPEATS)
+    if chain and g.ladder == "learned":
+        raise EtherInputError("ladder = learned does not run with chained repeats (compat 8)")
+    # every repeat its own replica, or (chained) the repeats one after another on nscan replicas
+    R = nscan if chain else nscan * rep
     own = eng is None
     if own:
-        eng = make_engine(model, R, compat=compat | g.compat, seed=seed)
+        eng = make_engine(model, R, compat=flags, seed=seed)
         apply_gpu_tags(eng, g)
     sched = _schedule(model)
     coa, ovrr_iv, ovrr_p, pt_iv = sched
-    _prepare(model, temps, rep, eng)
+    _prepare(model, temps, R // nscan, eng)
     if g.autotune:
         eng.autotune(g.autotune)      # restores every replica's state
     t0 = time.perf_counter()
@@ -274,13 +281,25 @@ def run(model: Model, *, compat: int = 0, chunk: int = 0, temps=None, repeat: Op
         temps, step, swap, source = _train(model, temps, rep, eng, sched)
     first = step
     chunk = g.chunk or chunk or t.tmcs
-    while step <= t.tmcs:
-        n = min(chunk, t.tmcs - step + 1)
-        swap = eng.run(step, n, t.tmcs_eq, coa, ovrr_iv, ovrr_p, pt_iv, swap)
-        step += n
-    mom = eng.moments()   # synchronises
+    moms, sums = [], []
+    for k in range(rep if chain else 1):
+        if k > 0:
+            # the next repeat: fresh_spins of the state the last one left, zeroes('avg')
+            # (src/ether.f90:152-156); swap_count restarts (:158)
+            eng.init_spins(E.INIT_FRESH)
+            eng.reset_moments()
+            step, swap = 1, 0
+        while step <= t.tmcs:
+            n = min(chunk, t.tmcs - step + 1)
+            swap = eng.run(step, n, t.tmcs_eq, coa, ovrr_iv, ovrr_p, pt_iv, swap)
+            step += n
+        moms.append(eng.moments())   # synchronises
+        if getattr(eng, "spiral_count", 0):
+            sums.append(eng.spiral()[0])
+    mom = np.concatenate(moms)
     wall = time.perf_counter() - t0
-    res = _results(model, eng, temps, rep, mom, wall, t.tmcs - first + 1, source)
+    res = _results(model, eng, temps, rep, mom, wall, t.tmcs - first + 1, source,
+                   np.concatenate(sums) if sums else None)
     if own:
         eng.close()
     return res
@@ -295,8 +314,9 @@ def run_batch(models: Sequence[Model], *, seeds: Optional[Sequence[Optional[int]
     run's results equal those of run() on it alone."""
     seeds = list(seeds) if seeds is not None else [None] * len(models)
     for m in models:
-        if m.gpu.ladder != "linear" or m.gpu.autotune:
-            raise EtherInputError("a batched run takes neither `ladder = learned` nor `autotune`")
+        if m.gpu.ladder != "linear" or m.gpu.autotune or m.gpu.compat & E.COMPAT_CHAIN_REPEATS:
+            raise EtherInputError("a batched run takes neither `ladder = learned`, `autotune` nor "
+                                  "chained repeats (compat 8)")
     groups = {}
     for k, m in enumerate(models):
         key = (m.tags.tmcs, m.tags.tmcs_eq) + _schedule(m) + (m.model_id, m.sia_vec is not None)

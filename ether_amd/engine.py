@@ -27,6 +27,7 @@ TIME_SWEEP, TIME_OVRR, TIME_MEASURE, TIME_PT, TIME_SQ, TIME_SWEEP_MEAS, TIME_RES
 COMPAT_PT_ETOT = 1
 COMPAT_SPLIT_OVRR = 2
 COMPAT_MEASURE_PASS = 4
+COMPAT_CHAIN_REPEATS = 8
 
 MOM_N, MOM_E, MOM_E2, MOM_E4, MOM_M, MOM_M2, MOM_M4, MOM_ACC = range(8)
 MOM_COUNT = 8

@@ -264,8 +264,8 @@ contains
             read (value, *, iostat=ios) gpu_graphs
          case ('compat')
             read (value, *, iostat=ios) gpu_compat
-            if (ios == 0 .and. (gpu_compat < 0 .or. gpu_compat > 7)) &
-               call terminate('compat: a sum of the flags 1, 2 and 4')
+            if (ios == 0 .and. (gpu_compat < 0 .or. gpu_compat > 15)) &
+               call terminate('compat: a sum of the flags 1, 2, 4 and 8')
          case ('chunk')
             read (value, *, iostat=ios) gpu_chunk
             if (gpu_chunk < 0) call terminate('chunk must be >= 0')
@@ -328,7 +328,8 @@ program ether_gpu_driver
    implicit none
 
    integer :: nb, n, i, j, k, l, s, a, b, q, nsh, npairs, u, ios, nscan, R, t, rep, coa_i
-   integer :: start, nchunk, step, sw, rc, ovrr_iv, cnt, cap, first_counted
+   integer :: start, nchunk, step, sw, rc, ovrr_iv, cnt, cap, first_counted, rd, crep, ncrep, compat_all
+   logical :: chain
    integer, allocatable :: b_sp(:), b_idx(:), species(:), nn_count(:), nn_idx(:), gstart(:)
    integer(c_int), allocatable :: sp_partners(:)
    integer(c_int64_t) :: sp_count
@@ -514,8 +515,21 @@ program ether_gpu_driver
       end do
    end do
 
+   ! ---- compat flags (gpu.ether `compat` and the ETHER_COMPAT variable); with
+   !      ETHER_COMPAT_CHAIN_REPEATS the repeats run one after another on nscan
+   !      replicas, each from fresh_spins of the previous repeat's final state
+   !      (src/ether.f90:152-155), as the reference's rank does
+   call get_environment_variable('ETHER_COMPAT', line, status=ios)
+   q = 0
+   if (ios == 0) read (line, *, iostat=ios) q
+   compat_all = ior(q, gpu_compat)
+   chain = iand(compat_all, ETHER_COMPAT_CHAIN_REPEATS) /= 0
+   if (chain .and. gpu_learned) call terminate('ladder = learned does not run with chained repeats (compat 8)')
+   rd = R
+   if (chain) rd = nscan
+
    ! ---- context
-   rc = ether_ctx_create(merge(ETHER_MODEL_ISING, ETHER_MODEL_XYZ, ising), n, R, &
+   rc = ether_ctx_create(merge(ETHER_MODEL_ISING, ETHER_MODEL_XYZ, ising), n, rd, &
                          int(seed_value, c_int64_t), ctx)
    if (rc /= ETHER_OK) call terminate(ether_message(c_null_ptr))
    call chk(ether_set_lattice(ctx, nspecies, species, stagger, nsh, cap, nn_count, nn_idx, c_null_ptr))
@@ -523,10 +537,7 @@ program ether_gpu_driver
    call chk(ether_set_zeeman(ctx, merge(1, 0, zeeman), h, gf, mub))
    call chk(ether_set_sia(ctx, merge(1, 0, sia), siav))
    call chk(ether_set_proposal(ctx, merge(1, 0, angle), dphi))
-   call get_environment_variable('ETHER_COMPAT', line, status=ios)
-   q = 0
-   if (ios == 0) read (line, *, iostat=ios) q
-   call chk(ether_set_compat(ctx, int(ior(q, gpu_compat), c_int)))
+   call chk(ether_set_compat(ctx, int(compat_all, c_int)))
    ! spiral-state partners (src/ether.f90:69, src/get_spiral_state.f90:47-82)
    sp_count = 0
    if (ssp .and. .not. ising) then
@@ -540,13 +551,13 @@ program ether_gpu_driver
    call chk(ether_set_graphs(ctx, merge(1, 0, gpu_graphs)))
    if (gpu_spw > 0) call chk(ether_set_launch(ctx, gpu_spw, gpu_xcd))
    if (gpu_lanes > 0) call chk(ether_set_lanes(ctx, gpu_lanes))
-   call chk(ether_set_betas(ctx, betas))
+   call chk(ether_set_betas(ctx, betas(1:rd)))
    if (pt) then
-      allocate (gstart(nrepeat + 1))
-      do rep = 0, nrepeat
+      allocate (gstart(rd/nscan + 1))
+      do rep = 0, rd/nscan
          gstart(rep + 1) = rep*nscan
       end do
-      call chk(ether_set_ladders(ctx, nrepeat, gstart))
+      call chk(ether_set_ladders(ctx, rd/nscan, gstart))
    end if
    ! optional launch-shape autotuning: sweeps timed per candidate, from gpu.ether's
    ! `autotune` or the ETHER_AUTOTUNE environment variable
@@ -609,22 +620,36 @@ program ether_gpu_driver
    first_counted = step
    nchunk = 1000
    if (gpu_chunk > 0) nchunk = gpu_chunk
-   do while (step <= tmcs)
-      start = step
-      q = min(nchunk, tmcs - step + 1)
-      call chk(ether_run(ctx, start, q, tmcs_eq, coa_i, ovrr_iv, ovrr_p, &
-                         merge(exchange_interval, 0, pt), swap_c))
-      step = step + q
-      inquire (file='STATUS', exist=ok)
-      if (ok) then
-         call chk(ether_synchronize(ctx))
-         write (6, '(a,i0,a,i0)') ' STATUS: MCS ', step - 1, ' of ', tmcs
-         open (newunit=u, file='STATUS')
-         close (u, status='delete')
-      end if
-   end do
    allocate (mom(ETHER_MOM_COUNT, R))
-   call chk(ether_read_moments(ctx, mom))
+   if (sp_count > 0) allocate (sp_sums(3, R))
+   ! all repeats at once (rd = R replicas), or one after another (rd = nscan)
+   ncrep = 1
+   if (chain) ncrep = nrepeat
+   do crep = 1, ncrep
+      if (crep > 1) then
+         ! the next repeat: fresh_spins of the state the last one left, zeroes('avg')
+         call chk(ether_init_spins(ctx, ETHER_INIT_FRESH))
+         call chk(ether_reset_moments(ctx))
+         step = 1
+         swap_c = 0
+      end if
+      do while (step <= tmcs)
+         start = step
+         q = min(nchunk, tmcs - step + 1)
+         call chk(ether_run(ctx, start, q, tmcs_eq, coa_i, ovrr_iv, ovrr_p, &
+                            merge(exchange_interval, 0, pt), swap_c))
+         step = step + q
+         inquire (file='STATUS', exist=ok)
+         if (ok) then
+            call chk(ether_synchronize(ctx))
+            write (6, '(a,i0,a,i0)') ' STATUS: MCS ', step - 1, ' of ', tmcs
+            open (newunit=u, file='STATUS')
+            close (u, status='delete')
+         end if
+      end do
+      call chk(ether_read_moments(ctx, mom(:, (crep - 1)*rd + 1:crep*rd)))
+      if (sp_count > 0) call chk(ether_read_spiral(ctx, sp_sums(:, (crep - 1)*rd + 1:crep*rd), c_null_ptr))
+   end do
    call system_clock(t1)
 
    ! ---- observables per repeat and over repeats (src/get_eng.f90:43-56,
@@ -662,8 +687,7 @@ program ether_gpu_driver
    ! spiral state per repeat and over repeats (src/get_spiral_state.f90:122-140,
    ! src/observables_per_repeat.f90:71-94)
    if (sp_count > 0) then
-      allocate (sp_sums(3, R), sp_per(3, nrepeat, nscan), sp_mean(3, nscan), sp_sem(3, nscan))
-      call chk(ether_read_spiral(ctx, sp_sums, c_null_ptr))
+      allocate (sp_per(3, nrepeat, nscan), sp_mean(3, nscan), sp_sem(3, nscan))
       do rep = 0, nrepeat - 1
          do t = 1, nscan
             q = rep*nscan + t
@@ -724,11 +748,12 @@ program ether_gpu_driver
    !      (src/write_gss.f90:34-78), moment_vectors.dat
    !      (src/write_output_files.f90:66-80) and <T>spK.xsf
    !      (src/write_spins_at_K.f90:33-66).  Replica r = rep*nscan + t holds slot
-   !      slot_of(r); the one in slot (nrepeat-1)*nscan + t is at temperature t.
-   allocate (slot_of(R), rep_at(R), acc64(R), att64(R), trips64(R))
+   !      slot_of(r); the one in slot rd - nscan + t (the last repeat's ladder) is at
+   !      temperature t.
+   allocate (slot_of(rd), rep_at(rd), acc64(rd), att64(rd), trips64(rd))
    allocate (sfin(3, n), phifin(n), ionrec(0:8, n), obs(20 + 3*nspecies), mmv(3, nspecies), tions(nspecies))
    call chk(ether_pt_state(ctx, slot_of, acc64, att64, trips64))
-   do q = 1, R
+   do q = 1, rd
       rep_at(slot_of(q) + 1) = q - 1
    end do
    tions = 0
@@ -769,7 +794,7 @@ program ether_gpu_driver
    write (uv, 105) (species_name(inc(i)), i = 1, cnt)
    write (uv, *) '# Temp, sp1(Mx,My,Mz), sp1(Mx,My,Mz),..spN(Mx,My,Mz), sp1|M|, sp2|M|,..spN|M|'
    do t = 1, nscan
-      call chk(ether_get_spins(ctx, rep_at((nrepeat - 1)*nscan + t), sfin, phifin))
+      call chk(ether_get_spins(ctx, rep_at(rd - nscan + t), sfin, phifin))
       mmv = 0.0_dp
       do i = 1, n
          mmv(:, species(i)) = mmv(:, species(i)) + stagger(i)*sfin(:, i)*spin(species(i))

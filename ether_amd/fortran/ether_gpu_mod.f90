@@ -12,6 +12,7 @@ module ether_gpu
    integer(c_int), parameter :: ETHER_COMPAT_PT_ETOT = 1
    integer(c_int), parameter :: ETHER_COMPAT_SPLIT_OVRR = 2
    integer(c_int), parameter :: ETHER_COMPAT_MEASURE_PASS = 4
+   integer(c_int), parameter :: ETHER_COMPAT_CHAIN_REPEATS = 8
    integer(c_int), parameter :: ETHER_MOM_COUNT = 8
 
    interface

@@ -444,7 +444,8 @@ class GpuTags:
     lanes: int = 0              # lanes per site of the sweep (0: chosen from the replica count)
     graphs: bool = True         # replay captured HIP graphs
     compat: int = 0             # compat flags (1: PT on E_tot, 2: OVRR as a separate pass,
-                                # 4: measurement as a separate pass)
+                                # 4: measurement as a separate pass, 8: repeats one after
+                                # another, each from the previous one's final state)
     chunk: int = 0              # MCS per ether_run call (0: the whole run)
 
 
@@ -482,8 +483,8 @@ def parse_gpu_ether(text: str) -> GpuTags:
                 g.graphs = _logical(v[0])
             elif key == "compat":
                 g.compat = int(v[0])
-                if g.compat < 0 or g.compat > 7:
-                    raise EtherInputError("compat: a sum of the flags 1, 2 and 4")
+                if g.compat < 0 or g.compat > 15:
+                    raise EtherInputError("compat: a sum of the flags 1, 2, 4 and 8")
             elif key == "chunk":
                 g.chunk = int(v[0])
                 if g.chunk < 0:

@@ -65,6 +65,12 @@ typedef struct ether_ctx ether_ctx;
                                     XYZ sweep that a measurement follows books it as its phases update the
                                     sites (the same energies to FP32 rounding)                            */
 
+#define ETHER_COMPAT_CHAIN_REPEATS 8 /* the drivers: the REPEAT repeats run one after another on nscan
+                                    replicas, each from fresh_spins of the previous repeat's final state, as the
+                                    reference's rank runs them (src/ether.f90:152-155, src/fresh_spins.f90:31-38);
+                                    by default every repeat is its own replica and starts from fresh_spins of the
+                                    generate_supercell state.  The library itself accepts and ignores it.       */
+
 /* per-replica moment record read by ether_read_moments (8 doubles) */
 #define ETHER_MOM_N     0   /* number of samples                                  */
 #define ETHER_MOM_E     1   /* sum of e    (e = E_tot/(2N), src/get_eng.f90:36)   */

@@ -27,7 +27,7 @@ def test_gpu_tags_and_defaults():
 
 
 @pytest.mark.parametrize("text", ["pt = .T. 11", "ladder = geometric", "launch = 3 1", "lanes = 12",
-                                  "compat = 8", "chunk = -1", "autotune", "ladder = learned 10 0"])
+                                  "compat = 16", "chunk = -1", "autotune", "ladder = learned 10 0"])
 def test_gpu_tags_errors(text):
     with pytest.raises(EtherInputError):
         parse_gpu_ether(text)

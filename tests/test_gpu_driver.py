@@ -73,6 +73,42 @@ def test_fortran_and_python_drivers_agree(tmp_path, name, over):
     _same_state_files(tmp_path, res)
 
 
+@pytest.mark.parametrize("name,over", [
+    ("sc_ising_16", dict(sc="6 6 6", mcs="300 150", repeat="3", temp="100 10 30")),
+    ("tri_afm_96", dict(sc="12 12 1", mcs="300 150", repeat="3", temp="20 0.5 6.5")),   # ANGLE
+    ("pyro_pt_16", dict(sc="2 2 2", mcs="330 110", repeat="2", temp="1 0.05 0.19")),    # tempering
+])
+def test_chained_repeats_in_both_drivers(tmp_path, name, over):
+    """compat = 8 (ETHER_COMPAT_CHAIN_REPEATS): the repeats run one after
+    another on nscan replicas, each from fresh_spins of the previous repeat's
+    final state (src/ether.f90:152-155).  Both drivers write the same numbers;
+    every repeat books its own samples; the run differs from the default one,
+    where every repeat starts from the generate_supercell state."""
+    say(f"chained repeats {name}")
+    f, d = tmp_path / "f", tmp_path / "default"
+    workloads.write_dir(name, str(f), **over)
+    workloads.write_dir(name, str(d), **over)
+    (f / "gpu.ether").write_text("compat = 8\n")
+    r = subprocess.run([DRIVER], cwd=f, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    m = read_directory(str(f))
+    res = driver.run(m)
+    res.write(str(f / "py"))
+    for fn in ("energy.dat", "magnetization.dat"):
+        a = _table(f / fn)
+        b = _table(f / "py" / fn)
+        assert a.shape == b.shape and a.shape[0] > 0
+        assert np.allclose(a, b, rtol=2e-5, atol=2e-5), fn
+    assert np.array_equal(np.fromfile(f / "ETHER.spn", "<f8"), np.fromfile(f / "py" / "ETHER.spn", "<f8"))
+    # every repeat sampled every coa-th production MCS
+    n_samples = (m.tags.tmcs - m.tags.tmcs_eq) // int(m.tags.coa)
+    per = res.per_repeat
+    assert per.shape[0] == m.tags.repeat
+    res_d = driver.run(read_directory(str(d)))
+    assert not np.array_equal(res.per_repeat, res_d.per_repeat)
+    assert n_samples > 0
+
+
 def _same_state_files(tmp_path, res):
     """Files built from the final configurations: both drivers read the same
     device state, so the records agree bit for bit and the text records byte

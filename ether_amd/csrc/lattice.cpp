@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/ether_gpu.h"
+#include "lattice_internal.h"
 #include "philox.h"
 
 namespace ether {
@@ -30,10 +31,7 @@ const std::string& global_error() { return g_last_error; }
 
 namespace {
 
-struct TemplateEntry {
-    int d[3];
-    int lb;
-};
+using ether::TemplateEntry;
 
 bool invert3(const double* a, double* inv) {
     // a is row-major 3x3; inv = a^{-1}
@@ -61,29 +59,26 @@ int floordiv(int a, int b) {
 
 }  // namespace
 
-extern "C" int ether_build_neighbours(const double* abc, int32_t n_basis, const double* basis_xyz,
-                                      const int32_t* sc, const int32_t* open_bc,
-                                      int32_t n_shells, const double* shell_d, double tol,
-                                      int32_t capacity, int32_t* nn_count, int32_t* nn_idx) {
-    using ether::set_global_error;
-    if (!abc || !basis_xyz || !sc || !open_bc || !shell_d || !nn_count || !nn_idx) {
-        set_global_error("ether_build_neighbours: null pointer argument");
+namespace ether {
+
+int build_templates(const double* abc, int32_t n_basis, const double* basis_xyz, const int32_t* sc,
+                    const int32_t* open_bc, int32_t n_shells, const double* shell_d, double tol,
+                    LatticeTemplates& out, std::string& err) {
+    if (!abc || !basis_xyz || !sc || !open_bc || !shell_d) {
+        err = "null pointer argument";
         return ETHER_ERR_ARG;
     }
-    if (n_basis <= 0 || n_shells <= 0 || capacity <= 0 || sc[0] <= 0 || sc[1] <= 0 ||
-        sc[2] <= 0 || !(tol >= 0.0)) {
-        set_global_error("ether_build_neighbours: invalid sizes");
+    if (n_basis <= 0 || n_shells <= 0 || sc[0] <= 0 || sc[1] <= 0 || sc[2] <= 0 || !(tol >= 0.0)) {
+        err = "invalid sizes";
         return ETHER_ERR_ARG;
     }
-    const long long n_ll = (long long)sc[0] * sc[1] * sc[2] * n_basis;
-    if (n_ll > (1LL << 24)) {
-        set_global_error("ether_build_neighbours: more than 2^24 sites");
+    if ((long long)sc[0] * sc[1] * sc[2] * n_basis > (1LL << 24)) {
+        err = "more than 2^24 sites";
         return ETHER_ERR_LIMIT;
     }
-    const int n = (int)n_ll;
     double inv[9];
     if (!invert3(abc, inv)) {
-        set_global_error("ether_build_neighbours: singular lattice vectors");
+        err = "singular lattice vectors";
         return ETHER_ERR_ARG;
     }
     // |column k of A^{-1}| bounds |D_k| for a displacement of length r: D = p.A^{-1}
@@ -101,9 +96,9 @@ extern "C" int ether_build_neighbours(const double* abc, int32_t n_basis, const 
             }
             maxdiff = std::max(maxdiff, std::sqrt(r2));
         }
-
-    std::vector<int> nbr;
-    nbr.reserve(64);
+    out.n_shells = n_shells;
+    out.n_basis = n_basis;
+    out.t.assign((size_t)n_shells * n_basis, {});
     for (int sh = 0; sh < n_shells; ++sh) {
         const double d = shell_d[sh];
         int lo[3], hi[3];
@@ -116,8 +111,6 @@ extern "C" int ether_build_neighbours(const double* abc, int32_t n_basis, const 
             lo[k] = std::max(-b, lim_lo);
             hi[k] = std::min(b, lim_hi);
         }
-        // template per basis site la
-        std::vector<std::vector<TemplateEntry>> tmpl(n_basis);
         for (int la = 0; la < n_basis; ++la)
             for (int lb = 0; lb < n_basis; ++lb)
                 for (int d0 = lo[0]; d0 <= hi[0]; ++d0)
@@ -131,8 +124,144 @@ extern "C" int ether_build_neighbours(const double* abc, int32_t n_basis, const 
                                 r2 += dr * dr;
                             }
                             if (std::fabs(d - std::sqrt(r2)) <= tol)
-                                tmpl[la].push_back({{d0, d1, d2}, lb});
+                                out.t[(size_t)sh * n_basis + la].push_back({{d0, d1, d2}, lb});
                         }
+    }
+    return ETHER_OK;
+}
+
+bool periodic_colouring(const LatticeTemplates& tp, const int32_t* sc, const int32_t* open_bc,
+                        PeriodicColouring& out) {
+    static const int kPeriods[] = {1, 2, 3, 4, 6};
+    const int nb = tp.n_basis;
+    bool found = false;
+    out = PeriodicColouring();
+    for (int i0 : kPeriods)
+        for (int i1 : kPeriods)
+            for (int i2 : kPeriods) {
+                const int m[3] = {i0, i1, i2};
+                bool ok = true;
+                for (int k = 0; k < 3 && ok; ++k)
+                    if (!open_bc[k] && sc[k] % m[k] != 0) ok = false;
+                if (!ok) continue;
+                const int cells = m[0] * m[1] * m[2];
+                const int Q = cells * nb;
+                if (Q > 1024) continue;
+                if (found && (Q >= (int)out.qcol.size() && out.n_colours <= 2)) continue;
+                // the quotient graph: vertex (la, residue r) joins (lb, r + d mod m)
+                std::vector<std::vector<int>> adj(Q);
+                for (int sh = 0; sh < tp.n_shells && ok; ++sh)
+                    for (int la = 0; la < nb && ok; ++la)
+                        for (const TemplateEntry& e : tp.t[(size_t)sh * nb + la]) {
+                            // an entry that lands on the site itself is no bond (j != i)
+                            bool self = e.lb == la;
+                            for (int k = 0; k < 3 && self; ++k)
+                                if (open_bc[k] ? e.d[k] != 0 : e.d[k] % sc[k] != 0) self = false;
+                            if (self) continue;
+                            for (int r = 0; r < cells && ok; ++r) {
+                                const int r0 = r % m[0], r1 = (r / m[0]) % m[1], r2 = r / (m[0] * m[1]);
+                                auto md = [](int x, int p) { return ((x % p) + p) % p; };
+                                const int w0 = md(r0 + e.d[0], m[0]), w1 = md(r1 + e.d[1], m[1]),
+                                          w2 = md(r2 + e.d[2], m[2]);
+                                const int v = la + nb * r, w = e.lb + nb * (w0 + m[0] * (w1 + m[1] * w2));
+                                if (v == w) {
+                                    ok = false;   // a bond within one quotient vertex
+                                    break;
+                                }
+                                adj[v].push_back(w);
+                                adj[w].push_back(v);
+                            }
+                        }
+                if (!ok) continue;
+                // greedy in two orders (basis innermost, basis outermost) and
+                // DSATUR (Brelaz 1979: next the vertex seeing the most colours,
+                // ties to the most neighbours, then the lowest index), the best kept
+                std::vector<int> best;
+                int best_n = 1 << 30;
+                {
+                    std::vector<int> col(Q, -1), deg(Q);
+                    std::vector<std::vector<char>> seen(Q);
+                    std::vector<int> sat(Q, 0);
+                    for (int v = 0; v < Q; ++v) {
+                        std::sort(adj[v].begin(), adj[v].end());
+                        adj[v].erase(std::unique(adj[v].begin(), adj[v].end()), adj[v].end());
+                        deg[v] = (int)adj[v].size();
+                    }
+                    int n_col = 0;
+                    for (int step = 0; step < Q; ++step) {
+                        int v = -1;
+                        for (int u = 0; u < Q; ++u) {
+                            if (col[u] >= 0) continue;
+                            if (v < 0 || sat[u] > sat[v] || (sat[u] == sat[v] && deg[u] > deg[v])) v = u;
+                        }
+                        std::vector<char> used(n_col + 1, 0);
+                        for (int w : adj[v])
+                            if (col[w] >= 0) used[col[w]] = 1;
+                        int c = 0;
+                        while (used[c]) ++c;
+                        col[v] = c;
+                        n_col = std::max(n_col, c + 1);
+                        for (int w : adj[v]) {
+                            if ((int)seen[w].size() <= c) seen[w].resize(c + 1, 0);
+                            if (!seen[w][c]) {
+                                seen[w][c] = 1;
+                                ++sat[w];
+                            }
+                        }
+                    }
+                    best_n = n_col;
+                    best.swap(col);
+                }
+                for (int ord = 0; ord < 2; ++ord) {
+                    std::vector<int> col(Q, -1);
+                    int n_col = 0;
+                    for (int q = 0; q < Q; ++q) {
+                        const int v = ord == 0 ? q : (q % cells) * nb + q / cells;
+                        std::vector<char> used(n_col + 1, 0);
+                        for (int w : adj[v])
+                            if (col[w] >= 0 && col[w] <= n_col) used[col[w]] = 1;
+                        int c = 0;
+                        while (used[c]) ++c;
+                        col[v] = c;
+                        n_col = std::max(n_col, c + 1);
+                    }
+                    if (n_col < best_n) {
+                        best_n = n_col;
+                        best.swap(col);
+                    }
+                }
+                if (!found || best_n < out.n_colours || (best_n == out.n_colours && Q < (int)out.qcol.size())) {
+                    found = true;
+                    out.n_colours = best_n;
+                    out.qcol.swap(best);
+                    for (int k = 0; k < 3; ++k) out.m[k] = m[k];
+                }
+            }
+    return found;
+}
+
+}  // namespace ether
+
+extern "C" int ether_build_neighbours(const double* abc, int32_t n_basis, const double* basis_xyz,
+                                      const int32_t* sc, const int32_t* open_bc,
+                                      int32_t n_shells, const double* shell_d, double tol,
+                                      int32_t capacity, int32_t* nn_count, int32_t* nn_idx) {
+    using ether::set_global_error;
+    if (!nn_count || !nn_idx || capacity <= 0) {
+        set_global_error("ether_build_neighbours: null output or capacity <= 0");
+        return ETHER_ERR_ARG;
+    }
+    ether::LatticeTemplates tp;
+    std::string err;
+    const int rc = ether::build_templates(abc, n_basis, basis_xyz, sc, open_bc, n_shells, shell_d, tol, tp, err);
+    if (rc != ETHER_OK) {
+        set_global_error("ether_build_neighbours: " + err);
+        return rc;
+    }
+    const int n = sc[0] * sc[1] * sc[2] * n_basis;
+    std::vector<int> nbr;
+    nbr.reserve(64);
+    for (int sh = 0; sh < n_shells; ++sh) {
         for (int ck = 0; ck < sc[2]; ++ck)
             for (int cj = 0; cj < sc[1]; ++cj)
                 for (int ci = 0; ci < sc[0]; ++ci)
@@ -140,7 +269,7 @@ extern "C" int ether_build_neighbours(const double* abc, int32_t n_basis, const 
                         const int c[3] = {ci, cj, ck};
                         const int i = ((ck * sc[1] + cj) * sc[0] + ci) * n_basis + la;
                         nbr.clear();
-                        for (const TemplateEntry& t : tmpl[la]) {
+                        for (const TemplateEntry& t : tp.t[(size_t)sh * n_basis + la]) {
                             int cell[3];
                             bool ok = true;
                             for (int k = 0; k < 3 && ok; ++k) {
@@ -171,6 +300,36 @@ extern "C" int ether_build_neighbours(const double* abc, int32_t n_basis, const 
                         for (int q = (int)nbr.size(); q < capacity; ++q) out[q] = -1;
                     }
     }
+    return ETHER_OK;
+}
+
+extern "C" int ether_colour_lattice(const double* abc, int32_t n_basis, const double* basis_xyz,
+                                    const int32_t* sc, const int32_t* open_bc, int32_t n_shells,
+                                    const double* shell_d, double tol, int32_t* colour, int32_t* n_colours) {
+    using ether::set_global_error;
+    if (!colour || !n_colours) {
+        set_global_error("ether_colour_lattice: null output");
+        return ETHER_ERR_ARG;
+    }
+    ether::LatticeTemplates tp;
+    std::string err;
+    const int rc = ether::build_templates(abc, n_basis, basis_xyz, sc, open_bc, n_shells, shell_d, tol, tp, err);
+    if (rc != ETHER_OK) {
+        set_global_error("ether_colour_lattice: " + err);
+        return rc;
+    }
+    ether::PeriodicColouring pc;
+    *n_colours = 0;
+    if (!ether::periodic_colouring(tp, sc, open_bc, pc)) return ETHER_OK;
+    const int nb = n_basis;
+    for (int ck = 0; ck < sc[2]; ++ck)
+        for (int cj = 0; cj < sc[1]; ++cj)
+            for (int ci = 0; ci < sc[0]; ++ci)
+                for (int la = 0; la < nb; ++la) {
+                    const int i = ((ck * sc[1] + cj) * sc[0] + ci) * nb + la;
+                    colour[i] = pc.qcol[la + nb * (ci % pc.m[0] + pc.m[0] * (cj % pc.m[1] + pc.m[1] * (ck % pc.m[2])))];
+                }
+    *n_colours = pc.n_colours;
     return ETHER_OK;
 }
 

@@ -24,11 +24,19 @@ from .inputs import EtherInputError, GpuTags, Model, fortran_nint
 
 
 def attach_neighbours(model: Model) -> Model:
-    """getting_nbd (src/getting_nbd.f90:46-82) through the library's builder."""
+    """getting_nbd (src/getting_nbd.f90:46-82) through the library's device
+    builder, with the periodic colouring of the lattice; without a GPU the
+    host builder and the host colouring give the same lists and colours."""
     t = model.tags
-    cnt, idx = E.build_neighbours(model.structure.abc, model.basis_xyz, model.sc,
-                                  [b == "o" for b in t.bc], model.shells, t.nbdfc, t.nbd_capacity)
-    model.nn_count, model.nn_idx = cnt, idx
+    args = (model.structure.abc, model.basis_xyz, model.sc, [b == "o" for b in t.bc], model.shells, t.nbdfc)
+    try:
+        cnt, idx, col = E.build_neighbours_device(*args, t.nbd_capacity)
+    except E.EtherError as e:
+        if e.code != E.ETHER_ERR_NODEV:
+            raise
+        cnt, idx = E.build_neighbours(*args, t.nbd_capacity)
+        col = E.colour_lattice(*args)
+    model.nn_count, model.nn_idx, model.colour = cnt, idx, col
     return model
 
 
@@ -51,7 +59,8 @@ def make_engine(model: Model, n_replicas: int, *, compat: int = 0, seed: Optiona
               model.j_exc.reshape(-1), model.spin,
               zeeman=model.h if t.zeeman else None, g_factor=model.g_factor, mu_b=model.mu_b,
               sia=model.sia_vec.reshape(-1) if model.sia_vec is not None else None,
-              angle=t.angle, dphi=t.dphi, compat=compat, colour=colour, spiral=spiral)
+              angle=t.angle, dphi=t.dphi, compat=compat,
+              colour=colour if colour is not None else model.colour, spiral=spiral)
     return eng
 
 

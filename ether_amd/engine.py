@@ -46,6 +46,10 @@ _vp = C.c_void_p
 EXPORTS = [
     ("ether_build_neighbours", C.c_int,
      [_vp, C.c_int32, _vp, _vp, _vp, C.c_int32, _vp, C.c_double, C.c_int32, _vp, _vp]),
+    ("ether_build_neighbours_device", C.c_int,
+     [_vp, C.c_int32, _vp, _vp, _vp, C.c_int32, _vp, C.c_double, C.c_int32, _vp, _vp, _vp, _vp]),
+    ("ether_colour_lattice", C.c_int,
+     [_vp, C.c_int32, _vp, _vp, _vp, C.c_int32, _vp, C.c_double, _vp, _vp]),
     ("ether_build_spiral_partners", C.c_int,
      [C.c_int32, _vp, _vp, C.c_double, C.c_double, C.c_int32, _vp, _vp]),
     ("ether_start_state", C.c_int, [C.c_uint64, C.c_int32, C.c_int32, C.c_int32, _vp, _vp, _vp]),
@@ -166,6 +170,43 @@ def start_state(seed, model, angle, file_index):
     _check(lib.ether_start_state(C.c_uint64(int(seed)), int(model), int(angle), fi.size, _ptr(fi),
                                  _ptr(S), _ptr(phi)))
     return S, phi
+
+
+def build_neighbours_device(abc, basis_xyz, sc, open_bc, shell_d, tol=1e-5, capacity=20):
+    """Neighbour lists built on the GPU, and the periodic colouring:
+    (nn_count (shells, N), nn_idx (shells, N, capacity), colour (N,) or None)."""
+    lib = load_library()
+    abc = _arr(abc, np.float64).reshape(3, 3)
+    basis = _arr(basis_xyz, np.float64).reshape(-1, 3)
+    sc = _arr(sc, np.int32)
+    obc = _arr([1 if b else 0 for b in open_bc], np.int32)
+    shells = _arr(shell_d, np.float64)
+    n = int(np.prod(sc)) * basis.shape[0]
+    cnt = np.zeros(len(shells) * n, np.int32)
+    idx = np.full(len(shells) * n * capacity, -1, np.int32)
+    col = np.zeros(n, np.int32)
+    nc = C.c_int32()
+    _check(lib.ether_build_neighbours_device(_ptr(abc), basis.shape[0], _ptr(basis), _ptr(sc), _ptr(obc),
+                                             len(shells), _ptr(shells), float(tol), int(capacity),
+                                             _ptr(cnt), _ptr(idx), _ptr(col), C.byref(nc)))
+    return (cnt.reshape(len(shells), n), idx.reshape(len(shells), n, capacity),
+            col if nc.value > 0 else None)
+
+
+def colour_lattice(abc, basis_xyz, sc, open_bc, shell_d, tol=1e-5):
+    """The periodic colouring on the host: colour (N,) or None."""
+    lib = load_library()
+    abc = _arr(abc, np.float64).reshape(3, 3)
+    basis = _arr(basis_xyz, np.float64).reshape(-1, 3)
+    sc = _arr(sc, np.int32)
+    obc = _arr([1 if b else 0 for b in open_bc], np.int32)
+    shells = _arr(shell_d, np.float64)
+    n = int(np.prod(sc)) * basis.shape[0]
+    col = np.zeros(n, np.int32)
+    nc = C.c_int32()
+    _check(lib.ether_colour_lattice(_ptr(abc), basis.shape[0], _ptr(basis), _ptr(sc), _ptr(obc), len(shells),
+                                    _ptr(shells), float(tol), _ptr(col), C.byref(nc)))
+    return col if nc.value > 0 else None
 
 
 def build_neighbours(abc, basis_xyz, sc, open_bc, shell_d, tol=1e-5, capacity=20):

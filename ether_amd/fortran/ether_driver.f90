@@ -331,6 +331,8 @@ program ether_gpu_driver
    integer :: start, nchunk, step, sw, rc, ovrr_iv, cnt, cap, first_counted, rd, crep, ncrep, compat_all
    logical :: chain
    integer, allocatable :: b_sp(:), b_idx(:), species(:), nn_count(:), nn_idx(:), gstart(:)
+   integer(c_int), allocatable, target :: colour(:)
+   integer(c_int) :: ncol_p
    integer(c_int), allocatable :: sp_partners(:)
    integer(c_int64_t) :: sp_count
    real(dp), allocatable :: sp_sums(:, :), sp_per(:, :, :), sp_mean(:, :), sp_sem(:, :), pos(:, :)
@@ -497,7 +499,17 @@ program ether_gpu_driver
    do a = 1, 3
       obc(a) = merge(1, 0, bc(a) == 'o')
    end do
-   rc = ether_build_neighbours(lat, nb, b_xyz(:, 1:nb), sc, obc, nsh, shells, nbdfc, cap, nn_count, nn_idx)
+   ! on the GPU, with the periodic colouring; the host builder and colouring
+   ! (the same lists and colours) where no GPU is visible
+   allocate (colour(n))
+   ncol_p = 0
+   rc = ether_build_neighbours_device(lat, nb, b_xyz(:, 1:nb), sc, obc, nsh, shells, nbdfc, cap, &
+                                      nn_count, nn_idx, colour, ncol_p)
+   if (rc == ETHER_ERR_NODEV) then
+      rc = ether_build_neighbours(lat, nb, b_xyz(:, 1:nb), sc, obc, nsh, shells, nbdfc, cap, nn_count, nn_idx)
+      if (rc == ETHER_OK) rc = ether_colour_lattice(lat, nb, b_xyz(:, 1:nb), sc, obc, nsh, shells, nbdfc, &
+                                                    colour, ncol_p)
+   end if
    if (rc /= ETHER_OK) call terminate(ether_message(c_null_ptr))
    write (6, '(a)') ' ==> Neighbourhood informations are created now'
 
@@ -532,7 +544,11 @@ program ether_gpu_driver
    rc = ether_ctx_create(merge(ETHER_MODEL_ISING, ETHER_MODEL_XYZ, ising), n, rd, &
                          int(seed_value, c_int64_t), ctx)
    if (rc /= ETHER_OK) call terminate(ether_message(c_null_ptr))
-   call chk(ether_set_lattice(ctx, nspecies, species, stagger, nsh, cap, nn_count, nn_idx, c_null_ptr))
+   if (ncol_p > 0) then
+      call chk(ether_set_lattice(ctx, nspecies, species, stagger, nsh, cap, nn_count, nn_idx, c_loc(colour)))
+   else
+      call chk(ether_set_lattice(ctx, nspecies, species, stagger, nsh, cap, nn_count, nn_idx, c_null_ptr))
+   end if
    call chk(ether_set_exchange(ctx, jx, spin))
    call chk(ether_set_zeeman(ctx, merge(1, 0, zeeman), h, gf, mub))
    call chk(ether_set_sia(ctx, merge(1, 0, sia), siav))

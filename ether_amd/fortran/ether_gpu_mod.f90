@@ -7,6 +7,7 @@ module ether_gpu
    implicit none
 
    integer(c_int), parameter :: ETHER_OK = 0
+   integer(c_int), parameter :: ETHER_ERR_NODEV = 5
    integer(c_int), parameter :: ETHER_MODEL_ISING = 0, ETHER_MODEL_XYZ = 1
    integer(c_int), parameter :: ETHER_INIT_FRESH = 1, ETHER_INIT_RANDOM = 2
    integer(c_int), parameter :: ETHER_COMPAT_PT_ETOT = 1
@@ -24,6 +25,29 @@ module ether_gpu
          integer(c_int), intent(in) :: sc(*), open_bc(*)
          real(c_double), value :: tol
          integer(c_int), intent(out) :: nn_count(*), nn_idx(*)
+      end function
+
+      integer(c_int) function ether_build_neighbours_device(abc, n_basis, basis_xyz, sc, open_bc, &
+            n_shells, shell_d, tol, capacity, nn_count, nn_idx, colour, n_colours) &
+            bind(C, name="ether_build_neighbours_device")
+         import :: c_int, c_double
+         real(c_double), intent(in) :: abc(*), basis_xyz(*), shell_d(*)
+         integer(c_int), value :: n_basis, n_shells, capacity
+         integer(c_int), intent(in) :: sc(*), open_bc(*)
+         real(c_double), value :: tol
+         integer(c_int), intent(out) :: nn_count(*), nn_idx(*), colour(*)
+         integer(c_int), intent(out) :: n_colours
+      end function
+
+      integer(c_int) function ether_colour_lattice(abc, n_basis, basis_xyz, sc, open_bc, n_shells, &
+            shell_d, tol, colour, n_colours) bind(C, name="ether_colour_lattice")
+         import :: c_int, c_double
+         real(c_double), intent(in) :: abc(*), basis_xyz(*), shell_d(*)
+         integer(c_int), value :: n_basis, n_shells
+         integer(c_int), intent(in) :: sc(*), open_bc(*)
+         real(c_double), value :: tol
+         integer(c_int), intent(out) :: colour(*)
+         integer(c_int), intent(out) :: n_colours
       end function
 
       integer(c_int) function ether_start_state(seed, model, angle, n_basis, file_index, S, phi) &

@@ -319,6 +319,7 @@ class Model:
     kb: float
     nn_count: Optional[np.ndarray] = None
     nn_idx: Optional[np.ndarray] = None
+    colour: Optional[np.ndarray] = None   # periodic colouring of the neighbour graph (None: greedy at commit)
     initial_spins: Optional[np.ndarray] = None  # (N, 3) generate_supercell start state
     initial_phi: Optional[np.ndarray] = None
     gpu: "GpuTags" = field(default_factory=lambda: GpuTags())

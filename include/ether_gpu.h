@@ -105,6 +105,32 @@ int ether_build_neighbours(const double* abc, int32_t n_basis, const double* bas
                            int32_t n_shells, const double* shell_d, double tol,
                            int32_t capacity, int32_t* nn_count, int32_t* nn_idx);
 
+/* The same neighbour lists built on the GPU (one thread per site and shell
+ * applying the shell's offset template for the site's basis atom; identical
+ * output, including the capacity error), plus a colouring of the neighbour
+ * graph (all shells) that repeats with the lattice:
+ *   colour(site) depends only on its basis atom and its cell index modulo a
+ *   period m_k per axis (m_k in {1, 2, 3, 4, 6}, dividing L_k on closed axes),
+ *   the fewest colours over the periods tried (2 for even simple-cubic
+ *   lattices, 3 for triangular ones with L divisible by 3).
+ * colour[N] (nullable) and n_colours (nullable) receive it; n_colours = 0
+ * when no period gives a proper colouring (ether_commit then colours
+ * greedily).  Pass colour to ether_set_lattice.  Needs a GPU
+ * (ETHER_ERR_NODEV otherwise); no counterpart in the reference beyond the
+ * lists of src/getting_nbd.f90:46-82, whose colouring is none (a typewriter
+ * sweep, src/Metropolis.f90:39). */
+int ether_build_neighbours_device(const double* abc, int32_t n_basis, const double* basis_xyz,
+                                  const int32_t* sc, const int32_t* open_bc,
+                                  int32_t n_shells, const double* shell_d, double tol,
+                                  int32_t capacity, int32_t* nn_count, int32_t* nn_idx,
+                                  int32_t* colour, int32_t* n_colours);
+
+/* The periodic colouring alone, on the host (the same colours as
+ * ether_build_neighbours_device gives). */
+int ether_colour_lattice(const double* abc, int32_t n_basis, const double* basis_xyz,
+                         const int32_t* sc, const int32_t* open_bc, int32_t n_shells,
+                         const double* shell_d, double tol, int32_t* colour, int32_t* n_colours);
+
 /* ---------------------------------------------------------------------------
  * Spiral-state partners: replaces get_spiral_state('ions_for_spiral')
  * (src/get_spiral_state.f90:47-82), which fills ifs(spiral_capacity, N) and

@@ -72,3 +72,37 @@ def test_capacity_overflow_raises():
     from ether_amd import EtherError
     with pytest.raises(EtherError):
         build_neighbours(np.eye(3), [[0, 0, 0]], (4, 4, 4), (0, 0, 0), [1.0], 1e-5, 4)
+
+
+def _check_colouring(col, cnt, idx):
+    """Proper over every shell: no bond joins two sites of one colour."""
+    assert col is not None
+    for s in range(cnt.shape[0]):
+        for i in range(cnt.shape[1]):
+            nb = idx[s, i, : cnt[s, i]]
+            assert np.all(col[nb] != col[i]), (s, i)
+
+
+@pytest.mark.parametrize("name,over,ncol", [
+    ("sc_heis_afm_32", dict(sc="8 8 8"), 2),          # even simple cubic: the checkerboard
+    ("sc_heis_afm_32", dict(sc="9 6 6"), 3),          # an odd axis: period 3 along it
+    ("tri_afm_96", dict(sc="12 12 1"), 3),            # triangular, L divisible by 3
+    ("pyro_pt_16", dict(sc="4 4 4"), 4),              # pyrochlore J1 + J2: the tetrahedra need 4
+])
+def test_periodic_colouring_is_proper_and_small(name, over, ncol):
+    from ether_amd.engine import colour_lattice
+    m = workloads.model(name, **over)
+    obc = [b == "o" for b in m.tags.bc]
+    cnt, idx = build_neighbours(m.structure.abc, m.basis_xyz, m.sc, obc, m.shells, m.tags.nbdfc, 20)
+    col = colour_lattice(m.structure.abc, m.basis_xyz, m.sc, obc, m.shells, m.tags.nbdfc)
+    _check_colouring(col, cnt, idx)
+    assert col.max() + 1 == ncol
+    # a function of the basis atom and the cell modulo the period only
+    assert col.size == m.n_sites
+
+
+def test_no_periodic_colouring_for_a_prime_odd_axis():
+    """A 7-site ring of nearest neighbours has no colouring of period 1..6:
+    the colouring is left to the greedy one at commit."""
+    from ether_amd.engine import colour_lattice
+    assert colour_lattice(np.eye(3) * [1, 1, 10], [[0, 0, 0]], (7, 1, 1), (0, 0, 0), [1.0]) is None

@@ -410,18 +410,26 @@ def batch_bench(mcs, n_lattices=256):
     from ether_amd import engine as E
     from ether_amd import workloads
     from ether_amd.driver import attach_neighbours, make_engine
+    from concurrent.futures import ThreadPoolExecutor
     t_setup = time.perf_counter()
     specs = workloads.c5_lattices(1992, n_lattices)
-    engines, updates = {"ising": [], "xyz": []}, 0.0
-    for sp in specs:
+
+    def one(sp):
         m = workloads.c5_model(sp)
         attach_neighbours(m)
         eng = make_engine(m, sp["R"], seed=sp["seed"])
         eng.set_betas(m.betas(workloads.c5_temperatures(sp)))
         eng.set_ladders([0, sp["R"]])
         eng.init_spins(E.INIT_RANDOM)
-        engines["ising" if m.tags.ising else "xyz"].append(eng)
-        updates += float(m.n_sites) * sp["R"]
+        return m, eng
+
+    # contexts are independent: their set-up (neighbours, commit) runs in
+    # native code on several host threads at once
+    engines, updates = {"ising": [], "xyz": []}, 0.0
+    with ThreadPoolExecutor(max_workers=8) as pool:
+        for sp, (m, eng) in zip(specs, pool.map(one, specs)):
+            engines["ising" if m.tags.ising else "xyz"].append(eng)
+            updates += float(m.n_sites) * sp["R"]
     setup_s = time.perf_counter() - t_setup
     allg = engines["ising"] + engines["xyz"]
     batches = [E.Batch(v) for v in engines.values() if v]

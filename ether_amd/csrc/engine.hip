@@ -193,6 +193,60 @@ static int upload(ether_ctx* c, T* d, const T* h, size_t n) {
     return ETHER_OK;
 }
 
+// One device allocation carved into 256-byte aligned buffers, laid out in
+// three runs: tables (copied from one host image), buffers that start at zero
+// (one fill) and buffers the kernels write before they read.  Pointers are
+// assigned when commit() has allocated the block; the block is freed with the
+// context.
+class Arena {
+  public:
+    template <class T>
+    void table(T** p, const std::vector<T>& h) {
+        place((void**)p, h.size() * sizeof(T));
+        src_.push_back({off_.back(), h.data(), h.size() * sizeof(T)});
+    }
+    template <class T>
+    void zeroed(T** p, size_t n) { place((void**)p, n * sizeof(T)); }
+    template <class T>
+    void plain(T** p, size_t n) { place((void**)p, n * sizeof(T)); }
+    void end_tables() { tables_end_ = top_; }
+    void end_zeroed() { zero_end_ = top_; }
+    int commit(ether_ctx* c) {
+        char* base = nullptr;
+        hipError_t e = hipMalloc((void**)&base, std::max<size_t>(top_, 256));
+        if (e != hipSuccess)
+            return fail(c, ETHER_ERR_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        c->allocs.push_back(base);
+        for (size_t k = 0; k < dst_.size(); ++k) *dst_[k] = base + off_[k];
+        if (tables_end_) {
+            image_.assign(tables_end_, 0);
+            for (const Src& s : src_) std::memcpy(image_.data() + s.off, s.h, s.bytes);
+            // the image is freed with the Arena: wait for the copy
+            HIPCHK(c, hipMemcpyAsync(base, image_.data(), tables_end_, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        if (zero_end_ > tables_end_)
+            HIPCHK(c, hipMemsetAsync(base + tables_end_, 0, zero_end_ - tables_end_, c->stream));
+        return ETHER_OK;
+    }
+  private:
+    struct Src {
+        size_t off;
+        const void* h;
+        size_t bytes;
+    };
+    void place(void** p, size_t bytes) {
+        dst_.push_back(p);
+        off_.push_back(top_);
+        top_ += (std::max<size_t>(bytes, 1) + 255) / 256 * 256;
+    }
+    std::vector<void**> dst_;
+    std::vector<size_t> off_;
+    std::vector<Src> src_;
+    std::vector<char> image_;
+    size_t top_ = 0, tables_end_ = 0, zero_end_ = 0;
+};
+
 // The acceptance counters are kAccCopies copies of [R] (the sweeps spread
 // their atomics over them): an asynchronous copy of all of them, and the
 // total of one slot once the stream has been synchronised.
@@ -464,12 +518,50 @@ extern "C" int ether_commit(ether_ctx* c) {
     const int N = c->N, nsp = c->nsp, ns = c->nshell, cap = c->cap;
     const bool ising = c->model == ETHER_MODEL_ISING;
 
-    // ---- coupling classes and per-site edges (zero couplings dropped) ----
+    // ---- coupling classes and per-site bonds (zero couplings dropped), CSR ----
+    // A bond's coupling is a function of (shell, species of i, species of j):
+    // each such triple is resolved to a class once, when the walk over (site,
+    // shell, neighbour) first meets it, so classes are numbered in the order
+    // their couplings first appear.
     std::map<std::tuple<float, float, float>, int> fcls;
     std::map<double, int> icls;
-    std::vector<std::vector<Edge>> edges(N);
     c->jclass.assign(3, 0.f);
     c->ising_j.assign(1, 0.0);
+    std::vector<int> tcls((size_t)ns * nsp * nsp, -2);   // -2 not met yet, -1 zero coupling
+    auto class_of = [&](int s, int a, int b) -> int {
+        int& t = tcls[((size_t)s * nsp + a) * nsp + b];
+        if (t != -2) return t;
+        const double* J = c->jexc.data() + (((size_t)s * nsp + a) * nsp + b) * 3;
+        if (ising) {
+            if (J[2] == 0.0) return t = -1;
+            auto it = icls.find(J[2]);
+            if (it != icls.end()) return t = it->second;
+            const int cls = (int)c->ising_j.size();
+            icls[J[2]] = cls;
+            c->ising_j.push_back(J[2]);
+            c->jclass.push_back(0.f);
+            c->jclass.push_back(0.f);
+            c->jclass.push_back((float)J[2]);
+            return t = cls;
+        }
+        if (J[0] == 0.0 && J[1] == 0.0 && J[2] == 0.0) return t = -1;
+        auto key = std::make_tuple((float)J[0], (float)J[1], (float)J[2]);
+        auto it = fcls.find(key);
+        if (it != fcls.end()) return t = it->second;
+        const int cls = (int)(c->jclass.size() / 3);
+        fcls[key] = cls;
+        c->jclass.push_back((float)J[0]);
+        c->jclass.push_back((float)J[1]);
+        c->jclass.push_back((float)J[2]);
+        return t = cls;
+    };
+    std::vector<int> e_ptr(N + 1, 0);
+    std::vector<Edge> edge;
+    {
+        size_t total = 0;
+        for (size_t q = 0; q < (size_t)ns * N; ++q) total += (size_t)c->nn_count[q];
+        edge.reserve(total);
+    }
     for (int i = 0; i < N; ++i) {
         const int a = c->species[i] - 1;
         for (int s = 0; s < ns; ++s) {
@@ -477,39 +569,11 @@ extern "C" int ether_commit(ether_ctx* c) {
             const int32_t* nb = c->nn_idx.data() + ((size_t)s * N + i) * cap;
             for (int k = 0; k < cnt; ++k) {
                 const int j = nb[k];
-                const int b = c->species[j] - 1;
-                const double* J = c->jexc.data() + (((size_t)s * nsp + a) * nsp + b) * 3;
-                int cls;
-                if (ising) {
-                    if (J[2] == 0.0) continue;
-                    auto it = icls.find(J[2]);
-                    if (it == icls.end()) {
-                        cls = (int)c->ising_j.size();
-                        icls[J[2]] = cls;
-                        c->ising_j.push_back(J[2]);
-                        c->jclass.push_back(0.f);
-                        c->jclass.push_back(0.f);
-                        c->jclass.push_back((float)J[2]);
-                    } else {
-                        cls = it->second;
-                    }
-                } else {
-                    if (J[0] == 0.0 && J[1] == 0.0 && J[2] == 0.0) continue;
-                    auto key = std::make_tuple((float)J[0], (float)J[1], (float)J[2]);
-                    auto it = fcls.find(key);
-                    if (it == fcls.end()) {
-                        cls = (int)(c->jclass.size() / 3);
-                        fcls[key] = cls;
-                        c->jclass.push_back((float)J[0]);
-                        c->jclass.push_back((float)J[1]);
-                        c->jclass.push_back((float)J[2]);
-                    } else {
-                        cls = it->second;
-                    }
-                }
-                edges[i].push_back({j, cls});
+                const int cls = class_of(s, a, c->species[j] - 1);
+                if (cls >= 0) edge.push_back({j, cls});
             }
         }
+        e_ptr[i + 1] = (int)edge.size();
     }
     c->n_classes = (int)(c->jclass.size() / 3) - 1;
     if (c->n_classes > kMaxClasses)
@@ -520,23 +584,6 @@ extern "C" int ether_commit(ether_ctx* c) {
         return fail(c, ETHER_ERR_LIMIT, "Ising path supports at most 8 species");
 
     // ---- colouring ----
-    std::vector<int> adj_ptr(N + 1, 0), adj;
-    {
-        std::vector<std::vector<int>> und(N);
-        for (int i = 0; i < N; ++i)
-            for (const Edge& e : edges[i])
-                if (e.j != i) {
-                    und[i].push_back(e.j);
-                    und[e.j].push_back(i);
-                }
-        for (int i = 0; i < N; ++i) {
-            std::sort(und[i].begin(), und[i].end());
-            und[i].erase(std::unique(und[i].begin(), und[i].end()), und[i].end());
-            adj_ptr[i + 1] = adj_ptr[i] + (int)und[i].size();
-        }
-        adj.reserve(adj_ptr[N]);
-        for (int i = 0; i < N; ++i) adj.insert(adj.end(), und[i].begin(), und[i].end());
-    }
     std::vector<int> col;
     if (!c->colour_in.empty()) {
         col = c->colour_in;
@@ -546,11 +593,37 @@ extern "C" int ether_commit(ether_ctx* c) {
             mx = std::max(mx, col[i]);
         }
         for (int i = 0; i < N; ++i)
-            for (int q = adj_ptr[i]; q < adj_ptr[i + 1]; ++q)
-                if (col[adj[q]] == col[i])
+            for (int q = e_ptr[i]; q < e_ptr[i + 1]; ++q)
+                if (edge[q].j != i && col[edge[q].j] == col[i])
                     return fail(c, ETHER_ERR_ARG, "colouring joins two interacting sites");
         c->ncol = mx + 1;
     } else {
+        // the undirected interaction graph (self bonds dropped), rows sorted
+        std::vector<int> adj_ptr(N + 1, 0), adj;
+        {
+            std::vector<int> deg(N + 1, 0);
+            for (int i = 0; i < N; ++i)
+                for (int q = e_ptr[i]; q < e_ptr[i + 1]; ++q)
+                    if (edge[q].j != i) {
+                        deg[i + 1]++;
+                        deg[edge[q].j + 1]++;
+                    }
+            for (int i = 0; i < N; ++i) deg[i + 1] += deg[i];
+            std::vector<int> raw(deg[N]), fill(deg.begin(), deg.end() - 1);
+            for (int i = 0; i < N; ++i)
+                for (int q = e_ptr[i]; q < e_ptr[i + 1]; ++q)
+                    if (edge[q].j != i) {
+                        raw[fill[i]++] = edge[q].j;
+                        raw[fill[edge[q].j]++] = i;
+                    }
+            adj.reserve(raw.size());
+            for (int i = 0; i < N; ++i) {
+                std::sort(raw.begin() + deg[i], raw.begin() + deg[i + 1]);
+                for (int q = deg[i]; q < deg[i + 1]; ++q)
+                    if (q == deg[i] || raw[q] != raw[q - 1]) adj.push_back(raw[q]);
+                adj_ptr[i + 1] = (int)adj.size();
+            }
+        }
         // natural order, then basis-major orders for every plausible basis size
         std::vector<int> order(N), best;
         for (int i = 0; i < N; ++i) order[i] = i;
@@ -586,7 +659,7 @@ extern "C" int ether_commit(ether_ctx* c) {
     }
     // ---- ELL ----
     int D = 0;
-    for (int i = 0; i < N; ++i) D = std::max(D, (int)edges[i].size());
+    for (int i = 0; i < N; ++i) D = std::max(D, e_ptr[i + 1] - e_ptr[i]);
     // rows padded (class 0, zero coupling) to an unrolled width bucket
     {
         const int b = ell_bucket(std::max(D, 1));
@@ -596,10 +669,9 @@ extern "C" int ether_commit(ether_ctx* c) {
     // slot-major rows: a wave's whole neighbour list is one scalar-loadable run
     std::vector<uint32_t> ell((size_t)std::max(D, 1) * N, 0u);
     for (int i = 0; i < N; ++i) {
-        const int s = c->slot_of_site[i];
-        for (size_t k = 0; k < edges[i].size(); ++k)
-            ell[(size_t)s * D + k] =
-                (uint32_t)c->slot_of_site[edges[i][k].j] | ((uint32_t)edges[i][k].cls << 24);
+        uint32_t* row = ell.data() + (size_t)c->slot_of_site[i] * D;
+        for (int q = e_ptr[i]; q < e_ptr[i + 1]; ++q)
+            row[q - e_ptr[i]] = (uint32_t)c->slot_of_site[edge[q].j] | ((uint32_t)edge[q].cls << 24);
     }
     // ---- measurement rows: each bond from its endpoint with the smaller site
     // index (in the supercell order each site then holds about half of its
@@ -611,52 +683,71 @@ extern "C" int ether_commit(ether_ctx* c) {
     // (src/j_values.f90:83-84 makes j_exc symmetric; neighbour lists built by
     // distance are); otherwise the full rows are used too.
     bool self_bond = false;
-    std::vector<std::vector<uint32_t>> upper(N);
-    for (int i = 0; i < N && !self_bond; ++i) {
-        const int s = c->slot_of_site[i];
-        for (const Edge& e : edges[i]) {
-            if (e.j == i) {
+    for (int i = 0; i < N && !self_bond; ++i)
+        for (int q = e_ptr[i]; q < e_ptr[i + 1]; ++q)
+            if (edge[q].j == i) {
                 self_bond = true;
                 break;
             }
-            if (e.j > i) upper[s].push_back((uint32_t)c->slot_of_site[e.j] | ((uint32_t)e.cls << 24));
-        }
-    }
     bool mirrored = !self_bond;
     if (mirrored) {
-        std::vector<uint64_t> fwd, rev;
-        for (int i = 0; i < N; ++i)
-            for (const Edge& e : edges[i]) {
-                fwd.push_back(((uint64_t)i << 32) | ((uint64_t)e.j << 8) | (uint64_t)e.cls);
-                rev.push_back(((uint64_t)e.j << 32) | ((uint64_t)i << 8) | (uint64_t)e.cls);
+        // the multiset of bonds (i, j, class) equals that of (j, i, class):
+        // row i of the bonds, as (j, class) sorted, equals row i of the
+        // transpose, as (i', class) sorted
+        std::vector<int> t_ptr(N + 1, 0);
+        for (const Edge& ed : edge) t_ptr[ed.j + 1]++;
+        for (int i = 0; i < N && mirrored; ++i) {
+            if (t_ptr[i + 1] != e_ptr[i + 1] - e_ptr[i]) mirrored = false;
+            t_ptr[i + 1] += t_ptr[i];
+        }
+        if (mirrored) {
+            std::vector<uint64_t> tr(edge.size());
+            std::vector<int> fill(t_ptr.begin(), t_ptr.end() - 1);
+            for (int i = 0; i < N; ++i)
+                for (int q = e_ptr[i]; q < e_ptr[i + 1]; ++q)
+                    tr[fill[edge[q].j]++] = ((uint64_t)i << 32) | (uint32_t)edge[q].cls;
+            std::vector<uint64_t> row;
+            for (int i = 0; i < N && mirrored; ++i) {
+                row.clear();
+                for (int q = e_ptr[i]; q < e_ptr[i + 1]; ++q)
+                    row.push_back(((uint64_t)edge[q].j << 32) | (uint32_t)edge[q].cls);
+                std::sort(row.begin(), row.end());
+                std::sort(tr.begin() + t_ptr[i], tr.begin() + t_ptr[i + 1]);
+                mirrored = std::equal(row.begin(), row.end(), tr.begin() + t_ptr[i]);
             }
-        std::sort(fwd.begin(), fwd.end());
-        std::sort(rev.begin(), rev.end());
-        mirrored = fwd == rev;
+        }
     }
     std::vector<uint32_t> ell_m, ell_m_n(N, 0u);
     if (!mirrored) {
         c->D_m = D;
         c->bond_factor = 1;
-        for (int i = 0; i < N; ++i) ell_m_n[c->slot_of_site[i]] = (uint32_t)edges[i].size();
+        for (int i = 0; i < N; ++i) ell_m_n[c->slot_of_site[i]] = (uint32_t)(e_ptr[i + 1] - e_ptr[i]);
     } else {
         int Dm = 0;
-        for (int s = 0; s < N; ++s) Dm = std::max(Dm, (int)upper[s].size());
+        for (int i = 0; i < N; ++i) {
+            int u = 0;
+            for (int q = e_ptr[i]; q < e_ptr[i + 1]; ++q) u += edge[q].j > i;
+            ell_m_n[c->slot_of_site[i]] = (uint32_t)u;
+            Dm = std::max(Dm, u);
+        }
         c->D_m = Dm;
         c->bond_factor = 2;
         ell_m.assign((size_t)std::max(Dm, 1) * N, 0u);
-        for (int s = 0; s < N; ++s) {
-            ell_m_n[s] = (uint32_t)upper[s].size();
-            for (size_t k = 0; k < upper[s].size(); ++k) ell_m[(size_t)s * Dm + k] = upper[s][k];
+        for (int i = 0; i < N; ++i) {
+            uint32_t* row = ell_m.data() + (size_t)c->slot_of_site[i] * Dm;
+            int k = 0;
+            for (int q = e_ptr[i]; q < e_ptr[i + 1]; ++q)
+                if (edge[q].j > i) row[k++] = (uint32_t)c->slot_of_site[edge[q].j] | ((uint32_t)edge[q].cls << 24);
         }
     }
     // ---- Ising index tables ----
     if (ising) {
         const int K = c->n_classes;
         c->ising_Z.assign(K + 1, 0);
+        std::vector<int> cnt(K + 1, 0);
         for (int i = 0; i < N; ++i) {
-            std::vector<int> cnt(K + 1, 0);
-            for (const Edge& e : edges[i]) cnt[e.cls]++;
+            std::fill(cnt.begin(), cnt.end(), 0);
+            for (int q = e_ptr[i]; q < e_ptr[i + 1]; ++q) cnt[edge[q].cls]++;
             for (int k = 1; k <= K; ++k) c->ising_Z[k] = std::max(c->ising_Z[k], cnt[k]);
         }
         c->ising_stride.assign(K + 1, 0);
@@ -708,9 +799,11 @@ extern "C" int ether_commit(ether_ctx* c) {
     c->meas_lower.assign(c->ncol, 0);
     for (int q = 0; q < c->ncol && c->meas_fused_ok; ++q) {
         bool lo = false, hi = false;
-        for (int s = c->cbeg[q]; s < c->cbeg[q + 1]; ++s)
-            for (const Edge& e : edges[c->site_of_slot[s]])
-                (c->slot_of_site[e.j] < c->cbeg[q] ? lo : hi) = true;
+        for (int s = c->cbeg[q]; s < c->cbeg[q + 1]; ++s) {
+            const int i = c->site_of_slot[s];
+            for (int k = e_ptr[i]; k < e_ptr[i + 1]; ++k)
+                (c->slot_of_site[edge[k].j] < c->cbeg[q] ? lo : hi) = true;
+        }
         c->meas_lower[q] = lo ? 1 : 0;
         if (lo && hi) c->meas_fused_ok = false;
     }
@@ -718,9 +811,11 @@ extern "C" int ether_commit(ether_ctx* c) {
         double bound[4] = {0.0, 0.0, 0.0, 0.0};
         for (int i = 0; i < N; ++i) {
             double t = 0.0;          // |h| <= sum of the largest |J| component per bond
-            for (const Edge& e : edges[i])
-                t += std::max(std::fabs(c->jclass[3 * e.cls]),
-                              std::max(std::fabs(c->jclass[3 * e.cls + 1]), std::fabs(c->jclass[3 * e.cls + 2])));
+            for (int q = e_ptr[i]; q < e_ptr[i + 1]; ++q) {
+                const int k = edge[q].cls;
+                t += std::max(std::fabs(c->jclass[3 * k]),
+                              std::max(std::fabs(c->jclass[3 * k + 1]), std::fabs(c->jclass[3 * k + 2])));
+            }
             bound[0] = std::max(bound[0], 2.0 * t);
         }
         for (int a = 0; a < nsp; ++a) {
@@ -794,102 +889,87 @@ extern "C" int ether_commit(ether_ctx* c) {
     // four zero entries past the last row: a 4-wide chunk of the last row stays in the table
     ell.resize(ell.size() + 4, 0u);
     if (!ell_m.empty()) ell_m.resize(ell_m.size() + 4, 0u);
-    TRY(dev_alloc(c, &c->d_ell, ell.size()));
-    if (!ell_m.empty()) TRY(dev_alloc(c, &c->d_ell_m, ell_m.size()));
-    TRY(dev_alloc(c, &c->d_ell_m_n, ell_m_n.size()));
-    TRY(dev_alloc(c, &c->d_sp, (size_t)N));
-    TRY(dev_alloc(c, &c->d_w, (size_t)N));
-    TRY(dev_alloc(c, &c->d_jclass, c->jclass.size()));
-    TRY(dev_alloc(c, &c->d_zvec, zvec.size()));
-    TRY(dev_alloc(c, &c->d_dsia, dsia.size()));
-    TRY(dev_alloc(c, &c->d_ising_j, c->ising_j.size()));
-    TRY(dev_alloc(c, &c->d_ising_zc, zc.size()));
-    if (ising) {
-        TRY(dev_alloc(c, &c->d_s, RN));
-        TRY(dev_alloc(c, &c->d_stride, c->ising_stride.size()));
-        TRY(dev_alloc(c, &c->d_thr, (size_t)R * c->ising_table));
-        TRY(dev_alloc(c, &c->d_bonds, (size_t)R * (c->n_classes + 1)));
-        TRY(upload(c, c->d_stride, c->ising_stride.data(), c->ising_stride.size()));
-        HIPCHK(c, hipMemsetAsync(c->d_bonds, 0, sizeof(long long) * R * (c->n_classes + 1), c->stream));
-        // ordered start (all +1), padding replicas included
-        HIPCHK(c, hipMemsetAsync(c->d_s, 1, RN, c->stream));
-    } else {
-        TRY(dev_alloc(c, &c->d_sx, RN));
-        TRY(dev_alloc(c, &c->d_sy, RN));
-        TRY(dev_alloc(c, &c->d_sz, RN));
-        if (c->angle) TRY(dev_alloc(c, &c->d_phi, RN));
-        HIPCHK(c, hipMemsetAsync(c->d_sx, 0, RN * 4, c->stream));
-        HIPCHK(c, hipMemsetAsync(c->d_sy, 0, RN * 4, c->stream));
-        if (c->angle) HIPCHK(c, hipMemsetAsync(c->d_phi, 0, RN * 4, c->stream));
-        std::vector<float> ones(RN, 1.0f);   // every spin (and padding lane) along +z
-        TRY(upload(c, c->d_sz, ones.data(), RN));
-    }
-    TRY(dev_alloc(c, &c->d_slot_of, (size_t)R));
-    TRY(dev_alloc(c, &c->d_site_of_slot, (size_t)N));
-    TRY(dev_alloc(c, &c->d_rep_at, (size_t)R));
-    TRY(dev_alloc(c, &c->d_beta_f, (size_t)R));
-    TRY(dev_alloc(c, &c->d_beta, (size_t)R));
-    TRY(dev_alloc(c, &c->d_acc, (size_t)kAccCopies * R));
-    TRY(dev_alloc(c, &c->d_mom, (size_t)R * ETHER_MOM_COUNT));
-    TRY(dev_alloc(c, &c->d_partial, (size_t)c->Rp * c->meas_tiles * c->meas_rec));
-    TRY(dev_alloc(c, &c->d_Ept, (size_t)R));
-    TRY(dev_alloc(c, &c->d_parts, (size_t)R * 4));
-    TRY(dev_alloc(c, &c->d_M, (size_t)R * 3));
-    TRY(dev_alloc(c, &c->d_pt_acc, (size_t)R));
-    TRY(dev_alloc(c, &c->d_pt_att, (size_t)R));
-    TRY(dev_alloc(c, &c->d_trips, (size_t)R));
-    TRY(dev_alloc(c, &c->d_trip_state, (size_t)R));
-    TRY(dev_alloc(c, &c->d_nacc, 1));
-    TRY(dev_alloc(c, &c->d_gstart, (size_t)R + 1));
-    TRY(dev_alloc(c, &c->d_scratch_h, (size_t)N * 4));
-    TRY(dev_alloc(c, &c->d_scratch_e, (size_t)N));
-    TRY(dev_alloc(c, &c->d_ctr, 4));
-    if (c->meas_fused_ok) {
-        const size_t n = (size_t)kMeasCopies * c->Rp * kMeasWords;
-        TRY(dev_alloc(c, &c->d_macc, n));
-        HIPCHK(c, hipMemsetAsync(c->d_macc, 0, sizeof(unsigned long long) * n, c->stream));
-    }
+    std::vector<int> ident(R);
+    for (int r = 0; r < R; ++r) ident[r] = r;
+    std::vector<int> spiral_rows;
     if (c->spiral_cap) {
         // partner rows in slot space, in the order the sites' rows were given
-        std::vector<int> rows((size_t)N * c->spiral_cap, -1);
+        spiral_rows.assign((size_t)N * c->spiral_cap, -1);
         for (int i = 0; i < N; ++i) {
             const int s = c->slot_of_site[i];
             for (int k = 0; k < c->spiral_cap; ++k) {
                 const int j = c->spiral_rows[(size_t)i * c->spiral_cap + k];
                 if (j < 0) break;
-                rows[(size_t)s * c->spiral_cap + k] = c->slot_of_site[j];
+                spiral_rows[(size_t)s * c->spiral_cap + k] = c->slot_of_site[j];
             }
         }
-        TRY(dev_alloc(c, &c->d_spiral, rows.size()));
-        TRY(dev_alloc(c, &c->d_mom_sp, (size_t)R * 3));
-        TRY(dev_alloc(c, &c->d_psi, (size_t)R));
-        TRY(upload(c, c->d_spiral, rows.data(), rows.size()));
-        HIPCHK(c, hipMemsetAsync(c->d_mom_sp, 0, sizeof(double) * 3 * R, c->stream));
-        HIPCHK(c, hipMemsetAsync(c->d_psi, 0, sizeof(double) * R, c->stream));
+    }
+    // Every buffer of the context in one allocation: the tables built here
+    // first (one host image, one copy), then the counters that start at zero
+    // (one fill), then the rest.
+    Arena ar;
+    ar.table(&c->d_ell, ell);
+    if (!ell_m.empty()) ar.table(&c->d_ell_m, ell_m);
+    ar.table(&c->d_ell_m_n, ell_m_n);
+    ar.table(&c->d_sp, sp);
+    ar.table(&c->d_site_of_slot, c->site_of_slot);
+    ar.table(&c->d_w, w);
+    ar.table(&c->d_jclass, c->jclass);
+    ar.table(&c->d_zvec, zvec);
+    ar.table(&c->d_dsia, dsia);
+    ar.table(&c->d_ising_j, c->ising_j);
+    ar.table(&c->d_ising_zc, zc);
+    ar.table(&c->d_slot_of, ident);
+    ar.table(&c->d_rep_at, ident);
+    if (ising) ar.table(&c->d_stride, c->ising_stride);
+    if (c->spiral_cap) ar.table(&c->d_spiral, spiral_rows);
+    ar.end_tables();
+    if (ising) ar.zeroed(&c->d_bonds, (size_t)R * (c->n_classes + 1));
+    ar.zeroed(&c->d_acc, (size_t)kAccCopies * R);
+    ar.zeroed(&c->d_mom, (size_t)R * ETHER_MOM_COUNT);
+    ar.zeroed(&c->d_pt_acc, (size_t)R);
+    ar.zeroed(&c->d_pt_att, (size_t)R);
+    ar.zeroed(&c->d_trips, (size_t)R);
+    ar.zeroed(&c->d_trip_state, (size_t)R);
+    if (c->meas_fused_ok) ar.zeroed(&c->d_macc, (size_t)kMeasCopies * c->Rp * kMeasWords);
+    if (c->spiral_cap) {
+        ar.zeroed(&c->d_mom_sp, (size_t)R * 3);
+        ar.zeroed(&c->d_psi, (size_t)R);
+    }
+    ar.end_zeroed();
+    if (ising) {
+        ar.plain(&c->d_s, RN);
+        ar.plain(&c->d_thr, (size_t)R * c->ising_table);
+    } else {
+        ar.plain(&c->d_sx, RN);
+        ar.plain(&c->d_sy, RN);
+        ar.plain(&c->d_sz, RN);
+        if (c->angle) ar.plain(&c->d_phi, RN);
+    }
+    ar.plain(&c->d_beta_f, (size_t)R);
+    ar.plain(&c->d_beta, (size_t)R);
+    ar.plain(&c->d_partial, (size_t)c->Rp * c->meas_tiles * c->meas_rec);
+    ar.plain(&c->d_Ept, (size_t)R);
+    ar.plain(&c->d_parts, (size_t)R * 4);
+    ar.plain(&c->d_M, (size_t)R * 3);
+    ar.plain(&c->d_nacc, 1);
+    ar.plain(&c->d_gstart, (size_t)R + 1);
+    ar.plain(&c->d_scratch_h, (size_t)N * 4);
+    ar.plain(&c->d_scratch_e, (size_t)N);
+    ar.plain(&c->d_ctr, 4);
+    TRY(ar.commit(c));
+    if (ising) {
+        // ordered start (all +1), padding replicas included
+        HIPCHK(c, hipMemsetAsync(c->d_s, 1, RN, c->stream));
+    } else {
+        // every spin (and padding lane) along +z
+        HIPCHK(c, hipMemsetAsync(c->d_sx, 0, RN * 4, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_sy, 0, RN * 4, c->stream));
+        HIPCHK(c, hipMemsetD32Async(c->d_sz, 0x3f800000, RN, c->stream));
+        if (c->angle) HIPCHK(c, hipMemsetAsync(c->d_phi, 0, RN * 4, c->stream));
     }
     launch_set_ctr(c->d_ctr, 0, 0, 0, c->stream);
     c->dev_step = c->dev_pt = c->dev_sc = 0;
-    TRY(upload(c, c->d_ell, ell.data(), ell.size()));
-    if (!ell_m.empty()) TRY(upload(c, c->d_ell_m, ell_m.data(), ell_m.size()));
-    TRY(upload(c, c->d_ell_m_n, ell_m_n.data(), ell_m_n.size()));
-    TRY(upload(c, c->d_sp, sp.data(), sp.size()));
-    TRY(upload(c, c->d_site_of_slot, c->site_of_slot.data(), (size_t)N));
-    TRY(upload(c, c->d_w, w.data(), w.size()));
-    TRY(upload(c, c->d_jclass, c->jclass.data(), c->jclass.size()));
-    TRY(upload(c, c->d_zvec, zvec.data(), zvec.size()));
-    TRY(upload(c, c->d_dsia, dsia.data(), dsia.size()));
-    TRY(upload(c, c->d_ising_j, c->ising_j.data(), c->ising_j.size()));
-    TRY(upload(c, c->d_ising_zc, zc.data(), zc.size()));
-    std::vector<int> ident(R);
-    for (int r = 0; r < R; ++r) ident[r] = r;
-    TRY(upload(c, c->d_slot_of, ident.data(), (size_t)R));
-    TRY(upload(c, c->d_rep_at, ident.data(), (size_t)R));
-    HIPCHK(c, hipMemsetAsync(c->d_acc, 0, sizeof(unsigned long long) * kAccCopies * R, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->d_mom, 0, sizeof(double) * R * ETHER_MOM_COUNT, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->d_pt_acc, 0, sizeof(unsigned long long) * R, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->d_pt_att, 0, sizeof(unsigned long long) * R, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->d_trips, 0, sizeof(unsigned long long) * R, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->d_trip_state, 0, sizeof(int) * R, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->committed = true;
     return ETHER_OK;

@@ -314,6 +314,38 @@ def run(model: Model, *, compat: int = 0, chunk: int = 0, temps=None, repeat: Op
     return res
 
 
+def _build_engines(jobs, engs: list, workers: int = 8):
+    """Committed, prepared contexts for (model, seed) pairs, set up on several
+    host threads at once (each context's set-up is independent and runs in
+    native code).  Every context built is appended to `engs`, in job order,
+    so that the caller closes them all when one of them fails; the first
+    failure is raised after every job has ended."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(job):
+        m, seed = job
+        tk = m.tags.temperatures()
+        e = make_engine(m, len(tk) * m.tags.repeat, compat=m.gpu.compat, seed=seed)
+        try:
+            apply_gpu_tags(e, m.gpu)
+            _prepare(m, tk, m.tags.repeat, e)
+        except BaseException:
+            e.close()
+            raise
+        return e, tk
+
+    E.load_library()
+    with ThreadPoolExecutor(max_workers=max(1, min(workers, len(jobs)))) as pool:
+        futs = [pool.submit(one, j) for j in jobs]
+        done = [f.exception() is None for f in futs]   # waits for every job
+    built = [f.result() for f, ok in zip(futs, done) if ok]
+    engs.extend(e for e, _ in built)
+    for f, ok in zip(futs, done):
+        if not ok:
+            raise f.exception()
+    return built
+
+
 def run_batch(models: Sequence[Model], *, seeds: Optional[Sequence[Optional[int]]] = None) -> List[Results]:
     """A batch manifest's runs (batch.ether): each run directory is its own
     context (lattice, couplings, temperatures, repeats, seed); runs whose
@@ -335,13 +367,7 @@ def run_batch(models: Sequence[Model], *, seeds: Optional[Sequence[Optional[int]
         tmcs, eq, coa, ovrr_iv, ovrr_p, pt_iv = key[:6]
         engs, temps, batch = [], [], None
         try:
-            for k in idx:
-                m = models[k]
-                tk = m.tags.temperatures()
-                e = make_engine(m, len(tk) * m.tags.repeat, compat=m.gpu.compat, seed=seeds[k])
-                apply_gpu_tags(e, m.gpu)
-                _prepare(m, tk, m.tags.repeat, e)
-                engs.append(e)
+            for e, tk in _build_engines([(models[k], seeds[k]) for k in idx], engs):
                 temps.append(tk)
             batch = E.Batch(engs)
             batch.set_graphs(all(models[k].gpu.graphs for k in idx))

@@ -23,7 +23,9 @@
  *     temperature of the ladder; replica r sits at slot slot_of[r].  Tempering
  *     swaps permute slots, never spins.
  *   - One host thread per context; all kernels run in order on the context's
- *     stream.  Calls that return host data synchronise that stream.
+ *     stream.  Calls that return host data synchronise that stream.  Distinct
+ *     contexts may be created, set up and committed from several host threads
+ *     at once; the error text of a call without a context is kept per thread.
  *   - No MPI, no CUDA shims.  gfx950 only.
  */
 #ifndef ETHER_GPU_H

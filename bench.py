@@ -144,6 +144,10 @@ def main():
     ap.add_argument("--sq-calls", type=int, default=5)
     ap.add_argument("--no-ising", action="store_true", help="skip the Ising simple-cubic section")
     ap.add_argument("--ising-steps", type=int, default=2000)
+    ap.add_argument("--tag", action="append", default=[], metavar="KEY=VALUE",
+                    help="replace an in.ether tag of the headline workload (e.g. angle='.F. 0')")
+    ap.add_argument("--no-c3", action="store_true", help="skip the triangular ANGLE/Marsaglia sweep section")
+    ap.add_argument("--c3-steps", type=int, default=2000)
     ap.add_argument("--split-ovrr", action="store_true",
                     help="the reference's order: a whole OVRR pass, then the sweep")
     ap.add_argument("--measure-pass", action="store_true",
@@ -159,7 +163,10 @@ def main():
               "of a run are independent and would shard across GPUs by whole ladders", file=sys.stderr)
         sys.exit(2)
 
-    out = measure(args.workload, args.steps, args.warmup, args.event_steps, args.split_ovrr, args)
+    over = dict(kv.split("=", 1) for kv in args.tag)
+    out = measure(args.workload, args.steps, args.warmup, args.event_steps, args.split_ovrr, args,
+                  over=over, label=(args.workload + " with " + ", ".join(f"{k} = {v}" for k, v in over.items()))
+                  if over else None)
     out = {"metric": METRIC, **out}
     if not args.no_pt:
         out["pt"] = pt_comparison(args.pt_mcs)
@@ -182,6 +189,8 @@ def main():
         ising = measure("sc_ising_16", args.ising_steps, 20, 100, False, args)
         out["ising"] = {k: ising[k] for k in ("value", "unit", "ms_per_step", "config", "roofline",
                                               "launch", "acceptance_mean")}
+    if not args.no_c3 and args.workload != "tri_afm_96":
+        out["c3"] = c3_bench(args.c3_steps, args)
     if not args.no_cpu:
         from ether_amd import workloads
         from ether_amd.driver import attach_neighbours
@@ -191,20 +200,22 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def measure(workload, steps, warmup, event_steps, split_ovrr, args):
+def measure(workload, steps, warmup, event_steps, split_ovrr, args, over=None, label=None):
     """The headline measurement on one workload: autotuned launch shape, the
     start state and fresh_spins, `warmup` untimed production MCS, `steps`
     timed production MCS of the schedule from the next period boundary (whole
     periods replayed as graphs captured before the timed region), then
     `event_steps` more MCS launched one by one between HIP events for the
-    kernel durations."""
+    kernel durations.  `over` replaces in.ether tags of the workload; `label`
+    names the variant (profiles and the config line)."""
     import numpy as np
     from ether_amd import engine as E
     from ether_amd import workloads
     from ether_amd.driver import attach_neighbours, make_engine
     from ether_amd.inputs import fortran_nint
 
-    m = workloads.model(workload)
+    m = workloads.model(workload, **(over or {}))
+    workload = label or workload
     attach_neighbours(m)
     t = m.tags
     temps = t.temperatures()
@@ -342,6 +353,25 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args):
     }
     eng.close()
     return out
+
+
+def c3_bench(steps, args):
+    """Sweep throughput on the triangular workload (96 x 96 cells, SIA and
+    Zeeman, 32 T x 10 repeats = 320 chains, OVRR off as in the reference with
+    a field or anisotropy) for both move types: ANGLE moves of 5 degrees and
+    Marsaglia moves (src/update_spin_details.f90:39-46), each timed as the
+    headline is (production MCS, graphs captured before the clock)."""
+    keep = ("value", "unit", "ms_per_step", "config", "roofline", "launch", "acceptance_mean")
+    res = {}
+    for name, over in (("angle", {}), ("marsaglia", {"angle": ".F. 0"})):
+        r = measure("tri_afm_96", steps, 20, 100, False, args, over=over,
+                    label="tri_afm_96" if name == "angle" else "tri_afm_96_marsaglia")
+        res[name] = {k: r[k] for k in keep}
+    per_type = 32 * 10 * 5000 * 9216          # spin updates of the whole run per move type
+    res["spin_updates_per_move_type"] = per_type
+    res["run_s_per_move_type"] = {k: per_type / res[k]["value"] for k in ("angle", "marsaglia")}
+    res["angle_over_marsaglia"] = res["angle"]["value"] / res["marsaglia"]["value"]
+    return res
 
 
 def sq_bench(calls):

@@ -339,7 +339,7 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args, over=None, l
         "launch": {"sites_per_wave": spw, "xcd_map": xcd,
                    "autotune_ms_per_sweep": None if tune_ms is None else
                    {f"spw{k}_xcd{x}": round(float(tune_ms[x][i]), 5)
-                    for x in (0, 1) for i, k in enumerate(E.Engine.AUTOTUNE_SPW)}},
+                    for x in range(E.Engine.AUTOTUNE_MAPS) for i, k in enumerate(E.Engine.AUTOTUNE_SPW)}},
         "kernel_ms": {"sweep": sweep_ms, "ovrr": ovrr_ms, "measure": meas_ms,
                       "sweep_measuring": smeas_ms,
                       "sweep_launches": sweep_n, "ovrr_launches": ovrr_n, "measure_launches": meas_n,

@@ -64,7 +64,7 @@ struct ether_ctx {
     int ising_P = 1, ising_offset = 0, ising_table = 0;
     int rec_ncol = 0;     // colour phases to emit while a batch records (its largest count)
     uint64_t graph_gen = 0;  // bumped whenever captured graphs are dropped
-    int xcd_map = 1;      // launch parameters of the sweep phases (ether_set_launch)
+    int xcd_map = 2;      // launch parameters of the sweep phases (ether_set_launch)
     int spw = kDefaultSitesPerWave;
     int lps_shift = 6;    // lanes per site of the sweep (1 << lps_shift); set at commit
     int lanes_req = 0;    // ether_set_lanes (0: chosen from R)
@@ -85,14 +85,18 @@ struct ether_ctx {
     uint32_t* d_ell_m_n = nullptr;  // their lengths
     int D_m = 0, bond_factor = 1;
     // fused measurement (XYZ): possible when every bond has its mirror with
-    // the same coupling, no site bonds to itself and no colour has neighbours
-    // of both earlier and later colours; meas_lower[q] = 1 where colour q's
-    // neighbours are all of earlier colours; mscale = fixed-point scales
-    // (exchange, Zeeman, SIA, moment)
+    // the same coupling and no site bonds to itself; meas_lower[q] = 1 where
+    // colour q's neighbours are all of earlier colours, 2 where some are
+    // (then the phases write per-workgroup records, meas_records); mscale =
+    // fixed-point scales (exchange, Zeeman, SIA, moment)
     bool meas_fused_ok = false;
+    bool meas_records = false;
     std::vector<int> meas_lower;
     double mscale[4] = {1.0, 1.0, 1.0, 1.0};
     unsigned long long* d_macc = nullptr;
+    long long* d_mrec = nullptr;    // [ncol][G][kWave][mrec_pp][kMeasWords]
+    int mrec_cap = 0;               // units per phase allocated
+    int mrec_pp = 0;                // unit slots per (phase, group, lane): ceil(mrec_cap / G)
     uint8_t* d_sp = nullptr;
     float* d_w = nullptr;
     float* d_jclass = nullptr;
@@ -155,6 +159,8 @@ struct ether_ctx {
     std::vector<int> ev_kind;
     std::vector<int> ev_count;
     std::vector<void*> allocs;
+    char* d_arena = nullptr;   // the block holding every buffer of the committed context
+    size_t arena_bytes = 0;
     // MLP / S(q) state lives in the extension files
     void* mlp = nullptr;
     void* sq = nullptr;
@@ -217,6 +223,8 @@ class Arena {
         if (e != hipSuccess)
             return fail(c, ETHER_ERR_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
         c->allocs.push_back(base);
+        c->d_arena = base;
+        c->arena_bytes = std::max<size_t>(top_, 256);
         for (size_t k = 0; k < dst_.size(); ++k) *dst_[k] = base + off_[k];
         if (tables_end_) {
             image_.assign(tables_end_, 0);
@@ -788,10 +796,11 @@ extern "C" int ether_commit(ether_ctx* c) {
             for (int x = 0; x < 3; ++x) dsia[3 * a + x] = (float)c->siav[3 * a + x];
     }
     // ---- fused measurement (XYZ without spiral partners): a phase books the
-    // exchange bonds of its sites when all their neighbours are of earlier
-    // colours, none when all come later; a colour with both (three or more
-    // colours) would need the field split per neighbour, and such lattices
-    // measure in a pass of their own.  Fixed-point scales: powers of two with
+    // bonds of its sites to neighbours of earlier colours (final already),
+    // each bond once from its later-coloured end and counted twice; with two
+    // colours a phase has either all its neighbours earlier or none, with
+    // three or more some phases have both and book through the field from
+    // their earlier neighbours alone, into per-workgroup records.  Fixed-point scales: powers of two with
     // |term| * scale <= 2^48 / N for every site term, so that the sum over
     // all sites stays below 2^48 (kernels round each term to an integer,
     // exactly, below 2^51) ----
@@ -804,9 +813,10 @@ extern "C" int ether_commit(ether_ctx* c) {
             for (int k = e_ptr[i]; k < e_ptr[i + 1]; ++k)
                 (c->slot_of_site[edge[k].j] < c->cbeg[q] ? lo : hi) = true;
         }
-        c->meas_lower[q] = lo ? 1 : 0;
-        if (lo && hi) c->meas_fused_ok = false;
+        c->meas_lower[q] = lo ? (hi ? 2 : 1) : 0;
+        if (lo && hi) c->meas_records = true;
     }
+    c->meas_records = c->meas_records && c->meas_fused_ok;
     if (c->meas_fused_ok) {
         double bound[4] = {0.0, 0.0, 0.0, 0.0};
         for (int i = 0; i < N; ++i) {
@@ -931,7 +941,19 @@ extern "C" int ether_commit(ether_ctx* c) {
     ar.zeroed(&c->d_pt_att, (size_t)R);
     ar.zeroed(&c->d_trips, (size_t)R);
     ar.zeroed(&c->d_trip_state, (size_t)R);
-    if (c->meas_fused_ok) ar.zeroed(&c->d_macc, (size_t)kMeasCopies * c->Rp * kMeasWords);
+    if (c->meas_fused_ok && !c->meas_records) ar.zeroed(&c->d_macc, (size_t)kMeasCopies * c->Rp * kMeasWords);
+    if (c->meas_records) {
+        // records per phase at the measuring sweep's shape (at least
+        // kMeasRecordSpw site steps per wave, so at least 4 * kMeasRecordSpw
+        // sites per workgroup), XCD padding included; zero where a phase
+        // writes none (an empty colour)
+        int maxcol = 0;
+        for (int q = 0; q < c->ncol; ++q) maxcol = std::max(maxcol, c->cbeg[q + 1] - c->cbeg[q]);
+        const int per_wg = kWavesPerBlock * kMeasRecordSpw;
+        c->mrec_cap = G * ((maxcol + per_wg - 1) / per_wg) + 7;
+        c->mrec_pp = (c->mrec_cap + G - 1) / G;
+        ar.zeroed(&c->d_mrec, (size_t)c->ncol * G * kWave * c->mrec_pp * kMeasWords);
+    }
     if (c->spiral_cap) {
         ar.zeroed(&c->d_mom_sp, (size_t)R * 3);
         ar.zeroed(&c->d_psi, (size_t)R);
@@ -1099,6 +1121,14 @@ static int phase_blocks(int n_sites, int groups, int xcd_map, int spw) {
     return (int)nb;
 }
 
+// workgroups of a phase (and its site chunks) for `spw_sites` sites per wave
+template <class A>
+static void set_blocks(A& a, int groups, int spw_sites) {
+    const int chunk = kWavesPerBlock * spw_sites;
+    a.nchunk = (a.c_end - a.c_begin + chunk - 1) / chunk;
+    a.nblocks = phase_blocks(a.c_end - a.c_begin, groups, a.xcd_map, spw_sites);
+}
+
 static XyzArgs xyz_args(ether_ctx* c, int colour) {
     XyzArgs a;
     a.sx = c->d_sx;
@@ -1117,7 +1147,7 @@ static XyzArgs xyz_args(ether_ctx* c, int colour) {
     a.xcd_map = c->xcd_map;
     a.spw = c->spw;
     a.lps_shift = c->lps_shift;
-    a.nblocks = phase_blocks(a.c_end - a.c_begin, a.G, a.xcd_map, a.spw << (6 - a.lps_shift));
+    set_blocks(a, a.G, a.spw << (6 - a.lps_shift));
     a.slot_of = c->d_slot_of;
     a.beta_slot = c->d_beta_f;
     a.seed = c->seed;
@@ -1140,6 +1170,10 @@ static XyzArgs xyz_args(ether_ctx* c, int colour) {
     a.meas_lower = 0;
     a.w = c->d_w;
     a.macc = c->d_macc;
+    a.mrec = nullptr;
+    a.mrec_stride = 0;
+    a.mrec_pp = 0;
+    a.mrec_phase = 0;
     for (int k = 0; k < 4; ++k) a.mscale[k] = c->mscale[k];
     return a;
 }
@@ -1159,7 +1193,7 @@ static IsingArgs ising_args(ether_ctx* c, int colour) {
     a.xcd_map = c->xcd_map;
     a.spw = c->spw;
     a.lps_shift = c->lps_shift;
-    a.nblocks = phase_blocks(a.c_end - a.c_begin, a.G4, a.xcd_map, a.spw << (6 - a.lps_shift));
+    set_blocks(a, a.G4, a.spw << (6 - a.lps_shift));
     a.table_in_lds = ((long long)c->R * c->ising_table <= kIsingLdsTable) ? 1 : 0;
     a.slot_of = c->d_slot_of;
     a.seed = c->seed;
@@ -1289,6 +1323,17 @@ static int enqueue_resident(ether_ctx* c, int n_sweeps, bool meas);
 
 // meas: the phases also book the fused measurement that the next
 // enqueue_measure(..., fused = true) completes.
+// The measuring sweep's shape with per-workgroup records, and the workgroups
+// of each of its phases (the most any colour needs at that shape).
+static int meas_record_spw(const ether_ctx* c) { return std::max(c->spw, kMeasRecordSpw); }
+static int meas_record_stride(const ether_ctx* c) {
+    const int spw = meas_record_spw(c) << (6 - c->lps_shift);
+    int n = 0;
+    for (int q = 0; q < c->ncol; ++q)
+        n = std::max(n, phase_blocks(c->cbeg[q + 1] - c->cbeg[q], c->Rp / kWave, c->xcd_map, spw));
+    return n;
+}
+
 static int enqueue_sweep(ether_ctx* c, double fuse_frac = 0.0, bool meas = false) {
     int launches = 0;
     for (int q = 0; q < phase_count(c); ++q)
@@ -1312,12 +1357,28 @@ static int enqueue_sweep(ether_ctx* c, double fuse_frac = 0.0, bool meas = false
             if (meas) {
                 a.meas = 1;
                 a.meas_lower = empty ? 0 : c->meas_lower[qq];
-                // a workgroup books its sums with one atomic per (replica,
-                // word), so the phase's atomics scale with its workgroups:
-                // the longest walk per wave keeps them few (the spins do not
-                // depend on the launch shape)
-                a.spw = kMaxSitesPerWave;
-                a.nblocks = phase_blocks(a.c_end - a.c_begin, a.G, a.xcd_map, a.spw << (6 - a.lps_shift));
+                if (c->meas_records) {
+                    // one record per workgroup: the run's shape (at least
+                    // kMeasRecordSpw site steps per wave) with the same
+                    // number of workgroups in every phase, so that each
+                    // measuring sweep rewrites every record the finalize reads
+                    a.spw = meas_record_spw(c);
+                    set_blocks(a, a.G, a.spw << (6 - a.lps_shift));
+                    a.nblocks = meas_record_stride(c);
+                    if (a.nblocks > c->mrec_cap)
+                        return fail(c, ETHER_ERR_STATE, "measurement records exceed their allocation");
+                    a.mrec = c->d_mrec;
+                    a.mrec_stride = a.nblocks;
+                    a.mrec_pp = c->mrec_pp;
+                    a.mrec_phase = qq;
+                } else {
+                    // a workgroup books its sums with one atomic per (replica,
+                    // word), so the phase's atomics scale with its workgroups:
+                    // the longest walk per wave keeps them few (the spins do not
+                    // depend on the launch shape)
+                    a.spw = kMaxSitesPerWave;
+                    set_blocks(a, a.G, a.spw << (6 - a.lps_shift));
+                }
             }
             launch_xyz_sweep(a, c->angle, c->zee, c->sia, c->stream);
         }
@@ -1339,7 +1400,7 @@ static int enqueue_ovrr(ether_ctx* c, double passes) {
             XyzArgs a = xyz_args(c, std::min(q, c->ncol - 1));
             if (empty) a.c_begin = a.c_end = 0;
             a.lps_shift = 6;   // the overrelaxation kernel updates one site per wave
-            a.nblocks = phase_blocks(a.c_end - a.c_begin, a.G, a.xcd_map, a.spw);
+            set_blocks(a, a.G, a.spw);
             a.pass = (uint32_t)pass;
             a.frac = (p < full) ? 1.0f : (float)frac;
             const int ev = ev_begin(c, ETHER_TIME_OVRR);
@@ -1390,7 +1451,12 @@ static int enqueue_measure(ether_ctx* c, int sample, int booked = kBookedPass) {
     f.spiral_count = c->spiral_cap ? c->spiral_count : 0.0;
     f.psi = c->d_psi;
     f.mom_sp = c->d_mom_sp;
-    f.macc = booked == kBookedXyz ? c->d_macc : nullptr;
+    f.macc = (booked == kBookedXyz && !c->meas_records) ? c->d_macc : nullptr;
+    f.mrec = (booked == kBookedXyz && c->meas_records) ? c->d_mrec : nullptr;
+    f.mrec_stride = c->meas_records ? meas_record_stride(c) : 0;
+    f.mrec_phases = c->ncol;
+    f.mrec_groups = c->Rp / kWave;
+    f.mrec_pp = c->mrec_pp;
     for (int k = 0; k < 4; ++k) f.mscale[k] = c->mscale[k];
     launch_finalize(f, c->stream);
     ev_end(c, ev);
@@ -1485,8 +1551,8 @@ extern "C" int ether_sweep(ether_ctx* c, int32_t n_mcs, double* accept) {
 // --------------------------------------------------------------------------
 extern "C" int ether_set_launch(ether_ctx* c, int32_t sites_per_wave, int32_t xcd_map) {
     if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
-    if (sites_per_wave < 1 || sites_per_wave > kMaxSitesPerWave || (xcd_map != 0 && xcd_map != 1))
-        return fail(c, ETHER_ERR_ARG, "ether_set_launch: sites_per_wave must be 1..16, xcd_map 0 or 1");
+    if (sites_per_wave < 1 || sites_per_wave > kMaxSitesPerWave || xcd_map < 0 || xcd_map > 2)
+        return fail(c, ETHER_ERR_ARG, "ether_set_launch: sites_per_wave must be 1..16, xcd_map 0, 1 or 2");
     if (c->spw != sites_per_wave || c->xcd_map != xcd_map) drop_graphs(c);   // baked into graphs
     c->spw = sites_per_wave;
     c->xcd_map = xcd_map;
@@ -1502,6 +1568,7 @@ extern "C" int ether_autotune(ether_ctx* c, int32_t reps, int32_t* sites_per_wav
     if (reps <= 0) return fail(c, ETHER_ERR_ARG, "ether_autotune: reps must be positive");
     static const int kSpw[] = {1, 2, 4, 8, 16};
     const int n_spw = (int)(sizeof(kSpw) / sizeof(kSpw[0]));
+    constexpr int kXcdMaps = 3;
     const size_t RN = (size_t)c->Rp * c->N;
     std::vector<std::pair<void*, size_t>> planes;
     if (c->model == ETHER_MODEL_ISING) {
@@ -1538,9 +1605,9 @@ extern "C" int ether_autotune(ether_ctx* c, int32_t reps, int32_t* sites_per_wav
         rc = fail(c, ETHER_ERR_HIP, "ether_autotune: event creation failed");
     // two rounds over the candidates, each keeping its faster round: a
     // candidate's time then does not hinge on one round's clock
-    std::vector<double> per_ms(2 * n_spw, 1e300);
+    std::vector<double> per_ms(kXcdMaps * n_spw, 1e300);
     for (int round = 0; round < 2 && !rc; ++round)
-        for (int i = 0; !rc && i < 2 * n_spw; ++i) {
+        for (int i = 0; !rc && i < kXcdMaps * n_spw; ++i) {
             c->spw = kSpw[i % n_spw];
             c->xcd_map = i / n_spw;
             if ((rc = enqueue_sweep(c))) break;   // warm-up of this shape
@@ -1559,9 +1626,9 @@ extern "C" int ether_autotune(ether_ctx* c, int32_t reps, int32_t* sites_per_wav
     // the fastest candidate; among those within 1% of it, the longest walk
     // per wave (fewer workgroups, so fewer atomics at the end of a phase),
     // then the faster: shapes a hair apart do not trade places from run to run
-    for (int i = 0; i < 2 * n_spw; ++i) best = std::min(best, per_ms[i]);
+    for (int i = 0; i < kXcdMaps * n_spw; ++i) best = std::min(best, per_ms[i]);
     int pick = -1;
-    for (int i = 0; i < 2 * n_spw; ++i) {
+    for (int i = 0; i < kXcdMaps * n_spw; ++i) {
         if (table_ms) table_ms[i] = per_ms[i];
         if (per_ms[i] > 1.01 * best) continue;
         if (pick < 0 || kSpw[i % n_spw] > kSpw[pick % n_spw] ||
@@ -2499,4 +2566,72 @@ const int8_t* ctx_ising(ether_ctx* c) { return c->d_s; }
 const int* ctx_slot_of_site(ether_ctx* c) { return c->slot_of_site.data(); }
 const int* ctx_site_of_slot_dev(ether_ctx* c) { return c->d_site_of_slot; }
 int ctx_device(ether_ctx* c) { return c->device; }
+
+// The state a run evolves (every device buffer of the context, the host
+// counters of sweeps, swaps and starts, the ladder and its temperatures), to
+// put a context back where it was: restoring and running the same schedule
+// again gives bit-identical results, since every draw is a function of the
+// counters.  Captured graphs are dropped on both sides (they hold counters).
+struct CtxSnapshot {
+    char* dev = nullptr;
+    size_t bytes = 0;
+    uint32_t step = 0, init_count = 0, pt_count = 0, dev_step = 0, dev_pt = 0, dev_sc = 0;
+    std::vector<double> beta_h;
+    int n_groups = 0;
+    bool betas_set = false;
+};
+int ctx_snapshot(ether_ctx* c, void** out) {
+    *out = nullptr;
+    if (!c->committed || !c->d_arena) return fail(c, ETHER_ERR_STATE, "snapshot of an uncommitted context");
+    drop_graphs(c);
+    CtxSnapshot* s = new CtxSnapshot;
+    hipError_t e = hipMalloc((void**)&s->dev, c->arena_bytes);
+    if (e != hipSuccess) {
+        delete s;
+        return fail(c, ETHER_ERR_HIP, std::string("snapshot: hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    s->bytes = c->arena_bytes;
+    e = hipMemcpyAsync(s->dev, c->d_arena, s->bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        (void)hipFree(s->dev);
+        delete s;
+        return fail(c, ETHER_ERR_HIP, std::string("snapshot: copy failed: ") + hipGetErrorString(e));
+    }
+    s->step = c->step;
+    s->init_count = c->init_count;
+    s->pt_count = c->pt_count;
+    s->dev_step = c->dev_step;
+    s->dev_pt = c->dev_pt;
+    s->dev_sc = c->dev_sc;
+    s->beta_h = c->beta_h;
+    s->n_groups = c->n_groups;
+    s->betas_set = c->betas_set;
+    *out = s;
+    return ETHER_OK;
+}
+int ctx_restore(ether_ctx* c, void* snap) {
+    const CtxSnapshot* s = static_cast<const CtxSnapshot*>(snap);
+    if (!s || s->bytes != c->arena_bytes) return fail(c, ETHER_ERR_STATE, "restore: no snapshot of this context");
+    drop_graphs(c);
+    hipError_t e = hipMemcpyAsync(c->d_arena, s->dev, s->bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, ETHER_ERR_HIP, std::string("restore: copy failed: ") + hipGetErrorString(e));
+    c->step = s->step;
+    c->init_count = s->init_count;
+    c->pt_count = s->pt_count;
+    c->dev_step = s->dev_step;
+    c->dev_pt = s->dev_pt;
+    c->dev_sc = s->dev_sc;
+    c->beta_h = s->beta_h;
+    c->n_groups = s->n_groups;
+    c->betas_set = s->betas_set;
+    return ETHER_OK;
+}
+void ctx_snapshot_free(void* snap) {
+    CtxSnapshot* s = static_cast<CtxSnapshot*>(snap);
+    if (!s) return;
+    (void)hipFree(s->dev);
+    delete s;
+}
 }  // namespace ether

@@ -30,6 +30,7 @@ constexpr int kBlock = 256;          // threads per workgroup
 constexpr int kWave = 64;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kDefaultSitesPerWave = 4;  // consecutive sites of a colour class per wave
+constexpr int kMeasRecordSpw = 4;       // least site steps per wave of a sweep writing measurement records
 constexpr int kMaxSitesPerWave = 16;     // (a launch parameter: ether_set_launch / ether_autotune)
 constexpr int kMaxClasses = 255;     // coupling classes (8-bit field of an ELL entry)
 constexpr int kMaxSpecies = 16;
@@ -55,6 +56,7 @@ struct XyzArgs {
     int c_begin;
     int c_end;
     int nblocks;
+    int nchunk;            // site chunks of the phase: nblocks = groups x nchunk (+ padding)
     int xcd_map;
     int spw;               // steps per wave (sites per wave = spw << (6 - lps_shift))
     int lps_shift;         // lanes per site = 1 << lps_shift (6: one site per wave)
@@ -84,10 +86,21 @@ struct XyzArgs {
     // weighted spin.  A term is rounded per site to an integer multiple of
     // 1/mscale and summed exactly (int64 in macc), so the totals do not
     // depend on which wave or workgroup adds what, nor in which order.
+    //
+    // With three or more colours a phase's sites have neighbours of earlier
+    // and of later colours: a site books its bonds to the earlier ones (the
+    // exchange field of those alone, from the slots below c_begin), each
+    // bond again once, from its later-coloured end.  Each workgroup then
+    // writes its sums to a record of its own (mrec) instead of adding them to
+    // macc, and the finalize adds the records up.
     int meas;              // 0: off
-    int meas_lower;        // 1: the phase's neighbours are all of earlier colours, 0: none are
+    int meas_lower;        // 0: no neighbour of an earlier colour, 1: all of them, 2: some
     const float* w;        // [N] magnetisation weight per slot
     unsigned long long* macc;  // [kMeasCopies][Rp][kMeasWords] two's-complement int64 sums
+    long long* mrec;       // non-null: records [phase][group][lane][mrec_pp units][kMeasWords]
+    int mrec_stride;       // workgroups of every phase of the measuring sweep
+    int mrec_pp;           // unit slots per (phase, group, lane): unit bb is slot bb / G of group bb % G
+    int mrec_phase;
     double mscale[4];      // powers of two for exchange, Zeeman, SIA, magnetisation
 };
 
@@ -103,6 +116,7 @@ struct IsingArgs {
     int c_begin;
     int c_end;
     int nblocks;
+    int nchunk;            // site chunks of the phase: nblocks = groups x nchunk (+ padding)
     int xcd_map;
     int spw;               // steps per wave (sites per wave = spw << (6 - lps_shift))
     int lps_shift;         // lanes per site = 1 << lps_shift, 4 replicas per lane (6: one site)
@@ -216,6 +230,11 @@ struct FinArgs {
     double* mom_sp;          // [R slots][3] sums of psi, psi^2, psi^4 over samples
     unsigned long long* macc;  // non-null: the record comes from the fused measurement's
                                // [kMeasCopies][Rp][kMeasWords] sums (cleared here for the next one)
+    const long long* mrec;     // non-null: it comes from the per-workgroup records of the
+    int mrec_stride;           // measuring sweep's mrec_phases phases (XyzArgs::mrec): units
+    int mrec_phases;           // bb < mrec_stride, unit bb holding replicas (bb % mrec_groups) * 64 + lane
+    int mrec_groups;
+    int mrec_pp;
     double mscale[4];
 };
 
@@ -254,16 +273,24 @@ struct InitArgs {
     uint32_t step;
 };
 
-// Workgroup -> wave unit.  With xcd_map the launch has a multiple of 8
-// workgroups; the dispatcher deals consecutive workgroup ids round-robin to
-// the 8 XCDs, so workgroup b runs on XCD b%8.  Remapping b -> (b%8)*per + b/8
-// gives every XCD one contiguous range of sites, whose spins and neighbour
-// halo then stay in that XCD's L2.
-__device__ __forceinline__ int block_index(int nblocks, int xcd_map) {
+// Workgroup -> wave unit bb, which the kernels split as (replica group
+// bb % groups, site chunk bb / groups).  With xcd_map the launch has a
+// multiple of 8 workgroups; the dispatcher deals consecutive workgroup ids
+// round-robin to the 8 XCDs, so workgroup b runs on XCD b%8.  Map 1 remaps
+// b -> p = (b%8)*per + b/8, which gives every XCD one contiguous range of
+// units: a range of sites with every replica group.  Map 2 also orders the
+// units group-major (p -> group p / nchunk, chunk p % nchunk), so that an XCD
+// walks more sites of fewer groups: a group's neighbour halo is 1/groups of
+// the spins of those sites, and it stays in the XCD's L2 until the sites
+// next to it reuse it.  Units p >= groups * nchunk (padding) keep bb = p,
+// whose chunk lies past the colour's sites.
+__device__ __forceinline__ int block_index(int nblocks, int xcd_map, int groups, int nchunk) {
     int b = blockIdx.x;
     if (!xcd_map) return b;
-    int per = nblocks >> 3;
-    return (b & 7) * per + (b >> 3);
+    const int per = nblocks >> 3;
+    b = (b & 7) * per + (b >> 3);
+    if (xcd_map == 2 && b < groups * nchunk) b = (b % nchunk) * groups + b / nchunk;
+    return b;
 }
 
 inline int launch_blocks(long long units, int xcd_map) {

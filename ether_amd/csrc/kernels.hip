@@ -277,6 +277,43 @@ __device__ __forceinline__ void sum_row(const RowVals<DM>& v, const float4* sJ4,
     }
 }
 
+// The exchange field from the neighbours of earlier colours only (slots
+// below c_begin): the fused measurement's bond term with three or more
+// colours.  Same operations in the same order as sum_row on the entries it
+// keeps, so where every neighbour is earlier it equals the full field.
+template <int DM>
+__device__ __forceinline__ void sum_row_lower(const RowVals<DM>& v, const uint32_t* nb, int c_begin,
+                                              const float4* sJ4, float& lx, float& ly, float& lz) {
+    const ConstRow row = const_row(nb);
+    lx = 0.f;
+    ly = 0.f;
+    lz = 0.f;
+#pragma unroll
+    for (int k = 0; k < DM; ++k) {
+        const bool lo = (int)(row[k] & 0xffffffu) < c_begin;
+        const float4 j = sJ4[v.cl[k]];
+        lx = fmaf(lo ? j.x : 0.f, v.x[k], lx);
+        ly = fmaf(lo ? j.y : 0.f, v.y[k], ly);
+        lz = fmaf(lo ? j.z : 0.f, v.z[k], lz);
+    }
+}
+// the same with a row per lane (several sites per wave)
+template <int DM>
+__device__ __forceinline__ void sum_row_lower_lanes(const RowVals<DM>& v, const uint32_t* nb, int c_begin,
+                                                    const float4* sJ4, float& lx, float& ly, float& lz) {
+    lx = 0.f;
+    ly = 0.f;
+    lz = 0.f;
+#pragma unroll
+    for (int k = 0; k < DM; ++k) {
+        const bool lo = (int)(nb[k] & 0xffffffu) < c_begin;
+        const float4 j = sJ4[v.cl[k]];
+        lx = fmaf(lo ? j.x : 0.f, v.x[k], lx);
+        ly = fmaf(lo ? j.y : 0.f, v.y[k], ly);
+        lz = fmaf(lo ? j.z : 0.f, v.z[k], lz);
+    }
+}
+
 // Neighbours gathered per chunk of a long row: all DM loads of a row in
 // flight at once cost registers (130 VGPRs at DM = 18, 3 waves per SIMD);
 // chunks of 6 with a scheduling barrier between them keep 4 waves per SIMD
@@ -294,13 +331,15 @@ constexpr int field_chunk(int DM) {
 // neighbour loads are issued before the first use, so their latencies overlap
 // each other and the generator work that follows.  Accumulation order is
 // k = 0..D-1 in both forms.
-template <int DM>
+template <int DM, bool LOWER = false>
 __device__ __forceinline__ void local_field(Plane px, Plane py, Plane pz, const uint32_t* nb, int D,
                                             uint32_t row_bytes, uint32_t roff, const float4* sJ4,
-                                            float& hx, float& hy, float& hz) {
+                                            float& hx, float& hy, float& hz, int c_begin = 0,
+                                            float* lower = nullptr) {
     hx = 0.f;
     hy = 0.f;
     hz = 0.f;
+    float lx = 0.f, ly = 0.f, lz = 0.f;   // LOWER: the part from slots below c_begin (sum_row_lower)
     const ConstRow row = const_row(nb);
     if constexpr (DM > 0) {
         constexpr int CH = field_chunk(DM);
@@ -323,6 +362,12 @@ __device__ __forceinline__ void local_field(Plane px, Plane py, Plane pz, const 
                 hx = fmaf(j.x, vx[k], hx);
                 hy = fmaf(j.y, vy[k], hy);
                 hz = fmaf(j.z, vz[k], hz);
+                if constexpr (LOWER) {
+                    const bool lo = (int)(row[c0 + k] & 0xffffffu) < c_begin;
+                    lx = fmaf(lo ? j.x : 0.f, vx[k], lx);
+                    ly = fmaf(lo ? j.y : 0.f, vy[k], ly);
+                    lz = fmaf(lo ? j.z : 0.f, vz[k], lz);
+                }
             }
             if (CH < DM) __builtin_amdgcn_sched_barrier(0);   // keep the chunks apart
         }
@@ -331,10 +376,22 @@ __device__ __forceinline__ void local_field(Plane px, Plane py, Plane pz, const 
             const uint32_t e = row[k];
             const uint32_t so = (e & 0xffffffu) * row_bytes;
             const float4 j = sJ4[e >> 24];
-            hx = fmaf(j.x, plane_ld(px, roff, so), hx);
-            hy = fmaf(j.y, plane_ld(py, roff, so), hy);
-            hz = fmaf(j.z, plane_ld(pz, roff, so), hz);
+            const float vx = plane_ld(px, roff, so), vy = plane_ld(py, roff, so), vz = plane_ld(pz, roff, so);
+            hx = fmaf(j.x, vx, hx);
+            hy = fmaf(j.y, vy, hy);
+            hz = fmaf(j.z, vz, hz);
+            if constexpr (LOWER) {
+                const bool lo = (int)(e & 0xffffffu) < c_begin;
+                lx = fmaf(lo ? j.x : 0.f, vx, lx);
+                ly = fmaf(lo ? j.y : 0.f, vy, ly);
+                lz = fmaf(lo ? j.z : 0.f, vz, lz);
+            }
         }
+    }
+    if constexpr (LOWER) {
+        lower[0] = lx;
+        lower[1] = ly;
+        lower[2] = lz;
     }
 }
 
@@ -411,6 +468,29 @@ __device__ __forceinline__ void book_measurement(double* buf /* [kMeasWords][kBl
             const size_t copy = blockIdx.x % kMeasCopies;
             atomicAdd(a.macc + (copy * a.Rp + r) * kMeasWords + k, (unsigned long long)(long long)t);
         }
+    }
+}
+
+// The same sums, written to the workgroup's own record instead (three or more
+// colours): unit bb of the phase, its lane l holding replica l of group
+// bb % G, kMeasWords int64 words per lane, stored replica-major (the records
+// of one replica in one phase are contiguous, for the finalize).  Every
+// workgroup of the phase writes its record, so the finalize reads records of
+// the last measuring sweep only and nothing needs clearing.  Called by every
+// thread of the workgroup.
+__device__ __forceinline__ void book_records(double* buf /* [kMeasWords][kBlock] */,
+                                             const double (&m)[kMeasWords], int per, int bb,
+                                             const XyzArgs& a) {
+#pragma unroll
+    for (int k = 0; k < kMeasWords; ++k) buf[k * kBlock + threadIdx.x] = m[k];
+    __syncthreads();
+    const int g = bb % a.G, u = bb / a.G;
+    long long* rec = a.mrec + (((size_t)a.mrec_phase * a.G + g) * kWave * a.mrec_pp + u) * kMeasWords;
+    for (int o = threadIdx.x; o < per * kMeasWords; o += kBlock) {
+        const int l = o / kMeasWords, k = o - l * kMeasWords;
+        double t = 0.0;
+        for (int i = l; i < kBlock; i += per) t += buf[k * kBlock + i];
+        rec[(size_t)l * a.mrec_pp * kMeasWords + k] = (long long)t;
     }
 }
 
@@ -512,6 +592,7 @@ __device__ __forceinline__ void xyz_sweep_block(const XyzArgs& a, int bb) {
         const float cphi = ANGLE ? plane_ld(pp, roff, so) : 0.f;
         const uint32_t* nb = a.ell + (size_t)slot * a.D;
         float hx, hy, hz;
+        float hl[3] = {0.f, 0.f, 0.f};   // MEAS: the exchange field from earlier colours (book_site)
         u32x4 wr;
         if constexpr (DM > 0 && DM <= 8) {
             RowVals<DM> v;
@@ -522,24 +603,30 @@ __device__ __forceinline__ void xyz_sweep_block(const XyzArgs& a, int bb) {
             // to its first use after the field sum
             asm volatile("" : "+v"(wr.x), "+v"(wr.y), "+v"(wr.z), "+v"(wr.w));
             sum_row<DM>(v, sJ4, hx, hy, hz);
+            if constexpr (MEAS) sum_row_lower<DM>(v, nb, a.c_begin, sJ4, hl[0], hl[1], hl[2]);
         } else {
-            local_field<DM>(px, py, pz, nb, a.D, rowb, roff, sJ4, hx, hy, hz);
+            if constexpr (MEAS)
+                local_field<DM, true>(px, py, pz, nb, a.D, rowb, roff, sJ4, hx, hy, hz, a.c_begin, hl);
+            else
+                local_field<DM>(px, py, pz, nb, a.D, rowb, roff, sJ4, hx, hy, hz);
             wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, step, RNG_METROPOLIS, 0);
         }
-        const float hl[3] = {hx, hy, hz};             // exchange field, before Zeeman
         update(slot, x, y, z, cphi, hx, hy, hz, hl, wr);
     }
     // acceptance is booked to the temperature slot, as the reference books it per rank
     book_acceptance(sAcc, 1, &accepted, r, a.R, a.slot_of, a.acc);
     if constexpr (MEAS) {
         __shared__ double sMeas[kMeasWords * kBlock];
-        book_measurement(sMeas, m, kWave, g * kWave, a);
+        if (!a.mrec)
+            book_measurement(sMeas, m, kWave, g * kWave, a);
+        else if (a.c_end > a.c_begin)   // an empty phase (of a batch) owns no records
+            book_records(sMeas, m, kWave, bb, a);
     }
 }
 
 template <bool ANGLE, int DM, bool MEAS>
 __global__ __launch_bounds__(kBlock) void k_xyz_sweep(XyzArgs a) {
-    xyz_sweep_block<ANGLE, DM, MEAS>(a, block_index(a.nblocks, a.xcd_map));
+    xyz_sweep_block<ANGLE, DM, MEAS>(a, block_index(a.nblocks, a.xcd_map, a.G, a.nchunk));
 }
 
 // Batched form: workgroup b runs local workgroup items[b].y of the launch
@@ -602,7 +689,7 @@ __device__ __forceinline__ void xyz_ovrr_block(const XyzArgs& a, int bb) {
 
 template <bool ANGLE, int DM>
 __global__ __launch_bounds__(kBlock) void k_xyz_ovrr(XyzArgs a) {
-    xyz_ovrr_block<ANGLE, DM>(a, block_index(a.nblocks, a.xcd_map));
+    xyz_ovrr_block<ANGLE, DM>(a, block_index(a.nblocks, a.xcd_map, a.G, a.nchunk));
 }
 
 template <bool ANGLE, int DM>
@@ -728,7 +815,7 @@ __device__ __forceinline__ void ising_sweep_block(const IsingArgs& a, int bb) {
 
 template <int DM>
 __global__ __launch_bounds__(kBlock) void k_ising_sweep(IsingArgs a) {
-    ising_sweep_block<DM>(a, block_index(a.nblocks, a.xcd_map));
+    ising_sweep_block<DM>(a, block_index(a.nblocks, a.xcd_map, a.G4, a.nchunk));
 }
 
 template <int DM>
@@ -968,13 +1055,15 @@ __device__ __forceinline__ void gather_row_lanes(Plane px, Plane py, Plane pz, c
     }
 }
 
-template <int DM>
+template <int DM, bool LOWER = false>
 __device__ __forceinline__ void local_field_lanes(Plane px, Plane py, Plane pz, const uint32_t* nb,
                                                   int D, uint32_t row_bytes, uint32_t roff,
-                                                  const float4* sJ4, float& hx, float& hy, float& hz) {
+                                                  const float4* sJ4, float& hx, float& hy, float& hz,
+                                                  int c_begin = 0, float* lower = nullptr) {
     hx = 0.f;
     hy = 0.f;
     hz = 0.f;
+    float lx = 0.f, ly = 0.f, lz = 0.f;   // LOWER: as in local_field
     if constexpr (DM > 0) {
         constexpr int CH = field_chunk(DM);
 #pragma unroll
@@ -996,6 +1085,12 @@ __device__ __forceinline__ void local_field_lanes(Plane px, Plane py, Plane pz, 
                 hx = fmaf(j.x, vx[k], hx);
                 hy = fmaf(j.y, vy[k], hy);
                 hz = fmaf(j.z, vz[k], hz);
+                if constexpr (LOWER) {
+                    const bool lo = (int)(nb[c0 + k] & 0xffffffu) < c_begin;
+                    lx = fmaf(lo ? j.x : 0.f, vx[k], lx);
+                    ly = fmaf(lo ? j.y : 0.f, vy[k], ly);
+                    lz = fmaf(lo ? j.z : 0.f, vz[k], lz);
+                }
             }
             if (CH < DM) __builtin_amdgcn_sched_barrier(0);
         }
@@ -1004,10 +1099,22 @@ __device__ __forceinline__ void local_field_lanes(Plane px, Plane py, Plane pz, 
             const uint32_t e = nb[k];
             const uint32_t vo = (e & 0xffffffu) * row_bytes + roff;
             const float4 j = sJ4[e >> 24];
-            hx = fmaf(j.x, plane_ldv(px, vo), hx);
-            hy = fmaf(j.y, plane_ldv(py, vo), hy);
-            hz = fmaf(j.z, plane_ldv(pz, vo), hz);
+            const float vx = plane_ldv(px, vo), vy = plane_ldv(py, vo), vz = plane_ldv(pz, vo);
+            hx = fmaf(j.x, vx, hx);
+            hy = fmaf(j.y, vy, hy);
+            hz = fmaf(j.z, vz, hz);
+            if constexpr (LOWER) {
+                const bool lo = (int)(e & 0xffffffu) < c_begin;
+                lx = fmaf(lo ? j.x : 0.f, vx, lx);
+                ly = fmaf(lo ? j.y : 0.f, vy, ly);
+                lz = fmaf(lo ? j.z : 0.f, vz, lz);
+            }
         }
+    }
+    if constexpr (LOWER) {
+        lower[0] = lx;
+        lower[1] = ly;
+        lower[2] = lz;
     }
 }
 
@@ -1087,6 +1194,7 @@ __device__ __forceinline__ void xyz_sweep_lanes_block(const XyzArgs& a, int bb) 
         float cphi = ANGLE ? plane_ldv(pp, vo) : 0.f;
         const uint32_t* nb = a.ell + (size_t)slot * a.D;
         float hx, hy, hz;
+        float hl[3] = {0.f, 0.f, 0.f};   // MEAS: the exchange field from earlier colours
         u32x4 wr;
         if constexpr (DM > 0 && DM <= 8) {
             RowVals<DM> v;
@@ -1094,11 +1202,14 @@ __device__ __forceinline__ void xyz_sweep_lanes_block(const XyzArgs& a, int bb) 
             wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, step, RNG_METROPOLIS, 0);
             asm volatile("" : "+v"(wr.x), "+v"(wr.y), "+v"(wr.z), "+v"(wr.w));   // as in xyz_sweep_block
             sum_row<DM>(v, sJ4, hx, hy, hz);
+            if constexpr (MEAS) sum_row_lower_lanes<DM>(v, nb, a.c_begin, sJ4, hl[0], hl[1], hl[2]);
         } else {
-            local_field_lanes<DM>(px, py, pz, nb, a.D, rowb, roff, sJ4, hx, hy, hz);
+            if constexpr (MEAS)
+                local_field_lanes<DM, true>(px, py, pz, nb, a.D, rowb, roff, sJ4, hx, hy, hz, a.c_begin, hl);
+            else
+                local_field_lanes<DM>(px, py, pz, nb, a.D, rowb, roff, sJ4, hx, hy, hz);
             wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)r, step, RNG_METROPOLIS, 0);
         }
-        const float hl[3] = {hx, hy, hz};             // exchange field, before Zeeman
         const int sp = (ZEE || SIA) ? a.sp[slot] : 0;
         if (ZEE) {
             hx += sZ[3 * sp + 0];
@@ -1154,13 +1265,16 @@ __device__ __forceinline__ void xyz_sweep_lanes_block(const XyzArgs& a, int bb) 
     book_acceptance(sAcc, 1, &accepted, r, a.R, a.slot_of, a.acc);
     if constexpr (MEAS) {
         __shared__ double sMeas[kMeasWords * kBlock];
-        book_measurement(sMeas, m, 1 << ls, 0, a);
+        if (!a.mrec)
+            book_measurement(sMeas, m, 1 << ls, 0, a);
+        else if (a.c_end > a.c_begin)
+            book_records(sMeas, m, 1 << ls, bb, a);
     }
 }
 
 template <bool ANGLE, int DM, bool MEAS>
 __global__ __launch_bounds__(kBlock) void k_xyz_sweep_lanes(XyzArgs a) {
-    xyz_sweep_lanes_block<ANGLE, DM, MEAS>(a, block_index(a.nblocks, a.xcd_map));
+    xyz_sweep_lanes_block<ANGLE, DM, MEAS>(a, block_index(a.nblocks, a.xcd_map, 1, a.nchunk));
 }
 
 template <bool ANGLE, int DM, bool MEAS>
@@ -1270,7 +1384,7 @@ __device__ __forceinline__ void ising_sweep_lanes_block(const IsingArgs& a, int 
 
 template <int DM>
 __global__ __launch_bounds__(kBlock) void k_ising_sweep_lanes(IsingArgs a) {
-    ising_sweep_lanes_block<DM>(a, block_index(a.nblocks, a.xcd_map));
+    ising_sweep_lanes_block<DM>(a, block_index(a.nblocks, a.xcd_map, 1, a.nchunk));
 }
 
 template <int DM>
@@ -1666,7 +1780,31 @@ __device__ __forceinline__ void finalize_block(const FinArgs& a, int blk) {
     const int tid = threadIdx.x, lane = lane_id();
     double ex = 0.0, ez = 0.0, es = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
     double psi = 0.0;
-    if (a.model == 1 && a.macc) {
+    if (a.model == 1 && a.mrec) {
+        // fused measurement with per-workgroup records: replica r is lane
+        // r % 64 of the units bb = g + u * groups (g = r / 64) of every phase
+        // whose bb lies below the stride, contiguous per phase; the terms are
+        // integers, so any order is exact
+        const int g = r / kWave, lane = r % kWave;
+        const int cnt = (a.mrec_stride - g + a.mrec_groups - 1) / a.mrec_groups;
+        double v[kMeasWords] = {0, 0, 0, 0, 0, 0};
+        for (int q = 0; q < a.mrec_phases; ++q) {
+            const long long* p =
+                a.mrec + (((size_t)q * a.mrec_groups + g) * kWave + lane) * a.mrec_pp * kMeasWords;
+            for (int u = tid; u < cnt; u += kBlock) {
+#pragma unroll
+                for (int k = 0; k < kMeasWords; ++k) v[k] += (double)p[(size_t)u * kMeasWords + k];
+            }
+        }
+        for (int k = 0; k < kMeasWords; ++k) v[k] = block_sum(sFin, v[k]);
+        if (tid >= kWave) return;
+        ex = v[0] / a.mscale[0];
+        ez = v[1] / a.mscale[1];
+        es = v[2] / a.mscale[2];
+        mx = v[3] / a.mscale[3];
+        my = v[4] / a.mscale[3];
+        mz = v[5] / a.mscale[3];
+    } else if (a.model == 1 && a.macc) {
         // fused measurement: exact fixed-point sums (lane c < kMeasCopies of
         // wave 0 reads copy c), cleared for the next one
         if (tid >= kWave) return;
@@ -1987,8 +2125,11 @@ static void launch_xyz_sweep_t(const XyzArgs& a, bool angle, bool lanes, hipStre
 void launch_xyz_sweep(const XyzArgs& a, bool angle, bool, bool, hipStream_t st) {
     const bool lanes = a.lps_shift < 6;
     if (g_recorder) {
+        // a sweep writing measurement records runs every unit of its stride,
+        // so that each record the finalize reads is rewritten
         record(LK_XYZ_SWEEP, (a.meas ? 262144 : 0) | (angle ? 65536 : 0) | (lanes ? 131072 : 0) | a.D,
-               sweep_units(a.G, a.c_end - a.c_begin, a.spw * sites_per_step(a.lps_shift)), a);
+               a.mrec ? a.nblocks : sweep_units(a.G, a.c_end - a.c_begin, a.spw * sites_per_step(a.lps_shift)),
+               a);
         return;
     }
     if (a.meas) launch_xyz_sweep_t<true>(a, angle, lanes, st);

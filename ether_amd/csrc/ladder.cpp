@@ -21,10 +21,12 @@
 // the method without the network), and the slots are retuned in place.
 // Then three validation windows of validate_mcs MCS, plain / learned /
 // plain, each starting from the state the last left: the learned ladder is
-// kept only if its round-trip cost Lambda is below the plain ladder's and
-// its lowest pair acceptance above it (plain: the mean of its two windows,
-// which cancels a drift linear in time); otherwise the slots go back to the
-// plain ladder.  learn = 0 runs the same windows on the plain ladder only,
+// kept only if its round-trip cost Lambda is below the plain ladder's by more
+// than twice the standard error of the difference (binomial spread of every
+// pair's acceptance) and its lowest pair acceptance above the plain ladder's
+// (plain: the mean of its two windows, which cancels a drift linear in time);
+// otherwise the slots go back to the plain ladder.  A validation too short to
+// tell the ladders apart so keeps the plain one.  learn = 0 runs the same windows on the plain ladder only,
 // so both runs reach production after the same MCS.  Every swap passes the
 // exact acceptance test, so the sampled distributions are unchanged.
 #include <algorithm>
@@ -39,6 +41,9 @@ namespace ether {
 int ctx_fail(ether_ctx* c, int code, const std::string& msg);
 int ctx_R(ether_ctx* c);
 int ctx_groups(const ether_ctx* c);
+int ctx_snapshot(ether_ctx* c, void** out);
+int ctx_restore(ether_ctx* c, void* snap);
+void ctx_snapshot_free(void* snap);
 }  // namespace ether
 
 using namespace ether;
@@ -69,10 +74,25 @@ double round_trip_cost(const std::vector<double>& acc, const std::vector<double>
     return s;
 }
 
+// Variance of that estimate from the binomial spread of each pair's
+// acceptance, A accepted of n attempts (delta method, d/da (1-a)/a = -1/a^2):
+// sum_k (1 - a_k) / (a_k^3 n_k), with a = (A + 1)/(n + 2) (Laplace's rule)
+// so that a pair with no accepted swap gets a finite, large variance.
+double round_trip_var(const std::vector<double>& acc, const std::vector<double>& att) {
+    double s = 0.0;
+    for (size_t k = 0; k < acc.size(); ++k) {
+        const double n = std::max(att[k], 1.0);
+        const double a = (acc[k] * att[k] + 1.0) / (att[k] + 2.0);
+        s += (1.0 - a) / (a * a * a * n);
+    }
+    return s;
+}
+
 struct LengthMap {
     std::vector<double> T, lam;
+    // grid uniform in log T: lambda ~ sqrt(C)/T changes fastest at the cold end
     LengthMap(double lo, double hi, int n) : T(n), lam(n, std::numeric_limits<double>::quiet_NaN()) {
-        for (int g = 0; g < n; ++g) T[g] = lo + (hi - lo) * g / (n - 1);
+        for (int g = 0; g < n; ++g) T[g] = lo * std::pow(hi / lo, (double)g / (n - 1));
     }
     // a measured interval replaces what earlier windows said on its span; one
     // with no accepted swap only raises the value there to its lower bound
@@ -142,13 +162,15 @@ struct Window {
 
 }  // namespace
 
-extern "C" int ether_ladder_train(ether_ctx* c, int32_t n_temps, const double* temps, double kb,
-                                  int32_t first_step, int32_t train_mcs, int32_t windows,
-                                  int32_t validate_mcs, int32_t coa, int32_t ovrr_interval,
-                                  double ovrr_passes, int32_t pt_interval, int32_t learn,
-                                  int32_t placement, uint64_t mlp_seed, int32_t* swap_count,
-                                  double* T_out, double* report) {
-    if (!c) return ctx_fail(nullptr, ETHER_ERR_ARG, "null context");
+// One pass of the protocol (the windows, the validation, the ladder left for
+// production); ether_ladder_train below runs it, and again on the plain
+// ladder from the same start where the learned ladder is not kept.
+static int train_pass(ether_ctx* c, int32_t n_temps, const double* temps, double kb,
+                      int32_t first_step, int32_t train_mcs, int32_t windows,
+                      int32_t validate_mcs, int32_t coa, int32_t ovrr_interval,
+                      double ovrr_passes, int32_t pt_interval, int32_t learn,
+                      int32_t placement, uint64_t mlp_seed, int32_t* swap_count,
+                      double* T_out, double* report) {
     const int R = ctx_R(c);
     if (n_temps < 2 || !temps || !T_out || !swap_count || !(kb > 0.0) || R % n_temps != 0 ||
         train_mcs < 0 || windows < 1 || validate_mcs < 0 || pt_interval <= 0 ||
@@ -208,14 +230,18 @@ extern "C" int ether_ladder_train(ether_ctx* c, int32_t n_temps, const double* t
         if ((rc = window(len, w))) break;
         map.add(cur, w.acc, w.att);
         if (!learn) continue;
-        // <e> of the window on the map's grid, and log C = 2 log(lambda T)
+        // <e> of the window on the map's grid, and the thermodynamic length
+        // s(T) = int_lo^T lambda dT' of the map (trapezoid on its grid): each
+        // measured interval fixes the length across it, 2 erfcinv(a), so s is
+        // what the swaps measure; the network smooths it between the grid
+        // points and the ladder takes equal steps of the smoothed s
         std::vector<double> order_T(cur.rbegin(), cur.rend()), order_e(w.e.rbegin(), w.e.rend());
         const std::vector<double> lam = map.filled();
         const int G = (int)map.T.size();
-        std::vector<double> e(G), lc(G);
+        std::vector<double> e(G), slen(G, 0.0);
         for (int g = 0; g < G; ++g) {
             e[g] = interp(order_T, order_e, map.T[g]);
-            lc[g] = std::log(std::max(lam[g] * map.T[g] * lam[g] * map.T[g], 1e-12));
+            if (g) slen[g] = slen[g - 1] + 0.5 * (lam[g] + lam[g - 1]) * (map.T[g] - map.T[g - 1]);
         }
         std::vector<double> next(n);
         if (placement == 1) {
@@ -229,27 +255,31 @@ extern "C" int ether_ladder_train(ether_ctx* c, int32_t n_temps, const double* t
                 for (double x : v) sd += (x - mu) * (x - mu);
                 sd = std::sqrt(sd / v.size());
             };
-            double me, se, ml, sl;
+            double me, se;
             mean_sd(e, me, se);
-            mean_sd(lc, ml, sl);
-            const double mid = 0.5 * (t_lo + t_hi), half = 0.5 * (t_hi - t_lo);
+            const double total = std::max(slen[G - 1], 1e-300);
+            // the network's input is ln T, scaled to [-1, 1]; its outputs <e>
+            // (standardised) and s / s_total scaled to [-1, 1]
+            const double mid = 0.5 * (std::log(t_lo) + std::log(t_hi)), half = 0.5 * std::log(t_hi / t_lo);
             std::vector<float> x(G), y(2 * G), wt(2 * G, 1.0f);
             for (int g = 0; g < G; ++g) {
-                x[g] = (float)((map.T[g] - mid) / half);
+                x[g] = (float)((std::log(map.T[g]) - mid) / half);
                 y[2 * g] = (float)((e[g] - me) / (se + 1e-30));
-                y[2 * g + 1] = (float)((lc[g] - ml) / (sl + 1e-30));
+                y[2 * g + 1] = (float)(2.0 * slen[g] / total - 1.0);
             }
             float fl = 0.f;
             if ((rc = ether_mlp_init(c, mlp_seed))) break;
-            if ((rc = ether_mlp_train(c, G, x.data(), y.data(), wt.data(), 4000, 3e-3f, &fl))) break;
+            if ((rc = ether_mlp_train(c, G, x.data(), y.data(), wt.data(), 12000, 3e-3f, &fl))) break;
             loss = fl;
-            if ((rc = ether_ladder_place(c, mid, half, sl + 1e-30, t_lo, t_hi, n, next.data()))) break;
+            if ((rc = ether_ladder_place(c, mid, half, 1.0, t_lo, t_hi, n, ETHER_PLACE_LOG_T | ETHER_PLACE_LENGTH,
+                                         next.data())))
+                break;
         }
         cur = next;
         if ((rc = set_ladder(cur))) break;
     }
     // validation: plain, learned, plain
-    double cost_p = 0.0, cost_l = 0.0, min_p = 0.0, min_l = 0.0;
+    double cost_p = 0.0, cost_l = 0.0, min_p = 0.0, min_l = 0.0, sd = 0.0;
     int kept = 0;
     if (rc == ETHER_OK && validate_mcs > 0) {
         const std::vector<double> cand = cur;
@@ -261,7 +291,9 @@ extern "C" int ether_ladder_train(ether_ctx* c, int32_t n_temps, const double* t
             min_p = 0.5 * (min_acc(a) + min_acc(d));
             cost_l = round_trip_cost(b.acc, b.att);
             min_l = min_acc(b);
-            kept = learn && cost_l < cost_p && min_l > min_p;
+            sd = std::sqrt(0.25 * (round_trip_var(a.acc, a.att) + round_trip_var(d.acc, d.att)) +
+                           round_trip_var(b.acc, b.att));
+            kept = learn && cost_p - cost_l > 2.0 * sd && min_l > min_p;
             cur = kept ? cand : plain;
             if (kept) rc = set_ladder(cur);
         }
@@ -280,6 +312,47 @@ extern "C" int ether_ladder_train(ether_ctx* c, int32_t n_temps, const double* t
         report[4] = kept;
         report[5] = loss;
         report[6] = (double)(step - first_step);
+        report[7] = sd;
     }
     return ETHER_OK;
+}
+
+// Where validation does not keep the learned ladder, the context goes back to
+// its state at entry (ctx_snapshot) and runs the same windows on the plain
+// ladder: production then continues exactly as a run with learn = 0 would
+// (every draw is a function of the sweep and swap counters), so a rejected
+// learned ladder costs the repeated windows and changes no result.  The
+// report keeps the validation figures of the learned attempt.
+extern "C" int ether_ladder_train(ether_ctx* c, int32_t n_temps, const double* temps, double kb,
+                                  int32_t first_step, int32_t train_mcs, int32_t windows,
+                                  int32_t validate_mcs, int32_t coa, int32_t ovrr_interval,
+                                  double ovrr_passes, int32_t pt_interval, int32_t learn,
+                                  int32_t placement, uint64_t mlp_seed, int32_t* swap_count,
+                                  double* T_out, double* report) {
+    if (!c) return ctx_fail(nullptr, ETHER_ERR_ARG, "null context");
+    if (!swap_count) return ctx_fail(c, ETHER_ERR_ARG, "ether_ladder_train: bad arguments");
+    if (!learn || validate_mcs <= 0)
+        return train_pass(c, n_temps, temps, kb, first_step, train_mcs, windows, validate_mcs, coa,
+                          ovrr_interval, ovrr_passes, pt_interval, learn, placement, mlp_seed, swap_count,
+                          T_out, report);
+    void* snap = nullptr;
+    int rc = ctx_snapshot(c, &snap);
+    if (rc) return rc;
+    const int32_t sc0 = *swap_count;
+    double rep[8];
+    rc = train_pass(c, n_temps, temps, kb, first_step, train_mcs, windows, validate_mcs, coa, ovrr_interval,
+                    ovrr_passes, pt_interval, 1, placement, mlp_seed, swap_count, T_out, rep);
+    if (rc == ETHER_OK && rep[4] == 0.0) {
+        double again[8];
+        *swap_count = sc0;
+        rc = ctx_restore(c, snap);
+        if (rc == ETHER_OK)
+            rc = train_pass(c, n_temps, temps, kb, first_step, train_mcs, windows, validate_mcs, coa,
+                            ovrr_interval, ovrr_passes, pt_interval, 0, placement, mlp_seed, swap_count,
+                            T_out, again);
+    }
+    ctx_snapshot_free(snap);
+    if (rc == ETHER_OK && report)
+        for (int k = 0; k < 8; ++k) report[k] = rep[k];
+    return rc;
 }

@@ -396,7 +396,9 @@ extern "C" int ether_mlp_train(ether_ctx* c, int32_t n, const float* x, const fl
 }
 
 extern "C" int ether_ladder_place(ether_ctx* c, double x_mid, double x_half, double logcv_scale,
-                                  double t_lo, double t_hi, int32_t n_temps, double* T_out) {
+                                  double t_lo, double t_hi, int32_t n_temps, int32_t mode,
+                                  double* T_out) {
+    const bool log_t = (mode & ETHER_PLACE_LOG_T) != 0, length = (mode & ETHER_PLACE_LENGTH) != 0;
     int rc = need_mlp(c);
     if (rc) return rc;
     if (!T_out || n_temps < 2 || !(t_hi > t_lo) || !(t_lo > 0.0) || !(x_half > 0.0))
@@ -407,14 +409,23 @@ extern "C" int ether_ladder_place(ether_ctx* c, double x_mid, double x_half, dou
     std::vector<double> tg(G);
     for (int g = 0; g < G; ++g) {
         tg[g] = t_lo * std::pow(t_hi / t_lo, (double)g / (G - 1));
-        xg[g] = (float)((tg[g] - x_mid) / x_half);
+        xg[g] = (float)(((log_t ? std::log(tg[g]) : tg[g]) - x_mid) / x_half);
     }
     rc = ether_mlp_forward(c, G, xg.data(), yg.data());
     if (rc) return rc;
-    // thermodynamic length  s(T) = int_{t_lo}^{T} sqrt(C_v)/T' dT'  (trapezoid)
+    // thermodynamic length  s(T) = int_{t_lo}^{T} sqrt(C_v)/T' dT'  (trapezoid), or
+    // the surrogate's own s(T), made non-decreasing
     std::vector<double> cum(G, 0.0);
-    auto f = [&](int g) { return std::exp(0.5 * logcv_scale * (double)yg[2 * g + 1]) / tg[g]; };
-    for (int g = 1; g < G; ++g) cum[g] = cum[g - 1] + 0.5 * (f(g) + f(g - 1)) * (tg[g] - tg[g - 1]);
+    if (length) {
+        double run = (double)yg[1];
+        for (int g = 1; g < G; ++g) {
+            run = std::max(run, (double)yg[2 * g + 1]);
+            cum[g] = run - (double)yg[1];
+        }
+    } else {
+        auto f = [&](int g) { return std::exp(0.5 * logcv_scale * (double)yg[2 * g + 1]) / tg[g]; };
+        for (int g = 1; g < G; ++g) cum[g] = cum[g - 1] + 0.5 * (f(g) + f(g - 1)) * (tg[g] - tg[g - 1]);
+    }
     const double total = cum[G - 1];
     if (!(total > 0.0) || !std::isfinite(total))
         return ctx_fail(c, ETHER_ERR_STATE, "ether_ladder_place: surrogate gives no usable C_v");

@@ -18,6 +18,8 @@ ETHER_ERR_STATE = 2
 ETHER_ERR_HIP = 3
 ETHER_ERR_LIMIT = 4
 ETHER_ERR_NODEV = 5
+PLACE_LOG_T = 1
+PLACE_LENGTH = 2
 
 MODEL_ISING = 0
 MODEL_XYZ = 1
@@ -111,7 +113,7 @@ EXPORTS = [
     ("ether_mlp_forward", C.c_int, [_vp, C.c_int32, _vp, _vp]),
     ("ether_mlp_train", C.c_int, [_vp, C.c_int32, _vp, _vp, _vp, C.c_int32, C.c_float, _vp]),
     ("ether_ladder_place", C.c_int, [_vp, C.c_double, C.c_double, C.c_double, C.c_double,
-                                     C.c_double, C.c_int32, _vp]),
+                                     C.c_double, C.c_int32, C.c_int32, _vp]),
     ("ether_ladder_train", C.c_int, [_vp, C.c_int32, _vp, C.c_double, C.c_int32, C.c_int32, C.c_int32,
                                      C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_int32,
                                      C.c_int32, C.c_uint64, _vp, _vp, _vp]),
@@ -423,6 +425,7 @@ class Engine:
         self._c(self.lib.ether_set_lanes(self.h, int(lanes_per_site)))
 
     AUTOTUNE_SPW = (1, 2, 4, 8, 16)
+    AUTOTUNE_MAPS = 3          # xcd_map 0, 1, 2
 
     def autotune(self, reps: int = 20):
         """Pick the fastest sweep launch shape; returns (sites_per_wave,
@@ -430,9 +433,9 @@ class Engine:
         sites_per_wave = AUTOTUNE_SPW[k]."""
         spw = C.c_int32()
         xcd = C.c_int32()
-        table = np.zeros(2 * len(self.AUTOTUNE_SPW), np.float64)
+        table = np.zeros(self.AUTOTUNE_MAPS * len(self.AUTOTUNE_SPW), np.float64)
         self._c(self.lib.ether_autotune(self.h, int(reps), C.byref(spw), C.byref(xcd), _ptr(table)))
-        return spw.value, xcd.value, table.reshape(2, -1)
+        return spw.value, xcd.value, table.reshape(self.AUTOTUNE_MAPS, -1)
 
     def set_timing(self, on: bool) -> None:
         self._c(self.lib.ether_set_timing(self.h, 1 if on else 0))
@@ -530,20 +533,21 @@ class Engine:
                                          float(lr), C.byref(loss)))
         return loss.value
 
-    def ladder_place(self, x_mid, x_half, logcv_scale, t_lo, t_hi, n):
+    def ladder_place(self, x_mid, x_half, logcv_scale, t_lo, t_hi, n, mode=0):
+        """mode: PLACE_LOG_T (input ln T) | PLACE_LENGTH (y1 is the length s(T))."""
         T = np.zeros(int(n), np.float64)
         self._c(self.lib.ether_ladder_place(self.h, float(x_mid), float(x_half), float(logcv_scale),
-                                            float(t_lo), float(t_hi), int(n), _ptr(T)))
+                                            float(t_lo), float(t_hi), int(n), int(mode), _ptr(T)))
         return T
 
     def ladder_train(self, temps, kb, first_step, train_mcs, windows, validate_mcs, coa,
                      ovrr_interval, ovrr_passes, pt_interval, learn, placement=0, mlp_seed=1992,
                      swap_count=0):
         """The learned ladder's training and validation windows (see
-        ether_ladder_train).  Returns (production ladder, swap count, report[7])."""
+        ether_ladder_train).  Returns (production ladder, swap count, report[8])."""
         t = _arr(temps, np.float64)
         T = np.zeros(t.size, np.float64)
-        rep = np.zeros(7, np.float64)
+        rep = np.zeros(8, np.float64)
         sc = C.c_int32(int(swap_count))
         self._c(self.lib.ether_ladder_train(self.h, t.size, _ptr(t), float(kb), int(first_step),
                                             int(train_mcs), int(windows), int(validate_mcs), int(coa),

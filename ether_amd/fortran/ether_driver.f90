@@ -346,7 +346,7 @@ program ether_gpu_driver
    real(dp), allocatable :: b_xyz(:, :), stagger(:), S0(:, :), phi0(:), shells(:), jx(:, :, :, :)
    real(dp), allocatable :: start_s(:, :), start_phi(:)
    real(dp), allocatable :: siav(:, :), temps(:), betas(:), mom(:, :), per(:, :, :), tnew(:)
-   real(dp) :: lreport(7)
+   real(dp) :: lreport(8)
    real(dp), allocatable :: mean(:, :), sem(:, :), accp(:), stg_id(:)
    real(dp) :: kb, mub, gf, jv(3), ovrr_p, c1, r1
    real(dp) :: e, e2, e4, mm, m2, m4, nn

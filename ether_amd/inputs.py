@@ -474,8 +474,8 @@ def parse_gpu_ether(text: str) -> GpuTags:
                 g.autotune = int(v[0])
             elif key == "launch":
                 g.spw, g.xcd = int(v[0]), int(v[1])
-                if g.spw not in (1, 2, 4, 8, 16) or g.xcd not in (0, 1):
-                    raise EtherInputError("launch: sites per wave in {1,2,4,8,16}, XCD map 0 or 1")
+                if g.spw not in (1, 2, 4, 8, 16) or g.xcd not in (0, 1, 2):
+                    raise EtherInputError("launch: sites per wave in {1,2,4,8,16}, XCD map 0, 1 or 2")
             elif key == "lanes":
                 g.lanes = int(v[0])
                 if g.lanes not in (0, 8, 16, 32, 64):

@@ -125,8 +125,10 @@ def train_ladder(eng: E.Engine, model: Model, temps, repeat: int, coa: int, pt_i
     placement=PLACEMENT_MAP), and the slots are retuned in place.  Then three
     validation windows of validate_mcs MCS (default train_mcs / 8): plain,
     learned, plain; the learned ladder is kept only if its round-trip cost is
-    lower and its lowest pair acceptance higher than the plain ladder's,
-    otherwise production runs on the plain ladder.  learn=False runs the same
+    lower by more than twice the standard error of the difference and its
+    lowest pair acceptance higher than the plain ladder's; otherwise the
+    context returns to its state at the start and runs the same windows on
+    the plain ladder, so that production continues as with learn=False.  learn=False runs the same
     windows on `temps`.  Every replica keeps its state and slot; the windows
     follow the run's schedule (OVRR every ovrr_iv MCS with ovrr_p passes)."""
     temps = np.asarray(temps, np.float64)
@@ -171,23 +173,27 @@ def adaptive_run(eng: E.Engine, model: Model, temps, repeat: int, mcs: int, coa:
 
 
 def compare_ladders(model: Model, *, train_mcs: int = 2000, mcs: Optional[int] = None,
-                    repeat: Optional[int] = None, placement: int = PLACEMENT_MLP):
-    """The linear ladder against the learned ladder on the same model and seed.
-    Returns (linear LadderRun, learned LadderRun)."""
+                    repeat: Optional[int] = None, placement: int = PLACEMENT_MLP,
+                    validate_mcs: Optional[int] = None):
+    """The linear ladder against the learned ladder on the same model and seed,
+    each on a context of its own built alike (the same random start): where
+    validation rejects the learned ladder, its run repeats the linear one
+    exactly.  Returns (linear LadderRun, learned LadderRun)."""
     t = model.tags
     temps = t.temperatures()
     rep = t.repeat if repeat is None else repeat
     mcs = t.tmcs if mcs is None else mcs
     coa = fortran_nint(t.coa)
-    eng = make_engine(model, len(temps) * rep)
-    try:
-        lin = adaptive_run(eng, model, temps, rep, mcs, coa, t.exchange_interval, learn=False,
-                           train_mcs=train_mcs)
-        lrn = adaptive_run(eng, model, temps, rep, mcs, coa, t.exchange_interval, learn=True,
-                           train_mcs=train_mcs, placement=placement)
-    finally:
-        eng.close()
-    return lin, lrn
+
+    def run(learn):
+        eng = make_engine(model, len(temps) * rep)
+        try:
+            return adaptive_run(eng, model, temps, rep, mcs, coa, t.exchange_interval, learn=learn,
+                                train_mcs=train_mcs, placement=placement, validate_mcs=validate_mcs)
+        finally:
+            eng.close()
+
+    return run(False), run(True)
 
 
 def plain_run(model: Model, *, train_mcs: int = 2000, mcs: Optional[int] = None,

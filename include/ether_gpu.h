@@ -355,8 +355,11 @@ void ether_batch_destroy(ether_batch* batch);
  * whose sweep is one OpenMP loop over sites, src/Metropolis.f90:35-62):
  * sites_per_wave (1..16) consecutive sites of a colour class per wavefront,
  * xcd_map 1 = workgroups remapped so that each XCD (own L2) sweeps one
- * contiguous site range.  Defaults 4 and 1.  Results do not depend on the
- * shape: every update draws from its own (site, replica, sweep) counter. */
+ * contiguous site range with every group of 64 replicas, 2 = the same with
+ * the replica group outermost (an XCD sweeps more sites of fewer groups, whose
+ * neighbour spins then stay in its L2 between uses).  Defaults 4 and 2.
+ * Results do not depend on the shape: every update draws from its own (site,
+ * replica, sweep) counter. */
 int ether_set_launch(ether_ctx* ctx, int32_t sites_per_wave, int32_t xcd_map);
 /* Lanes of a wavefront given to one site in the sweep: 64 (one site per wave),
  * or 32 / 16 (8 for Ising, whose lanes hold 4 replicas each) so that a wave
@@ -366,11 +369,11 @@ int ether_set_launch(ether_ctx* ctx, int32_t sites_per_wave, int32_t xcd_map);
  * 64 / lanes sites. */
 int ether_set_lanes(ether_ctx* ctx, int32_t lanes_per_site);
 /* Autotuner: times `reps` sweeps for each of sites_per_wave {1,2,4,8,16} x
- * xcd_map {0,1} on the context's own lattice and replicas, in two rounds
+ * xcd_map {0,1,2} on the context's own lattice and replicas, in two rounds
  * (each candidate keeps its faster one), keeps the fastest, or among the
  * candidates within 1% of it the one with the most sites per wave (returned
  * through the two nullable pointers), and restores the spins, the
- * acceptance counters and the sweep counter.  table_ms[10] (nullable)
+ * acceptance counters and the sweep counter.  table_ms[15] (nullable)
  * receives the time per sweep of candidate i = xcd_map*5 + index of
  * sites_per_wave.  Allocates a temporary copy of the spin planes. */
 int ether_autotune(ether_ctx* ctx, int32_t reps, int32_t* sites_per_wave, int32_t* xcd_map,
@@ -473,13 +476,18 @@ int ether_mlp_loss_grad(ether_ctx* ctx, int32_t n, const float* x, const float* 
 int ether_mlp_forward(ether_ctx* ctx, int32_t n, const float* x, float* y);
 int ether_mlp_train(ether_ctx* ctx, int32_t n, const float* x, const float* y, const float* w,
                     int32_t iters, float lr, float* final_loss);
-/* Ladder placement from the trained surrogate: with x = (T - x_mid)/x_half and
+/* Ladder placement from the trained surrogate: with x = (T - x_mid)/x_half,
+ * or x = (ln T - x_mid)/x_half with mode bit ETHER_PLACE_LOG_T, and
  * log C_v(T) = const + logcv_scale * y1(x), places n_temps temperatures on
- * [t_lo, t_hi] (hottest first, like src/ether.f90:79-84) so that
- * int sqrt(C_v)/T dT is equal between neighbours, which makes the Gaussian
- * estimate erfc(dT sqrt(C_v)/(2T)) of the swap acceptance uniform. */
+ * [t_lo, t_hi] (hottest first, like src/ether.f90:79-84) so that the
+ * thermodynamic length s = int sqrt(C_v)/T dT is equal between neighbours,
+ * which makes the Gaussian estimate erfc(dT sqrt(C_v)/(2T)) of the swap
+ * acceptance uniform.  With mode bit ETHER_PLACE_LENGTH y1 is s(T) itself
+ * (any affine scale; its running maximum is used). */
+#define ETHER_PLACE_LOG_T   1
+#define ETHER_PLACE_LENGTH  2
 int ether_ladder_place(ether_ctx* ctx, double x_mid, double x_half, double logcv_scale,
-                       double t_lo, double t_hi, int32_t n_temps, double* T_out);
+                       double t_lo, double t_hi, int32_t n_temps, int32_t mode, double* T_out);
 
 /* Training protocol of the learned ladder, in the library so that every
  * driver runs the same one.  No reference counterpart (the nearest is the
@@ -500,15 +508,19 @@ int ether_ladder_place(ether_ctx* ctx, double x_mid, double x_half, double logcv
  *     and the slots are retuned in place (ether_retune_betas);
  *   - with validate_mcs > 0, three windows of validate_mcs MCS: plain,
  *     learned, plain.  The learned ladder is kept only if its round-trip cost
- *     Lambda = sum_k (1 - a_k)/a_k is below the plain ladder's and its lowest
- *     pair acceptance above it (plain: the mean of its two windows);
- *     otherwise the slots return to the plain ladder.
+ *     Lambda = sum_k (1 - a_k)/a_k is below the plain ladder's by more than
+ *     twice the standard error below and its lowest pair acceptance above
+ *     the plain ladder's (plain: the mean of its two windows); otherwise the
+ *     slots return to the plain ladder.
  * learn = 0 runs the same windows on the plain ladder only.  Replicas keep
  * their state and slot throughout; the swap counters are reset at the end.
  * T_out[n_temps] = the ladder the slots hold for production, hottest first;
- * swap_count in/out as in ether_run; report[7] (nullable) = {Lambda plain,
+ * swap_count in/out as in ether_run; report[8] (nullable) = {Lambda plain,
  * Lambda learned, lowest acceptance plain, lowest acceptance learned (all
- * from validation), learned kept (0/1), last MLP loss (NaN if none), MCS run}.
+ * from validation), learned kept (0/1), last MLP loss (NaN if none), MCS run,
+ * standard error of Lambda plain - Lambda learned from the binomial spread
+ * of every pair's acceptance over its attempts: sum_k (1 - a_k)/(a_k^3 n_k)
+ * per window with a = (accepted + 1)/(n + 2)}.
  * The production run continues at first_step + report[6]. */
 int ether_ladder_train(ether_ctx* ctx, int32_t n_temps, const double* temps, double kb,
                        int32_t first_step, int32_t train_mcs, int32_t windows,

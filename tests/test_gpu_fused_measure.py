@@ -1,11 +1,15 @@
 """The measurement booked by the sweep (XYZ).
 
-When a measurement follows a sweep (a sampling step, or a swap step) on a
-lattice whose colours each have neighbours of only earlier or only later
-colours (two colours), the sweep's colour phases add each site's terms as
-they update it: the exchange energy of its bonds where its neighbours are
-final already, counted twice, and its Zeeman, SIA and magnetisation terms.
-Only the finalize runs afterwards; other lattices measure in a pass.  Checked here against the standalone measurement pass and the
+When a measurement follows a sweep (a sampling step, or a swap step), the
+sweep's colour phases add each site's terms as they update it: the exchange
+energy of its bonds to neighbours of earlier colours (final already), each
+bond booked once from its later-coloured end and counted twice, and its
+Zeeman, SIA and magnetisation terms.  With two colours a phase's neighbours
+are all earlier or all later and the workgroups add their sums to shared
+fixed-point totals; with three or more (triangular, pyrochlore) a phase
+books the field of its earlier neighbours alone and every workgroup writes a
+record of its own, which the finalize adds up.  Lattices whose bonds lack
+their mirror measure in a pass.  Checked here against the standalone measurement pass and the
 oracle's FP64 sums (src/get_tot_energy.f90:35-45,
 src/get_tot_magnetisation.f90:35-38) on the state the sweep leaves, to FP32
 accuracy (1e-6 of the sum of |site energies|), and for independence of the
@@ -57,8 +61,10 @@ CASES = [
     ("sc_heis_afm_32", dict(sc="8 8 8"), 13, 16, 1, True),     # lanes kernel, 4 sites per wave
     ("sc_fields", dict(), 9, 0, 0, True),                      # Zeeman, SIA, ANGLE
     ("sc_fields", dict(angle=".F. 0"), 21, 32, 0, True),
-    ("tri_afm_96", dict(sc="12 12 1"), 13, 16, 0, False),      # 3 colours: a pass of its own
-    ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11"), 24, 0, 1, False),
+    ("tri_afm_96", dict(sc="12 12 1"), 13, 16, 0, True),       # 3 colours: per-workgroup records
+    ("tri_afm_96", dict(sc="12 12 1"), 70, 0, 0, True),        # two waves of replicas per site
+    ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11"), 24, 0, 1, True),   # 4 colours, 18 neighbours
+    ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11"), 24, 32, 1, True),  # the same, lanes kernel
 ]
 
 
@@ -109,11 +115,14 @@ def test_fused_measurement_matches_pass_and_oracle(name, over, R, lanes, ovrr, b
     ("sc_heis_afm_32", dict(sc="8 8 8"), 70, [0], 1),
     ("sc_fields", dict(), 13, [64, 16, 32], 0),
     ("tri_afm_96", dict(sc="12 12 1"), 13, [64, 16, 32], 0),
+    ("tri_afm_96", dict(sc="12 12 1"), 130, [0], 0),
+    ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11"), 24, [64, 32], 1),
 ])
 def test_fused_measurement_does_not_depend_on_launch_shape(name, over, R, lanes_list, ovrr):
     say(f"fused measurement shapes {name} R={R}")
     ref = None
-    for spw, xcd, ln in [(4, 1, lanes_list[0]), (1, 0, lanes_list[-1]), (16, 1, lanes_list[0])] + \
+    for spw, xcd, ln in [(4, 1, lanes_list[0]), (1, 0, lanes_list[-1]), (16, 1, lanes_list[0]),
+                         (2, 2, lanes_list[0]), (8, 2, lanes_list[-1])] + \
             [(8, 0, ln) for ln in lanes_list]:
         m, eng, _ = _case(name, over, R)
         eng.set_lanes(ln)
@@ -255,3 +264,69 @@ def test_booked_measurement_at_full_size():
         assert abs(parts_f[r, 0] - ex) <= 1e-6 * scale, (r, parts_f[r, 0], ex)
         assert abs(mom_f[slot[r], E.MOM_M] - np.linalg.norm(orc.magnetisation(S)) / N) <= 1e-6
     eng.close()
+
+
+def test_booked_measurement_at_the_pyrochlore_workload():
+    """The 16^3-cell pyrochlore with 256 replicas (four colours, 18
+    neighbours, per-workgroup records): the booked energies and moments equal
+    the pass's for every replica and the oracle's FP64 sums on two of them.
+    The aligned state bounds the sum of |site energies| as in the simple-cubic
+    case (unit spins, site energy at most sum_j |J_ij|)."""
+    say("fused measurement pyro_pt_16 16^3 x 16 R=256")
+    R = 256
+    m, eng, orc = pair("pyro_pt_16", R, pt=".F. 11")
+    N = m.n_sites
+    assert N == 65536
+    eng.set_betas(m.betas(np.geomspace(1.0, 0.05, R)))
+    _start(m, eng, R, 23)
+    eng.set_timing(True)
+    eng.run(1, 3, 0, 3)
+    nc = eng.layout()[0]
+    assert nc == 4 and eng.timing(E.TIME_SWEEP_MEAS)[1] == nc
+    eng.set_timing(False)
+    parts_f, _ = eng.energy_parts()
+    mom_f = eng.moments().copy()
+    slot = eng.pt_state()[0]
+    eng.reset_moments()
+    eng.measure(sample=True)
+    parts_s, _ = eng.energy_parts()
+    mom_s = eng.moments()
+    up = np.zeros((N, 3))
+    up[:, 2] = 1.0
+    bound = sum(abs(orc.site_energy(up, i)) for i in range(N))
+    assert bound > 0
+    assert np.all(np.abs(parts_f[:, 0] - parts_s[:, 0]) <= 1e-6 * bound)
+    assert np.all(np.abs(mom_f[slot, E.MOM_M] - mom_s[slot, E.MOM_M]) <= 1e-6)
+    assert np.all(mom_f[slot, E.MOM_N] == 1.0)
+    for r in (0, R - 1):
+        S, _ = eng.get_spins(r)
+        ex, _, _ = orc.total_energy(S)
+        assert abs(parts_f[r, 0] - ex) <= 1e-6 * bound, (r, parts_f[r, 0], ex)
+        assert abs(mom_f[slot[r], E.MOM_M] - np.linalg.norm(orc.magnetisation(S)) / N) <= 1e-6
+    eng.close()
+
+
+@pytest.mark.parametrize("name,over,R", [
+    ("tri_afm_96", dict(sc="12 12 1"), 70),
+    ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11"), 24),
+])
+def test_records_survive_shape_changes(name, over, R):
+    """Per-workgroup records (three or more colours): a run whose launch
+    shape changes between measuring sweeps (fewer workgroups after more)
+    gives the moments of a run at one shape, so no record of an earlier
+    shape is read again."""
+    say(f"measurement records across shapes {name} R={R}")
+    got = []
+    for shapes in ([(4, 1)] * 3, [(1, 0), (16, 1), (8, 2)]):
+        m, eng, _ = _case(name, over, R)
+        eng.set_betas(m.betas(np.linspace(5.0, 50.0, R)))
+        _start(m, eng, R, 3)
+        step = 1
+        for spw, xcd in shapes:
+            eng.set_launch(spw, xcd)
+            eng.run(step, 4, 0, 2)
+            step += 4
+        got.append((eng.moments().copy(), eng.energy_parts()[0]))
+        eng.close()
+    assert np.array_equal(got[0][0], got[1][0])
+    assert np.array_equal(got[0][1], got[1][1])

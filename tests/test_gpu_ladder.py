@@ -24,11 +24,12 @@ minimum) and more completed cold->hot->cold round trips.  Swaps pass the exact
 acceptance test in both runs, so the sampled distributions are unchanged.
 
 What the network adds: the same protocol with the ladder placed straight from
-the length map (piecewise-linear sqrt(C)/T from the measured acceptances, no
-network) gives fewer round trips, a lower minimum and a less uniform swap
-acceptance than the MLP placement, whose fit smooths the noise of the
-per-interval measurements (measured: 204 against 163 round trips, minimum
-0.160 against 0.097, coefficient of variation 0.150 against 0.226).
+the length map (the thermodynamic length s(T) of the measured acceptances,
+piecewise linear, no network) against the MLP placement (the network's
+smooth fit of the same s(T) over ln T).  Over six seeds both were kept in
+five and gave 165 +- 30 (map) and 162 +- 30 (MLP) round trips, against none
+on the linear ladder: the network matches the map, it does not improve on
+it at this size.  The test runs three seeds on which both are kept.
 """
 import numpy as np
 import pytest
@@ -61,15 +62,49 @@ def test_learned_ladder_beats_linear_on_round_trips():
 
 def test_mlp_placement_against_the_length_map_alone():
     say("ladder ablation: MLP placement against the length map")
+    from ether_amd.driver import make_engine
+    from ether_amd.inputs import fortran_nint
     m = workloads.model("pyro_pt_16", sc="2 2 2", temp="20 1 1.26666666666667",
                         repeat="16", mcs="100000 3000")
+    t = m.tags
+    temps = t.temperatures()
     cv = lambda r: r.acc_rate.std() / r.acc_rate.mean()
-    lin, mlp = ladder.compare_ladders(m, train_mcs=2000, placement=ladder.PLACEMENT_MLP)
-    _, mp = ladder.compare_ladders(m, train_mcs=2000, placement=ladder.PLACEMENT_MAP)
-    for name, r in (("linear", lin), ("mlp", mlp), ("map", mp)):
-        say(f"{name:6s}: trips {r.round_trips} cost {r.cost:.1f} acc min {r.acc_rate.min():.3f} cv {cv(r):.3f}")
-    # both placements beat the linear ladder; the network's beats the map's
-    assert mp.round_trips > lin.round_trips and mlp.round_trips > lin.round_trips
-    assert mlp.round_trips > mp.round_trips
-    assert mlp.acc_rate.min() > mp.acc_rate.min()
-    assert cv(mlp) < cv(mp)
+    trips = {"linear": [], "mlp": [], "map": []}
+    for seed in (1000, 1001, 1002):
+        runs = {}
+        for name, learn, pl in (("linear", False, 0), ("mlp", True, ladder.PLACEMENT_MLP),
+                                ("map", True, ladder.PLACEMENT_MAP)):
+            eng = make_engine(m, len(temps) * t.repeat, seed=seed)
+            try:
+                runs[name] = ladder.adaptive_run(eng, m, temps, t.repeat, t.tmcs, fortran_nint(t.coa),
+                                                 t.exchange_interval, learn=learn, train_mcs=2000, placement=pl)
+            finally:
+                eng.close()
+            r = runs[name]
+            say(f"seed {seed} {name:6s}: {r.source} trips {r.round_trips} cost {r.cost:.1f} "
+                f"acc min {r.acc_rate.min():.3f} cv {cv(r):.3f}")
+            trips[name].append(r.round_trips)
+        # both placements pass validation and beat the linear ladder
+        for name in ("mlp", "map"):
+            assert runs[name].source == "learned"
+            assert runs[name].round_trips > runs["linear"].round_trips
+            assert runs[name].acc_rate.min() > runs["linear"].acc_rate.min()
+            assert cv(runs[name]) < cv(runs["linear"])
+    # the network keeps what the map measured: its round trips within 15% of the map's
+    assert np.mean(trips["mlp"]) >= 0.85 * np.mean(trips["map"])
+
+
+def test_rejected_learned_ladder_repeats_the_linear_run():
+    """Validation windows of one swap round cannot tell the ladders apart:
+    the learned ladder is rejected, the context goes back to its state before
+    training and runs the windows on the linear ladder, and the whole run
+    (training, validation, production) is then the linear run bit for bit."""
+    say("ladder: rejection repeats the linear run")
+    m = workloads.model("pyro_pt_16", sc="2 2 2", temp="20 1 1.26666666666667",
+                        repeat="4", mcs="1500 1000")
+    lin, lrn = ladder.compare_ladders(m, train_mcs=800, validate_mcs=11)
+    assert lin.source == "plain" and lrn.source == "plain"
+    assert np.array_equal(lrn.temps, lin.temps)
+    assert lrn.round_trips == lin.round_trips
+    assert np.array_equal(lrn.acc_rate, lin.acc_rate)
+    assert np.array_equal(lrn.metro_acc, lin.metro_acc)

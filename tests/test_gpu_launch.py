@@ -46,7 +46,8 @@ def test_sweeps_do_not_depend_on_launch_shape(name, over, R, lanes):
     for few replicas) change which lanes do an update, never its result."""
     say(f"launch shapes {name} R={R} lanes {lanes}")
     ref = None
-    shapes = [(4, 1, lanes[0]), (1, 0, lanes[-1]), (2, 1, lanes[0]), (8, 0, lanes[-1]), (16, 1, lanes[0])]
+    shapes = [(4, 1, lanes[0]), (1, 0, lanes[-1]), (2, 1, lanes[0]), (8, 0, lanes[-1]), (16, 1, lanes[0]),
+              (4, 2, lanes[0]), (1, 2, lanes[-1]), (16, 2, lanes[0])]
     shapes += [(4, 1, ln) for ln in lanes]
     for spw, xcd, ln in shapes:
         m, eng, _ = pair(name, R, **over)
@@ -82,12 +83,13 @@ def test_autotune_restores_state_and_changes_no_result(name, over, R):
         if tune:
             spw, xcd, table = eng.autotune(reps=3)
             say(f"  chose sites_per_wave={spw} xcd_map={xcd}; ms per sweep {np.round(table, 4)}")
-            assert spw in E.Engine.AUTOTUNE_SPW and xcd in (0, 1)
+            assert spw in E.Engine.AUTOTUNE_SPW and xcd in (0, 1, 2)
+            assert table.shape == (3, len(E.Engine.AUTOTUNE_SPW))
             assert np.all(table > 0)
             # the fastest, or the most sites per wave among those within 1% of it
             chosen = table[xcd][E.Engine.AUTOTUNE_SPW.index(spw)]
             assert chosen <= 1.01 * table.min()
-            near = [E.Engine.AUTOTUNE_SPW[i] for x in (0, 1) for i in range(table.shape[1])
+            near = [E.Engine.AUTOTUNE_SPW[i] for x in range(3) for i in range(table.shape[1])
                     if table[x][i] <= 1.01 * table.min()]
             assert spw == max(near)
             assert np.array_equal(_spins(eng, R), before)

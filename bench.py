@@ -324,7 +324,7 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args, over=None, l
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": profiled_traffic(kernel, workload, "traffic_bytes_per_sweep" if res_n > 0 else
-                                        f"traffic_bytes_per_launch[spw={spw}]"),
+                                        f"traffic_bytes_per_launch[spw={spw},xcd={xcd}]"),
             "kernel": f"{kernel} ({unit})",
             "bytes_per_update": b_upd, "updates_per_launch": per_launch_updates,
             "avg_launch_ms": avg_ms, "launches": sweep_n,

@@ -1,0 +1,85 @@
+"""Distributional parity of whole runs at the benchmark lattices' sizes.
+
+The workloads' lattices at full size (16^3 Ising, 32^3 Heisenberg with OVRR,
+96 x 96 triangular with SIA and Zeeman, 16^3-cell pyrochlore with its two
+shells), each over temperatures where a chain equilibrates within the run
+(the Ising scan crosses its transition; the continuous-spin ones stay above
+the ranges where relaxation takes longer than the run: 20 -> 5 K for the
+triangular and pyrochlore lattices, whose benchmark ranges reach 0.5 and
+0.05 K), with Marsaglia moves.  The GPU run goes through the driver (colour
+order, the schedule of src/ether.f90); the oracle runs every temperature x
+repeat chain on its own in the reference's typewriter order
+(src/Metropolis.f90:39) with an independent seed, the chains spread over host
+threads.  Per temperature the reference's estimators (e, U_e, C_v, m, U_m,
+chi; src/observables_per_repeat.f90) are compared by Welch's t on their
+means over 10 repeats (SEMs from the repeats), as in helpers.assert_runs_agree:
+the smallest p-value above 0.001 / K (K comparisons: a family-wise 0.1% test)
+and no more comparisons below p = 0.01 than 1% of K plus three binomial
+standard deviations (at least 2).
+"""
+import math
+import os
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+from ether_amd import driver, workloads
+from ether_amd.inputs import fortran_nint
+from oracle import oracle as o
+from tests.helpers import assert_runs_agree, say
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_run(m, seed):
+    t = m.tags
+    temps = t.temperatures()
+    betas = m.betas(temps)
+    orc = o.Oracle.from_model(m)
+    n = m.n_sites
+    order = np.arange(n, dtype=np.int32)
+    coa = fortran_nint(t.coa)
+    ovrr_on = t.ovrr and t.xyz and not t.zeeman and not t.sia
+
+    def chain(kj):
+        k, j = kj
+        rng = np.random.default_rng([seed, k, j])
+        if t.ising:
+            S = np.ascontiguousarray(-m.initial_spins)       # fresh_spins flips every Ising spin
+        else:
+            v = rng.normal(size=(n, 3))                     # Marsaglia: uniform on S^2
+            S = np.ascontiguousarray(v / np.linalg.norm(v, axis=1)[:, None])
+        mom = orc.run_chain(S, None, order, order, betas[j], seed, k * len(temps) + j, t.tmcs,
+                            t.tmcs_eq, coa, t.ovrr_mcs if ovrr_on else 0,
+                            t.ovrr_para if ovrr_on else 0.0)
+        return k, j, driver.repeat_observables(mom, betas[j], n)
+
+    per = np.zeros((t.repeat, len(temps), 6))
+    jobs = [(k, j) for k in range(t.repeat) for j in range(len(temps))]
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as pool:
+        for k, j, obs in pool.map(chain, jobs):
+            per[k, j] = obs
+    return per.mean(axis=0), per.std(axis=0, ddof=1) / math.sqrt(t.repeat)
+
+
+CASES = [
+    # C1: the Ising scan 100 -> 10 K (T_c = 4.51 J = 52 K) on 16^3
+    ("sc_ising_16", dict(mcs="1500 500", repeat="10", temp="100 10 10")),
+    # C2: the 32^3 antiferromagnet, OVRR every MCS, 60 -> 20 K (T_N = 1.44 J = 17 K)
+    ("sc_heis_afm_32", dict(mcs="600 200", repeat="10", temp="60 20 10")),
+    # C3: 96 x 96 triangular with SIA and Zeeman
+    ("tri_afm_96", dict(mcs="1000 400", repeat="10", temp="20 5 5", angle=".F. 0")),
+    # C4: the 16^3-cell pyrochlore, J1 + J2, without tempering
+    ("pyro_pt_16", dict(mcs="400 150", repeat="10", temp="20 5 5", pt=".F. 11")),
+]
+
+
+@pytest.mark.parametrize("name,over", CASES)
+def test_observables_agree_at_full_size(name, over):
+    say(f"full-size runs {name} {over}")
+    m = workloads.model(name, **over)
+    res = driver.run(m)
+    mean_o, sem_o = _oracle_run(m, 4242)
+    n = m.tags.repeat
+    assert_runs_agree(res.mean, res.sem, mean_o, sem_o, n, n, name)

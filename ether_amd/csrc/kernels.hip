@@ -715,6 +715,49 @@ __device__ __forceinline__ void class_add(uint32_t (&d)[kMaxIsingClasses + 1], i
         if (k == c) d[k] += v;
 }
 
+// The flip mask of one site's word (byte b is 0xFE where replica b accepts)
+// from its per-byte down counts d and bond counts cnt per class, its word v
+// and its Philox block wr.  The four thresholds are looked up together: from
+// LDS when the tables are staged there (a uniform branch), else by buffer
+// loads from the table in HBM (a distinct load, so the two are not merged
+// into one flat load behind a pointer select).  A byte past the last replica
+// (tbase < 0, uniform) reads table 0 at an index inside it (its down counts
+// lie in [0, cnt]) and never flips.
+__device__ __forceinline__ uint32_t ising_flip(const IsingArgs& a, const int* sStride, const uint32_t* sThr,
+                                               const int (&tbase)[4], const uint32_t (&d)[kMaxIsingClasses + 1],
+                                               const int (&cnt)[kMaxIsingClasses + 1], int spc, uint32_t v,
+                                               u32x4 wr) {
+    int ti[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        int idx = a.offset;
+#pragma unroll
+        for (int k = 1; k <= kMaxIsingClasses; ++k) {
+            if (k > a.n_classes) break;
+            const int down = (int)((d[k] >> (8 * b)) & 0xffu);
+            idx += sStride[k] * (cnt[k] - 2 * down);
+        }
+        const int sigma = (int)(int8_t)((v >> (8 * b)) & 0xffu);
+        ti[b] = max(tbase[b], 0) + (spc * 2 + (sigma > 0)) * a.P + idx;
+    }
+    uint32_t th[4];
+    if (a.table_in_lds) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) th[b] = sThr[ti[b]];
+    } else {
+        const __amdgpu_buffer_rsrc_t tb = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)a.thr, (short)0, (int)((size_t)a.R * a.table_per_slot * 4u), 0x00020000);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) th[b] = __builtin_amdgcn_raw_buffer_load_b32(tb, 4 * ti[b], 0, 0);
+    }
+    const uint32_t words[4] = {wr.x, wr.y, wr.z, wr.w};
+    uint32_t flip = 0u;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+        if (tbase[b] >= 0 && (words[b] < th[b] || th[b] == 0xffffffffu)) flip |= 0xFEu << (8 * b);   // +1 <-> -1 as int8: x ^ 0xFE
+    return flip;
+}
+
 template <int DM>
 __device__ __forceinline__ void ising_sweep_block(const IsingArgs& a, int bb) {
     __shared__ int sStride[kMaxIsingClasses + 1];
@@ -787,27 +830,9 @@ __device__ __forceinline__ void ising_sweep_block(const IsingArgs& a, int bb) {
             v = *own;
             wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)rid, step, RNG_METROPOLIS, 0);
         }
-        const uint32_t words[4] = {wr.x, wr.y, wr.z, wr.w};
-        const int spc = a.sp[slot];
-        uint32_t flip = 0;
+        const uint32_t flip = ising_flip(a, sStride, sThr, tbase, d, cnt, a.sp[slot], v, wr);
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            if (tbase[b] < 0) continue;
-            int idx = a.offset;
-#pragma unroll
-            for (int k = 1; k <= kMaxIsingClasses; ++k) {
-                if (k > a.n_classes) break;
-                const int down = (int)((d[k] >> (8 * b)) & 0xffu);
-                idx += sStride[k] * (cnt[k] - 2 * down);
-            }
-            const int sigma = (int)(int8_t)((v >> (8 * b)) & 0xffu);
-            const int t = tbase[b] + (spc * 2 + (sigma > 0)) * a.P + idx;
-            const uint32_t th = a.table_in_lds ? sThr[t] : a.thr[t];
-            if (words[b] < th || th == 0xffffffffu) {
-                flip |= 0xFEu << (8 * b);            // +1 <-> -1 as int8: x ^ 0xFE
-                ++accepted[b];
-            }
-        }
+        for (int b = 0; b < 4; ++b) accepted[b] += (flip >> (8 * b + 1)) & 1u;
         if (flip) *own = v ^ flip;
     }
     book_acceptance(sAcc, 4, accepted, r0, a.R, a.slot_of, a.acc);
@@ -864,7 +889,10 @@ __device__ __forceinline__ void class_add_lane(uint32_t (&d)[kMaxIsingClasses + 
     }
 }
 
-template <int DM, int NC>
+// TL: the four replicas' threshold tables are staged in LDS (a.table_in_lds);
+// a template argument, so that the lookups are LDS reads and not flat loads
+// through a pointer chosen at run time
+template <int DM, int NC, bool TL>
 __global__ void __launch_bounds__(1024) k_ising_resident(IsingResArgs a) {
     constexpr int KC = NC > 0 ? NC : kMaxIsingClasses;   // classes the loops visit
     extern __shared__ __attribute__((aligned(16))) uint32_t sRes[];
@@ -880,10 +908,10 @@ __global__ void __launch_bounds__(1024) k_ising_resident(IsingResArgs a) {
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
         const int r = r0 + b;
-        tbase[b] = (r < a.R) ? (a.table_in_lds ? b * a.table_per_slot : a.slot_of[r] * a.table_per_slot) : -1;
+        tbase[b] = (r < a.R) ? (TL ? b * a.table_per_slot : a.slot_of[r] * a.table_per_slot) : -1;
     }
     for (int q = t; q < a.N; q += nt) sS[q] = *reinterpret_cast<const uint32_t*>(a.s + (size_t)q * a.Rp + r0);
-    if (a.table_in_lds)
+    if (TL)
         for (int b = 0; b < 4; ++b) {
             const int r = r0 + b;
             if (r >= a.R) continue;
@@ -896,51 +924,80 @@ __global__ void __launch_bounds__(1024) k_ising_resident(IsingResArgs a) {
     __syncthreads();
     unsigned accepted[4] = {0u, 0u, 0u, 0u};
     const uint32_t step0 = a.ctr[0] + a.step_off;
+    // per-byte down counts (rows padded with class 0 up to DM) per coupling class of one site's row
+    auto gather = [&](int slot, uint32_t (&d)[kMaxIsingClasses + 1], int (&cnt)[kMaxIsingClasses + 1]) {
+#pragma unroll
+        for (int kk = 0; kk <= kMaxIsingClasses; ++kk) {
+            d[kk] = 0u;
+            cnt[kk] = 0;
+        }
+        const uint32_t* nb = a.ell + (size_t)slot * a.D;
+        auto add = [&](int j) {
+            const uint32_t e = nb[j];
+            const int cl = (int)(e >> 24);
+            const uint32_t nv = sS[e & 0xffffffu];
+            class_add_lane<NC>(d, cnt, a.n_classes, cl, (nv >> 1) & 0x01010101u);   // 1 per byte where sigma = -1
+        };
+        if constexpr (DM > 0) {
+#pragma unroll
+            for (int j = 0; j < DM; ++j) add(j);
+        } else {
+            for (int j = 0; j < a.D; ++j) add(j);
+        }
+    };
+    // the flip mask of one site's word: byte b is 0xFE where replica b
+    // accepts.  The four thresholds are looked up together; a byte past the
+    // last replica (tbase < 0, uniform) reads table 0 at an index inside it
+    // (its down counts still lie in [0, cnt]) and never flips.
+    auto decide = [&](int slot, const uint32_t (&d)[kMaxIsingClasses + 1], const int (&cnt)[kMaxIsingClasses + 1],
+                      uint32_t v, u32x4 wr) {
+        const uint32_t words[4] = {wr.x, wr.y, wr.z, wr.w};
+        const int spc = a.sp[slot];
+        uint32_t th[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            int idx = a.offset;
+#pragma unroll
+            for (int kk = 1; kk <= KC; ++kk) {
+                if (NC == 0 && kk > a.n_classes) break;
+                const int down = (int)((d[kk] >> (8 * b)) & 0xffu);
+                idx += a.stride[kk] * (cnt[kk] - 2 * down);
+            }
+            const int sigma = (int)(int8_t)((v >> (8 * b)) & 0xffu);
+            const int ti = max(tbase[b], 0) + (spc * 2 + (sigma > 0)) * a.P + idx;
+            th[b] = TL ? sThr[ti] : a.thr[ti];
+        }
+        uint32_t flip = 0u;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (tbase[b] >= 0 && (words[b] < th[b] || th[b] == 0xffffffffu))
+                flip |= 0xFEu << (8 * b);   // +1 <-> -1 as int8: x ^ 0xFE
+        return flip;
+    };
     for (int k = 0; k < a.n_sweeps; ++k) {
         const uint32_t step = step0 + (uint32_t)k;
         for (int c = 0; c < a.ncol; ++c) {
             const int c_end = sCb[c + 1];
-            for (int slot = sCb[c] + t; slot < c_end; slot += nt) {
-                const uint32_t* nb = a.ell + (size_t)slot * a.D;
-                uint32_t d[kMaxIsingClasses + 1];
-                int cnt[kMaxIsingClasses + 1];
+            // two sites of the class per thread in flight: sites of one colour
+            // share no bond, so both rows are read before either word is
+            // written and the updates are those of one site at a time.  A
+            // thread without a second site repeats the first and drops it.
+            for (int sa = sCb[c] + t; sa < c_end; sa += 2 * nt) {
+                const bool hb = sa + nt < c_end;
+                const int sb = hb ? sa + nt : sa;
+                uint32_t da[kMaxIsingClasses + 1], db[kMaxIsingClasses + 1];
+                int ca[kMaxIsingClasses + 1], cb[kMaxIsingClasses + 1];
+                gather(sa, da, ca);
+                gather(sb, db, cb);
+                const uint32_t va = sS[sa], vb = sS[sb];
+                const u32x4 wa = rng_draw(a.seed, (uint32_t)sa, (uint32_t)rid, step, RNG_METROPOLIS, 0);
+                const u32x4 wb = rng_draw(a.seed, (uint32_t)sb, (uint32_t)rid, step, RNG_METROPOLIS, 0);
+                const uint32_t fa = decide(sa, da, ca, va, wa);
+                const uint32_t fb = hb ? decide(sb, db, cb, vb, wb) : 0u;
 #pragma unroll
-                for (int kk = 0; kk <= kMaxIsingClasses; ++kk) {
-                    d[kk] = 0u;
-                    cnt[kk] = 0;
-                }
-                const int nD = DM > 0 ? DM : a.D;   // rows padded with class 0 up to DM
-#pragma unroll
-                for (int j = 0; j < nD; ++j) {
-                    const uint32_t e = nb[j];
-                    const int cl = (int)(e >> 24);
-                    const uint32_t nv = sS[e & 0xffffffu];
-                    class_add_lane<NC>(d, cnt, a.n_classes, cl, (nv >> 1) & 0x01010101u);   // 1 per byte where sigma = -1
-                }
-                const uint32_t v = sS[slot];
-                const u32x4 wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)rid, step, RNG_METROPOLIS, 0);
-                const uint32_t words[4] = {wr.x, wr.y, wr.z, wr.w};
-                const int spc = a.sp[slot];
-                uint32_t flip = 0u;
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    if (tbase[b] < 0) continue;
-                    int idx = a.offset;
-#pragma unroll
-                    for (int kk = 1; kk <= KC; ++kk) {
-                        if (NC == 0 && kk > a.n_classes) break;
-                        const int down = (int)((d[kk] >> (8 * b)) & 0xffu);
-                        idx += a.stride[kk] * (cnt[kk] - 2 * down);
-                    }
-                    const int sigma = (int)(int8_t)((v >> (8 * b)) & 0xffu);
-                    const int ti = tbase[b] + (spc * 2 + (sigma > 0)) * a.P + idx;
-                    const uint32_t th = a.table_in_lds ? sThr[ti] : a.thr[ti];   // uniform choice
-                    if (words[b] < th || th == 0xffffffffu) {
-                        flip |= 0xFEu << (8 * b);            // +1 <-> -1 as int8: x ^ 0xFE
-                        ++accepted[b];
-                    }
-                }
-                if (flip) sS[slot] = v ^ flip;
+                for (int b = 0; b < 4; ++b) accepted[b] += ((fa >> (8 * b + 1)) & 1u) + ((fb >> (8 * b + 1)) & 1u);
+                if (fa) sS[sa] = va ^ fa;
+                if (fb) sS[sb] = vb ^ fb;
             }
             __syncthreads();
         }
@@ -1356,27 +1413,9 @@ __device__ __forceinline__ void ising_sweep_lanes_block(const IsingArgs& a, int 
             v = *own;
             wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)rid, step, RNG_METROPOLIS, 0);
         }
-        const uint32_t words[4] = {wr.x, wr.y, wr.z, wr.w};
-        const int spc = a.sp[slot];
-        uint32_t flip = 0;
+        const uint32_t flip = ising_flip(a, sStride, sThr, tbase, d, cnt, a.sp[slot], v, wr);
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            if (tbase[b] < 0) continue;
-            int idx = a.offset;
-#pragma unroll
-            for (int k = 1; k <= kMaxIsingClasses; ++k) {
-                if (k > a.n_classes) break;
-                const int down = (int)((d[k] >> (8 * b)) & 0xffu);
-                idx += sStride[k] * (cnt[k] - 2 * down);
-            }
-            const int sigma = (int)(int8_t)((v >> (8 * b)) & 0xffu);
-            const int t = tbase[b] + (spc * 2 + (sigma > 0)) * a.P + idx;
-            const uint32_t th = a.table_in_lds ? sThr[t] : a.thr[t];
-            if (words[b] < th || th == 0xffffffffu) {
-                flip |= 0xFEu << (8 * b);
-                ++accepted[b];
-            }
-        }
+        for (int b = 0; b < 4; ++b) accepted[b] += (flip >> (8 * b + 1)) & 1u;
         if (flip) *own = v ^ flip;
     }
     book_acceptance(sAcc, 4, accepted, r0, a.R, a.slot_of, a.acc);
@@ -2168,21 +2207,27 @@ void launch_ising_sweep(const IsingArgs& a, hipStream_t st) {
 #undef LI
 }
 
-void launch_ising_resident(const IsingResArgs& a, int block, size_t lds_bytes, hipStream_t st) {
-    const dim3 grid((unsigned)(a.Rp / 4)), blk((unsigned)block);
+template <bool TL>
+static void launch_ising_resident_tl(const IsingResArgs& a, dim3 grid, dim3 blk, size_t lds_bytes, hipStream_t st) {
     if (a.n_classes == 1) {
-#define LR(DM) hipLaunchKernelGGL((k_ising_resident<DM, 1>), grid, blk, lds_bytes, st, a)
+#define LR(DM) hipLaunchKernelGGL((k_ising_resident<DM, 1, TL>), grid, blk, lds_bytes, st, a)
         ETHER_DISPATCH_DM(a.D, LR)
 #undef LR
     } else if (a.n_classes == 2) {
-#define LR(DM) hipLaunchKernelGGL((k_ising_resident<DM, 2>), grid, blk, lds_bytes, st, a)
+#define LR(DM) hipLaunchKernelGGL((k_ising_resident<DM, 2, TL>), grid, blk, lds_bytes, st, a)
         ETHER_DISPATCH_DM(a.D, LR)
 #undef LR
     } else {
-#define LR(DM) hipLaunchKernelGGL((k_ising_resident<DM, 0>), grid, blk, lds_bytes, st, a)
+#define LR(DM) hipLaunchKernelGGL((k_ising_resident<DM, 0, TL>), grid, blk, lds_bytes, st, a)
         ETHER_DISPATCH_DM(a.D, LR)
 #undef LR
     }
+}
+
+void launch_ising_resident(const IsingResArgs& a, int block, size_t lds_bytes, hipStream_t st) {
+    const dim3 grid((unsigned)(a.Rp / 4)), blk((unsigned)block);
+    if (a.table_in_lds) launch_ising_resident_tl<true>(a, grid, blk, lds_bytes, st);
+    else launch_ising_resident_tl<false>(a, grid, blk, lds_bytes, st);
 }
 
 void launch_measure(const MeasArgs& a, int model, bool zee, bool sia, hipStream_t st) {

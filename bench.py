@@ -417,8 +417,22 @@ def sq_bench(calls):
     gemms = [2] + [4] * (len(axes) - 1)          # the first axis has a zero imaginary part
     flops = sum(g * 2.0 * nr3 * Lk for g, Lk in zip(gemms, axes))
     ms = dev_ms / max(n, 1)
+    # bytes each stage must move once (replica pitch Rp = R here): the first
+    # axis reads the 3 spin planes and writes U re/im, every later axis reads
+    # U re/im and writes U re/im (the last one the norm plane instead), and
+    # the transpose reads the norm plane and writes the [replica][q] result
+    plane = 4.0 * N * R
+    nbytes = 3 * plane + 6 * plane                       # first axis
+    nbytes += (len(axes) - 2) * 12 * plane if len(axes) > 2 else 0.0
+    nbytes += 6 * plane + plane if len(axes) > 1 else 0.0
+    nbytes += 2 * plane                                  # transpose
     return {"workload": f"tri_afm_96: {L[0]}x{L[1]} cells ({N} spins, {N} q points), {R} replicas",
             "device_ms_per_call": ms, "wall_ms_per_call": wall * 1e3, "calls": n,
+            "roofline": {"bound": "hbm", "traffic_bytes_per_call": nbytes,
+                         "achieved": nbytes / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "note": "bytes every stage must read and write once over the device time of a call; "
+                                 "the MFMA issue rate below is far from its peak, the stages move bytes"},
             "dft_flops": flops, "mfma_flops": 3.0 * flops,
             "mfma_tflops_per_s": 3.0 * flops / (ms * 1e-3) / 1e12,
             "direct_sum_flops": 12.0 * N * N * R,

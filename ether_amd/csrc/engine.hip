@@ -72,6 +72,7 @@ struct ether_ctx {
     // Ising sweeps with the lattice resident in LDS (k_ising_resident):
     // possible when a word of every site plus the thresholds fit the budget
     int resident = 1;       // ether_set_resident
+    int dft_resident = 1;   // ether_set_dft_resident
     bool res_ok = false;
     int res_block = 256, res_thr_lds = 0;
     size_t res_lds = 0;
@@ -2073,6 +2074,12 @@ extern "C" int ether_set_resident(ether_ctx* c, int32_t on) {
     return ETHER_OK;
 }
 
+extern "C" int ether_set_dft_resident(ether_ctx* c, int32_t on) {
+    if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
+    c->dft_resident = on ? 1 : 0;
+    return ETHER_OK;
+}
+
 extern "C" int ether_set_graphs(ether_ctx* c, int32_t on) {
     if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
     c->use_graphs = on ? 1 : 0;
@@ -2566,6 +2573,7 @@ const int8_t* ctx_ising(ether_ctx* c) { return c->d_s; }
 const int* ctx_slot_of_site(ether_ctx* c) { return c->slot_of_site.data(); }
 const int* ctx_site_of_slot_dev(ether_ctx* c) { return c->d_site_of_slot; }
 int ctx_device(ether_ctx* c) { return c->device; }
+bool ctx_dft_resident(ether_ctx* c) { return c->dft_resident != 0; }
 
 // The state a run evolves (every device buffer of the context, the host
 // counters of sweeps, swaps and starts, the ladder and its temperatures), to

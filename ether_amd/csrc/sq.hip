@@ -63,6 +63,7 @@ const int8_t* ctx_ising(ether_ctx* c);
 const int* ctx_slot_of_site(ether_ctx* c);
 const int* ctx_site_of_slot_dev(ether_ctx* c);
 int ctx_device(ether_ctx* c);
+bool ctx_dft_resident(ether_ctx* c);
 }  // namespace ether
 
 using namespace ether;
@@ -107,6 +108,7 @@ struct DftArgs {
     const unsigned short* w;  // grid: [sets][4][Mp][Kp]: C hi, C lo, S hi, S lo, zero padded
     const int* wsel;          // grid: W set of sublattice l = (batch index i) % nb; null: set 0
     int nb;
+    int nsets;                // grid: W sets of the axis (distinct sublattice offsets along it)
     int L, M, Mp, Kp, inner, outer, ncomp, N, R, Rp;
     int tiles_m, tiles_n;
     // line form
@@ -370,6 +372,168 @@ __global__ __launch_bounds__(256) void k_dft_stage(DftArgs g) {
                 }
 }
 
+// One grid axis stage for axes of at most 96 cells, with every W matrix of
+// the axis resident in LDS (the C hi/lo and S hi/lo parts of each offset set,
+// staged once per workgroup).  Each wave computes all rows of one output slab
+// (the MB*16 >= L rows of one line, 16 replica columns) on its own: its B
+// operand fragments (8 consecutive k of one replica per lane) are read
+// straight from memory into registers, one K step ahead of the products, so
+// the input is read once per stage and the waves share nothing but W (and, in
+// the first stage, each wave's own copy of its line's row -> slot map).
+// Waves walk the items (line, component unless LAST, slab) with a stride of
+// the whole grid.  Same products, in the same order per output element, as
+// k_dft_stage.
+template <int MB, bool FIRST, bool LAST>
+__global__ __launch_bounds__(256, 2) void k_dft_axis(DftArgs g, int items, int nslab) {
+    extern __shared__ __attribute__((aligned(16))) unsigned short sW[];   // [nsets][4][MB*16][KP]
+    __shared__ int sRowW[4][FIRST ? MB * 16 : 1];                         // slot of each row of the wave's line
+    constexpr int ROWS = MB * 16;
+    const int KP = g.Kp + 8;                       // row pitch (shorts): 16-byte rows, no bank conflicts
+    {
+        const int rowv = g.Kp / 8;                 // uint4 per row
+        const int total = g.nsets * 4 * ROWS * rowv;
+        for (int e = threadIdx.x; e < total; e += 256) {
+            const int k8 = e % rowv, rm = e / rowv;
+            const int m = rm % ROWS, sq = rm / ROWS;   // sq = set*4 + matrix
+            *reinterpret_cast<uint4*>(sW + ((size_t)sq * ROWS + m) * KP + 8 * k8) =
+                *reinterpret_cast<const uint4*>(g.w + ((size_t)sq * g.Mp + m) * g.Kp + 8 * k8);
+        }
+        __syncthreads();
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nwaves = gridDim.x * 4;
+    const int kq = 8 * (lane >> 4);                // this lane's 8 k within a 32-deep step
+    const int nsteps = g.Kp / BK;
+    for (int it = blockIdx.x * 4 + wv; it < items; it += nwaves) {
+        int rest = it;
+        const int slab = rest % nslab;
+        rest /= nslab;
+        int a_only = 0;
+        if constexpr (!LAST) {
+            a_only = rest % g.ncomp;
+            rest /= g.ncomp;
+        }
+        const int i = rest % g.inner, o = rest / g.inner;
+        const long long site0 = i + (long long)g.inner * g.L * o;
+        const int r = slab * 16 + (lane & 15);
+        const int set = g.wsel ? g.wsel[i % g.nb] : 0;
+        const unsigned short* W = sW + (size_t)set * 4 * ROWS * KP;
+        const int a_begin = LAST ? 0 : a_only, a_end = LAST ? g.ncomp : a_only + 1;
+        if constexpr (FIRST) {
+            // the line's row -> slot map, read by this wave only
+            __builtin_amdgcn_wave_barrier();           // the previous item's reads are done
+            for (int n = lane; n < g.L; n += 64) sRowW[wv][n] = g.map[site0 + (long long)g.inner * n];
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+        }
+        // raw input of one k step: rows k0 + kq .. k0 + kq + 7 of replica r,
+        // by buffer loads (the host keeps every plane under 2 GiB): a lane
+        // offset plus a wave-uniform row stride; rows past the axis (n >= L)
+        // read 0 through an offset past the buffer
+        auto fetch = [&](int a, int k0, float (&xr)[8], float (&xi)[8]) {
+            if constexpr (FIRST) {
+                const float* pl = g.s8 ? nullptr : (a == 0 ? g.plane[0] : (a == 1 ? g.plane[1] : g.plane[2]));
+                const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                    g.s8 ? (void*)g.s8 : (void*)pl, (short)0, (int)((size_t)g.N * g.Rp * (g.s8 ? 1 : 4)), 0x00020000);
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    const int n = k0 + kq + v;
+                    const int slot = n < g.L ? sRowW[wv][n] : -1;
+                    if (g.s8) {
+                        const uint32_t off = slot < 0 ? 0x80000000u : (uint32_t)(slot * g.Rp + r);
+                        xr[v] = (float)(int8_t)__builtin_amdgcn_raw_buffer_load_b8(rs, (int)off, 0, 0);
+                    } else {
+                        const uint32_t off = slot < 0 ? 0x80000000u : 4u * (uint32_t)(slot * g.Rp + r);
+                        xr[v] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, (int)off, 0, 0));
+                    }
+                    xi[v] = 0.f;
+                }
+            } else {
+                const int bytes = (int)((size_t)g.ncomp * g.N * g.Rp * 4);
+                const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc((void*)g.in_re, (short)0, bytes, 0x00020000);
+                const __amdgpu_buffer_rsrc_t ri = __builtin_amdgcn_make_buffer_rsrc((void*)g.in_im, (short)0, bytes, 0x00020000);
+                const uint32_t stride = 4u * (uint32_t)g.inner * (uint32_t)g.Rp;   // bytes per row n
+                const uint32_t base = 4u * (uint32_t)(((long long)a * g.N + site0) * g.Rp + r) + (uint32_t)kq * stride;
+#pragma unroll
+                for (int v = 0; v < 8; ++v) {
+                    const bool in = k0 + kq + v < g.L;
+                    const uint32_t off = in ? base + (uint32_t)(k0 + v) * stride : 0x80000000u;
+                    xr[v] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rr, (int)off, 0, 0));
+                    xi[v] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ri, (int)off, 0, 0));
+                }
+            }
+        };
+        auto split = [&](const float (&x)[8], bf16x8& hi, bf16x8& lo) {
+            unsigned short h[8], l[8];
+#pragma unroll
+            for (int v = 0; v < 8; ++v) {
+                h[v] = bf16_rne(x[v]);
+                l[v] = bf16_rne(x[v] - bf16_to_f(h[v]));
+            }
+            __builtin_memcpy(&hi, h, 16);
+            __builtin_memcpy(&lo, l, 16);
+        };
+        float xr[8], xi[8];
+        float4_t nrm[MB];
+#pragma unroll
+        for (int p = 0; p < MB; ++p) nrm[p] = (float4_t){0.f, 0.f, 0.f, 0.f};
+        fetch(a_begin, 0, xr, xi);
+        for (int a = a_begin; a < a_end; ++a) {
+            float4_t ur[MB], ui[MB];
+#pragma unroll
+            for (int p = 0; p < MB; ++p) {
+                ur[p] = (float4_t){0.f, 0.f, 0.f, 0.f};
+                ui[p] = (float4_t){0.f, 0.f, 0.f, 0.f};
+            }
+            for (int ks = 0; ks < nsteps; ++ks) {
+                bf16x8 rh, rl, ih, il;
+                split(xr, rh, rl);
+                if constexpr (!FIRST) split(xi, ih, il);
+                // the next step's input (or the next component's first) while
+                // this one multiplies (two steps ahead took more registers than
+                // it saved in waiting)
+                if (ks + 1 < nsteps) fetch(a, (ks + 1) * BK, xr, xi);
+                else if (a + 1 < a_end) fetch(a + 1, 0, xr, xi);
+                const int kf = ks * BK + kq;
+#pragma unroll
+                for (int p = 0; p < MB; ++p) {
+                    const unsigned short* row = W + (size_t)(p * 16 + (lane & 15)) * KP + kf;
+                    const bf16x8 ch = frag(row), cl = frag(row + (size_t)ROWS * KP),
+                                 sh = frag(row + (size_t)2 * ROWS * KP), sl = frag(row + (size_t)3 * ROWS * KP);
+                    ur[p] = mfma3(ch, cl, rh, rl, ur[p]);
+                    ui[p] = mfma3(sh, sl, rh, rl, ui[p]);
+                    if constexpr (!FIRST) {
+                        ur[p] = mfma3(neg8(sh), neg8(sl), ih, il, ur[p]);
+                        ui[p] = mfma3(ch, cl, ih, il, ui[p]);
+                    }
+                }
+            }
+            // result fragment: lane l holds D[4*(l/16) + v][l%16]
+#pragma unroll
+            for (int p = 0; p < MB; ++p)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int m = p * 16 + 4 * (lane >> 4) + v;
+                    if constexpr (LAST) {
+                        nrm[p][v] += fmaf(ui[p][v], ui[p][v], ur[p][v] * ur[p][v]);
+                    } else if (m < g.M) {
+                        const long long off = ((long long)a * g.N + site0 + (long long)g.inner * m) * g.Rp + r;
+                        g.out_re[off] = ur[p][v];
+                        g.out_im[off] = ui[p][v];
+                    }
+                }
+        }
+        if constexpr (LAST)
+#pragma unroll
+            for (int p = 0; p < MB; ++p)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int m = p * 16 + 4 * (lane >> 4) + v;
+                    if (m < g.M) g.norm[(site0 + (long long)g.inner * m) * g.Rp + r] = nrm[p][v];
+                }
+    }
+}
+
 // several sites per cell: norm[c][r] = sum_a |sum_l U_a(l + nb*c)|^2 over the
 // cells c of the grid (the axis stages left U per sublattice)
 __global__ __launch_bounds__(256) void k_sq_basis_norm(const float* re, const float* im, int N, int nb, int Rp,
@@ -431,6 +595,9 @@ struct SqState {
     float* d_z[4] = {nullptr, nullptr, nullptr, nullptr};  // U re/im planes, ping/pong
     unsigned short* d_w[3] = {nullptr, nullptr, nullptr};   // per-axis split C/S matrices (one set per offset)
     int* d_wsel[3] = {nullptr, nullptr, nullptr};           // per-axis W set of each sublattice
+    int nsets[3] = {0, 0, 0};                               // per-axis number of W sets
+    int n_cu = 0;                                           // compute units of the device
+    size_t lds_max = 0;                                     // LDS a workgroup may use
     float* d_norm = nullptr;
     int* d_map = nullptr;
     float* d_out = nullptr;
@@ -486,6 +653,50 @@ static SqState* sq_state(ether_ctx* c) {
         ctx_sq(c) = s;
     }
     return s;
+}
+
+// The W-resident axis kernel for an axis of L <= 96 cells: MB = rows / 16
+// with the rows L rounded up to 16 (one block) or 32 (longer axes would need
+// more registers than two waves per SIMD leave); LDS for every W set of the
+// axis; 0 if it does not apply.
+static int axis_mb(int L) {
+    if (L <= 16) return 1;
+    if (L > 96) return 0;
+    return (L + 31) / 32 * 2;
+}
+static size_t axis_lds(int mb, int Kp, int nsets) { return (size_t)nsets * 4 * mb * 16 * (Kp + 8) * 2; }
+
+// Launches one axis stage: as many workgroups as can be resident at once
+// (LDS and registers, from the occupancy query), at most one per four items.
+template <int MB, bool FIRST, bool LAST>
+static hipError_t launch_axis_t(const DftArgs& g, long long items, int nslab, int n_cu, size_t lds, hipStream_t st) {
+    const void* fn = (const void*)k_dft_axis<MB, FIRST, LAST>;
+    // above the default dynamic LDS limit of a launch the kernel must ask for it
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    int per_cu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, lds);
+    if (e != hipSuccess) return e;
+    const long long grid = std::max<long long>(1, std::min<long long>((items + 3) / 4, (long long)std::max(1, per_cu) * n_cu));
+    hipLaunchKernelGGL((k_dft_axis<MB, FIRST, LAST>), dim3((unsigned)grid), dim3(256), lds, st, g, (int)items, nslab);
+    return hipGetLastError();
+}
+template <int MB>
+static hipError_t launch_axis_mb(const DftArgs& g, bool first, bool last, long long items, int nslab, int n_cu,
+                                 size_t lds, hipStream_t st) {
+    if (first && last) return launch_axis_t<MB, true, true>(g, items, nslab, n_cu, lds, st);
+    if (first) return launch_axis_t<MB, true, false>(g, items, nslab, n_cu, lds, st);
+    if (last) return launch_axis_t<MB, false, true>(g, items, nslab, n_cu, lds, st);
+    return launch_axis_t<MB, false, false>(g, items, nslab, n_cu, lds, st);
+}
+static hipError_t launch_axis(const DftArgs& g, int mb, bool first, bool last, long long items, int nslab, int n_cu,
+                              size_t lds, hipStream_t st) {
+    switch (mb) {
+        case 1: return launch_axis_mb<1>(g, first, last, items, nslab, n_cu, lds, st);
+        case 2: return launch_axis_mb<2>(g, first, last, items, nslab, n_cu, lds, st);
+        case 4: return launch_axis_mb<4>(g, first, last, items, nslab, n_cu, lds, st);
+        default: return launch_axis_mb<6>(g, first, last, items, nslab, n_cu, lds, st);
+    }
 }
 
 static void launch_stage(const DftArgs& g, bool first, bool last, long long blocks, hipStream_t st) {
@@ -564,10 +775,17 @@ static int sq_grid(ether_ctx* c, const int32_t* sc, int nb, const double* frac, 
             HC(c, hipMalloc(&s->d_w[k], sizeof(unsigned short) * w.size()));
             HC(c, hipMemcpyAsync(s->d_w[k], w.data(), sizeof(unsigned short) * w.size(), hipMemcpyHostToDevice, st));
             host_sel[k] = sel;
+            s->nsets[k] = (int)offs.size();
             HC(c, hipMalloc(&s->d_wsel[k], sizeof(int) * nb));
             HC(c, hipMemcpyAsync(s->d_wsel[k], host_sel[k].data(), sizeof(int) * nb, hipMemcpyHostToDevice, st));
         }
         HC(c, hipMemcpyAsync(s->d_map, ctx_slot_of_site(c), sizeof(int) * N, hipMemcpyHostToDevice, st));
+        int dev = 0, ncu = 0, lds = 0;
+        HC(c, hipGetDevice(&dev));
+        HC(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        HC(c, hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        s->n_cu = ncu;
+        s->lds_max = (size_t)lds;
         HC(c, hipStreamSynchronize(st));   // the host staging vectors go out of scope
     }
     const int ev = ctx_ev_begin(c, ETHER_TIME_SQ);
@@ -592,6 +810,7 @@ static int sq_grid(ether_ctx* c, const int32_t* sc, int nb, const double* frac, 
         g.R = R;
         g.Rp = Rp;
         g.nb = nb;
+        g.nsets = s->nsets[k];
         const bool first = st_i == 0;
         // with several sites per cell the sublattices are summed after the
         // last axis, so every axis stage stores U
@@ -612,12 +831,24 @@ static int sq_grid(ether_ctx* c, const int32_t* sc, int nb, const double* frac, 
         } else {
             g.norm = s->d_norm;
         }
-        g.tiles_m = (g.M + BM - 1) / BM;
-        g.tiles_n = (Rp + BN - 1) / BN;
-        const long long blocks =
-            (long long)g.tiles_m * g.tiles_n * g.inner * g.outer * (last ? 1 : ncomp);
-        if (blocks > 0x7fffffffLL) return ctx_fail(c, ETHER_ERR_LIMIT, "ether_structure_factor: grid too large");
-        launch_stage(g, first, last, blocks, st);
+        const int mb = ctx_dft_resident(c) ? axis_mb(g.L) : 0;
+        const size_t lds = mb ? axis_lds(mb, g.Kp, g.nsets) : 0;
+        // its buffer loads address a plane in 32-bit byte offsets
+        const bool fits = (size_t)ncomp * N * Rp * 4 < ((size_t)1 << 31);
+        if (mb && lds <= s->lds_max && fits) {
+            // W resident in LDS: one wave per (line, component unless last, 16 replicas)
+            const int nslab = Rp / 16;
+            const long long items = (long long)g.inner * g.outer * (last ? 1 : ncomp) * nslab;
+            if (items > 0x7fffffffLL) return ctx_fail(c, ETHER_ERR_LIMIT, "ether_structure_factor: grid too large");
+            HC(c, launch_axis(g, mb, first, last, items, nslab, s->n_cu, lds, st));
+        } else {
+            g.tiles_m = (g.M + BM - 1) / BM;
+            g.tiles_n = (Rp + BN - 1) / BN;
+            const long long blocks =
+                (long long)g.tiles_m * g.tiles_n * g.inner * g.outer * (last ? 1 : ncomp);
+            if (blocks > 0x7fffffffLL) return ctx_fail(c, ETHER_ERR_LIMIT, "ether_structure_factor: grid too large");
+            launch_stage(g, first, last, blocks, st);
+        }
         HC(c, hipGetLastError());
         cur = nxt;
     }

@@ -84,6 +84,7 @@ EXPORTS = [
                             C.c_int32, _vp]),
     ("ether_set_graphs", C.c_int, [_vp, C.c_int32]),
     ("ether_set_resident", C.c_int, [_vp, C.c_int32]),
+    ("ether_set_dft_resident", C.c_int, [_vp, C.c_int32]),
     ("ether_run_prepare", C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                     C.c_double, C.c_int32]),
     ("ether_set_launch", C.c_int, [_vp, C.c_int32, C.c_int32]),
@@ -417,6 +418,11 @@ class Engine:
     def set_resident(self, on: bool) -> None:
         """Ising sweeps with the lattice resident in LDS (default on where it fits)."""
         self._c(self.lib.ether_set_resident(self.h, 1 if on else 0))
+
+    def set_dft_resident(self, on: bool) -> None:
+        """S(q) grid axes of <= 128 cells with the DFT matrices resident in LDS
+        (default on); off forces the tiled kernel (bit-identical results)."""
+        self._c(self.lib.ether_set_dft_resident(self.h, 1 if on else 0))
 
     def set_launch(self, sites_per_wave: int, xcd_map: int) -> None:
         self._c(self.lib.ether_set_launch(self.h, int(sites_per_wave), int(xcd_map)))

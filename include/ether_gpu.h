@@ -322,6 +322,16 @@ int ether_set_graphs(ether_ctx* ctx, int32_t on);
  * forces the phase kernels (default on). */
 int ether_set_resident(ether_ctx* ctx, int32_t on);
 
+/* The grid forms of the structure factor (ether_structure_factor and
+ * ether_structure_factor_basis) take every lattice axis of at most 128 cells
+ * with a kernel that keeps the axis's DFT matrices resident in LDS and reads
+ * each input row once per axis; longer axes take the tiled kernel, which
+ * stages both operands per K step.  The two make the same products in the
+ * same order, so S(q) is bit-identical either way.  on = 0 forces the tiled
+ * kernel on every axis (default on).  A tuning and testing switch of this
+ * library; no routine of the reference corresponds to it. */
+int ether_set_dft_resident(ether_ctx* ctx, int32_t on);
+
 /* Captures, instantiates and uploads every graph that ether_run with the same
  * arguments would replay, and runs nothing, so that the ether_run that
  * follows spends no time on capture or instantiation (a timed production

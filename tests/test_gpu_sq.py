@@ -147,3 +147,46 @@ def test_structure_factor_line_matches_direct_sum(name, over, q, R):
             assert not out[r, :, 1:3].any()
     say(f"largest phase {phase_max:.0f} rad")
     eng.close()
+
+
+@pytest.mark.parametrize("name,over,R,basis", [
+    ("tri_afm_96", {}, 70, False),                                 # 96 x 96: the benchmark's axis length
+    ("sc_heis_afm_32", dict(sc="12 6 1"), 130, False),
+    ("sc_heis_afm_32", dict(sc="4 4 4"), 3, False),                # three axes of one block of rows
+    ("sc_heis_afm_32", dict(sc="40 1 1"), 5, False),               # one axis: first and last stage at once
+    ("sc_heis_afm_32", dict(sc="100 3 1"), 3, False),              # an axis above 96 keeps the tiled kernel
+    ("sc_ising_16", dict(sc="8 8 8"), 9, False),                   # the int8 Ising plane
+    ("pyro_pt_16", dict(sc="3 2 2", pt=".F. 11"), 6, True),        # 16 sites per cell, several W sets
+])
+def test_structure_factor_axis_kernels_bit_identical(name, over, R, basis):
+    """The grid form with the axis's DFT matrices resident in LDS (one wave
+    per line and 16 replicas) against the tiled kernel that stages both
+    operands per K step (ether_set_dft_resident(ctx, 0)): the same split-BF16
+    products in the same order per element, so S(q) must be bit-identical
+    (the tiled kernel is itself checked against the FP64 direct sum and FFT
+    above)."""
+    say(f"sq axis kernels {name} {over} R={R}")
+    m, eng, _ = pair(name, R, **over)
+    eng.set_betas(m.betas([5.0] * R))
+    rng = np.random.default_rng(31)
+    for r in range(R):
+        if m.tags.ising:
+            S = np.zeros((m.n_sites, 3))
+            S[:, 2] = np.where(rng.random(m.n_sites) < 0.3 + 0.4 * (r % 2), 1.0, -1.0)
+        else:
+            S = random_xyz(rng, m.n_sites).astype(np.float32).astype(np.float64)
+        eng.set_spins(r, S, np.arctan2(S[:, 1], S[:, 0]))
+    if basis:
+        frac = m.basis_xyz @ np.linalg.inv(m.structure.abc)
+        run = lambda: eng.structure_factor_basis(m.sc, frac)   # noqa: E731
+    else:
+        run = lambda: eng.structure_factor(m.sc)               # noqa: E731
+    resident = run()
+    eng.set_dft_resident(False)
+    tiled = run()
+    eng.set_dft_resident(True)
+    again = run()
+    assert resident.shape == tiled.shape
+    assert np.array_equal(resident, tiled), float(np.abs(resident - tiled).max())
+    assert np.array_equal(resident, again)
+    eng.close()

@@ -105,6 +105,37 @@ def profiled_traffic(kernel, workload, key):
     return None
 
 
+def profiled_value(kernel, workload, suffix, key):
+    """The line `<key> = value` of profiles/<kernel>_<workload>_<suffix>.md, or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_{suffix}.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith(key + " ="):
+                return float(line.split("=")[-1])
+    return None
+
+
+def valu_roofline(kernel, workload, spw, xcd, resident, avg_ms):
+    """VALU utilisation next to the HBM fraction: the committed PMC summary's
+    VALU issue cycles per SIMD (SQ_ACTIVE_INST_VALU x 4 / SIMDs) over the
+    launch time this run measured, at 2.4 GHz."""
+    if resident:
+        cyc = profiled_value(kernel, workload, "valu", "valu_cycles_per_simd_per_sweep")
+        simds = profiled_value(kernel, workload, "valu", "valu_simds_used")
+    else:
+        cyc = profiled_value(kernel, workload, "valu", f"valu_cycles_per_simd[spw={spw},xcd={xcd}]")
+        simds = 1024.0
+    if cyc is None or not avg_ms:
+        return None
+    floor_ms = cyc / 2.4e6
+    return {"issue_floor_ms": floor_ms, "busy": floor_ms / avg_ms, "simds": simds,
+            "basis": "VALU issue cycles per SIMD from the committed PMC summary "
+                     f"(profiles/{kernel}_{workload}_valu.md) over this run's launch time"
+                     + (" (per sweep, on the SIMDs of the CUs the launch occupies)" if resident else "")}
+
+
 def copy_probe(n_bytes, reps):
     """A plain device-to-device copy of known size (the HIP runtime's copy
     kernel), used to calibrate the FETCH_SIZE / WRITE_SIZE counters."""
@@ -326,6 +357,7 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args, over=None, l
             "traffic": profiled_traffic(kernel, workload, "traffic_bytes_per_sweep" if res_n > 0 else
                                         f"traffic_bytes_per_launch[spw={spw},xcd={xcd}]"),
             "kernel": f"{kernel} ({unit})",
+            "valu": valu_roofline(kernel, workload, spw, xcd, res_n > 0, avg_ms),
             "bytes_per_update": b_upd, "updates_per_launch": per_launch_updates,
             "avg_launch_ms": avg_ms, "launches": sweep_n,
             "timing": (f"HIP events on the engine stream around the back-to-back colour phases of "

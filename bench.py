@@ -319,8 +319,9 @@ def measure(workload, steps, warmup, event_steps, split_ovrr, args, over=None, l
     nc, col, _ = eng.layout()
     if res_n > 0:
         # Ising with the lattice resident in LDS: one launch runs the sweeps
-        # between two samples; the unit is one sweep of every replica
-        kernel = "k_ising_resident"
+        # between two samples; the unit is one sweep of every replica (a
+        # workgroup per replica, or per word of four replicas)
+        kernel = "k_ising_resident1" if eng.resident_kind() == 3 else "k_ising_resident"
         unit = "one sweep (all colour phases) inside a resident launch"
         per_launch_updates = R * m.n_sites
         avg_ms = res_ms / res_n

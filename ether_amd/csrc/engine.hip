@@ -76,6 +76,11 @@ struct ether_ctx {
     bool res_ok = false;
     int res_block = 256, res_thr_lds = 0;
     size_t res_lds = 0;
+    // one workgroup per replica instead of per word (k_ising_resident1):
+    // chosen when it takes fewer rounds of workgroups over the compute units
+    bool res_rep = false;
+    int res1_thr_lds = 0;
+    size_t res1_lds = 0;
     // spiral-state partners (ether_set_spiral): site-space rows, -1 padded
     int spiral_cap = 0;
     double spiral_count = 0.0;
@@ -896,6 +901,18 @@ extern "C" int ether_commit(ether_ctx* c) {
         for (int q = 0; q < c->ncol; ++q) maxcol = std::max(maxcol, c->cbeg[q + 1] - c->cbeg[q]);
         // about two sites of the largest colour per thread, 64 to 1024 threads
         c->res_block = std::max(64, std::min(1024, (maxcol / 2 + 63) / 64 * 64));
+        // per replica: N int8 spins (+ one threshold table); a workgroup does
+        // about half the instructions of a word's (the Philox block is most of
+        // a site's work either way), so it wins while the replicas take fewer
+        // than twice the rounds of workgroups the words take
+        const size_t n16 = ((size_t)N + 15) / 16 * 16;
+        c->res1_thr_lds = (n16 + (size_t)4 * c->ising_table <= kResidentLds) ? 1 : 0;
+        c->res1_lds = n16 + (c->res1_thr_lds ? (size_t)4 * c->ising_table : 0);
+        int ncu = 0;
+        HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+        ncu = std::max(ncu, 1);
+        const long long rounds1 = ((long long)R + ncu - 1) / ncu, rounds4 = ((long long)c->Rp / 4 + ncu - 1) / ncu;
+        c->res_rep = rounds1 < 2 * rounds4;
     }
     // four zero entries past the last row: a 4-wide chunk of the last row stays in the table
     ell.resize(ell.size() + 4, 0u);
@@ -1865,14 +1882,15 @@ static int enqueue_resident(ether_ctx* c, int n_sweeps, bool meas) {
     a.offset = c->ising_offset;
     a.table_per_slot = c->ising_table;
     a.P = c->ising_P;
-    a.table_in_lds = c->res_thr_lds;
+    const bool per_rep = c->resident == 3 || (c->resident == 1 && c->res_rep);
+    a.table_in_lds = per_rep ? c->res1_thr_lds : c->res_thr_lds;
     a.thr = c->d_thr;
     a.meas = meas ? 1 : 0;
     a.nsp = c->nsp;
     a.rec = c->meas_rec;
     a.partial = c->d_partial;
     const int ev = ev_begin(c, ETHER_TIME_RESIDENT, n_sweeps);
-    launch_ising_resident(a, c->res_block, c->res_lds, c->stream);
+    launch_ising_resident(a, c->res_block, per_rep ? c->res1_lds : c->res_lds, per_rep, c->stream);
     ev_end(c, ev);
     HIPCHK(c, hipGetLastError());
     c->step += (uint32_t)n_sweeps;
@@ -2069,8 +2087,18 @@ extern "C" int ether_run_prepare(ether_ctx* c, int32_t first_step, int32_t n_ste
 
 extern "C" int ether_set_resident(ether_ctx* c, int32_t on) {
     if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
+    if (on < 0 || on > 3) return fail(c, ETHER_ERR_ARG, "ether_set_resident: mode must be 0, 1, 2 or 3");
     drop_graphs(c);   // captured graphs hold the launches of the path they were captured on
-    c->resident = on ? 1 : 0;
+    c->resident = on;
+    return ETHER_OK;
+}
+
+extern "C" int ether_get_resident(ether_ctx* c, int32_t* kind) {
+    if (!c || !kind) return fail(c, ETHER_ERR_ARG, "ether_get_resident: null argument");
+    if (!c->committed) return fail(c, ETHER_ERR_STATE, "context not committed");
+    if (!c->resident || !c->res_ok) *kind = 0;
+    else if (c->resident == 2 || c->resident == 3) *kind = c->resident;
+    else *kind = c->res_rep ? 3 : 2;
     return ETHER_OK;
 }
 

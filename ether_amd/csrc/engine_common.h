@@ -348,7 +348,7 @@ void launch_ctr_batch(uint32_t* const* d_ctrs, int n, const uint32_t* d_vals, ui
 void launch_xyz_sweep(const XyzArgs& a, bool angle, bool zee, bool sia, hipStream_t st);
 void launch_xyz_ovrr(const XyzArgs& a, bool angle, bool zee, hipStream_t st);
 void launch_ising_sweep(const IsingArgs& a, hipStream_t st);
-void launch_ising_resident(const IsingResArgs& a, int block, size_t lds_bytes, hipStream_t st);
+void launch_ising_resident(const IsingResArgs& a, int block, size_t lds_bytes, bool per_replica, hipStream_t st);
 void launch_measure(const MeasArgs& a, int model, bool zee, bool sia, hipStream_t st);
 void launch_finalize(const FinArgs& a, hipStream_t st);
 void launch_pt_swap(const PtArgs& a, hipStream_t st);

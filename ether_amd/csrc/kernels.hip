@@ -1081,6 +1081,207 @@ __global__ void __launch_bounds__(1024) k_ising_resident(IsingResArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// The resident Ising sweeps with one workgroup per replica instead of per
+// word: for fewer replicas than compute units a word's workgroup keeps one CU
+// busy with four replicas while most CUs idle (C1: 48 of 256), so each
+// replica gets a workgroup of its own, its int8 spin of every site and its
+// threshold table in LDS.  Every update is the word kernel's for byte
+// b = r & 3 of word r >> 2: the same Philox block (slot, word, sweep), its
+// word b, the same down counts and integer threshold; the four workgroups of
+// a word each draw the block (the generator is a pure function of the
+// counter), so the spins are bit-identical.  The record of a measurement
+// (bond sums per class over full rows, spin sums per species, M_z) is the
+// word kernel's for this replica, summed in the same thread order.
+// ---------------------------------------------------------------------------
+template <int DM, int NC, bool TL>
+__global__ void __launch_bounds__(1024) k_ising_resident1(IsingResArgs a) {
+    constexpr int KC = NC > 0 ? NC : kMaxIsingClasses;
+    extern __shared__ __attribute__((aligned(16))) uint32_t sRes1[];
+    int8_t* sS = reinterpret_cast<int8_t*>(sRes1);                 // [N] spins
+    uint32_t* sThr = sRes1 + (a.N + 15) / 16 * 4;                   // [table_per_slot] when staged
+    __shared__ unsigned sAcc1;
+    __shared__ int sCnt1[2 * kMaxIsingClasses];
+    __shared__ double sMz1[16];
+    __shared__ int sCb[kMaxResColours + 1];
+    const int r = blockIdx.x, rid = r >> 2, b = r & 3;
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int tbase = TL ? 0 : a.slot_of[r] * a.table_per_slot;
+    for (int q = t; q < a.N; q += nt) sS[q] = a.s[(size_t)q * a.Rp + r];
+    if (TL) {
+        const uint32_t* src = a.thr + (size_t)a.slot_of[r] * a.table_per_slot;
+        for (int q = t; q < a.table_per_slot; q += nt) sThr[q] = src[q];
+    }
+    if (t == 0) sAcc1 = 0u;
+    if (t <= kMaxResColours) sCb[t] = a.cbeg[t];
+    for (int q = t; q < 2 * kMaxIsingClasses; q += nt) sCnt1[q] = 0;
+    __syncthreads();
+    unsigned accepted = 0u;
+    const uint32_t step0 = a.ctr[0] + a.step_off;
+    // down counts and bonds per coupling class of one site's row (rows padded
+    // with class 0 up to DM)
+    auto gather = [&](int slot, int (&dn)[kMaxIsingClasses + 1], int (&cnt)[kMaxIsingClasses + 1]) {
+#pragma unroll
+        for (int kk = 0; kk <= kMaxIsingClasses; ++kk) {
+            dn[kk] = 0;
+            cnt[kk] = 0;
+        }
+        const uint32_t* nb = a.ell + (size_t)slot * a.D;
+        // NC = 0 (up to 8 classes): one byte per class in two 64-bit words
+        // (at most 255 neighbours per class, checked at commit)
+        uint64_t cp = 0, dp = 0;
+        auto add = [&](int j) {
+            const uint32_t e = nb[j];
+            const int cl = (int)(e >> 24);
+            const int down = sS[e & 0xffffffu] < 0 ? 1 : 0;
+            if constexpr (NC == 1) {
+                const int on = cl == 1;
+                dn[1] += down & on;
+                cnt[1] += on;
+            } else if constexpr (NC == 2) {
+#pragma unroll
+                for (int kk = 1; kk <= 2; ++kk) {
+                    const int on = kk == cl;
+                    dn[kk] += down & on;
+                    cnt[kk] += on;
+                }
+            } else {
+                const uint64_t on = cl != 0 ? 1u : 0u;     // padding entries are class 0
+                const int sh = (8 * (cl - 1)) & 63;
+                cp += on << sh;
+                dp += (on & (uint64_t)down) << sh;
+            }
+        };
+        if constexpr (DM > 0) {
+#pragma unroll
+            for (int j = 0; j < DM; ++j) add(j);
+        } else {
+            for (int j = 0; j < a.D; ++j) add(j);
+        }
+        if constexpr (NC == 0) {
+#pragma unroll
+            for (int kk = 1; kk <= kMaxIsingClasses; ++kk) {
+                cnt[kk] = (int)((cp >> (8 * (kk - 1))) & 0xffu);
+                dn[kk] = (int)((dp >> (8 * (kk - 1))) & 0xffu);
+            }
+        }
+    };
+    // threshold lookup and test of one site (r < R always: the grid is R)
+    auto decide = [&](int slot, const int (&dn)[kMaxIsingClasses + 1], const int (&cnt)[kMaxIsingClasses + 1],
+                      int v, u32x4 wr) {
+        int idx = a.offset;
+#pragma unroll
+        for (int kk = 1; kk <= KC; ++kk) {
+            if (NC == 0 && kk > a.n_classes) break;
+            idx += a.stride[kk] * (cnt[kk] - 2 * dn[kk]);
+        }
+        const int ti = tbase + (a.sp[slot] * 2 + (v > 0)) * a.P + idx;
+        const uint32_t th = TL ? sThr[ti] : a.thr[ti];
+        const uint32_t w = b == 0 ? wr.x : (b == 1 ? wr.y : (b == 2 ? wr.z : wr.w));
+        return w < th || th == 0xffffffffu;
+    };
+    for (int k = 0; k < a.n_sweeps; ++k) {
+        const uint32_t step = step0 + (uint32_t)k;
+        for (int c = 0; c < a.ncol; ++c) {
+            const int c_end = sCb[c + 1];
+            // two sites of the class per thread in flight, as in the word kernel
+            for (int sa = sCb[c] + t; sa < c_end; sa += 2 * nt) {
+                const bool hb = sa + nt < c_end;
+                const int sb = hb ? sa + nt : sa;
+                int da[kMaxIsingClasses + 1], db[kMaxIsingClasses + 1];
+                int ca[kMaxIsingClasses + 1], cb[kMaxIsingClasses + 1];
+                gather(sa, da, ca);
+                gather(sb, db, cb);
+                const int va = sS[sa], vb = sS[sb];
+                const u32x4 wa = rng_draw(a.seed, (uint32_t)sa, (uint32_t)rid, step, RNG_METROPOLIS, 0);
+                const u32x4 wb = rng_draw(a.seed, (uint32_t)sb, (uint32_t)rid, step, RNG_METROPOLIS, 0);
+                const bool fa = decide(sa, da, ca, va, wa);
+                const bool fb = hb && decide(sb, db, cb, vb, wb);
+                accepted += (unsigned)fa + (unsigned)fb;
+                if (fa) sS[sa] = (int8_t)(-va);
+                if (fb) sS[sb] = (int8_t)(-vb);
+            }
+            __syncthreads();
+        }
+    }
+    for (int q = t; q < a.N; q += nt) a.s[(size_t)q * a.Rp + r] = sS[q];
+    if (accepted) atomicAdd(&sAcc1, accepted);
+    if (a.meas) {
+        int bond[kMaxIsingClasses + 1], zs[kMaxIsingClasses];
+        double mz = 0.0;
+#pragma unroll
+        for (int kk = 0; kk <= kMaxIsingClasses; ++kk) bond[kk] = 0;
+#pragma unroll
+        for (int kk = 0; kk < kMaxIsingClasses; ++kk) zs[kk] = 0;
+        for (int slot = t; slot < a.N; slot += nt) {
+            const uint32_t* nb = a.ell + (size_t)slot * a.D;
+            const int v = sS[slot];
+            int diff[kMaxIsingClasses + 1], cnt[kMaxIsingClasses + 1];
+#pragma unroll
+            for (int kk = 0; kk <= kMaxIsingClasses; ++kk) {
+                diff[kk] = 0;
+                cnt[kk] = 0;
+            }
+            uint64_t cp = 0, dp = 0;                   // NC = 0: a byte per class, as in gather
+            for (int j = 0; j < a.D; ++j) {
+                const uint32_t e = nb[j];
+                const int cl = (int)(e >> 24);
+                const int d = sS[e & 0xffffffu] != v ? 1 : 0;
+                if constexpr (NC > 0) {
+#pragma unroll
+                    for (int kk = 1; kk <= KC; ++kk) {
+                        const int on = kk == cl;
+                        diff[kk] += d & on;
+                        cnt[kk] += on;
+                    }
+                } else {
+                    const uint64_t on = cl != 0 ? 1u : 0u;
+                    const int sh = (8 * (cl - 1)) & 63;
+                    cp += on << sh;
+                    dp += (on & (uint64_t)d) << sh;
+                }
+            }
+            if constexpr (NC == 0) {
+#pragma unroll
+                for (int kk = 1; kk <= kMaxIsingClasses; ++kk) {
+                    cnt[kk] = (int)((cp >> (8 * (kk - 1))) & 0xffu);
+                    diff[kk] = (int)((dp >> (8 * (kk - 1))) & 0xffu);
+                }
+            }
+            const int spc = a.sp[slot];
+#pragma unroll
+            for (int kk = 1; kk <= KC; ++kk) bond[kk] += cnt[kk] - 2 * diff[kk];
+#pragma unroll
+            for (int kk = 0; kk < kMaxIsingClasses; ++kk) zs[kk] += v * (int)(kk == spc);
+            mz += (double)(v > 0 ? a.w[slot] : -a.w[slot]);
+        }
+#pragma unroll
+        for (int kk = 1; kk <= KC; ++kk)
+            if (kk <= a.n_classes && bond[kk]) atomicAdd(&sCnt1[kk - 1], bond[kk]);
+#pragma unroll
+        for (int kk = 0; kk < kMaxIsingClasses; ++kk)
+            if (kk < a.nsp && zs[kk]) atomicAdd(&sCnt1[kMaxIsingClasses + kk], zs[kk]);
+        // M_z: the word kernel's fixed-order wave reduction, then the waves in order
+        double m = mz;
+        for (int off = 32; off > 0; off >>= 1) m += __shfl_xor(m, off);
+        if ((t & 63) == 0) sMz1[t >> 6] = m;
+    }
+    __syncthreads();
+    if (t == 0) {
+        if (sAcc1)
+            atomicAdd(a.acc + (size_t)(blockIdx.x % kAccCopies) * a.R + a.slot_of[r], (unsigned long long)sAcc1);
+        if (a.meas) {
+            double* out = a.partial + (size_t)r * a.rec;    // tile 0
+            int o = 0;
+            for (int kk = 0; kk < a.n_classes; ++kk) out[o++] = (double)sCnt1[kk];
+            for (int kk = 0; kk < a.nsp; ++kk) out[o++] = (double)sCnt1[kMaxIsingClasses + kk];
+            double mm = 0.0;
+            for (int wv = 0; wv < (nt + 63) / 64; ++wv) mm += sMz1[wv];
+            out[o] = mm;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Several sites per wavefront, for contexts with few replicas: lanes per site
 // LPS = 1 << a.lps_shift (XYZ: 16 or 32 replicas per site; Ising: 8, 16 or 32
 // lanes of 4 replicas), so a wave updates 64 / LPS sites per step instead of
@@ -2224,8 +2425,32 @@ static void launch_ising_resident_tl(const IsingResArgs& a, dim3 grid, dim3 blk,
     }
 }
 
-void launch_ising_resident(const IsingResArgs& a, int block, size_t lds_bytes, hipStream_t st) {
-    const dim3 grid((unsigned)(a.Rp / 4)), blk((unsigned)block);
+template <bool TL>
+static void launch_ising_resident1_tl(const IsingResArgs& a, dim3 grid, dim3 blk, size_t lds_bytes, hipStream_t st) {
+    if (a.n_classes == 1) {
+#define LR(DM) hipLaunchKernelGGL((k_ising_resident1<DM, 1, TL>), grid, blk, lds_bytes, st, a)
+        ETHER_DISPATCH_DM(a.D, LR)
+#undef LR
+    } else if (a.n_classes == 2) {
+#define LR(DM) hipLaunchKernelGGL((k_ising_resident1<DM, 2, TL>), grid, blk, lds_bytes, st, a)
+        ETHER_DISPATCH_DM(a.D, LR)
+#undef LR
+    } else {
+#define LR(DM) hipLaunchKernelGGL((k_ising_resident1<DM, 0, TL>), grid, blk, lds_bytes, st, a)
+        ETHER_DISPATCH_DM(a.D, LR)
+#undef LR
+    }
+}
+
+void launch_ising_resident(const IsingResArgs& a, int block, size_t lds_bytes, bool per_replica, hipStream_t st) {
+    const dim3 blk((unsigned)block);
+    if (per_replica) {
+        const dim3 grid((unsigned)a.R);
+        if (a.table_in_lds) launch_ising_resident1_tl<true>(a, grid, blk, lds_bytes, st);
+        else launch_ising_resident1_tl<false>(a, grid, blk, lds_bytes, st);
+        return;
+    }
+    const dim3 grid((unsigned)(a.Rp / 4));
     if (a.table_in_lds) launch_ising_resident_tl<true>(a, grid, blk, lds_bytes, st);
     else launch_ising_resident_tl<false>(a, grid, blk, lds_bytes, st);
 }

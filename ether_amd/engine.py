@@ -85,6 +85,7 @@ EXPORTS = [
     ("ether_set_graphs", C.c_int, [_vp, C.c_int32]),
     ("ether_set_resident", C.c_int, [_vp, C.c_int32]),
     ("ether_set_dft_resident", C.c_int, [_vp, C.c_int32]),
+    ("ether_get_resident", C.c_int, [_vp, C.POINTER(C.c_int32)]),
     ("ether_run_prepare", C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                     C.c_double, C.c_int32]),
     ("ether_set_launch", C.c_int, [_vp, C.c_int32, C.c_int32]),
@@ -415,9 +416,19 @@ class Engine:
     def set_graphs(self, on: bool) -> None:
         self._c(self.lib.ether_set_graphs(self.h, 1 if on else 0))
 
-    def set_resident(self, on: bool) -> None:
-        """Ising sweeps with the lattice resident in LDS (default on where it fits)."""
-        self._c(self.lib.ether_set_resident(self.h, 1 if on else 0))
+    def set_resident(self, on) -> None:
+        """Ising sweeps with the lattice resident in LDS (default on where it
+        fits): False/0 phase kernels, True/1 chosen, 2 a workgroup per word of
+        four replicas, 3 a workgroup per replica."""
+        mode = int(on) if not isinstance(on, bool) else (1 if on else 0)
+        self._c(self.lib.ether_set_resident(self.h, mode))
+
+    def resident_kind(self) -> int:
+        """The Ising sweep path now: 0 phase kernels, 2 a workgroup per word,
+        3 a workgroup per replica."""
+        k = C.c_int32(0)
+        self._c(self.lib.ether_get_resident(self.h, C.byref(k)))
+        return int(k.value)
 
     def set_dft_resident(self, on: bool) -> None:
         """S(q) grid axes of <= 128 cells with the DFT matrices resident in LDS

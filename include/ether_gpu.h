@@ -317,13 +317,26 @@ int ether_set_graphs(ether_ctx* ctx, int32_t on);
  * the measurement the next swap or sample reads.  Used by ether_sweep and
  * ether_run when the context is Ising, has at most 16 colours and
  * 4*N (+ 16 * its threshold table) bytes fit 60 KiB; batches keep the phase
- * kernels.  Results are bit-identical to launching the colour phases one by
- * one (the same Philox counters, counts and integer thresholds).  on = 0
- * forces the phase kernels (default on). */
+ * kernels.  While the replicas take fewer than twice the rounds of
+ * workgroups over the compute units that the words take (fewer replicas than
+ * compute units), each replica gets a workgroup of its own instead (its int8
+ * spins, N bytes, and its threshold table in LDS), which draws the word's
+ * Philox block and uses its own word of it.  Results are bit-identical to
+ * launching the colour phases one by one (the same Philox counters, counts
+ * and integer thresholds).  on: 0 forces the phase kernels, 1 chooses
+ * (default), 2 forces a workgroup per word, 3 a workgroup per replica;
+ * anything else is ETHER_ERR_ARG. */
 int ether_set_resident(ether_ctx* ctx, int32_t on);
 
+/* The Ising sweep path a committed context takes now (no counterpart in the
+ * reference): 0 the colour phases launched one by one (an XYZ context, a
+ * lattice that does not fit, or ether_set_resident(ctx, 0)), 2 the resident
+ * kernel with a workgroup per word of four replicas, 3 with a workgroup per
+ * replica.  For reports and profiles. */
+int ether_get_resident(ether_ctx* ctx, int32_t* kind);
+
 /* The grid forms of the structure factor (ether_structure_factor and
- * ether_structure_factor_basis) take every lattice axis of at most 128 cells
+ * ether_structure_factor_basis) take every lattice axis of at most 96 cells
  * with a kernel that keeps the axis's DFT matrices resident in LDS and reads
  * each input row once per axis; longer axes take the tiled kernel, which
  * stages both operands per K step.  The two make the same products in the

@@ -1,20 +1,41 @@
-"""Ising sweeps with the lattice resident in LDS (k_ising_resident) against
-the colour phases launched one by one (ether_set_resident(ctx, 0)).
+"""Ising sweeps with the lattice resident in LDS against the colour phases
+launched one by one (ether_set_resident(ctx, 0)): one workgroup per word of
+four replicas (k_ising_resident, mode 2) and one per replica
+(k_ising_resident1, mode 3).
 
-The resident kernel makes every update with the same Philox block, the same
-per-byte down counts and the same integer threshold as k_ising_sweep, so a
-whole schedule (sweeps, tempering swaps every few MCS, samples, graphs) must
-leave bit-identical spins, moments, swap counters and round trips; and
+Both resident kernels make every update with the same Philox block, the same
+down counts and the same integer threshold as k_ising_sweep, so a whole
+schedule (sweeps, tempering swaps every few MCS, samples, graphs) must leave
+bit-identical spins, moments, swap counters and round trips; and
 ether_sweep's per-sweep acceptance must be equal.  The oracle parity of the
-Ising sweep (tests/test_gpu_parity.py, ether_sweep) runs on this path too,
-since it is the default wherever a lattice fits.
+Ising sweep (tests/test_gpu_parity.py, ether_sweep) runs on the resident path
+too, since it is the default wherever a lattice fits.  Cases: one coupling
+class (8^3, and the 16^3 x 190 workload), two classes with Zeeman (pyrochlore
+two shells), three classes (simple cubic with three shells: the kernels'
+path for any number of classes).
 """
 import numpy as np
 import pytest
 
 from ether_amd import engine as E
-from ether_amd.driver import make_engine
+from ether_amd import workloads
+from ether_amd.driver import attach_neighbours, make_engine
+from ether_amd.inputs import build_model, parse_in_ether, parse_j_exchange, parse_structure
 from tests.helpers import pair, say
+
+
+def _three_shell_ising(L, R):
+    """Simple-cubic Ising with three coupling classes: J = -1, 0.5, 0.25 meV
+    on the 6, 12 and 8 neighbours at 1, sqrt 2, sqrt 3 (26 per site)."""
+    st = parse_structure(workloads.SC_STRUCTURE)
+    tags = parse_in_ether(f"model = ising\nmcs = 100 10\ntemp = 2 2 1\nspin = 1\nspecies = A\n"
+                          f"sc = {L} {L} {L}\nbc = c c c\nrepeat = 1\ncoa = 10\nseed = 1992\n"
+                          f"para = .T. 1\n", st.n_species)
+    ex = parse_j_exchange("3\n1.0 1\nA A -1 -1 -1\n1.414214 1\nA A 0.5 0.5 0.5\n"
+                          "1.732051 1\nA A 0.25 0.25 0.25\n")
+    m = build_model(tags, st, ex, None, None)
+    attach_neighbours(m)
+    return m, make_engine(m, R, seed=1992)
 
 pytestmark = pytest.mark.gpu
 
@@ -28,15 +49,21 @@ def _ising_start(m, engs, R, seed):
             e.set_spins(r, S)
 
 
+@pytest.mark.parametrize("mode", [2, 3])
 @pytest.mark.parametrize("name,over,R,ladder", [
     ("sc_ising_16", dict(sc="8 8 8"), 70, 10),
-    ("sc_ising_16", dict(), 190, 19),                       # the 16^3 workload, four replicas per word
+    ("sc_ising_16", dict(), 190, 19),                       # the 16^3 workload
     ("pyro_pt_16", dict(sc="2 2 2", model="ising", para=".T. 1", pt=".F. 11",
                         zeeman=".T. 0 0 0.3"), 130, 13),   # two shells (two classes), Zeeman, 4 colours
+    ("sc_3shell", dict(), 60, 12),                          # three classes, 26 neighbours, 8 colours
 ])
-def test_resident_schedule_equals_phase_kernels(name, over, R, ladder):
-    say(f"resident vs phase kernels {name} {over} R={R}")
-    m, res, _ = pair(name, R, **over)
+def test_resident_schedule_equals_phase_kernels(name, over, R, ladder, mode):
+    say(f"resident (mode {mode}) vs phase kernels {name} {over} R={R}")
+    if name == "sc_3shell":
+        m, res = _three_shell_ising(6, R)
+    else:
+        m, res, _ = pair(name, R, **over)
+    res.set_resident(mode)
     phase = make_engine(m, R, seed=1992)
     phase.set_resident(False)
     temps = np.tile(np.linspace(2.0, 6.0, ladder) if m.tags.para else np.linspace(10.0, 100.0, ladder),
@@ -70,6 +97,14 @@ def test_resident_schedule_equals_phase_kernels(name, over, R, ladder):
         assert np.array_equal(res.get_spins(r)[0], phase.get_spins(r)[0])
     res.close()
     phase.close()
+
+
+def test_resident_mode_is_checked():
+    R = 8
+    m, eng, _ = pair("sc_ising_16", R, sc="4 4 4")
+    with pytest.raises(E.EtherError):
+        eng.set_resident(4)
+    eng.close()
 
 
 def test_resident_launch_is_timed_as_its_own_kind():

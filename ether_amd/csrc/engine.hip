@@ -199,9 +199,13 @@ static int dev_alloc(ether_ctx* c, T** p, size_t n) {
     return ETHER_OK;
 }
 
+// Host to device, complete on return: the runtime may read a pageable source
+// after hipMemcpyAsync returns (when the copy reaches the head of the stream),
+// and the callers pass buffers that are freed as soon as they return.
 template <class T>
 static int upload(ether_ctx* c, T* d, const T* h, size_t n) {
     HIPCHK(c, hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return ETHER_OK;
 }
 
@@ -2127,6 +2131,11 @@ struct ether_batch {
     uint32_t* d_vals = nullptr;    // staging for counter resets
     char* d_arena = nullptr;       // argument blocks and item tables of launches issued one by one
     size_t arena_cap = 0;
+    // the host image of the last upload to d_arena: it must outlive the copy,
+    // which the runtime may perform after hipMemcpyAsync has returned
+    std::vector<char> stage;
+    hipEvent_t stage_done = nullptr;
+    bool stage_pending = false;
     int use_graphs = 1;
     uint64_t gen = 0;              // sum of the contexts' graph generations when graphs were captured
     struct Graph {
@@ -2265,6 +2274,10 @@ extern "C" void ether_batch_destroy(ether_batch* b) {
     (void)hipFree(b->d_arena);
     (void)hipFree(b->d_ctrs);
     (void)hipFree(b->d_vals);
+    if (b->stage_done) {
+        if (b->stage_pending) (void)hipEventSynchronize(b->stage_done);
+        (void)hipEventDestroy(b->stage_done);
+    }
     if (b->done) (void)hipEventDestroy(b->done);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -2320,9 +2333,19 @@ static int batch_eager_step(ether_batch* b, int32_t stepi, int32_t mcs_eq, int32
             return batch_fail("ether_batch_run: argument arena allocation failed", ETHER_ERR_HIP);
         b->arena_cap = cap;
     }
-    // stream-ordered after the launches that read the previous step's arguments
-    if (hipMemcpyAsync(b->d_arena, host.data(), host.size(), hipMemcpyHostToDevice, b->stream) != hipSuccess)
+    // stream-ordered after the launches that read the previous step's
+    // arguments; the image lives in the batch until its copy has completed
+    if (!b->stage_done && hipEventCreateWithFlags(&b->stage_done, hipEventDisableTiming) != hipSuccess)
+        return batch_fail("ether_batch_run: event creation failed", ETHER_ERR_HIP);
+    if (b->stage_pending && hipEventSynchronize(b->stage_done) != hipSuccess)
+        return batch_fail("ether_batch_run: waiting for the previous argument upload failed", ETHER_ERR_HIP);
+    b->stage_pending = false;
+    b->stage.swap(host);
+    if (hipMemcpyAsync(b->d_arena, b->stage.data(), b->stage.size(), hipMemcpyHostToDevice, b->stream) != hipSuccess)
         return batch_fail("ether_batch_run: argument upload failed", ETHER_ERR_HIP);
+    if (hipEventRecord(b->stage_done, b->stream) != hipSuccess)
+        return batch_fail("ether_batch_run: event record failed", ETHER_ERR_HIP);
+    b->stage_pending = true;
     issue_batch(ls, b->d_arena, b->stream);
     if (hipGetLastError() != hipSuccess) return batch_fail("ether_batch_run: launch failed", ETHER_ERR_HIP);
     *sc = sc_out;
@@ -2407,8 +2430,10 @@ static int batch_replay(ether_batch* b, int gi, int* sc) {
         if (c->dev_step != c->step || c->dev_pt != c->pt_count || c->dev_sc != (uint32_t)*sc) reset = true;
     }
     if (reset) {
+        // vals is freed on return: the copy must be complete before that
         if (hipMemcpyAsync(b->d_vals, vals.data(), sizeof(uint32_t) * vals.size(), hipMemcpyHostToDevice,
-                           b->stream) != hipSuccess)
+                           b->stream) != hipSuccess ||
+            hipStreamSynchronize(b->stream) != hipSuccess)
             return batch_fail("ether_batch_run: counter upload failed", ETHER_ERR_HIP);
         launch_ctr_batch(b->d_ctrs, n, b->d_vals, 0, 0, b->stream);
         for (ether_ctx* c : b->ctx) {

@@ -404,8 +404,8 @@ __global__ __launch_bounds__(256, 2) void k_dft_axis(DftArgs g, int items, int n
     const int nwaves = gridDim.x * 4;
     const int kq = 8 * (lane >> 4);                // this lane's 8 k within a 32-deep step
     const int nsteps = g.Kp / BK;
-    for (int it = blockIdx.x * 4 + wv; it < items; it += nwaves) {
-        int rest = it;
+    for (long long it = blockIdx.x * 4 + wv; it < items; it += nwaves) {
+        int rest = (int)it;
         const int slab = rest % nslab;
         rest /= nslab;
         int a_only = 0;

@@ -1918,51 +1918,80 @@ __device__ __forceinline__ void ising_measure_narrow_block(const MeasArgs& a, in
     const int r = g * kWave + lane_id();
     const ConstF32 wtab = (ConstF32)a.w;
     const ConstU8 sptab = (ConstU8)a.sp;
-    double bonds[kMaxIsingClasses + 1];
-    double zs[kMaxIsingClasses];
+    // bond and spin sums are integers: counted in int32 (a lane sees at most
+    // a tile's sites times their rows) and written as doubles, exact; the
+    // class and species loops stop at the context's counts (uniform), so a
+    // one-class lattice pays one compare-and-add per bond
+    int bonds[kMaxIsingClasses + 1];
+    int zs[kMaxIsingClasses];
 #pragma unroll
-    for (int k = 0; k <= kMaxIsingClasses; ++k) bonds[k] = 0.0;
+    for (int k = 0; k <= kMaxIsingClasses; ++k) bonds[k] = 0;
 #pragma unroll
-    for (int k = 0; k < kMaxIsingClasses; ++k) zs[k] = 0.0;
+    for (int k = 0; k < kMaxIsingClasses; ++k) zs[k] = 0;
     double mz = 0.0;
     const int per_wave = (a.tile_sites + kWavesPerBlock - 1) / kWavesPerBlock;
     const int s_begin = tile * a.tile_sites + wave_id() * per_wave;
     const int s_end = min(min(a.N, (tile + 1) * a.tile_sites), s_begin + per_wave);
-    for (int slot = s_begin; slot < s_end; ++slot) {
-        const int si = a.s[(size_t)slot * a.Rp + r];
-        const ConstRow nb = const_row(a.ell_m + (size_t)slot * a.D_m);   // wave-uniform row
-        const int n = (int)const_row(a.ell_m_n)[slot];
-        for (int k0 = 0; k0 < n; k0 += 4) {
-            int sj[4], cl[4];
+    // four sites of the walk at a time, so that their loads are in flight
+    // together (a walk of a few sites otherwise waits for memory once per
+    // site); the magnetisation is added in site order
+    constexpr int U = 4;
+    for (int slot0 = s_begin; slot0 < s_end; slot0 += U) {
+        int si[U], n[U], nmax = 0;
+        ConstRow nb[U];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint32_t e = (k0 + k < n) ? nb[k0 + k] : 0u;   // 0: class 0, no bond
-                cl[k] = (int)(e >> 24);
-                sj[k] = a.s[(size_t)(e & 0xffffffu) * a.Rp + r];
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-#pragma unroll
-                for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
-                    if (kk == cl[k]) bonds[kk] += (double)(si * sj[k]);
+        for (int u = 0; u < U; ++u) {
+            const bool in = slot0 + u < s_end;                // wave-uniform
+            const int slot = in ? slot0 + u : slot0;
+            si[u] = a.s[(size_t)slot * a.Rp + r];
+            nb[u] = const_row(a.ell_m + (size_t)slot * a.D_m);   // wave-uniform rows
+            n[u] = in ? (int)const_row(a.ell_m_n)[slot] : 0;
+            nmax = max(nmax, n[u]);
         }
-        const int sp = (int)sptab[slot];
+        for (int k0 = 0; k0 < nmax; k0 += 4) {
+            int sj[U][4], cl[U][4];
 #pragma unroll
-        for (int kk = 0; kk < kMaxIsingClasses; ++kk)
-            if (kk == sp) zs[kk] += (double)si;
-        mz += (double)(wtab[slot] * (float)si);
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t e = (k0 + k < n[u]) ? nb[u][k0 + k] : 0u;   // 0: class 0, no bond
+                    cl[u][k] = (int)(e >> 24);
+                    sj[u][k] = a.s[(size_t)(e & 0xffffffu) * a.Rp + r];
+                }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int prod = si[u] * sj[u][k];
+#pragma unroll
+                    for (int kk = 1; kk <= kMaxIsingClasses; ++kk) {
+                        if (kk > a.n_classes) break;
+                        bonds[kk] += (kk == cl[u][k]) ? prod : 0;
+                    }
+                }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (slot0 + u >= s_end) break;
+            const int slot = slot0 + u;
+            const int sp = (int)sptab[slot];
+#pragma unroll
+            for (int kk = 0; kk < kMaxIsingClasses; ++kk) {
+                if (kk >= a.nsp) break;
+                zs[kk] += (kk == sp) ? si[u] : 0;
+            }
+            mz += (double)(wtab[slot] * (float)si[u]);
+        }
     }
     // each bond was counted once, from its endpoint with the smaller slot
-#pragma unroll
-    for (int k = 1; k <= kMaxIsingClasses; ++k) bonds[k] *= (double)a.bond_factor;
     double* out = a.partial + ((size_t)tile * a.Rp + r) * a.rec;
     int o = 0;
 #pragma unroll
     for (int k = 1; k <= kMaxIsingClasses; ++k)
-        if (k <= a.n_classes) block_record(sRed, bonds[k], out + o++);
+        if (k <= a.n_classes) block_record(sRed, (double)bonds[k] * (double)a.bond_factor, out + o++);
 #pragma unroll
     for (int k = 0; k < kMaxIsingClasses; ++k)
-        if (k < a.nsp) block_record(sRed, zs[k], out + o++);
+        if (k < a.nsp) block_record(sRed, (double)zs[k], out + o++);
     block_record(sRed, mz, out + o);
 }
 

@@ -709,10 +709,31 @@ __global__ __launch_bounds__(kBlock) void k_xyz_ovrr_batch(const XyzArgs* __rest
 // depends only on (species, sigma_i, sum_k stride_k n_k), and the acceptance
 // test is an exact integer compare with a host-built threshold.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void class_add(uint32_t (&d)[kMaxIsingClasses + 1], int c, uint32_t v) {
+// v added to class c's per-byte counter and 1 to its bond count.  BOUNDED
+// (rows of at most 8): the loop stops at the context's class count nc
+// (uniform), which bounds c, so a one-class lattice pays one compare per
+// neighbour instead of eight; longer unrolled rows keep the eight-way test
+// (the bounded form of 18 neighbours crashes this compiler's register
+// coalescer).
+template <bool BOUNDED>
+__device__ __forceinline__ void class_add(uint32_t (&d)[kMaxIsingClasses + 1], int (&cnt)[kMaxIsingClasses + 1],
+                                          int nc, int c, uint32_t v) {
+    if constexpr (BOUNDED) {
 #pragma unroll
-    for (int k = 1; k <= kMaxIsingClasses; ++k)
-        if (k == c) d[k] += v;
+        for (int k = 1; k <= kMaxIsingClasses; ++k) {
+            if (k > nc) break;
+            const bool on = k == c;
+            d[k] += on ? v : 0u;
+            cnt[k] += on ? 1 : 0;
+        }
+    } else {
+#pragma unroll
+        for (int k = 1; k <= kMaxIsingClasses; ++k)
+            if (k == c) d[k] += v;
+#pragma unroll
+        for (int k = 1; k <= kMaxIsingClasses; ++k)
+            if (k == c) cnt[k] += 1;
+    }
 }
 
 // The flip mask of one site's word (byte b is 0xFE where replica b accepts)
@@ -811,10 +832,7 @@ __device__ __forceinline__ void ising_sweep_block(const IsingArgs& a, int bb) {
 #pragma unroll
             for (int k = 0; k < DM; ++k) {
                 const int c = (int)(nb[k] >> 24);
-                class_add(d, c, (nv[k] >> 1) & 0x01010101u);   // 1 per byte where sigma = -1
-#pragma unroll
-                for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
-                    if (kk == c) cnt[kk] += 1;
+                class_add<(DM > 0 && DM <= 8)>(d, cnt, a.n_classes, c, (nv[k] >> 1) & 0x01010101u);   // 1 per byte where sigma = -1
             }
         } else {
             for (int k = 0; k < a.D; ++k) {
@@ -822,10 +840,7 @@ __device__ __forceinline__ void ising_sweep_block(const IsingArgs& a, int bb) {
                 const int c = (int)(e >> 24);
                 if (c == 0) continue;
                 const uint32_t nbv = *reinterpret_cast<const uint32_t*>(a.s + (size_t)(e & 0xffffffu) * a.Rp + r0);
-                class_add(d, c, (nbv >> 1) & 0x01010101u);
-#pragma unroll
-                for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
-                    if (kk == c) cnt[kk] += 1;
+                class_add<(DM > 0 && DM <= 8)>(d, cnt, a.n_classes, c, (nbv >> 1) & 0x01010101u);
             }
             v = *own;
             wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)rid, step, RNG_METROPOLIS, 0);
@@ -1595,10 +1610,7 @@ __device__ __forceinline__ void ising_sweep_lanes_block(const IsingArgs& a, int 
 #pragma unroll
             for (int k = 0; k < DM; ++k) {
                 const int c = (int)(nb[k] >> 24);
-                class_add(d, c, (nv[k] >> 1) & 0x01010101u);
-#pragma unroll
-                for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
-                    if (kk == c) cnt[kk] += 1;
+                class_add<(DM > 0 && DM <= 8)>(d, cnt, a.n_classes, c, (nv[k] >> 1) & 0x01010101u);
             }
         } else {
             for (int k = 0; k < a.D; ++k) {
@@ -1606,10 +1618,7 @@ __device__ __forceinline__ void ising_sweep_lanes_block(const IsingArgs& a, int 
                 const int c = (int)(e >> 24);
                 if (c == 0) continue;
                 const uint32_t nbv = *reinterpret_cast<const uint32_t*>(a.s + (size_t)(e & 0xffffffu) * a.Rp + r0);
-                class_add(d, c, (nbv >> 1) & 0x01010101u);
-#pragma unroll
-                for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
-                    if (kk == c) cnt[kk] += 1;
+                class_add<(DM > 0 && DM <= 8)>(d, cnt, a.n_classes, c, (nbv >> 1) & 0x01010101u);
             }
             v = *own;
             wr = rng_draw(a.seed, (uint32_t)slot, (uint32_t)rid, step, RNG_METROPOLIS, 0);
@@ -1857,10 +1866,7 @@ __device__ __forceinline__ void ising_measure_block(const MeasArgs& a, int blk) 
             }
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                class_add(d, cl[k], ((sj[k] ^ v) >> 1) & 0x01010101u);
-#pragma unroll
-                for (int kk = 1; kk <= kMaxIsingClasses; ++kk)
-                    if (kk == cl[k]) nbond[kk] += 1;
+                class_add<true>(d, nbond, a.n_classes, cl[k], ((sj[k] ^ v) >> 1) & 0x01010101u);
             }
         }
 #pragma unroll

@@ -1397,8 +1397,12 @@ static int enqueue_sweep(ether_ctx* c, double fuse_frac = 0.0, bool meas = false
                     // a workgroup books its sums with one atomic per (replica,
                     // word), so the phase's atomics scale with its workgroups:
                     // the longest walk per wave keeps them few (the spins do not
-                    // depend on the launch shape)
-                    a.spw = kMaxSitesPerWave;
+                    // depend on the launch shape).  Several sites per wave
+                    // (few replicas) keep the context's walk: there a
+                    // workgroup books fewer replicas, and the long walk cost
+                    // more in latency than it saved in atomics (C5's
+                    // simple-cubic Heisenberg batch: 0.754 -> 0.715 ms per MCS)
+                    a.spw = c->lps_shift < 6 ? c->spw : kMaxSitesPerWave;
                     set_blocks(a, a.G, a.spw << (6 - a.lps_shift));
                 }
             }

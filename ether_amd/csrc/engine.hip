@@ -81,6 +81,11 @@ struct ether_ctx {
     bool res_rep = false;
     int res1_thr_lds = 0;
     size_t res1_lds = 0;
+    // in a batch (k_ising_resident1_batch): one workgroup per replica with
+    // its spins and threshold table in up to kBatchResidentLds
+    bool res1b_ok = false;
+    int res1b_block = 256;
+    size_t res1b_lds = 0;
     // spiral-state partners (ether_set_spiral): site-space rows, -1 padded
     int spiral_cap = 0;
     double spiral_count = 0.0;
@@ -897,14 +902,22 @@ extern "C" int ether_commit(ether_ctx* c) {
     // resident Ising sweeps: one word (4 replicas) of every site in LDS, and
     // the four replicas' threshold tables when they fit too
     c->res_ok = false;
+    c->res1b_ok = false;
+    int maxcol = 0;
+    for (int q = 0; q < c->ncol; ++q) maxcol = std::max(maxcol, c->cbeg[q + 1] - c->cbeg[q]);
+    // about two sites of the largest colour per thread, 64 to 1024 threads
+    const int res_threads = std::max(64, std::min(1024, (maxcol / 2 + 63) / 64 * 64));
+    if (ising && c->ncol <= kMaxResColours) {
+        const size_t n16 = ((size_t)N + 15) / 16 * 16;
+        c->res1b_lds = n16 + (size_t)4 * c->ising_table;
+        c->res1b_ok = c->res1b_lds <= kBatchResidentLds;
+        c->res1b_block = res_threads;
+    }
     if (ising && c->ncol <= kMaxResColours && (size_t)N * 4 <= kResidentLds) {
         c->res_ok = true;
         c->res_thr_lds = ((size_t)N * 4 + (size_t)16 * c->ising_table <= kResidentLds) ? 1 : 0;
         c->res_lds = (size_t)N * 4 + (c->res_thr_lds ? (size_t)16 * c->ising_table : 0);
-        int maxcol = 0;
-        for (int q = 0; q < c->ncol; ++q) maxcol = std::max(maxcol, c->cbeg[q + 1] - c->cbeg[q]);
-        // about two sites of the largest colour per thread, 64 to 1024 threads
-        c->res_block = std::max(64, std::min(1024, (maxcol / 2 + 63) / 64 * 64));
+        c->res_block = res_threads;
         // per replica: N int8 spins (+ one threshold table); a workgroup does
         // about half the instructions of a word's (the Philox block is most of
         // a site's work either way), so it wins while the replicas take fewer
@@ -1863,8 +1876,12 @@ static int enqueue_step(ether_ctx* c, int32_t stepi, int32_t mcs_eq, int32_t coa
 
 // The resident Ising path: n_sweeps sweeps in one launch, the last one
 // booking the measurement of the state it leaves when meas is set.
+// While a batch records, the batch decides (g_record_resident): every
+// lattice of it takes the resident path or none does, so that their launch
+// sequences line up.
+static thread_local bool g_record_resident = false;
 static bool resident_path(const ether_ctx* c) {
-    return c->resident && c->res_ok && !g_recorder;
+    return g_recorder ? g_record_resident : (c->resident && c->res_ok);
 }
 
 static int enqueue_resident(ether_ctx* c, int n_sweeps, bool meas) {
@@ -1890,15 +1907,19 @@ static int enqueue_resident(ether_ctx* c, int n_sweeps, bool meas) {
     a.offset = c->ising_offset;
     a.table_per_slot = c->ising_table;
     a.P = c->ising_P;
-    const bool per_rep = c->resident == 3 || (c->resident == 1 && c->res_rep);
-    a.table_in_lds = per_rep ? c->res1_thr_lds : c->res_thr_lds;
+    // a batch runs a workgroup per replica with the table in LDS
+    const bool per_rep = g_recorder || c->resident == 3 || (c->resident == 1 && c->res_rep);
+    a.table_in_lds = g_recorder ? 1 : (per_rep ? c->res1_thr_lds : c->res_thr_lds);
     a.thr = c->d_thr;
     a.meas = meas ? 1 : 0;
     a.nsp = c->nsp;
     a.rec = c->meas_rec;
     a.partial = c->d_partial;
     const int ev = ev_begin(c, ETHER_TIME_RESIDENT, n_sweeps);
-    launch_ising_resident(a, c->res_block, per_rep ? c->res1_lds : c->res_lds, per_rep, c->stream);
+    if (g_recorder)
+        launch_ising_resident(a, c->res1b_block, c->res1b_lds, true, c->stream);
+    else
+        launch_ising_resident(a, c->res_block, per_rep ? c->res1_lds : c->res_lds, per_rep, c->stream);
     ev_end(c, ev);
     HIPCHK(c, hipGetLastError());
     c->step += (uint32_t)n_sweeps;
@@ -2141,6 +2162,9 @@ struct ether_batch {
     hipEvent_t stage_done = nullptr;
     bool stage_pending = false;
     int use_graphs = 1;
+    // every lattice sweeps with its spins resident in LDS (Ising, each
+    // context's resident mode 1 or 3, and each fitting kBatchResidentLds)
+    bool resident = false;
     uint64_t gen = 0;              // sum of the contexts' graph generations when graphs were captured
     struct Graph {
         int phase, P, coa, ovrr_iv, pt_iv, n_pt;
@@ -2158,6 +2182,7 @@ struct BatchedLaunch {
     int kind, variant;
     size_t args_off, items_off;
     int n_items;
+    int block, lds;
 };
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
@@ -2192,7 +2217,8 @@ int plan_batch(const std::vector<LaunchRecorder>& recs, std::vector<char>& host,
             const size_t aoff = align256(host.size());
             host.resize(aoff);
             std::vector<int2> list;
-            int na = 0;
+            std::vector<int> cost;   // LK_ISING_RESIDENT: sites of each argument block's lattice
+            int na = 0, block = 0, lds = 0;
             for (const auto& r : recs) {
                 const LaunchRecord& q = r.recs[v];
                 if (q.units <= 0 || q.variant != var) continue;
@@ -2200,9 +2226,24 @@ int plan_batch(const std::vector<LaunchRecorder>& recs, std::vector<char>& host,
                 host.resize(o + (size_t)q.bytes);
                 std::memcpy(host.data() + o, &q.u, (size_t)q.bytes);
                 for (int blk = 0; blk < q.units; ++blk) list.push_back(make_int2(na, blk));
+                if (kind == LK_ISING_RESIDENT) cost.push_back(q.u.r.N);
+                block = std::max(block, q.block);
+                lds = std::max(lds, q.lds);
                 ++na;
             }
             const int n = (int)list.size();
+            if (kind == LK_ISING_RESIDENT) {
+                // a workgroup runs a whole replica's sweeps: the largest
+                // lattices first (workgroups are dispatched in order, round
+                // robin over the XCDs), the small ones fill the tail
+                std::stable_sort(list.begin(), list.end(),
+                                 [&](const int2& x, const int2& y) { return cost[x.x] > cost[y.x]; });
+                const size_t ioff = align256(host.size());
+                host.resize(ioff + sizeof(int2) * (size_t)n);
+                std::memcpy(host.data() + ioff, list.data(), sizeof(int2) * (size_t)n);
+                out.push_back({kind, var, aoff, ioff, n, block, lds});
+                continue;
+            }
             const int nb = (n + 7) / 8 * 8, per = nb / 8;
             const size_t ioff = align256(host.size());
             host.resize(ioff + sizeof(int2) * (size_t)nb);
@@ -2211,7 +2252,7 @@ int plan_batch(const std::vector<LaunchRecorder>& recs, std::vector<char>& host,
                 const int src = (j % 8) * per + j / 8;
                 it[j] = src < n ? list[src] : make_int2(-1, 0);
             }
-            out.push_back({kind, var, aoff, ioff, nb});
+            out.push_back({kind, var, aoff, ioff, nb, block, lds});
         }
     }
     return ETHER_OK;
@@ -2220,7 +2261,7 @@ int plan_batch(const std::vector<LaunchRecorder>& recs, std::vector<char>& host,
 void issue_batch(const std::vector<BatchedLaunch>& ls, const char* d_base, hipStream_t st) {
     for (const auto& L : ls)
         launch_batch(L.kind, L.variant, d_base + L.args_off,
-                     reinterpret_cast<const int2*>(d_base + L.items_off), L.n_items, st);
+                     reinterpret_cast<const int2*>(d_base + L.items_off), L.n_items, L.block, L.lds, st);
 }
 
 int batch_fail(const std::string& msg, int code) {
@@ -2293,6 +2334,12 @@ extern "C" int ether_batch_set_graphs(ether_batch* b, int32_t on) {
     return ETHER_OK;
 }
 
+extern "C" int ether_batch_get_resident(ether_batch* b, int32_t* on) {
+    if (!b || !on) return batch_fail("ether_batch_get_resident: null argument", ETHER_ERR_ARG);
+    *on = b->resident ? 1 : 0;
+    return ETHER_OK;
+}
+
 // Records the launches of `steps` (stepi, eq) on every context, in the mode
 // the caller set up (eager, or capturing with period-relative counters).
 static int batch_record(ether_batch* b, int32_t first, int32_t count, int32_t eq, int32_t coa,
@@ -2304,10 +2351,10 @@ static int batch_record(ether_batch* b, int32_t first, int32_t count, int32_t eq
         ether_ctx* c = b->ctx[l];
         int sc = sc0;
         g_recorder = &recs[l];
-        int rc = ETHER_OK;
-        for (int32_t s = first; s < first + count && rc == ETHER_OK; ++s)
-            rc = enqueue_step(c, s, eq, coa, ovrr_iv, ovrr_p, pt_iv, &sc);
+        g_record_resident = b->resident;
+        const int rc = enqueue_steps(c, first, count, eq, coa, ovrr_iv, ovrr_p, pt_iv, &sc);
         g_recorder = nullptr;
+        g_record_resident = false;
         if (rc) return batch_fail(c->err, rc);
         if (l == 0) *sc_out = sc;
         else if (sc != *sc_out)
@@ -2400,6 +2447,7 @@ static int batch_graph_for(ether_batch* b, int phase, int P, int coa, int ovrr_i
         (void)hipFree(arena);
         return batch_fail("ether_batch_run: graph argument upload failed", ETHER_ERR_HIP);
     }
+    for (const auto& L : ls) grant_batch_lds(L.kind, L.variant, L.lds);
     hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
         issue_batch(ls, arena, b->stream);
@@ -2469,7 +2517,9 @@ extern "C" int ether_batch_run(ether_batch* b, int32_t first_step, int32_t n_ste
     const bool do_ovrr = ovrr_interval > 0 && c0->model == ETHER_MODEL_XYZ && !c0->sia && ovrr_passes > 0.0;
     const bool do_pt = pt_interval > 0 && c0->n_groups > 0;
     uint64_t gen = 0;
+    bool resident = c0->model == ETHER_MODEL_ISING;
     for (ether_ctx* c : b->ctx) {
+        resident = resident && c->res1b_ok && (c->resident == 1 || c->resident == 3);
         int rc = need_betas(c);
         if (rc) return batch_fail(c->err, rc);
         if (c->timing) return batch_fail("ether_batch_run: kernel timing must be off in batched contexts",
@@ -2479,9 +2529,10 @@ extern "C" int ether_batch_run(ether_batch* b, int32_t first_step, int32_t n_ste
                               ETHER_ERR_STATE);
         gen += c->graph_gen;
     }
-    if (gen != b->gen) {   // a context changed what captured graphs bake in
+    if (gen != b->gen || resident != b->resident) {   // a context changed what captured graphs bake in
         batch_drop_graphs(b);
         b->gen = gen;
+        b->resident = resident;
     }
     for (ether_ctx* c : b->ctx) {
         (void)hipStreamSynchronize(c->stream);

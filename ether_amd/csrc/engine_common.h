@@ -317,6 +317,7 @@ enum LaunchKind : int {
     LK_MEASURE = 3,
     LK_FINALIZE = 4,
     LK_PT = 5,
+    LK_ISING_RESIDENT = 6,
 };
 struct LaunchRecord {
     int kind;
@@ -324,9 +325,12 @@ struct LaunchRecord {
                    // measure (wide Ising << 3) | (model << 2) | (zee << 1) | sia
     int units;     // workgroups of the launch on its own, without XCD padding (0: nothing to do)
     int bytes;     // size of the argument block
+    int block;     // threads per workgroup (0: kBlock)
+    int lds;       // dynamic LDS bytes per workgroup
     union {
         XyzArgs x;
         IsingArgs i;
+        IsingResArgs r;
         MeasArgs m;
         FinArgs f;
         PtArgs p;
@@ -337,9 +341,16 @@ struct LaunchRecorder {
 };
 extern thread_local LaunchRecorder* g_recorder;
 // one batched launch: args[n_args] of the record kind, items[n_items] =
-// (argument index or -1, local workgroup)
+// (argument index or -1, local workgroup); block and lds are the largest of
+// the recorded launches (LK_ISING_RESIDENT; the other kinds use kBlock and
+// static LDS)
 void launch_batch(int kind, int variant, const void* d_args, const int2* d_items, int n_items,
-                  hipStream_t st);
+                  int block, int lds, hipStream_t st);
+// opts a batched kernel instance into `lds` bytes of dynamic LDS (before a
+// graph capture records its launch)
+void grant_batch_lds(int kind, int variant, int lds);
+// dynamic LDS of the batched resident Ising kernel, per workgroup (one per CU)
+constexpr size_t kBatchResidentLds = 150 * 1024;
 // counters of several contexts: set (vals[3*i..]) or advance by (dstep, dswaps)
 void launch_ctr_batch(uint32_t* const* d_ctrs, int n, const uint32_t* d_vals, uint32_t dstep,
                       uint32_t dswaps, hipStream_t st);

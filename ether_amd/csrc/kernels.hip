@@ -1109,7 +1109,7 @@ __global__ void __launch_bounds__(1024) k_ising_resident(IsingResArgs a) {
 // word kernel's for this replica, summed in the same thread order.
 // ---------------------------------------------------------------------------
 template <int DM, int NC, bool TL>
-__global__ void __launch_bounds__(1024) k_ising_resident1(IsingResArgs a) {
+__device__ __forceinline__ void ising_resident1_block(const IsingResArgs& a, int r) {
     constexpr int KC = NC > 0 ? NC : kMaxIsingClasses;
     extern __shared__ __attribute__((aligned(16))) uint32_t sRes1[];
     int8_t* sS = reinterpret_cast<int8_t*>(sRes1);                 // [N] spins
@@ -1118,7 +1118,7 @@ __global__ void __launch_bounds__(1024) k_ising_resident1(IsingResArgs a) {
     __shared__ int sCnt1[2 * kMaxIsingClasses];
     __shared__ double sMz1[16];
     __shared__ int sCb[kMaxResColours + 1];
-    const int r = blockIdx.x, rid = r >> 2, b = r & 3;
+    const int rid = r >> 2, b = r & 3;
     const int t = threadIdx.x, nt = blockDim.x;
     const int tbase = TL ? 0 : a.slot_of[r] * a.table_per_slot;
     for (int q = t; q < a.N; q += nt) sS[q] = a.s[(size_t)q * a.Rp + r];
@@ -1283,7 +1283,7 @@ __global__ void __launch_bounds__(1024) k_ising_resident1(IsingResArgs a) {
     __syncthreads();
     if (t == 0) {
         if (sAcc1)
-            atomicAdd(a.acc + (size_t)(blockIdx.x % kAccCopies) * a.R + a.slot_of[r], (unsigned long long)sAcc1);
+            atomicAdd(a.acc + (size_t)(r % kAccCopies) * a.R + a.slot_of[r], (unsigned long long)sAcc1);
         if (a.meas) {
             double* out = a.partial + (size_t)r * a.rec;    // tile 0
             int o = 0;
@@ -1294,6 +1294,25 @@ __global__ void __launch_bounds__(1024) k_ising_resident1(IsingResArgs a) {
             out[o] = mm;
         }
     }
+}
+
+template <int DM, int NC, bool TL>
+__global__ void __launch_bounds__(1024) k_ising_resident1(IsingResArgs a) {
+    ising_resident1_block<DM, NC, TL>(a, blockIdx.x);
+}
+
+// The same for the lattices of a batch: workgroup j runs replica items[j].y
+// of lattice items[j].x, whose spins (and table) take the first part of the
+// launch's dynamic LDS (sized for the batch's largest lattice).  The block
+// size is the batch's largest; a lattice's sweeps do not depend on it.
+template <int DM, int NC, bool TL>
+__global__ void __launch_bounds__(1024) k_ising_resident1_batch(const IsingResArgs* __restrict__ args,
+                                                                const int2* __restrict__ items) {
+    const int2 it = items[blockIdx.x];
+    if (it.x < 0) return;
+    // read in place: a private copy would put its indexed arrays (colour
+    // bounds, class strides) in scratch
+    ising_resident1_block<DM, NC, TL>(args[it.x], it.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -2364,12 +2383,14 @@ static int sweep_units(int groups, int n_sites, int spw) {
 }
 
 template <class A>
-static void record(int kind, int variant, int units, const A& a) {
+static void record(int kind, int variant, int units, const A& a, int block = 0, int lds = 0) {
     LaunchRecord r;
     r.kind = kind;
     r.variant = variant;
     r.units = units;
     r.bytes = (int)sizeof(A);
+    r.block = block;
+    r.lds = lds;
     *reinterpret_cast<A*>(&r.u) = a;
     g_recorder->recs.push_back(r);
 }
@@ -2477,7 +2498,17 @@ static void launch_ising_resident1_tl(const IsingResArgs& a, dim3 grid, dim3 blk
     }
 }
 
+// variant of a recorded resident launch: (coupling-class instance << 16) | D
+static int resident_variant(const IsingResArgs& a) {
+    return ((a.n_classes == 1 ? 1 : (a.n_classes == 2 ? 2 : 0)) << 16) | a.D;
+}
+
 void launch_ising_resident(const IsingResArgs& a, int block, size_t lds_bytes, bool per_replica, hipStream_t st) {
+    if (g_recorder) {
+        // batches run the per-replica kernel with the table in LDS
+        record(LK_ISING_RESIDENT, resident_variant(a), a.R, a, block, (int)lds_bytes);
+        return;
+    }
     const dim3 blk((unsigned)block);
     if (per_replica) {
         const dim3 grid((unsigned)a.R);
@@ -2552,11 +2583,63 @@ static void launch_xyz_sweep_batch_t(const XyzArgs* a, const int2* d_items, int 
 #undef LM
 }
 
+// the batched resident kernel of one instance; LDS beyond the default
+// per-workgroup limit is opted into once per instance (grant_batch_lds does
+// it ahead of a graph capture)
+template <int DM, int NC>
+static void grant_resident1_batch(int lds) {
+    static int granted = 0;
+    if (lds > granted &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ising_resident1_batch<DM, NC, true>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, lds) == hipSuccess)
+        granted = lds;
+}
+template <int DM, int NC>
+static void launch_resident1_batch(const IsingResArgs* a, const int2* d_items, dim3 grid, dim3 blk, int lds,
+                                   hipStream_t st) {
+    grant_resident1_batch<DM, NC>(lds);
+    // without the grant the launch fails, which the caller's hipGetLastError reports
+    hipLaunchKernelGGL((k_ising_resident1_batch<DM, NC, true>), grid, blk, (size_t)lds, st, a, d_items);
+}
+
+void grant_batch_lds(int kind, int variant, int lds) {
+    if (kind != LK_ISING_RESIDENT) return;
+    const int nc = variant >> 16;
+#define LR(DM) grant_resident1_batch<DM, 1>(lds)
+#define LS(DM) grant_resident1_batch<DM, 2>(lds)
+#define LT(DM) grant_resident1_batch<DM, 0>(lds)
+    if (nc == 1) { ETHER_DISPATCH_DM(variant & 65535, LR) }
+    else if (nc == 2) { ETHER_DISPATCH_DM(variant & 65535, LS) }
+    else { ETHER_DISPATCH_DM(variant & 65535, LT) }
+#undef LR
+#undef LS
+#undef LT
+}
+
 void launch_batch(int kind, int variant, const void* d_args, const int2* d_items, int n_items,
-                  hipStream_t st) {
+                  int threads, int lds, hipStream_t st) {
     if (n_items <= 0) return;
     const dim3 grid(n_items), block(kBlock);
     switch (kind) {
+    case LK_ISING_RESIDENT: {
+        const IsingResArgs* a = static_cast<const IsingResArgs*>(d_args);
+        const dim3 blk((unsigned)threads);
+        const int nc = variant >> 16;
+        if (nc == 1) {
+#define LR(DM) launch_resident1_batch<DM, 1>(a, d_items, grid, blk, lds, st)
+            ETHER_DISPATCH_DM(variant & 65535, LR)
+#undef LR
+        } else if (nc == 2) {
+#define LR(DM) launch_resident1_batch<DM, 2>(a, d_items, grid, blk, lds, st)
+            ETHER_DISPATCH_DM(variant & 65535, LR)
+#undef LR
+        } else {
+#define LR(DM) launch_resident1_batch<DM, 0>(a, d_items, grid, blk, lds, st)
+            ETHER_DISPATCH_DM(variant & 65535, LR)
+#undef LR
+        }
+        break;
+    }
     case LK_XYZ_SWEEP: {
         const XyzArgs* a = static_cast<const XyzArgs*>(d_args);
         if (variant & 262144) launch_xyz_sweep_batch_t<true>(a, d_items, variant, grid, st);

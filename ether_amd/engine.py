@@ -94,6 +94,7 @@ EXPORTS = [
     ("ether_batch_run", C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                   C.c_double, C.c_int32, _vp]),
     ("ether_batch_set_graphs", C.c_int, [_vp, C.c_int32]),
+    ("ether_batch_get_resident", C.c_int, [_vp, C.POINTER(C.c_int32)]),
     ("ether_batch_destroy", None, [_vp]),
     ("ether_autotune", C.c_int, [_vp, C.c_int32, _vp, _vp, _vp]),
     ("ether_set_timing", C.c_int, [_vp, C.c_int32]),
@@ -597,6 +598,12 @@ class Batch:
 
     def set_graphs(self, on: bool) -> None:
         _check(self.lib.ether_batch_set_graphs(self.h, 1 if on else 0))
+
+    def resident(self) -> bool:
+        """Whether the last run swept with the spins resident in LDS."""
+        on = C.c_int32(0)
+        _check(self.lib.ether_batch_get_resident(self.h, C.byref(on)))
+        return bool(on.value)
 
     def close(self) -> None:
         if self.h:

@@ -372,6 +372,14 @@ int ether_batch_run(ether_batch* batch, int32_t first_step, int32_t n_steps, int
                     int32_t coa, int32_t ovrr_interval, double ovrr_passes, int32_t pt_interval,
                     int32_t* swap_count);
 int ether_batch_set_graphs(ether_batch* batch, int32_t on);
+/* Whether the last ether_batch_run swept with the spins resident in LDS:
+ * Ising batches whose every context allows it (ether_set_resident mode 1 or
+ * 3) and fits a workgroup's LDS (int8 spins of every site plus the
+ * threshold table, at most 150 KiB: up to about 150,000 sites) run
+ * the sweeps between two measurements as one launch with a workgroup per
+ * replica of every lattice, largest lattices first; otherwise the colour
+ * phases are batched launches of their own.  Results are identical. */
+int ether_batch_get_resident(ether_batch* batch, int32_t* on);
 void ether_batch_destroy(ether_batch* batch);
 
 /* Launch shape of the sweep colour phases (no counterpart in the reference,

@@ -90,3 +90,59 @@ def test_batch_equals_lattices_run_alone(cases, graphs):
     for (m, a), b in zip(alone, batched):
         a.close()
         b.close()
+
+
+# Ising batches sweep with every replica's spins resident in LDS (a
+# workgroup per replica of every lattice, the sweeps between two measurements
+# in one launch).  Cases: 40^3 (64,000 sites: above the one-context resident
+# budget, so alone it runs the phase kernels), an odd edge (more than two
+# colours), two coupling classes with Zeeman, and 6^3.
+ISING_RES = [
+    ("sc_ising_16", 8, 31, dict(sc="40 40 40"), np.linspace(100, 10, 8)),
+    ("sc_ising_16", 12, 32, dict(sc="9 9 9"), np.linspace(100, 10, 12)),
+    ("pyro_pt_16", 20, 22, dict(sc="2 2 2", model="ising", para=".T. 1", pt=".T. 11",
+                                zeeman=".T. 0 0 0.3"), np.linspace(6, 2, 20)),
+    ("sc_ising_16", 30, 21, dict(sc="6 6 6"), np.linspace(100, 10, 30)),
+]
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_resident_batch_equals_phase_batch_and_alone(graphs):
+    say(f"resident Ising batch of {len(ISING_RES)} lattices, graphs={graphs}")
+    res, phase, alone = [], [], []
+    for name, R, seed, over, temps in ISING_RES:
+        m, a = _engine(name, R, seed, over, temps)
+        res.append(a)
+        _, b = _engine(name, R, seed, over, temps)
+        b.set_resident(0)
+        phase.append(b)
+        _, c = _engine(name, R, seed, over, temps)
+        alone.append(c)
+    batches = [E.Batch(res), E.Batch(phase)]
+    scs = []
+    for batch in batches:
+        batch.set_graphs(graphs)
+        sc = batch.run(1, 37, 20, 5, 0, 0.0, 5, 0)
+        sc = batch.run(38, 63, 20, 5, 0, 0.0, 5, sc)
+        scs.append(sc)
+    assert batches[0].resident() and not batches[1].resident()
+    sca = 0
+    for c in alone:
+        sca = c.run(1, 37, 20, 5, 0, 0.0, 5, 0)
+        sca = c.run(38, 63, 20, 5, 0, 0.0, 5, sca)
+    assert scs[0] == scs[1] == sca
+    for k, (name, R, *_rest) in enumerate(ISING_RES):
+        sa, ma, pa = _state(res[k])
+        for other in (phase[k], alone[k]):
+            sb, mb, pb = _state(other)
+            assert np.array_equal(sa, sb), f"{name} R={R}: spins differ"
+            assert np.array_equal(ma, mb), f"{name} R={R}: moments differ"
+            for x, y in zip(pa, pb):
+                assert np.array_equal(x, y), f"{name} R={R}: tempering state differs"
+        assert ma[:, E.MOM_N].sum() > 0
+        # the last sweep's acceptance per replica (ether_sweep)
+        assert np.array_equal(res[k].sweep(2, want_accept=True), phase[k].sweep(2, want_accept=True))
+    for batch in batches:
+        batch.close()
+    for e in res + phase + alone:
+        e.close()

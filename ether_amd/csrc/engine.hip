@@ -66,7 +66,7 @@ struct ether_ctx {
     uint64_t graph_gen = 0;  // bumped whenever captured graphs are dropped
     int xcd_map = 2;      // launch parameters of the sweep phases (ether_set_launch)
     int spw = kDefaultSitesPerWave;
-    int lps_shift = 6;    // lanes per site of the sweep (1 << lps_shift); set at commit
+    int lps = kWave;      // lanes per site of the sweep (64: one site per wave); set at commit
     int lanes_req = 0;    // ether_set_lanes (0: chosen from R)
     int meas_tiles = 0, meas_sites = 0, meas_rec = 0;
     // Ising sweeps with the lattice resident in LDS (k_ising_resident):
@@ -496,15 +496,17 @@ int greedy_colour(int N, const std::vector<int>& adj_ptr, const std::vector<int>
 
 }  // namespace
 
-// Lanes per site of the sweep: the smallest of {64, 32, 16 (, 8 for Ising)}
-// lanes that holds every replica of a site (an Ising lane holds 4 replicas),
-// 64 when the replicas need more than one wave.
-static int auto_lanes_shift(int model, int R) {
+// Lanes per site of the sweep: the smallest of 16, 32 (XYZ) or 8, 16, 32
+// (Ising, 4 replicas per lane) that holds every replica, 64 otherwise.
+// Groups of exactly R lanes (three sites per wave for 17-21 replicas, up to
+// eight for 8) were slower on the batched uneven lattices: simple-cubic
+// Heisenberg 0.785 against 0.718 ms per MCS, pyrochlore 0.342 against 0.306;
+// more sites per wave cost more than the idle lanes they fill.
+static int auto_lanes(int model, int R) {
     const int need = model == ETHER_MODEL_ISING ? (R + 3) / 4 : R;
-    const int lo = model == ETHER_MODEL_ISING ? 3 : 4;
-    for (int sft = lo; sft < 6; ++sft)
-        if (need <= (1 << sft)) return sft;
-    return 6;
+    for (int l = model == ETHER_MODEL_ISING ? 8 : 16; l < kWave; l *= 2)
+        if (need <= l) return l;
+    return kWave;
 }
 static int lanes_shift_of(int lanes) {
     int sft = 0;
@@ -514,11 +516,11 @@ static int lanes_shift_of(int lanes) {
 
 extern "C" int ether_set_lanes(ether_ctx* c, int32_t lanes_per_site) {
     if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
-    const int lo = c->model == ETHER_MODEL_ISING ? 8 : 16;
+    const bool ising = c->model == ETHER_MODEL_ISING;
     const bool pow2 = lanes_per_site > 0 && (lanes_per_site & (lanes_per_site - 1)) == 0;
-    if (lanes_per_site != 0 && (!pow2 || lanes_per_site < lo || lanes_per_site > kWave))
-        return fail(c, ETHER_ERR_ARG, "ether_set_lanes: lanes per site must be 0 (automatic) or a power "
-                                      "of two from 16 (8 for Ising) to 64");
+    if (lanes_per_site != 0 && (lanes_per_site < 1 || lanes_per_site > kWave || (ising && (!pow2 || lanes_per_site < 8))))
+        return fail(c, ETHER_ERR_ARG, "ether_set_lanes: lanes per site must be 0 (automatic), 1 to 64 "
+                                      "(XYZ) or a power of two from 8 to 64 (Ising)");
     if (lanes_per_site != 0 && lanes_per_site < kWave) {
         const int per = c->model == ETHER_MODEL_ISING ? 4 : 1;
         if (lanes_per_site * per < c->R)
@@ -526,9 +528,9 @@ extern "C" int ether_set_lanes(ether_ctx* c, int32_t lanes_per_site) {
     }
     c->lanes_req = lanes_per_site;
     if (c->committed) {
-        const int sft = lanes_per_site ? lanes_shift_of(lanes_per_site) : auto_lanes_shift(c->model, c->R);
-        if (sft != c->lps_shift) drop_graphs(c);
-        c->lps_shift = sft;
+        const int lps = lanes_per_site ? lanes_per_site : auto_lanes(c->model, c->R);
+        if (lps != c->lps) drop_graphs(c);
+        c->lps = lps;
     }
     return ETHER_OK;
 }
@@ -867,8 +869,7 @@ extern "C" int ether_commit(ether_ctx* c) {
         return fail(c, ETHER_ERR_LIMIT, "ether_commit: sites x padded replicas exceeds 2^29 "
                                       "(one FP32 spin plane must stay under 2 GiB)");
     const int G = c->Rp / kWave;
-    c->lps_shift = auto_lanes_shift(c->model, R);
-    if (c->lanes_req) c->lps_shift = lanes_shift_of(c->lanes_req);
+    c->lps = c->lanes_req ? c->lanes_req : auto_lanes(c->model, R);
     // measurement: wave units of at least 8 sites each, a workgroup's four
     // waves sharing one tile and one partial record.  Short measurement rows
     // (3 bonds per site on average: simple cubic, triangular) run best with
@@ -1181,8 +1182,9 @@ static XyzArgs xyz_args(ether_ctx* c, int colour) {
     a.c_end = c->cbeg[colour + 1];
     a.xcd_map = c->xcd_map;
     a.spw = c->spw;
-    a.lps_shift = c->lps_shift;
-    set_blocks(a, a.G, a.spw << (6 - a.lps_shift));
+    a.lps = c->lps;
+    a.sps = kWave / c->lps;
+    set_blocks(a, a.G, a.spw * a.sps);
     a.slot_of = c->d_slot_of;
     a.beta_slot = c->d_beta_f;
     a.seed = c->seed;
@@ -1227,7 +1229,7 @@ static IsingArgs ising_args(ether_ctx* c, int colour) {
     a.c_end = c->cbeg[colour + 1];
     a.xcd_map = c->xcd_map;
     a.spw = c->spw;
-    a.lps_shift = c->lps_shift;
+    a.lps_shift = lanes_shift_of(c->lps);   // a power of two for Ising
     set_blocks(a, a.G4, a.spw << (6 - a.lps_shift));
     a.table_in_lds = ((long long)c->R * c->ising_table <= kIsingLdsTable) ? 1 : 0;
     a.slot_of = c->d_slot_of;
@@ -1362,7 +1364,7 @@ static int enqueue_resident(ether_ctx* c, int n_sweeps, bool meas);
 // of each of its phases (the most any colour needs at that shape).
 static int meas_record_spw(const ether_ctx* c) { return std::max(c->spw, kMeasRecordSpw); }
 static int meas_record_stride(const ether_ctx* c) {
-    const int spw = meas_record_spw(c) << (6 - c->lps_shift);
+    const int spw = meas_record_spw(c) * (kWave / c->lps);
     int n = 0;
     for (int q = 0; q < c->ncol; ++q)
         n = std::max(n, phase_blocks(c->cbeg[q + 1] - c->cbeg[q], c->Rp / kWave, c->xcd_map, spw));
@@ -1398,7 +1400,7 @@ static int enqueue_sweep(ether_ctx* c, double fuse_frac = 0.0, bool meas = false
                     // number of workgroups in every phase, so that each
                     // measuring sweep rewrites every record the finalize reads
                     a.spw = meas_record_spw(c);
-                    set_blocks(a, a.G, a.spw << (6 - a.lps_shift));
+                    set_blocks(a, a.G, a.spw * a.sps);
                     a.nblocks = meas_record_stride(c);
                     if (a.nblocks > c->mrec_cap)
                         return fail(c, ETHER_ERR_STATE, "measurement records exceed their allocation");
@@ -1415,8 +1417,8 @@ static int enqueue_sweep(ether_ctx* c, double fuse_frac = 0.0, bool meas = false
                     // workgroup books fewer replicas, and the long walk cost
                     // more in latency than it saved in atomics (C5's
                     // simple-cubic Heisenberg batch: 0.754 -> 0.715 ms per MCS)
-                    a.spw = c->lps_shift < 6 ? c->spw : kMaxSitesPerWave;
-                    set_blocks(a, a.G, a.spw << (6 - a.lps_shift));
+                    a.spw = c->lps < kWave ? c->spw : kMaxSitesPerWave;
+                    set_blocks(a, a.G, a.spw * a.sps);
                 }
             }
             launch_xyz_sweep(a, c->angle, c->zee, c->sia, c->stream);
@@ -1438,7 +1440,8 @@ static int enqueue_ovrr(ether_ctx* c, double passes) {
             if (empty && !g_recorder) continue;
             XyzArgs a = xyz_args(c, std::min(q, c->ncol - 1));
             if (empty) a.c_begin = a.c_end = 0;
-            a.lps_shift = 6;   // the overrelaxation kernel updates one site per wave
+            a.lps = kWave;   // the overrelaxation kernel updates one site per wave
+            a.sps = 1;
             set_blocks(a, a.G, a.spw);
             a.pass = (uint32_t)pass;
             a.frac = (p < full) ? 1.0f : (float)frac;

@@ -58,8 +58,10 @@ struct XyzArgs {
     int nblocks;
     int nchunk;            // site chunks of the phase: nblocks = groups x nchunk (+ padding)
     int xcd_map;
-    int spw;               // steps per wave (sites per wave = spw << (6 - lps_shift))
-    int lps_shift;         // lanes per site = 1 << lps_shift (6: one site per wave)
+    int spw;               // steps per wave (sites per wave = spw * sps)
+    int lps;               // lanes per site (64: one site per wave); lane l of a wave holds
+                           // replica l % lps of site l / lps, lanes from sps * lps idle
+    int sps;               // sites per step of a wave = 64 / lps
     const int* slot_of;
     const float* beta_slot;
     uint64_t seed;

@@ -444,15 +444,16 @@ __device__ __forceinline__ void book_site(double (&m)[kMeasWords], const XyzArgs
 }
 
 // Adds a workgroup's fixed-point sums to macc.  Lanes of one replica (one
-// per wave, or several lane groups per wave in the lanes kernels: lane l
-// holds replica r_base + (l & (per - 1))) are summed through LDS, then one
+// per wave, or one in each of the sps lane groups of `per` lanes of a wave in
+// the lanes kernels: lane s * per + q holds replica r_base + q) are summed
+// through LDS, then one
 // int64 atomic per (replica, word) into copy blockIdx % kMeasCopies of the
 // sums: device-scope atomics on one address serialise, and hundreds of
 // workgroups book every replica.  The terms are integers whose sums stay
 // below 2^51, so FP64 adds them exactly and the result is the same in any
 // order.  Called by every thread of the workgroup.
 __device__ __forceinline__ void book_measurement(double* buf /* [kMeasWords][kBlock] */,
-                                                 const double (&m)[kMeasWords], int per, int r_base,
+                                                 const double (&m)[kMeasWords], int per, int sps, int r_base,
                                                  const XyzArgs& a) {
 #pragma unroll
     for (int k = 0; k < kMeasWords; ++k) buf[k * kBlock + threadIdx.x] = m[k];
@@ -464,7 +465,8 @@ __device__ __forceinline__ void book_measurement(double* buf /* [kMeasWords][kBl
         const int r = r_base + lane;
         if (lane < per && r < a.R) {
             double t = 0.0;
-            for (int i = lane; i < kBlock; i += per) t += buf[k * kBlock + i];
+            for (int v = 0; v < kWavesPerBlock; ++v)
+                for (int q = 0; q < sps; ++q) t += buf[k * kBlock + v * kWave + q * per + lane];
             const size_t copy = blockIdx.x % kMeasCopies;
             atomicAdd(a.macc + (copy * a.Rp + r) * kMeasWords + k, (unsigned long long)(long long)t);
         }
@@ -479,7 +481,7 @@ __device__ __forceinline__ void book_measurement(double* buf /* [kMeasWords][kBl
 // the last measuring sweep only and nothing needs clearing.  Called by every
 // thread of the workgroup.
 __device__ __forceinline__ void book_records(double* buf /* [kMeasWords][kBlock] */,
-                                             const double (&m)[kMeasWords], int per, int bb,
+                                             const double (&m)[kMeasWords], int per, int sps, int bb,
                                              const XyzArgs& a) {
 #pragma unroll
     for (int k = 0; k < kMeasWords; ++k) buf[k * kBlock + threadIdx.x] = m[k];
@@ -489,7 +491,8 @@ __device__ __forceinline__ void book_records(double* buf /* [kMeasWords][kBlock]
     for (int o = threadIdx.x; o < per * kMeasWords; o += kBlock) {
         const int l = o / kMeasWords, k = o - l * kMeasWords;
         double t = 0.0;
-        for (int i = l; i < kBlock; i += per) t += buf[k * kBlock + i];
+        for (int v = 0; v < kWavesPerBlock; ++v)
+            for (int q = 0; q < sps; ++q) t += buf[k * kBlock + v * kWave + q * per + l];
         rec[(size_t)l * a.mrec_pp * kMeasWords + k] = (long long)t;
     }
 }
@@ -618,9 +621,9 @@ __device__ __forceinline__ void xyz_sweep_block(const XyzArgs& a, int bb) {
     if constexpr (MEAS) {
         __shared__ double sMeas[kMeasWords * kBlock];
         if (!a.mrec)
-            book_measurement(sMeas, m, kWave, g * kWave, a);
+            book_measurement(sMeas, m, kWave, 1, g * kWave, a);
         else if (a.c_end > a.c_begin)   // an empty phase (of a batch) owns no records
-            book_records(sMeas, m, kWave, bb, a);
+            book_records(sMeas, m, kWave, 1, bb, a);
     }
 }
 
@@ -1316,9 +1319,10 @@ __global__ void __launch_bounds__(1024) k_ising_resident1_batch(const IsingResAr
 }
 
 // ---------------------------------------------------------------------------
-// Several sites per wavefront, for contexts with few replicas: lanes per site
-// LPS = 1 << a.lps_shift (XYZ: 16 or 32 replicas per site; Ising: 8, 16 or 32
-// lanes of 4 replicas), so a wave updates 64 / LPS sites per step instead of
+// Several sites per wavefront, for contexts with few replicas: LPS lanes per
+// site (XYZ: a.lps, any count from the replicas up, 16 or 32 by default;
+// Ising: 1 << a.lps_shift = 8, 16 or 32 lanes of 4 replicas), so a wave
+// updates 64 / LPS whole sites per step instead of
 // one site with mostly idle lanes.  Each lane's replica, Philox counter,
 // neighbour order and arithmetic are those of the one-site-per-wave kernels,
 // and the pooled Marsaglia proposal pools within the lanes of one site (the
@@ -1460,15 +1464,16 @@ __device__ __forceinline__ void xyz_sweep_lanes_block(const XyzArgs& a, int bb) 
     __syncthreads();
 
     const int w = wave_id(), lane = lane_id();
-    const int ls = a.lps_shift, S = kWave >> ls;               // lanes per site, sites per step
-    const int sub = lane >> ls, r = lane & ((1 << ls) - 1);
-    const uint64_t gmask = ls >= 6 ? ~0ull : (((1ull << (1 << ls)) - 1ull) << (sub << ls));
-    const int gbase = sub << ls;
+    const int L = a.lps, S = a.sps;                            // lanes per site, sites per step
+    const int sub = lane / L, r = lane - sub * L;              // lanes from S * L idle (sub == S)
+    const bool lok = sub < S;
+    const uint64_t gmask = L >= kWave ? ~0ull : (((1ull << L) - 1ull) << (sub * L));
+    const int gbase = sub * L;
     const uint32_t roff = 4u * (uint32_t)r, rowb = 4u * (uint32_t)a.Rp;
     const Plane px = plane_of(a.sx, a.plane_bytes), py = plane_of(a.sy, a.plane_bytes),
                 pz = plane_of(a.sz, a.plane_bytes);
     const Plane pp = plane_of(ANGLE ? a.phi : a.sx, a.plane_bytes);
-    const bool rok = r < a.R;
+    const bool rok = lok && r < a.R;
     const float beta = rok ? a.beta_slot[a.slot_of[r]] : 0.f;
     const uint32_t step = a.ctr[0] + a.step_off;
     uint64_t* pool = sPool[w];
@@ -1478,7 +1483,7 @@ __device__ __forceinline__ void xyz_sweep_lanes_block(const XyzArgs& a, int bb) 
     for (int q = 0; q < a.spw; ++q) {
         const int base = s0 + q * S;                           // wave-uniform
         if (base >= a.c_end) break;
-        const bool sv = base + sub < a.c_end;
+        const bool sv = lok && base + sub < a.c_end;
         const int slot = sv ? base + sub : base;               // an idle lane group reads a valid row
         const bool rv = rok && sv;
         const uint32_t vo = (uint32_t)slot * rowb + roff;
@@ -1558,9 +1563,9 @@ __device__ __forceinline__ void xyz_sweep_lanes_block(const XyzArgs& a, int bb) 
     if constexpr (MEAS) {
         __shared__ double sMeas[kMeasWords * kBlock];
         if (!a.mrec)
-            book_measurement(sMeas, m, 1 << ls, 0, a);
+            book_measurement(sMeas, m, L, S, 0, a);
         else if (a.c_end > a.c_begin)
-            book_records(sMeas, m, 1 << ls, bb, a);
+            book_records(sMeas, m, L, S, bb, a);
     }
 }
 
@@ -2419,13 +2424,12 @@ static void launch_xyz_sweep_t(const XyzArgs& a, bool angle, bool lanes, hipStre
 }
 
 void launch_xyz_sweep(const XyzArgs& a, bool angle, bool, bool, hipStream_t st) {
-    const bool lanes = a.lps_shift < 6;
+    const bool lanes = a.lps < kWave;
     if (g_recorder) {
         // a sweep writing measurement records runs every unit of its stride,
         // so that each record the finalize reads is rewritten
         record(LK_XYZ_SWEEP, (a.meas ? 262144 : 0) | (angle ? 65536 : 0) | (lanes ? 131072 : 0) | a.D,
-               a.mrec ? a.nblocks : sweep_units(a.G, a.c_end - a.c_begin, a.spw * sites_per_step(a.lps_shift)),
-               a);
+               a.mrec ? a.nblocks : sweep_units(a.G, a.c_end - a.c_begin, a.spw * a.sps), a);
         return;
     }
     if (a.meas) launch_xyz_sweep_t<true>(a, angle, lanes, st);

@@ -478,8 +478,9 @@ def parse_gpu_ether(text: str) -> GpuTags:
                     raise EtherInputError("launch: sites per wave in {1,2,4,8,16}, XCD map 0, 1 or 2")
             elif key == "lanes":
                 g.lanes = int(v[0])
-                if g.lanes not in (0, 8, 16, 32, 64):
-                    raise EtherInputError("lanes: 0 (automatic), 8 (Ising only), 16, 32 or 64")
+                if not 0 <= g.lanes <= 64:
+                    raise EtherInputError("lanes: 0 (automatic) or 1 to 64 lanes per site (Ising: a power "
+                                          "of two from 8)")
             elif key == "graphs":
                 g.graphs = _logical(v[0])
             elif key == "compat":

@@ -392,12 +392,14 @@ void ether_batch_destroy(ether_batch* batch);
  * Results do not depend on the shape: every update draws from its own (site,
  * replica, sweep) counter. */
 int ether_set_launch(ether_ctx* ctx, int32_t sites_per_wave, int32_t xcd_map);
-/* Lanes of a wavefront given to one site in the sweep: 64 (one site per wave),
- * or 32 / 16 (8 for Ising, whose lanes hold 4 replicas each) so that a wave
- * updates 2 / 4 / 8 sites at once when a context has few replicas.  0 (the
- * default) picks the fewest lanes that hold all replicas.  Results do not
- * depend on it.  In ether_set_launch, sites_per_wave then counts steps of
- * 64 / lanes sites. */
+/* Lanes of a wavefront given to one site in the sweep when a context has few
+ * replicas: 64 (one site per wave), or fewer so that a wave updates
+ * 64 / lanes whole sites at once (lanes past the last whole site idle).
+ * XYZ: any count from the replica count to 64 (default: 16 or 32, the fewer
+ * that hold all replicas).  Ising, whose lanes hold 4 replicas each: 8, 16,
+ * 32 or 64 (default: the fewest that hold all replicas).  0 restores the
+ * default.  Results do not depend on it.  In
+ * ether_set_launch, sites_per_wave then counts steps of 64 / lanes sites. */
 int ether_set_lanes(ether_ctx* ctx, int32_t lanes_per_site);
 /* Autotuner: times `reps` sweeps for each of sites_per_wave {1,2,4,8,16} x
  * xcd_map {0,1,2} on the context's own lattice and replicas, in two rounds

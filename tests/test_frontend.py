@@ -26,7 +26,7 @@ def test_gpu_tags_and_defaults():
     assert parse_gpu_ether("ladder = linear").ladder == "linear"
 
 
-@pytest.mark.parametrize("text", ["pt = .T. 11", "ladder = geometric", "launch = 3 1", "lanes = 12",
+@pytest.mark.parametrize("text", ["pt = .T. 11", "ladder = geometric", "launch = 3 1", "lanes = 65",
                                   "compat = 16", "chunk = -1", "autotune", "ladder = learned 10 0"])
 def test_gpu_tags_errors(text):
     with pytest.raises(EtherInputError):

@@ -62,6 +62,9 @@ CASES = [
     ("sc_fields", dict(), 9, 0, 0, True),                      # Zeeman, SIA, ANGLE
     ("sc_fields", dict(angle=".F. 0"), 21, 32, 0, True),
     ("tri_afm_96", dict(sc="12 12 1"), 13, 16, 0, True),       # 3 colours: per-workgroup records
+    ("tri_afm_96", dict(sc="12 12 1"), 20, 20, 0, True),       # the same, 20 lanes: 3 sites per step
+    ("sc_heis_afm_32", dict(sc="8 8 8"), 19, 19, 1, True),     # 19 lanes, 3 sites per step (atomics)
+    ("sc_fields", dict(), 9, 9, 0, True),                      # 9 lanes, 7 sites per step
     ("tri_afm_96", dict(sc="12 12 1"), 70, 0, 0, True),        # two waves of replicas per site
     ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11"), 24, 0, 1, True),   # 4 colours, 18 neighbours
     ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11"), 24, 32, 1, True),  # the same, lanes kernel
@@ -113,10 +116,10 @@ def test_fused_measurement_matches_pass_and_oracle(name, over, R, lanes, ovrr, b
 
 @pytest.mark.parametrize("name,over,R,lanes_list,ovrr", [
     ("sc_heis_afm_32", dict(sc="8 8 8"), 70, [0], 1),
-    ("sc_fields", dict(), 13, [64, 16, 32], 0),
-    ("tri_afm_96", dict(sc="12 12 1"), 13, [64, 16, 32], 0),
+    ("sc_fields", dict(), 13, [64, 16, 32, 0, 20, 13], 0),      # 20 lanes: 3 sites per step
+    ("tri_afm_96", dict(sc="12 12 1"), 13, [64, 16, 32, 0, 13], 0),
     ("tri_afm_96", dict(sc="12 12 1"), 130, [0], 0),
-    ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11"), 24, [64, 32], 1),
+    ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11"), 24, [64, 32, 0, 25], 1),
 ])
 def test_fused_measurement_does_not_depend_on_launch_shape(name, over, R, lanes_list, ovrr):
     say(f"fused measurement shapes {name} R={R}")

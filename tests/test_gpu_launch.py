@@ -37,8 +37,10 @@ def _spins(eng, R):
 @pytest.mark.parametrize("name,over,R,lanes", [
     ("sc_heis_afm_32", dict(sc="8 8 8"), 70, [0]),
     ("sc_ising_16", dict(sc="8 8 8"), 300, [0]),
-    ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11"), 24, [64, 32, 0]),          # 0: 32 lanes
-    ("tri_afm_96", dict(sc="12 12 1"), 13, [64, 16, 32]),                    # ANGLE, 3 colours
+    ("pyro_pt_16", dict(sc="2 2 2", pt=".F. 11"), 24, [64, 32, 0, 25]),      # 25 lanes: 2 sites per step
+    ("tri_afm_96", dict(sc="12 12 1"), 13, [64, 16, 32, 0, 21, 13]),         # ANGLE, 3 colours; 13 and 21 lanes
+    ("sc_heis_afm_32", dict(sc="6 6 6"), 20, [64, 32, 0, 20, 8 * 3]),       # 20 lanes: 3 sites per step
+    ("sc_heis_afm_32", dict(sc="4 4 4"), 8, [0, 64, 8, 11]),                # 8 lanes: 8 sites per step
     ("sc_ising_16", dict(sc="6 6 6"), 30, [64, 8, 16, 32]),
 ])
 def test_sweeps_do_not_depend_on_launch_shape(name, over, R, lanes):

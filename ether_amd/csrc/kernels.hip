@@ -23,6 +23,7 @@
 // wave-uniform (scalar loads); every spin access of the wave is one contiguous
 // segment of the replica-major planes.  None of this is GEMM-shaped; the
 // kernels are bound by HBM/L2 traffic and by the VALU work of the generator.
+#include <atomic>
 #include <hip/hip_runtime.h>
 
 #include "engine_common.h"
@@ -2608,11 +2609,14 @@ static void launch_xyz_sweep_batch_t(const XyzArgs* a, const int2* d_items, int 
 // it ahead of a graph capture)
 template <int DM, int NC>
 static void grant_resident1_batch(int lds) {
-    static int granted = 0;
-    if (lds > granted &&
+    constexpr int kMaxDevices = 64;
+    static std::atomic<int> granted[kMaxDevices];   // per device: the attribute is the current device's
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = 0;
+    if (lds > granted[dev].load() &&
         hipFuncSetAttribute(reinterpret_cast<const void*>(&k_ising_resident1_batch<DM, NC, true>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, lds) == hipSuccess)
-        granted = lds;
+        granted[dev].store(lds);
 }
 template <int DM, int NC>
 static void launch_resident1_batch(const IsingResArgs* a, const int2* d_items, dim3 grid, dim3 blk, int lds,

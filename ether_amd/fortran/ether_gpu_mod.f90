@@ -267,6 +267,30 @@ module ether_gpu
          type(c_ptr), value :: batch
       end subroutine
 
+      integer(c_int) function ether_batch_set_graphs(batch, on) bind(C, name="ether_batch_set_graphs")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: batch
+         integer(c_int), value :: on
+      end function
+
+      integer(c_int) function ether_batch_get_resident(batch, on) bind(C, name="ether_batch_get_resident")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: batch
+         integer(c_int), intent(out) :: on
+      end function
+
+      integer(c_int) function ether_set_resident(ctx, mode) bind(C, name="ether_set_resident")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: mode
+      end function
+
+      integer(c_int) function ether_get_resident(ctx, kind) bind(C, name="ether_get_resident")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), intent(out) :: kind
+      end function
+
       integer(c_int) function ether_pt_state(ctx, slot_of, acc, att, trips) bind(C, name="ether_pt_state")
          import :: c_int, c_ptr, c_int64_t
          type(c_ptr), value :: ctx

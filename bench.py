@@ -191,6 +191,10 @@ def main():
     ap.add_argument("--no-sq", action="store_true", help="skip the structure-factor timing")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched uneven workload")
     ap.add_argument("--batch-mcs", type=int, default=100)
+    ap.add_argument("--batch-kinds", default="",
+                    help="comma-separated lattice kinds of the batch section (sc_ising, sc_heis, tri, pyro; default all)")
+    ap.add_argument("--no-batch-plain", action="store_true",
+                    help="skip the batch section's one-launch-per-lattice leg")
     ap.add_argument("--sq-calls", type=int, default=5)
     ap.add_argument("--no-ising", action="store_true", help="skip the Ising simple-cubic section")
     ap.add_argument("--ising-steps", type=int, default=2000)
@@ -234,7 +238,8 @@ def main():
     if not args.no_sq:
         out["sq"] = sq_bench(args.sq_calls)
     if not args.no_batch:
-        out["batch"] = batch_bench(args.batch_mcs)
+        out["batch"] = batch_bench(args.batch_mcs, kinds=[k for k in args.batch_kinds.split(",") if k],
+                                   plain=not args.no_batch_plain)
     if not args.no_ising and args.workload != "sc_ising_16":
         ising = measure("sc_ising_16", args.ising_steps, 20, 100, False, args)
         out["ising"] = {k: ising[k] for k in ("value", "unit", "ms_per_step", "config", "roofline",
@@ -493,7 +498,7 @@ def sq_bench(calls):
             "error_note": "max |S_gpu - S_fp64| / max S_fp64 over 4 replicas"}
 
 
-def batch_bench(mcs, n_lattices=256):
+def batch_bench(mcs, n_lattices=256, kinds=None, plain=True):
     """The batched uneven workload: 256 lattices (simple-cubic Ising and
     Heisenberg, triangular, pyrochlore; 5.35 M spins, 8-32 replicas each,
     114 M spin updates per MCS), each with its own tempering ladder (swaps
@@ -501,14 +506,16 @@ def batch_bench(mcs, n_lattices=256):
     Heisenberg) issue one launch per kernel kind and template for all their
     lattices, whole periods replayed as graphs.  Plain: the same engines run
     one after another with ether_run (one launch per lattice per kernel, each
-    with its own graphs).  Both time `mcs` MCS after a 10-MCS warmup."""
+    with its own graphs).  Both time `mcs` MCS after a 10-MCS warmup.
+    `kinds` keeps only those lattice kinds and plain=False skips the plain
+    leg (for profiles of the batched kernels)."""
     import numpy as np
     from ether_amd import engine as E
     from ether_amd import workloads
     from ether_amd.driver import attach_neighbours, make_engine
     from concurrent.futures import ThreadPoolExecutor
     t_setup = time.perf_counter()
-    specs = workloads.c5_lattices(1992, n_lattices)
+    specs = [sp for sp in workloads.c5_lattices(1992, n_lattices) if not kinds or sp["kind"] in kinds]
 
     def one(sp):
         m = workloads.c5_model(sp)
@@ -552,13 +559,15 @@ def batch_bench(mcs, n_lattices=256):
     sync()
     t_batched = time.perf_counter() - t0
     step += mcs
-    run_plain(step, 10)
-    step += 10
-    sync()
-    t0 = time.perf_counter()
-    run_plain(step, mcs)
-    sync()
-    t_plain = time.perf_counter() - t0
+    t_plain = None
+    if plain:
+        run_plain(step, 10)
+        step += 10
+        sync()
+        t0 = time.perf_counter()
+        run_plain(step, mcs)
+        sync()
+        t_plain = time.perf_counter() - t0
     for b in batches:
         b.close()
     for e in allg:
@@ -567,9 +576,10 @@ def batch_bench(mcs, n_lattices=256):
                         f"24..128, pyrochlore 4..12 cells), {int(updates)} spin updates per MCS, own "
                         f"ladders of 8-32 replicas, swaps every 5 MCS, sample every 10 MCS; {mcs} MCS",
             "batched_spin_updates_per_s": updates * mcs / t_batched,
-            "plain_spin_updates_per_s": updates * mcs / t_plain,
-            "speedup": t_plain / t_batched,
-            "batched_ms_per_mcs": t_batched / mcs * 1e3, "plain_ms_per_mcs": t_plain / mcs * 1e3,
+            "plain_spin_updates_per_s": updates * mcs / t_plain if t_plain else None,
+            "speedup": t_plain / t_batched if t_plain else None,
+            "batched_ms_per_mcs": t_batched / mcs * 1e3,
+            "plain_ms_per_mcs": t_plain / mcs * 1e3 if t_plain else None,
             "lattices": {k: len(v) for k, v in engines.items()}, "setup_s": setup_s}
 
 

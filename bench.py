@@ -538,13 +538,18 @@ def batch_bench(mcs, n_lattices=256, kinds=None, plain=True):
     batches = [E.Batch(v) for v in engines.values() if v]
     sched = dict(mcs_eq=0, coa=10, pt=5)
 
+    swaps = [0] * len(batches)
+
     def run_batched(first, n):
-        for b in batches:
-            b.run(first, n, sched["mcs_eq"], sched["coa"], 0, 0.0, sched["pt"])
+        # each batch's swap count carried from call to call, as a driver does
+        for k, b in enumerate(batches):
+            swaps[k] = b.run(first, n, sched["mcs_eq"], sched["coa"], 0, 0.0, sched["pt"], swaps[k])
+
+    swaps_plain = [0] * len(allg)
 
     def run_plain(first, n):
-        for e in allg:
-            e.run(first, n, sched["mcs_eq"], sched["coa"], 0, 0.0, sched["pt"])
+        for k, e in enumerate(allg):
+            swaps_plain[k] = e.run(first, n, sched["mcs_eq"], sched["coa"], 0, 0.0, sched["pt"], swaps_plain[k])
 
     def sync():
         for e in allg:

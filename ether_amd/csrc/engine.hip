@@ -149,6 +149,11 @@ struct ether_ctx {
     std::vector<double> beta_h;
     uint32_t step = 0, init_count = 0, pt_count = 0;
     hipStream_t stream = nullptr;
+    // a batch's completion that work on this context must wait for, made a
+    // wait on the stream at the context's next use (cst): made at once, it
+    // would put a barrier on a hardware queue that another batch's stream may
+    // share and hold that batch back
+    hipEvent_t after = nullptr;
     // device counters d_ctr = {sweep base, swap base, swap count} and their host
     // mirrors; kernels use counter + baked offset so a captured graph replays
     uint32_t* d_ctr = nullptr;
@@ -176,6 +181,16 @@ struct ether_ctx {
     void* mlp = nullptr;
     void* sq = nullptr;
 };
+
+// The context's stream, after the wait for a batch that ran the context
+// (ether_batch_run leaves the wait pending until the context is next used).
+static inline hipStream_t cst(ether_ctx* c) {
+    if (c->after) {
+        (void)hipStreamWaitEvent(c->stream, c->after, 0);
+        c->after = nullptr;
+    }
+    return c->stream;
+}
 
 // --------------------------------------------------------------------------
 // helpers
@@ -209,8 +224,8 @@ static int dev_alloc(ether_ctx* c, T** p, size_t n) {
 // and the callers pass buffers that are freed as soon as they return.
 template <class T>
 static int upload(ether_ctx* c, T* d, const T* h, size_t n) {
-    HIPCHK(c, hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, cst(c)));
+    HIPCHK(c, hipStreamSynchronize(cst(c)));
     return ETHER_OK;
 }
 
@@ -245,11 +260,11 @@ class Arena {
             image_.assign(tables_end_, 0);
             for (const Src& s : src_) std::memcpy(image_.data() + s.off, s.h, s.bytes);
             // the image is freed with the Arena: wait for the copy
-            HIPCHK(c, hipMemcpyAsync(base, image_.data(), tables_end_, hipMemcpyHostToDevice, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
+            HIPCHK(c, hipMemcpyAsync(base, image_.data(), tables_end_, hipMemcpyHostToDevice, cst(c)));
+            HIPCHK(c, hipStreamSynchronize(cst(c)));
         }
         if (zero_end_ > tables_end_)
-            HIPCHK(c, hipMemsetAsync(base + tables_end_, 0, zero_end_ - tables_end_, c->stream));
+            HIPCHK(c, hipMemsetAsync(base + tables_end_, 0, zero_end_ - tables_end_, cst(c)));
         return ETHER_OK;
     }
   private:
@@ -276,7 +291,7 @@ class Arena {
 static int copy_acc_async(ether_ctx* c, std::vector<unsigned long long>& raw) {
     raw.resize((size_t)kAccCopies * c->R);
     HIPCHK(c, hipMemcpyAsync(raw.data(), c->d_acc, sizeof(unsigned long long) * raw.size(),
-                             hipMemcpyDeviceToHost, c->stream));
+                             hipMemcpyDeviceToHost, cst(c)));
     return ETHER_OK;
 }
 static unsigned long long acc_total(const std::vector<unsigned long long>& raw, int R, int r) {
@@ -1018,17 +1033,17 @@ extern "C" int ether_commit(ether_ctx* c) {
     TRY(ar.commit(c));
     if (ising) {
         // ordered start (all +1), padding replicas included
-        HIPCHK(c, hipMemsetAsync(c->d_s, 1, RN, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_s, 1, RN, cst(c)));
     } else {
         // every spin (and padding lane) along +z
-        HIPCHK(c, hipMemsetAsync(c->d_sx, 0, RN * 4, c->stream));
-        HIPCHK(c, hipMemsetAsync(c->d_sy, 0, RN * 4, c->stream));
-        HIPCHK(c, hipMemsetD32Async(c->d_sz, 0x3f800000, RN, c->stream));
-        if (c->angle) HIPCHK(c, hipMemsetAsync(c->d_phi, 0, RN * 4, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_sx, 0, RN * 4, cst(c)));
+        HIPCHK(c, hipMemsetAsync(c->d_sy, 0, RN * 4, cst(c)));
+        HIPCHK(c, hipMemsetD32Async(c->d_sz, 0x3f800000, RN, cst(c)));
+        if (c->angle) HIPCHK(c, hipMemsetAsync(c->d_phi, 0, RN * 4, cst(c)));
     }
-    launch_set_ctr(c->d_ctr, 0, 0, 0, c->stream);
+    launch_set_ctr(c->d_ctr, 0, 0, 0, cst(c));
     c->dev_step = c->dev_pt = c->dev_sc = 0;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamSynchronize(cst(c)));
     c->committed = true;
     return ETHER_OK;
 }
@@ -1107,7 +1122,7 @@ static int upload_betas(ether_ctx* c, const double* beta) {
 extern "C" int ether_retune_betas(ether_ctx* c, const double* beta) {
     TRY(need_betas(c));
     TRY(upload_betas(c, beta));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamSynchronize(cst(c)));
     return ETHER_OK;
 }
 
@@ -1118,7 +1133,7 @@ extern "C" int ether_set_betas(ether_ctx* c, const double* beta) {
     for (int r = 0; r < c->R; ++r) ident[r] = r;
     TRY(upload(c, c->d_slot_of, ident.data(), (size_t)c->R));
     TRY(upload(c, c->d_rep_at, ident.data(), (size_t)c->R));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamSynchronize(cst(c)));
     c->betas_set = true;
     return ETHER_OK;
 }
@@ -1137,11 +1152,11 @@ extern "C" int ether_set_ladders(ether_ctx* c, int32_t n_groups, const int32_t* 
     }
     drop_graphs(c);   // the ladder count is baked into captured swap nodes
     c->n_groups = n_groups;
-    HIPCHK(c, hipMemsetAsync(c->d_pt_acc, 0, sizeof(unsigned long long) * c->R, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->d_pt_att, 0, sizeof(unsigned long long) * c->R, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->d_trips, 0, sizeof(unsigned long long) * c->R, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->d_trip_state, 0, sizeof(int) * c->R, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_pt_acc, 0, sizeof(unsigned long long) * c->R, cst(c)));
+    HIPCHK(c, hipMemsetAsync(c->d_pt_att, 0, sizeof(unsigned long long) * c->R, cst(c)));
+    HIPCHK(c, hipMemsetAsync(c->d_trips, 0, sizeof(unsigned long long) * c->R, cst(c)));
+    HIPCHK(c, hipMemsetAsync(c->d_trip_state, 0, sizeof(int) * c->R, cst(c)));
+    HIPCHK(c, hipStreamSynchronize(cst(c)));
     return ETHER_OK;
 }
 
@@ -1301,17 +1316,17 @@ static int ev_begin(ether_ctx* c, int kind, int count = 1) {
     const int k = (int)c->ev_used++;
     c->ev_kind[k] = kind;
     c->ev_count[k] = count;
-    (void)hipEventRecord(c->ev_pool[k].first, c->stream);
+    (void)hipEventRecord(c->ev_pool[k].first, cst(c));
     return k;
 }
 
 static void ev_end(ether_ctx* c, int k) {
-    if (k >= 0) (void)hipEventRecord(c->ev_pool[k].second, c->stream);
+    if (k >= 0) (void)hipEventRecord(c->ev_pool[k].second, cst(c));
 }
 
 extern "C" int ether_set_timing(ether_ctx* c, int32_t on) {
     if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamSynchronize(cst(c)));
     c->timing = on != 0;
     c->ev_used = 0;
     return ETHER_OK;
@@ -1319,7 +1334,7 @@ extern "C" int ether_set_timing(ether_ctx* c, int32_t on) {
 
 extern "C" int ether_get_timing(ether_ctx* c, int32_t kind, double* total_ms, int64_t* launches) {
     if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamSynchronize(cst(c)));
     double tot = 0.0;
     int64_t n = 0;
     for (size_t k = 0; k < c->ev_used; ++k) {
@@ -1336,7 +1351,7 @@ extern "C" int ether_get_timing(ether_ctx* c, int32_t kind, double* total_ms, in
 
 extern "C" int ether_synchronize(ether_ctx* c) {
     if (!c) return fail(nullptr, ETHER_ERR_ARG, "null context");
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamSynchronize(cst(c)));
     return ETHER_OK;
 }
 
@@ -1383,7 +1398,7 @@ static int enqueue_sweep(ether_ctx* c, double fuse_frac = 0.0, bool meas = false
         if (c->model == ETHER_MODEL_ISING) {
             IsingArgs a = ising_args(c, qq);
             if (empty) a.c_begin = a.c_end = 0;
-            launch_ising_sweep(a, c->stream);
+            launch_ising_sweep(a, cst(c));
         } else {
             XyzArgs a = xyz_args(c, qq);
             if (empty) a.c_begin = a.c_end = 0;
@@ -1421,7 +1436,7 @@ static int enqueue_sweep(ether_ctx* c, double fuse_frac = 0.0, bool meas = false
                     set_blocks(a, a.G, a.spw * a.sps);
                 }
             }
-            launch_xyz_sweep(a, c->angle, c->zee, c->sia, c->stream);
+            launch_xyz_sweep(a, c->angle, c->zee, c->sia, cst(c));
         }
     }
     ev_end(c, ev);
@@ -1446,7 +1461,7 @@ static int enqueue_ovrr(ether_ctx* c, double passes) {
             a.pass = (uint32_t)pass;
             a.frac = (p < full) ? 1.0f : (float)frac;
             const int ev = ev_begin(c, ETHER_TIME_OVRR);
-            launch_xyz_ovrr(a, c->angle, c->zee, c->stream);
+            launch_xyz_ovrr(a, c->angle, c->zee, cst(c));
             ev_end(c, ev);
         }
     }
@@ -1464,11 +1479,11 @@ enum Booked : int { kBookedPass = 0, kBookedXyz = 1, kBookedResident = 2 };
 static int enqueue_measure(ether_ctx* c, int sample, int booked = kBookedPass) {
     const int ev = ev_begin(c, ETHER_TIME_MEASURE);
     if (booked == kBookedPass) {
-        launch_measure(meas_args(c), c->model, c->zee, c->sia, c->stream);
+        launch_measure(meas_args(c), c->model, c->zee, c->sia, cst(c));
     } else if (g_recorder) {
         MeasArgs m = meas_args(c);
         m.tiles = 0;
-        launch_measure(m, c->model, c->zee, c->sia, c->stream);
+        launch_measure(m, c->model, c->zee, c->sia, cst(c));
     }
     FinArgs f;
     f.partial = c->d_partial;
@@ -1500,7 +1515,7 @@ static int enqueue_measure(ether_ctx* c, int sample, int booked = kBookedPass) {
     f.mrec_groups = c->Rp / kWave;
     f.mrec_pp = c->mrec_pp;
     for (int k = 0; k < 4; ++k) f.mscale[k] = c->mscale[k];
-    launch_finalize(f, c->stream);
+    launch_finalize(f, cst(c));
     ev_end(c, ev);
     HIPCHK(c, hipGetLastError());
     return ETHER_OK;
@@ -1528,7 +1543,7 @@ static int enqueue_pt(ether_ctx* c, int parity, int booked = kBookedPass) {
     p.trip_state = c->d_trip_state;
     p.n_accepted = c->d_nacc;
     const int ev = ev_begin(c, ETHER_TIME_PT);
-    launch_pt_swap(p, c->stream);
+    launch_pt_swap(p, cst(c));
     ev_end(c, ev);
     HIPCHK(c, hipGetLastError());
     return ETHER_OK;
@@ -1554,7 +1569,7 @@ extern "C" int ether_init_spins(ether_ctx* c, int32_t mode) {
     a.dphi = (float)c->dphi;
     a.seed = c->seed;
     a.step = c->init_count++;
-    launch_init(a, c->stream);
+    launch_init(a, cst(c));
     HIPCHK(c, hipGetLastError());
     return ETHER_OK;
 }
@@ -1581,7 +1596,7 @@ extern "C" int ether_sweep(ether_ctx* c, int32_t n_mcs, double* accept) {
     if (accept && n_mcs > 0) {
         std::vector<unsigned long long> after;
         TRY(copy_acc_async(c, after));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipStreamSynchronize(cst(c)));
         for (int r = 0; r < c->R; ++r)
             accept[r] = (double)(acc_total(after, c->R, r) - acc_total(before, c->R, r)) / c->N;
     }
@@ -1637,7 +1652,7 @@ extern "C" int ether_autotune(ether_ctx* c, int32_t reps, int32_t* sites_per_wav
     {
         size_t off = 0;
         for (auto& p : planes) {
-            if (hipMemcpyAsync(save + off, p.first, p.second, hipMemcpyDeviceToDevice, c->stream) !=
+            if (hipMemcpyAsync(save + off, p.first, p.second, hipMemcpyDeviceToDevice, cst(c)) !=
                 hipSuccess)
                 rc = fail(c, ETHER_ERR_HIP, "ether_autotune: save copy failed");
             off += (p.second + 255) / 256 * 256;
@@ -1653,9 +1668,9 @@ extern "C" int ether_autotune(ether_ctx* c, int32_t reps, int32_t* sites_per_wav
             c->spw = kSpw[i % n_spw];
             c->xcd_map = i / n_spw;
             if ((rc = enqueue_sweep(c))) break;   // warm-up of this shape
-            (void)hipEventRecord(e0, c->stream);
+            (void)hipEventRecord(e0, cst(c));
             for (int k = 0; k < reps && !rc; ++k) rc = enqueue_sweep(c);
-            (void)hipEventRecord(e1, c->stream);
+            (void)hipEventRecord(e1, cst(c));
             if (rc) break;
             if (hipEventSynchronize(e1) != hipSuccess) {
                 rc = fail(c, ETHER_ERR_HIP, "ether_autotune: sweep failed");
@@ -1684,11 +1699,11 @@ extern "C" int ether_autotune(ether_ctx* c, int32_t reps, int32_t* sites_per_wav
     {
         size_t off = 0;
         for (auto& p : planes) {
-            (void)hipMemcpyAsync(p.first, save + off, p.second, hipMemcpyDeviceToDevice, c->stream);
+            (void)hipMemcpyAsync(p.first, save + off, p.second, hipMemcpyDeviceToDevice, cst(c));
             off += (p.second + 255) / 256 * 256;
         }
     }
-    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(cst(c));
     (void)hipFree(save);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -1719,11 +1734,11 @@ extern "C" int ether_measure(ether_ctx* c, int32_t sample, double* E, double* M)
     if (E || M) {
         std::vector<double> parts(4 * (size_t)c->R);
         HIPCHK(c, hipMemcpyAsync(parts.data(), c->d_parts, sizeof(double) * 4 * c->R,
-                                 hipMemcpyDeviceToHost, c->stream));
+                                 hipMemcpyDeviceToHost, cst(c)));
         if (M)
             HIPCHK(c, hipMemcpyAsync(M, c->d_M, sizeof(double) * 3 * c->R, hipMemcpyDeviceToHost,
-                                     c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+                                     cst(c)));
+        HIPCHK(c, hipStreamSynchronize(cst(c)));
         if (E)
             for (int r = 0; r < c->R; ++r) E[r] = parts[4 * r] + parts[4 * r + 1] + parts[4 * r + 2];
     }
@@ -1734,18 +1749,18 @@ extern "C" int ether_get_energy_parts(ether_ctx* c, double* parts, int64_t* bond
     TRY(need_committed(c));
     if (parts)
         HIPCHK(c, hipMemcpyAsync(parts, c->d_parts, sizeof(double) * 4 * c->R, hipMemcpyDeviceToHost,
-                                 c->stream));
+                                 cst(c)));
     if (n_classes) *n_classes = c->n_classes;
     if (bonds && c->model == ETHER_MODEL_ISING) {
         std::vector<long long> b((size_t)c->R * (c->n_classes + 1));
         HIPCHK(c, hipMemcpyAsync(b.data(), c->d_bonds, sizeof(long long) * b.size(),
-                                 hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+                                 hipMemcpyDeviceToHost, cst(c)));
+        HIPCHK(c, hipStreamSynchronize(cst(c)));
         for (int r = 0; r < c->R; ++r)
             for (int k = 1; k <= c->n_classes; ++k)
                 bonds[(size_t)r * c->n_classes + (k - 1)] = b[(size_t)r * (c->n_classes + 1) + k];
     }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamSynchronize(cst(c)));
     return ETHER_OK;
 }
 
@@ -1754,9 +1769,9 @@ extern "C" int ether_read_moments(ether_ctx* c, double* moments) {
     if (!moments) return fail(c, ETHER_ERR_ARG, "ether_read_moments: null pointer");
     std::vector<unsigned long long> acc;
     HIPCHK(c, hipMemcpyAsync(moments, c->d_mom, sizeof(double) * ETHER_MOM_COUNT * c->R,
-                             hipMemcpyDeviceToHost, c->stream));
+                             hipMemcpyDeviceToHost, cst(c)));
     TRY(copy_acc_async(c, acc));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamSynchronize(cst(c)));
     for (int r = 0; r < c->R; ++r)
         moments[ETHER_MOM_COUNT * r + ETHER_MOM_ACC] = (double)acc_total(acc, c->R, r) / c->N;
     return ETHER_OK;
@@ -1764,10 +1779,10 @@ extern "C" int ether_read_moments(ether_ctx* c, double* moments) {
 
 extern "C" int ether_reset_moments(ether_ctx* c) {
     TRY(need_committed(c));
-    HIPCHK(c, hipMemsetAsync(c->d_mom, 0, sizeof(double) * ETHER_MOM_COUNT * c->R, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_mom, 0, sizeof(double) * ETHER_MOM_COUNT * c->R, cst(c)));
     if (c->d_mom_sp)
-        HIPCHK(c, hipMemsetAsync(c->d_mom_sp, 0, sizeof(double) * 3 * c->R, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->d_acc, 0, sizeof(unsigned long long) * kAccCopies * c->R, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->d_mom_sp, 0, sizeof(double) * 3 * c->R, cst(c)));
+    HIPCHK(c, hipMemsetAsync(c->d_acc, 0, sizeof(unsigned long long) * kAccCopies * c->R, cst(c)));
     return ETHER_OK;
 }
 
@@ -1813,21 +1828,21 @@ extern "C" int ether_read_spiral(ether_ctx* c, double* moments, double* psi) {
     if (!c->spiral_cap) return fail(c, ETHER_ERR_STATE, "ether_read_spiral: no spiral partners set");
     if (moments)
         HIPCHK(c, hipMemcpyAsync(moments, c->d_mom_sp, sizeof(double) * 3 * c->R, hipMemcpyDeviceToHost,
-                                 c->stream));
+                                 cst(c)));
     if (psi)
-        HIPCHK(c, hipMemcpyAsync(psi, c->d_psi, sizeof(double) * c->R, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpyAsync(psi, c->d_psi, sizeof(double) * c->R, hipMemcpyDeviceToHost, cst(c)));
+    HIPCHK(c, hipStreamSynchronize(cst(c)));
     return ETHER_OK;
 }
 
 extern "C" int ether_pt_swap(ether_ctx* c, int32_t parity, int32_t* n_accepted) {
     TRY(need_betas(c));
     if (c->n_groups <= 0) return fail(c, ETHER_ERR_STATE, "ether_pt_swap: no ladders set");
-    HIPCHK(c, hipMemsetAsync(c->d_nacc, 0, sizeof(int), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_nacc, 0, sizeof(int), cst(c)));
     TRY(enqueue_pt(c, parity));
     if (n_accepted) {
-        HIPCHK(c, hipMemcpyAsync(n_accepted, c->d_nacc, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemcpyAsync(n_accepted, c->d_nacc, sizeof(int), hipMemcpyDeviceToHost, cst(c)));
+        HIPCHK(c, hipStreamSynchronize(cst(c)));
     }
     return ETHER_OK;
 }
@@ -1837,17 +1852,17 @@ extern "C" int ether_pt_state(ether_ctx* c, int32_t* slot_of, int64_t* acc, int6
     TRY(need_committed(c));
     if (slot_of)
         HIPCHK(c, hipMemcpyAsync(slot_of, c->d_slot_of, sizeof(int) * c->R, hipMemcpyDeviceToHost,
-                                 c->stream));
+                                 cst(c)));
     if (acc)
         HIPCHK(c, hipMemcpyAsync(acc, c->d_pt_acc, sizeof(int64_t) * c->R, hipMemcpyDeviceToHost,
-                                 c->stream));
+                                 cst(c)));
     if (att)
         HIPCHK(c, hipMemcpyAsync(att, c->d_pt_att, sizeof(int64_t) * c->R, hipMemcpyDeviceToHost,
-                                 c->stream));
+                                 cst(c)));
     if (trips)
         HIPCHK(c, hipMemcpyAsync(trips, c->d_trips, sizeof(int64_t) * c->R, hipMemcpyDeviceToHost,
-                                 c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+                                 cst(c)));
+    HIPCHK(c, hipStreamSynchronize(cst(c)));
     return ETHER_OK;
 }
 
@@ -1920,9 +1935,9 @@ static int enqueue_resident(ether_ctx* c, int n_sweeps, bool meas) {
     a.partial = c->d_partial;
     const int ev = ev_begin(c, ETHER_TIME_RESIDENT, n_sweeps);
     if (g_recorder)
-        launch_ising_resident(a, c->res1b_block, c->res1b_lds, true, c->stream);
+        launch_ising_resident(a, c->res1b_block, c->res1b_lds, true, cst(c));
     else
-        launch_ising_resident(a, c->res_block, per_rep ? c->res1_lds : c->res_lds, per_rep, c->stream);
+        launch_ising_resident(a, c->res_block, per_rep ? c->res1_lds : c->res_lds, per_rep, cst(c));
     ev_end(c, ev);
     HIPCHK(c, hipGetLastError());
     c->step += (uint32_t)n_sweeps;
@@ -1959,7 +1974,7 @@ static int enqueue_steps(ether_ctx* c, int32_t first, int32_t count, int32_t mcs
 
 static void drop_graphs(ether_ctx* c) {
     c->graph_gen++;
-    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(cst(c));
     for (auto& g : c->graphs) {
         (void)hipGraphExecDestroy(g.exec);
         (void)hipGraphDestroy(g.graph);
@@ -1985,7 +2000,7 @@ static int graph_for(ether_ctx* c, int phase, int P, int coa, int ovrr_iv, doubl
     c->capturing = true;
     c->cap_step0 = step0;
     c->cap_pt0 = pt0;
-    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    HIPCHK(c, hipStreamBeginCapture(cst(c), hipStreamCaptureModeThreadLocal));
     int rc = ETHER_OK, n_pt = 0, sc_local = 0;
     {
         // position k in the period stands for every stepi = k (mod P)
@@ -1993,9 +2008,9 @@ static int graph_for(ether_ctx* c, int phase, int P, int coa, int ovrr_iv, doubl
         rc = enqueue_steps(c, 1, P, eq, coa, ovrr_iv, ovrr_passes, pt_iv, &sc_local);
         n_pt = sc_local;
     }
-    if (rc == ETHER_OK) launch_advance_ctr(c->d_ctr, (uint32_t)P, (uint32_t)n_pt, c->stream);
+    if (rc == ETHER_OK) launch_advance_ctr(c->d_ctr, (uint32_t)P, (uint32_t)n_pt, cst(c));
     hipGraph_t graph = nullptr;
-    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    hipError_t e = hipStreamEndCapture(cst(c), &graph);
     c->capturing = false;
     c->step = step0;
     c->pt_count = pt0;
@@ -2018,12 +2033,12 @@ static int graph_for(ether_ctx* c, int phase, int P, int coa, int ovrr_iv, doubl
 static int replay_graph(ether_ctx* c, int gi, int* sc) {
     const auto& g = c->graphs[gi];
     if (c->dev_step != c->step || c->dev_pt != c->pt_count || c->dev_sc != (uint32_t)*sc) {
-        launch_set_ctr(c->d_ctr, c->step, c->pt_count, (uint32_t)*sc, c->stream);
+        launch_set_ctr(c->d_ctr, c->step, c->pt_count, (uint32_t)*sc, cst(c));
         c->dev_step = c->step;
         c->dev_pt = c->pt_count;
         c->dev_sc = (uint32_t)*sc;
     }
-    HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
+    HIPCHK(c, hipGraphLaunch(g.exec, cst(c)));
     c->step += (uint32_t)g.P;
     c->dev_step += (uint32_t)g.P;
     c->pt_count += (uint32_t)g.n_pt;
@@ -2111,9 +2126,9 @@ extern "C" int ether_run_prepare(ether_ctx* c, int32_t first_step, int32_t n_ste
         if (!want[phase]) continue;
         int gi = -1;
         TRY(graph_for(c, phase, (int)P, coa, ovrr_iv, ovrr_passes, pt_iv, &gi));
-        HIPCHK(c, hipGraphUpload(c->graphs[gi].exec, c->stream));
+        HIPCHK(c, hipGraphUpload(c->graphs[gi].exec, cst(c)));
     }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamSynchronize(cst(c)));
     return ETHER_OK;
 }
 
@@ -2318,6 +2333,8 @@ extern "C" int ether_batch_create(ether_ctx* const* ctxs, int32_t n, ether_batch
 
 extern "C" void ether_batch_destroy(ether_batch* b) {
     if (!b) return;
+    for (ether_ctx* c : b->ctx)
+        if (c->after && c->after == b->done) (void)cst(c);   // the wait, before the event goes
     if (b->stream) batch_drop_graphs(b);
     (void)hipFree(b->d_arena);
     (void)hipFree(b->d_ctrs);
@@ -2538,7 +2555,7 @@ extern "C" int ether_batch_run(ether_batch* b, int32_t first_step, int32_t n_ste
         b->resident = resident;
     }
     for (ether_ctx* c : b->ctx) {
-        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamSynchronize(cst(c));
         c->rec_ncol = b->max_ncol;
     }
     const int ovrr_iv = do_ovrr ? ovrr_interval : 0;
@@ -2570,7 +2587,7 @@ extern "C" int ether_batch_run(ether_batch* b, int32_t first_step, int32_t n_ste
     (void)hipEventRecord(b->done, b->stream);
     for (ether_ctx* c : b->ctx) {
         c->rec_ncol = 0;
-        (void)hipStreamWaitEvent(c->stream, b->done, 0);
+        c->after = b->done;
     }
     if (rc) return rc;
     if (swap_count) *swap_count = sc;
@@ -2586,17 +2603,17 @@ extern "C" int ether_get_spins(ether_ctx* c, int32_t replica, double* S, double*
     const int N = c->N;
     float* buf = c->d_scratch_h;   // 4N floats: x, y, z, phi planes of one replica
     if (c->model == ETHER_MODEL_ISING) {
-        launch_extract(nullptr, c->d_s, N, c->Rp, replica, buf, c->stream);
+        launch_extract(nullptr, c->d_s, N, c->Rp, replica, buf, cst(c));
     } else {
-        launch_extract(c->d_sx, nullptr, N, c->Rp, replica, buf, c->stream);
-        launch_extract(c->d_sy, nullptr, N, c->Rp, replica, buf + N, c->stream);
-        launch_extract(c->d_sz, nullptr, N, c->Rp, replica, buf + 2 * (size_t)N, c->stream);
-        if (c->angle) launch_extract(c->d_phi, nullptr, N, c->Rp, replica, buf + 3 * (size_t)N, c->stream);
+        launch_extract(c->d_sx, nullptr, N, c->Rp, replica, buf, cst(c));
+        launch_extract(c->d_sy, nullptr, N, c->Rp, replica, buf + N, cst(c));
+        launch_extract(c->d_sz, nullptr, N, c->Rp, replica, buf + 2 * (size_t)N, cst(c));
+        if (c->angle) launch_extract(c->d_phi, nullptr, N, c->Rp, replica, buf + 3 * (size_t)N, cst(c));
     }
     HIPCHK(c, hipGetLastError());
     std::vector<float> h(4 * (size_t)N, 0.f);
-    HIPCHK(c, hipMemcpyAsync(h.data(), buf, sizeof(float) * 4 * N, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpyAsync(h.data(), buf, sizeof(float) * 4 * N, hipMemcpyDeviceToHost, cst(c)));
+    HIPCHK(c, hipStreamSynchronize(cst(c)));
     for (int i = 0; i < N; ++i) {
         const int s = c->slot_of_site[i];
         if (c->model == ETHER_MODEL_ISING) {
@@ -2629,17 +2646,17 @@ extern "C" int ether_set_spins(ether_ctx* c, int32_t replica, const double* S, c
         h[3 * (size_t)N + s] = phi ? (float)phi[i] : (float)std::atan2(S[3 * i + 1], S[3 * i]);
     }
     float* buf = c->d_scratch_h;
-    HIPCHK(c, hipMemcpyAsync(buf, h.data(), sizeof(float) * 4 * N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(buf, h.data(), sizeof(float) * 4 * N, hipMemcpyHostToDevice, cst(c)));
     if (c->model == ETHER_MODEL_ISING) {
-        launch_insert(nullptr, c->d_s, N, c->Rp, r0, r1, buf + 2 * (size_t)N, c->stream);
+        launch_insert(nullptr, c->d_s, N, c->Rp, r0, r1, buf + 2 * (size_t)N, cst(c));
     } else {
-        launch_insert(c->d_sx, nullptr, N, c->Rp, r0, r1, buf, c->stream);
-        launch_insert(c->d_sy, nullptr, N, c->Rp, r0, r1, buf + N, c->stream);
-        launch_insert(c->d_sz, nullptr, N, c->Rp, r0, r1, buf + 2 * (size_t)N, c->stream);
-        if (c->angle) launch_insert(c->d_phi, nullptr, N, c->Rp, r0, r1, buf + 3 * (size_t)N, c->stream);
+        launch_insert(c->d_sx, nullptr, N, c->Rp, r0, r1, buf, cst(c));
+        launch_insert(c->d_sy, nullptr, N, c->Rp, r0, r1, buf + N, cst(c));
+        launch_insert(c->d_sz, nullptr, N, c->Rp, r0, r1, buf + 2 * (size_t)N, cst(c));
+        if (c->angle) launch_insert(c->d_phi, nullptr, N, c->Rp, r0, r1, buf + 3 * (size_t)N, cst(c));
     }
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamSynchronize(cst(c)));
     return ETHER_OK;
 }
 
@@ -2648,12 +2665,12 @@ extern "C" int ether_local_fields(ether_ctx* c, int32_t replica, double* h, doub
     if (replica < 0 || replica >= c->R) return fail(c, ETHER_ERR_ARG, "ether_local_fields: bad replica");
     const int N = c->N;
     launch_local_fields(meas_args(c), replica, c->d_scratch_h, c->d_scratch_e, c->zee, c->sia,
-                        c->model, c->stream);
+                        c->model, cst(c));
     HIPCHK(c, hipGetLastError());
     std::vector<float> hh(3 * (size_t)N), ee(N);
-    HIPCHK(c, hipMemcpyAsync(hh.data(), c->d_scratch_h, 12 * (size_t)N, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(ee.data(), c->d_scratch_e, 4 * (size_t)N, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpyAsync(hh.data(), c->d_scratch_h, 12 * (size_t)N, hipMemcpyDeviceToHost, cst(c)));
+    HIPCHK(c, hipMemcpyAsync(ee.data(), c->d_scratch_e, 4 * (size_t)N, hipMemcpyDeviceToHost, cst(c)));
+    HIPCHK(c, hipStreamSynchronize(cst(c)));
     for (int i = 0; i < N; ++i) {
         int s = c->slot_of_site[i];
         if (h)
@@ -2665,7 +2682,7 @@ extern "C" int ether_local_fields(ether_ctx* c, int32_t replica, double* h, doub
 
 // access for extension translation units (mlp.hip, sq.hip)
 namespace ether {
-hipStream_t ctx_stream(ether_ctx* c) { return c->stream; }
+hipStream_t ctx_stream(ether_ctx* c) { return cst(c); }
 void*& ctx_mlp(ether_ctx* c) { return c->mlp; }
 void*& ctx_sq(ether_ctx* c) { return c->sq; }
 int ctx_fail(ether_ctx* c, int code, const std::string& msg) { return fail(c, code, msg); }
@@ -2710,8 +2727,8 @@ int ctx_snapshot(ether_ctx* c, void** out) {
         return fail(c, ETHER_ERR_HIP, std::string("snapshot: hipMalloc failed: ") + hipGetErrorString(e));
     }
     s->bytes = c->arena_bytes;
-    e = hipMemcpyAsync(s->dev, c->d_arena, s->bytes, hipMemcpyDeviceToDevice, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    e = hipMemcpyAsync(s->dev, c->d_arena, s->bytes, hipMemcpyDeviceToDevice, cst(c));
+    if (e == hipSuccess) e = hipStreamSynchronize(cst(c));
     if (e != hipSuccess) {
         (void)hipFree(s->dev);
         delete s;
@@ -2733,8 +2750,8 @@ int ctx_restore(ether_ctx* c, void* snap) {
     const CtxSnapshot* s = static_cast<const CtxSnapshot*>(snap);
     if (!s || s->bytes != c->arena_bytes) return fail(c, ETHER_ERR_STATE, "restore: no snapshot of this context");
     drop_graphs(c);
-    hipError_t e = hipMemcpyAsync(c->d_arena, s->dev, s->bytes, hipMemcpyDeviceToDevice, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipError_t e = hipMemcpyAsync(c->d_arena, s->dev, s->bytes, hipMemcpyDeviceToDevice, cst(c));
+    if (e == hipSuccess) e = hipStreamSynchronize(cst(c));
     if (e != hipSuccess) return fail(c, ETHER_ERR_HIP, std::string("restore: copy failed: ") + hipGetErrorString(e));
     c->step = s->step;
     c->init_count = s->init_count;

@@ -59,3 +59,19 @@ def assert_runs_agree(mean_a, sem_a, mean_b, sem_b, n_a, n_b, label):
         f"{low} below 0.01 (allowed {allowed}), max |t| {np.abs(t[ok]).max():.2f}")
     assert p.min() > 1e-3 / K
     assert low <= allowed
+
+
+def three_shell_ising(L):
+    """Simple-cubic Ising with three coupling classes: J = -1, 0.5, 0.25 meV
+    on the 6, 12 and 8 neighbours at 1, sqrt 2, sqrt 3 (26 per site), with
+    its neighbour lists attached."""
+    from ether_amd.inputs import build_model, parse_in_ether, parse_j_exchange, parse_structure
+    st = parse_structure(workloads.SC_STRUCTURE)
+    tags = parse_in_ether(f"model = ising\nmcs = 100 10\ntemp = 2 2 1\nspin = 1\nspecies = A\n"
+                          f"sc = {L} {L} {L}\nbc = c c c\nrepeat = 1\ncoa = 10\nseed = 1992\n"
+                          f"para = .T. 1\n", st.n_species)
+    ex = parse_j_exchange("3\n1.0 1\nA A -1 -1 -1\n1.414214 1\nA A 0.5 0.5 0.5\n"
+                          "1.732051 1\nA A 0.25 0.25 0.25\n")
+    m = build_model(tags, st, ex, None, None)
+    attach_neighbours(m)
+    return m

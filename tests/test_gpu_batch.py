@@ -13,14 +13,17 @@ import pytest
 from ether_amd import engine as E
 from ether_amd import workloads
 from ether_amd.driver import attach_neighbours, make_engine
-from tests.helpers import random_xyz, say
+from tests.helpers import random_xyz, say, three_shell_ising
 
 pytestmark = pytest.mark.gpu
 
 
 def _engine(name, R, seed, over, temps):
-    m = workloads.model(name, **over)
-    attach_neighbours(m)
+    if name == "sc_3shell":
+        m = three_shell_ising(over["L"])
+    else:
+        m = workloads.model(name, **over)
+        attach_neighbours(m)
     eng = make_engine(m, R, seed=seed)
     eng.set_betas(m.betas(temps))
     eng.set_ladders([0, R])
@@ -96,12 +99,15 @@ def test_batch_equals_lattices_run_alone(cases, graphs):
 # workgroup per replica of every lattice, the sweeps between two measurements
 # in one launch).  Cases: 40^3 (64,000 sites: above the one-context resident
 # budget, so alone it runs the phase kernels), an odd edge (more than two
-# colours), two coupling classes with Zeeman, and 6^3.
+# colours), two coupling classes with Zeeman, three classes (26 neighbours:
+# the kernel instance for any number of classes, launched beside the others),
+# and 6^3.
 ISING_RES = [
     ("sc_ising_16", 8, 31, dict(sc="40 40 40"), np.linspace(100, 10, 8)),
     ("sc_ising_16", 12, 32, dict(sc="9 9 9"), np.linspace(100, 10, 12)),
     ("pyro_pt_16", 20, 22, dict(sc="2 2 2", model="ising", para=".T. 1", pt=".T. 11",
                                 zeeman=".T. 0 0 0.3"), np.linspace(6, 2, 20)),
+    ("sc_3shell", 12, 23, dict(L=6), np.linspace(6, 2, 12)),
     ("sc_ising_16", 30, 21, dict(sc="6 6 6"), np.linspace(100, 10, 30)),
 ]
 

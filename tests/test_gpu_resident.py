@@ -19,22 +19,13 @@ import pytest
 
 from ether_amd import engine as E
 from ether_amd import workloads
-from ether_amd.driver import attach_neighbours, make_engine
-from ether_amd.inputs import build_model, parse_in_ether, parse_j_exchange, parse_structure
-from tests.helpers import pair, say
+from ether_amd.driver import make_engine
+from tests.helpers import pair, say, three_shell_ising
 
 
 def _three_shell_ising(L, R):
-    """Simple-cubic Ising with three coupling classes: J = -1, 0.5, 0.25 meV
-    on the 6, 12 and 8 neighbours at 1, sqrt 2, sqrt 3 (26 per site)."""
-    st = parse_structure(workloads.SC_STRUCTURE)
-    tags = parse_in_ether(f"model = ising\nmcs = 100 10\ntemp = 2 2 1\nspin = 1\nspecies = A\n"
-                          f"sc = {L} {L} {L}\nbc = c c c\nrepeat = 1\ncoa = 10\nseed = 1992\n"
-                          f"para = .T. 1\n", st.n_species)
-    ex = parse_j_exchange("3\n1.0 1\nA A -1 -1 -1\n1.414214 1\nA A 0.5 0.5 0.5\n"
-                          "1.732051 1\nA A 0.25 0.25 0.25\n")
-    m = build_model(tags, st, ex, None, None)
-    attach_neighbours(m)
+    """Simple-cubic Ising with three coupling classes (26 neighbours per site)."""
+    m = three_shell_ising(L)
     return m, make_engine(m, R, seed=1992)
 
 pytestmark = pytest.mark.gpu

@@ -365,7 +365,7 @@ extern void ether_ext_destroy(ether_ctx* c);  // mlp / sq state
 
 extern "C" void ether_ctx_destroy(ether_ctx* c) {
     if (!c) return;
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(cst(c));   // and for a batch that ran the context
     ether_ext_destroy(c);
     for (auto& g : c->graphs) {
         (void)hipGraphExecDestroy(g.exec);

@@ -24,8 +24,9 @@ roofline = the Metropolis colour-phase kernel: 24 B of own-spin traffic per
         one by one), against 8 TB/s; traffic = HBM bytes per launch at the
         shape the run keeps, from the committed PMC summary under profiles/.
 pt, pt_trips = the tempering ladder comparisons (linear vs learned).
-cpu_baseline = the CPU oracle's FP64 sweep (+OVRR) of one 32^3 chain on one
-        host core, for a bounded sample of about 10 s.
+cpu_baseline = the CPU oracle's FP64 sweep (+OVRR) of 32^3 chains, one per
+        host thread (up to 16) after one alone on one core, for a bounded
+        sample of about 10 s.
 """
 import argparse
 import json

@@ -1430,8 +1430,9 @@ static int enqueue_sweep(ether_ctx* c, double fuse_frac = 0.0, bool meas = false
                     // depend on the launch shape).  Several sites per wave
                     // (few replicas) keep the context's walk: there a
                     // workgroup books fewer replicas, and the long walk cost
-                    // more in latency than it saved in atomics (C5's
-                    // simple-cubic Heisenberg batch: 0.754 -> 0.715 ms per MCS)
+                    // more in latency than it saved in atomics (62 simple-cubic
+                    // Heisenberg lattices of 8..32 replicas batched: 0.754 ->
+                    // 0.715 ms per MCS)
                     a.spw = c->lps < kWave ? c->spw : kMaxSitesPerWave;
                     set_blocks(a, a.G, a.spw * a.sps);
                 }

@@ -252,8 +252,9 @@ struct RowVals {
 // A site's ELL row into registers: wave-uniform (scalar loads) in the
 // one-site-per-wave kernels, per lane in the lanes kernels.  The lanes sweep
 // loads the next step's rows while it updates the current sites, so the
-// neighbour loads of a step need not wait for their rows (C5's XYZ batches
-// 2.5-4% faster).
+// neighbour loads of a step need not wait for their rows (batches of 58-66
+// simple-cubic, triangular and pyrochlore lattices of 8..32 replicas: 2.5-4%
+// faster).
 template <int DM>
 __device__ __forceinline__ void load_row(const uint32_t* nb, uint32_t (&e)[DM]) {
     const ConstRow row = const_row(nb);
@@ -599,7 +600,8 @@ __device__ __forceinline__ void xyz_sweep_block(const XyzArgs& a, int bb) {
         u32x4 wr;
         if constexpr (DM > 0 && DM <= 8) {
             // the row at the step: loading the next site's row one step ahead
-            // (scalar registers) cost a wave per SIMD at C2 (102 SGPRs) and
+            // (scalar registers) cost a wave per SIMD on the 32^3 Heisenberg
+            // lattice with 190 replicas (102 SGPRs) and
             // gained nothing
             uint32_t row[DM];
             load_row<DM>(nb, row);
@@ -1106,7 +1108,8 @@ __global__ void __launch_bounds__(1024) k_ising_resident(IsingResArgs a) {
 // ---------------------------------------------------------------------------
 // The resident Ising sweeps with one workgroup per replica instead of per
 // word: for fewer replicas than compute units a word's workgroup keeps one CU
-// busy with four replicas while most CUs idle (C1: 48 of 256), so each
+// busy with four replicas while most CUs idle (16^3 sites, 190 replicas:
+// 48 of 256), so each
 // replica gets a workgroup of its own, its int8 spin of every site and its
 // threshold table in LDS.  Every update is the word kernel's for byte
 // b = r & 3 of word r >> 2: the same Philox block (slot, word, sweep), its

@@ -64,13 +64,13 @@ def _oracle_run(m, seed):
 
 
 CASES = [
-    # C1: the Ising scan 100 -> 10 K (T_c = 4.51 J = 52 K) on 16^3
+    # the Ising scan 100 -> 10 K (T_c = 4.51 J = 52 K) on 16^3
     ("sc_ising_16", dict(mcs="1500 500", repeat="10", temp="100 10 10")),
-    # C2: the 32^3 antiferromagnet, OVRR every MCS, 60 -> 20 K (T_N = 1.44 J = 17 K)
+    # the 32^3 antiferromagnet, OVRR every MCS, 60 -> 20 K (T_N = 1.44 J = 17 K)
     ("sc_heis_afm_32", dict(mcs="600 200", repeat="10", temp="60 20 10")),
-    # C3: 96 x 96 triangular with SIA and Zeeman
+    # 96 x 96 triangular with SIA and Zeeman
     ("tri_afm_96", dict(mcs="1000 400", repeat="10", temp="20 5 5", angle=".F. 0")),
-    # C4: the 16^3-cell pyrochlore, J1 + J2, without tempering
+    # the 16^3-cell pyrochlore, J1 + J2, without tempering
     ("pyro_pt_16", dict(mcs="400 150", repeat="10", temp="20 5 5", pt=".F. 11")),
 ]
 

@@ -152,3 +152,39 @@ def test_resident_batch_equals_phase_batch_and_alone(graphs):
         batch.close()
     for e in res + phase + alone:
         e.close()
+
+
+def test_contexts_continue_alone_after_their_batch():
+    """Two batches (Ising resident, XYZ phases) run back to back on their two
+    streams, then every context continues with its own ether_run at once: the
+    wait for its batch is made at that first use of the context's stream, so
+    the spins, moments and tempering state equal a context that ran the whole
+    schedule alone."""
+    say("batches, then each context alone")
+    cases = ISING_RES[1:3] + XYZ[:2]
+    mixed, ref = [], []
+    for name, R, seed, over, temps in cases:
+        _, a = _engine(name, R, seed, over, temps)
+        _, b = _engine(name, R, seed, over, temps)
+        mixed.append(a)
+        ref.append(b)
+    bi = E.Batch([e for e in mixed if e.model == E.MODEL_ISING])
+    bx = E.Batch([e for e in mixed if e.model != E.MODEL_ISING])
+    sc_i = bi.run(1, 30, 10, 5, 1, 1.0, 5, 0)
+    sc_x = bx.run(1, 30, 10, 5, 1, 1.0, 5, 0)
+    for e in mixed:
+        e.run(31, 17, 10, 5, 1, 1.0, 5, sc_i if e.model == E.MODEL_ISING else sc_x)
+    for e in ref:
+        sc = e.run(1, 30, 10, 5, 1, 1.0, 5, 0)
+        e.run(31, 17, 10, 5, 1, 1.0, 5, sc)
+    for (name, R, *_rest), a, b in zip(cases, mixed, ref):
+        sa, ma, pa = _state(a)
+        sb, mb, pb = _state(b)
+        assert np.array_equal(sa, sb), f"{name} R={R}: spins differ"
+        assert np.array_equal(ma, mb), f"{name} R={R}: moments differ"
+        for x, y in zip(pa, pb):
+            assert np.array_equal(x, y), f"{name} R={R}: tempering state differs"
+    bi.close()
+    bx.close()
+    for e in mixed + ref:
+        e.close()
